@@ -1,0 +1,63 @@
+#!/bin/bash
+# build_ref.sh -- compile the reference MCM time-step path from the read-only
+# reference sources, in place, into oracle/_ref/ (git-ignored).
+#
+# TEST INFRASTRUCTURE ONLY: the result is a checker (and the timed CPU
+# baseline), never part of the product path.
+#
+# Recipe:
+#   * ROCm flang compiles the reference's own .f90 files listed below (the
+#     module database, the material set-up, the initialisation chain and every
+#     routine of the SPH step) from /root/reference/MCM, -O2, no -march, so no
+#     FMA contraction is possible on x86-64.
+#   * They are linked into oracle/_ref/libmcmref.so.  Two symbols stay
+#     unresolved and are bound lazily: mcm_shutdown_ (the fatal-error exit;
+#     its file needs Intel time/ctime/etime, which flang does not provide) and
+#     mcm_transducer_nbr_ (only called when pressure transducers are defined).
+#     No stand-in is written for either: reaching them aborts the probe with a
+#     loader error.
+#   * oracle/ref_probe.f90 (project-written) is compiled against the module
+#     files and linked to the library.
+#
+# Usage: oracle/build_ref.sh   (no-op with a message when /root/reference is absent)
+set -euo pipefail
+HERE="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"
+REF=/root/reference/MCM
+OUT="$HERE/_ref"
+FC=${FC:-/opt/rocm/lib/llvm/bin/flang}
+if [ ! -d "$REF" ]; then
+  echo "build_ref: $REF not present; reference probe not built"
+  exit 0
+fi
+if [ ! -x "$FC" ]; then
+  echo "build_ref: flang not found at $FC; reference probe not built"
+  exit 0
+fi
+mkdir -p "$OUT/obj" "$OUT/mod"
+FILES="mcm_database mcm_math_func
+mcm_sets1 mcm_sets3 mcm_sets4 mcm_sets10 mcm_printm
+mcm_initialise_database mcm_init_boundcond mcm_calcmass mcm_initvbls mcm_calcg mcm_calcrhopc mcm_eoscalc
+mcm_eos1p mcm_eos4p mcm_eos13p mcm_eos28p mcm_eos41p
+mcm_eos1c mcm_eos4c mcm_eos13c mcm_eos28c mcm_eos41c
+mcm_initsigma mcm_initrod mcm_init_h mcm_aviscosity mcm_nodalav mcm_init_neighb
+mcm_init_boundary mcm_calccritts mcm_calctotale mcm_initold
+mcm_neighbours mcm_setuplist mcm_ghost_setup mcm_ll_neighbours
+mcm_strain mcm_oldstrain mcm_corr_strain mcm_kernel mcm_gradw mcm_get_j_strain_info
+mcm_update_h mcm_constitutive mcm_rhoupd mcm_stress_rot1
+mcm_f3dm1 mcm_f3dm3 mcm_f3dm9 mcm_f3dm10 mcm_f3dm25
+mcm_lieupd mcm_hieupd mcm_sueos mcm_eqos mcm_eqos1 mcm_eqos4 mcm_eqos13 mcm_eqos28 mcm_eqos41
+mcm_stress_up mcm_momentum mcm_update_ghost_stress mcm_oldacceleration mcm_corr_acceleration
+mcm_get_j_moment_info mcm_boundcond mcm_update_velocity mcm_move mcm_check_sym_pen"
+OBJS=""
+for f in $FILES; do
+  src="$REF/$f.f90"
+  obj="$OUT/obj/$f.o"
+  if [ ! -f "$obj" ] || [ "$src" -nt "$obj" ]; then
+    "$FC" -O2 -fPIC -module-dir "$OUT/mod" -c "$src" -o "$obj"
+  fi
+  OBJS="$OBJS $obj"
+done
+"$FC" -shared -fPIC -Wl,-z,lazy $OBJS -o "$OUT/libmcmref.so"
+"$FC" -O2 -I "$OUT/mod" -module-dir "$OUT/mod" "$HERE/ref_probe.f90" -o "$OUT/ref_probe" \
+  -L"$OUT" -lmcmref -Wl,--allow-shlib-undefined -Wl,-z,lazy -Wl,-rpath,'$ORIGIN'
+echo "build_ref: built $OUT/ref_probe"
