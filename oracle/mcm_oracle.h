@@ -1,0 +1,104 @@
+/* mcm_oracle.h -- CPU restatement of the MCM SPH time step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the oracle the HIP path is checked
+ * against (tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg).
+ * Nothing in the product links it.
+ *
+ * It restates, in plain C over structure-of-arrays state, the reference's
+ * initialisation chain and every stage of one time step, keeping each
+ * floating-point expression in the reference's evaluation order so that it
+ * reproduces the flang build of the reference bit for bit on the same deck
+ * (pinned by tests/test_oracle_vs_reference.py).  Each function cites the
+ * reference routine it follows (MCM/<file>.f90:<lines>).
+ *
+ * Conventions: 0-based particle and ghost indices; 3x3 tensors stored
+ * column-major, element (r,c) at [9*i + r + 3*c]; vectors at [3*i + d].
+ */
+#ifndef MCM_ORACLE_H
+#define MCM_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    int model, eos, visc_type;
+    double rho, g, av_l, av_q, mass, h, rho_min, rho_max;
+    double strinput[48];
+    double eosinput[24];
+} orc_material;
+
+typedef struct {
+    /* control */
+    int np, nummat, disctype, h_opt, tcrit_opt, contacttype, krtype;
+    int massopt, init_h_opt, init_rhoe, boundary;
+    int boundary_code[3][2];   /* [dim][0=min,1=max]: 0 none, 1 symmetry, 2 periodic */
+    int boundary_type[3];
+    double boundary_x[3][2];
+    double tssf, itss, endtime, pi;
+    /* time */
+    double dt, dtold, ptime, critts, hmax;
+    int timestep;
+    /* grid */
+    double coord_maxmin[3][2]; /* [dim][0=min,1=max] */
+    double gridmin[3], gridsize[3];
+    int gridlim[3];
+    int *llgrid;                /* gridlim product; >0 real id+1, <0 -(ghost id+1), 0 empty */
+    long llgrid_cap;
+    double plane_mult[6][3], plane_add[6][3];  /* xmin xmax ymin ymax zmin zmax */
+    orc_material *mat;
+    /* particles */
+    int *pmat, *dispbc, *llpointer;
+    double *x, *xzero, *v, *a;                 /* 3 per particle */
+    double *mass, *h, *hold, *h0, *rho, *rho0, *rhoold, *p, *e, *etry, *einc, *c, *vabs;
+    double *tracerod, *pcut, *p_cut, *efps, *fail, *mindist, *critts_p;
+    double *sigma, *s, *alfa, *q, *qold, *rod, *spin; /* 9 per particle */
+    /* neighbour lists (0-based ids), capacity ncap / gcap per particle */
+    int ncap, gcap;
+    int *nnbr, *g_nnbr, *ncont, *g_ncont;
+    int *nbr, *gnbr, *cont, *gcont;
+    /* ghosts */
+    int ngp, ngp_cap;
+    int *g_par, *g_mat, *g_ll;
+    double *g_x, *g_xzero, *g_v, *g_mass, *g_h, *g_hold, *g_rho, *g_sigma, *g_q;
+    /* energies */
+    double *mat_ke, *mat_ie, *mat_mom;
+    double kinetice, internale, totale;
+    /* f3dm10 keeps its plastic-strain increment between calls when a particle
+       has failed (the reference reads it uninitialised, mcm_f3dm10.f90:82-89) */
+    double f3dm10_defps;
+    int error;                 /* first error code seen */
+} orc_state;
+
+/* error codes */
+#define ORC_OK 0
+#define ORC_ERR_CAPACITY -2
+#define ORC_ERR_MODEL -3
+#define ORC_ERR_IO -4
+
+orc_state *orc_load_deck(const char *path, int ncap, int gcap);
+void orc_free(orc_state *st);
+int orc_initial(orc_state *st);
+int orc_neighbours(orc_state *st);
+int orc_strain(orc_state *st);
+int orc_update_h(orc_state *st);
+int orc_constitutive(orc_state *st);
+int orc_dt_update(orc_state *st);
+int orc_momentum(orc_state *st);
+int orc_boundcond(orc_state *st);
+int orc_calctotale(orc_state *st);
+int orc_update_velocity(orc_state *st);
+int orc_move(orc_state *st);
+int orc_step(orc_state *st);
+int orc_run(orc_state *st, int nsteps);
+int orc_dump(orc_state *st, const char *path, int step, int stage);
+
+/* closed-form pieces exposed for unit tests */
+double orc_kernel_w(const double *xi, const double *xj, double havg, double pi);
+void orc_kernel_gradw(const double *xi, const double *xj, double havg, double pi, double *dwdx);
+void orc_inverse3(const double *m, double *inv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
