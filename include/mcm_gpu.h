@@ -1,0 +1,127 @@
+/* mcm_gpu.h -- C ABI of the MI355X (gfx950) MCM SPH time step.
+ *
+ * The reference drives its time step through argument-less Fortran
+ * subroutines that share module state (MCM/mcm_solution.f90:41-116).  This
+ * library replaces the bodies of those subroutines: a Fortran host keeps its
+ * call sites and calls these entry points through ISO_C_BINDING
+ * (mcmtop_amd/fortran/mcm_gpu_iface.f90).  Every entry point is plain C:
+ * opaque context, plain pointers and sizes, int status codes (0 = success,
+ * negative = error, message via mcm_gpu_last_error).
+ *
+ * Threading: a context is driven by one host thread; calls enqueue work on
+ * the context's HIP stream.  Calls that return host-visible values
+ * (scalars, downloads, dumps) synchronise that stream.
+ */
+#ifndef MCM_GPU_H
+#define MCM_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mcm_gpu_ctx mcm_gpu_ctx;
+
+/* status codes */
+#define MCM_GPU_OK 0
+#define MCM_GPU_ERR_ARG -1        /* bad argument / unsupported option */
+#define MCM_GPU_ERR_IO -2         /* deck could not be read */
+#define MCM_GPU_ERR_HIP -3        /* HIP runtime error */
+#define MCM_GPU_ERR_CAPACITY -4   /* neighbour / ghost / cell capacity exceeded (after regrow) */
+#define MCM_GPU_ERR_MODEL -5      /* unsupported material model or EOS */
+#define MCM_GPU_ERR_NODEVICE -6   /* no HIP device visible */
+
+/* options for context creation (all zero = defaults) */
+typedef struct {
+    int32_t nbr_capacity;   /* neighbour slots per particle (0 = sized from the first search) */
+    int32_t ghost_capacity; /* ghost slots (0 = 2*np + 1024) */
+    int32_t device;         /* HIP device ordinal */
+    int32_t reserved[5];
+} mcm_gpu_options;
+
+/* host-visible scalar state (mcm_database.f90:29-31,92-99 counterparts) */
+typedef struct {
+    double dt, dtold, ptime, critts, hmax;
+    double kinetice, internale, totale;
+    double coord_min[3], coord_max[3];
+    int32_t np, ngp, max_count, timestep;
+    int32_t error_flags;      /* bit 0: neighbour capacity, 1: ghost capacity, 2: cell capacity,
+                                 3: particle in a ghost-layer cell */
+    int32_t nummat;
+    int32_t reserved[2];
+} mcm_gpu_scalars;
+
+/* Create a context from a binary deck (format: mcmtop_amd/deck.py).
+   Replaces the deck reading of mcm_getinput.f90:90-128 for the GPU path. */
+int mcm_gpu_create_from_deck(const char *deck_path, const mcm_gpu_options *opt, mcm_gpu_ctx **out);
+void mcm_gpu_destroy(mcm_gpu_ctx *ctx);
+const char *mcm_gpu_last_error(const mcm_gpu_ctx *ctx);
+
+/* Initialisation chain of mcm_initial.f90:33-101 (init_boundcond, calcmass,
+   initvbls, calcg, calcrhopc, initsigma, initrod, init_h, aviscosity,
+   init_neighb, calccritts, first dt, calctotale, initold). */
+int mcm_gpu_initial(mcm_gpu_ctx *ctx);
+
+/* Stage entry points, one per reference subroutine called by mcm_solution:
+   mcm_neighbours (mcm_neighbours.f90:29-49: setuplist + ghost_setup + ll_neighbours) */
+int mcm_gpu_neighbours(mcm_gpu_ctx *ctx);
+/* mcm_strain (mcm_strain.f90:27-60: half-step shift, oldstrain / corr_strain) */
+int mcm_gpu_strain(mcm_gpu_ctx *ctx);
+/* mcm_update_h (mcm_update_h.f90:25-54) */
+int mcm_gpu_update_h(mcm_gpu_ctx *ctx);
+/* mcm_constitutive (mcm_constitutive.f90:26-87: rhoupd, per-particle models, calccritts) */
+int mcm_gpu_constitutive(mcm_gpu_ctx *ctx);
+/* dt/ptime update of mcm_solution.f90:58-62 (kept on the device) */
+int mcm_gpu_dt_update(mcm_gpu_ctx *ctx);
+/* mcm_momentum (mcm_momentum.f90:22-43: ghost stress, old/corr acceleration) */
+int mcm_gpu_momentum(mcm_gpu_ctx *ctx);
+/* mcm_boundcond (mcm_boundcond.f90:23-62) */
+int mcm_gpu_boundcond(mcm_gpu_ctx *ctx);
+/* mcm_calctotale (mcm_calctotale.f90:27-60) */
+int mcm_gpu_calctotale(mcm_gpu_ctx *ctx);
+/* mcm_update_velocity (mcm_update_velocity.f90:29-56) */
+int mcm_gpu_update_velocity(mcm_gpu_ctx *ctx);
+/* mcm_move (mcm_move.f90:25-55, with mcm_check_sym_pen.f90) */
+int mcm_gpu_move(mcm_gpu_ctx *ctx);
+/* mcm_calccritts (mcm_calccritts.f90:32-87); writes the global minimum */
+int mcm_gpu_calccritts(mcm_gpu_ctx *ctx, double *critts_out);
+
+/* Fused fast path: nsteps complete loop bodies of mcm_solution.f90:41-126
+   (no output), dt/ptime advanced on the device, no host synchronisation. */
+int mcm_gpu_step(mcm_gpu_ctx *ctx, int32_t nsteps);
+
+/* Host-visible state */
+int mcm_gpu_get_scalars(mcm_gpu_ctx *ctx, mcm_gpu_scalars *out);
+/* Download one per-particle field into host memory (n = np values, or 3*np
+   for x/v/a, 9*np for tensors, column-major 3x3 per particle).  Field codes: */
+#define MCM_F_X 0
+#define MCM_F_V 1
+#define MCM_F_A 2
+#define MCM_F_RHO 3
+#define MCM_F_H 4
+#define MCM_F_P 5
+#define MCM_F_E 6
+#define MCM_F_C 7
+#define MCM_F_EFPS 8
+#define MCM_F_SIGMA 9
+#define MCM_F_VABS 10
+#define MCM_F_MASS 11
+int mcm_gpu_download(mcm_gpu_ctx *ctx, int32_t field, double *host, int64_t n);
+/* Neighbour lists in reference order: counts[np] (real incl. self),
+   gcounts[np] (ghost), ids[np*cap] row-major with 0-based ids (ghosts as
+   np + ghost index); returns the capacity used per row in *cap. */
+int mcm_gpu_download_neighbours(mcm_gpu_ctx *ctx, int32_t *counts, int32_t *gcounts, int32_t *ids,
+                                int64_t ids_len, int32_t *cap);
+/* Full state dump in the layout of the reference probe (oracle/ref_probe.f90). */
+int mcm_gpu_dump(mcm_gpu_ctx *ctx, const char *path, int32_t step, int32_t stage);
+
+/* Time nsteps of the fused path with HIP events on the context stream;
+   returns the average milliseconds per step of each kernel group in ms[4]
+   (neighbour build, strain+constitutive, momentum+integration, whole step). */
+int mcm_gpu_time_steps(mcm_gpu_ctx *ctx, int32_t nsteps, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,76 @@
+// mcm_device.h -- device-resident state of one MCM run and the scalar block
+// shared by every kernel of a time step.
+#pragma once
+
+#include "mcm_physics.h"
+
+#define MCM_MAX_MAT 8
+#define MCM_BLOCK 256
+
+// Grid of the cell search (mcm_setuplist.f90:30-91); cell indices 1-based
+// as in the reference, linear id = ((k-1)*lim1 + (j-1))*lim0 + (i-1).
+struct GridDev {
+    double gridmin[3], gridsize[3];
+    int lim[3];
+    int ncell;
+};
+
+// Scalars that the reference keeps in module variables, kept on the device
+// so that a step needs no host round trip.
+struct Scalars {
+    double dt, dtold, ptime, critts, hmax;
+    double coord_min[3], coord_max[3];
+    double kinetice, internale, totale;
+    double mat_ke[MCM_MAX_MAT], mat_ie[MCM_MAX_MAT], mat_mom[MCM_MAX_MAT][3];
+    GridDev grid;
+    int ngp;            // ghosts generated this step
+    int max_count;      // largest neighbour count seen by the last search
+    int error_flags;    // MCM error bits, see mcm_gpu.h
+    int abort;          // set when a capacity was exceeded: later kernels of the step do nothing
+    int timestep;
+    unsigned int done_b, done_c, done_r; // last-block counters (self-resetting)
+};
+
+// Boundary planes in the order xmin xmax ymin ymax zmin zmax
+struct PlaneDev {
+    double mult[3], add[3];
+    int dim, side, code, pad_;
+};
+
+// Device pointers of one run.  Arrays marked [U] span the unified index
+// space of np real particles followed by the ghost slots.
+struct DevState {
+    int np, npad, ngcap, nummat, disctype, h_opt, tcrit_opt, boundary, ncap;
+    int nplanes, cellcap, need_s, has_model9;
+    int bcode[3][2];
+    int btype[3];
+    double bx[3][2];
+    double tssf, pi;
+    PlaneDev planes[6];
+    MatParams* mat;            // [nummat]
+    Scalars* sc;
+    // particle state
+    double *x0[3], *x1[3];     // [U] x^n (strain input) and x after the strain pass
+    double *v[3];              // [U]
+    double* mass;              // [U]
+    double *hbuf[2], *rbuf[2]; // [U] h and rho, double buffered (cur = step parity)
+    int* pmat;                 // [U] 0-based material
+    int* dispbc;
+    int* gpar;                 // [ngcap] parent of each ghost (0-based real id)
+    double *hold, *h0, *rho0, *p, *e, *c, *vabs, *pcut, *p_cut, *efps, *fail, *q, *mindist;
+    double *sig[6], *sd[6], *alfa[6];
+    double* A[6];              // [U] (sigma - q)/rho^2 for the momentum pass
+    // neighbour search
+    int *cell_of, *slot_of, *cell_count, *cell_start, *sorted;
+    double *sx, *sy, *sz, *sh;  // cell-sorted copies of x^n and h (candidate scan)
+    int* smat;
+    int *gstart[3], *gcount[3];
+    int* nbr;                  // [ncap][npad]
+    int *nreal, *ntot;
+    int* scan_tmp;
+    // diagnostics / stage outputs
+    double *rod[6], *spin[3], *tracerod, *a[3], *einc, *etry, *rhoold, *critts_p;
+    int* yield;                // active-set mask of the plastic models
+    double* red;               // per-block partial reductions
+    double* defps_prev;        // f3dm10 increment carried for failed particles
+};

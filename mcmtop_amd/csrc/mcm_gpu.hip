@@ -1,0 +1,974 @@
+// mcm_gpu.hip -- host runtime and C ABI of the MI355X MCM SPH time step.
+//
+// One context = one run on one GPU: device-resident structure-of-arrays
+// state, one HIP stream, a captured two-step hipGraph for the fused path,
+// and the granular stage entry points that replace the reference's
+// argument-less step subroutines (MCM/mcm_solution.f90:41-116).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mcm_gpu.h"
+#include "mcm_step.hip"
+
+namespace {
+
+#define HIPCHK(call)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess) {                                                                   \
+            ctx->err = std::string(#call) + ": " + hipGetErrorString(e_);                         \
+            return MCM_GPU_ERR_HIP;                                                               \
+        }                                                                                         \
+    } while (0)
+
+struct HostDeck {
+    int np = 0, nummat = 0, disctype = 0, h_opt = 0, tcrit_opt = 0, contacttype = 0, krtype = 1;
+    int maxnbr = 0, massopt = 0, init_h_opt = 0, init_rhoe = 0, mirr_opt = 0;
+    int sym_flag[6] = {0, 0, 0, 0, 0, 0};
+    double tssf = 0, itss = 0, endtime = 0, sym_x[6] = {0, 0, 0, 0, 0, 0};
+    std::vector<MatParams> mat;
+    std::vector<double> x, v, pm, ph, pr, pe;
+    std::vector<int> pmat, dispbc;
+};
+
+}  // namespace
+
+struct mcm_gpu_ctx {
+    HostDeck deck;
+    DevState d{};
+    Scalars* sc = nullptr;
+    hipStream_t stream = nullptr;
+    hipGraphExec_t graph2 = nullptr;   // two fused steps: parity 0 then 1
+    int cur = 0;                       // parity of the current h / rho buffers
+    int phase = 0;                     // 0: step start, 1: after strain (x in x1, h/rho in next)
+    int nb = 0;                        // blocks over np
+    std::vector<double> xzero;
+    std::vector<void*> allocs;
+    std::string err;
+    int device = 0;
+};
+
+namespace {
+
+// ------------------------------------------------------------------- deck
+constexpr int32_t kDeckMagic = 0x4D434D31;
+
+template <typename T>
+bool rd(FILE* f, T* p, size_t n) { return fread(p, sizeof(T), n, f) == n; }
+
+// material constants as mcm_matin.f90:46-56 / mcm_sets{1,3,10}.f90 /
+// mcm_printm.f90:84-100 leave them
+void setup_material(MatParams& m) {
+    if (m.visc_type == 0) m.visc_type = 1;
+    if (m.av_l == 0.0) m.av_l = 0.06;
+    if (m.av_q == 0.0) m.av_q = 1.5;
+    double* cm = m.strinput;
+#define C(a, b) cm[((a) - 1) + 5 * ((b) - 1)]
+    if (m.model == 1) {                       // mcm_sets1.f90:23-51
+        double prop6 = C(2, 1), prop7 = C(3, 1);
+        for (int i = 2; i <= 5; i++)
+            for (int j = 1; j <= 5; j++) C(i, j) = 0.0;
+        double q1 = C(1, 1) * C(1, 2) / ((1.0 + C(1, 2)) * (1.0 - 2.0 * C(1, 2)));
+        double q2 = C(1, 1) * 0.5 / (1.0 + C(1, 2));
+        double q3 = q1 + 2.0 * q2;
+        C(2, 1) = q3; C(3, 2) = q3; C(4, 3) = q3; C(5, 4) = q2;
+        C(2, 2) = q1; C(2, 3) = q1; C(3, 1) = q1; C(3, 3) = q1; C(4, 1) = q1; C(4, 2) = q1;
+        C(2, 6) = C(1, 6);
+        C(1, 6) = C(1, 1) * C(1, 4) / (C(1, 1) - C(1, 4));
+        C(1, 7) = prop6;
+        C(2, 7) = prop7;
+    } else if (m.model == 3) {                // mcm_sets3.f90:25-43
+        double prop7 = C(2, 6);
+        for (int i = 2; i <= 5; i++)
+            for (int j = 1; j <= 6; j++) C(i, j) = 0.0;
+        double q1 = C(1, 1) * C(1, 2) / ((1.0 + C(1, 2)) * (1.0 - 2.0 * C(1, 2)));
+        double q2 = C(1, 1) * 0.5 / (1.0 + C(1, 2));
+        double q3 = q1 + 2.0 * q2;
+        C(2, 1) = q3;
+        C(5, 4) = q2;
+        C(2, 4) = C(1, 1) * C(1, 4) / (C(1, 1) - C(1, 4));
+        C(2, 6) = prop7;
+        m.efps_flag = (int)lround(C(1, 6));
+    } else if (m.model == 10) {               // mcm_sets10.f90:23-36
+        double prop[48];
+        for (int i = 0; i < 48; i++) prop[i] = cm[i];
+        if (prop[3] == 0.0) prop[3] = (double)(-1.e30f);
+        if (prop[6] == 0.0) prop[6] = (double)(1.000000001f);
+        for (int i = 0; i < 48; i++) cm[i] = prop[i];
+        for (int i = 2; i <= 16; i++)
+            if (prop[i + 14] < prop[i + 15]) cm[15] = (double)i;
+        m.ispall = (int)cm[6];
+    }
+#undef C
+    if ((m.eos == 4 || m.eos == 41) && m.eosinput[7] == 0.0) m.eosinput[7] = 1.0;
+    if ((m.eos == 13 || m.eos == 28) && m.eosinput[3] == 0.0) m.eosinput[3] = 1.0;
+    // mcm_calcg.f90:31-59
+    if (m.model == 1 || m.model == 3) m.g = m.strinput[0] / (2.0 * (1.0 + m.strinput[5]));
+    else if (m.model == 9) m.g = 0.0;
+    else if (m.model == 10) m.g = m.strinput[0];
+}
+
+int read_deck(mcm_gpu_ctx* ctx, const char* path) {
+    FILE* f = fopen(path, "rb");
+    if (!f) { ctx->err = std::string("cannot open deck ") + path; return MCM_GPU_ERR_IO; }
+    HostDeck& D = ctx->deck;
+    int32_t magic, ints[19];
+    double reals[9];
+    bool ok = rd(f, &magic, 1) && magic == kDeckMagic && rd(f, ints, 19) && rd(f, reals, 9);
+    if (!ok) { fclose(f); ctx->err = "not an MCM binary deck"; return MCM_GPU_ERR_IO; }
+    D.np = ints[0]; D.nummat = ints[1]; D.disctype = ints[2]; D.h_opt = ints[3]; D.tcrit_opt = ints[4];
+    D.contacttype = ints[5]; D.krtype = ints[6]; D.maxnbr = ints[7]; D.massopt = ints[8];
+    D.init_h_opt = ints[9]; D.init_rhoe = ints[10]; D.mirr_opt = ints[11];
+    for (int k = 0; k < 6; k++) D.sym_flag[k] = ints[12 + k];
+    D.tssf = reals[0]; D.itss = reals[1]; D.endtime = reals[2];
+    for (int k = 0; k < 6; k++) D.sym_x[k] = reals[3 + k];
+    if (D.tssf == 0.0) D.tssf = 0.8;   // mcm_control_v2.f90:193
+    if (D.np <= 0 || D.nummat <= 0 || D.nummat > MCM_MAX_MAT) {
+        fclose(f); ctx->err = "deck: bad particle or material count"; return MCM_GPU_ERR_ARG;
+    }
+    D.mat.assign(D.nummat, MatParams{});
+    std::vector<double> mat_mass(D.nummat, 0.0);
+    for (int m = 0; m < D.nummat; m++) {
+        int32_t mi[3];
+        double mr[7];
+        MatParams& M = D.mat[m];
+        if (!(rd(f, mi, 3) && rd(f, mr, 7) && rd(f, M.strinput, 48) && rd(f, M.eosinput, 24))) {
+            fclose(f); ctx->err = "deck truncated in material cards"; return MCM_GPU_ERR_IO;
+        }
+        M.model = mi[0]; M.eos = mi[1]; M.visc_type = mi[2];
+        M.rho = mr[0]; M.av_q = mr[1]; M.av_l = mr[2]; M.h = mr[4];
+        M.rho_max = mr[6] == 0.0 ? 1.0E+10 : mr[6];
+        M.pad_ = 0;
+        M.efps_flag = 0; M.ispall = 0;
+        mat_mass[m] = mr[3];
+        setup_material(M);
+        if (M.model == 9 || M.model == 10) {
+            if (M.eos != 1 && M.eos != 4 && M.eos != 13 && M.eos != 28 && M.eos != 41) {
+                fclose(f); ctx->err = "unsupported equation of state"; return MCM_GPU_ERR_MODEL;
+            }
+        } else if (M.model == 1 || M.model == 3) {
+            if (M.eos != 0) { fclose(f); ctx->err = "models 1/3 take no EOS (mcm_matin.f90:72-79)"; return MCM_GPU_ERR_MODEL; }
+        } else {
+            fclose(f); ctx->err = "material model not supported on the GPU path (1, 3, 9, 10 are)";
+            return MCM_GPU_ERR_MODEL;
+        }
+        if (M.visc_type != 1) { fclose(f); ctx->err = "only nodal viscosity (visc_type 1) exists"; return MCM_GPU_ERR_MODEL; }
+    }
+    long n = D.np;
+    D.x.resize(3 * n); D.v.resize(3 * n); D.pmat.resize(n); D.dispbc.resize(n);
+    D.pm.resize(n); D.ph.resize(n); D.pr.resize(n); D.pe.resize(n);
+    ok = rd(f, D.x.data(), 3 * n) && rd(f, D.v.data(), 3 * n) && rd(f, D.pmat.data(), n) &&
+         rd(f, D.dispbc.data(), n) && rd(f, D.pm.data(), n) && rd(f, D.ph.data(), n) &&
+         rd(f, D.pr.data(), n) && rd(f, D.pe.data(), n);
+    fclose(f);
+    if (!ok) { ctx->err = "deck truncated in particle data"; return MCM_GPU_ERR_IO; }
+    for (long i = 0; i < n; i++) {
+        D.pmat[i] -= 1;
+        if (D.pmat[i] < 0 || D.pmat[i] >= D.nummat) { ctx->err = "particle with bad material id"; return MCM_GPU_ERR_ARG; }
+    }
+    // mcm_calcmass.f90:122-137 (massopt 0: equal share of the material mass)
+    if (D.massopt == 0) {
+        std::vector<int> cnt(D.nummat, 0);
+        for (long i = 0; i < n; i++) cnt[D.pmat[i]]++;
+        for (long i = 0; i < n; i++) D.pm[i] = mat_mass[D.pmat[i]] / (double)cnt[D.pmat[i]];
+    }
+    if (D.contacttype != 0) { ctx->err = "repulsive contact (contacttype > 0) is not on the GPU path"; return MCM_GPU_ERR_ARG; }
+    if (D.krtype != 1 && D.krtype != 0) { ctx->err = "only the cubic B-spline kernel exists"; return MCM_GPU_ERR_ARG; }
+    if (D.disctype != 0 && D.disctype != 1) { ctx->err = "disctype must be 0 or 1"; return MCM_GPU_ERR_ARG; }
+    return MCM_GPU_OK;
+}
+
+// ------------------------------------------------------------- allocation
+template <typename T>
+int dalloc(mcm_gpu_ctx* ctx, T** p, size_t n) {
+    void* q = nullptr;
+    if (n == 0) n = 1;
+    HIPCHK(hipMalloc(&q, n * sizeof(T)));
+    HIPCHK(hipMemsetAsync(q, 0, n * sizeof(T), ctx->stream));
+    ctx->allocs.push_back(q);
+    *p = (T*)q;
+    return 0;
+}
+
+void dfree(mcm_gpu_ctx* ctx, void* p) {
+    if (!p) return;
+    auto it = std::find(ctx->allocs.begin(), ctx->allocs.end(), p);
+    if (it != ctx->allocs.end()) ctx->allocs.erase(it);
+    (void)hipFree(p);
+}
+
+// grow an array keeping its first keep elements
+template <typename T>
+int dregrow(mcm_gpu_ctx* ctx, T** p, size_t n_new, size_t keep) {
+    T* q = nullptr;
+    int rc = dalloc(ctx, &q, n_new);
+    if (rc) return rc;
+    if (keep) HIPCHK(hipMemcpyAsync(q, *p, keep * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
+    dfree(ctx, *p);
+    *p = q;
+    return 0;
+}
+
+int alloc_neighbour_arrays(mcm_gpu_ctx* ctx) {
+    DevState& d = ctx->d;
+    int rc = 0;
+    rc |= dalloc(ctx, &d.nbr, (size_t)d.ncap * d.npad);
+    rc |= dalloc(ctx, &d.cell_count, d.cellcap);
+    rc |= dalloc(ctx, &d.cell_start, d.cellcap);
+    for (int k = 0; k < 3; k++) {
+        rc |= dalloc(ctx, &d.gstart[k], d.cellcap);
+        rc |= dalloc(ctx, &d.gcount[k], d.cellcap);
+    }
+    rc |= dalloc(ctx, &d.scan_tmp, d.cellcap / SCAN_TILE + 2);
+    return rc;
+}
+
+void free_neighbour_arrays(mcm_gpu_ctx* ctx) {
+    DevState& d = ctx->d;
+    dfree(ctx, d.nbr); dfree(ctx, d.cell_count); dfree(ctx, d.cell_start); dfree(ctx, d.scan_tmp);
+    for (int k = 0; k < 3; k++) { dfree(ctx, d.gstart[k]); dfree(ctx, d.gcount[k]); }
+}
+
+int alloc_all(mcm_gpu_ctx* ctx) {
+    DevState& d = ctx->d;
+    const size_t n = d.np, u = (size_t)d.np + d.ngcap;
+    int rc = 0;
+    for (int k = 0; k < 3; k++) {
+        rc |= dalloc(ctx, &d.x0[k], u); rc |= dalloc(ctx, &d.x1[k], u); rc |= dalloc(ctx, &d.v[k], u);
+    }
+    rc |= dalloc(ctx, &d.mass, u);
+    for (int k = 0; k < 2; k++) { rc |= dalloc(ctx, &d.hbuf[k], u); rc |= dalloc(ctx, &d.rbuf[k], u); }
+    rc |= dalloc(ctx, &d.pmat, u);
+    for (int k = 0; k < 6; k++) rc |= dalloc(ctx, &d.A[k], u);
+    rc |= dalloc(ctx, &d.gpar, d.ngcap);
+    rc |= dalloc(ctx, &d.dispbc, n);
+    double** per[] = {&d.hold, &d.h0, &d.rho0, &d.p, &d.e, &d.c, &d.vabs, &d.pcut, &d.p_cut, &d.efps, &d.fail,
+                      &d.q, &d.mindist, &d.tracerod, &d.einc, &d.etry, &d.rhoold, &d.critts_p, &d.defps_prev};
+    for (auto pp : per) rc |= dalloc(ctx, pp, n);
+    for (int k = 0; k < 6; k++) {
+        rc |= dalloc(ctx, &d.sig[k], n); rc |= dalloc(ctx, &d.sd[k], n); rc |= dalloc(ctx, &d.alfa[k], n);
+        rc |= dalloc(ctx, &d.rod[k], n);
+    }
+    for (int k = 0; k < 3; k++) { rc |= dalloc(ctx, &d.spin[k], n); rc |= dalloc(ctx, &d.a[k], n); }
+    rc |= dalloc(ctx, &d.yield, n);
+    rc |= dalloc(ctx, &d.cell_of, n); rc |= dalloc(ctx, &d.slot_of, n); rc |= dalloc(ctx, &d.sorted, n);
+    rc |= dalloc(ctx, &d.sx, n); rc |= dalloc(ctx, &d.sy, n); rc |= dalloc(ctx, &d.sz, n); rc |= dalloc(ctx, &d.sh, n);
+    rc |= dalloc(ctx, &d.smat, n);
+    rc |= dalloc(ctx, &d.nreal, n); rc |= dalloc(ctx, &d.ntot, n);
+    rc |= dalloc(ctx, &d.red, (size_t)ctx->nb * RED_C + 64);
+    rc |= dalloc(ctx, &d.mat, d.nummat);
+    rc |= dalloc(ctx, &d.sc, 1);
+    ctx->sc = d.sc;
+    rc |= alloc_neighbour_arrays(ctx);
+    return rc;
+}
+
+// ----------------------------------------------------------------- launches
+int blocks_for(long n, int bs) { return (int)((n + bs - 1) / bs); }
+
+void launch_neighbours(mcm_gpu_ctx* ctx, int cur) {
+    DevState& d = ctx->d;
+    hipStream_t s = ctx->stream;
+    int gcl = std::min(blocks_for(d.cellcap, MCM_BLOCK), 4096);
+    int ntile = blocks_for(d.cellcap, SCAN_TILE);
+    k_grid_setup<<<1, 1, 0, s>>>(d);
+    k_clear_cells<<<gcl, MCM_BLOCK, 0, s>>>(d);
+    k_bin<<<ctx->nb, MCM_BLOCK, 0, s>>>(d);
+    k_scan_tiles<<<ntile, 1024, 0, s>>>(d);
+    k_scan_top<<<1, 1024, 0, s>>>(d);
+    k_scan_apply<<<ntile, 1024, 0, s>>>(d);
+    k_scatter<<<ctx->nb, MCM_BLOCK, 0, s>>>(d);
+    k_sort_cells<<<gcl, MCM_BLOCK, 0, s>>>(d, cur);
+    for (int p = 0; p < d.nplanes; p++) k_ghost_plane<<<1, 1024, 0, s>>>(d, p, cur);
+    k_list<<<ctx->nb, MCM_BLOCK, 0, s>>>(d, cur);
+}
+
+int ghost_blocks(const DevState& d) { return std::max(1, std::min(blocks_for(d.ngcap, MCM_BLOCK), 1024)); }
+
+void launch_strain_const(mcm_gpu_ctx* ctx, int cur) {
+    DevState& d = ctx->d;
+    if (d.disctype == 1) k_strain_const<MODE_FUSED, true, false><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur);
+    else k_strain_const<MODE_FUSED, false, false><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur);
+}
+
+void launch_momentum(mcm_gpu_ctx* ctx, int cur) {
+    DevState& d = ctx->d;
+    if (d.boundary) k_ghost_stress<<<ghost_blocks(d), MCM_BLOCK, 0, ctx->stream>>>(d, cur);
+    if (d.disctype == 1) k_momentum<MODE_FUSED, true, false><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur);
+    else k_momentum<MODE_FUSED, false, false><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur);
+}
+
+void launch_step(mcm_gpu_ctx* ctx, int cur) {
+    launch_neighbours(ctx, cur);
+    launch_strain_const(ctx, cur);
+    launch_momentum(ctx, cur);
+}
+
+int get_scalars_raw(mcm_gpu_ctx* ctx, Scalars* out) {
+    HIPCHK(hipMemcpyAsync(out, ctx->sc, sizeof(Scalars), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int put_scalars_raw(mcm_gpu_ctx* ctx, const Scalars& in) {
+    HIPCHK(hipMemcpyAsync(ctx->sc, &in, sizeof(Scalars), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+void drop_graph(mcm_gpu_ctx* ctx) {
+    if (ctx->graph2) { (void)hipGraphExecDestroy(ctx->graph2); ctx->graph2 = nullptr; }
+}
+
+int build_graph(mcm_gpu_ctx* ctx) {
+    drop_graph(ctx);
+    hipGraph_t g;
+    HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    launch_step(ctx, 0);
+    launch_step(ctx, 1);
+    HIPCHK(hipStreamEndCapture(ctx->stream, &g));
+    HIPCHK(hipGraphInstantiate(&ctx->graph2, g, nullptr, nullptr, 0));
+    HIPCHK(hipGraphDestroy(g));
+    return 0;
+}
+
+// Grow the capacity named by the error flags (neighbour slots, ghost slots,
+// cells), keep the particle state, clear the flags.
+int regrow(mcm_gpu_ctx* ctx, const Scalars& sc_host) {
+    DevState& d = ctx->d;
+    int flags = sc_host.error_flags;
+    drop_graph(ctx);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (flags & 2) {   // ghosts: unified arrays grow, real part kept
+        int ng_new = d.ngcap * 2 + 1024;
+        size_t keep = d.np, u = (size_t)d.np + ng_new;
+        int rc = 0;
+        for (int k = 0; k < 3; k++) {
+            rc |= dregrow(ctx, &d.x0[k], u, keep); rc |= dregrow(ctx, &d.x1[k], u, keep); rc |= dregrow(ctx, &d.v[k], u, keep);
+        }
+        rc |= dregrow(ctx, &d.mass, u, keep);
+        for (int k = 0; k < 2; k++) { rc |= dregrow(ctx, &d.hbuf[k], u, keep); rc |= dregrow(ctx, &d.rbuf[k], u, keep); }
+        rc |= dregrow(ctx, &d.pmat, u, keep);
+        for (int k = 0; k < 6; k++) rc |= dregrow(ctx, &d.A[k], u, keep);
+        rc |= dregrow(ctx, &d.gpar, ng_new, 0);
+        if (rc) return rc;
+        d.ngcap = ng_new;
+    }
+    if (flags & (1 | 4)) {
+        free_neighbour_arrays(ctx);
+        if (flags & 1) d.ncap = std::max(d.ncap + 16, (int)(sc_host.max_count * 1.25) + 16);
+        if (flags & 4) d.cellcap = d.cellcap * 2 + 4096;
+        int rc = alloc_neighbour_arrays(ctx);
+        if (rc) return rc;
+    }
+    Scalars s2 = sc_host;
+    s2.abort = 0;
+    s2.error_flags = 0;
+    s2.done_b = s2.done_c = s2.done_r = 0;
+    return put_scalars_raw(ctx, s2);
+}
+
+// ---------------------------------------------------------- initialisation
+// mcm_initial.f90:33-101: the per-particle part runs once on the host, the
+// first neighbour search, critts and energy totals on the device.
+int initial(mcm_gpu_ctx* ctx) {
+    HostDeck& D = ctx->deck;
+    DevState& d = ctx->d;
+    const long n = D.np;
+    static const int mask[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {0, 1, 1}, {1, 0, 1}, {1, 1, 1}};
+    std::vector<double> v = D.v;
+    for (long i = 0; i < n; i++) {   // mcm_init_boundcond.f90:25-54
+        int bc = D.dispbc[i];
+        if (bc >= 1 && bc <= 7)
+            for (int k = 0; k < 3; k++)
+                if (mask[bc][k]) v[3 * i + k] = 0.0;
+    }
+    std::vector<double> hx[3], hv[3];
+    for (int k = 0; k < 3; k++) { hx[k].resize(n); hv[k].resize(n); }
+    std::vector<double> mass(n), h(n), rho(n), rho0(n), p(n), e(n), c(n), vabs(n), pcut(n), fail(n, 1.0);
+    std::vector<double> sig[6];
+    for (int k = 0; k < 6; k++) sig[k].assign(n, 0.0);
+    Scalars sc{};
+    for (int k = 0; k < 3; k++) { sc.coord_min[k] = 1.0e20; sc.coord_max[k] = -1.0e20; }
+    ctx->xzero.assign(3 * n, 0.0);
+    for (long i = 0; i < n; i++) {
+        const MatParams& M = D.mat[D.pmat[i]];
+        for (int k = 0; k < 3; k++) {
+            hx[k][i] = D.x[3 * i + k];
+            hv[k][i] = v[3 * i + k];
+            ctx->xzero[3 * i + k] = D.x[3 * i + k];
+        }
+        mass[i] = D.pm[i];
+        // mcm_initvbls.f90:48-91
+        if (M.model == 1 || M.model == 3 || M.model == 4) pcut[i] = -9.90E+20;
+        else if (M.model == 9) pcut[i] = M.strinput[0];
+        else if (M.model == 10) pcut[i] = M.strinput[3];
+        double va = 0.0;
+        for (int k = 0; k < 3; k++) va = va + hv[k][i] * hv[k][i];
+        vabs[i] = std::sqrt(va);
+        for (int k = 0; k < 3; k++) {
+            if (hx[k][i] < sc.coord_min[k]) sc.coord_min[k] = hx[k][i];
+            if (hx[k][i] > sc.coord_max[k]) sc.coord_max[k] = hx[k][i];
+        }
+        // mcm_calcrhopc.f90:31-95 with mcm_eoscalc.f90:27-113
+        PState s{};
+        s.mass = mass[i];
+        s.rho0 = M.rho;
+        s.rho = D.init_rhoe == 1 ? D.pr[i] : 0.0;
+        s.e = D.init_rhoe == 1 ? D.pe[i] : 0.0;
+        s.pcut = pcut[i];
+        if (M.model == 1 || M.model == 3) {
+            if (D.init_rhoe != 1) { s.rho = s.rho0; s.e = 0.0; }
+            s.p = 0.0;
+            s.c = std::sqrt(M.strinput[1] / s.rho);
+        } else {
+            const double* ei = M.eosinput;
+            if (D.init_rhoe != 1) {
+                switch (M.eos) {
+                    case 1: s.rho = s.rho0 / ei[8]; s.e = ei[7] * s.mass / s.rho; break;
+                    case 4: s.rho = s.rho0 / ei[7]; s.e = ei[6] * s.mass / s.rho; break;
+                    case 13: s.rho = s.rho0 / ei[3]; s.e = ei[2] * s.mass / s.rho; break;
+                    case 28: s.rho = s.rho0 / ei[3]; s.e = ei[2] * s.mass; break;
+                    case 41: s.rho = s.rho0 / ei[5]; s.e = ei[2] * ei[3] * s.mass / s.rho; break;
+                }
+            }
+            eos_p(s, M);
+            sueos(s, M);
+            if (M.model == 9 && s.p < M.strinput[0]) s.p = M.strinput[0];
+        }
+        rho[i] = s.rho; rho0[i] = s.rho0; p[i] = s.p; e[i] = s.e; c[i] = s.c;
+        for (int k = 0; k < 3; k++) sig[k][i] = -s.p;   // mcm_initsigma.f90:35-43
+        h[i] = D.init_h_opt == 1 ? D.ph[i] : M.h;         // mcm_init_h.f90:22-48
+    }
+    double hmax = 0.0;
+    if (D.init_h_opt == 0) {
+        for (int m = 0; m < D.nummat; m++) hmax = std::max(hmax, D.mat[D.pmat[m]].h);
+    } else {
+        for (long i = 0; i < n; i++) hmax = std::max(hmax, h[i]);
+    }
+    sc.hmax = hmax;
+    sc.timestep = 1;
+    hipStream_t st = ctx->stream;
+    auto up = [&](double* dst, const std::vector<double>& src) {
+        return hipMemcpyAsync(dst, src.data(), n * sizeof(double), hipMemcpyHostToDevice, st);
+    };
+    hipError_t e2 = hipSuccess;
+    for (int k = 0; k < 3; k++) {
+        e2 = e2 ? e2 : up(d.x0[k], hx[k]);
+        e2 = e2 ? e2 : up(d.v[k], hv[k]);
+    }
+    e2 = e2 ? e2 : up(d.mass, mass);
+    for (int b = 0; b < 2; b++) { e2 = e2 ? e2 : up(d.hbuf[b], h); e2 = e2 ? e2 : up(d.rbuf[b], rho); }
+    e2 = e2 ? e2 : up(d.hold, h);
+    e2 = e2 ? e2 : up(d.h0, h);
+    e2 = e2 ? e2 : up(d.rho0, rho0);
+    e2 = e2 ? e2 : up(d.rhoold, rho);
+    e2 = e2 ? e2 : up(d.p, p);
+    e2 = e2 ? e2 : up(d.e, e);
+    e2 = e2 ? e2 : up(d.c, c);
+    e2 = e2 ? e2 : up(d.vabs, vabs);
+    e2 = e2 ? e2 : up(d.pcut, pcut);
+    e2 = e2 ? e2 : up(d.fail, fail);
+    for (int k = 0; k < 6; k++) e2 = e2 ? e2 : up(d.sig[k], sig[k]);
+    if (!e2) e2 = hipMemcpyAsync(d.pmat, D.pmat.data(), n * sizeof(int), hipMemcpyHostToDevice, st);
+    if (!e2) e2 = hipMemcpyAsync(d.dispbc, D.dispbc.data(), n * sizeof(int), hipMemcpyHostToDevice, st);
+    if (!e2) e2 = hipMemcpyAsync(d.mat, D.mat.data(), D.nummat * sizeof(MatParams), hipMemcpyHostToDevice, st);
+    if (e2) { ctx->err = std::string("upload: ") + hipGetErrorString(e2); return MCM_GPU_ERR_HIP; }
+    int rc = put_scalars_raw(ctx, sc);
+    if (rc) return rc;
+    ctx->cur = 0;
+    ctx->phase = 0;
+    // mcm_init_neighb -> calccritts -> dt -> calctotale
+    rc = mcm_gpu_neighbours(ctx);
+    if (rc) return rc;
+    k_calccritts<<<ctx->nb, MCM_BLOCK, 0, st>>>(d, 0);
+    Scalars s3;
+    rc = get_scalars_raw(ctx, &s3);
+    if (rc) return rc;
+    s3.timestep = 1;
+    if (D.itss != 0.0) s3.dt = std::min(D.itss, s3.critts * D.tssf);
+    else s3.dt = s3.critts * D.tssf;
+    rc = put_scalars_raw(ctx, s3);
+    if (rc) return rc;
+    k_calctotale<<<ctx->nb, MCM_BLOCK, 0, st>>>(d);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+}
+
+int check_sticky(mcm_gpu_ctx* ctx) {
+    Scalars s;
+    int rc = get_scalars_raw(ctx, &s);
+    if (rc) return rc;
+    if (s.abort) {
+        ctx->err = "capacity exceeded (flags " + std::to_string(s.error_flags) + ")";
+        return MCM_GPU_ERR_CAPACITY;
+    }
+    return 0;
+}
+
+}  // namespace
+
+// =============================================================== C ABI
+extern "C" {
+
+int mcm_gpu_create_from_deck(const char* deck_path, const mcm_gpu_options* opt, mcm_gpu_ctx** out) {
+    if (!out || !deck_path) return MCM_GPU_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return MCM_GPU_ERR_NODEVICE;
+    mcm_gpu_ctx* ctx = new mcm_gpu_ctx();
+    *out = ctx;
+    ctx->device = opt ? opt->device : 0;
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    int rc = read_deck(ctx, deck_path);
+    if (rc) return rc;
+    HostDeck& D = ctx->deck;
+    DevState& d = ctx->d;
+    d.np = D.np;
+    d.npad = (D.np + 63) / 64 * 64;
+    d.nummat = D.nummat;
+    d.disctype = D.disctype;
+    d.h_opt = D.h_opt;
+    d.tcrit_opt = D.tcrit_opt;
+    d.tssf = D.tssf;
+    d.pi = acos(-1.0);
+    d.ngcap = (opt && opt->ghost_capacity > 0) ? opt->ghost_capacity : D.np / 2 + 4096;
+    d.ncap = (opt && opt->nbr_capacity > 0) ? opt->nbr_capacity : 128;
+    // boundary planes (mcm_inp_symmetry_v2.f90:83-137, mcm_init_boundary.f90:28-116)
+    d.boundary = 0;
+    d.nplanes = 0;
+    for (int k = 0; k < 6; k++)
+        if (D.mirr_opt == 1 && D.sym_flag[k] != 0) d.boundary = 1;
+    for (int dim = 0; dim < 3; dim++) {
+        for (int side = 0; side < 2; side++) {
+            int code = D.mirr_opt == 1 ? D.sym_flag[2 * dim + side] : 0;
+            d.bcode[dim][side] = code;
+            d.bx[dim][side] = D.sym_x[2 * dim + side];
+        }
+        d.btype[dim] = 1 + (d.bcode[dim][0] > 0 ? 1 : 0) + (d.bcode[dim][1] > 0 ? 2 : 0);
+    }
+    for (int dim = 0; dim < 3; dim++)
+        for (int side = 0; side < 2; side++) {
+            int code = d.bcode[dim][side];
+            if (code != 1 && code != 2) continue;
+            PlaneDev& P = d.planes[d.nplanes++];
+            P.dim = dim; P.side = side; P.code = code; P.pad_ = 0;
+            for (int k = 0; k < 3; k++) { P.mult[k] = 1.0; P.add[k] = 0.0; }
+            if (code == 1) {
+                P.mult[dim] = -1.0;
+                P.add[dim] = 2.0 * d.bx[dim][side];
+            } else {
+                double span = d.bx[dim][1] - d.bx[dim][0];
+                P.add[dim] = side == 0 ? -span : span;
+            }
+        }
+    // cell capacity from the initial bounding box, with room for the bar to spread
+    {
+        double hmax = 0.0;
+        for (int m = 0; m < D.nummat; m++) hmax = std::max(hmax, D.mat[m].h);
+        if (D.init_h_opt == 1) for (int i = 0; i < D.np; i++) hmax = std::max(hmax, D.ph[i]);
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int i = 0; i < D.np; i++)
+            for (int k = 0; k < 3; k++) { lo[k] = std::min(lo[k], D.x[3 * i + k]); hi[k] = std::max(hi[k], D.x[3 * i + k]); }
+        double cells = 1.0;
+        for (int k = 0; k < 3; k++) {
+            double span = (hi[k] - lo[k]);
+            if (d.bcode[k][0] || d.bcode[k][1]) span = std::max(span, std::fabs(d.bx[k][1] - d.bx[k][0]));
+            cells *= (span / (2.0 * hmax) + 4.0) * 2.0;
+        }
+        d.cellcap = (int)std::min(cells + 4096.0, 2.0e9);
+    }
+    ctx->nb = blocks_for(D.np, MCM_BLOCK);
+    rc = alloc_all(ctx);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return MCM_GPU_OK;
+}
+
+void mcm_gpu_destroy(mcm_gpu_ctx* ctx) {
+    if (!ctx) return;
+    drop_graph(ctx);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (void* p : ctx->allocs) (void)hipFree(p);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* mcm_gpu_last_error(const mcm_gpu_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int mcm_gpu_initial(mcm_gpu_ctx* ctx) {
+    if (!ctx) return MCM_GPU_ERR_ARG;
+    return initial(ctx);
+}
+
+int mcm_gpu_neighbours(mcm_gpu_ctx* ctx) {
+    if (!ctx || ctx->phase != 0) return MCM_GPU_ERR_ARG;
+    for (int attempt = 0; attempt < 8; attempt++) {
+        launch_neighbours(ctx, ctx->cur);
+        HIPCHK(hipGetLastError());
+        Scalars s;
+        int rc = get_scalars_raw(ctx, &s);
+        if (rc) return rc;
+        if (!s.abort) return MCM_GPU_OK;
+        rc = regrow(ctx, s);   // mcm_ll_neighbours.f90:44-311 regrow, done right
+        if (rc) return rc;
+    }
+    ctx->err = "neighbour search did not fit after regrowing";
+    return MCM_GPU_ERR_CAPACITY;
+}
+
+int mcm_gpu_strain(mcm_gpu_ctx* ctx) {
+    if (!ctx || ctx->phase != 0) return MCM_GPU_ERR_ARG;
+    DevState& d = ctx->d;
+    if (d.disctype == 1) k_strain_const<MODE_STRAIN, true, true><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, ctx->cur);
+    else k_strain_const<MODE_STRAIN, false, true><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, ctx->cur);
+    HIPCHK(hipGetLastError());
+    ctx->phase = 1;
+    return 0;
+}
+
+int mcm_gpu_update_h(mcm_gpu_ctx* ctx) {
+    if (!ctx || ctx->phase != 1) return MCM_GPU_ERR_ARG;
+    if (ctx->d.h_opt > 0) k_update_h<<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(ctx->d, ctx->cur);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int mcm_gpu_constitutive(mcm_gpu_ctx* ctx) {
+    if (!ctx || ctx->phase != 1) return MCM_GPU_ERR_ARG;
+    k_strain_const<MODE_CONST, false, true><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(ctx->d, ctx->cur);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int mcm_gpu_dt_update(mcm_gpu_ctx* ctx) {
+    if (!ctx) return MCM_GPU_ERR_ARG;
+    k_dt_update<<<1, 1, 0, ctx->stream>>>(ctx->d);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int mcm_gpu_momentum(mcm_gpu_ctx* ctx) {
+    if (!ctx || ctx->phase != 1) return MCM_GPU_ERR_ARG;
+    DevState& d = ctx->d;
+    if (d.boundary) k_ghost_stress<<<ghost_blocks(d), MCM_BLOCK, 0, ctx->stream>>>(d, ctx->cur);
+    if (d.disctype == 1) k_momentum<MODE_MOMENTUM, true, true><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, ctx->cur);
+    else k_momentum<MODE_MOMENTUM, false, true><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, ctx->cur);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int mcm_gpu_boundcond(mcm_gpu_ctx* ctx) {
+    if (!ctx) return MCM_GPU_ERR_ARG;
+    k_boundcond<<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(ctx->d);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int mcm_gpu_calctotale(mcm_gpu_ctx* ctx) {
+    if (!ctx) return MCM_GPU_ERR_ARG;
+    k_calctotale<<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(ctx->d);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int mcm_gpu_update_velocity(mcm_gpu_ctx* ctx) {
+    if (!ctx) return MCM_GPU_ERR_ARG;
+    k_update_velocity<<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(ctx->d);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int mcm_gpu_move(mcm_gpu_ctx* ctx) {
+    if (!ctx || ctx->phase != 1) return MCM_GPU_ERR_ARG;
+    k_move<<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(ctx->d);
+    HIPCHK(hipGetLastError());
+    ctx->cur ^= 1;
+    ctx->phase = 0;
+    return 0;
+}
+
+int mcm_gpu_calccritts(mcm_gpu_ctx* ctx, double* critts_out) {
+    if (!ctx) return MCM_GPU_ERR_ARG;
+    int hb = ctx->phase == 1 ? ctx->cur ^ 1 : ctx->cur;
+    k_calccritts<<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(ctx->d, hb);
+    HIPCHK(hipGetLastError());
+    Scalars s;
+    int rc = get_scalars_raw(ctx, &s);
+    if (rc) return rc;
+    if (critts_out) *critts_out = s.critts;
+    return 0;
+}
+
+int mcm_gpu_step(mcm_gpu_ctx* ctx, int32_t nsteps) {
+    if (!ctx || ctx->phase != 0 || nsteps < 0) return MCM_GPU_ERR_ARG;
+    Scalars s0;
+    int rc = get_scalars_raw(ctx, &s0);
+    if (rc) return rc;
+    const int t_start = s0.timestep;
+    const int cur0 = ctx->cur;
+    int done = 0;
+    for (int attempt = 0; attempt < 16; attempt++) {
+        if (!ctx->graph2) {
+            rc = build_graph(ctx);
+            if (rc) return rc;
+        }
+        for (int k = done; k < nsteps;) {
+            if (ctx->cur == 0 && nsteps - k >= 2) {
+                HIPCHK(hipGraphLaunch(ctx->graph2, ctx->stream));
+                k += 2;
+            } else {
+                launch_step(ctx, ctx->cur);
+                HIPCHK(hipGetLastError());
+                ctx->cur ^= 1;
+                k += 1;
+            }
+        }
+        Scalars s;
+        rc = get_scalars_raw(ctx, &s);
+        if (rc) return rc;
+        // the step counter advances only for steps that completed, so a
+        // step that hit a capacity limit is redone from its start
+        done = s.timestep - t_start;
+        ctx->cur = cur0 ^ (done & 1);
+        if (!s.abort) return MCM_GPU_OK;
+        rc = regrow(ctx, s);
+        if (rc) return rc;
+    }
+    ctx->err = "fused step did not fit after regrowing";
+    return MCM_GPU_ERR_CAPACITY;
+}
+
+int mcm_gpu_get_scalars(mcm_gpu_ctx* ctx, mcm_gpu_scalars* out) {
+    if (!ctx || !out) return MCM_GPU_ERR_ARG;
+    Scalars s;
+    int rc = get_scalars_raw(ctx, &s);
+    if (rc) return rc;
+    memset(out, 0, sizeof(*out));
+    out->dt = s.dt; out->dtold = s.dtold; out->ptime = s.ptime; out->critts = s.critts; out->hmax = s.hmax;
+    out->kinetice = s.kinetice; out->internale = s.internale; out->totale = s.totale;
+    for (int k = 0; k < 3; k++) { out->coord_min[k] = s.coord_min[k]; out->coord_max[k] = s.coord_max[k]; }
+    out->np = ctx->d.np; out->ngp = s.ngp; out->max_count = s.max_count; out->timestep = s.timestep;
+    out->error_flags = s.error_flags;
+    out->nummat = ctx->d.nummat;
+    return 0;
+}
+
+static int copy_d2h(mcm_gpu_ctx* ctx, double* dst, const double* src, size_t n) {
+    HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    return 0;
+}
+
+int mcm_gpu_download(mcm_gpu_ctx* ctx, int32_t field, double* host, int64_t n) {
+    if (!ctx || !host) return MCM_GPU_ERR_ARG;
+    DevState& d = ctx->d;
+    const size_t np = d.np;
+    const int hb = ctx->phase == 1 ? ctx->cur ^ 1 : ctx->cur;
+    std::vector<double> tmp(np);
+    auto need = [&](size_t k) { return (size_t)n >= k * np; };
+    int rc = 0;
+    switch (field) {
+        case MCM_F_X: case MCM_F_V: case MCM_F_A: {
+            if (!need(3)) return MCM_GPU_ERR_ARG;
+            double* const* src = field == MCM_F_X ? (ctx->phase == 1 ? d.x1 : d.x0) : (field == MCM_F_V ? d.v : d.a);
+            for (int k = 0; k < 3; k++) {
+                rc |= copy_d2h(ctx, tmp.data(), src[k], np);
+                HIPCHK(hipStreamSynchronize(ctx->stream));
+                for (size_t i = 0; i < np; i++) host[3 * i + k] = tmp[i];
+            }
+            return rc;
+        }
+        case MCM_F_SIGMA: {
+            if (!need(9)) return MCM_GPU_ERR_ARG;
+            static const int map[9] = {S11, S12, S13, S12, S22, S23, S13, S23, S33};
+            for (int k = 0; k < 6; k++) {
+                rc |= copy_d2h(ctx, tmp.data(), d.sig[k], np);
+                HIPCHK(hipStreamSynchronize(ctx->stream));
+                for (int e = 0; e < 9; e++)
+                    if (map[e] == k)
+                        for (size_t i = 0; i < np; i++) host[9 * i + e] = tmp[i];
+            }
+            return rc;
+        }
+        default: break;
+    }
+    if (!need(1)) return MCM_GPU_ERR_ARG;
+    const double* src = nullptr;
+    switch (field) {
+        case MCM_F_RHO: src = d.rbuf[hb]; break;
+        case MCM_F_H: src = d.hbuf[hb]; break;
+        case MCM_F_P: src = d.p; break;
+        case MCM_F_E: src = d.e; break;
+        case MCM_F_C: src = d.c; break;
+        case MCM_F_EFPS: src = d.efps; break;
+        case MCM_F_VABS: src = d.vabs; break;
+        case MCM_F_MASS: src = d.mass; break;
+        default: return MCM_GPU_ERR_ARG;
+    }
+    rc = copy_d2h(ctx, host, src, np);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return rc;
+}
+
+int mcm_gpu_download_neighbours(mcm_gpu_ctx* ctx, int32_t* counts, int32_t* gcounts, int32_t* ids, int64_t ids_len,
+                                int32_t* cap) {
+    if (!ctx || !counts || !gcounts || !ids || !cap) return MCM_GPU_ERR_ARG;
+    DevState& d = ctx->d;
+    const size_t np = d.np;
+    std::vector<int> nr(np), nt(np), lst((size_t)d.ncap * d.npad);
+    HIPCHK(hipMemcpyAsync(nr.data(), d.nreal, np * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(nt.data(), d.ntot, np * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(lst.data(), d.nbr, lst.size() * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    *cap = d.ncap;
+    if (ids_len < (int64_t)np * d.ncap) return MCM_GPU_ERR_ARG;
+    for (size_t i = 0; i < np; i++) {
+        counts[i] = nr[i];
+        gcounts[i] = nt[i] - nr[i];
+        for (int k = 0; k < nt[i] && k < d.ncap; k++) ids[i * d.ncap + k] = lst[(size_t)k * d.npad + i];
+    }
+    return 0;
+}
+
+int mcm_gpu_dump(mcm_gpu_ctx* ctx, const char* path, int32_t step, int32_t stage) {
+    if (!ctx || !path) return MCM_GPU_ERR_ARG;
+    DevState& d = ctx->d;
+    const size_t np = d.np;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    Scalars s;
+    int rc = get_scalars_raw(ctx, &s);
+    if (rc) return rc;
+    const int hb = ctx->phase == 1 ? ctx->cur ^ 1 : ctx->cur;
+    auto get = [&](const double* src, size_t n) {
+        std::vector<double> v(n);
+        (void)hipMemcpy(v.data(), src, n * sizeof(double), hipMemcpyDeviceToHost);
+        return v;
+    };
+    auto geti = [&](const int* src, size_t n) {
+        std::vector<int> v(n);
+        (void)hipMemcpy(v.data(), src, n * sizeof(int), hipMemcpyDeviceToHost);
+        return v;
+    };
+    const size_t ng = s.ngp;
+    std::vector<double> X[3], V[3], Acc[3], SG[6], SD[6], AL[6], ROD[6], SP[3];
+    for (int k = 0; k < 3; k++) {
+        X[k] = get(ctx->phase == 1 ? d.x1[k] : d.x0[k], np + ng);
+        V[k] = get(d.v[k], np + ng);
+        Acc[k] = get(d.a[k], np);
+        SP[k] = get(d.spin[k], np);
+    }
+    for (int k = 0; k < 6; k++) {
+        SG[k] = get(d.sig[k], np); SD[k] = get(d.sd[k], np); AL[k] = get(d.alfa[k], np); ROD[k] = get(d.rod[k], np);
+    }
+    auto rho = get(d.rbuf[hb], np + ng), h = get(d.hbuf[hb], np + ng), mass = get(d.mass, np + ng);
+    auto rhoold = get(d.rhoold, np), hold = get(d.hold, np), p = get(d.p, np), e = get(d.e, np);
+    auto etry = get(d.etry, np), einc = get(d.einc, np), c = get(d.c, np), vabs = get(d.vabs, np);
+    auto tr = get(d.tracerod, np), pcut = get(d.pcut, np), p_cut = get(d.p_cut, np), efps = get(d.efps, np);
+    auto fail = get(d.fail, np), mind = get(d.mindist, np), crit = get(d.critts_p, np), rho0 = get(d.rho0, np);
+    auto h0 = get(d.h0, np), q = get(d.q, np);
+    auto nreal = geti(d.nreal, np), ntot = geti(d.ntot, np), cell_of = geti(d.cell_of, np);
+    auto lst = geti(d.nbr, (size_t)d.ncap * d.npad);
+    auto gpar = geti(d.gpar, ng), pmat = geti(d.pmat + np, ng);
+    FILE* f = fopen(path, "wb");
+    if (!f) { ctx->err = "cannot write dump"; return MCM_GPU_ERR_IO; }
+    int32_t hdr[6] = {0x4D434D44, step, stage, (int32_t)np, (int32_t)ng, d.nummat};
+    fwrite(hdr, 4, 6, f);
+    double scv[8] = {s.dt, s.dtold, s.ptime, s.critts, s.hmax, s.kinetice, s.internale, s.totale};
+    fwrite(scv, 8, 8, f);
+    double gr[12];
+    for (int k = 0; k < 3; k++) { gr[2 * k] = s.coord_min[k]; gr[2 * k + 1] = s.coord_max[k]; }
+    for (int k = 0; k < 3; k++) { gr[6 + k] = s.grid.gridmin[k]; gr[9 + k] = s.grid.gridsize[k]; }
+    fwrite(gr, 8, 12, f);
+    fwrite(s.grid.lim, 4, 3, f);
+    for (int m = 0; m < d.nummat; m++) {
+        double mm[5] = {s.mat_ke[m], s.mat_ie[m], s.mat_mom[m][0], s.mat_mom[m][1], s.mat_mom[m][2]};
+        fwrite(mm, 8, 5, f);
+    }
+    static const int map[9] = {S11, S12, S13, S12, S22, S23, S13, S23, S33};
+    double row[86];
+    for (size_t i = 0; i < np; i++) {
+        for (int k = 0; k < 3; k++) { row[k] = X[k][i]; row[3 + k] = V[k][i]; row[6 + k] = Acc[k][i]; }
+        row[9] = rho[i]; row[10] = rhoold[i]; row[11] = h[i]; row[12] = hold[i]; row[13] = p[i]; row[14] = e[i];
+        row[15] = etry[i]; row[16] = einc[i]; row[17] = c[i]; row[18] = vabs[i]; row[19] = tr[i]; row[20] = pcut[i];
+        row[21] = p_cut[i]; row[22] = efps[i]; row[23] = fail[i]; row[24] = mind[i]; row[25] = crit[i];
+        row[26] = mass[i]; row[27] = rho0[i];
+        for (int k = 0; k < 9; k++) {
+            int m = map[k];
+            row[28 + k] = SG[m][i]; row[37 + k] = SD[m][i]; row[46 + k] = AL[m][i];
+            row[55 + k] = (k == 0 || k == 4 || k == 8) ? q[i] : 0.0;
+            row[64 + k] = ROD[m][i];
+        }
+        // spin: column-major (r,c) -> W12 at (0,1)=k3, W13 at (0,2)=k6, W23 at (1,2)=k7
+        double w12 = SP[0][i], w13 = SP[1][i], w23 = SP[2][i];
+        double sp[9] = {0.0, -w12, -w13, w12, 0.0, -w23, w13, w23, 0.0};
+        for (int k = 0; k < 9; k++) row[73 + k] = sp[k];
+        row[82] = h0[i];
+        for (int k = 0; k < 3; k++) row[83 + k] = ctx->xzero[3 * i + k];
+        fwrite(row, 8, 86, f);
+    }
+    std::vector<int32_t> nn(np), gn(np), llp(np);
+    for (size_t i = 0; i < np; i++) { nn[i] = nreal[i]; gn[i] = ntot[i] - nreal[i]; llp[i] = cell_of[i] >= 0 ? 0 : -1; }
+    fwrite(nn.data(), 4, np, f);
+    fwrite(gn.data(), 4, np, f);
+    fwrite(llp.data(), 4, np, f);
+    for (size_t i = 0; i < np; i++)
+        for (int k = 0; k < nreal[i] && k < d.ncap; k++) {
+            int32_t v = lst[(size_t)k * d.npad + i] + 1;
+            fwrite(&v, 4, 1, f);
+        }
+    if (d.boundary) {
+        for (size_t i = 0; i < np; i++)
+            for (int k = nreal[i]; k < ntot[i] && k < d.ncap; k++) {
+                int32_t v = lst[(size_t)k * d.npad + i] - (int32_t)np + 1;
+                fwrite(&v, 4, 1, f);
+            }
+        for (size_t g = 0; g < ng; g++) { int32_t v = gpar[g] + 1; fwrite(&v, 4, 1, f); }
+        for (size_t g = 0; g < ng; g++) { int32_t v = pmat[g] + 1; fwrite(&v, 4, 1, f); }
+        for (size_t g = 0; g < ng; g++) {
+            size_t u = np + g;
+            double r8[8] = {X[0][u], X[1][u], X[2][u], V[0][u], V[1][u], V[2][u], mass[u], h[u]};
+            fwrite(r8, 8, 8, f);
+        }
+        for (size_t g = 0; g < ng; g++) fwrite(&rho[np + g], 8, 1, f);
+        double z = 0.0;
+        for (size_t g = 0; g < ng; g++) fwrite(&z, 8, 1, f);
+    }
+    fclose(f);
+    return 0;
+}
+
+int mcm_gpu_time_steps(mcm_gpu_ctx* ctx, int32_t nsteps, double* ms) {
+    if (!ctx || !ms || nsteps <= 0 || ctx->phase != 0) return MCM_GPU_ERR_ARG;
+    hipEvent_t ev[4];
+    for (int k = 0; k < 4; k++) HIPCHK(hipEventCreate(&ev[k]));
+    double acc[4] = {0, 0, 0, 0};
+    for (int it = 0; it < nsteps; it++) {
+        HIPCHK(hipEventRecord(ev[0], ctx->stream));
+        launch_neighbours(ctx, ctx->cur);
+        HIPCHK(hipEventRecord(ev[1], ctx->stream));
+        launch_strain_const(ctx, ctx->cur);
+        HIPCHK(hipEventRecord(ev[2], ctx->stream));
+        launch_momentum(ctx, ctx->cur);
+        HIPCHK(hipEventRecord(ev[3], ctx->stream));
+        HIPCHK(hipEventSynchronize(ev[3]));
+        float t;
+        HIPCHK(hipEventElapsedTime(&t, ev[0], ev[1])); acc[0] += t;
+        HIPCHK(hipEventElapsedTime(&t, ev[1], ev[2])); acc[1] += t;
+        HIPCHK(hipEventElapsedTime(&t, ev[2], ev[3])); acc[2] += t;
+        HIPCHK(hipEventElapsedTime(&t, ev[0], ev[3])); acc[3] += t;
+        ctx->cur ^= 1;
+    }
+    for (int k = 0; k < 4; k++) { ms[k] = acc[k] / nsteps; (void)hipEventDestroy(ev[k]); }
+    return check_sticky(ctx);
+}
+
+}  // extern "C"

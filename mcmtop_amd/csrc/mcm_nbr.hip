@@ -1,0 +1,418 @@
+// mcm_nbr.hip -- neighbour search of one MCM step on gfx950.
+//
+// Replaces mcm_setuplist.f90:30-135, mcm_ghost_setup.f90:32-425 and
+// mcm_ll_neighbours.f90:44-311.  The reference pushes particles onto
+// per-cell linked lists every step; here the cells are built by a counting
+// sort (atomic histogram, exclusive scan, scatter) into cell-sorted copies of
+// x and h, and each cell's members are ordered by descending particle id, the
+// order the reference's pushed lists are walked in.  Ghost (symmetry /
+// periodic) particles are generated on the device, plane by plane, with ids
+// in the reference's creation order, so the neighbour lists come out in the
+// reference's order: cells in z-y-x order, descending id within a cell,
+// same-material before other-material, self (corrected SPH), then ghosts.
+#pragma once
+#include "mcm_device.h"
+
+// ---------------------------------------------------------------- helpers
+__device__ __forceinline__ int wave_incl_scan(int x) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    return x;
+}
+
+// exclusive scan over a 1024-thread block; *total receives the block sum
+__device__ int block_excl_scan_1024(int v, int* total) {
+    __shared__ int wsum[16];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int inc = wave_incl_scan(v);
+    if (lane == 63) wsum[wid] = inc;
+    __syncthreads();
+    if (wid == 0) {
+        int w = lane < 16 ? wsum[lane] : 0;
+        w = wave_incl_scan(w);
+        if (lane < 16) wsum[lane] = w;
+    }
+    __syncthreads();
+    int base = wid > 0 ? wsum[wid - 1] : 0;
+    *total = wsum[15];
+    __syncthreads();
+    return base + inc - v;
+}
+
+// (int)(t) + 1 as the reference computes a 1-based cell index; out-of-range
+// values map to 0 (outside the sort domain)
+__device__ __forceinline__ int cell_index(double t) {
+    if (!(t > -1.0e9 && t < 1.0e9)) return 0;
+    return (int)t + 1;
+}
+
+// ------------------------------------------------------------ grid set-up
+// mcm_setuplist.f90:240-291, on the device from hmax and the last coordinate
+// bounds; also resets the per-step counters.
+__global__ void k_grid_setup(DevState d) {
+    Scalars* sc = d.sc;
+    GridDev g;
+    const double factor = 2.0;
+    for (int i = 0; i < 3; i++) {
+        g.gridmin[i] = 0.0;
+        g.lim[i] = 1;
+        g.gridsize[i] = 1.0 / (factor * sc->hmax);
+    }
+    bool bad = false;
+    for (int i = 0; i < 3; i++) {
+        double t = 0.0;
+        switch (d.btype[i]) {
+            case 1:
+                g.gridmin[i] = sc->coord_min[i] - 0.01 * sc->hmax;
+                t = g.gridsize[i] * (sc->coord_max[i] - g.gridmin[i]);
+                break;
+            case 2:
+                g.gridmin[i] = d.bx[i][0];
+                t = g.gridsize[i] * (sc->coord_max[i] - g.gridmin[i]);
+                break;
+            case 3:
+                t = g.gridsize[i] * (d.bx[i][1] - sc->coord_min[i]);
+                break;
+            case 4:
+                t = g.gridsize[i] * (d.bx[i][1] - d.bx[i][0]) - 1.0;
+                break;
+        }
+        if (!(t > -1.0e9 && t < 1.0e9)) { bad = true; t = 0.0; }
+        g.lim[i] = (int)t + 1;
+        if (d.btype[i] == 3) g.gridmin[i] = d.bx[i][1] - g.lim[i] * factor * sc->hmax;
+        if (d.btype[i] == 4) {
+            if (g.lim[i] < 1) g.lim[i] = 1;
+            g.gridsize[i] = (double)(float)g.lim[i] / (d.bx[i][1] - d.bx[i][0]);
+        }
+    }
+    for (int i = 0; i < 3; i++) {
+        int c0 = d.bcode[i][0], c1 = d.bcode[i][1];
+        if (c0 == 1 || c0 == 2) {
+            g.lim[i] += 1;
+            g.gridmin[i] = g.gridmin[i] - (1.0 / g.gridsize[i]);
+        }
+        if (c1 == 1 || c1 == 2) g.lim[i] += 1;
+        if (g.lim[i] < 1) bad = true;
+    }
+    long long ncell = (long long)g.lim[0] * g.lim[1] * g.lim[2];
+    if (sc->abort) return;   // sticky until the host has grown the capacity
+    if (bad || ncell > (long long)d.cellcap) {
+        sc->error_flags |= 4;
+        sc->abort = 1;
+        g.ncell = 0;
+    } else {
+        g.ncell = (int)ncell;
+    }
+    sc->grid = g;
+    sc->ngp = 0;
+    sc->max_count = 0;
+}
+
+__global__ void k_clear_cells(DevState d) {
+    int ncell = d.sc->grid.ncell;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += gridDim.x * blockDim.x) {
+        d.cell_count[c] = 0;
+        d.gcount[0][c] = 0; d.gcount[1][c] = 0; d.gcount[2][c] = 0;
+    }
+}
+
+// histogram: cell of every real particle (mcm_setuplist.f90:320-345)
+__global__ __launch_bounds__(256) void k_bin(DevState d) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.np || d.sc->abort) return;
+    const GridDev g = d.sc->grid;
+    int box[3];
+    bool in = true;
+    for (int j = 0; j < 3; j++) {
+        box[j] = cell_index(g.gridsize[j] * (d.x0[j][i] - g.gridmin[j]));
+        if (box[j] < 1 || box[j] > g.lim[j]) in = false;
+    }
+    if (in) {
+        int c = ((box[2] - 1) * g.lim[1] + (box[1] - 1)) * g.lim[0] + (box[0] - 1);
+        d.cell_of[i] = c;
+        d.slot_of[i] = atomicAdd(&d.cell_count[c], 1);
+    } else {
+        d.cell_of[i] = -1;
+    }
+}
+
+// exclusive scan of cell_count -> cell_start, in three launches
+#define SCAN_ITEMS 4
+#define SCAN_TILE (1024 * SCAN_ITEMS)
+__global__ __launch_bounds__(1024) void k_scan_tiles(DevState d) {
+    int ncell = d.sc->grid.ncell;
+    int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    if (blockIdx.x * SCAN_TILE >= ncell) return;
+    int s = 0;
+    for (int k = 0; k < SCAN_ITEMS; k++) s += (base + k < ncell) ? d.cell_count[base + k] : 0;
+    int total;
+    block_excl_scan_1024(s, &total);
+    if (threadIdx.x == 0) d.scan_tmp[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_top(DevState d) {
+    int ntile = (d.sc->grid.ncell + SCAN_TILE - 1) / SCAN_TILE;
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int c0 = 0; c0 < ntile; c0 += 1024) {
+        int k = c0 + threadIdx.x;
+        int v = k < ntile ? d.scan_tmp[k] : 0;
+        int total;
+        int ex = block_excl_scan_1024(v, &total);
+        if (k < ntile) d.scan_tmp[k] = carry + ex;
+        __syncthreads();
+        if (threadIdx.x == 0) carry += total;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_scan_apply(DevState d) {
+    int ncell = d.sc->grid.ncell;
+    if (blockIdx.x * SCAN_TILE >= ncell) return;
+    int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    int vals[SCAN_ITEMS];
+    int s = 0;
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        vals[k] = (base + k < ncell) ? d.cell_count[base + k] : 0;
+        s += vals[k];
+    }
+    int total;
+    int ex = block_excl_scan_1024(s, &total) + d.scan_tmp[blockIdx.x];
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        if (base + k < ncell) d.cell_start[base + k] = ex;
+        ex += vals[k];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_scatter(DevState d) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.np || d.sc->abort) return;
+    int c = d.cell_of[i];
+    if (c >= 0) d.sorted[d.cell_start[c] + d.slot_of[i]] = i;
+}
+
+// order each cell by descending id (the walk order of the reference's pushed
+// lists) and lay down cell-sorted copies of x, h and the material
+__global__ __launch_bounds__(256) void k_sort_cells(DevState d, int cur) {
+    int ncell = d.sc->grid.ncell;
+    const double* h = d.hbuf[cur];
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += gridDim.x * blockDim.x) {
+        int n = d.cell_count[c];
+        if (n == 0) continue;
+        int* seg = d.sorted + d.cell_start[c];
+        for (int a = 1; a < n; a++) {
+            int key = seg[a];
+            int b = a - 1;
+            while (b >= 0 && seg[b] < key) { seg[b + 1] = seg[b]; b--; }
+            seg[b + 1] = key;
+        }
+        int st = d.cell_start[c];
+        for (int a = 0; a < n; a++) {
+            int j = seg[a];
+            d.sx[st + a] = d.x0[0][j]; d.sy[st + a] = d.x0[1][j]; d.sz[st + a] = d.x0[2][j];
+            d.sh[st + a] = h[j]; d.smat[st + a] = d.pmat[j];
+        }
+    }
+}
+
+// ------------------------------------------------------------------ ghosts
+__device__ __forceinline__ void make_ghost(const DevState& d, int cur, const PlaneDev& P, bool from_real, int src,
+                                           int gid) {
+    int u = d.np + gid;
+    int us = from_real ? src : d.np + src;
+    for (int k = 0; k < 3; k++) {
+        double xv = d.x0[k][us] * P.mult[k] + P.add[k];
+        d.x0[k][u] = xv;
+        d.x1[k][u] = xv;
+        d.v[k][u] = d.v[k][us] * P.mult[k];
+    }
+    d.mass[u] = d.mass[us];
+    double hv = d.hbuf[cur][us], rv = d.rbuf[cur][us];
+    d.hbuf[0][u] = hv; d.hbuf[1][u] = hv;
+    d.rbuf[0][u] = rv; d.rbuf[1][u] = rv;
+    d.pmat[u] = d.pmat[us];
+    d.gpar[gid] = from_real ? src : d.gpar[src];
+}
+
+// One plane of mcm_ghost_setup.f90:32-266.  A single 1024-thread workgroup
+// walks the source layer's cells in ascending cell id (the reference's loop
+// order), counts each cell's members (its ghosts if it holds any, else its
+// real particles), scans the counts to assign ghost ids in creation order and
+// records each target cell's run of ghosts.
+__global__ __launch_bounds__(1024) void k_ghost_plane(DevState d, int pi, int cur) {
+    Scalars* sc = d.sc;
+    if (sc->abort) return;
+    const PlaneDev P = d.planes[pi];
+    const GridDev g = sc->grid;
+    const int dim = P.dim;
+    // source and target layers, mcm_ghost_setup.f90:38-43 and siblings
+    // (1-based: symmetry copies layer 2 / lim-1, periodic the opposite end)
+    const int lim = g.lim[dim];
+    const int S = (P.side == 0 ? (P.code == 1 ? 2 : lim - 1) : (P.code == 1 ? lim - 1 : 2)) - 1;
+    const int T = (P.side == 0 ? 1 : lim) - 1;
+    const int L0 = g.lim[0], L1 = g.lim[1];
+    const int nlayer = g.ncell / g.lim[dim];
+    __shared__ int s_ngp;
+    __shared__ int s_over;
+    if (threadIdx.x == 0) { s_ngp = sc->ngp; s_over = 0; }
+    __syncthreads();
+    for (int r0 = 0; r0 < nlayer; r0 += 1024) {
+        int r = r0 + threadIdx.x;
+        int cnt = 0, c = -1, tc = -1, gsum = 0;
+        if (r < nlayer) {
+            int i, j, k;
+            if (dim == 2) { i = r % L0; j = r / L0; k = S; }
+            else if (dim == 1) { i = r % L0; k = r / L0; j = S; }
+            else { j = r % L1; k = r / L1; i = S; }
+            c = (k * L1 + j) * L0 + i;
+            if (dim == 2) tc = (T * L1 + j) * L0 + i;
+            else if (dim == 1) tc = (k * L1 + T) * L0 + i;
+            else tc = (k * L1 + j) * L0 + T;
+            gsum = d.gcount[0][c] + d.gcount[1][c] + d.gcount[2][c];
+            cnt = gsum > 0 ? gsum : d.cell_count[c];
+        }
+        int total;
+        int ex = block_excl_scan_1024(cnt, &total);
+        int base = s_ngp + ex;
+        if (s_ngp + total > d.ngcap) {
+            if (threadIdx.x == 0) s_over = 1;
+        } else if (cnt > 0) {
+            int m = 0;
+            if (gsum > 0) {
+                for (int dd = 2; dd >= 0; dd--) {
+                    int st = d.gstart[dd][c], n = d.gcount[dd][c];
+                    for (int q = n - 1; q >= 0; q--) make_ghost(d, cur, P, false, st + q, base + m++);
+                }
+            } else {
+                int st = d.cell_start[c];
+                for (int q = 0; q < cnt; q++) make_ghost(d, cur, P, true, d.sorted[st + q], base + m++);
+            }
+            d.gstart[dim][tc] = base;
+            d.gcount[dim][tc] = cnt;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0 && !s_over) s_ngp += total;
+        __syncthreads();
+        if (s_over) break;
+    }
+    if (threadIdx.x == 0) {
+        if (s_over) { sc->error_flags |= 2; sc->abort = 1; }
+        sc->ngp = s_ngp;
+    }
+}
+
+// ------------------------------------------------------------ list build
+// mcm_ll_neighbours.f90:96-299 for one real particle; lists are written in
+// [slot][particle] layout so that the pair kernels' loads coalesce.
+__global__ __launch_bounds__(256) void k_list(DevState d, int cur) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    Scalars* sc = d.sc;
+    if (i >= d.np || sc->abort) return;
+    const GridDev g = sc->grid;
+    const int L0 = g.lim[0], L1 = g.lim[1];
+    const int npad = d.npad, ncap = d.ncap;
+    int* out = d.nbr + i;
+    int cnt = 0;
+    double mind = 1.0e20, mind_cp = 1.0e20;
+    const int c = d.cell_of[i];
+    int nreal;
+    if (c >= 0) {
+        const double xi = d.x0[0][i], yi = d.x0[1][i], zi = d.x0[2][i];
+        const double hi = d.hbuf[cur][i];
+        const int mi = d.pmat[i];
+        int b0 = c % L0, b1 = (c / L0) % L1, b2 = c / (L0 * L1);
+        int lo0 = max(b0 - 1, 0), hi0 = min(b0 + 1, L0 - 1);
+        int lo1 = max(b1 - 1, 0), hi1 = min(b1 + 1, L1 - 1);
+        int lo2 = max(b2 - 1, 0), hi2 = min(b2 + 1, g.lim[2] - 1);
+        const int npass = d.nummat > 1 ? 2 : 1;
+        for (int pass = 0; pass < npass; pass++) {
+            for (int n = lo2; n <= hi2; n++)
+                for (int m = lo1; m <= hi1; m++)
+                    for (int l = lo0; l <= hi0; l++) {
+                        int cc = (n * L1 + m) * L0 + l;
+                        if (d.gcount[0][cc] + d.gcount[1][cc] + d.gcount[2][cc] > 0) continue;
+                        int st = d.cell_start[cc], en = st + d.cell_count[cc];
+                        for (int s = st; s < en; s++) {
+                            int j = d.sorted[s];
+                            if (j == i) continue;
+                            double dist_sq = 0.0;
+                            double dx = d.sx[s] - xi; dist_sq = dist_sq + dx * dx;
+                            double dy = d.sy[s] - yi; dist_sq = dist_sq + dy * dy;
+                            double dz = d.sz[s] - zi; dist_sq = dist_sq + dz * dz;
+                            double h_avg = 2.0 * 0.5 * (d.sh[s] + hi);
+                            if (dist_sq < h_avg * h_avg) {
+                                bool same = d.smat[s] == mi;
+                                if (pass == 0 && same) {
+                                    mind = fmin(mind, dist_sq);
+                                    if (cnt < ncap) out[(long)cnt * npad] = j;
+                                    cnt++;
+                                } else if (pass == 1 && !same) {
+                                    mind_cp = fmin(mind_cp, dist_sq);
+                                    if (cnt < ncap) out[(long)cnt * npad] = j;
+                                    cnt++;
+                                }
+                            }
+                        }
+                    }
+        }
+        if (d.disctype == 1) {
+            if (cnt < ncap) out[(long)cnt * npad] = i;
+            cnt++;
+        }
+        nreal = cnt;
+        if (d.boundary) {
+            for (int pass = 0; pass < npass; pass++) {
+                for (int n = lo2; n <= hi2; n++)
+                    for (int m = lo1; m <= hi1; m++)
+                        for (int l = lo0; l <= hi0; l++) {
+                            int cc = (n * L1 + m) * L0 + l;
+                            for (int dd = 2; dd >= 0; dd--) {
+                                int gn = d.gcount[dd][cc];
+                                if (gn == 0) continue;
+                                int gs = d.gstart[dd][cc];
+                                for (int q = gn - 1; q >= 0; q--) {
+                                    int u = d.np + gs + q;
+                                    double dist_sq = 0.0;
+                                    double dx = d.x0[0][u] - xi; dist_sq = dist_sq + dx * dx;
+                                    double dy = d.x0[1][u] - yi; dist_sq = dist_sq + dy * dy;
+                                    double dz = d.x0[2][u] - zi; dist_sq = dist_sq + dz * dz;
+                                    double h_avg = 2.0 * 0.5 * (d.hbuf[cur][u] + hi);
+                                    if (dist_sq < h_avg * h_avg) {
+                                        bool same = d.pmat[u] == mi;
+                                        if (pass == 0 && same) {
+                                            mind = fmin(mind, dist_sq);
+                                            if (cnt < ncap) out[(long)cnt * npad] = u;
+                                            cnt++;
+                                        } else if (pass == 1 && !same) {
+                                            mind_cp = fmin(mind_cp, dist_sq);
+                                            if (cnt < ncap) out[(long)cnt * npad] = u;
+                                            cnt++;
+                                        }
+                                    }
+                                }
+                            }
+                        }
+            }
+        }
+        mind = sqrt(fmin(mind, mind_cp));
+    } else {
+        if (d.disctype == 1) {
+            if (cnt < ncap) out[0] = i;
+            cnt++;
+        }
+        nreal = cnt;
+    }
+    d.nreal[i] = nreal;
+    d.ntot[i] = cnt;
+    d.mindist[i] = mind;
+    atomicMax(&sc->max_count, cnt);
+    if (cnt > ncap) {
+        atomicOr(&sc->error_flags, 1);
+        atomicExch(&sc->abort, 1);
+    }
+}

@@ -1,0 +1,738 @@
+// mcm_physics.h -- per-particle physics of the MCM time step, written once for
+// the host (initialisation) and the device (fused gfx950 kernels).
+//
+// Symmetric 3x3 tensors are held as 6 components in the order
+// 11, 22, 33, 12, 13, 23; the spin tensor as its 3 upper components
+// W12, W13, W23.  Every expression keeps the evaluation order of the
+// reference routine it restates (cited per function), and the library is
+// compiled without FP contraction, so for the same inputs these functions
+// return the reference's values up to the last-place differences of the
+// transcendental functions (tan, pow).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#define MCM_HD __host__ __device__ __forceinline__
+
+enum { S11 = 0, S22 = 1, S33 = 2, S12 = 3, S13 = 4, S23 = 5 };
+
+// Material constants (mcm_database.f90:235-257 plus derived values)
+struct MatParams {
+    int model, eos, visc_type, ispall, efps_flag;
+    int pad_;
+    double rho, g, av_l, av_q, h, rho_max;
+    double strinput[48];
+    double eosinput[24];
+};
+
+// Per-particle state held in registers by the fused kernels.
+struct PState {
+    double mass, rho, rhoold, rho0, h, hold, h0;
+    double p, e, etry, einc, c, vabs, pcut, p_cut, efps, fail, q, critts;
+    double sig[6], s[6], alfa[6];
+    double rod[6], spin[3], tracerod;
+};
+
+MCM_HD double fsign1(double x) { return signbit(x) ? -1.0 : 1.0; }
+MCM_HD double dmax(double a, double b) { return a > b ? a : b; }
+MCM_HD double dmin(double a, double b) { return a < b ? a : b; }
+MCM_HD double ddim(double a, double b) { return a - b > 0.0 ? a - b : 0.0; }
+
+// ------------------------------------------------------------------ kernels
+// Cubic B-spline, 3-D (mcm_kernel.f90:42-95); r = |xi - xj|
+MCM_HD double kernel_w(double rad, double havg, double pi) {
+    double const1 = 1.0 / (pi * havg * havg * havg);
+    double const2 = const1 / 4.0;
+    double z = rad / havg;
+    if (z < 1.0) return const1 * (((0.75 * z - 1.5) * z) * z + 1.0);
+    if (z < 2.0) {
+        double t = 2.0 - z;
+        return const2 * (t * (t * t));
+    }
+    return 0.0;
+}
+
+// dW/dr factor of mcm_gradw.f90:41-103; gradient_n = dw * xij_n / (rad * havg)
+MCM_HD double kernel_dw(double rad, double havg, double pi) {
+    double const1 = 1.0 / (pi * havg * havg * havg);
+    double const2 = const1 * 0.25;
+    double z = rad / havg;
+    if (z < 1.0) return const1 * ((2.25 * z - 3.0) * z);
+    if (z < 2.0) {
+        double t = 2.0 - z;
+        return -(3.0 * const2 * (t * t));
+    }
+    return 0.0;
+}
+
+// gradient of W with respect to xi, xij = xi - xj (mcm_gradw.f90:76-101)
+MCM_HD void kernel_grad(double xij0, double xij1, double xij2, double havg, double pi, double* dwdx,
+                        double* rad_out) {
+    double rad = 0.0;
+    rad = rad + xij0 * xij0;
+    rad = rad + xij1 * xij1;
+    rad = rad + xij2 * xij2;
+    rad = sqrt(rad);
+    *rad_out = rad;
+    double dw = kernel_dw(rad, havg, pi);
+    if (rad == 0.0) {
+        dwdx[0] = 0.0; dwdx[1] = 0.0; dwdx[2] = 0.0;
+    } else {
+        double rh = rad * havg;
+        dwdx[0] = dw * xij0 / rh;
+        dwdx[1] = dw * xij1 / rh;
+        dwdx[2] = dw * xij2 / rh;
+    }
+}
+
+// 3x3 cofactor inverse, identity when |det| < 1e-8 (mcm_math_func.f90:50-79).
+// m and inv are column-major (element (r,c) at r + 3c).
+MCM_HD void inverse3(const double* m, double* inv) {
+#define M_(r, c) m[((r) - 1) + 3 * ((c) - 1)]
+#define I_(r, c) inv[((r) - 1) + 3 * ((c) - 1)]
+    double sd11 = M_(2, 2) * M_(3, 3) - M_(2, 3) * M_(3, 2);
+    double sd12 = M_(2, 1) * M_(3, 3) - M_(2, 3) * M_(3, 1);
+    double sd13 = M_(2, 1) * M_(3, 2) - M_(2, 2) * M_(3, 1);
+    double sd21 = M_(1, 2) * M_(3, 3) - M_(1, 3) * M_(3, 2);
+    double sd22 = M_(1, 1) * M_(3, 3) - M_(1, 3) * M_(3, 1);
+    double sd23 = M_(1, 1) * M_(3, 2) - M_(1, 2) * M_(3, 1);
+    double sd31 = M_(1, 2) * M_(2, 3) - M_(1, 3) * M_(2, 2);
+    double sd32 = M_(1, 1) * M_(2, 3) - M_(1, 3) * M_(2, 1);
+    double sd33 = M_(1, 1) * M_(2, 2) - M_(1, 2) * M_(2, 1);
+    double det = M_(1, 1) * sd11 - M_(1, 2) * sd12 + M_(1, 3) * sd13;
+    if (fabs(det) < 1.0e-08) {
+        for (int k = 0; k < 9; k++) inv[k] = 0.0;
+        I_(1, 1) = 1.0; I_(2, 2) = 1.0; I_(3, 3) = 1.0;
+        return;
+    }
+    double invdet = 1.0 / det;
+    I_(1, 1) = sd11 * invdet; I_(1, 2) = -sd21 * invdet; I_(1, 3) = sd31 * invdet;
+    I_(2, 1) = -sd12 * invdet; I_(2, 2) = sd22 * invdet; I_(2, 3) = -sd32 * invdet;
+    I_(3, 1) = sd13 * invdet; I_(3, 2) = -sd23 * invdet; I_(3, 3) = sd33 * invdet;
+#undef M_
+#undef I_
+}
+
+// rod / spin / trace from a velocity-gradient accumulator g[n][m]
+// (mcm_oldstrain.f90:56-63)
+MCM_HD void rod_spin(const double g[3][3], PState& s) {
+    s.rod[S11] = 0.5 * (g[0][0] + g[0][0]);
+    s.rod[S22] = 0.5 * (g[1][1] + g[1][1]);
+    s.rod[S33] = 0.5 * (g[2][2] + g[2][2]);
+    s.rod[S12] = 0.5 * (g[0][1] + g[1][0]);
+    s.rod[S13] = 0.5 * (g[0][2] + g[2][0]);
+    s.rod[S23] = 0.5 * (g[1][2] + g[2][1]);
+    s.spin[0] = 0.5 * (g[0][1] - g[1][0]);
+    s.spin[1] = 0.5 * (g[0][2] - g[2][0]);
+    s.spin[2] = 0.5 * (g[1][2] - g[2][1]);
+    s.tracerod = s.rod[S11] + s.rod[S22] + s.rod[S33];
+}
+
+// --------------------------------------------------------------- update_h
+// mcm_update_h.f90:25-54; returns true when h was updated (enters hmax)
+MCM_HD bool update_h(PState& s, int h_opt, double dt) {
+    if (h_opt == 1) {
+        if (s.p >= s.pcut) {
+            s.hold = s.h;
+            s.h = s.hold * (1.0 + s.tracerod * dt / 3.0);
+            return true;
+        }
+    } else if (h_opt == 2) {
+        if (s.p >= s.pcut) {
+            s.hold = s.h;
+            s.h = s.h0 * pow(s.rho0 / s.rho, 1.0 / 3.0);
+            return true;
+        }
+    }
+    return false;
+}
+
+// ----------------------------------------------------------------- rhoupd
+// mcm_rhoupd.f90:24-71 (the out-of-range warnings are not reproduced)
+MCM_HD void rhoupd(PState& s, const MatParams& m, double dt) {
+    const double othird = 1.0 / 3.0;
+    s.rhoold = s.rho;
+    s.rho = s.rho * (1.0 - s.tracerod * dt);
+    for (int pass = 0; pass < 2; pass++) {
+        if (s.rho > m.rho_max) {
+            s.rho = s.rho / (1.0 - s.tracerod * dt);
+            s.rod[S11] = s.rod[S11] - othird * s.tracerod;
+            s.rod[S22] = s.rod[S22] - othird * s.tracerod;
+            s.rod[S33] = s.rod[S33] - othird * s.tracerod;
+            s.tracerod = 0.0;
+            if (pass == 1) s.h = s.hold;
+            s.h = m.h * pow(m.rho / s.rho, 1.0 / 3.0);
+        }
+    }
+}
+
+// ------------------------------------------------------------ stress_rot1
+// mcm_stress_rot1.f90:22-184 (iorder = 3); the qq accumulation multiplies a
+// zero matrix in the reference and is not carried.
+MCM_HD void stress_rot1(PState& s, double dt) {
+    const double etol = (double)1.0e-3f;   // single-precision literal of the reference
+    const double z1 = 1.0 / 3.0, z2 = 2.0 / 15.0, z3 = 17.0 / 315.0, z4 = 62.0 / 2835.0;
+    double s11 = s.sig[S11], s22 = s.sig[S22], s33 = s.sig[S33];
+    double s12 = s.sig[S12], s23 = s.sig[S23], s31 = s.sig[S13];
+    double dt2 = 0.5 * dt;
+    double wzz = -(dt2 * s.spin[0]);
+    double wyy = dt2 * s.spin[1];
+    double wxx = -(dt2 * s.spin[2]);
+    double wxx2 = wxx * wxx, wyy2 = wyy * wyy, wzz2 = wzz * wzz;
+    double qm2 = wxx2 + wyy2 + wzz2;
+    double qm = sqrt(qm2);
+    double dd = tan(qm * 0.5);
+    double aa, cc;
+    if (qm < etol) {
+        aa = qm2 * 0.25;
+        cc = 0.5 * (1.0 + aa * (z1 + aa * (z2 + aa * (z3 + aa * z4))));
+    } else {
+        cc = dd / qm;
+    }
+    double bb = 2.0 * cc / (1.0 + dd * dd);
+    aa = bb * cc;
+    double Q00 = 1.0 - aa * (wyy2 + wzz2);
+    double Q11 = 1.0 - aa * (wxx2 + wzz2);
+    double Q22 = 1.0 - aa * (wxx2 + wyy2);
+    double Q01 = aa * wxx * wyy;
+    double Q02 = aa * wxx * wzz;
+    double Q12 = aa * wyy * wzz;
+    double Q10 = Q01, Q20 = Q02, Q21 = Q12;
+    double bwxx = bb * wxx, bwyy = bb * wyy, bwzz = bb * wzz;
+    Q01 = Q01 - bwzz; Q10 = Q10 + bwzz;
+    Q02 = Q02 + bwyy; Q20 = Q20 - bwyy;
+    Q12 = Q12 - bwxx; Q21 = Q21 + bwxx;
+    double qs00 = Q00 * s11 + Q01 * s12 + Q02 * s31;
+    double qs01 = Q00 * s12 + Q01 * s22 + Q02 * s23;
+    double qs02 = Q00 * s31 + Q01 * s23 + Q02 * s33;
+    double qs10 = Q10 * s11 + Q11 * s12 + Q12 * s31;
+    double qs11 = Q10 * s12 + Q11 * s22 + Q12 * s23;
+    double qs12 = Q10 * s31 + Q11 * s23 + Q12 * s33;
+    double qs20 = Q20 * s11 + Q21 * s12 + Q22 * s31;
+    double qs21 = Q20 * s12 + Q21 * s22 + Q22 * s23;
+    double qs22 = Q20 * s31 + Q21 * s23 + Q22 * s33;
+    s.sig[S11] = qs00 * Q00 + qs01 * Q01 + qs02 * Q02;
+    s.sig[S22] = qs10 * Q10 + qs11 * Q11 + qs12 * Q12;
+    s.sig[S33] = qs20 * Q20 + qs21 * Q21 + qs22 * Q22;
+    s.sig[S12] = qs00 * Q10 + qs01 * Q11 + qs02 * Q12;
+    s.sig[S23] = qs10 * Q20 + qs11 * Q21 + qs12 * Q22;
+    s.sig[S13] = qs00 * Q20 + qs01 * Q21 + qs02 * Q22;
+}
+
+// energy increment before the stress update (mcm_f3dm3.f90:54-56); q is isotropic
+MCM_HD double einc_pre(const PState& s, double dt) {
+    const double* r = s.rod;
+    const double* g = s.sig;
+    return (r[S11] * (g[S11] + s.q) + r[S22] * (g[S22] + s.q) + r[S33] * (g[S33] + s.q) +
+            2.0 * (r[S12] * (g[S12] + 0.0) + r[S13] * (g[S13] + 0.0) + r[S23] * (g[S23] + 0.0))) *
+           dt;
+}
+
+// energy increment after the stress update (mcm_f3dm3.f90:154-156)
+MCM_HD double einc_post(const PState& s, double dt) {
+    const double* r = s.rod;
+    const double* g = s.sig;
+    return 0.5 * ((r[S11] * g[S11] + r[S22] * g[S22] + r[S33] * g[S33] +
+                   2.0 * (r[S12] * g[S12] + r[S13] * g[S13] + r[S23] * g[S23])) *
+                      dt +
+                  s.einc);
+}
+
+#define CM5(a, b) m.strinput[((a) - 1) + 5 * ((b) - 1)]
+
+// mcm_f3dm1.f90:27-71 (elastic)
+MCM_HD void f3dm1(PState& s, const MatParams& m, double dt) {
+    const double othird = 1.0 / 3.0;
+    double ym = CM5(1, 1), pr = CM5(1, 2), gg = CM5(5, 4);
+    double gdt = dt * gg;
+    double gd2 = 2.0 * gdt;
+    double blk = dt * ym / (1.0 - 2.0 * pr);
+    double davg = othird * s.tracerod;
+    double p_incr = blk * davg;
+    s.einc = einc_pre(s, dt);
+    s.sig[S11] = s.sig[S11] + p_incr + gd2 * (s.rod[S11] - davg);
+    s.sig[S22] = s.sig[S22] + p_incr + gd2 * (s.rod[S22] - davg);
+    s.sig[S33] = s.sig[S33] + p_incr + gd2 * (s.rod[S33] - davg);
+    s.sig[S12] = s.sig[S12] + gd2 * s.rod[S12];
+    s.sig[S13] = s.sig[S13] + gd2 * s.rod[S13];
+    s.sig[S23] = s.sig[S23] + gd2 * s.rod[S23];
+    s.einc = einc_post(s, dt);
+    s.p = -(othird * (s.sig[S11] + s.sig[S22] + s.sig[S33]));
+}
+
+// mcm_f3dm3.f90:31-156 (elastic-plastic with kinematic/isotropic hardening);
+// returns the yield active-set flag (scle)
+MCM_HD int f3dm3(PState& s, const MatParams& m, double dt) {
+    const double othird = 1.0 / 3.0;
+    double qh = CM5(2, 4), qb = CM5(1, 5), qs = CM5(1, 3), ym = CM5(1, 1), pr = CM5(1, 2);
+    double efps_f = CM5(2, 6);
+    double gg = 2.0 * CM5(5, 4);
+    double gdt = dt * gg;
+    double qbqh = qb * qh;
+    double blk = dt * ym / (1.0 - 2.0 * pr);
+    double davg = othird * s.tracerod;
+    double p_incr = blk * davg;
+    s.p = s.p - p_incr;
+    s.einc = einc_pre(s, dt);
+    s.sig[S11] = s.sig[S11] + p_incr + gdt * (s.rod[S11] - davg);
+    s.sig[S22] = s.sig[S22] + p_incr + gdt * (s.rod[S22] - davg);
+    s.sig[S33] = s.sig[S33] + p_incr + gdt * (s.rod[S33] - davg);
+    s.sig[S12] = s.sig[S12] + gdt * s.rod[S12];
+    s.sig[S13] = s.sig[S13] + gdt * s.rod[S13];
+    s.sig[S23] = s.sig[S23] + gdt * s.rod[S23];
+    double k = qs + qbqh * s.efps;
+    double d11 = s.p + s.sig[S11] - s.alfa[S11];
+    double d22 = s.p + s.sig[S22] - s.alfa[S22];
+    double d33 = s.p + s.sig[S33] - s.alfa[S33];
+    double d12 = s.sig[S12] - s.alfa[S12];
+    double d13 = s.sig[S13] - s.alfa[S13];
+    double d23 = s.sig[S23] - s.alfa[S23];
+    s.s[S11] = d11; s.s[S22] = d22; s.s[S33] = d33; s.s[S12] = d12; s.s[S13] = d13; s.s[S23] = d23;
+    double j2 = d23 * d23 + d13 * d13 + d12 * d12 - d11 * d22 - d22 * d33 - d11 * d33;
+    double k2 = 3.0 * j2 - k * k;
+    double scle = 0.5 * (1.0 + fsign1(k2));
+    double fac1 = 1.0 / (1.5 * gg + qh);
+    double fac2 = 1.5 * gg;
+    double fac3 = (1.0 - qb) * qh;
+    double j1 = sqrt(3.0 * fabs(j2)) + 1.0 - scle;
+    double defps = scle * fac1 * (j1 - k);
+    s.efps = s.efps + defps;
+    double des = scle * fac2 * defps / j1;
+    double depn = scle * fac3 * defps / j1;
+    s.sig[S11] = s.sig[S11] - des * d11;
+    s.sig[S22] = s.sig[S22] - des * d22;
+    s.sig[S33] = s.sig[S33] - des * d33;
+    s.sig[S12] = s.sig[S12] - des * d12;
+    s.sig[S13] = s.sig[S13] - des * d13;
+    s.sig[S23] = s.sig[S23] - des * d23;
+    s.alfa[S11] = s.alfa[S11] + depn * d11;
+    s.alfa[S22] = s.alfa[S22] + depn * d22;
+    s.alfa[S33] = s.alfa[S33] + depn * d33;
+    s.alfa[S12] = s.alfa[S12] + depn * d12;
+    s.alfa[S13] = s.alfa[S13] + depn * d13;
+    s.alfa[S23] = s.alfa[S23] + depn * d23;
+    if (m.efps_flag == 1 && s.efps > efps_f) {
+        double press = othird * (s.sig[S11] + s.sig[S22] + s.sig[S33]);
+        s.sig[S11] = press; s.sig[S22] = press; s.sig[S33] = press;
+        s.sig[S12] = 0.0; s.sig[S13] = 0.0; s.sig[S23] = 0.0;
+    }
+    s.einc = einc_post(s, dt);
+    return scle > 0.5 ? 1 : 0;
+}
+
+// mcm_f3dm9.f90:26-48 (viscous fluid).  The reference indexes cm(1:2, mat)
+// by the material id and never writes s(1,2) (s(1,3) is written twice).
+MCM_HD void f3dm9(PState& s, const MatParams& m, int matid1) {
+    const double othird = 1.0 / 3.0;
+    double xmu = m.strinput[1 + 5 * (matid1 - 1)];
+    if (xmu == 0.0) {
+        for (int k = 0; k < 6; k++) s.s[k] = 0.0;
+    } else {
+        double davg = -(othird * s.tracerod);
+        s.s[S11] = xmu * (s.rod[S11] + davg);
+        s.s[S22] = xmu * (s.rod[S22] + davg);
+        s.s[S33] = xmu * (s.rod[S33] + davg);
+        s.s[S13] = xmu * s.rod[S12];
+        s.s[S13] = xmu * s.rod[S13];
+        s.s[S23] = xmu * s.rod[S23];
+    }
+}
+
+// mcm_f3dm10.f90:28-157 (elastic-plastic-hydrodynamic).  defps_prev stands
+// in for the plastic-strain increment the reference reads uninitialised when
+// a particle has failed (:82-89); returns the yield active-set flag.
+MCM_HD int f3dm10(PState& s, const MatParams& m, double dt, double& defps_prev) {
+    const double othird = 1.0 / 3.0;
+    const double* cm = m.strinput;
+    double gg = cm[0], qs = cm[1], qh = cm[2], a1 = cm[4], a2 = cm[5];
+    int ispall = m.ispall;
+    double epf = cm[7];
+    double g2 = 2.0 * dt * gg;
+    double davg = othird * s.tracerod;
+    s.p = -(othird * (s.sig[S11] + s.sig[S22] + s.sig[S33]));
+    s.p_cut = s.pcut;
+    double ak = qs + qh * s.efps + (a1 + a2 * s.p) * dmax(0.0, s.p);
+    s.s[S11] = s.sig[S11] + s.p + g2 * (s.rod[S11] + davg);
+    s.s[S22] = s.sig[S22] + s.p + g2 * (s.rod[S22] + davg);
+    s.s[S33] = s.sig[S33] + s.p + g2 * (s.rod[S33] + davg);
+    s.s[S12] = s.sig[S12] + 0.5 * g2 * s.rod[S12];
+    s.s[S13] = s.sig[S13] + 0.5 * g2 * s.rod[S13];
+    s.s[S23] = s.sig[S23] + 0.5 * g2 * s.rod[S23];
+    if (ispall == 3) {
+        double deltap = s.p - s.p_cut;
+        double scle = 0.5 * (1.0 + fsign1(deltap));
+        s.pcut = scle * s.p_cut;
+        for (int k = 0; k < 6; k++) s.s[k] = scle * s.s[k];
+        s.p_cut = s.pcut;
+    }
+    double* d = s.s;
+    double aj2 = 0.5 * (d[S11] * d[S11] + d[S22] * d[S22] + d[S33] * d[S33]) + d[S12] * d[S12] +
+                 d[S13] * d[S13] + d[S23] * d[S23];
+    double sj2 = sqrt(3.0 * aj2);
+    double defps = defps_prev;
+    int active = 0;
+    if (s.fail == 1.0) {
+        double thrg = othird / gg;
+        defps = ddim(sj2, ak) / ((double)1.e-30f + ddim(1.0, -(thrg * qh)));
+        defps = thrg * dmin(sj2, defps);
+        s.efps = s.efps + defps;
+        active = sj2 > ak ? 1 : 0;
+    }
+    defps_prev = defps;
+    ak = dmax(ak + qh * defps, 0.0);
+    double seps = (double)1.e-30f * gg;
+    double scle = (ak + seps) / (dmax(ak, sj2) + seps);
+    for (int k = 0; k < 6; k++) d[k] = scle * d[k];
+    if (epf != 0.0 && s.efps > epf) {
+        s.fail = 0.0;
+        for (int k = 0; k < 6; k++) d[k] = 0.0;
+    }
+    if (ispall == 2) {
+        aj2 = 0.5 * (d[S11] * d[S11] + d[S22] * d[S22] + d[S33] * d[S33]) + d[S12] * d[S12] + d[S13] * d[S13] +
+              d[S23] * d[S23] + (double)1.e-12f;
+        sj2 = d[S11] * d[S23] * d[S23] + d[S22] * d[S13] * d[S13] + d[S33] * d[S12] * d[S12] -
+              d[S11] * d[S22] * d[S33] - 2.0 * d[S12] * d[S13] * d[S23];
+        double akt = -(sqrt(27.0 / aj2) * sj2 * 0.5 / aj2);
+        akt = copysign(dmin(fabs(akt), 1.0), akt);
+        double ywh = acos(akt) * othird;
+        sj2 = 2.0 * sqrt(aj2 * othird) * cos(ywh);
+        double davg1 = s.p - sj2 - s.p_cut;
+        double sc2 = 0.5 * (1.0 + fsign1(davg1));
+        s.pcut = sc2 * s.p_cut;
+        for (int k = 0; k < 6; k++) d[k] = sc2 * d[k];
+        s.p_cut = s.pcut;
+    }
+    return active;
+}
+
+// mcm_nodalav.f90:32-47 (isotropic nodal viscosity; visc_type 1 only)
+MCM_HD void nodalav(PState& s, const MatParams& m) {
+    s.q = 0.0;
+    if (s.tracerod < 0.0) {
+        double ht = s.h * s.tracerod;
+        s.q = m.av_l * s.rho * s.c * s.h * fabs(s.tracerod) + m.av_q * s.rho * (ht * ht);
+    }
+}
+
+// mcm_lieupd.f90:26-33 (q:D over the column-major flattening; q is diagonal)
+MCM_HD void lieupd(PState& s, double dt) {
+    double qpm = 0.0;
+    qpm = qpm + s.q * s.rod[S11];
+    qpm = qpm + s.q * s.rod[S22];
+    qpm = qpm + s.q * s.rod[S33];
+    s.e = s.e + (s.einc - dt * qpm) * s.mass / s.rho;
+}
+
+// mcm_hieupd.f90:28-64 (trial energy)
+MCM_HD void hieupd(PState& s, double dt) {
+    const double othird = 1.0 / 3.0;
+    double pold = -(othird * (s.sig[S11] + s.sig[S22] + s.sig[S33]));
+    double trace_q = othird * (s.q + s.q + s.q);
+    double volold = s.mass / s.rhoold;
+    double volnew = s.mass / s.rho;
+    // column-major sum over (k,l): 11 21 31 12 22 32 13 23 33
+    double e11 = s.rod[S11] * (0.5 * (s.sig[S11] + pold + s.s[S11]) - s.q + trace_q);
+    double e22 = s.rod[S22] * (0.5 * (s.sig[S22] + pold + s.s[S22]) - s.q + trace_q);
+    double e33 = s.rod[S33] * (0.5 * (s.sig[S33] + pold + s.s[S33]) - s.q + trace_q);
+    double e12 = s.rod[S12] * (0.5 * (s.sig[S12] + s.s[S12]) - 0.0);
+    double e13 = s.rod[S13] * (0.5 * (s.sig[S13] + s.s[S13]) - 0.0);
+    double e23 = s.rod[S23] * (0.5 * (s.sig[S23] + s.s[S23]) - 0.0);
+    double vavg = 0.5 * (volnew + volold);
+    double eincr = 0.0;
+    eincr = eincr + e11; eincr = eincr + e12; eincr = eincr + e13;
+    eincr = eincr + e12; eincr = eincr + e22; eincr = eincr + e23;
+    eincr = eincr + e13; eincr = eincr + e23; eincr = eincr + e33;
+    eincr = vavg * eincr;
+    double dvol = volnew - volold;
+    double mean_incr = dvol * (0.5 * pold + trace_q);
+    s.etry = s.e + dt * eincr - mean_incr;
+}
+
+// Sound speed (mcm_sueos.f90:24-46 -> mcm_eos{1,4,13,28,41}c.f90); returns
+// nonzero when eos41 meets a negative c^2 (the reference stops there)
+MCM_HD int sueos(PState& s, const MatParams& m) {
+    const double* ei = m.eosinput;
+    double rho = s.rho, rho0 = s.rho0, mass = s.mass;
+    switch (m.eos) {
+        case 1: {
+            double c1 = ei[1], c2 = ei[2], c3 = ei[3], c4 = ei[4], c5 = ei[5], c6 = ei[6];
+            double mu = rho / rho0 - 1.0;
+            double mu_t = dmax(mu, 0.0);
+            double e1 = 0.0;
+            double dmudrho = 1.0 / rho0;
+            double termone = c1 * dmudrho + (c2 * (dmudrho * dmudrho)) + (3.0 * c3 * dmudrho) * mu * mu;
+            double termtwo = (c5 * dmudrho + c6 * dmudrho * dmudrho) * e1;
+            double dpdrho = termone + termtwo;
+            double dpde = c4 + c5 * mu + c6 * (mu_t * mu_t);
+            double mr = mass / rho;
+            double c_sq = ((4.0 / 3.0) * m.g / rho0 + dpdrho + s.p / rho0 * (mr * mr) * dpde);
+            s.c = c_sq > 0.0 ? sqrt(c_sq) : sqrt(fabs(c_sq));
+            return 0;
+        }
+        case 4: {
+            double c1 = ei[0], s1 = ei[1], s2 = ei[2], s3 = ei[3], gamma = ei[4], a1 = ei[5];
+            double e1 = s.e * rho0 / mass;
+            double mu = rho / rho0 - 1.0;
+            double dpdrho;
+            if (mu > 0.0) {
+                double termone = mu * (1.0 - (gamma / 2.0));
+                double termtwo = (a1 / 2.0) * mu * mu;
+                double eosnum = mu * rho0 * (c1 * c1) * (1.0 + termone - termtwo);
+                double mp1 = mu + 1.0;
+                double mu11 = mu / mp1;
+                double mu21 = (mu * mu) / mp1;
+                double mu22 = (mu * mu) / (mp1 * mp1);
+                double mu32 = (mu * mu * mu) / (mp1 * mp1);
+                double mu33 = (mu * mu * mu) / (mp1 * mp1 * mp1);
+                double eosden = 1.0 - (s1 - 1.0) * mu - s2 * mu21 - s3 * mu32;
+                double termthree = rho0 * (c1 * c1) * (1.0 + 2.0 * termone - 3.0 * termtwo);
+                double termfour = 1.0 - s1 - 2.0 * s2 * mu11 + s2 * mu22 - 3.0 * s3 * mu22 + 2.0 * s3 * mu33;
+                double ed2 = eosden * eosden;
+                double ed3 = eosden * ed2;
+                dpdrho = (termthree / ed2 - 2.0 * eosnum * termfour / ed3) / rho0 + a1 * e1 / rho0;
+            } else {
+                dpdrho = c1 * c1 + a1 * e1 / rho0;
+            }
+            double dpde = gamma + a1 * mu;
+            double mr = mass / rho;
+            double c_sq = ((4.0 / 3.0) * m.g / rho0 + dpdrho + s.p / rho0 * (mr * mr) * dpde);
+            s.c = c_sq > 0.0 ? sqrt(c_sq) : sqrt(fabs(c_sq));
+            return 0;
+        }
+        case 13: {
+            double gamma = ei[0];
+            double e1 = s.e / mass;
+            s.c = sqrt((gamma - 1.0) * gamma * e1);
+            return 0;
+        }
+        case 28: s.c = sqrt(ei[0] * ei[1] / rho0); return 0;
+        case 41: {
+            double cc = ei[0], sl = ei[1], gamma0 = ei[4];
+            double e1 = s.e / mass;
+            double ao = rho0 * (cc * cc);
+            double mu = rho / rho0 - 1.0;
+            double gamma = gamma0 * rho0 / rho;
+            double bo = ao * (1.0 + 2.0 * (sl - 1.0));
+            double slm = sl - 1.0;
+            double co = ao * (2.0 * (sl - 1.0) + 3.0 * (slm * slm));
+            double f = gamma * mu;
+            double a1 = ao * (1.0 - f);
+            double b1 = bo * (2.0 - 1.5 * f);
+            double c1 = co * (3.0 - 2.0 * f);
+            double dpdrho, c2;
+            if (mu > 0.0) dpdrho = (a1 + b1 * mu + c1 * (mu * mu)) / rho0 + gamma * e1;
+            else dpdrho = ao * (1.0 - f) / rho0 + gamma * e1;
+            if (s.p == s.pcut) {
+                c2 = cc * cc;
+            } else {
+                double dpdi = gamma * rho;
+                double didrho = s.p / (rho * rho);
+                dpdrho = dpdrho + dpdi * didrho;
+                c2 = dpdrho;
+            }
+            s.c = sqrt(c2);
+            return c2 < 0.0 ? 1 : 0;
+        }
+        default: return 0;
+    }
+}
+
+// Pressure and energy (mcm_eqos.f90:24-55 -> mcm_eqos{1,4,13,28,41}.f90)
+MCM_HD void eqos(PState& s, const MatParams& m, double dt) {
+    const double* ei = m.eosinput;
+    double rho = s.rho, rho0 = s.rho0, mass = s.mass;
+    switch (m.eos) {
+        case 1: {
+            double c0 = ei[0], c1 = ei[1], c2 = ei[2], c3 = ei[3], c4 = ei[4], c5 = ei[5], c6 = ei[6];
+            double mu = rho / rho0 - 1.0;
+            double mu_t = dmax(mu, 0.0);
+            double dvol = mass / rho - mass / s.rhoold;
+            double vol0 = mass / rho0;
+            double e1try = s.etry / vol0;
+            double termone = c0 + (c1 + c3 * (mu * mu)) * mu + c2 * (mu_t * mu_t);
+            double termtwo = c4 + c5 * mu + c6 * (mu_t * mu_t);
+            s.p = (termone + termtwo * e1try) / (1.0 + termtwo * (dvol / vol0));
+            if (s.p < s.pcut) {
+                s.p = s.pcut;
+                s.rho = s.rho / (1.0 - s.tracerod * dt);
+            } else {
+                s.e = s.etry - 0.5 * dvol * s.p;
+            }
+            break;
+        }
+        case 4: {
+            double c1 = ei[0], s1 = ei[1], s2 = ei[2], s3 = ei[3], gamma = ei[4], a1 = ei[5];
+            double mu = rho / rho0 - 1.0;
+            double dvol = mass / rho - mass / s.rhoold;
+            double vol0 = mass / rho0;
+            double e1try = s.etry / vol0;
+            if (mu > 0.0) {
+                double termone = mu * (1.0 - (gamma / 2.0));
+                double termtwo = (a1 / 2.0) * mu * mu;
+                double eosnum = mu * rho0 * (c1 * c1) * (1.0 + termone - termtwo);
+                double mp1 = mu + 1.0;
+                double mu21 = (mu * mu) / mp1;
+                double mu32 = (mu * mu * mu) / (mp1 * mp1);
+                double eosden = 1.0 - (s1 - 1.0) * mu - s2 * mu21 - s3 * mu32;
+                s.p = (eosnum / (eosden * eosden) + (gamma + a1 * mu) * e1try) /
+                      (1.0 + 0.5 * (gamma + a1 * mu) * dvol / vol0);
+            } else {
+                s.p = (rho0 * (c1 * c1) * mu + (gamma + a1 * mu) * e1try) /
+                      (1.0 + 0.5 * (gamma + a1 * mu) * dvol / vol0);
+            }
+            if (s.p < s.pcut) {
+                s.p = s.pcut;
+                s.rho = s.rho / (1.0 - s.tracerod * dt);
+            } else {
+                s.e = s.etry - 0.5 * dvol * s.p;
+            }
+            break;
+        }
+        case 13: {
+            double gamma_m1 = ei[0] - 1.0;
+            double e1try = s.etry / mass;
+            double dvol = mass / rho - mass / s.rhoold;
+            s.p = (rho * gamma_m1 * e1try) / (1.0 + 0.5 * rho * gamma_m1 * dvol / mass);
+            s.e = s.etry - 0.5 * dvol * s.p;
+            break;
+        }
+        case 28: {
+            double b = ei[0], gamma = ei[1];
+            double dvol = mass / rho - mass / s.rhoold;
+            s.p = b * (pow(rho / rho0, gamma) - 1.0);
+            s.e = s.etry - 0.5 * dvol * s.p;
+            break;
+        }
+        case 41: {
+            double cc = ei[0], sl = ei[1], gama0 = ei[4];
+            double etry1 = s.etry / mass;
+            double ao = rho0 * (cc * cc);
+            double mu = rho / rho0 - 1.0;
+            double dvol = mass / rho - mass / s.rhoold;
+            double vol0 = mass / rho0;
+            double gama = gama0 * rho0 / rho;
+            double slm = sl - 1.0;
+            double bo = ao * (1.0 + 2.0 * (sl - 1.0));
+            double co = ao * (2.0 * (sl - 1.0) + 3.0 * (slm * slm));
+            double f = gama * mu;
+            double ph;
+            if (mu > 0.0) ph = ao * mu + bo * (mu * mu) + co * (mu * (mu * mu));
+            else ph = ao * mu;
+            double pn = (ph * (1.0 - 0.5 * f) + rho * gama * etry1) / (1.0 + 0.5 * rho * gama * dvol / vol0);
+            if (pn < s.pcut) {
+                pn = s.pcut;
+                s.rho = s.rho / (1.0 - s.tracerod * dt);
+            } else {
+                s.e = etry1 * mass - 0.5 * dvol * pn;
+            }
+            s.p = pn;
+            break;
+        }
+        default: break;
+    }
+}
+
+// mcm_stress_up.f90:28-36
+MCM_HD void stress_up(PState& s) {
+    s.sig[S11] = s.s[S11] - s.p;
+    s.sig[S22] = s.s[S22] - s.p;
+    s.sig[S33] = s.s[S33] - s.p;
+    s.sig[S12] = s.s[S12];
+    s.sig[S13] = s.s[S13];
+    s.sig[S23] = s.s[S23];
+}
+
+// per-particle critical timestep (mcm_calccritts.f90:40-78)
+MCM_HD double critts(const PState& s, const MatParams& m, int tcrit_opt, double mindist) {
+    if (tcrit_opt == 0) {
+        double mr = s.mass / s.rho;
+        double veae = pow(mr, 1.0 / 3.0);
+        double ae = pow(mr, 2.0 / 3.0);
+        double qu = (m.av_l * s.c + m.av_q * s.mass / s.rho * s.tracerod) / ae;
+        return veae / (qu + sqrt(qu * qu + s.c * s.c));
+    }
+    if (tcrit_opt == 1) return s.h / (s.c + s.vabs);
+    return mindist / (s.c + s.vabs);
+}
+
+// Material update of one particle in the order of mcm_constitutive.f90:29-83.
+// Returns the yield active-set flag of the plastic models.
+MCM_HD int constitutive_one(PState& s, const MatParams& m, int matid1, double dt, double& defps_prev,
+                            int* eos_err) {
+    int active = 0;
+    switch (m.model) {
+        case 1:
+            stress_rot1(s, dt); f3dm1(s, m, dt); nodalav(s, m); lieupd(s, dt);
+            break;
+        case 3:
+            stress_rot1(s, dt); active = f3dm3(s, m, dt); nodalav(s, m); lieupd(s, dt);
+            break;
+        case 9:
+            stress_rot1(s, dt); f3dm9(s, m, matid1);
+            *eos_err |= sueos(s, m);
+            nodalav(s, m); hieupd(s, dt); eqos(s, m, dt); stress_up(s);
+            break;
+        case 10:
+            stress_rot1(s, dt); active = f3dm10(s, m, dt, defps_prev);
+            *eos_err |= sueos(s, m);
+            nodalav(s, m); hieupd(s, dt); eqos(s, m, dt); stress_up(s);
+            break;
+        default: break;
+    }
+    return active;
+}
+
+// eos{1,4,13,28,41}p.f90 (initial pressure), used by the initialisation only
+MCM_HD void eos_p(PState& s, const MatParams& m) {
+    const double* ei = m.eosinput;
+    double rho = s.rho, rho0 = s.rho0;
+    switch (m.eos) {
+        case 1: {
+            double mu = rho / rho0 - 1.0, mu_t = dmax(mu, 0.0), e1 = 0.0;
+            s.p = ei[0] + (ei[1] + ei[3] * (mu * mu)) * mu + ei[2] * (mu_t * mu_t) +
+                  (ei[4] + ei[5] * mu + ei[6] * (mu_t * mu_t)) * e1;
+            break;
+        }
+        case 4: {
+            double c1 = ei[0], s1 = ei[1], s2 = ei[2], s3 = ei[3], gamma = ei[4], a1 = ei[5];
+            double e1 = s.e * rho0 / s.mass;
+            double mu = rho / rho0 - 1.0;
+            if (mu > 0.0) {
+                double termone = mu * (1.0 - (gamma / 2.0));
+                double termtwo = (a1 / 2.0) * mu * mu;
+                double eosnum = mu * rho0 * (c1 * c1) * (1.0 + termone - termtwo);
+                double mp1 = mu + 1.0;
+                double mu21 = (mu * mu) / mp1;
+                double mu32 = (mu * mu * mu) / (mp1 * mp1);
+                double eosden = 1.0 - (s1 - 1.0) * mu - s2 * mu21 - s3 * mu32;
+                s.p = eosnum / (eosden * eosden) + (gamma + a1 * mu) * e1;
+            } else {
+                s.p = rho0 * (c1 * c1) * mu + (gamma + a1 * mu) * e1;
+            }
+            break;
+        }
+        case 13: s.p = rho * (ei[0] - 1.0) * (s.e / s.mass); break;
+        case 28: s.p = ei[0] * (pow(rho / rho0, ei[1]) - 1.0); break;
+        case 41: {
+            double cc = ei[0], sl = ei[1], gamma = ei[4];
+            double e1 = s.e / s.mass;
+            double ao = rho0 * (cc * cc);
+            double mu = rho / rho0 - 1.0;
+            double slm = sl - 1.0;
+            double bo = ao * (1.0 + 2.0 * (sl - 1.0));
+            double co = ao * (2.0 * (sl - 1.0) + 3.0 * (slm * slm));
+            gamma = gamma * rho0 / rho;
+            double f = gamma * mu, ph;
+            if (mu > 0.0) ph = ao * mu + bo * (mu * mu) + co * (mu * (mu * mu));
+            else ph = ao * mu;
+            double pn = ph * (1.0 - 0.5 * f) + rho * gamma * e1;
+            if (pn < s.pcut) pn = s.pcut;
+            s.p = pn;
+            break;
+        }
+        default: break;
+    }
+}
+#undef CM5

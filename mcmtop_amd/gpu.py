@@ -1,0 +1,192 @@
+"""Python front end of the C ABI in include/mcm_gpu.h (libmcm_gpu.so).
+
+The reference's time step is a set of argument-less subroutines called from
+mcm_solution (MCM/mcm_solution.f90:41-116).  ``Solver`` exposes the same
+stages under the same names (``neighbours``, ``strain``, ``update_h``,
+``constitutive``, ``momentum``, ``boundcond``, ``calctotale``,
+``update_velocity``, ``move``) plus the fused ``step(n)`` fast path.  Every
+call goes through the C ABI; there is no CPU fallback: if the library is
+missing or no GPU is visible the constructor raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmcm_gpu.so")
+
+ERRORS = {0: "ok", -1: "bad argument", -2: "io error", -3: "HIP error", -4: "capacity exceeded",
+          -5: "unsupported model", -6: "no HIP device"}
+
+F_X, F_V, F_A, F_RHO, F_H, F_P, F_E, F_C, F_EFPS, F_SIGMA, F_VABS, F_MASS = range(12)
+
+# C-ABI symbols declared in include/mcm_gpu.h
+SYMBOLS = (
+    "mcm_gpu_create_from_deck", "mcm_gpu_destroy", "mcm_gpu_last_error", "mcm_gpu_initial",
+    "mcm_gpu_neighbours", "mcm_gpu_strain", "mcm_gpu_update_h", "mcm_gpu_constitutive",
+    "mcm_gpu_dt_update", "mcm_gpu_momentum", "mcm_gpu_boundcond", "mcm_gpu_calctotale",
+    "mcm_gpu_update_velocity", "mcm_gpu_move", "mcm_gpu_calccritts", "mcm_gpu_step",
+    "mcm_gpu_get_scalars", "mcm_gpu_download", "mcm_gpu_download_neighbours", "mcm_gpu_dump",
+    "mcm_gpu_time_steps",
+)
+
+
+class McmGpuError(RuntimeError):
+    pass
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("nbr_capacity", ctypes.c_int32), ("ghost_capacity", ctypes.c_int32),
+                ("device", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5)]
+
+
+class Scalars(ctypes.Structure):
+    _fields_ = [("dt", ctypes.c_double), ("dtold", ctypes.c_double), ("ptime", ctypes.c_double),
+                ("critts", ctypes.c_double), ("hmax", ctypes.c_double), ("kinetice", ctypes.c_double),
+                ("internale", ctypes.c_double), ("totale", ctypes.c_double),
+                ("coord_min", ctypes.c_double * 3), ("coord_max", ctypes.c_double * 3),
+                ("np", ctypes.c_int32), ("ngp", ctypes.c_int32), ("max_count", ctypes.c_int32),
+                ("timestep", ctypes.c_int32), ("error_flags", ctypes.c_int32), ("nummat", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 2)]
+
+
+_LIB = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libmcm_gpu.so and declare its signatures; raises if it is absent."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise McmGpuError("libmcm_gpu.so not built (%s); run __graft_entry__.build()" % path)
+    lib = ctypes.CDLL(path)
+    vp = ctypes.c_void_p
+    lib.mcm_gpu_create_from_deck.argtypes = [ctypes.c_char_p, ctypes.POINTER(Options), ctypes.POINTER(vp)]
+    lib.mcm_gpu_destroy.argtypes = [vp]
+    lib.mcm_gpu_destroy.restype = None
+    lib.mcm_gpu_last_error.argtypes = [vp]
+    lib.mcm_gpu_last_error.restype = ctypes.c_char_p
+    for name in ("initial", "neighbours", "strain", "update_h", "constitutive", "dt_update", "momentum",
+                 "boundcond", "calctotale", "update_velocity", "move"):
+        getattr(lib, "mcm_gpu_" + name).argtypes = [vp]
+    lib.mcm_gpu_calccritts.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
+    lib.mcm_gpu_step.argtypes = [vp, ctypes.c_int32]
+    lib.mcm_gpu_get_scalars.argtypes = [vp, ctypes.POINTER(Scalars)]
+    lib.mcm_gpu_download.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_double), ctypes.c_int64]
+    i32p = ctypes.POINTER(ctypes.c_int32)
+    lib.mcm_gpu_download_neighbours.argtypes = [vp, i32p, i32p, i32p, ctypes.c_int64, i32p]
+    lib.mcm_gpu_dump.argtypes = [vp, ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32]
+    lib.mcm_gpu_time_steps.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
+    for name in SYMBOLS:
+        if name not in ("mcm_gpu_destroy", "mcm_gpu_last_error"):
+            getattr(lib, name).restype = ctypes.c_int
+    _LIB = lib
+    return lib
+
+
+class Solver:
+    """One MCM run on one GPU, created from a binary deck (mcmtop_amd/deck.py)."""
+
+    def __init__(self, deck_path: str, nbr_capacity: int = 0, ghost_capacity: int = 0, device: int = 0):
+        self._lib = load_library()
+        opt = Options(nbr_capacity, ghost_capacity, device)
+        h = ctypes.c_void_p()
+        rc = self._lib.mcm_gpu_create_from_deck(deck_path.encode(), ctypes.byref(opt), ctypes.byref(h))
+        self._h = h
+        self._check(rc, "create")
+
+    def _check(self, rc: int, what: str):
+        if rc != 0:
+            msg = self._lib.mcm_gpu_last_error(self._h).decode() if self._h else ""
+            raise McmGpuError("%s failed: %s (%s)" % (what, ERRORS.get(rc, rc), msg))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.mcm_gpu_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # stages, named as the reference subroutines they replace
+    def initial(self):
+        self._check(self._lib.mcm_gpu_initial(self._h), "initial")
+
+    def _stage(self, name):
+        self._check(getattr(self._lib, "mcm_gpu_" + name)(self._h), name)
+
+    def neighbours(self): self._stage("neighbours")
+    def strain(self): self._stage("strain")
+    def update_h(self): self._stage("update_h")
+    def constitutive(self): self._stage("constitutive")
+    def dt_update(self): self._stage("dt_update")
+    def momentum(self): self._stage("momentum")
+    def boundcond(self): self._stage("boundcond")
+    def calctotale(self): self._stage("calctotale")
+    def update_velocity(self): self._stage("update_velocity")
+    def move(self): self._stage("move")
+
+    def stage(self, name: str):
+        self._stage(name)
+
+    def calccritts(self) -> float:
+        out = ctypes.c_double()
+        self._check(self._lib.mcm_gpu_calccritts(self._h, ctypes.byref(out)), "calccritts")
+        return out.value
+
+    def step_granular(self):
+        """One step through the stage entry points, in mcm_solution's order."""
+        for name in ("neighbours", "strain", "update_h", "constitutive", "dt_update", "momentum",
+                     "boundcond", "calctotale", "update_velocity", "move"):
+            self._stage(name)
+
+    def step(self, n: int = 1):
+        """n fused steps (device-side dt, captured hipGraph)."""
+        self._check(self._lib.mcm_gpu_step(self._h, int(n)), "step")
+
+    def scalars(self) -> Scalars:
+        s = Scalars()
+        self._check(self._lib.mcm_gpu_get_scalars(self._h, ctypes.byref(s)), "get_scalars")
+        return s
+
+    def field(self, code: int) -> np.ndarray:
+        n = self.scalars().np
+        width = 3 if code in (F_X, F_V, F_A) else (9 if code == F_SIGMA else 1)
+        out = np.zeros(n * width)
+        self._check(self._lib.mcm_gpu_download(self._h, code, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                               out.size), "download")
+        return out.reshape(n, width) if width > 1 else out
+
+    def neighbour_lists(self):
+        """(counts, ghost_counts, ids[np, cap]) with 0-based ids, ghosts as np + ghost index."""
+        n = self.scalars().np
+        cap = ctypes.c_int32()
+        counts = np.zeros(n, np.int32)
+        gcounts = np.zeros(n, np.int32)
+        # query capacity with an empty call first
+        i32p = ctypes.POINTER(ctypes.c_int32)
+        rc = self._lib.mcm_gpu_download_neighbours(self._h, counts.ctypes.data_as(i32p), gcounts.ctypes.data_as(i32p),
+                                                   counts.ctypes.data_as(i32p), 0, ctypes.byref(cap))
+        ids = np.full((n, cap.value), -1, np.int32)
+        rc = self._lib.mcm_gpu_download_neighbours(self._h, counts.ctypes.data_as(i32p), gcounts.ctypes.data_as(i32p),
+                                                   ids.ctypes.data_as(i32p), ids.size, ctypes.byref(cap))
+        self._check(rc, "download_neighbours")
+        return counts, gcounts, ids
+
+    def dump(self, path: str, step: int = 0, stage: int = 0):
+        self._check(self._lib.mcm_gpu_dump(self._h, path.encode(), step, stage), "dump")
+
+    def time_steps(self, n: int) -> np.ndarray:
+        ms = (ctypes.c_double * 4)()
+        self._check(self._lib.mcm_gpu_time_steps(self._h, int(n), ms), "time_steps")
+        return np.array(ms[:])
