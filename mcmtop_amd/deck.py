@@ -322,3 +322,65 @@ def config2(dx=7.3e-5, seed=1) -> Deck:
     reference accepts: it rejects f3dm3 with an EOS, mcm_matin.f90:72-79),
     variable h (h_opt=1)."""
     return taylor_bar(dx=dx, seed=seed, model="f3dm10", corrected=True, h_opt=1)
+
+
+# ---------------------------------------------------- generic deck builders
+def material(model: int, rho: float, mass: float, h: float, card=None, eos: int = 0, eoscard=None,
+             av_q: float = 2.0, av_l: float = 1.0, rho_max: float = 0.0) -> Material:
+    """A material card set with every real fitted to its text field.  ``card``
+    and ``eoscard`` map strinput/eosinput index (0-based) -> value."""
+    c = np.zeros(48)
+    for k, val in (card or {}).items():
+        c[k] = _fit(val, 10)
+    e = np.zeros(24)
+    for k, val in (eoscard or {}).items():
+        e[k] = _fit(val, 10)
+    return Material(model, eos, _fit(rho, 10), _fit(av_q, 10), _fit(av_l, 10), _fit(mass, 10), _fit(h, 10),
+                    0.0, _fit(rho_max, 10) if rho_max else 0.0, card=c, eoscard=e)
+
+
+def f3dm1(E, nu):
+    """model 1 (elastic) property cards: E (card 1), nu (card 2)"""
+    return {0: E, 5: nu}
+
+
+def f3dm3(E, nu, sy, Et, beta=0.0, fail_flag=0, efps_fail=0.0):
+    return {0: E, 5: nu, 10: sy, 15: Et, 20: beta, 25: fail_flag, 26: efps_fail}
+
+
+def f3dm9(pcut, mu, mat_id=1):
+    """model 9 (fluid): the reference reads cm(1:2, mat) (mcm_f3dm9.f90:27-28)"""
+    b = 5 * (mat_id - 1)
+    return {0: pcut, b + 1: mu} if b == 0 else {0: pcut, b: pcut, b + 1: mu}
+
+
+def f3dm10(G, sy, Eh, pcut=0.0, a1=0.0, a2=0.0, spall=0.0, epf=0.0):
+    return {0: G, 1: sy, 2: Eh, 3: pcut, 4: a1, 5: a2, 6: spall, 7: epf}
+
+
+def block(dx, nx, ny, nz, origin=(0.0, 0.0, 0.0)):
+    """simple-cubic lattice block, nodes at origin + (i+1/2) dx"""
+    i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    pts = np.stack([i.ravel(order="F"), j.ravel(order="F"), k.ravel(order="F")], axis=1).astype(float)
+    return (pts + 0.5) * dx + np.asarray(origin)
+
+
+def make_deck(pts, vel, mats, matid=None, dispbc=None, disctype=0, h_opt=0, tcrit_opt=1, tssf=0.3,
+              sym_flag=(0, 0, 0, 0, 0, 0), sym_x=(0, 0, 0, 0, 0, 0), seed=1, jitter_dx=0.0,
+              massopt=0, pmass=None) -> Deck:
+    """Generic deck from points, velocities and materials (1-based matid)."""
+    pts = np.asarray(pts, dtype=float)
+    n = len(pts)
+    if jitter_dx:
+        pts = pts + np.random.default_rng(seed).uniform(-jitter_dx, jitter_dx, size=pts.shape)
+    pts = _fit_array(pts, 20)
+    v = np.asarray(vel, dtype=float) * np.ones((n, 3))
+    v = np.vectorize(lambda c: _fit(c, 10))(v)
+    matid = np.ones(n, np.int32) if matid is None else np.asarray(matid, np.int32)
+    bc = np.zeros(n, np.int32) if dispbc is None else np.asarray(dispbc, np.int32)
+    d = Deck(n, disctype, h_opt, tcrit_opt, 80, _fit(tssf, 10), 0.0, _fit(1.0, 10), list(mats), pts, v, matid,
+             bc, np.asarray(sym_flag, np.int32), np.array([_fit(s, 10) if s else 0.0 for s in sym_x]),
+             massopt=massopt)
+    if massopt:
+        d.pmass = np.array([_fit(m, 20) for m in pmass])
+    return d

@@ -58,7 +58,7 @@ static void alloc_particles(orc_state *st) {
     st->cont = xcalloc((size_t)n * st->ncap, sizeof(int));
     st->gnbr = xcalloc((size_t)n * st->gcap, sizeof(int));
     st->gcont = xcalloc((size_t)n * st->gcap, sizeof(int));
-    st->ngp_cap = 2 * st->np + 16;
+    st->ngp_cap = 4 * st->np + 1024;
     long g = st->ngp_cap;
     st->g_par = xcalloc(g, sizeof(int)); st->g_mat = xcalloc(g, sizeof(int)); st->g_ll = xcalloc(g, sizeof(int));
     st->g_x = xcalloc(3 * g, 8); st->g_xzero = xcalloc(3 * g, 8); st->g_v = xcalloc(3 * g, 8);
@@ -400,7 +400,7 @@ static void mirror_cell(orc_state *st, int si, int sj, int sk, int ti, int tj, i
     }
 }
 
-/* mcm_ghost_setup.f90:32-289; ghost storage is sized 2N+16 here, so the
+/* mcm_ghost_setup.f90:32-289; ghost storage is sized 4N+1024 here, so the
    capacity-retry branch (:269-282) is never taken */
 static void ghost_setup(orc_state *st) {
     st->ngp = 0;

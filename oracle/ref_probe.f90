@@ -255,10 +255,10 @@ contains
   ! but the source layer fills as the bar moves onto the plane; the overflow
   ! retry keeps stale state (mcm_ghost_setup.f90:29,269-282) and aborts.  Ghosts
   ! are regenerated from scratch every search, so give the array room for
-  ! 2N entries between steps.
-  if (mcm_boundary .and. mcm_max_ngp .lt. 2*mcm_np) then
+  ! 4N + 1024 entries between steps.
+  if (mcm_boundary .and. mcm_max_ngp .lt. 4*mcm_np + 1024) then
     deallocate(gpar)
-    mcm_max_ngp = 2*mcm_np
+    mcm_max_ngp = 4*mcm_np + 1024
     allocate(gpar(mcm_max_ngp))
   endif
   call mcm_neighbours

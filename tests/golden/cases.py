@@ -1,0 +1,114 @@
+"""Small decks that exercise every branch of the SPH step: both
+discretisations, every material model and EOS of the GPU path, symmetry and
+periodic planes (including corner ghosts), two materials in kernel contact,
+displacement boundary codes, the three critical-timestep options and both
+smoothing-length updates.  Used by make_golden.py (reference digests) and by
+the oracle and GPU parity tests.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from mcmtop_amd import deck
+
+DX = 4.0e-4
+
+
+def _quarter_bar(dx, r, nz):
+    pts = deck.block(dx, int(r / dx) + 1, int(r / dx) + 1, nz)
+    keep = pts[:, 0] ** 2 + pts[:, 1] ** 2 <= r * r
+    return pts[keep]
+
+
+def build(name: str) -> deck.Deck:
+    dx = DX
+    h = 1.2 * dx
+    if name == "plain_f3dm3":
+        return deck.taylor_bar(radius=0.8e-3, length=3.2e-3)
+    if name == "corr_f3dm10_eos4_hopt1":
+        return deck.taylor_bar(radius=0.8e-3, length=3.2e-3, model="f3dm10", corrected=True, h_opt=1)
+    if name == "quarter_f3dm1_xyz_sym_tcrit2":
+        # quarter bar on three symmetry planes: corner ghosts are made from ghosts
+        pts = _quarter_bar(dx, 1.2e-3, 7)
+        rho = 7850.0
+        m = deck.material(1, rho, rho * dx ** 3 * len(pts), h, deck.f3dm1(200e9, 0.3))
+        return deck.make_deck(pts, (0.0, 0.0, -150.0), [m], tcrit_opt=2, jitter_dx=0.01 * dx,
+                              sym_flag=(1, 0, 1, 0, 1, 0))
+    if name == "fluid_f3dm9_eos28_dispbc":
+        # water column (Murnaghan EOS, viscous) dropping on a plane; side walls held by dispbc
+        pts = deck.block(dx, 4, 4, 6)
+        rho = 1000.0
+        m = deck.material(9, rho, rho * dx ** 3 * len(pts), h, deck.f3dm9(-1.0e5, 1.0e-3), eos=28,
+                          eoscard={0: 3.0e8, 1: 7.0}, av_q=1.5, av_l=0.06)
+        bc = np.zeros(len(pts), np.int32)
+        bc[pts[:, 0] < dx] = 1
+        bc[pts[:, 1] < dx] = 2
+        bc[(pts[:, 0] < dx) & (pts[:, 1] < dx)] = 4
+        return deck.make_deck(pts, (0.0, 0.0, -50.0), [m], dispbc=bc, jitter_dx=0.01 * dx,
+                              sym_flag=(0, 0, 0, 0, 1, 0))
+    if name == "corr_f3dm10_eos1_periodic_x":
+        # slab periodic in x, corrected SPH, polynomial EOS
+        nx = 5
+        pts = deck.block(dx, nx, 4, 5)
+        rho = 2700.0
+        G = 26e9
+        m = deck.material(10, rho, rho * dx ** 3 * len(pts), h, deck.f3dm10(G, 300e6, 500e6), eos=1,
+                          eoscard={1: 70e9, 2: 30e9, 8: 1.0})
+        v = np.zeros((len(pts), 3))
+        v[:, 2] = -100.0
+        v[:, 0] = 20.0 * np.sin(2 * np.pi * pts[:, 2] / (5 * dx))
+        return deck.make_deck(pts, v, [m], disctype=1, jitter_dx=0.01 * dx,
+                              sym_flag=(2, 2, 0, 0, 1, 0), sym_x=(0.0, nx * dx, 0, 0, 0, 0))
+    if name == "f3dm10_eos41_hopt2_tcrit0":
+        pts = deck.block(dx, 4, 4, 6)
+        rho = 8930.0
+        m = deck.material(10, rho, rho * dx ** 3 * len(pts), h,
+                          deck.f3dm10(45e9, 120e6, 300e6, pcut=-5e8, a1=0.01, a2=1e-11), eos=41,
+                          eoscard={0: 3940.0, 1: 1.49, 2: 383.0, 3: 300.0, 4: 2.0, 5: 1.0})
+        return deck.make_deck(pts, (0.0, 0.0, -200.0), [m], h_opt=2, tcrit_opt=0, jitter_dx=0.01 * dx,
+                              sym_flag=(0, 0, 0, 0, 1, 0))
+    if name == "gas_f3dm10_eos13":
+        # hot ideal-gas cube expanding freely (no planes)
+        pts = deck.block(dx, 4, 4, 4)
+        rho = 1.2
+        m = deck.material(10, rho, rho * dx ** 3 * len(pts), h, deck.f3dm10(0.0 + 1.0, 1.0e30, 0.0), eos=13,
+                          eoscard={0: 1.4, 2: 2.5e5, 3: 1.0})
+        v = (pts - pts.mean(axis=0)) * 1.0e4
+        return deck.make_deck(pts, v, [m], jitter_dx=0.01 * dx)
+    if name == "two_materials_contact":
+        # f3dm3 rod (material 1) striking an f3dm10+eqos4 plate (material 2): kernel contact
+        rod = deck.block(dx, 3, 3, 5, origin=(dx, dx, 5 * dx + 1e-6))
+        plate = deck.block(dx, 5, 5, 4)
+        pts = np.concatenate([rod, plate])
+        mid = np.concatenate([np.ones(len(rod), np.int32), 2 * np.ones(len(plate), np.int32)])
+        v = np.zeros((len(pts), 3))
+        v[: len(rod), 2] = -300.0
+        m1 = deck.material(3, 8930.0, 8930.0 * dx ** 3 * len(rod), h,
+                           deck.f3dm3(117e9, 0.35, 400e6, 100e6, beta=0.5))
+        m2 = deck.material(10, 2700.0, 2700.0 * dx ** 3 * len(plate), h, deck.f3dm10(26e9, 300e6, 500e6),
+                           eos=4, eoscard={0: 5240.0, 1: 1.4, 4: 2.0, 5: 0.48})
+        return deck.make_deck(pts, v, [m1, m2], matid=mid, jitter_dx=0.01 * dx)
+    if name == "f3dm10_spall2_failure":
+        pts = deck.block(dx, 4, 4, 6)
+        rho = 7850.0
+        m = deck.material(10, rho, rho * dx ** 3 * len(pts), h,
+                          deck.f3dm10(80e9, 250e6, 1e9, pcut=-3e8, spall=2.0, epf=0.05), eos=4,
+                          eoscard={0: 4570.0, 1: 1.49, 4: 1.93})
+        v = np.zeros((len(pts), 3))
+        v[:, 2] = np.where(pts[:, 2] > 3 * dx, 150.0, -150.0)
+        return deck.make_deck(pts, v, [m], jitter_dx=0.01 * dx)
+    raise KeyError(name)
+
+
+# name -> (number of steps, steps dumped at every stage)
+CASES = {
+    "plain_f3dm3": (40, (1, 2)),
+    "corr_f3dm10_eos4_hopt1": (40, (1, 2)),
+    "quarter_f3dm1_xyz_sym_tcrit2": (30, (2,)),
+    "fluid_f3dm9_eos28_dispbc": (30, (2,)),
+    "corr_f3dm10_eos1_periodic_x": (30, (2,)),
+    "f3dm10_eos41_hopt2_tcrit0": (30, (2,)),
+    "gas_f3dm10_eos13": (30, (2,)),
+    "two_materials_contact": (30, (2,)),
+    "f3dm10_spall2_failure": (30, (2,)),
+}

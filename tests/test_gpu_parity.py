@@ -1,0 +1,201 @@
+"""Parity of the HIP path with the CPU oracle, through the C ABI.
+
+Per-step fields must agree to 1e-10 norm-relative (the per-step tolerance
+of the target); neighbour lists must be equal as sets, and are checked to
+come out in the reference's order as well.  Long runs check x - x0, v and
+sigma to 1e-7 L2 and the energy totals to 1e-9 after 1000 steps.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import cases
+from dumpio import FIELDS, Dump
+from mcmtop_amd import deck, gpu
+from oracle import STAGES, Oracle
+
+pytestmark = pytest.mark.gpu
+
+DATA = os.path.join(os.path.dirname(__file__), "golden", "data")
+PER_STEP_TOL = 1e-10
+STATE = ("x", "v", "rho", "h", "p", "e", "sigma", "alfa", "efps", "q", "c", "vabs", "pcut", "fail")
+
+
+def rel(a, b):
+    nb = np.linalg.norm(b)
+    d = np.linalg.norm(a - b)
+    return d / nb if nb > 0 else d
+
+
+def field_errors(g: Dump, o: Dump, fields=None):
+    out = {}
+    for k in fields or FIELDS:
+        if k in ("hold",) and fields is None:
+            continue
+        out[k] = rel(g.f(k), o.f(k))
+    return out
+
+
+def lists_equal_as_sets(g: Dump, o: Dump):
+    if not (np.array_equal(g.nnbr, o.nnbr) and np.array_equal(g.g_nnbr, o.g_nnbr)):
+        return False
+    for i in range(o.np):
+        if set(g.neighbours(i)) != set(o.neighbours(i)):
+            return False
+        if o.g_nnbr[i]:
+            gp = set(g.g_par[g.ghost_neighbours(i) - 1])
+            op = set(o.g_par[o.ghost_neighbours(i) - 1])
+            if gp != op:
+                return False
+    return True
+
+
+def gdump(s: gpu.Solver, tmp_path, step, stage):
+    p = str(tmp_path / ("g_%d_%d.bin" % (step, stage)))
+    s.dump(p, step, stage)
+    return Dump(p)
+
+
+@pytest.mark.parametrize("name", sorted(cases.CASES))
+def test_stage_by_stage(name, tmp_path):
+    path = os.path.join(DATA, name + ".bin")
+    o = Oracle(path)
+    o.initial()
+    with gpu.Solver(path) as s:
+        s.initial()
+        g0, o0 = gdump(s, tmp_path, 0, 0), o.state(0, 0)
+        assert lists_equal_as_sets(g0, o0)
+        errs = field_errors(g0, o0)
+        assert max(errs.values()) <= PER_STEP_TOL, errs
+        for step in (1, 2):
+            for k, st in enumerate(STAGES, start=1):
+                s.stage(st)
+                o.stage(st)
+                gd, od = gdump(s, tmp_path, step, k), o.state(step, k)
+                assert lists_equal_as_sets(gd, od), (step, st)
+                # same order as the reference walks its linked lists
+                assert np.array_equal(gd.nbr, od.nbr) and np.array_equal(gd.gnbr, od.gnbr), (step, st)
+                assert gd.ngp == od.ngp
+                errs = field_errors(gd, od)
+                worst = max(errs, key=errs.get)
+                assert errs[worst] <= PER_STEP_TOL, (step, st, worst, errs[worst])
+                for sc in ("dt", "dtold", "ptime", "critts", "kinetice", "internale"):
+                    a, b = getattr(gd, sc), getattr(od, sc)
+                    assert abs(a - b) <= PER_STEP_TOL * max(abs(b), 1e-300), (step, st, sc, a, b)
+
+
+@pytest.mark.parametrize("name", sorted(cases.CASES))
+def test_fused_steps(name, tmp_path):
+    """The fused kernels with device-side dt over the whole case."""
+    path = os.path.join(DATA, name + ".bin")
+    nsteps = cases.CASES[name][0]
+    o = Oracle(path)
+    o.initial()
+    o.run(nsteps)
+    od = o.state(nsteps, 10)
+    with gpu.Solver(path) as s:
+        s.initial()
+        s.step(nsteps)
+        gd = gdump(s, tmp_path, nsteps, 10)
+    assert lists_equal_as_sets(gd, od)
+    errs = field_errors(gd, od, STATE)
+    assert max(errs.values()) <= 1e-9, errs
+    assert gd.ptime == pytest.approx(od.ptime, rel=1e-12)
+    assert gd.kinetice == pytest.approx(od.kinetice, rel=1e-9)
+    assert gd.internale == pytest.approx(od.internale, rel=1e-9)
+
+
+def test_plain_path_is_bit_exact(tmp_path):
+    """Plain SPH with f3dm3 takes no reassociated sums: the GPU state equals
+    the oracle's (hence the reference's) bit for bit."""
+    path = os.path.join(DATA, "plain_f3dm3.bin")
+    o = Oracle(path)
+    o.initial()
+    o.run(40)
+    od = o.state(40, 10)
+    with gpu.Solver(path) as s:
+        s.initial()
+        s.step(40)
+        gd = gdump(s, tmp_path, 40, 10)
+    for k in STATE:
+        np.testing.assert_array_equal(gd.f(k), od.f(k), err_msg=k)
+    assert gd.ptime == od.ptime
+
+
+def test_runs_are_deterministic(tmp_path):
+    path = os.path.join(DATA, "corr_f3dm10_eos4_hopt1.bin")
+    outs = []
+    for r in range(2):
+        with gpu.Solver(path) as s:
+            s.initial()
+            s.step(25)
+            outs.append(gdump(s, tmp_path, 25, r).rows.copy())
+    np.testing.assert_array_equal(outs[0], outs[1])
+
+
+def test_capacity_regrow_gives_the_same_lists(tmp_path):
+    """Tiny neighbour and ghost capacities overflow on the device; the host
+    grows them and repeats the search (the reference's regrow,
+    mcm_ll_neighbours.f90:44-311, aborts instead).  Lists and results equal
+    a large-capacity run."""
+    path = os.path.join(DATA, "plain_f3dm3.bin")
+    with gpu.Solver(path) as big:
+        big.initial()
+        big.step(12)
+        ref = gdump(big, tmp_path, 12, 0)
+    with gpu.Solver(path, nbr_capacity=4, ghost_capacity=8) as small:
+        small.initial()
+        small.step(12)
+        got = gdump(small, tmp_path, 12, 1)
+    assert lists_equal_as_sets(got, ref)
+    np.testing.assert_array_equal(got.rows, ref.rows)
+
+
+def test_momentum_is_conserved_on_the_gpu(tmp_path):
+    path = os.path.join(DATA, "two_materials_contact.bin")
+    with gpu.Solver(path) as s:
+        s.initial()
+        s.step(3)
+        for st in ("neighbours", "strain", "update_h", "constitutive", "dt_update", "momentum"):
+            s.stage(st)
+        d = gdump(s, tmp_path, 4, 6)
+    m = d.f("mass")[:, None]
+    a = d.f("a")
+    assert np.abs((m * a).sum(axis=0)).max() <= 1e-12 * (m * np.abs(a)).sum(axis=0).max()
+
+
+def _long_run(tmp_path, dk, nsteps):
+    path = str(tmp_path / "deck.bin")
+    dk.write_bin(path)
+    o = Oracle(path)
+    o.initial()
+    o.run(nsteps)
+    od = o.state(nsteps, 10)
+    with gpu.Solver(path) as s:
+        s.initial()
+        s.step(nsteps)
+        gd = gdump(s, tmp_path, nsteps, 10)
+    x_err = rel(gd.f("x") - gd.f("xzero"), od.f("x") - od.f("xzero"))
+    return gd, od, {"x-x0": x_err, "v": rel(gd.f("v"), od.f("v")), "sigma": rel(gd.f("sigma"), od.f("sigma"))}
+
+
+@pytest.mark.slow
+@pytest.mark.timeout(600)
+def test_taylor_bar_plain_1000_steps(tmp_path):
+    """12.8k-particle plain Taylor bar (f3dm3), 1000 steps."""
+    gd, od, errs = _long_run(tmp_path, deck.config1(), 1000)
+    assert max(errs.values()) <= 1e-7, errs
+    assert gd.kinetice == pytest.approx(od.kinetice, rel=1e-9)
+    assert gd.internale == pytest.approx(od.internale, rel=1e-9)
+
+
+@pytest.mark.slow
+@pytest.mark.timeout(600)
+def test_taylor_bar_corrected_1000_steps(tmp_path):
+    """Corrected SPH, f3dm10 + eqos4, variable h, 1000 steps."""
+    dk = deck.taylor_bar(radius=1.2e-3, length=4.0e-3, model="f3dm10", corrected=True, h_opt=1)
+    gd, od, errs = _long_run(tmp_path, dk, 1000)
+    assert max(errs.values()) <= 1e-7, errs
+    assert gd.kinetice == pytest.approx(od.kinetice, rel=1e-9)
+    assert gd.internale == pytest.approx(od.internale, rel=1e-9)
