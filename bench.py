@@ -87,6 +87,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--dx", type=float, default=7.3e-5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--corrected-one-pass", action="store_true",
+                    help="accumulate the corrected kernels' three passes in one (rounding differs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs the SPH step on one GPU; --gpus %d is not supported "
@@ -98,7 +100,7 @@ def main():
     t = time.perf_counter()
     d = build_deck(deck_path, args.dx, 25.4e-3)
     print("bench: deck %d particles in %.1f s" % (d.np_, time.perf_counter() - t), file=sys.stderr, flush=True)
-    s = gpu.Solver(deck_path)
+    s = gpu.Solver(deck_path, corrected_one_pass=args.corrected_one_pass)
     s.initial()
     s.step(args.warmup)
     sc = s.scalars()
@@ -134,6 +136,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic Taylor-bar deck (seeded lattice + jitter)",
         "config": {"workload": "taylor_bar_corrected_f3dm10_eqos4_hopt1", "particles": n, "dx_mm": args.dx * 1e3,
+                   "corrected_passes": 1 if args.corrected_one_pass else 3,
                    "mean_neighbours": round(nbar, 2), "ghosts": sc1.ngp, "ptime_end": sc1.ptime,
                    "kernel_ms": {"neighbours": ms[0], "strain_constitutive": ms[1], "momentum_move": ms[2],
                                  "step": ms[3]}},

@@ -34,10 +34,12 @@ typedef struct mcm_gpu_ctx mcm_gpu_ctx;
 
 /* options for context creation (all zero = defaults) */
 typedef struct {
-    int32_t nbr_capacity;   /* neighbour slots per particle (0 = sized from the first search) */
-    int32_t ghost_capacity; /* ghost slots (0 = 2*np + 1024) */
+    int32_t nbr_capacity;   /* neighbour slots per particle (0 = 128; grown on overflow) */
+    int32_t ghost_capacity; /* ghost slots (0 = np/2 + 4096; grown on overflow) */
     int32_t device;         /* HIP device ordinal */
-    int32_t reserved[5];
+    int32_t corrected_one_pass; /* corrected SPH: accumulate the reference's three neighbour
+                                   passes in one (same result up to rounding); 0 = reference order */
+    int32_t reserved[4];
 } mcm_gpu_options;
 
 /* host-visible scalar state (mcm_database.f90:29-31,92-99 counterparts) */

@@ -59,11 +59,12 @@ struct DevState {
     int* gpar;                 // [ngcap] parent of each ghost (0-based real id)
     double *hold, *h0, *rho0, *p, *e, *c, *vabs, *pcut, *p_cut, *efps, *fail, *q, *mindist;
     double *sig[6], *sd[6], *alfa[6];
-    double* A[6];              // [U] (sigma - q)/rho^2 for the momentum pass
+    double* rs;                // [U][8] strain gather record {x, v, m/rho, h} (start of step)
+    double* rm;                // [U][12] momentum gather record {x, h, m, m/rho, (sigma - q)/rho^2}
     // neighbour search
     int *cell_of, *slot_of, *cell_count, *cell_start, *sorted;
-    double *sx, *sy, *sz, *sh;  // cell-sorted copies of x^n and h (candidate scan)
-    int* smat;
+    double* cand;              // [np][4] cell-sorted {x, y, z, h} of the candidates
+    int* cand_id;              // [np][2] cell-sorted {particle id, material}
     int *gstart[3], *gcount[3];
     int* nbr;                  // [ncap][npad]
     int *nreal, *ntot;

@@ -49,6 +49,7 @@ struct mcm_gpu_ctx {
     int cur = 0;                       // parity of the current h / rho buffers
     int phase = 0;                     // 0: step start, 1: after strain (x in x1, h/rho in next)
     int nb = 0;                        // blocks over np
+    int corr_one_pass = 0;             // corrected SPH: one-pass moments instead of the reference's three passes
     std::vector<double> xzero;
     std::vector<void*> allocs;
     std::string err;
@@ -246,7 +247,8 @@ int alloc_all(mcm_gpu_ctx* ctx) {
     rc |= dalloc(ctx, &d.mass, u);
     for (int k = 0; k < 2; k++) { rc |= dalloc(ctx, &d.hbuf[k], u); rc |= dalloc(ctx, &d.rbuf[k], u); }
     rc |= dalloc(ctx, &d.pmat, u);
-    for (int k = 0; k < 6; k++) rc |= dalloc(ctx, &d.A[k], u);
+    rc |= dalloc(ctx, &d.rs, 8 * u);
+    rc |= dalloc(ctx, &d.rm, 12 * u);
     rc |= dalloc(ctx, &d.gpar, d.ngcap);
     rc |= dalloc(ctx, &d.dispbc, n);
     double** per[] = {&d.hold, &d.h0, &d.rho0, &d.p, &d.e, &d.c, &d.vabs, &d.pcut, &d.p_cut, &d.efps, &d.fail,
@@ -259,8 +261,8 @@ int alloc_all(mcm_gpu_ctx* ctx) {
     for (int k = 0; k < 3; k++) { rc |= dalloc(ctx, &d.spin[k], n); rc |= dalloc(ctx, &d.a[k], n); }
     rc |= dalloc(ctx, &d.yield, n);
     rc |= dalloc(ctx, &d.cell_of, n); rc |= dalloc(ctx, &d.slot_of, n); rc |= dalloc(ctx, &d.sorted, n);
-    rc |= dalloc(ctx, &d.sx, n); rc |= dalloc(ctx, &d.sy, n); rc |= dalloc(ctx, &d.sz, n); rc |= dalloc(ctx, &d.sh, n);
-    rc |= dalloc(ctx, &d.smat, n);
+    rc |= dalloc(ctx, &d.cand, 4 * n);
+    rc |= dalloc(ctx, &d.cand_id, 2 * n);
     rc |= dalloc(ctx, &d.nreal, n); rc |= dalloc(ctx, &d.ntot, n);
     rc |= dalloc(ctx, &d.red, (size_t)ctx->nb * RED_C + 64);
     rc |= dalloc(ctx, &d.mat, d.nummat);
@@ -280,7 +282,7 @@ void launch_neighbours(mcm_gpu_ctx* ctx, int cur) {
     int ntile = blocks_for(d.cellcap, SCAN_TILE);
     k_grid_setup<<<1, 1, 0, s>>>(d);
     k_clear_cells<<<gcl, MCM_BLOCK, 0, s>>>(d);
-    k_bin<<<ctx->nb, MCM_BLOCK, 0, s>>>(d);
+    k_bin<<<ctx->nb, MCM_BLOCK, 0, s>>>(d, cur);
     k_scan_tiles<<<ntile, 1024, 0, s>>>(d);
     k_scan_top<<<1, 1024, 0, s>>>(d);
     k_scan_apply<<<ntile, 1024, 0, s>>>(d);
@@ -292,17 +294,26 @@ void launch_neighbours(mcm_gpu_ctx* ctx, int cur) {
 
 int ghost_blocks(const DevState& d) { return std::max(1, std::min(blocks_for(d.ngcap, MCM_BLOCK), 1024)); }
 
+// 0 plain SPH, 1 corrected in the reference's pass order, 2 corrected in one pass
+int corr_variant(const mcm_gpu_ctx* ctx) { return ctx->d.disctype == 1 ? (ctx->corr_one_pass ? 2 : 1) : 0; }
+
 void launch_strain_const(mcm_gpu_ctx* ctx, int cur) {
     DevState& d = ctx->d;
-    if (d.disctype == 1) k_strain_const<MODE_FUSED, true, false><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur);
-    else k_strain_const<MODE_FUSED, false, false><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur);
+    switch (corr_variant(ctx)) {
+        case 1: k_strain_const<MODE_FUSED, 1, false><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur); break;
+        case 2: k_strain_const<MODE_FUSED, 2, false><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur); break;
+        default: k_strain_const<MODE_FUSED, 0, false><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur); break;
+    }
 }
 
 void launch_momentum(mcm_gpu_ctx* ctx, int cur) {
     DevState& d = ctx->d;
     if (d.boundary) k_ghost_stress<<<ghost_blocks(d), MCM_BLOCK, 0, ctx->stream>>>(d, cur);
-    if (d.disctype == 1) k_momentum<MODE_FUSED, true, false><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur);
-    else k_momentum<MODE_FUSED, false, false><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur);
+    switch (corr_variant(ctx)) {
+        case 1: k_momentum<MODE_FUSED, 1, false><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur); break;
+        case 2: k_momentum<MODE_FUSED, 2, false><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur); break;
+        default: k_momentum<MODE_FUSED, 0, false><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur); break;
+    }
 }
 
 void launch_step(mcm_gpu_ctx* ctx, int cur) {
@@ -356,7 +367,8 @@ int regrow(mcm_gpu_ctx* ctx, const Scalars& sc_host) {
         rc |= dregrow(ctx, &d.mass, u, keep);
         for (int k = 0; k < 2; k++) { rc |= dregrow(ctx, &d.hbuf[k], u, keep); rc |= dregrow(ctx, &d.rbuf[k], u, keep); }
         rc |= dregrow(ctx, &d.pmat, u, keep);
-        for (int k = 0; k < 6; k++) rc |= dregrow(ctx, &d.A[k], u, keep);
+        rc |= dregrow(ctx, &d.rs, 8 * u, 8 * keep);
+        rc |= dregrow(ctx, &d.rm, 12 * u, 12 * keep);
         rc |= dregrow(ctx, &d.gpar, ng_new, 0);
         if (rc) return rc;
         d.ngcap = ng_new;
@@ -527,6 +539,7 @@ int mcm_gpu_create_from_deck(const char* deck_path, const mcm_gpu_options* opt, 
     mcm_gpu_ctx* ctx = new mcm_gpu_ctx();
     *out = ctx;
     ctx->device = opt ? opt->device : 0;
+    ctx->corr_one_pass = opt ? opt->corrected_one_pass : 0;
     HIPCHK(hipSetDevice(ctx->device));
     HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     int rc = read_deck(ctx, deck_path);
@@ -629,8 +642,11 @@ int mcm_gpu_neighbours(mcm_gpu_ctx* ctx) {
 int mcm_gpu_strain(mcm_gpu_ctx* ctx) {
     if (!ctx || ctx->phase != 0) return MCM_GPU_ERR_ARG;
     DevState& d = ctx->d;
-    if (d.disctype == 1) k_strain_const<MODE_STRAIN, true, true><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, ctx->cur);
-    else k_strain_const<MODE_STRAIN, false, true><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, ctx->cur);
+    switch (corr_variant(ctx)) {
+        case 1: k_strain_const<MODE_STRAIN, 1, true><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, ctx->cur); break;
+        case 2: k_strain_const<MODE_STRAIN, 2, true><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, ctx->cur); break;
+        default: k_strain_const<MODE_STRAIN, 0, true><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, ctx->cur); break;
+    }
     HIPCHK(hipGetLastError());
     ctx->phase = 1;
     return 0;
@@ -645,7 +661,7 @@ int mcm_gpu_update_h(mcm_gpu_ctx* ctx) {
 
 int mcm_gpu_constitutive(mcm_gpu_ctx* ctx) {
     if (!ctx || ctx->phase != 1) return MCM_GPU_ERR_ARG;
-    k_strain_const<MODE_CONST, false, true><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(ctx->d, ctx->cur);
+    k_strain_const<MODE_CONST, 0, true><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(ctx->d, ctx->cur);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -661,8 +677,11 @@ int mcm_gpu_momentum(mcm_gpu_ctx* ctx) {
     if (!ctx || ctx->phase != 1) return MCM_GPU_ERR_ARG;
     DevState& d = ctx->d;
     if (d.boundary) k_ghost_stress<<<ghost_blocks(d), MCM_BLOCK, 0, ctx->stream>>>(d, ctx->cur);
-    if (d.disctype == 1) k_momentum<MODE_MOMENTUM, true, true><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, ctx->cur);
-    else k_momentum<MODE_MOMENTUM, false, true><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, ctx->cur);
+    switch (corr_variant(ctx)) {
+        case 1: k_momentum<MODE_MOMENTUM, 1, true><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, ctx->cur); break;
+        case 2: k_momentum<MODE_MOMENTUM, 2, true><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, ctx->cur); break;
+        default: k_momentum<MODE_MOMENTUM, 0, true><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, ctx->cur); break;
+    }
     HIPCHK(hipGetLastError());
     return 0;
 }

@@ -120,15 +120,25 @@ __global__ void k_clear_cells(DevState d) {
     }
 }
 
-// histogram: cell of every real particle (mcm_setuplist.f90:320-345)
-__global__ __launch_bounds__(256) void k_bin(DevState d) {
+// histogram: cell of every real particle (mcm_setuplist.f90:320-345); also
+// lays down the particle's strain gather record for this step
+__global__ __launch_bounds__(256) void k_bin(DevState d, int cur) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.np || d.sc->abort) return;
     const GridDev g = d.sc->grid;
+    const double x[3] = {d.x0[0][i], d.x0[1][i], d.x0[2][i]};
+    {
+        double2* r = reinterpret_cast<double2*>(d.rs) + 4 * (long)i;
+        double vx = d.v[0][i], vy = d.v[1][i], vz = d.v[2][i];
+        r[0] = make_double2(x[0], x[1]);
+        r[1] = make_double2(x[2], vx);
+        r[2] = make_double2(vy, vz);
+        r[3] = make_double2(d.mass[i] / d.rbuf[cur][i], d.hbuf[cur][i]);
+    }
     int box[3];
     bool in = true;
     for (int j = 0; j < 3; j++) {
-        box[j] = cell_index(g.gridsize[j] * (d.x0[j][i] - g.gridmin[j]));
+        box[j] = cell_index(g.gridsize[j] * (x[j] - g.gridmin[j]));
         if (box[j] < 1 || box[j] > g.lim[j]) in = false;
     }
     if (in) {
@@ -212,10 +222,13 @@ __global__ __launch_bounds__(256) void k_sort_cells(DevState d, int cur) {
             seg[b + 1] = key;
         }
         int st = d.cell_start[c];
+        double2* cd = reinterpret_cast<double2*>(d.cand) + 2 * (long)st;
+        int2* ci = reinterpret_cast<int2*>(d.cand_id) + st;
         for (int a = 0; a < n; a++) {
             int j = seg[a];
-            d.sx[st + a] = d.x0[0][j]; d.sy[st + a] = d.x0[1][j]; d.sz[st + a] = d.x0[2][j];
-            d.sh[st + a] = h[j]; d.smat[st + a] = d.pmat[j];
+            cd[2 * a] = make_double2(d.x0[0][j], d.x0[1][j]);
+            cd[2 * a + 1] = make_double2(d.x0[2][j], h[j]);
+            ci[a] = make_int2(j, d.pmat[j]);
         }
     }
 }
@@ -231,12 +244,24 @@ __device__ __forceinline__ void make_ghost(const DevState& d, int cur, const Pla
         d.x1[k][u] = xv;
         d.v[k][u] = d.v[k][us] * P.mult[k];
     }
-    d.mass[u] = d.mass[us];
+    double mv = d.mass[us];
+    d.mass[u] = mv;
     double hv = d.hbuf[cur][us], rv = d.rbuf[cur][us];
     d.hbuf[0][u] = hv; d.hbuf[1][u] = hv;
     d.rbuf[0][u] = rv; d.rbuf[1][u] = rv;
     d.pmat[u] = d.pmat[us];
     d.gpar[gid] = from_real ? src : d.gpar[src];
+    // gather records of the ghost: start-of-step x, v, m, rho and h are
+    // frozen for the whole step (mcm_ghost_setup.f90:330-341)
+    double2* rs = reinterpret_cast<double2*>(d.rs) + 4 * (long)u;
+    rs[0] = make_double2(d.x0[0][u], d.x0[1][u]);
+    rs[1] = make_double2(d.x0[2][u], d.v[0][u]);
+    rs[2] = make_double2(d.v[1][u], d.v[2][u]);
+    rs[3] = make_double2(mv / rv, hv);
+    double2* rm = reinterpret_cast<double2*>(d.rm) + 6 * (long)u;
+    rm[0] = make_double2(d.x0[0][u], d.x0[1][u]);
+    rm[1] = make_double2(d.x0[2][u], hv);
+    rm[2] = make_double2(mv, mv / rv);
 }
 
 // One plane of mcm_ghost_setup.f90:32-266.  A single 1024-thread workgroup
@@ -330,23 +355,37 @@ __global__ __launch_bounds__(256) void k_list(DevState d, int cur) {
         int lo1 = max(b1 - 1, 0), hi1 = min(b1 + 1, L1 - 1);
         int lo2 = max(b2 - 1, 0), hi2 = min(b2 + 1, g.lim[2] - 1);
         const int npass = d.nummat > 1 ? 2 : 1;
+        const double2* cd = reinterpret_cast<const double2*>(d.cand);
+        const int2* ci = reinterpret_cast<const int2*>(d.cand_id);
         for (int pass = 0; pass < npass; pass++) {
             for (int n = lo2; n <= hi2; n++)
-                for (int m = lo1; m <= hi1; m++)
-                    for (int l = lo0; l <= hi0; l++) {
-                        int cc = (n * L1 + m) * L0 + l;
-                        if (d.gcount[0][cc] + d.gcount[1][cc] + d.gcount[2][cc] > 0) continue;
-                        int st = d.cell_start[cc], en = st + d.cell_count[cc];
+                for (int m = lo1; m <= hi1; m++) {
+                    // the x-run of cells (lo0..hi0) of this row is contiguous in the
+                    // sorted order; a row holding a ghost-headed cell is walked cell
+                    // by cell so that its real particles stay hidden
+                    int c0 = (n * L1 + m) * L0 + lo0;
+                    int nrun = hi0 - lo0 + 1;
+                    bool ghosty = false;
+                    for (int l = 0; l < nrun; l++)
+                        ghosty |= (d.gcount[0][c0 + l] + d.gcount[1][c0 + l] + d.gcount[2][c0 + l]) > 0;
+                    for (int l = 0; l < (ghosty ? nrun : 1); l++) {
+                        int cc = c0 + l;
+                        if (ghosty && (d.gcount[0][cc] + d.gcount[1][cc] + d.gcount[2][cc]) > 0) continue;
+                        int st = d.cell_start[cc];
+                        int en = ghosty ? st + d.cell_count[cc]
+                                        : d.cell_start[c0 + nrun - 1] + d.cell_count[c0 + nrun - 1];
                         for (int s = st; s < en; s++) {
-                            int j = d.sorted[s];
+                            int2 idm = ci[s];
+                            int j = idm.x;
                             if (j == i) continue;
+                            double2 pa = cd[2 * s], pb = cd[2 * s + 1];
                             double dist_sq = 0.0;
-                            double dx = d.sx[s] - xi; dist_sq = dist_sq + dx * dx;
-                            double dy = d.sy[s] - yi; dist_sq = dist_sq + dy * dy;
-                            double dz = d.sz[s] - zi; dist_sq = dist_sq + dz * dz;
-                            double h_avg = 2.0 * 0.5 * (d.sh[s] + hi);
+                            double dx = pa.x - xi; dist_sq = dist_sq + dx * dx;
+                            double dy = pa.y - yi; dist_sq = dist_sq + dy * dy;
+                            double dz = pb.x - zi; dist_sq = dist_sq + dz * dz;
+                            double h_avg = 2.0 * 0.5 * (pb.y + hi);
                             if (dist_sq < h_avg * h_avg) {
-                                bool same = d.smat[s] == mi;
+                                bool same = idm.y == mi;
                                 if (pass == 0 && same) {
                                     mind = fmin(mind, dist_sq);
                                     if (cnt < ncap) out[(long)cnt * npad] = j;
@@ -359,6 +398,7 @@ __global__ __launch_bounds__(256) void k_list(DevState d, int cur) {
                             }
                         }
                     }
+                }
         }
         if (d.disctype == 1) {
             if (cnt < ncap) out[(long)cnt * npad] = i;

@@ -104,28 +104,44 @@ __device__ __forceinline__ void load_state(const DevState& d, int i, int cur, PS
 }
 
 // -------------------------------------------------------------- strain
+// Neighbour records: the strain passes read rs[u] = {x, y, z, vx, vy, vz,
+// m/rho, h} (8 doubles, 64 B) and the momentum passes rm[u] = {x, y, z, h,
+// m, m/rho, A11, A22, A33, A12, A13, A23} (12 doubles, 96 B), one aligned
+// record per neighbour instead of one load per state array.
+struct SRec { double x, y, z, vx, vy, vz, vol, h; };
+struct MRec { double x, y, z, h, m, vol, A[6]; };
+
+__device__ __forceinline__ SRec load_srec(const double* rs, int j) {
+    const double2* r = reinterpret_cast<const double2*>(rs) + 4 * (long)j;
+    double2 a = r[0], b = r[1], c = r[2], e = r[3];
+    return SRec{a.x, a.y, b.x, b.y, c.x, c.y, e.x, e.y};
+}
+__device__ __forceinline__ MRec load_mrec(const double* rm, int j) {
+    const double2* r = reinterpret_cast<const double2*>(rm) + 6 * (long)j;
+    double2 a = r[0], b = r[1], c = r[2], e = r[3], f = r[4], g = r[5];
+    return MRec{a.x, a.y, b.x, b.y, c.x, c.y, {e.x, e.y, f.x, f.y, g.x, g.y}};
+}
+
 // plain strain rate, mcm_oldstrain.f90:34-64, positions half a step back
 // for real particles only (mcm_strain.f90:27-31)
-__device__ __forceinline__ void strain_plain(const DevState& d, int i, int cur, double dt, const double xi[3],
+__device__ __forceinline__ void strain_plain(const DevState& d, int i, double dt, const double xi[3],
                                              const double vi[3], double hi, double g[3][3]) {
     const int n = d.ntot[i];
     const int* lst = d.nbr + i;
-    const double* h = d.hbuf[cur];
-    const double* rho = d.rbuf[cur];
     for (int a = 0; a < 3; a++)
         for (int b = 0; b < 3; b++) g[a][b] = 0.0;
     for (int k = 0; k < n; k++) {
         int j = lst[(long)k * d.npad];
+        SRec r = load_srec(d.rs, j);
         double sh = j < d.np ? dt : 0.0;
-        double vj0 = d.v[0][j], vj1 = d.v[1][j], vj2 = d.v[2][j];
-        double xj0 = d.x0[0][j] - 0.5 * vj0 * sh;
-        double xj1 = d.x0[1][j] - 0.5 * vj1 * sh;
-        double xj2 = d.x0[2][j] - 0.5 * vj2 * sh;
-        double volj = d.mass[j] / rho[j];
-        double havg = 0.5 * (hi + h[j]);
+        double xj0 = r.x - 0.5 * r.vx * sh;
+        double xj1 = r.y - 0.5 * r.vy * sh;
+        double xj2 = r.z - 0.5 * r.vz * sh;
+        double havg = 0.5 * (hi + r.h);
         double dwdx[3], rad;
         kernel_grad(xi[0] - xj0, xi[1] - xj1, xi[2] - xj2, havg, d.pi, dwdx, &rad);
-        double dv0 = vj0 - vi[0], dv1 = vj1 - vi[1], dv2 = vj2 - vi[2];
+        double dv0 = r.vx - vi[0], dv1 = r.vy - vi[1], dv2 = r.vz - vi[2];
+        double volj = r.vol;
         g[0][0] = g[0][0] + volj * dv0 * dwdx[0];
         g[0][1] = g[0][1] + volj * dv0 * dwdx[1];
         g[0][2] = g[0][2] + volj * dv0 * dwdx[2];
@@ -138,28 +154,78 @@ __device__ __forceinline__ void strain_plain(const DevState& d, int i, int cur, 
     }
 }
 
-// corrected strain rate, mcm_corr_strain.f90:37-146, one pass over the list
-__device__ __forceinline__ void strain_corr(const DevState& d, int i, int cur, double dt, const double xi[3],
-                                            double hi, double f[3][3]) {
+// kernel value and gradient of one pair of the corrected passes, centred at
+// a (xij = a - b), with the neighbour's half-step position
+__device__ __forceinline__ void corr_pair(const SRec& r, double sh, const double xi[3], double hi, double pi,
+                                          double xj[3], double& w, double dwdx[3]) {
+    xj[0] = r.x - 0.5 * r.vx * sh;
+    xj[1] = r.y - 0.5 * r.vy * sh;
+    xj[2] = r.z - 0.5 * r.vz * sh;
+    double havg = 0.5 * (hi + r.h);
+    double rad;
+    kernel_grad(xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2], havg, pi, dwdx, &rad);
+    w = kernel_w(rad, havg, pi);
+}
+
+// corrected strain rate, mcm_corr_strain.f90:37-146, in the reference's three
+// passes and evaluation order
+__device__ __forceinline__ void strain_corr(const DevState& d, int i, double dt, const double xi[3], double hi,
+                                            double f[3][3]) {
     const int n = d.ntot[i];
     const int* lst = d.nbr + i;
-    const double* h = d.hbuf[cur];
-    const double* rho = d.rbuf[cur];
+    double sw = 0.0, sg[3] = {0.0, 0.0, 0.0};
+    for (int k = 0; k < n; k++) {
+        int j = lst[(long)k * d.npad];
+        SRec r = load_srec(d.rs, j);
+        double xj[3], w, dwdx[3];
+        corr_pair(r, j < d.np ? dt : 0.0, xi, hi, d.pi, xj, w, dwdx);
+        sw = sw + r.vol * w;
+        for (int l = 0; l < 3; l++) sg[l] = sg[l] + r.vol * dwdx[l];
+    }
+    const double sw2 = sw * sw;
+    double B[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, invB[9];
+    for (int k = 0; k < n; k++) {
+        int j = lst[(long)k * d.npad];
+        SRec r = load_srec(d.rs, j);
+        double xj[3], w, dwdx[3], cg[3];
+        corr_pair(r, j < d.np ? dt : 0.0, xi, hi, d.pi, xj, w, dwdx);
+        for (int m = 0; m < 3; m++) cg[m] = (dwdx[m] * sw - w * sg[m]) / sw2;
+        for (int m = 0; m < 3; m++)
+            for (int nn = 0; nn < 3; nn++) B[nn + 3 * m] = B[nn + 3 * m] + r.vol * (xj[nn] - xi[nn]) * cg[m];
+    }
+    inverse3(B, invB);
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) f[a][b] = 0.0;
+    for (int k = 0; k < n; k++) {
+        int j = lst[(long)k * d.npad];
+        SRec r = load_srec(d.rs, j);
+        double xj[3], w, dwdx[3], cg[3], g[3];
+        corr_pair(r, j < d.np ? dt : 0.0, xi, hi, d.pi, xj, w, dwdx);
+        for (int m = 0; m < 3; m++) cg[m] = (dwdx[m] * sw - w * sg[m]) / sw2;
+        for (int q = 0; q < 3; q++) g[q] = invB[q] * cg[0] + invB[q + 3] * cg[1] + invB[q + 6] * cg[2];
+        double vj[3] = {r.vx, r.vy, r.vz};
+        for (int nn = 0; nn < 3; nn++)
+            for (int m = 0; m < 3; m++) f[nn][m] = f[nn][m] + r.vol * vj[nn] * g[m];
+    }
+}
+
+// corrected strain rate in one pass: the correction is linear in the pass-1
+// sums, so B and the contracted gradient follow from single-pass moments
+// (same result up to rounding; the fast variant)
+__device__ __forceinline__ void strain_corr_fast(const DevState& d, int i, double dt, const double xi[3],
+                                                 double hi, double f[3][3]) {
+    const int n = d.ntot[i];
+    const int* lst = d.nbr + i;
     double sw = 0.0, sg[3] = {0, 0, 0}, P[3][3] = {{0}}, Pw[3] = {0, 0, 0}, Q[3][3] = {{0}}, Qw[3] = {0, 0, 0};
     for (int k = 0; k < n; k++) {
         int j = lst[(long)k * d.npad];
-        double sh = j < d.np ? dt : 0.0;
-        double vj[3] = {d.v[0][j], d.v[1][j], d.v[2][j]};
-        double xj[3];
-        for (int a = 0; a < 3; a++) xj[a] = d.x0[a][j] - 0.5 * vj[a] * sh;
-        double volj = d.mass[j] / rho[j];
-        double havg = 0.5 * (hi + h[j]);
-        double dwdx[3], rad;
-        kernel_grad(xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2], havg, d.pi, dwdx, &rad);
-        double w = kernel_w(rad, havg, d.pi);
-        double vw = volj * w;
+        SRec r = load_srec(d.rs, j);
+        double xj[3], w, dwdx[3];
+        corr_pair(r, j < d.np ? dt : 0.0, xi, hi, d.pi, xj, w, dwdx);
+        double vw = r.vol * w;
+        double vj[3] = {r.vx, r.vy, r.vz};
         sw += vw;
-        double vg[3] = {volj * dwdx[0], volj * dwdx[1], volj * dwdx[2]};
+        double vg[3] = {r.vol * dwdx[0], r.vol * dwdx[1], r.vol * dwdx[2]};
         for (int a = 0; a < 3; a++) {
             sg[a] += vg[a];
             double dxa = xj[a] - xi[a];
@@ -178,13 +244,14 @@ __device__ __forceinline__ void strain_corr(const DevState& d, int i, int cur, d
     inverse3(B, invB);
     for (int a = 0; a < 3; a++) {
         double t[3];
-        for (int s = 0; s < 3; s++) t[s] = (Q[a][s] * sw - Qw[a] * sg[s]) / sw2;
+        for (int q = 0; q < 3; q++) t[q] = (Q[a][q] * sw - Qw[a] * sg[q]) / sw2;
         for (int m = 0; m < 3; m++) f[a][m] = invB[m] * t[0] + invB[m + 3] * t[1] + invB[m + 6] * t[2];
     }
 }
 
 // ------------------------------------------------------- k_strain_const
-template <int MODE, bool CORR, bool DIAG>
+// CORR: 0 plain SPH, 1 corrected (reference order), 2 corrected (one pass)
+template <int MODE, int CORR, bool DIAG>
 __global__ __launch_bounds__(MCM_BLOCK) void k_strain_const(DevState d, int cur) {
     const int i = blockIdx.x * MCM_BLOCK + threadIdx.x;
     const int nxt = cur ^ 1;
@@ -195,6 +262,7 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_strain_const(DevState d, int cur)
     int eos_err = 0;
     if (live) {
         PState s;
+        double x1z_keep = 0.0;
         if (MODE != MODE_CONST) {
             double xr[3], vi[3], xi[3];
             for (int a = 0; a < 3; a++) {
@@ -204,11 +272,20 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_strain_const(DevState d, int cur)
             }
             double hi = d.hbuf[cur][i];
             double g[3][3];
-            if (CORR) strain_corr(d, i, cur, dt, xi, hi, g);
-            else strain_plain(d, i, cur, dt, xi, vi, hi, g);
+            if (CORR == 1) strain_corr(d, i, dt, xi, hi, g);
+            else if (CORR == 2) strain_corr_fast(d, i, dt, xi, hi, g);
+            else strain_plain(d, i, dt, xi, vi, hi, g);
             rod_spin(g, s);
             // the reference moves x back by the same half step: (x - t) + t
-            for (int a = 0; a < 3; a++) d.x1[a][i] = xi[a] + 0.5 * vi[a] * dt;
+            double x1[3];
+            for (int a = 0; a < 3; a++) {
+                x1[a] = xi[a] + 0.5 * vi[a] * dt;
+                d.x1[a][i] = x1[a];
+            }
+            x1z_keep = x1[2];
+            double2* rm = reinterpret_cast<double2*>(d.rm) + 6 * (long)i;
+            rm[0] = make_double2(x1[0], x1[1]);
+            if (MODE == MODE_STRAIN) rm[1] = make_double2(x1[2], hi);
             if (MODE == MODE_STRAIN) {
                 for (int k = 0; k < 6; k++) d.rod[k][i] = s.rod[k];
                 for (int k = 0; k < 3; k++) d.spin[k][i] = s.spin[k];
@@ -256,13 +333,16 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_strain_const(DevState d, int cur)
                 d.fail[i] = s.fail;
                 d.defps_prev[i] = defps_prev;
             }
+            // momentum record: x after the strain pass, new h, mass, volume and
+            // (sigma - q)/rho^2 (mcm_oldacceleration.f90:79-80, per particle)
             double r2 = s.rho * s.rho;
-            d.A[S11][i] = (s.sig[S11] - s.q) / r2;
-            d.A[S22][i] = (s.sig[S22] - s.q) / r2;
-            d.A[S33][i] = (s.sig[S33] - s.q) / r2;
-            d.A[S12][i] = (s.sig[S12] - 0.0) / r2;
-            d.A[S13][i] = (s.sig[S13] - 0.0) / r2;
-            d.A[S23][i] = (s.sig[S23] - 0.0) / r2;
+            double2* rm = reinterpret_cast<double2*>(d.rm) + 6 * (long)i;
+            double x1z = MODE == MODE_CONST ? d.x1[2][i] : x1z_keep;
+            rm[1] = make_double2(x1z, s.h);
+            rm[2] = make_double2(s.mass, s.mass / s.rho);
+            rm[3] = make_double2((s.sig[S11] - s.q) / r2, (s.sig[S22] - s.q) / r2);
+            rm[4] = make_double2((s.sig[S33] - s.q) / r2, (s.sig[S12] - 0.0) / r2);
+            rm[5] = make_double2((s.sig[S13] - 0.0) / r2, (s.sig[S23] - 0.0) / r2);
             if (DIAG) {
                 for (int k = 0; k < 6; k++) d.rod[k][i] = s.rod[k];
                 for (int k = 0; k < 3; k++) d.spin[k][i] = s.spin[k];
@@ -344,90 +424,136 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_ghost_stress(DevState d, int cur)
     if (d.sc->abort) return;
     const int ngp = d.sc->ngp;
     for (int g = blockIdx.x * MCM_BLOCK + threadIdx.x; g < ngp; g += gridDim.x * MCM_BLOCK) {
-    const int u = d.np + g, p = d.gpar[g];
-    double rg = d.rbuf[nxt][u];
-    double r2 = rg * rg;
-    double q = d.q[p];
-    d.A[S11][u] = (d.sig[S11][p] - q) / r2;
-    d.A[S22][u] = (d.sig[S22][p] - q) / r2;
-    d.A[S33][u] = (d.sig[S33][p] - q) / r2;
-    d.A[S12][u] = -d.sig[S12][p] / r2;
-    d.A[S13][u] = -d.sig[S13][p] / r2;
-    d.A[S23][u] = -d.sig[S23][p] / r2;
+        const int u = d.np + g, p = d.gpar[g];
+        double rg = d.rbuf[nxt][u];
+        double r2 = rg * rg;
+        double q = d.q[p];
+        double2* rm = reinterpret_cast<double2*>(d.rm) + 6 * (long)u;
+        rm[3] = make_double2((d.sig[S11][p] - q) / r2, (d.sig[S22][p] - q) / r2);
+        rm[4] = make_double2((d.sig[S33][p] - q) / r2, -d.sig[S12][p] / r2);
+        rm[5] = make_double2(-d.sig[S13][p] / r2, -d.sig[S23][p] / r2);
     }
 }
 
 // ------------------------------------------------------------- momentum
-__device__ __forceinline__ double symA(const double a[6], int r, int c) {
-    if (r == c) return a[r];
-    int k = (r + c == 1) ? S12 : (r + c == 2 ? S13 : S23);
-    return a[k];
+__device__ __forceinline__ int sym_index(int r, int c) {
+    return r == c ? r : ((r + c == 1) ? S12 : (r + c == 2 ? S13 : S23));
 }
 
 // mcm_oldacceleration.f90:60-89
-__device__ __forceinline__ void accel_plain(const DevState& d, int i, int nxt, const double xi[3], double hi,
-                                            const double Ai[6], double acc[3]) {
+__device__ __forceinline__ void accel_plain(const DevState& d, int i, const MRec& me, double acc[3]) {
     const int n = d.ntot[i];
     const int* lst = d.nbr + i;
-    const double* h = d.hbuf[nxt];
     acc[0] = 0.0; acc[1] = 0.0; acc[2] = 0.0;
     for (int k = 0; k < n; k++) {
         int j = lst[(long)k * d.npad];
-        double havg = 0.5 * (hi + h[j]);
+        MRec r = load_mrec(d.rm, j);
+        double havg = 0.5 * (me.h + r.h);
         double dwdx[3], rad;
-        kernel_grad(xi[0] - d.x1[0][j], xi[1] - d.x1[1][j], xi[2] - d.x1[2][j], havg, d.pi, dwdx, &rad);
-        double Aj[6];
-        for (int c = 0; c < 6; c++) Aj[c] = d.A[c][j];
-        double mj = d.mass[j];
+        kernel_grad(me.x - r.x, me.y - r.y, me.z - r.z, havg, d.pi, dwdx, &rad);
         for (int nn = 0; nn < 3; nn++) {
             double ds = 0.0;
-            for (int mm = 0; mm < 3; mm++) ds = ds + (symA(Aj, mm, nn) + symA(Ai, mm, nn)) * dwdx[mm];
-            acc[nn] = acc[nn] + mj * ds;
+            for (int mm = 0; mm < 3; mm++) {
+                int c = sym_index(mm, nn);
+                ds = ds + (r.A[c] + me.A[c]) * dwdx[mm];
+            }
+            acc[nn] = acc[nn] + r.m * ds;
         }
     }
 }
 
-// mcm_corr_acceleration.f90:37-163 in one pass: kernel centred at j, the
-// self slot (k == nnbr) contributes nothing to deltasig, the mass of the last
-// neighbour multiplies the sum and B^-1 enters transposed.
-__device__ __forceinline__ void accel_corr(const DevState& d, int i, int nxt, const double xi[3], double hi,
-                                           const double Ai[6], double acc[3]) {
+// kernel value and gradient centred at the neighbour: W(x_j, x_i),
+// grad_j W (mcm_corr_acceleration.f90:73-74)
+__device__ __forceinline__ void corr_pair_j(const MRec& r, const MRec& me, double pi, double& w, double dwdx[3]) {
+    double havg = 0.5 * (me.h + r.h);
+    double rad;
+    kernel_grad(r.x - me.x, r.y - me.y, r.z - me.z, havg, pi, dwdx, &rad);
+    w = kernel_w(rad, havg, pi);
+}
+
+// mcm_corr_acceleration.f90:37-163 in the reference's three passes: the self
+// slot (k == nnbr) contributes nothing to deltasig, the mass of the last
+// neighbour multiplies the sum and B^-1 enters transposed
+__device__ __forceinline__ void accel_corr(const DevState& d, int i, const MRec& me, double acc[3]) {
     const int n = d.ntot[i];
     const int self_k = d.nreal[i] - 1;
     const int* lst = d.nbr + i;
-    const double* h = d.hbuf[nxt];
-    const double* rho = d.rbuf[nxt];
+    double sw = 0.0, sg[3] = {0.0, 0.0, 0.0}, massj = 0.0;
+    for (int k = 0; k < n; k++) {
+        MRec r = load_mrec(d.rm, lst[(long)k * d.npad]);
+        double w, dwdx[3];
+        corr_pair_j(r, me, d.pi, w, dwdx);
+        sw = sw + r.vol * w;
+        for (int l = 0; l < 3; l++) sg[l] = sg[l] + r.vol * dwdx[l];
+        massj = r.m;
+    }
+    const double sw2 = sw * sw;
+    double B[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, invB[9];
+    for (int k = 0; k < n; k++) {
+        MRec r = load_mrec(d.rm, lst[(long)k * d.npad]);
+        double w, dwdx[3], cg[3];
+        corr_pair_j(r, me, d.pi, w, dwdx);
+        for (int m = 0; m < 3; m++) cg[m] = (dwdx[m] * sw - w * sg[m]) / sw2;
+        double dx[3] = {r.x - me.x, r.y - me.y, r.z - me.z};
+        for (int m = 0; m < 3; m++)
+            for (int nn = 0; nn < 3; nn++) B[nn + 3 * m] = B[nn + 3 * m] + r.vol * dx[nn] * cg[m];
+    }
+    inverse3(B, invB);
+    // deltasig(m,n,l) is symmetric in (m,n): 6 x 3 accumulators
+    double D[6][3] = {{0.0}};
+    for (int k = 0; k < n; k++) {
+        if (k == self_k) continue;   // dsig = 0 there
+        MRec r = load_mrec(d.rm, lst[(long)k * d.npad]);
+        double w, dwdx[3], cg[3];
+        corr_pair_j(r, me, d.pi, w, dwdx);
+        for (int m = 0; m < 3; m++) cg[m] = (dwdx[m] * sw - w * sg[m]) / sw2;
+        for (int c = 0; c < 6; c++) {
+            double dsig = r.A[c] + me.A[c];
+            D[c][0] = D[c][0] + dsig * cg[0];
+            D[c][1] = D[c][1] + dsig * cg[1];
+            D[c][2] = D[c][2] + dsig * cg[2];
+        }
+    }
+    for (int nn = 0; nn < 3; nn++) {
+        double a = 0.0;
+        for (int mm = 0; mm < 3; mm++) {
+            int c = sym_index(nn, mm);
+            for (int l = 0; l < 3; l++) a = a + massj * D[c][l] * invB[l + 3 * mm];
+        }
+        acc[nn] = a;
+    }
+}
+
+// corrected acceleration in one pass from single-pass moments (fast variant)
+__device__ __forceinline__ void accel_corr_fast(const DevState& d, int i, const MRec& me, double acc[3]) {
+    const int n = d.ntot[i];
+    const int self_k = d.nreal[i] - 1;
+    const int* lst = d.nbr + i;
     double sw = 0.0, sg[3] = {0, 0, 0}, P[3][3] = {{0}}, Pw[3] = {0, 0, 0};
     double E1[6][3] = {{0}}, E2[6] = {0, 0, 0, 0, 0, 0};
     double mlast = 0.0;
     for (int k = 0; k < n; k++) {
-        int j = lst[(long)k * d.npad];
-        double xj[3] = {d.x1[0][j], d.x1[1][j], d.x1[2][j]};
-        double mj = d.mass[j];
-        double volj = mj / rho[j];
-        double havg = 0.5 * (hi + h[j]);
-        double dwdx[3], rad;
-        kernel_grad(xj[0] - xi[0], xj[1] - xi[1], xj[2] - xi[2], havg, d.pi, dwdx, &rad);
-        double w = kernel_w(rad, havg, d.pi);
-        double vw = volj * w;
+        MRec r = load_mrec(d.rm, lst[(long)k * d.npad]);
+        double w, dwdx[3];
+        corr_pair_j(r, me, d.pi, w, dwdx);
+        double dx[3] = {r.x - me.x, r.y - me.y, r.z - me.z};
+        double vw = r.vol * w;
         sw += vw;
         for (int a = 0; a < 3; a++) {
-            double vg = volj * dwdx[a];
-            sg[a] += vg;
-            double dxa = xj[a] - xi[a];
-            Pw[a] += dxa * vw;
-            for (int b = 0; b < 3; b++) P[a][b] += dxa * (volj * dwdx[b]);
+            sg[a] += r.vol * dwdx[a];
+            Pw[a] += dx[a] * vw;
+            for (int b = 0; b < 3; b++) P[a][b] += dx[a] * (r.vol * dwdx[b]);
         }
         if (k != self_k) {
             for (int c = 0; c < 6; c++) {
-                double ds = d.A[c][j] + Ai[c];
+                double ds = r.A[c] + me.A[c];
                 E2[c] += ds * w;
                 E1[c][0] += ds * dwdx[0];
                 E1[c][1] += ds * dwdx[1];
                 E1[c][2] += ds * dwdx[2];
             }
         }
-        mlast = mj;
+        mlast = r.m;
     }
     double sw2 = sw * sw;
     double B[9], invB[9];
@@ -435,15 +561,12 @@ __device__ __forceinline__ void accel_corr(const DevState& d, int i, int nxt, co
         for (int a = 0; a < 3; a++) B[a + 3 * m] = (P[a][m] * sw - Pw[a] * sg[m]) / sw2;
     inverse3(B, invB);
     for (int nn = 0; nn < 3; nn++) {
-        double s = 0.0;
+        double sacc = 0.0;
         for (int mm = 0; mm < 3; mm++) {
-            int c = (nn == mm) ? nn : ((nn + mm == 1) ? S12 : (nn + mm == 2 ? S13 : S23));
-            for (int l = 0; l < 3; l++) {
-                double D = (E1[c][l] * sw - E2[c] * sg[l]) / sw2;
-                s += D * invB[l + 3 * mm];
-            }
+            int c = sym_index(nn, mm);
+            for (int l = 0; l < 3; l++) sacc += (E1[c][l] * sw - E2[c] * sg[l]) / sw2 * invB[l + 3 * mm];
         }
-        acc[nn] = mlast * s;
+        acc[nn] = mlast * sacc;
     }
 }
 
@@ -543,22 +666,20 @@ __device__ __forceinline__ void energy_terms(double (&vals)[RED_C], int nm, int 
     }
 }
 
-template <int MODE, bool CORR, bool DIAG>
+template <int MODE, int CORR, bool DIAG>
 __global__ __launch_bounds__(MCM_BLOCK) void k_momentum(DevState d, int cur) {
     const int i = blockIdx.x * MCM_BLOCK + threadIdx.x;
-    const int nxt = cur ^ 1;
     Scalars* sc = d.sc;
     const bool live = i < d.np && !sc->abort;
     double vals[RED_C];
     for (int k = 0; k < RED_C; k++) vals[k] = k < 3 ? 1.0e20 : (k < 6 ? -1.0e20 : 0.0);
     if (live) {
-        double xi[3] = {d.x1[0][i], d.x1[1][i], d.x1[2][i]};
-        double hi = d.hbuf[nxt][i];
-        double Ai[6];
-        for (int c = 0; c < 6; c++) Ai[c] = d.A[c][i];
+        const MRec me = load_mrec(d.rm, i);
+        const double xi[3] = {me.x, me.y, me.z};
         double acc[3];
-        if (CORR) accel_corr(d, i, nxt, xi, hi, Ai, acc);
-        else accel_plain(d, i, nxt, xi, hi, Ai, acc);
+        if (CORR == 1) accel_corr(d, i, me, acc);
+        else if (CORR == 2) accel_corr_fast(d, i, me, acc);
+        else accel_plain(d, i, me, acc);
         if (MODE == MODE_MOMENTUM) {
             for (int a = 0; a < 3; a++) d.a[a][i] = acc[a];
         } else {

@@ -11,7 +11,8 @@ module mcm_gpu_iface
     integer(c_int32_t) :: nbr_capacity = 0
     integer(c_int32_t) :: ghost_capacity = 0
     integer(c_int32_t) :: device = 0
-    integer(c_int32_t) :: reserved(5) = 0
+    integer(c_int32_t) :: corrected_one_pass = 0
+    integer(c_int32_t) :: reserved(4) = 0
   end type
 
   type, bind(c) :: mcm_gpu_scalars
