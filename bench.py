@@ -89,11 +89,24 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--corrected-one-pass", action="store_true",
                     help="accumulate the corrected kernels' three passes in one (rounding differs)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 5 device copies of MIB MiB (a known-size kernel for scaling the "
+                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs the SPH step on one GPU; --gpus %d is not supported "
               "(multi-GPU slab decomposition is design-only)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration << 18   # float32 words
+        src = torch.ones(n, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_mib": args.copy_calibration, "bytes_each_way": 4 * n}), flush=True)
+        return
     from mcmtop_amd import gpu
     tmpdir = tempfile.mkdtemp(prefix="mcmbench")
     deck_path = os.path.join(tmpdir, "cfg2.bin")

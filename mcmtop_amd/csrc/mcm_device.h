@@ -7,6 +7,17 @@
 #define MCM_MAX_MAT 8
 #define MCM_BLOCK 256
 
+// gfx950 dispatches the workgroups of a launch to its 8 XCDs round-robin
+// (workgroup b runs on XCD b % 8), and each XCD has its own L2.  Gather
+// kernels renumber their workgroups so that every XCD walks one contiguous
+// range of particles: particles that share neighbours then share the cached
+// neighbour records in one L2 instead of fetching them into all eight.
+__device__ __forceinline__ int xcd_block() {
+    const int b = blockIdx.x, nb = gridDim.x;
+    const int q = nb >> 3, r = nb & 7, x = b & 7;
+    return x * q + (x < r ? x : r) + (b >> 3);
+}
+
 // Grid of the cell search (mcm_setuplist.f90:30-91); cell indices 1-based
 // as in the reference, linear id = ((k-1)*lim1 + (j-1))*lim0 + (i-1).
 struct GridDev {

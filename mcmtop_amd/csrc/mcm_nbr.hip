@@ -335,7 +335,7 @@ __global__ __launch_bounds__(1024) void k_ghost_plane(DevState d, int pi, int cu
 // mcm_ll_neighbours.f90:96-299 for one real particle; lists are written in
 // [slot][particle] layout so that the pair kernels' loads coalesce.
 __global__ __launch_bounds__(256) void k_list(DevState d, int cur) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block() * blockDim.x + threadIdx.x;
     Scalars* sc = d.sc;
     if (i >= d.np || sc->abort) return;
     const GridDev g = sc->grid;

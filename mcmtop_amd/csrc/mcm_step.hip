@@ -14,10 +14,11 @@
 // neighbours; nothing is scattered, so results do not depend on scheduling.
 // Reductions use per-block partials reduced in a fixed order.
 //
-// The corrected (normalised) kernels accumulate the reference's three
-// neighbour passes in one pass: the kernel-gradient correction is linear in
-// the pass-1 sums, so B, the corrected gradient contraction and the
-// deltasig contraction are assembled from single-pass moments.
+// The corrected (normalised) kernels walk the reference's three neighbour
+// passes in its order (CORR 1).  A variant (CORR 2) assembles B, the
+// corrected gradient contraction and the deltasig contraction from
+// single-pass moments instead: the correction is linear in the pass-1 sums,
+// so it is the same mathematics with different rounding.
 #pragma once
 #include "mcm_nbr.hip"
 
@@ -253,7 +254,8 @@ __device__ __forceinline__ void strain_corr_fast(const DevState& d, int i, doubl
 // CORR: 0 plain SPH, 1 corrected (reference order), 2 corrected (one pass)
 template <int MODE, int CORR, bool DIAG>
 __global__ __launch_bounds__(MCM_BLOCK) void k_strain_const(DevState d, int cur) {
-    const int i = blockIdx.x * MCM_BLOCK + threadIdx.x;
+    const int lb = xcd_block();
+    const int i = lb * MCM_BLOCK + threadIdx.x;
     const int nxt = cur ^ 1;
     Scalars* sc = d.sc;
     const bool live = i < d.np && !sc->abort;
@@ -361,8 +363,8 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_strain_const(DevState d, int cur)
     double cb = block_reduce<1>(crit);
     double hb = block_reduce<2>(hm);
     if (threadIdx.x == 0) {
-        d.red[2 * blockIdx.x] = cb;
-        d.red[2 * blockIdx.x + 1] = hb;
+        d.red[2 * lb] = cb;
+        d.red[2 * lb + 1] = hb;
     }
     if (!last_block(&sc->done_b)) return;
     double c1 = fold_partials<1>(d.red, 2, 0, 1.0e20);
@@ -601,11 +603,11 @@ __device__ __forceinline__ void sym_pen(const DevState& d, double x[3], double v
 // energy and coordinate-bound partials: [cmin 3][cmax 3][ke nm][ie nm][mom 3nm]
 #define RED_C (6 + 5 * MCM_MAX_MAT)
 
-__device__ void reduce_c_partials(const DevState& d, const double (&vals)[RED_C], bool finalize_bounds) {
+__device__ void reduce_c_partials(const DevState& d, const double (&vals)[RED_C], bool finalize_bounds, int lb) {
     Scalars* sc = d.sc;
     const int nm = d.nummat;
     const int nv = 6 + 5 * nm;
-    double* out = d.red + (long)blockIdx.x * RED_C;
+    double* out = d.red + (long)lb * RED_C;
 #pragma unroll
     for (int k = 0; k < RED_C; k++) {
         if (k < nv) {
@@ -668,7 +670,8 @@ __device__ __forceinline__ void energy_terms(double (&vals)[RED_C], int nm, int 
 
 template <int MODE, int CORR, bool DIAG>
 __global__ __launch_bounds__(MCM_BLOCK) void k_momentum(DevState d, int cur) {
-    const int i = blockIdx.x * MCM_BLOCK + threadIdx.x;
+    const int lb = xcd_block();
+    const int i = lb * MCM_BLOCK + threadIdx.x;
     Scalars* sc = d.sc;
     const bool live = i < d.np && !sc->abort;
     double vals[RED_C];
@@ -714,7 +717,7 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_momentum(DevState d, int cur) {
                 for (int a = 0; a < 3; a++) d.a[a][i] = acc[a];
         }
     }
-    if (MODE == MODE_FUSED) reduce_c_partials(d, vals, true);
+    if (MODE == MODE_FUSED) reduce_c_partials(d, vals, true, lb);
 }
 
 // ---------------------------------------------- granular integration stages
@@ -736,7 +739,7 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_calctotale(DevState d) {
         double v[3] = {d.v[0][i], d.v[1][i], d.v[2][i]};
         energy_terms(vals, d.nummat, mi, mass, vabs, d.e[i], v);
     }
-    reduce_c_partials(d, vals, false);
+    reduce_c_partials(d, vals, false, blockIdx.x);
 }
 
 __global__ __launch_bounds__(MCM_BLOCK) void k_update_velocity(DevState d) {

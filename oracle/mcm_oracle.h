@@ -8,8 +8,15 @@
  * initialisation chain and every stage of one time step, keeping each
  * floating-point expression in the reference's evaluation order so that it
  * reproduces the flang build of the reference bit for bit on the same deck
- * (pinned by tests/test_oracle_vs_reference.py).  Each function cites the
+ * (pinned by tests/test_oracle_golden.py against SHA-256 digests of the
+ * reference's own state dumps, and by tests/test_oracle_closed_form.py
+ * against closed forms and brute force).  Each function cites the
  * reference routine it follows (MCM/<file>.f90:<lines>).
+ *
+ * Parity unpinned: branches restated here that no reference digest
+ * exercises -- f3dm10 spall type 3 and type-1 pcut clamping of a failed
+ * particle, the rho_max clamp of rhoupd, and drelax_scale in
+ * update_velocity (never set by the decks).
  *
  * Conventions: 0-based particle and ghost indices; 3x3 tensors stored
  * column-major, element (r,c) at [9*i + r + 3*c]; vectors at [3*i + d].
