@@ -39,7 +39,10 @@ typedef struct {
     int32_t device;         /* HIP device ordinal */
     int32_t corrected_one_pass; /* corrected SPH: accumulate the reference's three neighbour
                                    passes in one (same result up to rounding); 0 = reference order */
-    int32_t reserved[4];
+    int32_t pair_mapping;   /* list build and pair kernels of the fused step: 0 = one thread per
+                               particle, 1 = one wavefront per particle (lists up to 128
+                               entries, else 0); both give identical results */
+    int32_t reserved[3];
 } mcm_gpu_options;
 
 /* host-visible scalar state (mcm_database.f90:29-31,92-99 counterparts) */

@@ -39,7 +39,6 @@ struct Scalars {
     int error_flags;    // MCM error bits, see mcm_gpu.h
     int abort;          // set when a capacity was exceeded: later kernels of the step do nothing
     int timestep;
-    unsigned int done_b, done_c, done_r; // last-block counters (self-resetting)
 };
 
 // Boundary planes in the order xmin xmax ymin ymax zmin zmax
@@ -77,7 +76,7 @@ struct DevState {
     double* cand;              // [np][4] cell-sorted {x, y, z, h} of the candidates
     int* cand_id;              // [np][2] cell-sorted {particle id, material}
     int *gstart[3], *gcount[3];
-    int* nbr;                  // [ncap][npad]
+    int* nbr;                  // [npad/64][ncap][64]: entry k of particle i at ((i/64)*ncap + k)*64 + i%64
     int *nreal, *ntot;
     int* scan_tmp;
     // diagnostics / stage outputs
@@ -86,3 +85,11 @@ struct DevState {
     double* red;               // per-block partial reductions
     double* defps_prev;        // f3dm10 increment carried for failed particles
 };
+
+// Neighbour lists are blocked by 64 particles: for each block of 64
+// consecutive particles, slot k of all 64 is one contiguous 256-byte row, so
+// a wavefront of threads reading slot k of its particles reads one row.
+// Entry k of particle i is nbr_list(d, i)[k * 64].
+__device__ __forceinline__ int* nbr_list(const DevState& d, int i) {
+    return d.nbr + (long)(i >> 6) * d.ncap * 64 + (i & 63);
+}

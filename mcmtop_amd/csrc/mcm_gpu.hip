@@ -15,7 +15,7 @@
 #include <vector>
 
 #include "../../include/mcm_gpu.h"
-#include "mcm_step.hip"
+#include "mcm_wave.hip"
 
 namespace {
 
@@ -50,6 +50,7 @@ struct mcm_gpu_ctx {
     int phase = 0;                     // 0: step start, 1: after strain (x in x1, h/rho in next)
     int nb = 0;                        // blocks over np
     int corr_one_pass = 0;             // corrected SPH: one-pass moments instead of the reference's three passes
+    int pair_mapping = 0;              // 0 thread per particle, 1 wavefront per particle (lists up to 128)
     std::vector<double> xzero;
     std::vector<void*> allocs;
     std::string err;
@@ -289,7 +290,8 @@ void launch_neighbours(mcm_gpu_ctx* ctx, int cur) {
     k_scatter<<<ctx->nb, MCM_BLOCK, 0, s>>>(d);
     k_sort_cells<<<gcl, MCM_BLOCK, 0, s>>>(d, cur);
     for (int p = 0; p < d.nplanes; p++) k_ghost_plane<<<1, 1024, 0, s>>>(d, p, cur);
-    k_list<<<ctx->nb, MCM_BLOCK, 0, s>>>(d, cur);
+    if (ctx->pair_mapping == 1) k_list_wave<<<ctx->nb, MCM_BLOCK, 0, s>>>(d, cur);
+    else k_list<<<ctx->nb, MCM_BLOCK, 0, s>>>(d, cur);
 }
 
 int ghost_blocks(const DevState& d) { return std::max(1, std::min(blocks_for(d.ngcap, MCM_BLOCK), 1024)); }
@@ -297,23 +299,44 @@ int ghost_blocks(const DevState& d) { return std::max(1, std::min(blocks_for(d.n
 // 0 plain SPH, 1 corrected in the reference's pass order, 2 corrected in one pass
 int corr_variant(const mcm_gpu_ctx* ctx) { return ctx->d.disctype == 1 ? (ctx->corr_one_pass ? 2 : 1) : 0; }
 
+// the wavefront-per-particle pair kernels hold up to 2 x 64 list entries
+bool use_wave(const mcm_gpu_ctx* ctx) {
+    return ctx->pair_mapping == 1 && ctx->d.ncap <= 64 * WAVE_SLOTS && corr_variant(ctx) != 2;
+}
+
 void launch_strain_const(mcm_gpu_ctx* ctx, int cur) {
     DevState& d = ctx->d;
-    switch (corr_variant(ctx)) {
-        case 1: k_strain_const<MODE_FUSED, 1, false><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur); break;
-        case 2: k_strain_const<MODE_FUSED, 2, false><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur); break;
-        default: k_strain_const<MODE_FUSED, 0, false><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur); break;
+    if (use_wave(ctx)) {
+        if (corr_variant(ctx) == 1) k_strain_wave<1><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur);
+        else k_strain_wave<0><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur);
+        k_strain_const<MODE_FCONST, 0, false><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur);
+        k_strain_fold<MODE_FCONST><<<1, MCM_BLOCK, 0, ctx->stream>>>(d, ctx->nb);
+        return;
     }
+    switch (corr_variant(ctx)) {
+        case 1: k_strain_const<MODE_FSTRAIN, 1, false><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur); break;
+        case 2: k_strain_const<MODE_FSTRAIN, 2, false><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur); break;
+        default: k_strain_const<MODE_FSTRAIN, 0, false><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur); break;
+    }
+    k_strain_const<MODE_FCONST, 0, false><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur);
+    k_strain_fold<MODE_FCONST><<<1, MCM_BLOCK, 0, ctx->stream>>>(d, ctx->nb);
 }
 
 void launch_momentum(mcm_gpu_ctx* ctx, int cur) {
     DevState& d = ctx->d;
     if (d.boundary) k_ghost_stress<<<ghost_blocks(d), MCM_BLOCK, 0, ctx->stream>>>(d, cur);
+    if (use_wave(ctx)) {
+        if (corr_variant(ctx) == 1) k_momentum_wave<1><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur);
+        else k_momentum_wave<0><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur);
+        k_c_fold<<<1, MCM_BLOCK, 0, ctx->stream>>>(d, ctx->nb, 1);
+        return;
+    }
     switch (corr_variant(ctx)) {
         case 1: k_momentum<MODE_FUSED, 1, false><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur); break;
         case 2: k_momentum<MODE_FUSED, 2, false><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur); break;
         default: k_momentum<MODE_FUSED, 0, false><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur); break;
     }
+    k_c_fold<<<1, MCM_BLOCK, 0, ctx->stream>>>(d, ctx->nb, 1);
 }
 
 void launch_step(mcm_gpu_ctx* ctx, int cur) {
@@ -383,7 +406,6 @@ int regrow(mcm_gpu_ctx* ctx, const Scalars& sc_host) {
     Scalars s2 = sc_host;
     s2.abort = 0;
     s2.error_flags = 0;
-    s2.done_b = s2.done_c = s2.done_r = 0;
     return put_scalars_raw(ctx, s2);
 }
 
@@ -501,6 +523,7 @@ int initial(mcm_gpu_ctx* ctx) {
     rc = mcm_gpu_neighbours(ctx);
     if (rc) return rc;
     k_calccritts<<<ctx->nb, MCM_BLOCK, 0, st>>>(d, 0);
+    k_calccritts_fold<<<1, MCM_BLOCK, 0, st>>>(d, ctx->nb);
     Scalars s3;
     rc = get_scalars_raw(ctx, &s3);
     if (rc) return rc;
@@ -510,6 +533,7 @@ int initial(mcm_gpu_ctx* ctx) {
     rc = put_scalars_raw(ctx, s3);
     if (rc) return rc;
     k_calctotale<<<ctx->nb, MCM_BLOCK, 0, st>>>(d);
+    k_c_fold<<<1, MCM_BLOCK, 0, st>>>(d, ctx->nb, 0);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));
     return 0;
@@ -540,6 +564,8 @@ int mcm_gpu_create_from_deck(const char* deck_path, const mcm_gpu_options* opt, 
     *out = ctx;
     ctx->device = opt ? opt->device : 0;
     ctx->corr_one_pass = opt ? opt->corrected_one_pass : 0;
+    ctx->pair_mapping = opt ? opt->pair_mapping : 0;
+    if (ctx->pair_mapping < 0 || ctx->pair_mapping > 1) { ctx->err = "pair_mapping must be 0 or 1"; return MCM_GPU_ERR_ARG; }
     HIPCHK(hipSetDevice(ctx->device));
     HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     int rc = read_deck(ctx, deck_path);
@@ -654,7 +680,10 @@ int mcm_gpu_strain(mcm_gpu_ctx* ctx) {
 
 int mcm_gpu_update_h(mcm_gpu_ctx* ctx) {
     if (!ctx || ctx->phase != 1) return MCM_GPU_ERR_ARG;
-    if (ctx->d.h_opt > 0) k_update_h<<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(ctx->d, ctx->cur);
+    if (ctx->d.h_opt > 0) {
+        k_update_h<<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(ctx->d, ctx->cur);
+        k_update_h_fold<<<1, MCM_BLOCK, 0, ctx->stream>>>(ctx->d, ctx->nb);
+    }
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -662,6 +691,7 @@ int mcm_gpu_update_h(mcm_gpu_ctx* ctx) {
 int mcm_gpu_constitutive(mcm_gpu_ctx* ctx) {
     if (!ctx || ctx->phase != 1) return MCM_GPU_ERR_ARG;
     k_strain_const<MODE_CONST, 0, true><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(ctx->d, ctx->cur);
+    k_strain_fold<MODE_CONST><<<1, MCM_BLOCK, 0, ctx->stream>>>(ctx->d, ctx->nb);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -696,6 +726,7 @@ int mcm_gpu_boundcond(mcm_gpu_ctx* ctx) {
 int mcm_gpu_calctotale(mcm_gpu_ctx* ctx) {
     if (!ctx) return MCM_GPU_ERR_ARG;
     k_calctotale<<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(ctx->d);
+    k_c_fold<<<1, MCM_BLOCK, 0, ctx->stream>>>(ctx->d, ctx->nb, 0);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -710,6 +741,7 @@ int mcm_gpu_update_velocity(mcm_gpu_ctx* ctx) {
 int mcm_gpu_move(mcm_gpu_ctx* ctx) {
     if (!ctx || ctx->phase != 1) return MCM_GPU_ERR_ARG;
     k_move<<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(ctx->d);
+    k_move_fold<<<1, MCM_BLOCK, 0, ctx->stream>>>(ctx->d, ctx->nb);
     HIPCHK(hipGetLastError());
     ctx->cur ^= 1;
     ctx->phase = 0;
@@ -720,6 +752,7 @@ int mcm_gpu_calccritts(mcm_gpu_ctx* ctx, double* critts_out) {
     if (!ctx) return MCM_GPU_ERR_ARG;
     int hb = ctx->phase == 1 ? ctx->cur ^ 1 : ctx->cur;
     k_calccritts<<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(ctx->d, hb);
+    k_calccritts_fold<<<1, MCM_BLOCK, 0, ctx->stream>>>(ctx->d, ctx->nb);
     HIPCHK(hipGetLastError());
     Scalars s;
     int rc = get_scalars_raw(ctx, &s);
@@ -853,7 +886,7 @@ int mcm_gpu_download_neighbours(mcm_gpu_ctx* ctx, int32_t* counts, int32_t* gcou
     for (size_t i = 0; i < np; i++) {
         counts[i] = nr[i];
         gcounts[i] = nt[i] - nr[i];
-        for (int k = 0; k < nt[i] && k < d.ncap; k++) ids[i * d.ncap + k] = lst[(size_t)k * d.npad + i];
+        for (int k = 0; k < nt[i] && k < d.ncap; k++) ids[i * d.ncap + k] = lst[((i >> 6) * d.ncap + k) * 64 + (i & 63)];
     }
     return 0;
 }
@@ -941,13 +974,13 @@ int mcm_gpu_dump(mcm_gpu_ctx* ctx, const char* path, int32_t step, int32_t stage
     fwrite(llp.data(), 4, np, f);
     for (size_t i = 0; i < np; i++)
         for (int k = 0; k < nreal[i] && k < d.ncap; k++) {
-            int32_t v = lst[(size_t)k * d.npad + i] + 1;
+            int32_t v = lst[((i >> 6) * d.ncap + k) * 64 + (i & 63)] + 1;
             fwrite(&v, 4, 1, f);
         }
     if (d.boundary) {
         for (size_t i = 0; i < np; i++)
             for (int k = nreal[i]; k < ntot[i] && k < d.ncap; k++) {
-                int32_t v = lst[(size_t)k * d.npad + i] - (int32_t)np + 1;
+                int32_t v = lst[((i >> 6) * d.ncap + k) * 64 + (i & 63)] - (int32_t)np + 1;
                 fwrite(&v, 4, 1, f);
             }
         for (size_t g = 0; g < ng; g++) { int32_t v = gpar[g] + 1; fwrite(&v, 4, 1, f); }

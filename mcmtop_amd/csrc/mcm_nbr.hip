@@ -332,16 +332,16 @@ __global__ __launch_bounds__(1024) void k_ghost_plane(DevState d, int pi, int cu
 }
 
 // ------------------------------------------------------------ list build
-// mcm_ll_neighbours.f90:96-299 for one real particle; lists are written in
-// [slot][particle] layout so that the pair kernels' loads coalesce.
+// mcm_ll_neighbours.f90:96-299 for one real particle, one thread per
+// particle (list layout: nbr_list in mcm_device.h).
 __global__ __launch_bounds__(256) void k_list(DevState d, int cur) {
     int i = xcd_block() * blockDim.x + threadIdx.x;
     Scalars* sc = d.sc;
     if (i >= d.np || sc->abort) return;
     const GridDev g = sc->grid;
     const int L0 = g.lim[0], L1 = g.lim[1];
-    const int npad = d.npad, ncap = d.ncap;
-    int* out = d.nbr + i;
+    const int ncap = d.ncap;
+    int* out = nbr_list(d, i);
     int cnt = 0;
     double mind = 1.0e20, mind_cp = 1.0e20;
     const int c = d.cell_of[i];
@@ -374,11 +374,11 @@ __global__ __launch_bounds__(256) void k_list(DevState d, int cur) {
                         int st = d.cell_start[cc];
                         int en = ghosty ? st + d.cell_count[cc]
                                         : d.cell_start[c0 + nrun - 1] + d.cell_count[c0 + nrun - 1];
-                        for (int s = st; s < en; s++) {
-                            int2 idm = ci[s];
-                            int j = idm.x;
-                            if (j == i) continue;
-                            double2 pa = cd[2 * s], pb = cd[2 * s + 1];
+                        // candidates four at a time: the loads of a group are
+                        // issued together, the tests run in walk order
+                        auto consider = [&](int2 idm, double2 pa, double2 pb) {
+                            const int j = idm.x;
+                            if (j == i) return;
                             double dist_sq = 0.0;
                             double dx = pa.x - xi; dist_sq = dist_sq + dx * dx;
                             double dy = pa.y - yi; dist_sq = dist_sq + dy * dy;
@@ -388,20 +388,34 @@ __global__ __launch_bounds__(256) void k_list(DevState d, int cur) {
                                 bool same = idm.y == mi;
                                 if (pass == 0 && same) {
                                     mind = fmin(mind, dist_sq);
-                                    if (cnt < ncap) out[(long)cnt * npad] = j;
+                                    if (cnt < ncap) out[(long)cnt * 64] = j;
                                     cnt++;
                                 } else if (pass == 1 && !same) {
                                     mind_cp = fmin(mind_cp, dist_sq);
-                                    if (cnt < ncap) out[(long)cnt * npad] = j;
+                                    if (cnt < ncap) out[(long)cnt * 64] = j;
                                     cnt++;
                                 }
                             }
+                        };
+                        int s = st;
+                        for (; s + 4 <= en; s += 4) {
+                            int2 idm[4];
+                            double2 pa[4], pb[4];
+#pragma unroll
+                            for (int u = 0; u < 4; u++) {
+                                idm[u] = ci[s + u];
+                                pa[u] = cd[2 * (s + u)];
+                                pb[u] = cd[2 * (s + u) + 1];
+                            }
+#pragma unroll
+                            for (int u = 0; u < 4; u++) consider(idm[u], pa[u], pb[u]);
                         }
+                        for (; s < en; s++) consider(ci[s], cd[2 * s], cd[2 * s + 1]);
                     }
                 }
         }
         if (d.disctype == 1) {
-            if (cnt < ncap) out[(long)cnt * npad] = i;
+            if (cnt < ncap) out[(long)cnt * 64] = i;
             cnt++;
         }
         nreal = cnt;
@@ -426,11 +440,11 @@ __global__ __launch_bounds__(256) void k_list(DevState d, int cur) {
                                         bool same = d.pmat[u] == mi;
                                         if (pass == 0 && same) {
                                             mind = fmin(mind, dist_sq);
-                                            if (cnt < ncap) out[(long)cnt * npad] = u;
+                                            if (cnt < ncap) out[(long)cnt * 64] = u;
                                             cnt++;
                                         } else if (pass == 1 && !same) {
                                             mind_cp = fmin(mind_cp, dist_sq);
-                                            if (cnt < ncap) out[(long)cnt * npad] = u;
+                                            if (cnt < ncap) out[(long)cnt * 64] = u;
                                             cnt++;
                                         }
                                     }

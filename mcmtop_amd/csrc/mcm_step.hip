@@ -22,10 +22,15 @@
 #pragma once
 #include "mcm_nbr.hip"
 
-enum { MODE_FUSED = 0, MODE_STRAIN = 1, MODE_CONST = 2, MODE_MOMENTUM = 1 };
+// MODE_FUSED: whole stage in one kernel; MODE_STRAIN / MODE_CONST /
+// MODE_MOMENTUM: one reference subroutine per launch (granular entry
+// points); MODE_FSTRAIN + MODE_FCONST: the fused step's strain gather, which
+// stages rod, spin and tracerod, then update_h + constitutive + dt in a
+// kernel of its own, so that the constitutive code's register demand does
+// not limit the occupancy of the gathers.
+enum { MODE_FUSED = 0, MODE_STRAIN = 1, MODE_CONST = 2, MODE_MOMENTUM = 1, MODE_FCONST = 3, MODE_FSTRAIN = 4 };
 
 // ------------------------------------------------------------- reductions
-__device__ __forceinline__ double vload(const double* p) { return *(volatile const double*)p; }
 __device__ __forceinline__ double wave_min(double v) {
     for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
     return v;
@@ -56,31 +61,21 @@ __device__ __forceinline__ double block_reduce(double v) {
     return r;
 }
 
-// The last block of a grid folds the per-block partials red[b*stride + k]
-// of all blocks: thread t walks blocks t, t+BLOCK, ... in order, then the
-// block reduction combines the threads in a fixed order.
+// Final fold of per-block partials red[b*stride + k], b < nblk, by one
+// workgroup launched after the kernel that wrote them (the kernel boundary
+// makes every block's partials visible; no device-wide fence per block).
+// Thread t combines blocks t, t + BLOCK, ... in order, then the block
+// reduction combines the threads in a fixed order.  Every thread calls it.
 template <int OP>
-__device__ __forceinline__ double fold_partials(const double* red, int stride, int k, double init) {
+__device__ __forceinline__ double fold(const double* red, int stride, int k, int nblk, double init) {
     double v = init;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += MCM_BLOCK) {
-        double x = vload(&red[(long)b * stride + k]);
+#pragma unroll 4
+    for (int b = threadIdx.x; b < nblk; b += MCM_BLOCK) {
+        const double x = red[(long)b * stride + k];
         v = OP == 0 ? v + x : (OP == 1 ? fmin(v, x) : fmax(v, x));
     }
     return block_reduce<OP>(v);
 }
-
-__device__ __forceinline__ bool last_block(unsigned int* counter) {
-    __shared__ int s_last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned int t = atomicAdd(counter, 1u);
-        s_last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    return s_last != 0;
-}
-
 
 // ----------------------------------------------------------- state I/O
 __device__ __forceinline__ void load_state(const DevState& d, int i, int cur, PState& s) {
@@ -123,17 +118,49 @@ __device__ __forceinline__ MRec load_mrec(const double* rm, int j) {
     return MRec{a.x, a.y, b.x, b.y, c.x, c.y, {e.x, e.y, f.x, f.y, g.x, g.y}};
 }
 
+template <typename Rec> __device__ __forceinline__ Rec load_rec(const double* base, int j);
+template <> __device__ __forceinline__ SRec load_rec<SRec>(const double* base, int j) { return load_srec(base, j); }
+template <> __device__ __forceinline__ MRec load_rec<MRec>(const double* base, int j) { return load_mrec(base, j); }
+
+// Walks one particle's list in order with the next entry's record loaded
+// one entry ahead and its index two entries ahead, so that the gathers of a
+// thread overlap its pair arithmetic.  Past the end it re-reads the
+// particle's own record (in bounds, never used).
+template <typename Rec>
+struct NbrStream {
+    const int* lst;
+    const double* base;
+    int n, k, self, jn, jnn;
+    Rec rn;
+    __device__ __forceinline__ NbrStream(const int* l, const double* b, int n_, int self_)
+        : lst(l), base(b), n(n_), k(0), self(self_) {
+        jn = n > 0 ? lst[0] : self;
+        rn = load_rec<Rec>(base, jn);
+        jnn = n > 1 ? lst[64] : self;
+    }
+    __device__ __forceinline__ void next(int& j, Rec& r) {
+        j = jn;
+        r = rn;
+        jn = jnn;
+        rn = load_rec<Rec>(base, jn);
+        jnn = k + 2 < n ? lst[(long)(k + 2) * 64] : self;
+        k++;
+    }
+};
+
 // plain strain rate, mcm_oldstrain.f90:34-64, positions half a step back
 // for real particles only (mcm_strain.f90:27-31)
 __device__ __forceinline__ void strain_plain(const DevState& d, int i, double dt, const double xi[3],
                                              const double vi[3], double hi, double g[3][3]) {
     const int n = d.ntot[i];
-    const int* lst = d.nbr + i;
+    const int* lst = nbr_list(d, i);
     for (int a = 0; a < 3; a++)
         for (int b = 0; b < 3; b++) g[a][b] = 0.0;
+    NbrStream<SRec> st(lst, d.rs, n, i);
     for (int k = 0; k < n; k++) {
-        int j = lst[(long)k * d.npad];
-        SRec r = load_srec(d.rs, j);
+        int j;
+        SRec r;
+        st.next(j, r);
         double sh = j < d.np ? dt : 0.0;
         double xj0 = r.x - 0.5 * r.vx * sh;
         double xj1 = r.y - 0.5 * r.vy * sh;
@@ -173,11 +200,13 @@ __device__ __forceinline__ void corr_pair(const SRec& r, double sh, const double
 __device__ __forceinline__ void strain_corr(const DevState& d, int i, double dt, const double xi[3], double hi,
                                             double f[3][3]) {
     const int n = d.ntot[i];
-    const int* lst = d.nbr + i;
+    const int* lst = nbr_list(d, i);
     double sw = 0.0, sg[3] = {0.0, 0.0, 0.0};
+    NbrStream<SRec> s1(lst, d.rs, n, i);
     for (int k = 0; k < n; k++) {
-        int j = lst[(long)k * d.npad];
-        SRec r = load_srec(d.rs, j);
+        int j;
+        SRec r;
+        s1.next(j, r);
         double xj[3], w, dwdx[3];
         corr_pair(r, j < d.np ? dt : 0.0, xi, hi, d.pi, xj, w, dwdx);
         sw = sw + r.vol * w;
@@ -185,9 +214,11 @@ __device__ __forceinline__ void strain_corr(const DevState& d, int i, double dt,
     }
     const double sw2 = sw * sw;
     double B[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, invB[9];
+    NbrStream<SRec> s2(lst, d.rs, n, i);
     for (int k = 0; k < n; k++) {
-        int j = lst[(long)k * d.npad];
-        SRec r = load_srec(d.rs, j);
+        int j;
+        SRec r;
+        s2.next(j, r);
         double xj[3], w, dwdx[3], cg[3];
         corr_pair(r, j < d.np ? dt : 0.0, xi, hi, d.pi, xj, w, dwdx);
         for (int m = 0; m < 3; m++) cg[m] = (dwdx[m] * sw - w * sg[m]) / sw2;
@@ -197,9 +228,11 @@ __device__ __forceinline__ void strain_corr(const DevState& d, int i, double dt,
     inverse3(B, invB);
     for (int a = 0; a < 3; a++)
         for (int b = 0; b < 3; b++) f[a][b] = 0.0;
+    NbrStream<SRec> s3(lst, d.rs, n, i);
     for (int k = 0; k < n; k++) {
-        int j = lst[(long)k * d.npad];
-        SRec r = load_srec(d.rs, j);
+        int j;
+        SRec r;
+        s3.next(j, r);
         double xj[3], w, dwdx[3], cg[3], g[3];
         corr_pair(r, j < d.np ? dt : 0.0, xi, hi, d.pi, xj, w, dwdx);
         for (int m = 0; m < 3; m++) cg[m] = (dwdx[m] * sw - w * sg[m]) / sw2;
@@ -216,11 +249,13 @@ __device__ __forceinline__ void strain_corr(const DevState& d, int i, double dt,
 __device__ __forceinline__ void strain_corr_fast(const DevState& d, int i, double dt, const double xi[3],
                                                  double hi, double f[3][3]) {
     const int n = d.ntot[i];
-    const int* lst = d.nbr + i;
+    const int* lst = nbr_list(d, i);
     double sw = 0.0, sg[3] = {0, 0, 0}, P[3][3] = {{0}}, Pw[3] = {0, 0, 0}, Q[3][3] = {{0}}, Qw[3] = {0, 0, 0};
+    NbrStream<SRec> st(lst, d.rs, n, i);
     for (int k = 0; k < n; k++) {
-        int j = lst[(long)k * d.npad];
-        SRec r = load_srec(d.rs, j);
+        int j;
+        SRec r;
+        st.next(j, r);
         double xj[3], w, dwdx[3];
         corr_pair(r, j < d.np ? dt : 0.0, xi, hi, d.pi, xj, w, dwdx);
         double vw = r.vol * w;
@@ -251,7 +286,115 @@ __device__ __forceinline__ void strain_corr_fast(const DevState& d, int i, doubl
 }
 
 // ------------------------------------------------------- k_strain_const
-// CORR: 0 plain SPH, 1 corrected (reference order), 2 corrected (one pass)
+// After the strain pass of particle i: the reference moves x back by the
+// same half step, (x - t) + t (mcm_strain.f90:51-58); x1 and the momentum
+// record's position are written.  Returns x1z.
+__device__ __forceinline__ double strain_finish(const DevState& d, int i, double dt, const double xi[3],
+                                                const double vi[3]) {
+    double x1[3];
+    for (int a = 0; a < 3; a++) {
+        x1[a] = xi[a] + 0.5 * vi[a] * dt;
+        d.x1[a][i] = x1[a];
+    }
+    double2* rm = reinterpret_cast<double2*>(d.rm) + 6 * (long)i;
+    rm[0] = make_double2(x1[0], x1[1]);
+    return x1[2];
+}
+
+// mcm_update_h + mcm_constitutive for particle i with the strain rate in s
+// (rod, spin, tracerod); MODE_CONST reads the state staged by the strain and
+// update_h stages.  Returns the particle's critical timestep, the updated h
+// (0 when not updated) and sets eos_err.
+template <int MODE, bool DIAG>
+__device__ __forceinline__ void constitutive_particle(const DevState& d, int i, int cur, double dt, PState& s,
+                                                      double x1z, double& crit, double& hm, int& eos_err) {
+    const int nxt = cur ^ 1;
+    load_state(d, i, MODE == MODE_CONST ? nxt : cur, s);
+    const int mi = d.pmat[i];
+    const MatParams& m = d.mat[mi];
+    if ((MODE == MODE_FUSED || MODE == MODE_FCONST) && d.h_opt > 0) {
+        if (update_h(s, d.h_opt, dt)) hm = s.h;
+    }
+    rhoupd(s, m, dt);
+    if (m.model == 3) for (int k = 0; k < 6; k++) s.alfa[k] = d.alfa[k][i];
+    if (m.model == 9) for (int k = 0; k < 6; k++) s.s[k] = d.sd[k][i];
+    double defps_prev = d.defps_prev[i];
+    int active = constitutive_one(s, m, mi + 1, dt, defps_prev, &eos_err);
+    s.critts = critts(s, m, d.tcrit_opt, d.mindist[i]);
+    crit = s.critts;
+    // write back
+    d.rbuf[nxt][i] = s.rho;
+    d.hbuf[nxt][i] = s.h;
+    if (d.h_opt > 0) d.hold[i] = s.hold;
+    for (int k = 0; k < 6; k++) d.sig[k][i] = s.sig[k];
+    if (m.model == 3) for (int k = 0; k < 6; k++) d.alfa[k][i] = s.alfa[k];
+    if (m.model == 9 || (DIAG && m.model == 10)) for (int k = 0; k < 6; k++) d.sd[k][i] = s.s[k];
+    if (DIAG && m.model == 3) for (int k = 0; k < 6; k++) d.sd[k][i] = s.s[k];
+    d.p[i] = s.p;
+    d.e[i] = s.e;
+    d.q[i] = s.q;
+    d.efps[i] = s.efps;
+    if (m.model == 9 || m.model == 10) {
+        d.c[i] = s.c;
+        d.pcut[i] = s.pcut;
+        d.p_cut[i] = s.p_cut;
+        d.fail[i] = s.fail;
+        d.defps_prev[i] = defps_prev;
+    }
+    // momentum record: x after the strain pass, new h, mass, volume and
+    // (sigma - q)/rho^2 (mcm_oldacceleration.f90:79-80, per particle)
+    double r2 = s.rho * s.rho;
+    double2* rm = reinterpret_cast<double2*>(d.rm) + 6 * (long)i;
+    rm[1] = make_double2(x1z, s.h);
+    rm[2] = make_double2(s.mass, s.mass / s.rho);
+    rm[3] = make_double2((s.sig[S11] - s.q) / r2, (s.sig[S22] - s.q) / r2);
+    rm[4] = make_double2((s.sig[S33] - s.q) / r2, (s.sig[S12] - 0.0) / r2);
+    rm[5] = make_double2((s.sig[S13] - 0.0) / r2, (s.sig[S23] - 0.0) / r2);
+    if (DIAG) {
+        for (int k = 0; k < 6; k++) d.rod[k][i] = s.rod[k];
+        for (int k = 0; k < 3; k++) d.spin[k][i] = s.spin[k];
+        d.tracerod[i] = s.tracerod;
+        d.einc[i] = s.einc;
+        d.etry[i] = s.etry;
+        d.rhoold[i] = s.rhoold;
+        d.critts_p[i] = s.critts;
+        d.yield[i] = active;
+    }
+}
+
+// Block min of critts and max of the updated h into the block's partials
+// (folded by k_strain_fold).  Every thread of the block calls it.
+__device__ __forceinline__ void strain_block_partials(const DevState& d, int lb, double crit, double hm, int eos_err) {
+    if (eos_err) atomicOr(&d.sc->error_flags, 16);
+    double cb = block_reduce<1>(crit);
+    double hb = block_reduce<2>(hm);
+    if (threadIdx.x == 0) {
+        d.red[2 * lb] = cb;
+        d.red[2 * lb + 1] = hb;
+    }
+}
+
+// critts and hmax from the partials of nblk blocks; in the fused step dt and
+// ptime advance (mcm_solution.f90:58-62).  One workgroup.
+template <int MODE>
+__global__ __launch_bounds__(MCM_BLOCK) void k_strain_fold(DevState d, int nblk) {
+    Scalars* sc = d.sc;
+    double c1 = fold<1>(d.red, 2, 0, nblk, 1.0e20);
+    double h1 = fold<2>(d.red, 2, 1, nblk, 0.0);
+    if (threadIdx.x == 0 && !sc->abort) {
+        sc->critts = c1;
+        if ((MODE == MODE_FUSED || MODE == MODE_FCONST) && d.h_opt > 0) sc->hmax = h1;
+        if (MODE == MODE_FUSED || MODE == MODE_FCONST) {
+            sc->dtold = sc->dt;
+            sc->dt = d.tssf * sc->critts;
+            sc->dt = fmin(sc->dt, 1.1 * sc->dtold);
+            sc->ptime = sc->ptime + sc->dt;
+        }
+    }
+}
+
+// Thread-per-particle strain (+ constitutive) kernel.  CORR: 0 plain SPH,
+// 1 corrected (reference order), 2 corrected (one pass).
 template <int MODE, int CORR, bool DIAG>
 __global__ __launch_bounds__(MCM_BLOCK) void k_strain_const(DevState d, int cur) {
     const int lb = xcd_block();
@@ -264,8 +407,8 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_strain_const(DevState d, int cur)
     int eos_err = 0;
     if (live) {
         PState s;
-        double x1z_keep = 0.0;
-        if (MODE != MODE_CONST) {
+        double x1z = 0.0;
+        if (MODE == MODE_FUSED || MODE == MODE_STRAIN || MODE == MODE_FSTRAIN) {
             double xr[3], vi[3], xi[3];
             for (int a = 0; a < 3; a++) {
                 xr[a] = d.x0[a][i];
@@ -278,110 +421,32 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_strain_const(DevState d, int cur)
             else if (CORR == 2) strain_corr_fast(d, i, dt, xi, hi, g);
             else strain_plain(d, i, dt, xi, vi, hi, g);
             rod_spin(g, s);
-            // the reference moves x back by the same half step: (x - t) + t
-            double x1[3];
-            for (int a = 0; a < 3; a++) {
-                x1[a] = xi[a] + 0.5 * vi[a] * dt;
-                d.x1[a][i] = x1[a];
-            }
-            x1z_keep = x1[2];
-            double2* rm = reinterpret_cast<double2*>(d.rm) + 6 * (long)i;
-            rm[0] = make_double2(x1[0], x1[1]);
-            if (MODE == MODE_STRAIN) rm[1] = make_double2(x1[2], hi);
+            x1z = strain_finish(d, i, dt, xi, vi);
             if (MODE == MODE_STRAIN) {
+                double2* rm = reinterpret_cast<double2*>(d.rm) + 6 * (long)i;
+                rm[1] = make_double2(x1z, hi);
                 for (int k = 0; k < 6; k++) d.rod[k][i] = s.rod[k];
                 for (int k = 0; k < 3; k++) d.spin[k][i] = s.spin[k];
                 d.tracerod[i] = s.tracerod;
                 d.hbuf[nxt][i] = hi;
                 d.rbuf[nxt][i] = d.rbuf[cur][i];
             }
+            if (MODE == MODE_FSTRAIN) {
+                for (int k = 0; k < 6; k++) d.rod[k][i] = s.rod[k];
+                for (int k = 0; k < 3; k++) d.spin[k][i] = s.spin[k];
+                d.tracerod[i] = s.tracerod;
+            }
         } else {
             for (int k = 0; k < 6; k++) s.rod[k] = d.rod[k][i];
             for (int k = 0; k < 3; k++) s.spin[k] = d.spin[k][i];
             s.tracerod = d.tracerod[i];
+            x1z = d.x1[2][i];
         }
-        if (MODE != MODE_STRAIN) {
-            // granular mode: h and rho were staged in the next buffers by the
-            // strain and update_h stages
-            load_state(d, i, MODE == MODE_CONST ? nxt : cur, s);
-            const int mi = d.pmat[i];
-            const MatParams& m = d.mat[mi];
-            if (MODE == MODE_FUSED && d.h_opt > 0) {
-                if (update_h(s, d.h_opt, dt)) hm = s.h;
-            }
-            rhoupd(s, m, dt);
-            if (m.model == 3) for (int k = 0; k < 6; k++) s.alfa[k] = d.alfa[k][i];
-            if (m.model == 9) for (int k = 0; k < 6; k++) s.s[k] = d.sd[k][i];
-            double defps_prev = d.defps_prev[i];
-            int active = constitutive_one(s, m, mi + 1, dt, defps_prev, &eos_err);
-            s.critts = critts(s, m, d.tcrit_opt, d.mindist[i]);
-            crit = s.critts;
-            // write back
-            d.rbuf[nxt][i] = s.rho;
-            d.hbuf[nxt][i] = s.h;
-            if (d.h_opt > 0) d.hold[i] = s.hold;
-            for (int k = 0; k < 6; k++) d.sig[k][i] = s.sig[k];
-            if (m.model == 3) for (int k = 0; k < 6; k++) d.alfa[k][i] = s.alfa[k];
-            if (m.model == 9 || (DIAG && m.model == 10)) for (int k = 0; k < 6; k++) d.sd[k][i] = s.s[k];
-            if (DIAG && m.model == 3) for (int k = 0; k < 6; k++) d.sd[k][i] = s.s[k];
-            d.p[i] = s.p;
-            d.e[i] = s.e;
-            d.q[i] = s.q;
-            d.efps[i] = s.efps;
-            if (m.model == 9 || m.model == 10) {
-                d.c[i] = s.c;
-                d.pcut[i] = s.pcut;
-                d.p_cut[i] = s.p_cut;
-                d.fail[i] = s.fail;
-                d.defps_prev[i] = defps_prev;
-            }
-            // momentum record: x after the strain pass, new h, mass, volume and
-            // (sigma - q)/rho^2 (mcm_oldacceleration.f90:79-80, per particle)
-            double r2 = s.rho * s.rho;
-            double2* rm = reinterpret_cast<double2*>(d.rm) + 6 * (long)i;
-            double x1z = MODE == MODE_CONST ? d.x1[2][i] : x1z_keep;
-            rm[1] = make_double2(x1z, s.h);
-            rm[2] = make_double2(s.mass, s.mass / s.rho);
-            rm[3] = make_double2((s.sig[S11] - s.q) / r2, (s.sig[S22] - s.q) / r2);
-            rm[4] = make_double2((s.sig[S33] - s.q) / r2, (s.sig[S12] - 0.0) / r2);
-            rm[5] = make_double2((s.sig[S13] - 0.0) / r2, (s.sig[S23] - 0.0) / r2);
-            if (DIAG) {
-                for (int k = 0; k < 6; k++) d.rod[k][i] = s.rod[k];
-                for (int k = 0; k < 3; k++) d.spin[k][i] = s.spin[k];
-                d.tracerod[i] = s.tracerod;
-                d.einc[i] = s.einc;
-                d.etry[i] = s.etry;
-                d.rhoold[i] = s.rhoold;
-                d.critts_p[i] = s.critts;
-                d.yield[i] = active;
-            }
-        }
+        if (MODE != MODE_STRAIN && MODE != MODE_FSTRAIN)
+            constitutive_particle<MODE, DIAG>(d, i, cur, dt, s, x1z, crit, hm, eos_err);
     }
-    if (MODE == MODE_STRAIN) return;
-    if (eos_err) atomicOr(&sc->error_flags, 16);
-    // block min of critts, max of updated h -> partials -> last block
-    double cb = block_reduce<1>(crit);
-    double hb = block_reduce<2>(hm);
-    if (threadIdx.x == 0) {
-        d.red[2 * lb] = cb;
-        d.red[2 * lb + 1] = hb;
-    }
-    if (!last_block(&sc->done_b)) return;
-    double c1 = fold_partials<1>(d.red, 2, 0, 1.0e20);
-    double h1 = fold_partials<2>(d.red, 2, 1, 0.0);
-    if (threadIdx.x == 0) {
-        if (!sc->abort) {
-            sc->critts = c1;
-            if (MODE == MODE_FUSED && d.h_opt > 0) sc->hmax = h1;
-            if (MODE == MODE_FUSED) {   // mcm_solution.f90:58-62
-                sc->dtold = sc->dt;
-                sc->dt = d.tssf * sc->critts;
-                sc->dt = fmin(sc->dt, 1.1 * sc->dtold);
-                sc->ptime = sc->ptime + sc->dt;
-            }
-        }
-        sc->done_b = 0;
-    }
+    if (MODE == MODE_STRAIN || MODE == MODE_FSTRAIN) return;
+    strain_block_partials(d, lb, crit, hm, eos_err);
 }
 
 // mcm_update_h.f90:25-54 as a stage of its own (granular path)
@@ -400,12 +465,11 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_update_h(DevState d, int cur) {
     }
     double hb = block_reduce<2>(hm);
     if (threadIdx.x == 0) d.red[blockIdx.x] = hb;
-    if (!last_block(&sc->done_r)) return;
-    double h1 = fold_partials<2>(d.red, 1, 0, 0.0);
-    if (threadIdx.x == 0) {
-        sc->hmax = h1;
-        sc->done_r = 0;
-    }
+}
+
+__global__ __launch_bounds__(MCM_BLOCK) void k_update_h_fold(DevState d, int nblk) {
+    double h1 = fold<2>(d.red, 1, 0, nblk, 0.0);
+    if (threadIdx.x == 0) d.sc->hmax = h1;
 }
 
 // dt/ptime update of mcm_solution.f90:58-62 (granular path)
@@ -445,11 +509,13 @@ __device__ __forceinline__ int sym_index(int r, int c) {
 // mcm_oldacceleration.f90:60-89
 __device__ __forceinline__ void accel_plain(const DevState& d, int i, const MRec& me, double acc[3]) {
     const int n = d.ntot[i];
-    const int* lst = d.nbr + i;
+    const int* lst = nbr_list(d, i);
     acc[0] = 0.0; acc[1] = 0.0; acc[2] = 0.0;
+    NbrStream<MRec> st(lst, d.rm, n, i);
     for (int k = 0; k < n; k++) {
-        int j = lst[(long)k * d.npad];
-        MRec r = load_mrec(d.rm, j);
+        int j;
+        MRec r;
+        st.next(j, r);
         double havg = 0.5 * (me.h + r.h);
         double dwdx[3], rad;
         kernel_grad(me.x - r.x, me.y - r.y, me.z - r.z, havg, d.pi, dwdx, &rad);
@@ -479,10 +545,13 @@ __device__ __forceinline__ void corr_pair_j(const MRec& r, const MRec& me, doubl
 __device__ __forceinline__ void accel_corr(const DevState& d, int i, const MRec& me, double acc[3]) {
     const int n = d.ntot[i];
     const int self_k = d.nreal[i] - 1;
-    const int* lst = d.nbr + i;
+    const int* lst = nbr_list(d, i);
     double sw = 0.0, sg[3] = {0.0, 0.0, 0.0}, massj = 0.0;
+    int j;
+    NbrStream<MRec> s1(lst, d.rm, n, i);
     for (int k = 0; k < n; k++) {
-        MRec r = load_mrec(d.rm, lst[(long)k * d.npad]);
+        MRec r;
+        s1.next(j, r);
         double w, dwdx[3];
         corr_pair_j(r, me, d.pi, w, dwdx);
         sw = sw + r.vol * w;
@@ -491,8 +560,10 @@ __device__ __forceinline__ void accel_corr(const DevState& d, int i, const MRec&
     }
     const double sw2 = sw * sw;
     double B[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, invB[9];
+    NbrStream<MRec> s2(lst, d.rm, n, i);
     for (int k = 0; k < n; k++) {
-        MRec r = load_mrec(d.rm, lst[(long)k * d.npad]);
+        MRec r;
+        s2.next(j, r);
         double w, dwdx[3], cg[3];
         corr_pair_j(r, me, d.pi, w, dwdx);
         for (int m = 0; m < 3; m++) cg[m] = (dwdx[m] * sw - w * sg[m]) / sw2;
@@ -503,9 +574,11 @@ __device__ __forceinline__ void accel_corr(const DevState& d, int i, const MRec&
     inverse3(B, invB);
     // deltasig(m,n,l) is symmetric in (m,n): 6 x 3 accumulators
     double D[6][3] = {{0.0}};
+    NbrStream<MRec> s3(lst, d.rm, n, i);
     for (int k = 0; k < n; k++) {
+        MRec r;
+        s3.next(j, r);
         if (k == self_k) continue;   // dsig = 0 there
-        MRec r = load_mrec(d.rm, lst[(long)k * d.npad]);
         double w, dwdx[3], cg[3];
         corr_pair_j(r, me, d.pi, w, dwdx);
         for (int m = 0; m < 3; m++) cg[m] = (dwdx[m] * sw - w * sg[m]) / sw2;
@@ -530,12 +603,15 @@ __device__ __forceinline__ void accel_corr(const DevState& d, int i, const MRec&
 __device__ __forceinline__ void accel_corr_fast(const DevState& d, int i, const MRec& me, double acc[3]) {
     const int n = d.ntot[i];
     const int self_k = d.nreal[i] - 1;
-    const int* lst = d.nbr + i;
+    const int* lst = nbr_list(d, i);
     double sw = 0.0, sg[3] = {0, 0, 0}, P[3][3] = {{0}}, Pw[3] = {0, 0, 0};
     double E1[6][3] = {{0}}, E2[6] = {0, 0, 0, 0, 0, 0};
     double mlast = 0.0;
+    int j;
+    NbrStream<MRec> st(lst, d.rm, n, i);
     for (int k = 0; k < n; k++) {
-        MRec r = load_mrec(d.rm, lst[(long)k * d.npad]);
+        MRec r;
+        st.next(j, r);
         double w, dwdx[3];
         corr_pair_j(r, me, d.pi, w, dwdx);
         double dx[3] = {r.x - me.x, r.y - me.y, r.z - me.z};
@@ -603,79 +679,127 @@ __device__ __forceinline__ void sym_pen(const DevState& d, double x[3], double v
 // energy and coordinate-bound partials: [cmin 3][cmax 3][ke nm][ie nm][mom 3nm]
 #define RED_C (6 + 5 * MCM_MAX_MAT)
 
-__device__ void reduce_c_partials(const DevState& d, const double (&vals)[RED_C], bool finalize_bounds, int lb) {
+// One thread's share of the step's end-of-step reductions: its particle's
+// new position (coordinate bounds) and its terms of mcm_calctotale.f90:33-51
+// for its material mi (mi < 0: no particle).
+struct CPart {
+    double x[3];
+    double ke, ie, mom[3];
+    int mi;
+};
+
+__device__ __forceinline__ void init_c_part(CPart& c) {
+    for (int a = 0; a < 3; a++) { c.x[a] = 0.0; c.mom[a] = 0.0; }
+    c.ke = 0.0; c.ie = 0.0; c.mi = -1;
+}
+
+__device__ __forceinline__ void energy_terms(CPart& c, int mi, double mass, double vabs, double e, const double v[3]) {
+    c.mi = mi;
+    c.ke = 0.5 * mass * (vabs * vabs);
+    c.ie = e;
+    for (int a = 0; a < 3; a++) c.mom[a] = mass * fabs(v[a]);
+}
+
+// Block partials [cmin 3][cmax 3][ke nm][ie nm][mom 3nm] of every block
+// (folded by k_c_fold).  Every thread of the block calls it.
+__device__ void reduce_c_partials(const DevState& d, const CPart& c, bool with_bounds, int lb) {
+    const int nm = d.nummat;
+    double* out = d.red + (long)lb * RED_C;
+    const bool has = c.mi >= 0;
+    for (int a = 0; a < 3; a++) {
+        double lo = block_reduce<1>(has && with_bounds ? c.x[a] : 1.0e20);
+        double hi = block_reduce<2>(has && with_bounds ? c.x[a] : -1.0e20);
+        if (threadIdx.x == 0) { out[a] = lo; out[3 + a] = hi; }
+    }
+    for (int m = 0; m < nm; m++) {
+        const bool mine = c.mi == m;
+        double ke = block_reduce<0>(mine ? c.ke : 0.0);
+        double ie = block_reduce<0>(mine ? c.ie : 0.0);
+        if (threadIdx.x == 0) { out[6 + m] = ke; out[6 + nm + m] = ie; }
+        for (int a = 0; a < 3; a++) {
+            double mo = block_reduce<0>(mine ? c.mom[a] : 0.0);
+            if (threadIdx.x == 0) out[6 + 2 * nm + 3 * m + a] = mo;
+        }
+    }
+}
+
+// Coordinate bounds (with_bounds: the step's move), per-material energies
+// and their totals from the partials of nblk blocks.  One workgroup.
+__global__ __launch_bounds__(MCM_BLOCK) void k_c_fold(DevState d, int nblk, int with_bounds) {
     Scalars* sc = d.sc;
     const int nm = d.nummat;
-    const int nv = 6 + 5 * nm;
-    double* out = d.red + (long)lb * RED_C;
-#pragma unroll
-    for (int k = 0; k < RED_C; k++) {
-        if (k < nv) {
-            double r = k < 3 ? block_reduce<1>(vals[k]) : (k < 6 ? block_reduce<2>(vals[k]) : block_reduce<0>(vals[k]));
-            if (threadIdx.x == 0) out[k] = r;
+    for (int a = 0; a < 3; a++) {
+        double lo = fold<1>(d.red, RED_C, a, nblk, 1.0e20);
+        double hi = fold<2>(d.red, RED_C, 3 + a, nblk, -1.0e20);
+        if (threadIdx.x == 0 && with_bounds && !sc->abort) { sc->coord_min[a] = lo; sc->coord_max[a] = hi; }
+    }
+    double ke = 0.0, ie = 0.0;
+    for (int m = 0; m < nm; m++) {
+        double mk = fold<0>(d.red, RED_C, 6 + m, nblk, 0.0);
+        double mie = fold<0>(d.red, RED_C, 6 + nm + m, nblk, 0.0);
+        ke = ke + mk;
+        ie = ie + mie;
+        if (threadIdx.x == 0 && !sc->abort) { sc->mat_ke[m] = mk; sc->mat_ie[m] = mie; }
+        for (int a = 0; a < 3; a++) {
+            double mo = fold<0>(d.red, RED_C, 6 + 2 * nm + 3 * m + a, nblk, 0.0);
+            if (threadIdx.x == 0 && !sc->abort) sc->mat_mom[m][a] = mo;
         }
     }
-    if (!last_block(&sc->done_c)) return;
-    double tot[RED_C];
-#pragma unroll
-    for (int k = 0; k < RED_C; k++) {
-        tot[k] = 0.0;
-        if (k < nv)
-            tot[k] = k < 3 ? fold_partials<1>(d.red, RED_C, k, 1.0e20)
-                           : (k < 6 ? fold_partials<2>(d.red, RED_C, k, -1.0e20) : fold_partials<0>(d.red, RED_C, k, 0.0));
-    }
-    if (threadIdx.x == 0) {
-        if (finalize_bounds && !sc->abort) {
-            for (int k = 0; k < 3; k++) { sc->coord_min[k] = tot[k]; sc->coord_max[k] = tot[3 + k]; }
-        }
-        double ke = 0.0, ie = 0.0;
-        for (int m = 0; m < MCM_MAX_MAT; m++) {
-            if (m >= nm || sc->abort) break;
-            sc->mat_ke[m] = tot[6 + m];
-            sc->mat_ie[m] = tot[6 + nm + m];
-            for (int a = 0; a < 3; a++) sc->mat_mom[m][a] = tot[6 + 2 * nm + 3 * m + a];
-            ke = ke + sc->mat_ke[m];
-            ie = ie + sc->mat_ie[m];
-        }
-        if (!sc->abort) {
-            sc->kinetice = ke;
-            sc->internale = ie;
-            sc->totale = ke + ie;
-            if (finalize_bounds) sc->timestep += 1;
-        }
-        sc->done_c = 0;
+    if (threadIdx.x == 0 && !sc->abort) {
+        sc->kinetice = ke;
+        sc->internale = ie;
+        sc->totale = ke + ie;
+        if (with_bounds) sc->timestep += 1;
     }
 }
 
-// per-particle energy contributions into the partial vector (mcm_calctotale.f90:33-51)
-__device__ __forceinline__ void energy_terms(double (&vals)[RED_C], int nm, int mi, double mass, double vabs,
-                                             double e, const double v[3]) {
-#pragma unroll
-    for (int m = 0; m < MCM_MAX_MAT; m++) {
-        if (m == mi) {
-            vals[6 + m] = 0.5 * mass * (vabs * vabs);
-        }
+// Fused epilogue of the momentum pass for particle i with acceleration acc:
+// mcm_boundcond, the particle's terms of mcm_calctotale (kinetic energy
+// from the previous |v|), mcm_update_velocity and mcm_move with the
+// symmetry reflection of mcm_check_sym_pen.
+template <bool DIAG>
+__device__ __forceinline__ void integrate_particle(const DevState& d, int i, double acc[3], const double xi[3],
+                                                   CPart& cp) {
+    Scalars* sc = d.sc;
+    boundcond(d.dispbc[i], acc);
+    const int mi = d.pmat[i];
+    const double mass = d.mass[i];
+    double v[3] = {d.v[0][i], d.v[1][i], d.v[2][i]};
+    double vabs = d.vabs[i];
+    // mcm_calctotale.f90:33-51
+    energy_terms(cp, mi, mass, vabs, d.e[i], v);
+    // mcm_update_velocity.f90:29-56
+    double dtn = 0.5 * (sc->dt + sc->dtold);
+    double va = 0.0;
+    for (int a = 0; a < 3; a++) {
+        v[a] = v[a] + acc[a] * dtn;
+        va = va + v[a] * v[a];
     }
-    // ie and mom slots depend on nm; nm <= MCM_MAX_MAT
-#pragma unroll
-    for (int k = 6; k < RED_C; k++) {
-        int r = k - 6;
-        if (r >= nm && r < 2 * nm && r - nm == mi) vals[k] = e;
-        if (r >= 2 * nm && r < 5 * nm) {
-            int q = r - 2 * nm;
-            if (q / 3 == mi) vals[k] = mass * fabs(v[q % 3]);
-        }
+    d.vabs[i] = sqrt(va);
+    // mcm_move.f90:32-55
+    double x[3];
+    for (int a = 0; a < 3; a++) {
+        x[a] = xi[a] + v[a] * sc->dt;
+        cp.x[a] = x[a];
     }
+    if (d.boundary) sym_pen(d, x, v);
+    for (int a = 0; a < 3; a++) {
+        d.v[a][i] = v[a];
+        d.x0[a][i] = x[a];
+    }
+    if (DIAG)
+        for (int a = 0; a < 3; a++) d.a[a][i] = acc[a];
 }
 
+// Thread-per-particle momentum kernel (CORR as k_strain_const).
 template <int MODE, int CORR, bool DIAG>
 __global__ __launch_bounds__(MCM_BLOCK) void k_momentum(DevState d, int cur) {
     const int lb = xcd_block();
     const int i = lb * MCM_BLOCK + threadIdx.x;
     Scalars* sc = d.sc;
     const bool live = i < d.np && !sc->abort;
-    double vals[RED_C];
-    for (int k = 0; k < RED_C; k++) vals[k] = k < 3 ? 1.0e20 : (k < 6 ? -1.0e20 : 0.0);
+    CPart cp;
+    init_c_part(cp);
     if (live) {
         const MRec me = load_mrec(d.rm, i);
         const double xi[3] = {me.x, me.y, me.z};
@@ -686,38 +810,10 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_momentum(DevState d, int cur) {
         if (MODE == MODE_MOMENTUM) {
             for (int a = 0; a < 3; a++) d.a[a][i] = acc[a];
         } else {
-            boundcond(d.dispbc[i], acc);
-            const int mi = d.pmat[i];
-            const double mass = d.mass[i];
-            double v[3] = {d.v[0][i], d.v[1][i], d.v[2][i]};
-            double vabs = d.vabs[i];
-            // mcm_calctotale.f90:33-51 (kinetic energy from the previous |v|)
-            energy_terms(vals, d.nummat, mi, mass, vabs, d.e[i], v);
-            // mcm_update_velocity.f90:29-56
-            double dtn = 0.5 * (sc->dt + sc->dtold);
-            double va = 0.0;
-            for (int a = 0; a < 3; a++) {
-                v[a] = v[a] + acc[a] * dtn;
-                va = va + v[a] * v[a];
-            }
-            d.vabs[i] = sqrt(va);
-            // mcm_move.f90:32-55
-            double x[3];
-            for (int a = 0; a < 3; a++) {
-                x[a] = xi[a] + v[a] * sc->dt;
-                vals[a] = x[a];
-                vals[3 + a] = x[a];
-            }
-            if (d.boundary) sym_pen(d, x, v);
-            for (int a = 0; a < 3; a++) {
-                d.v[a][i] = v[a];
-                d.x0[a][i] = x[a];
-            }
-            if (DIAG)
-                for (int a = 0; a < 3; a++) d.a[a][i] = acc[a];
+            integrate_particle<DIAG>(d, i, acc, xi, cp);
         }
     }
-    if (MODE == MODE_FUSED) reduce_c_partials(d, vals, true, lb);
+    if (MODE == MODE_FUSED) reduce_c_partials(d, cp, true, lb);
 }
 
 // ---------------------------------------------- granular integration stages
@@ -731,15 +827,15 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_boundcond(DevState d) {
 
 __global__ __launch_bounds__(MCM_BLOCK) void k_calctotale(DevState d) {
     const int i = blockIdx.x * MCM_BLOCK + threadIdx.x;
-    double vals[RED_C];
-    for (int k = 0; k < RED_C; k++) vals[k] = k < 3 ? 1.0e20 : (k < 6 ? -1.0e20 : 0.0);
+    CPart cp;
+    init_c_part(cp);
     if (i < d.np && !d.sc->abort) {
         const int mi = d.pmat[i];
         const double mass = d.mass[i], vabs = d.vabs[i];
         double v[3] = {d.v[0][i], d.v[1][i], d.v[2][i]};
-        energy_terms(vals, d.nummat, mi, mass, vabs, d.e[i], v);
+        energy_terms(cp, mi, mass, vabs, d.e[i], v);
     }
-    reduce_c_partials(d, vals, false, blockIdx.x);
+    reduce_c_partials(d, cp, false, blockIdx.x);
 }
 
 __global__ __launch_bounds__(MCM_BLOCK) void k_update_velocity(DevState d) {
@@ -776,16 +872,16 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_move(DevState d) {
         double r = k < 3 ? block_reduce<1>(vals[k]) : block_reduce<2>(vals[k]);
         if (threadIdx.x == 0) d.red[(long)blockIdx.x * 6 + k] = r;
     }
-    if (!last_block(&sc->done_r)) return;
+}
+
+__global__ __launch_bounds__(MCM_BLOCK) void k_move_fold(DevState d, int nblk) {
+    Scalars* sc = d.sc;
     double r[6];
 #pragma unroll
-    for (int k = 0; k < 6; k++) r[k] = k < 3 ? fold_partials<1>(d.red, 6, k, 1.0e20) : fold_partials<2>(d.red, 6, k, -1.0e20);
-    if (threadIdx.x == 0) {
-        if (!sc->abort) {
-            for (int k = 0; k < 3; k++) { sc->coord_min[k] = r[k]; sc->coord_max[k] = r[3 + k]; }
-            sc->timestep += 1;
-        }
-        sc->done_r = 0;
+    for (int k = 0; k < 6; k++) r[k] = k < 3 ? fold<1>(d.red, 6, k, nblk, 1.0e20) : fold<2>(d.red, 6, k, nblk, -1.0e20);
+    if (threadIdx.x == 0 && !sc->abort) {
+        for (int k = 0; k < 3; k++) { sc->coord_min[k] = r[k]; sc->coord_max[k] = r[3 + k]; }
+        sc->timestep += 1;
     }
 }
 
@@ -805,10 +901,9 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_calccritts(DevState d, int cur) {
     }
     double cb = block_reduce<1>(crit);
     if (threadIdx.x == 0) d.red[blockIdx.x] = cb;
-    if (!last_block(&sc->done_r)) return;
-    double c1 = fold_partials<1>(d.red, 1, 0, 1.0e20);
-    if (threadIdx.x == 0) {
-        sc->critts = c1;
-        sc->done_r = 0;
-    }
+}
+
+__global__ __launch_bounds__(MCM_BLOCK) void k_calccritts_fold(DevState d, int nblk) {
+    double c1 = fold<1>(d.red, 1, 0, nblk, 1.0e20);
+    if (threadIdx.x == 0) d.sc->critts = c1;
 }

@@ -12,7 +12,8 @@ module mcm_gpu_iface
     integer(c_int32_t) :: ghost_capacity = 0
     integer(c_int32_t) :: device = 0
     integer(c_int32_t) :: corrected_one_pass = 0
-    integer(c_int32_t) :: reserved(4) = 0
+    integer(c_int32_t) :: pair_mapping = 0
+    integer(c_int32_t) :: reserved(3) = 0
   end type
 
   type, bind(c) :: mcm_gpu_scalars

@@ -41,7 +41,7 @@ class McmGpuError(RuntimeError):
 class Options(ctypes.Structure):
     _fields_ = [("nbr_capacity", ctypes.c_int32), ("ghost_capacity", ctypes.c_int32),
                 ("device", ctypes.c_int32), ("corrected_one_pass", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 4)]
+                ("pair_mapping", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
 
 
 class Scalars(ctypes.Structure):
@@ -93,9 +93,10 @@ class Solver:
     """One MCM run on one GPU, created from a binary deck (mcmtop_amd/deck.py)."""
 
     def __init__(self, deck_path: str, nbr_capacity: int = 0, ghost_capacity: int = 0, device: int = 0,
-                 corrected_one_pass: bool = False):
+                 corrected_one_pass: bool = False, pair_mapping: int = 0):
+        """pair_mapping: 0 one thread per particle, 1 one wavefront per particle (same results)."""
         self._lib = load_library()
-        opt = Options(nbr_capacity, ghost_capacity, device, int(bool(corrected_one_pass)))
+        opt = Options(nbr_capacity, ghost_capacity, device, int(bool(corrected_one_pass)), int(pair_mapping))
         h = ctypes.c_void_p()
         rc = self._lib.mcm_gpu_create_from_deck(deck_path.encode(), ctypes.byref(opt), ctypes.byref(h))
         self._h = h

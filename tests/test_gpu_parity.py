@@ -199,3 +199,67 @@ def test_taylor_bar_corrected_1000_steps(tmp_path):
     assert max(errs.values()) <= 1e-7, errs
     assert gd.kinetice == pytest.approx(od.kinetice, rel=1e-9)
     assert gd.internale == pytest.approx(od.internale, rel=1e-9)
+
+
+def _assert_same_state(a: Dump, b: Dump):
+    """State fields bit for bit (the fused paths stage different diagnostics:
+    the wavefront strain pass keeps rod and spin, the thread path does not)."""
+    for k in STATE:
+        np.testing.assert_array_equal(a.f(k), b.f(k), err_msg=k)
+
+
+def _fused_dump(path, tmp_path, nsteps, tag, **opts):
+    with gpu.Solver(path, **opts) as s:
+        s.initial()
+        s.step(nsteps)
+        return gdump(s, tmp_path, nsteps, tag)
+
+
+@pytest.mark.parametrize("name", sorted(cases.CASES))
+def test_wave_and_thread_mappings_agree(name, tmp_path):
+    """The wavefront-per-particle list build and pair kernels fold every sum
+    in list order, as the thread-per-particle kernels do: same state bit for
+    bit."""
+    path = os.path.join(DATA, name + ".bin")
+    n = cases.CASES[name][0]
+    a = _fused_dump(path, tmp_path, n, 1, pair_mapping=1)
+    b = _fused_dump(path, tmp_path, n, 2, pair_mapping=0)
+    _assert_same_state(a, b)
+    np.testing.assert_array_equal(a.nbr, b.nbr)
+    np.testing.assert_array_equal(a.gnbr, b.gnbr)
+    assert (a.ptime, a.kinetice, a.internale) == (b.ptime, b.kinetice, b.internale)
+
+
+def _dense_deck(corrected):
+    """Smoothing length 1.45 dx: 80-120 neighbours per particle, so lists
+    spill into the second wavefront slot (entries 64..127)."""
+    pts = deck.block(4.0e-4, 7, 7, 7)
+    rho = 8930.0
+    h = 1.45 * 4.0e-4
+    if corrected:
+        m = deck.material(10, rho, rho * 4.0e-4 ** 3 * len(pts), h, deck.f3dm10(45e9, 400e6, 100e6), eos=4,
+                          eoscard={0: 3940.0, 1: 1.49, 4: 2.0, 5: 0.47})
+    else:
+        m = deck.material(3, rho, rho * 4.0e-4 ** 3 * len(pts), h, deck.f3dm3(117e9, 0.35, 400e6, 100e6))
+    return deck.make_deck(pts, (0.0, 0.0, -190.0), [m], disctype=1 if corrected else 0, h_opt=1 if corrected else 0,
+                          jitter_dx=0.01 * 4.0e-4, sym_flag=(0, 0, 0, 0, 1, 0))
+
+
+@pytest.mark.parametrize("corrected", [False, True])
+def test_long_lists_use_both_wave_slots(corrected, tmp_path):
+    path = str(tmp_path / "dense.bin")
+    _dense_deck(corrected).write_bin(path)
+    o = Oracle(path)
+    o.initial()
+    o.run(15)
+    od = o.state(15, 10)
+    a = _fused_dump(path, tmp_path, 15, 1, pair_mapping=1)
+    assert (a.nnbr + a.g_nnbr).max() > 64
+    assert lists_equal_as_sets(a, od)
+    errs = field_errors(a, od, STATE)
+    assert max(errs.values()) <= 1e-9, errs
+    # lists longer than 2 x 64 slots go to the thread-per-particle kernels
+    b = _fused_dump(path, tmp_path, 15, 2, pair_mapping=1, nbr_capacity=200)
+    _assert_same_state(a, b)
+    c = _fused_dump(path, tmp_path, 15, 3, pair_mapping=0)
+    _assert_same_state(a, c)
