@@ -122,8 +122,9 @@ int mcm_gpu_download_neighbours(mcm_gpu_ctx *ctx, int32_t *counts, int32_t *gcou
 int mcm_gpu_dump(mcm_gpu_ctx *ctx, const char *path, int32_t step, int32_t stage);
 
 /* Time nsteps of the fused path with HIP events on the context stream;
-   returns the average milliseconds per step of each kernel group in ms[4]
-   (neighbour build, strain+constitutive, momentum+integration, whole step). */
+   returns the average milliseconds per step of each kernel group in ms[5]:
+   neighbour build, strain pass, constitutive update (+ dt), momentum pass
+   with integration (+ ghost stress), whole step. */
 int mcm_gpu_time_steps(mcm_gpu_ctx *ctx, int32_t nsteps, double *ms);
 
 #ifdef __cplusplus

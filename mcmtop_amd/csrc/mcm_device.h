@@ -35,6 +35,7 @@ struct Scalars {
     double mat_ke[MCM_MAX_MAT], mat_ie[MCM_MAX_MAT], mat_mom[MCM_MAX_MAT][3];
     GridDev grid;
     int ngp;            // ghosts generated this step
+    int gplane_lo, gplane_hi; // ghost ids made by the plane being generated
     int max_count;      // largest neighbour count seen by the last search
     int error_flags;    // MCM error bits, see mcm_gpu.h
     int abort;          // set when a capacity was exceeded: later kernels of the step do nothing
@@ -76,6 +77,7 @@ struct DevState {
     double* cand;              // [np][4] cell-sorted {x, y, z, h} of the candidates
     int* cand_id;              // [np][2] cell-sorted {particle id, material}
     int *gstart[3], *gcount[3];
+    int* gbase;                // [cellcap] first ghost id of each source-layer cell of a plane
     int* nbr;                  // [npad/64][ncap][64]: entry k of particle i at ((i/64)*ncap + k)*64 + i%64
     int *nreal, *ntot;
     int* scan_tmp;

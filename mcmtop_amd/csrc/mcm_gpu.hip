@@ -229,12 +229,13 @@ int alloc_neighbour_arrays(mcm_gpu_ctx* ctx) {
         rc |= dalloc(ctx, &d.gcount[k], d.cellcap);
     }
     rc |= dalloc(ctx, &d.scan_tmp, d.cellcap / SCAN_TILE + 2);
+    rc |= dalloc(ctx, &d.gbase, d.cellcap);
     return rc;
 }
 
 void free_neighbour_arrays(mcm_gpu_ctx* ctx) {
     DevState& d = ctx->d;
-    dfree(ctx, d.nbr); dfree(ctx, d.cell_count); dfree(ctx, d.cell_start); dfree(ctx, d.scan_tmp);
+    dfree(ctx, d.nbr); dfree(ctx, d.cell_count); dfree(ctx, d.cell_start); dfree(ctx, d.scan_tmp); dfree(ctx, d.gbase);
     for (int k = 0; k < 3; k++) { dfree(ctx, d.gstart[k]); dfree(ctx, d.gcount[k]); }
 }
 
@@ -281,6 +282,7 @@ void launch_neighbours(mcm_gpu_ctx* ctx, int cur) {
     hipStream_t s = ctx->stream;
     int gcl = std::min(blocks_for(d.cellcap, MCM_BLOCK), 4096);
     int ntile = blocks_for(d.cellcap, SCAN_TILE);
+    const int ghost_fill_blocks = std::max(1, std::min(blocks_for(d.ngcap, MCM_BLOCK), 512));
     k_grid_setup<<<1, 1, 0, s>>>(d);
     k_clear_cells<<<gcl, MCM_BLOCK, 0, s>>>(d);
     k_bin<<<ctx->nb, MCM_BLOCK, 0, s>>>(d, cur);
@@ -289,7 +291,10 @@ void launch_neighbours(mcm_gpu_ctx* ctx, int cur) {
     k_scan_apply<<<ntile, 1024, 0, s>>>(d);
     k_scatter<<<ctx->nb, MCM_BLOCK, 0, s>>>(d);
     k_sort_cells<<<gcl, MCM_BLOCK, 0, s>>>(d, cur);
-    for (int p = 0; p < d.nplanes; p++) k_ghost_plane<<<1, 1024, 0, s>>>(d, p, cur);
+    for (int p = 0; p < d.nplanes; p++) {
+        k_ghost_scan<<<1, 1024, 0, s>>>(d, p);
+        k_ghost_fill<<<ghost_fill_blocks, MCM_BLOCK, 0, s>>>(d, p, cur);
+    }
     if (ctx->pair_mapping == 1) k_list_wave<<<ctx->nb, MCM_BLOCK, 0, s>>>(d, cur);
     else k_list<<<ctx->nb, MCM_BLOCK, 0, s>>>(d, cur);
 }
@@ -304,13 +309,27 @@ bool use_wave(const mcm_gpu_ctx* ctx) {
     return ctx->pair_mapping == 1 && ctx->d.ncap <= 64 * WAVE_SLOTS && corr_variant(ctx) != 2;
 }
 
-void launch_strain_const(mcm_gpu_ctx* ctx, int cur) {
+// energies (and, after a move, coordinate bounds) from the block partials
+void launch_c_fold(mcm_gpu_ctx* ctx, hipStream_t st, int with_bounds) {
+    DevState& d = ctx->d;
+    switch (d.nummat) {
+        case 1: k_c_fold<1><<<1, MCM_BLOCK, 0, st>>>(d, ctx->nb, with_bounds); break;
+        case 2: k_c_fold<2><<<1, MCM_BLOCK, 0, st>>>(d, ctx->nb, with_bounds); break;
+        case 3: k_c_fold<3><<<1, MCM_BLOCK, 0, st>>>(d, ctx->nb, with_bounds); break;
+        case 4: k_c_fold<4><<<1, MCM_BLOCK, 0, st>>>(d, ctx->nb, with_bounds); break;
+        case 5: k_c_fold<5><<<1, MCM_BLOCK, 0, st>>>(d, ctx->nb, with_bounds); break;
+        case 6: k_c_fold<6><<<1, MCM_BLOCK, 0, st>>>(d, ctx->nb, with_bounds); break;
+        case 7: k_c_fold<7><<<1, MCM_BLOCK, 0, st>>>(d, ctx->nb, with_bounds); break;
+        default: k_c_fold<8><<<1, MCM_BLOCK, 0, st>>>(d, ctx->nb, with_bounds); break;
+    }
+}
+
+// strain pass of the fused step: stages rod, spin, tracerod and x1
+void launch_strain(mcm_gpu_ctx* ctx, int cur) {
     DevState& d = ctx->d;
     if (use_wave(ctx)) {
         if (corr_variant(ctx) == 1) k_strain_wave<1><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur);
         else k_strain_wave<0><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur);
-        k_strain_const<MODE_FCONST, 0, false><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur);
-        k_strain_fold<MODE_FCONST><<<1, MCM_BLOCK, 0, ctx->stream>>>(d, ctx->nb);
         return;
     }
     switch (corr_variant(ctx)) {
@@ -318,6 +337,11 @@ void launch_strain_const(mcm_gpu_ctx* ctx, int cur) {
         case 2: k_strain_const<MODE_FSTRAIN, 2, false><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur); break;
         default: k_strain_const<MODE_FSTRAIN, 0, false><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur); break;
     }
+}
+
+// update_h + constitutive + critts of the fused step, then dt and ptime
+void launch_constitutive(mcm_gpu_ctx* ctx, int cur) {
+    DevState& d = ctx->d;
     k_strain_const<MODE_FCONST, 0, false><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur);
     k_strain_fold<MODE_FCONST><<<1, MCM_BLOCK, 0, ctx->stream>>>(d, ctx->nb);
 }
@@ -328,7 +352,7 @@ void launch_momentum(mcm_gpu_ctx* ctx, int cur) {
     if (use_wave(ctx)) {
         if (corr_variant(ctx) == 1) k_momentum_wave<1><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur);
         else k_momentum_wave<0><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur);
-        k_c_fold<<<1, MCM_BLOCK, 0, ctx->stream>>>(d, ctx->nb, 1);
+        launch_c_fold(ctx, ctx->stream, 1);
         return;
     }
     switch (corr_variant(ctx)) {
@@ -336,12 +360,13 @@ void launch_momentum(mcm_gpu_ctx* ctx, int cur) {
         case 2: k_momentum<MODE_FUSED, 2, false><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur); break;
         default: k_momentum<MODE_FUSED, 0, false><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur); break;
     }
-    k_c_fold<<<1, MCM_BLOCK, 0, ctx->stream>>>(d, ctx->nb, 1);
+    launch_c_fold(ctx, ctx->stream, 1);
 }
 
 void launch_step(mcm_gpu_ctx* ctx, int cur) {
     launch_neighbours(ctx, cur);
-    launch_strain_const(ctx, cur);
+    launch_strain(ctx, cur);
+    launch_constitutive(ctx, cur);
     launch_momentum(ctx, cur);
 }
 
@@ -533,7 +558,7 @@ int initial(mcm_gpu_ctx* ctx) {
     rc = put_scalars_raw(ctx, s3);
     if (rc) return rc;
     k_calctotale<<<ctx->nb, MCM_BLOCK, 0, st>>>(d);
-    k_c_fold<<<1, MCM_BLOCK, 0, st>>>(d, ctx->nb, 0);
+    launch_c_fold(ctx, st, 0);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));
     return 0;
@@ -726,7 +751,7 @@ int mcm_gpu_boundcond(mcm_gpu_ctx* ctx) {
 int mcm_gpu_calctotale(mcm_gpu_ctx* ctx) {
     if (!ctx) return MCM_GPU_ERR_ARG;
     k_calctotale<<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(ctx->d);
-    k_c_fold<<<1, MCM_BLOCK, 0, ctx->stream>>>(ctx->d, ctx->nb, 0);
+    launch_c_fold(ctx, ctx->stream, 0);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -1000,26 +1025,30 @@ int mcm_gpu_dump(mcm_gpu_ctx* ctx, const char* path, int32_t step, int32_t stage
 
 int mcm_gpu_time_steps(mcm_gpu_ctx* ctx, int32_t nsteps, double* ms) {
     if (!ctx || !ms || nsteps <= 0 || ctx->phase != 0) return MCM_GPU_ERR_ARG;
-    hipEvent_t ev[4];
-    for (int k = 0; k < 4; k++) HIPCHK(hipEventCreate(&ev[k]));
-    double acc[4] = {0, 0, 0, 0};
+    hipEvent_t ev[5];
+    for (int k = 0; k < 5; k++) HIPCHK(hipEventCreate(&ev[k]));
+    double acc[5] = {0, 0, 0, 0, 0};
     for (int it = 0; it < nsteps; it++) {
         HIPCHK(hipEventRecord(ev[0], ctx->stream));
         launch_neighbours(ctx, ctx->cur);
         HIPCHK(hipEventRecord(ev[1], ctx->stream));
-        launch_strain_const(ctx, ctx->cur);
+        launch_strain(ctx, ctx->cur);
         HIPCHK(hipEventRecord(ev[2], ctx->stream));
-        launch_momentum(ctx, ctx->cur);
+        launch_constitutive(ctx, ctx->cur);
         HIPCHK(hipEventRecord(ev[3], ctx->stream));
-        HIPCHK(hipEventSynchronize(ev[3]));
+        launch_momentum(ctx, ctx->cur);
+        HIPCHK(hipEventRecord(ev[4], ctx->stream));
+        HIPCHK(hipEventSynchronize(ev[4]));
         float t;
-        HIPCHK(hipEventElapsedTime(&t, ev[0], ev[1])); acc[0] += t;
-        HIPCHK(hipEventElapsedTime(&t, ev[1], ev[2])); acc[1] += t;
-        HIPCHK(hipEventElapsedTime(&t, ev[2], ev[3])); acc[2] += t;
-        HIPCHK(hipEventElapsedTime(&t, ev[0], ev[3])); acc[3] += t;
+        for (int k = 0; k < 4; k++) {
+            HIPCHK(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+            acc[k] += t;
+        }
+        HIPCHK(hipEventElapsedTime(&t, ev[0], ev[4]));
+        acc[4] += t;
         ctx->cur ^= 1;
     }
-    for (int k = 0; k < 4; k++) { ms[k] = acc[k] / nsteps; (void)hipEventDestroy(ev[k]); }
+    for (int k = 0; k < 5; k++) { ms[k] = acc[k] / nsteps; (void)hipEventDestroy(ev[k]); }
     return check_sticky(ctx);
 }
 

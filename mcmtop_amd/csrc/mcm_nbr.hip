@@ -264,61 +264,68 @@ __device__ __forceinline__ void make_ghost(const DevState& d, int cur, const Pla
     rm[2] = make_double2(mv, mv / rv);
 }
 
-// One plane of mcm_ghost_setup.f90:32-266.  A single 1024-thread workgroup
-// walks the source layer's cells in ascending cell id (the reference's loop
-// order), counts each cell's members (its ghosts if it holds any, else its
-// real particles), scans the counts to assign ghost ids in creation order and
-// records each target cell's run of ghosts.
-__global__ __launch_bounds__(1024) void k_ghost_plane(DevState d, int pi, int cur) {
+// Layer geometry of one plane (mcm_ghost_setup.f90:38-43 and siblings):
+// cell r of the source layer S and the cell of the target layer T that its
+// ghosts go to (1-based in the reference: symmetry copies layer 2 / lim-1,
+// periodic the opposite end).
+struct PlaneLayers {
+    int S, T, dim, nlayer, L0, L1;
+    __device__ __forceinline__ PlaneLayers(const PlaneDev& P, const GridDev& g) {
+        dim = P.dim;
+        const int lim = g.lim[dim];
+        S = (P.side == 0 ? (P.code == 1 ? 2 : lim - 1) : (P.code == 1 ? lim - 1 : 2)) - 1;
+        T = (P.side == 0 ? 1 : lim) - 1;
+        L0 = g.lim[0];
+        L1 = g.lim[1];
+        nlayer = g.ncell / lim;
+    }
+    __device__ __forceinline__ void cells(int r, int& c, int& tc) const {
+        int i, j, k;
+        if (dim == 2) { i = r % L0; j = r / L0; k = S; }
+        else if (dim == 1) { i = r % L0; k = r / L0; j = S; }
+        else { j = r % L1; k = r / L1; i = S; }
+        c = (k * L1 + j) * L0 + i;
+        if (dim == 2) tc = (T * L1 + j) * L0 + i;
+        else if (dim == 1) tc = (k * L1 + T) * L0 + i;
+        else tc = (k * L1 + j) * L0 + T;
+    }
+};
+
+// One plane of mcm_ghost_setup.f90:32-266, first half.  A single 1024-thread
+// workgroup walks the source layer's cells in ascending cell id (the
+// reference's loop order), counts each cell's members (its ghosts if it
+// holds any, else its real particles) and scans the counts: gbase[r] is the
+// id of the first ghost made from source cell r, in the reference's creation
+// order.  Records each target cell's run of ghosts and the plane's ghost id
+// range for k_ghost_fill.
+__global__ __launch_bounds__(1024) void k_ghost_scan(DevState d, int pi) {
     Scalars* sc = d.sc;
     if (sc->abort) return;
-    const PlaneDev P = d.planes[pi];
-    const GridDev g = sc->grid;
-    const int dim = P.dim;
-    // source and target layers, mcm_ghost_setup.f90:38-43 and siblings
-    // (1-based: symmetry copies layer 2 / lim-1, periodic the opposite end)
-    const int lim = g.lim[dim];
-    const int S = (P.side == 0 ? (P.code == 1 ? 2 : lim - 1) : (P.code == 1 ? lim - 1 : 2)) - 1;
-    const int T = (P.side == 0 ? 1 : lim) - 1;
-    const int L0 = g.lim[0], L1 = g.lim[1];
-    const int nlayer = g.ncell / g.lim[dim];
+    const PlaneLayers PL(d.planes[pi], sc->grid);
     __shared__ int s_ngp;
     __shared__ int s_over;
     if (threadIdx.x == 0) { s_ngp = sc->ngp; s_over = 0; }
     __syncthreads();
-    for (int r0 = 0; r0 < nlayer; r0 += 1024) {
-        int r = r0 + threadIdx.x;
-        int cnt = 0, c = -1, tc = -1, gsum = 0;
-        if (r < nlayer) {
-            int i, j, k;
-            if (dim == 2) { i = r % L0; j = r / L0; k = S; }
-            else if (dim == 1) { i = r % L0; k = r / L0; j = S; }
-            else { j = r % L1; k = r / L1; i = S; }
-            c = (k * L1 + j) * L0 + i;
-            if (dim == 2) tc = (T * L1 + j) * L0 + i;
-            else if (dim == 1) tc = (k * L1 + T) * L0 + i;
-            else tc = (k * L1 + j) * L0 + T;
-            gsum = d.gcount[0][c] + d.gcount[1][c] + d.gcount[2][c];
+    const int lo = s_ngp;
+    for (int r0 = 0; r0 < PL.nlayer; r0 += 1024) {
+        const int r = r0 + threadIdx.x;
+        int cnt = 0, c = -1, tc = -1;
+        if (r < PL.nlayer) {
+            PL.cells(r, c, tc);
+            const int gsum = d.gcount[0][c] + d.gcount[1][c] + d.gcount[2][c];
             cnt = gsum > 0 ? gsum : d.cell_count[c];
         }
         int total;
-        int ex = block_excl_scan_1024(cnt, &total);
-        int base = s_ngp + ex;
+        const int ex = block_excl_scan_1024(cnt, &total);
+        const int base = s_ngp + ex;
         if (s_ngp + total > d.ngcap) {
             if (threadIdx.x == 0) s_over = 1;
-        } else if (cnt > 0) {
-            int m = 0;
-            if (gsum > 0) {
-                for (int dd = 2; dd >= 0; dd--) {
-                    int st = d.gstart[dd][c], n = d.gcount[dd][c];
-                    for (int q = n - 1; q >= 0; q--) make_ghost(d, cur, P, false, st + q, base + m++);
-                }
-            } else {
-                int st = d.cell_start[c];
-                for (int q = 0; q < cnt; q++) make_ghost(d, cur, P, true, d.sorted[st + q], base + m++);
+        } else if (r < PL.nlayer) {
+            d.gbase[r] = base;
+            if (cnt > 0) {
+                d.gstart[PL.dim][tc] = base;
+                d.gcount[PL.dim][tc] = cnt;
             }
-            d.gstart[dim][tc] = base;
-            d.gcount[dim][tc] = cnt;
         }
         __syncthreads();
         if (threadIdx.x == 0 && !s_over) s_ngp += total;
@@ -328,6 +335,46 @@ __global__ __launch_bounds__(1024) void k_ghost_plane(DevState d, int pi, int cu
     if (threadIdx.x == 0) {
         if (s_over) { sc->error_flags |= 2; sc->abort = 1; }
         sc->ngp = s_ngp;
+        sc->gplane_lo = lo;
+        sc->gplane_hi = s_ngp;
+    }
+}
+
+// Second half: one thread per ghost of the plane.  The source cell of ghost
+// g is the last layer cell r with gbase[r] <= g (cells without members share
+// the next cell's base); within it the members are taken in the order the
+// reference pushes them: a cell holding ghosts gives its ghosts of the z,
+// then y, then x planes, each run from its last ghost down; a cell of real
+// particles gives them in descending id (its sorted order).
+__global__ __launch_bounds__(MCM_BLOCK) void k_ghost_fill(DevState d, int pi, int cur) {
+    Scalars* sc = d.sc;
+    if (sc->abort) return;
+    const PlaneDev P = d.planes[pi];
+    const PlaneLayers PL(P, sc->grid);
+    const int lo = sc->gplane_lo, hi = sc->gplane_hi;
+    for (int g = lo + blockIdx.x * MCM_BLOCK + threadIdx.x; g < hi; g += gridDim.x * MCM_BLOCK) {
+        int a = 0, b = PL.nlayer - 1;   // last r with gbase[r] <= g
+        while (a < b) {
+            const int mid = (a + b + 1) >> 1;
+            if (d.gbase[mid] <= g) a = mid;
+            else b = mid - 1;
+        }
+        int c, tc;
+        PL.cells(a, c, tc);
+        int m = g - d.gbase[a];
+        const int gsum = d.gcount[0][c] + d.gcount[1][c] + d.gcount[2][c];
+        if (gsum > 0) {
+            for (int dd = 2; dd >= 0; dd--) {
+                const int n = d.gcount[dd][c];
+                if (m < n) {
+                    make_ghost(d, cur, P, false, d.gstart[dd][c] + (n - 1 - m), g);
+                    break;
+                }
+                m -= n;
+            }
+        } else {
+            make_ghost(d, cur, P, true, d.sorted[d.cell_start[c] + m], g);
+        }
     }
 }
 

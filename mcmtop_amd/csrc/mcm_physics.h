@@ -40,6 +40,43 @@ MCM_HD double dmax(double a, double b) { return a > b ? a : b; }
 MCM_HD double dmin(double a, double b) { return a < b ? a : b; }
 MCM_HD double ddim(double a, double b) { return a - b > 0.0 ? a - b : 0.0; }
 
+// Correctly rounded quotients a / b for several a and one divisor b.  On the
+// device a / b compiles to a Newton-refined reciprocal of b, the residual
+// correction fma(a - b*q, r, q), and scaling/fix-up steps that change
+// nothing when |a| and |b| lie in [2^-300, 2^300] (no scaling for those
+// exponents, normal quotient).  For such operands the refined reciprocal is
+// computed once and shared, which gives every quotient bit for bit; other
+// operands (and host code) divide.
+struct Divisor {
+    double b, r;
+    bool fast;
+    MCM_HD static bool in_range(double x) {
+        const double ax = fabs(x);
+        return ax >= 0x1p-300 && ax <= 0x1p300;
+    }
+    MCM_HD explicit Divisor(double b_) : b(b_), r(0.0), fast(false) {
+#if defined(__HIP_DEVICE_COMPILE__)
+        fast = in_range(b);
+        double y = __builtin_amdgcn_rcp(b);
+        double e = fma(-b, y, 1.0);
+        y = fma(y, e, y);
+        e = fma(-b, y, 1.0);
+        r = fma(y, e, y);
+#endif
+    }
+    MCM_HD double div(double a) const {
+#if defined(__HIP_DEVICE_COMPILE__)
+        if (fast && in_range(a)) {
+            const double q = a * r;
+            const double rem = fma(-b, q, a);
+            return fma(rem, r, q);
+        }
+#endif
+        return a / b;
+    }
+};
+
+
 // ------------------------------------------------------------------ kernels
 // Cubic B-spline, 3-D (mcm_kernel.f90:42-95); r = |xi - xj|
 MCM_HD double kernel_w(double rad, double havg, double pi) {
@@ -80,10 +117,10 @@ MCM_HD void kernel_grad(double xij0, double xij1, double xij2, double havg, doub
     if (rad == 0.0) {
         dwdx[0] = 0.0; dwdx[1] = 0.0; dwdx[2] = 0.0;
     } else {
-        double rh = rad * havg;
-        dwdx[0] = dw * xij0 / rh;
-        dwdx[1] = dw * xij1 / rh;
-        dwdx[2] = dw * xij2 / rh;
+        const Divisor rh(rad * havg);
+        dwdx[0] = rh.div(dw * xij0);
+        dwdx[1] = rh.div(dw * xij1);
+        dwdx[2] = rh.div(dw * xij2);
     }
 }
 

@@ -213,6 +213,7 @@ __device__ __forceinline__ void strain_corr(const DevState& d, int i, double dt,
         for (int l = 0; l < 3; l++) sg[l] = sg[l] + r.vol * dwdx[l];
     }
     const double sw2 = sw * sw;
+    const Divisor isw2(sw2);
     double B[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, invB[9];
     NbrStream<SRec> s2(lst, d.rs, n, i);
     for (int k = 0; k < n; k++) {
@@ -221,7 +222,7 @@ __device__ __forceinline__ void strain_corr(const DevState& d, int i, double dt,
         s2.next(j, r);
         double xj[3], w, dwdx[3], cg[3];
         corr_pair(r, j < d.np ? dt : 0.0, xi, hi, d.pi, xj, w, dwdx);
-        for (int m = 0; m < 3; m++) cg[m] = (dwdx[m] * sw - w * sg[m]) / sw2;
+        for (int m = 0; m < 3; m++) cg[m] = isw2.div(dwdx[m] * sw - w * sg[m]);
         for (int m = 0; m < 3; m++)
             for (int nn = 0; nn < 3; nn++) B[nn + 3 * m] = B[nn + 3 * m] + r.vol * (xj[nn] - xi[nn]) * cg[m];
     }
@@ -235,7 +236,7 @@ __device__ __forceinline__ void strain_corr(const DevState& d, int i, double dt,
         s3.next(j, r);
         double xj[3], w, dwdx[3], cg[3], g[3];
         corr_pair(r, j < d.np ? dt : 0.0, xi, hi, d.pi, xj, w, dwdx);
-        for (int m = 0; m < 3; m++) cg[m] = (dwdx[m] * sw - w * sg[m]) / sw2;
+        for (int m = 0; m < 3; m++) cg[m] = isw2.div(dwdx[m] * sw - w * sg[m]);
         for (int q = 0; q < 3; q++) g[q] = invB[q] * cg[0] + invB[q + 3] * cg[1] + invB[q + 6] * cg[2];
         double vj[3] = {r.vx, r.vy, r.vz};
         for (int nn = 0; nn < 3; nn++)
@@ -559,6 +560,7 @@ __device__ __forceinline__ void accel_corr(const DevState& d, int i, const MRec&
         massj = r.m;
     }
     const double sw2 = sw * sw;
+    const Divisor isw2(sw2);
     double B[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, invB[9];
     NbrStream<MRec> s2(lst, d.rm, n, i);
     for (int k = 0; k < n; k++) {
@@ -566,7 +568,7 @@ __device__ __forceinline__ void accel_corr(const DevState& d, int i, const MRec&
         s2.next(j, r);
         double w, dwdx[3], cg[3];
         corr_pair_j(r, me, d.pi, w, dwdx);
-        for (int m = 0; m < 3; m++) cg[m] = (dwdx[m] * sw - w * sg[m]) / sw2;
+        for (int m = 0; m < 3; m++) cg[m] = isw2.div(dwdx[m] * sw - w * sg[m]);
         double dx[3] = {r.x - me.x, r.y - me.y, r.z - me.z};
         for (int m = 0; m < 3; m++)
             for (int nn = 0; nn < 3; nn++) B[nn + 3 * m] = B[nn + 3 * m] + r.vol * dx[nn] * cg[m];
@@ -581,7 +583,7 @@ __device__ __forceinline__ void accel_corr(const DevState& d, int i, const MRec&
         if (k == self_k) continue;   // dsig = 0 there
         double w, dwdx[3], cg[3];
         corr_pair_j(r, me, d.pi, w, dwdx);
-        for (int m = 0; m < 3; m++) cg[m] = (dwdx[m] * sw - w * sg[m]) / sw2;
+        for (int m = 0; m < 3; m++) cg[m] = isw2.div(dwdx[m] * sw - w * sg[m]);
         for (int c = 0; c < 6; c++) {
             double dsig = r.A[c] + me.A[c];
             D[c][0] = D[c][0] + dsig * cg[0];
@@ -724,33 +726,44 @@ __device__ void reduce_c_partials(const DevState& d, const CPart& c, bool with_b
 }
 
 // Coordinate bounds (with_bounds: the step's move), per-material energies
-// and their totals from the partials of nblk blocks.  One workgroup.
+// and their totals from the partials of nblk blocks.  One workgroup; every
+// thread folds all components of its blocks in one sweep (block order per
+// component, as fold() does), then the block reductions combine the threads.
+template <int NM>
 __global__ __launch_bounds__(MCM_BLOCK) void k_c_fold(DevState d, int nblk, int with_bounds) {
     Scalars* sc = d.sc;
-    const int nm = d.nummat;
-    for (int a = 0; a < 3; a++) {
-        double lo = fold<1>(d.red, RED_C, a, nblk, 1.0e20);
-        double hi = fold<2>(d.red, RED_C, 3 + a, nblk, -1.0e20);
-        if (threadIdx.x == 0 && with_bounds && !sc->abort) { sc->coord_min[a] = lo; sc->coord_max[a] = hi; }
-    }
-    double ke = 0.0, ie = 0.0;
-    for (int m = 0; m < nm; m++) {
-        double mk = fold<0>(d.red, RED_C, 6 + m, nblk, 0.0);
-        double mie = fold<0>(d.red, RED_C, 6 + nm + m, nblk, 0.0);
-        ke = ke + mk;
-        ie = ie + mie;
-        if (threadIdx.x == 0 && !sc->abort) { sc->mat_ke[m] = mk; sc->mat_ie[m] = mie; }
-        for (int a = 0; a < 3; a++) {
-            double mo = fold<0>(d.red, RED_C, 6 + 2 * nm + 3 * m + a, nblk, 0.0);
-            if (threadIdx.x == 0 && !sc->abort) sc->mat_mom[m][a] = mo;
+    constexpr int nm = NM;
+    constexpr int nv = 6 + 5 * NM;
+    double acc[nv];
+#pragma unroll
+    for (int k = 0; k < nv; k++) acc[k] = k < 3 ? 1.0e20 : (k < 6 ? -1.0e20 : 0.0);
+    for (int b = threadIdx.x; b < nblk; b += MCM_BLOCK) {
+        const double* p = d.red + (long)b * RED_C;
+#pragma unroll
+        for (int k = 0; k < nv; k++) {
+            const double x = p[k];
+            acc[k] = k < 3 ? fmin(acc[k], x) : (k < 6 ? fmax(acc[k], x) : acc[k] + x);
         }
     }
-    if (threadIdx.x == 0 && !sc->abort) {
-        sc->kinetice = ke;
-        sc->internale = ie;
-        sc->totale = ke + ie;
-        if (with_bounds) sc->timestep += 1;
+    double tot[nv];
+#pragma unroll
+    for (int k = 0; k < nv; k++)
+        tot[k] = k < 3 ? block_reduce<1>(acc[k]) : (k < 6 ? block_reduce<2>(acc[k]) : block_reduce<0>(acc[k]));
+    if (threadIdx.x != 0 || sc->abort) return;
+    if (with_bounds)
+        for (int a = 0; a < 3; a++) { sc->coord_min[a] = tot[a]; sc->coord_max[a] = tot[3 + a]; }
+    double ke = 0.0, ie = 0.0;
+    for (int m = 0; m < nm; m++) {
+        sc->mat_ke[m] = tot[6 + m];
+        sc->mat_ie[m] = tot[6 + nm + m];
+        for (int a = 0; a < 3; a++) sc->mat_mom[m][a] = tot[6 + 2 * nm + 3 * m + a];
+        ke = ke + tot[6 + m];
+        ie = ie + tot[6 + nm + m];
     }
+    sc->kinetice = ke;
+    sc->internale = ie;
+    sc->totale = ke + ie;
+    if (with_bounds) sc->timestep += 1;
 }
 
 // Fused epilogue of the momentum pass for particle i with acceleration acc:

@@ -311,6 +311,7 @@ __device__ __forceinline__ void strain_wave(const DevState& d, double* buf, int 
     }
     const double sw = sums[0], sg[3] = {sums[1], sums[2], sums[3]};
     const double sw2 = sw * sw;
+    const Divisor isw2(sw2);
     // pass 2: B(n,m) = sum V (xj - xi)_n cg_m
     double B[9], invB[9];
     {
@@ -319,7 +320,7 @@ __device__ __forceinline__ void strain_wave(const DevState& d, double* buf, int 
         for (int s = 0; s < WAVE_SLOTS; s++) {
             if (s > 0 && !two) break;
 #pragma unroll
-            for (int m = 0; m < 3; m++) q[s].cg[m] = (q[s].dwdx[m] * sw - q[s].w * sg[m]) / sw2;
+            for (int m = 0; m < 3; m++) q[s].cg[m] = isw2.div(q[s].dwdx[m] * sw - q[s].w * sg[m]);
 #pragma unroll
             for (int m = 0; m < 3; m++)
 #pragma unroll
@@ -471,6 +472,7 @@ __device__ __forceinline__ void accel_wave(const DevState& d, double* buf, int i
     }
     const double sw = sums[0], sg[3] = {sums[1], sums[2], sums[3]};
     const double sw2 = sw * sw;
+    const Divisor isw2(sw2);
     double B[9], invB[9];
     {
         double t[WAVE_SLOTS][9];
@@ -478,7 +480,7 @@ __device__ __forceinline__ void accel_wave(const DevState& d, double* buf, int i
         for (int s = 0; s < WAVE_SLOTS; s++) {
             if (s > 0 && !two) break;
 #pragma unroll
-            for (int m = 0; m < 3; m++) q[s].cg[m] = (q[s].dwdx[m] * sw - q[s].w * sg[m]) / sw2;
+            for (int m = 0; m < 3; m++) q[s].cg[m] = isw2.div(q[s].dwdx[m] * sw - q[s].w * sg[m]);
 #pragma unroll
             for (int m = 0; m < 3; m++)
 #pragma unroll
