@@ -29,17 +29,43 @@ sys.path.insert(0, HERE)
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
+# Algorithmic cost of the two pair kernels of the corrected step (f3dm10 +
+# eqos4, h_opt 1, tcrit_opt 1); nbar = mean list length (self and ghosts
+# included).  Bytes: every per-particle word the kernel must read or write,
+# once (cache-ideal gathers), plus the 4-byte list entries.
+#   strain pass : reads the strain record {x, v, m/rho, h} (64 B), the list
+#                 length (4 B) and the list; writes rod (6), spin (3),
+#                 tracerod, x after the pass (3) and the momentum record's
+#                 x, y (2): 188 + 4 nbar bytes
+#   momentum    : reads the momentum record {x, h, m, m/rho, A(6)} (96 B), two
+#                 list lengths (8 B), the list, v (3), |v|, e, dispbc, material
+#                 (48 B); writes v, x, |v| (56 B): 208 + 4 nbar bytes
+# Flops (add, mul, div, sqrt = 1; no FMA contraction): per list entry over the
+# reference's three passes, and per particle (inverse of B, contractions,
+# integration): strain 242 / 81, momentum 209 / 163.
+KERNELS = {
+    "strain": {"name": "k_strain_const<MODE_FSTRAIN, corrected>", "symbol": "void k_strain_const<4, 1, false>(DevState, int)",
+               "bytes": (188.0, 4.0), "flops": (242.0, 81.0)},
+    "momentum": {"name": "k_momentum<MODE_FUSED, corrected>", "symbol": "void k_momentum<0, 1, false>(DevState, int)",
+                 "bytes": (208.0, 4.0), "flops": (209.0, 163.0)},
+}
+PROFILE = os.path.join(HERE, "profiles", "step_kernels_taylor_bar_corrected_2.1M.md")
 
-def kernel_bytes_per_particle(nbar: float, corrected: bool) -> float:
-    """Algorithmic HBM bytes per particle of the strain+constitutive kernel
-    (f3dm10 + eqos4, h_opt = 1, tcrit_opt = 1): every per-particle array it
-    must read or write is moved once (cache-ideal neighbour gathers), plus
-    the 4-byte neighbour ids.
-    reads : x(3) v(3) mass rho h | sigma(6) p e c q pcut efps fail vabs h0 rho0 = 23 words
-    writes: x_half(3) rho h hold | sigma(6) p e c q pcut p_cut efps fail | A(6) = 27 words
-    """
-    words = 23 + 27
-    return 8.0 * words + 4.0 * (nbar + 1)
+
+def measured_traffic(symbol: str):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 summary
+    (FETCH_SIZE and WRITE_SIZE runs scaled by the copy calibration), or None."""
+    if not os.path.exists(PROFILE):
+        return None
+    in_table = False
+    for line in open(PROFILE):
+        if line.startswith("#"):
+            in_table = "HBM traffic per dispatch" in line
+            continue
+        cells = [c.strip().strip("`") for c in line.split("|")]
+        if in_table and len(cells) > 6 and cells[1] == symbol and cells[-2].replace(".", "").isdigit():
+            return float(cells[-2]) * 1e9
+    return None
 
 
 def build_deck(path: str, dx: float, length: float):
@@ -126,15 +152,20 @@ def main():
     secs = t1 - t0
     n = d.np_
     value = n * args.steps / secs
-    # per-kernel-group durations with HIP events on the context stream
+    # per-kernel durations with HIP events on the context stream
     ms = s.time_steps(10)
-    sc2 = s.scalars()
-    nbar = float(sc2.max_count)
-    # mean neighbour count from the lists
     counts, gcounts, _ = s.neighbour_lists()
     nbar = float((counts + gcounts).mean())
-    bpp = kernel_bytes_per_particle(nbar, True)
-    achieved = bpp * n / (ms[1] * 1e-3) / 1e9
+    # the dominant pair kernel: roofline against HBM (algorithmic bytes) and
+    # its FP64 rate (algorithmic flops)
+    kms = {"strain": ms[1], "momentum": ms[3]}
+    dom = max(kms, key=kms.get)
+    K = KERNELS[dom]
+    t = kms[dom] * 1e-3
+    bytes_launch = n * (K["bytes"][0] + K["bytes"][1] * nbar)
+    flops_launch = n * (K["flops"][0] * nbar + K["flops"][1])
+    achieved = bytes_launch / t / 1e9
+    traffic = measured_traffic(K["symbol"])
     res = {
         "metric": "FP64 particle-steps/s (plain per-instance launch)",
         "value": value,
@@ -151,11 +182,14 @@ def main():
         "config": {"workload": "taylor_bar_corrected_f3dm10_eqos4_hopt1", "particles": n, "dx_mm": args.dx * 1e3,
                    "corrected_passes": 1 if args.corrected_one_pass else 3,
                    "mean_neighbours": round(nbar, 2), "ghosts": sc1.ngp, "ptime_end": sc1.ptime,
-                   "kernel_ms": {"neighbours": ms[0], "strain_constitutive": ms[1], "momentum_move": ms[2],
-                                 "step": ms[3]}},
-        "roofline": {"bound": "hbm", "kernel": "k_strain_const", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_particle": bpp},
+                   "pair_interactions_per_s": n * nbar * args.steps / secs,
+                   "kernel_ms": {"neighbours": ms[0], "strain_pass": ms[1], "constitutive": ms[2],
+                                 "momentum": ms[3], "step": ms[4]}},
+        "roofline": {"bound": "hbm", "kernel": K["name"], "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic, "algorithmic_bytes": bytes_launch, "launch_ms": kms[dom],
+                     "fp64": {"achieved": flops_launch / t / 1e12, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                              "frac": flops_launch / t / 1e12 / FP64_PEAK_TFLOPS}},
     }
     s.close()
     if not args.no_cpu_baseline:

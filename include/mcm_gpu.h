@@ -122,9 +122,10 @@ int mcm_gpu_download_neighbours(mcm_gpu_ctx *ctx, int32_t *counts, int32_t *gcou
 int mcm_gpu_dump(mcm_gpu_ctx *ctx, const char *path, int32_t step, int32_t stage);
 
 /* Time nsteps of the fused path with HIP events on the context stream;
-   returns the average milliseconds per step of each kernel group in ms[5]:
-   neighbour build, strain pass, constitutive update (+ dt), momentum pass
-   with integration (+ ghost stress), whole step. */
+   returns average milliseconds per step in ms[5]: neighbour build (all its
+   kernels), strain-pass kernel, constitutive update (+ dt fold), momentum
+   kernel (with boundcond, velocity and move; without the ghost-stress
+   refresh and the energy fold), whole step. */
 int mcm_gpu_time_steps(mcm_gpu_ctx *ctx, int32_t nsteps, double *ms);
 
 #ifdef __cplusplus

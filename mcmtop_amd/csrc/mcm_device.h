@@ -7,6 +7,21 @@
 #define MCM_MAX_MAT 8
 #define MCM_BLOCK 256
 
+// butterfly reductions over the 64 lanes of a wavefront (all lanes active)
+__device__ __forceinline__ double wave_min(double v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v = v + __shfl_xor(v, off, 64);
+    return v;
+}
+
 // gfx950 dispatches the workgroups of a launch to its 8 XCDs round-robin
 // (workgroup b runs on XCD b % 8), and each XCD has its own L2.  Gather
 // kernels renumber their workgroups so that every XCD walks one contiguous
@@ -36,6 +51,7 @@ struct Scalars {
     GridDev grid;
     int ngp;            // ghosts generated this step
     int gplane_lo, gplane_hi; // ghost ids made by the plane being generated
+    unsigned long long hcap_bits;  // bit pattern of the largest h among the binned particles
     int max_count;      // largest neighbour count seen by the last search
     int error_flags;    // MCM error bits, see mcm_gpu.h
     int abort;          // set when a capacity was exceeded: later kernels of the step do nothing
@@ -78,6 +94,7 @@ struct DevState {
     int* cand_id;              // [np][2] cell-sorted {particle id, material}
     int *gstart[3], *gcount[3];
     int* gbase;                // [cellcap] first ghost id of each source-layer cell of a plane
+    int* gany;                 // [cellcap] 1 where a cell holds ghosts
     int* nbr;                  // [npad/64][ncap][64]: entry k of particle i at ((i/64)*ncap + k)*64 + i%64
     int *nreal, *ntot;
     int* scan_tmp;

@@ -230,12 +230,13 @@ int alloc_neighbour_arrays(mcm_gpu_ctx* ctx) {
     }
     rc |= dalloc(ctx, &d.scan_tmp, d.cellcap / SCAN_TILE + 2);
     rc |= dalloc(ctx, &d.gbase, d.cellcap);
+    rc |= dalloc(ctx, &d.gany, d.cellcap);
     return rc;
 }
 
 void free_neighbour_arrays(mcm_gpu_ctx* ctx) {
     DevState& d = ctx->d;
-    dfree(ctx, d.nbr); dfree(ctx, d.cell_count); dfree(ctx, d.cell_start); dfree(ctx, d.scan_tmp); dfree(ctx, d.gbase);
+    dfree(ctx, d.nbr); dfree(ctx, d.cell_count); dfree(ctx, d.cell_start); dfree(ctx, d.scan_tmp); dfree(ctx, d.gbase); dfree(ctx, d.gany);
     for (int k = 0; k < 3; k++) { dfree(ctx, d.gstart[k]); dfree(ctx, d.gcount[k]); }
 }
 
@@ -290,13 +291,15 @@ void launch_neighbours(mcm_gpu_ctx* ctx, int cur) {
     k_scan_top<<<1, 1024, 0, s>>>(d);
     k_scan_apply<<<ntile, 1024, 0, s>>>(d);
     k_scatter<<<ctx->nb, MCM_BLOCK, 0, s>>>(d);
-    k_sort_cells<<<gcl, MCM_BLOCK, 0, s>>>(d, cur);
+    k_sort_cells<<<gcl, MCM_BLOCK, 0, s>>>(d);
+    k_fill_cand<<<ctx->nb, MCM_BLOCK, 0, s>>>(d, cur);
     for (int p = 0; p < d.nplanes; p++) {
         k_ghost_scan<<<1, 1024, 0, s>>>(d, p);
         k_ghost_fill<<<ghost_fill_blocks, MCM_BLOCK, 0, s>>>(d, p, cur);
     }
     if (ctx->pair_mapping == 1) k_list_wave<<<ctx->nb, MCM_BLOCK, 0, s>>>(d, cur);
-    else k_list<<<ctx->nb, MCM_BLOCK, 0, s>>>(d, cur);
+    else if (d.nummat > 1) k_list<true><<<ctx->nb, MCM_BLOCK, 0, s>>>(d, cur);
+    else k_list<false><<<ctx->nb, MCM_BLOCK, 0, s>>>(d, cur);
 }
 
 int ghost_blocks(const DevState& d) { return std::max(1, std::min(blocks_for(d.ngcap, MCM_BLOCK), 1024)); }
@@ -346,13 +349,13 @@ void launch_constitutive(mcm_gpu_ctx* ctx, int cur) {
     k_strain_fold<MODE_FCONST><<<1, MCM_BLOCK, 0, ctx->stream>>>(d, ctx->nb);
 }
 
-void launch_momentum(mcm_gpu_ctx* ctx, int cur) {
+// momentum + boundcond + energies + velocity + move of the fused step (the
+// ghosts' stress is refreshed first, the energy and bounds fold follows)
+void launch_momentum_kernel(mcm_gpu_ctx* ctx, int cur) {
     DevState& d = ctx->d;
-    if (d.boundary) k_ghost_stress<<<ghost_blocks(d), MCM_BLOCK, 0, ctx->stream>>>(d, cur);
     if (use_wave(ctx)) {
         if (corr_variant(ctx) == 1) k_momentum_wave<1><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur);
         else k_momentum_wave<0><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur);
-        launch_c_fold(ctx, ctx->stream, 1);
         return;
     }
     switch (corr_variant(ctx)) {
@@ -360,6 +363,12 @@ void launch_momentum(mcm_gpu_ctx* ctx, int cur) {
         case 2: k_momentum<MODE_FUSED, 2, false><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur); break;
         default: k_momentum<MODE_FUSED, 0, false><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur); break;
     }
+}
+
+void launch_momentum(mcm_gpu_ctx* ctx, int cur) {
+    DevState& d = ctx->d;
+    if (d.boundary) k_ghost_stress<<<ghost_blocks(d), MCM_BLOCK, 0, ctx->stream>>>(d, cur);
+    launch_momentum_kernel(ctx, cur);
     launch_c_fold(ctx, ctx->stream, 1);
 }
 
@@ -1025,30 +1034,36 @@ int mcm_gpu_dump(mcm_gpu_ctx* ctx, const char* path, int32_t step, int32_t stage
 
 int mcm_gpu_time_steps(mcm_gpu_ctx* ctx, int32_t nsteps, double* ms) {
     if (!ctx || !ms || nsteps <= 0 || ctx->phase != 0) return MCM_GPU_ERR_ARG;
-    hipEvent_t ev[5];
-    for (int k = 0; k < 5; k++) HIPCHK(hipEventCreate(&ev[k]));
+    DevState& d = ctx->d;
+    hipEvent_t ev[7];
+    for (int k = 0; k < 7; k++) HIPCHK(hipEventCreate(&ev[k]));
     double acc[5] = {0, 0, 0, 0, 0};
     for (int it = 0; it < nsteps; it++) {
+        const int cur = ctx->cur;
         HIPCHK(hipEventRecord(ev[0], ctx->stream));
-        launch_neighbours(ctx, ctx->cur);
+        launch_neighbours(ctx, cur);
         HIPCHK(hipEventRecord(ev[1], ctx->stream));
-        launch_strain(ctx, ctx->cur);
+        launch_strain(ctx, cur);
         HIPCHK(hipEventRecord(ev[2], ctx->stream));
-        launch_constitutive(ctx, ctx->cur);
+        launch_constitutive(ctx, cur);
         HIPCHK(hipEventRecord(ev[3], ctx->stream));
-        launch_momentum(ctx, ctx->cur);
+        if (d.boundary) k_ghost_stress<<<ghost_blocks(d), MCM_BLOCK, 0, ctx->stream>>>(d, cur);
         HIPCHK(hipEventRecord(ev[4], ctx->stream));
-        HIPCHK(hipEventSynchronize(ev[4]));
+        launch_momentum_kernel(ctx, cur);
+        HIPCHK(hipEventRecord(ev[5], ctx->stream));
+        launch_c_fold(ctx, ctx->stream, 1);
+        HIPCHK(hipEventRecord(ev[6], ctx->stream));
+        HIPCHK(hipEventSynchronize(ev[6]));
         float t;
-        for (int k = 0; k < 4; k++) {
-            HIPCHK(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
-            acc[k] += t;
-        }
-        HIPCHK(hipEventElapsedTime(&t, ev[0], ev[4]));
-        acc[4] += t;
+        HIPCHK(hipEventElapsedTime(&t, ev[0], ev[1])); acc[0] += t;
+        HIPCHK(hipEventElapsedTime(&t, ev[1], ev[2])); acc[1] += t;
+        HIPCHK(hipEventElapsedTime(&t, ev[2], ev[3])); acc[2] += t;
+        HIPCHK(hipEventElapsedTime(&t, ev[4], ev[5])); acc[3] += t;
+        HIPCHK(hipEventElapsedTime(&t, ev[0], ev[6])); acc[4] += t;
         ctx->cur ^= 1;
     }
-    for (int k = 0; k < 5; k++) { ms[k] = acc[k] / nsteps; (void)hipEventDestroy(ev[k]); }
+    for (int k = 0; k < 5; k++) ms[k] = acc[k] / nsteps;
+    for (int k = 0; k < 7; k++) (void)hipEventDestroy(ev[k]);
     return check_sticky(ctx);
 }
 

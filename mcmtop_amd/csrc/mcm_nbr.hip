@@ -110,6 +110,7 @@ __global__ void k_grid_setup(DevState d) {
     sc->grid = g;
     sc->ngp = 0;
     sc->max_count = 0;
+    sc->hcap_bits = 0;
 }
 
 __global__ void k_clear_cells(DevState d) {
@@ -117,36 +118,66 @@ __global__ void k_clear_cells(DevState d) {
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += gridDim.x * blockDim.x) {
         d.cell_count[c] = 0;
         d.gcount[0][c] = 0; d.gcount[1][c] = 0; d.gcount[2][c] = 0;
+        d.gany[c] = 0;
     }
 }
 
 // histogram: cell of every real particle (mcm_setuplist.f90:320-345); also
 // lays down the particle's strain gather record for this step
 __global__ __launch_bounds__(256) void k_bin(DevState d, int cur) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= d.np || d.sc->abort) return;
-    const GridDev g = d.sc->grid;
-    const double x[3] = {d.x0[0][i], d.x0[1][i], d.x0[2][i]};
-    {
-        double2* r = reinterpret_cast<double2*>(d.rs) + 4 * (long)i;
-        double vx = d.v[0][i], vy = d.v[1][i], vz = d.v[2][i];
-        r[0] = make_double2(x[0], x[1]);
-        r[1] = make_double2(x[2], vx);
-        r[2] = make_double2(vy, vz);
-        r[3] = make_double2(d.mass[i] / d.rbuf[cur][i], d.hbuf[cur][i]);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d.sc->abort) return;
+    const bool live = i < d.np;
+    double hb = 0.0;   // h of a binned particle: bounds every list partner's h
+    int cell = -1;
+    if (live) {
+        const GridDev g = d.sc->grid;
+        const double x[3] = {d.x0[0][i], d.x0[1][i], d.x0[2][i]};
+        const double h = d.hbuf[cur][i];
+        {
+            double2* r = reinterpret_cast<double2*>(d.rs) + 4 * (long)i;
+            double vx = d.v[0][i], vy = d.v[1][i], vz = d.v[2][i];
+            r[0] = make_double2(x[0], x[1]);
+            r[1] = make_double2(x[2], vx);
+            r[2] = make_double2(vy, vz);
+            r[3] = make_double2(d.mass[i] / d.rbuf[cur][i], h);
+        }
+        int box[3];
+        bool in = true;
+        for (int j = 0; j < 3; j++) {
+            box[j] = cell_index(g.gridsize[j] * (x[j] - g.gridmin[j]));
+            if (box[j] < 1 || box[j] > g.lim[j]) in = false;
+        }
+        if (in) {
+            cell = ((box[2] - 1) * g.lim[1] + (box[1] - 1)) * g.lim[0] + (box[0] - 1);
+            hb = h;
+        }
+        d.cell_of[i] = cell;
     }
-    int box[3];
-    bool in = true;
-    for (int j = 0; j < 3; j++) {
-        box[j] = cell_index(g.gridsize[j] * (x[j] - g.gridmin[j]));
-        if (box[j] < 1 || box[j] > g.lim[j]) in = false;
+    // histogram with one atomic per distinct cell of the wavefront: lanes of
+    // the same cell take consecutive slots (the cell sort orders them later)
+    const int lane = threadIdx.x & 63;
+    const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    unsigned long long todo = __ballot(cell >= 0);
+    while (todo) {
+        const int leader = __builtin_ctzll(todo);
+        const int lc = __builtin_amdgcn_readlane(cell, leader);
+        const unsigned long long same = __ballot(cell == lc);
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&d.cell_count[lc], __popcll(same));
+        base = __builtin_amdgcn_readlane(base, leader);
+        if (cell == lc) d.slot_of[i] = base + __popcll(same & below);
+        todo &= ~same;
     }
-    if (in) {
-        int c = ((box[2] - 1) * g.lim[1] + (box[1] - 1)) * g.lim[0] + (box[0] - 1);
-        d.cell_of[i] = c;
-        d.slot_of[i] = atomicAdd(&d.cell_count[c], 1);
-    } else {
-        d.cell_of[i] = -1;
+    // workgroup maximum, one atomic per workgroup (positive doubles order as
+    // their bit patterns)
+    __shared__ double wmax[4];
+    hb = wave_max(hb);
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = hb;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        hb = fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3]));
+        if (hb > 0.0) atomicMax(&d.sc->hcap_bits, (unsigned long long)__double_as_longlong(hb));
     }
 }
 
@@ -207,13 +238,12 @@ __global__ __launch_bounds__(256) void k_scatter(DevState d) {
 }
 
 // order each cell by descending id (the walk order of the reference's pushed
-// lists) and lay down cell-sorted copies of x, h and the material
-__global__ __launch_bounds__(256) void k_sort_cells(DevState d, int cur) {
+// lists), one thread per cell
+__global__ __launch_bounds__(256) void k_sort_cells(DevState d) {
     int ncell = d.sc->grid.ncell;
-    const double* h = d.hbuf[cur];
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += gridDim.x * blockDim.x) {
         int n = d.cell_count[c];
-        if (n == 0) continue;
+        if (n < 2) continue;
         int* seg = d.sorted + d.cell_start[c];
         for (int a = 1; a < n; a++) {
             int key = seg[a];
@@ -221,16 +251,22 @@ __global__ __launch_bounds__(256) void k_sort_cells(DevState d, int cur) {
             while (b >= 0 && seg[b] < key) { seg[b + 1] = seg[b]; b--; }
             seg[b + 1] = key;
         }
-        int st = d.cell_start[c];
-        double2* cd = reinterpret_cast<double2*>(d.cand) + 2 * (long)st;
-        int2* ci = reinterpret_cast<int2*>(d.cand_id) + st;
-        for (int a = 0; a < n; a++) {
-            int j = seg[a];
-            cd[2 * a] = make_double2(d.x0[0][j], d.x0[1][j]);
-            cd[2 * a + 1] = make_double2(d.x0[2][j], h[j]);
-            ci[a] = make_int2(j, d.pmat[j]);
-        }
     }
+}
+
+// cell-sorted copies of x, h and the material of the binned particles, one
+// thread per sorted slot (the candidate records of the list build)
+__global__ __launch_bounds__(256) void k_fill_cand(DevState d, int cur) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const GridDev& g = d.sc->grid;
+    if (d.sc->abort || g.ncell <= 0) return;
+    const int nsorted = d.cell_start[g.ncell - 1] + d.cell_count[g.ncell - 1];
+    if (s >= nsorted) return;
+    const int j = d.sorted[s];
+    double2* cd = reinterpret_cast<double2*>(d.cand) + 2 * (long)s;
+    cd[0] = make_double2(d.x0[0][j], d.x0[1][j]);
+    cd[1] = make_double2(d.x0[2][j], d.hbuf[cur][j]);
+    reinterpret_cast<int2*>(d.cand_id)[s] = make_int2(j, d.pmat[j]);
 }
 
 // ------------------------------------------------------------------ ghosts
@@ -325,6 +361,7 @@ __global__ __launch_bounds__(1024) void k_ghost_scan(DevState d, int pi) {
             if (cnt > 0) {
                 d.gstart[PL.dim][tc] = base;
                 d.gcount[PL.dim][tc] = cnt;
+                d.gany[tc] = 1;
             }
         }
         __syncthreads();
@@ -380,7 +417,18 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_ghost_fill(DevState d, int pi, in
 
 // ------------------------------------------------------------ list build
 // mcm_ll_neighbours.f90:96-299 for one real particle, one thread per
-// particle (list layout: nbr_list in mcm_device.h).
+// particle (list layout: nbr_list in mcm_device.h).  MULTI: more than one
+// material, so same-material and other-material passes.
+//
+// Real candidates are read from cell runs of the sorted order.  A pair is
+// accepted only when its distance is below h_i + h_j <= h_i + hcap (hcap:
+// largest h of the binned particles), so the real particles of a cell whose
+// box lies at least that far away can never be accepted: such rows and
+// x-run end cells are skipped (with a relative margin of 1e-9 that covers the
+// rounding of the box faces).  Ghosts are not binned by position and are
+// never skipped.  Skipping rejected candidates leaves the list, its order
+// and mindist unchanged.
+template <bool MULTI>
 __global__ __launch_bounds__(256) void k_list(DevState d, int cur) {
     int i = xcd_block() * blockDim.x + threadIdx.x;
     Scalars* sc = d.sc;
@@ -397,69 +445,93 @@ __global__ __launch_bounds__(256) void k_list(DevState d, int cur) {
         const double xi = d.x0[0][i], yi = d.x0[1][i], zi = d.x0[2][i];
         const double hi = d.hbuf[cur][i];
         const int mi = d.pmat[i];
-        int b0 = c % L0, b1 = (c / L0) % L1, b2 = c / (L0 * L1);
-        int lo0 = max(b0 - 1, 0), hi0 = min(b0 + 1, L0 - 1);
-        int lo1 = max(b1 - 1, 0), hi1 = min(b1 + 1, L1 - 1);
-        int lo2 = max(b2 - 1, 0), hi2 = min(b2 + 1, g.lim[2] - 1);
-        const int npass = d.nummat > 1 ? 2 : 1;
+        const int b0 = c % L0, b1 = (c / L0) % L1, b2 = c / (L0 * L1);
+        const int lo0 = max(b0 - 1, 0), hi0 = min(b0 + 1, L0 - 1);
+        const int lo1 = max(b1 - 1, 0), hi1 = min(b1 + 1, L1 - 1);
+        const int lo2 = max(b2 - 1, 0), hi2 = min(b2 + 1, g.lim[2] - 1);
+        // distances from the particle to the lower and upper faces of its cell
+        const double reach = (hi + __longlong_as_double((long long)sc->hcap_bits)) * (1.0 + 1.0e-9);
+        const double reach2 = reach * reach;
+        double flo[3], fhi[3];
+        {
+            const double xv[3] = {xi, yi, zi};
+            const int bv[3] = {b0, b1, b2};
+            for (int a = 0; a < 3; a++) {
+                const double cw = 1.0 / g.gridsize[a];
+                const double face = g.gridmin[a] + bv[a] * cw;
+                flo[a] = fmax(xv[a] - face, 0.0);
+                fhi[a] = fmax(face + cw - xv[a], 0.0);
+            }
+        }
         const double2* cd = reinterpret_cast<const double2*>(d.cand);
         const int2* ci = reinterpret_cast<const int2*>(d.cand_id);
-        for (int pass = 0; pass < npass; pass++) {
-            for (int n = lo2; n <= hi2; n++)
-                for (int m = lo1; m <= hi1; m++) {
-                    // the x-run of cells (lo0..hi0) of this row is contiguous in the
-                    // sorted order; a row holding a ghost-headed cell is walked cell
-                    // by cell so that its real particles stay hidden
-                    int c0 = (n * L1 + m) * L0 + lo0;
-                    int nrun = hi0 - lo0 + 1;
-                    bool ghosty = false;
-                    for (int l = 0; l < nrun; l++)
-                        ghosty |= (d.gcount[0][c0 + l] + d.gcount[1][c0 + l] + d.gcount[2][c0 + l]) > 0;
-                    for (int l = 0; l < (ghosty ? nrun : 1); l++) {
-                        int cc = c0 + l;
-                        if (ghosty && (d.gcount[0][cc] + d.gcount[1][cc] + d.gcount[2][cc]) > 0) continue;
-                        int st = d.cell_start[cc];
-                        int en = ghosty ? st + d.cell_count[cc]
-                                        : d.cell_start[c0 + nrun - 1] + d.cell_count[c0 + nrun - 1];
-                        // candidates four at a time: the loads of a group are
-                        // issued together, the tests run in walk order
-                        auto consider = [&](int2 idm, double2 pa, double2 pb) {
-                            const int j = idm.x;
-                            if (j == i) return;
-                            double dist_sq = 0.0;
-                            double dx = pa.x - xi; dist_sq = dist_sq + dx * dx;
-                            double dy = pa.y - yi; dist_sq = dist_sq + dy * dy;
-                            double dz = pb.x - zi; dist_sq = dist_sq + dz * dz;
-                            double h_avg = 2.0 * 0.5 * (pb.y + hi);
-                            if (dist_sq < h_avg * h_avg) {
-                                bool same = idm.y == mi;
-                                if (pass == 0 && same) {
-                                    mind = fmin(mind, dist_sq);
-                                    if (cnt < ncap) out[(long)cnt * 64] = j;
-                                    cnt++;
-                                } else if (pass == 1 && !same) {
-                                    mind_cp = fmin(mind_cp, dist_sq);
-                                    if (cnt < ncap) out[(long)cnt * 64] = j;
-                                    cnt++;
-                                }
-                            }
-                        };
-                        int s = st;
-                        for (; s + 4 <= en; s += 4) {
-                            int2 idm[4];
-                            double2 pa[4], pb[4];
-#pragma unroll
-                            for (int u = 0; u < 4; u++) {
-                                idm[u] = ci[s + u];
-                                pa[u] = cd[2 * (s + u)];
-                                pb[u] = cd[2 * (s + u) + 1];
-                            }
-#pragma unroll
-                            for (int u = 0; u < 4; u++) consider(idm[u], pa[u], pb[u]);
-                        }
-                        for (; s < en; s++) consider(ci[s], cd[2 * s], cd[2 * s + 1]);
+        for (int pass = 0; pass < (MULTI ? 2 : 1); pass++) {
+            auto consider = [&](int2 idm, double2 pa, double2 pb) {
+                const int j = idm.x;
+                if (j == i) return;
+                double dx = pa.x - xi;
+                double dist_sq = dx * dx;   // 0.0 + dx*dx is dx*dx exactly
+                double dy = pa.y - yi; dist_sq = dist_sq + dy * dy;
+                double dz = pb.x - zi; dist_sq = dist_sq + dz * dz;
+                double h_avg = 2.0 * 0.5 * (pb.y + hi);
+                if (dist_sq < h_avg * h_avg) {
+                    const bool same = !MULTI || idm.y == mi;
+                    if (pass == 0 && same) {
+                        mind = fmin(mind, dist_sq);
+                        if (cnt < ncap) out[(long)cnt * 64] = j;
+                        cnt++;
+                    } else if (MULTI && pass == 1 && !same) {
+                        mind_cp = fmin(mind_cp, dist_sq);
+                        if (cnt < ncap) out[(long)cnt * 64] = j;
+                        cnt++;
                     }
                 }
+            };
+            auto walk = [&](int st, int en) {
+                // candidates four at a time: the loads of a group are issued
+                // together, the tests run in walk order
+                int s = st;
+                for (; s + 4 <= en; s += 4) {
+                    int2 idm[4];
+                    double2 pa[4], pb[4];
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        idm[u] = ci[s + u];
+                        pa[u] = cd[2 * (s + u)];
+                        pb[u] = cd[2 * (s + u) + 1];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; u++) consider(idm[u], pa[u], pb[u]);
+                }
+                for (; s < en; s++) consider(ci[s], cd[2 * s], cd[2 * s + 1]);
+            };
+            for (int n = lo2; n <= hi2; n++) {
+                const double fz = n < b2 ? flo[2] : (n > b2 ? fhi[2] : 0.0);
+                for (int m = lo1; m <= hi1; m++) {
+                    const double fy = m < b1 ? flo[1] : (m > b1 ? fhi[1] : 0.0);
+                    const double dyz = fz * fz + fy * fy;
+                    if (dyz >= reach2) continue;
+                    // x-run of the row, without end cells out of reach
+                    int l0 = lo0, l1 = hi0;
+                    if (l0 < b0 && dyz + flo[0] * flo[0] >= reach2) l0 = b0;
+                    if (l1 > b0 && dyz + fhi[0] * fhi[0] >= reach2) l1 = b0;
+                    // the run is contiguous in the sorted order; a row holding a
+                    // ghost-headed cell is walked cell by cell so that that
+                    // cell's real particles stay hidden
+                    const int c0 = (n * L1 + m) * L0;
+                    bool ghosty = false;
+                    for (int l = l0; l <= l1; l++) ghosty |= d.gany[c0 + l] != 0;
+                    if (!ghosty) {
+                        walk(d.cell_start[c0 + l0], d.cell_start[c0 + l1] + d.cell_count[c0 + l1]);
+                    } else {
+                        for (int l = l0; l <= l1; l++) {
+                            const int cc = c0 + l;
+                            if (d.gany[cc]) continue;
+                            walk(d.cell_start[cc], d.cell_start[cc] + d.cell_count[cc]);
+                        }
+                    }
+                }
+            }
         }
         if (d.disctype == 1) {
             if (cnt < ncap) out[(long)cnt * 64] = i;
@@ -467,11 +539,12 @@ __global__ __launch_bounds__(256) void k_list(DevState d, int cur) {
         }
         nreal = cnt;
         if (d.boundary) {
-            for (int pass = 0; pass < npass; pass++) {
+            for (int pass = 0; pass < (MULTI ? 2 : 1); pass++) {
                 for (int n = lo2; n <= hi2; n++)
                     for (int m = lo1; m <= hi1; m++)
                         for (int l = lo0; l <= hi0; l++) {
                             int cc = (n * L1 + m) * L0 + l;
+                            if (!d.gany[cc]) continue;
                             for (int dd = 2; dd >= 0; dd--) {
                                 int gn = d.gcount[dd][cc];
                                 if (gn == 0) continue;
@@ -484,12 +557,12 @@ __global__ __launch_bounds__(256) void k_list(DevState d, int cur) {
                                     double dz = d.x0[2][u] - zi; dist_sq = dist_sq + dz * dz;
                                     double h_avg = 2.0 * 0.5 * (d.hbuf[cur][u] + hi);
                                     if (dist_sq < h_avg * h_avg) {
-                                        bool same = d.pmat[u] == mi;
+                                        const bool same = !MULTI || d.pmat[u] == mi;
                                         if (pass == 0 && same) {
                                             mind = fmin(mind, dist_sq);
                                             if (cnt < ncap) out[(long)cnt * 64] = u;
                                             cnt++;
-                                        } else if (pass == 1 && !same) {
+                                        } else if (MULTI && pass == 1 && !same) {
                                             mind_cp = fmin(mind_cp, dist_sq);
                                             if (cnt < ncap) out[(long)cnt * 64] = u;
                                             cnt++;

@@ -52,7 +52,7 @@ struct Divisor {
     bool fast;
     MCM_HD static bool in_range(double x) {
         const double ax = fabs(x);
-        return ax >= 0x1p-300 && ax <= 0x1p300;
+        return (int)(ax >= 0x1p-300) & (int)(ax <= 0x1p300);
     }
     MCM_HD explicit Divisor(double b_) : b(b_), r(0.0), fast(false) {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -64,15 +64,31 @@ struct Divisor {
         r = fma(y, e, y);
 #endif
     }
+    MCM_HD double quot(double a) const {
+        const double q = a * r;
+        const double rem = fma(-b, q, a);
+        return fma(rem, r, q);
+    }
     MCM_HD double div(double a) const {
 #if defined(__HIP_DEVICE_COMPILE__)
-        if (fast && in_range(a)) {
-            const double q = a * r;
-            const double rem = fma(-b, q, a);
-            return fma(rem, r, q);
-        }
+        if ((int)fast & (int)in_range(a)) return quot(a);
 #endif
         return a / b;
+    }
+    // three quotients with one range test
+    MCM_HD void div3(double a0, double a1, double a2, double q[3]) const {
+#if defined(__HIP_DEVICE_COMPILE__)
+        // bitwise, not short-circuit: one branch instead of four
+        if ((int)fast & (int)in_range(a0) & (int)in_range(a1) & (int)in_range(a2)) {
+            q[0] = quot(a0);
+            q[1] = quot(a1);
+            q[2] = quot(a2);
+            return;
+        }
+#endif
+        q[0] = a0 / b;
+        q[1] = a1 / b;
+        q[2] = a2 / b;
     }
 };
 
@@ -83,12 +99,13 @@ MCM_HD double kernel_w(double rad, double havg, double pi) {
     double const1 = 1.0 / (pi * havg * havg * havg);
     double const2 = const1 / 4.0;
     double z = rad / havg;
-    if (z < 1.0) return const1 * (((0.75 * z - 1.5) * z) * z + 1.0);
-    if (z < 2.0) {
-        double t = 2.0 - z;
-        return const2 * (t * (t * t));
-    }
-    return 0.0;
+    // both branches of mcm_kernel.f90:78-92 are evaluated and one selected
+    // (no divergence; each branch's expression as in the reference)
+    double w1 = const1 * (((0.75 * z - 1.5) * z) * z + 1.0);
+    double t = 2.0 - z;
+    double w2 = const2 * (t * (t * t));
+    const double w12 = __builtin_unpredictable(z < 2.0) ? w2 : 0.0;
+    return __builtin_unpredictable(z < 1.0) ? w1 : w12;
 }
 
 // dW/dr factor of mcm_gradw.f90:41-103; gradient_n = dw * xij_n / (rad * havg)
@@ -96,15 +113,15 @@ MCM_HD double kernel_dw(double rad, double havg, double pi) {
     double const1 = 1.0 / (pi * havg * havg * havg);
     double const2 = const1 * 0.25;
     double z = rad / havg;
-    if (z < 1.0) return const1 * ((2.25 * z - 3.0) * z);
-    if (z < 2.0) {
-        double t = 2.0 - z;
-        return -(3.0 * const2 * (t * t));
-    }
-    return 0.0;
+    double d1 = const1 * ((2.25 * z - 3.0) * z);
+    double t = 2.0 - z;
+    double d2 = -(3.0 * const2 * (t * t));
+    const double d12 = __builtin_unpredictable(z < 2.0) ? d2 : 0.0;
+    return __builtin_unpredictable(z < 1.0) ? d1 : d12;
 }
 
-// gradient of W with respect to xi, xij = xi - xj (mcm_gradw.f90:76-101)
+// gradient of W with respect to xi, xij = xi - xj (mcm_gradw.f90:76-101);
+// zero at rad == 0
 MCM_HD void kernel_grad(double xij0, double xij1, double xij2, double havg, double pi, double* dwdx,
                         double* rad_out) {
     double rad = 0.0;
@@ -114,14 +131,13 @@ MCM_HD void kernel_grad(double xij0, double xij1, double xij2, double havg, doub
     rad = sqrt(rad);
     *rad_out = rad;
     double dw = kernel_dw(rad, havg, pi);
-    if (rad == 0.0) {
-        dwdx[0] = 0.0; dwdx[1] = 0.0; dwdx[2] = 0.0;
-    } else {
-        const Divisor rh(rad * havg);
-        dwdx[0] = rh.div(dw * xij0);
-        dwdx[1] = rh.div(dw * xij1);
-        dwdx[2] = rh.div(dw * xij2);
-    }
+    const Divisor rh(rad * havg);
+    double g[3];
+    rh.div3(dw * xij0, dw * xij1, dw * xij2, g);
+    const bool zero = rad == 0.0;
+    dwdx[0] = __builtin_unpredictable(zero) ? 0.0 : g[0];
+    dwdx[1] = __builtin_unpredictable(zero) ? 0.0 : g[1];
+    dwdx[2] = __builtin_unpredictable(zero) ? 0.0 : g[2];
 }
 
 // 3x3 cofactor inverse, identity when |det| < 1e-8 (mcm_math_func.f90:50-79).

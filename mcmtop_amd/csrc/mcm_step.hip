@@ -31,20 +31,6 @@
 enum { MODE_FUSED = 0, MODE_STRAIN = 1, MODE_CONST = 2, MODE_MOMENTUM = 1, MODE_FCONST = 3, MODE_FSTRAIN = 4 };
 
 // ------------------------------------------------------------- reductions
-__device__ __forceinline__ double wave_min(double v) {
-    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
-    return v;
-}
-__device__ __forceinline__ double wave_max(double v) {
-    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-    return v;
-}
-
-__device__ __forceinline__ double wave_sum(double v) {
-    for (int off = 32; off > 0; off >>= 1) v = v + __shfl_xor(v, off, 64);
-    return v;
-}
-
 // Block-wide reduction (OP 0 sum, 1 min, 2 max) in a fixed order: butterfly
 // within each wavefront, then the wavefronts in index order.  Every thread
 // of the block must call it; the result is valid in every thread.
@@ -132,18 +118,23 @@ struct NbrStream {
     const double* base;
     int n, k, self, jn, jnn;
     Rec rn;
+    // list slots are read unconditionally (slots 0..ncap-1 are allocated),
+    // values past the end are replaced by self before use
     __device__ __forceinline__ NbrStream(const int* l, const double* b, int n_, int self_)
         : lst(l), base(b), n(n_), k(0), self(self_) {
-        jn = n > 0 ? lst[0] : self;
+        const int l0 = lst[0], l1 = lst[n > 1 ? 64 : 0];
+        jn = n > 0 ? l0 : self;
         rn = load_rec<Rec>(base, jn);
-        jnn = n > 1 ? lst[64] : self;
+        jnn = n > 1 ? l1 : self;
     }
     __device__ __forceinline__ void next(int& j, Rec& r) {
         j = jn;
         r = rn;
         jn = jnn;
         rn = load_rec<Rec>(base, jn);
-        jnn = k + 2 < n ? lst[(long)(k + 2) * 64] : self;
+        const int kk = k + 2 < n ? k + 2 : k;
+        const int v = lst[(long)kk * 64];
+        jnn = k + 2 < n ? v : self;
         k++;
     }
 };
@@ -222,7 +213,7 @@ __device__ __forceinline__ void strain_corr(const DevState& d, int i, double dt,
         s2.next(j, r);
         double xj[3], w, dwdx[3], cg[3];
         corr_pair(r, j < d.np ? dt : 0.0, xi, hi, d.pi, xj, w, dwdx);
-        for (int m = 0; m < 3; m++) cg[m] = isw2.div(dwdx[m] * sw - w * sg[m]);
+        isw2.div3(dwdx[0] * sw - w * sg[0], dwdx[1] * sw - w * sg[1], dwdx[2] * sw - w * sg[2], cg);
         for (int m = 0; m < 3; m++)
             for (int nn = 0; nn < 3; nn++) B[nn + 3 * m] = B[nn + 3 * m] + r.vol * (xj[nn] - xi[nn]) * cg[m];
     }
@@ -236,7 +227,7 @@ __device__ __forceinline__ void strain_corr(const DevState& d, int i, double dt,
         s3.next(j, r);
         double xj[3], w, dwdx[3], cg[3], g[3];
         corr_pair(r, j < d.np ? dt : 0.0, xi, hi, d.pi, xj, w, dwdx);
-        for (int m = 0; m < 3; m++) cg[m] = isw2.div(dwdx[m] * sw - w * sg[m]);
+        isw2.div3(dwdx[0] * sw - w * sg[0], dwdx[1] * sw - w * sg[1], dwdx[2] * sw - w * sg[2], cg);
         for (int q = 0; q < 3; q++) g[q] = invB[q] * cg[0] + invB[q + 3] * cg[1] + invB[q + 6] * cg[2];
         double vj[3] = {r.vx, r.vy, r.vz};
         for (int nn = 0; nn < 3; nn++)
@@ -568,7 +559,7 @@ __device__ __forceinline__ void accel_corr(const DevState& d, int i, const MRec&
         s2.next(j, r);
         double w, dwdx[3], cg[3];
         corr_pair_j(r, me, d.pi, w, dwdx);
-        for (int m = 0; m < 3; m++) cg[m] = isw2.div(dwdx[m] * sw - w * sg[m]);
+        isw2.div3(dwdx[0] * sw - w * sg[0], dwdx[1] * sw - w * sg[1], dwdx[2] * sw - w * sg[2], cg);
         double dx[3] = {r.x - me.x, r.y - me.y, r.z - me.z};
         for (int m = 0; m < 3; m++)
             for (int nn = 0; nn < 3; nn++) B[nn + 3 * m] = B[nn + 3 * m] + r.vol * dx[nn] * cg[m];
@@ -583,7 +574,7 @@ __device__ __forceinline__ void accel_corr(const DevState& d, int i, const MRec&
         if (k == self_k) continue;   // dsig = 0 there
         double w, dwdx[3], cg[3];
         corr_pair_j(r, me, d.pi, w, dwdx);
-        for (int m = 0; m < 3; m++) cg[m] = isw2.div(dwdx[m] * sw - w * sg[m]);
+        isw2.div3(dwdx[0] * sw - w * sg[0], dwdx[1] * sw - w * sg[1], dwdx[2] * sw - w * sg[2], cg);
         for (int c = 0; c < 6; c++) {
             double dsig = r.A[c] + me.A[c];
             D[c][0] = D[c][0] + dsig * cg[0];

@@ -319,8 +319,7 @@ __device__ __forceinline__ void strain_wave(const DevState& d, double* buf, int 
 #pragma unroll
         for (int s = 0; s < WAVE_SLOTS; s++) {
             if (s > 0 && !two) break;
-#pragma unroll
-            for (int m = 0; m < 3; m++) q[s].cg[m] = isw2.div(q[s].dwdx[m] * sw - q[s].w * sg[m]);
+            isw2.div3(q[s].dwdx[0] * sw - q[s].w * sg[0], q[s].dwdx[1] * sw - q[s].w * sg[1], q[s].dwdx[2] * sw - q[s].w * sg[2], q[s].cg);
 #pragma unroll
             for (int m = 0; m < 3; m++)
 #pragma unroll
@@ -479,8 +478,7 @@ __device__ __forceinline__ void accel_wave(const DevState& d, double* buf, int i
 #pragma unroll
         for (int s = 0; s < WAVE_SLOTS; s++) {
             if (s > 0 && !two) break;
-#pragma unroll
-            for (int m = 0; m < 3; m++) q[s].cg[m] = isw2.div(q[s].dwdx[m] * sw - q[s].w * sg[m]);
+            isw2.div3(q[s].dwdx[0] * sw - q[s].w * sg[0], q[s].dwdx[1] * sw - q[s].w * sg[1], q[s].dwdx[2] * sw - q[s].w * sg[2], q[s].cg);
 #pragma unroll
             for (int m = 0; m < 3; m++)
 #pragma unroll

@@ -190,8 +190,9 @@ class Solver:
         self._check(self._lib.mcm_gpu_dump(self._h, path.encode(), step, stage), "dump")
 
     def time_steps(self, n: int) -> np.ndarray:
-        """Average ms per step of: neighbour build, strain pass, constitutive
-        update, momentum pass, whole step (HIP events on the run's stream)."""
+        """Average ms per step of: neighbour build, strain-pass kernel,
+        constitutive update, momentum kernel, whole step (HIP events on the
+        run's stream)."""
         ms = (ctypes.c_double * 5)()
         self._check(self._lib.mcm_gpu_time_steps(self._h, int(n), ms), "time_steps")
         return np.array(ms[:])
