@@ -92,6 +92,7 @@ struct DevState {
     int *cell_of, *slot_of, *cell_count, *cell_start, *sorted;
     double* cand;              // [np][4] cell-sorted {x, y, z, h} of the candidates
     int* cand_id;              // [np][2] cell-sorted {particle id, material}
+    int* spos;                 // [np] sorted slot of each binned particle
     int *gstart[3], *gcount[3];
     int* gbase;                // [cellcap] first ghost id of each source-layer cell of a plane
     int* gany;                 // [cellcap] 1 where a cell holds ghosts

@@ -266,6 +266,7 @@ int alloc_all(mcm_gpu_ctx* ctx) {
     rc |= dalloc(ctx, &d.cell_of, n); rc |= dalloc(ctx, &d.slot_of, n); rc |= dalloc(ctx, &d.sorted, n);
     rc |= dalloc(ctx, &d.cand, 4 * n);
     rc |= dalloc(ctx, &d.cand_id, 2 * n);
+    rc |= dalloc(ctx, &d.spos, n);
     rc |= dalloc(ctx, &d.nreal, n); rc |= dalloc(ctx, &d.ntot, n);
     rc |= dalloc(ctx, &d.red, (size_t)ctx->nb * RED_C + 64);
     rc |= dalloc(ctx, &d.mat, d.nummat);

@@ -263,6 +263,7 @@ __global__ __launch_bounds__(256) void k_fill_cand(DevState d, int cur) {
     const int nsorted = d.cell_start[g.ncell - 1] + d.cell_count[g.ncell - 1];
     if (s >= nsorted) return;
     const int j = d.sorted[s];
+    d.spos[j] = s;
     double2* cd = reinterpret_cast<double2*>(d.cand) + 2 * (long)s;
     cd[0] = make_double2(d.x0[0][j], d.x0[1][j]);
     cd[1] = make_double2(d.x0[2][j], d.hbuf[cur][j]);
@@ -465,10 +466,12 @@ __global__ __launch_bounds__(256) void k_list(DevState d, int cur) {
         }
         const double2* cd = reinterpret_cast<const double2*>(d.cand);
         const int2* ci = reinterpret_cast<const int2*>(d.cand_id);
+        const int spos_i = d.spos[i];
         for (int pass = 0; pass < (MULTI ? 2 : 1); pass++) {
-            auto consider = [&](int2 idm, double2 pa, double2 pb) {
-                const int j = idm.x;
-                if (j == i) return;
+            // candidate at sorted slot s; with one material its id is read
+            // only when it is accepted (the self test uses the sorted slot)
+            auto consider = [&](int s, double2 pa, double2 pb, int2 idm) {
+                if (MULTI ? idm.x == i : s == spos_i) return;
                 double dx = pa.x - xi;
                 double dist_sq = dx * dx;   // 0.0 + dx*dx is dx*dx exactly
                 double dy = pa.y - yi; dist_sq = dist_sq + dy * dy;
@@ -478,11 +481,11 @@ __global__ __launch_bounds__(256) void k_list(DevState d, int cur) {
                     const bool same = !MULTI || idm.y == mi;
                     if (pass == 0 && same) {
                         mind = fmin(mind, dist_sq);
-                        if (cnt < ncap) out[(long)cnt * 64] = j;
+                        if (cnt < ncap) out[(long)cnt * 64] = MULTI ? idm.x : ci[s].x;
                         cnt++;
                     } else if (MULTI && pass == 1 && !same) {
                         mind_cp = fmin(mind_cp, dist_sq);
-                        if (cnt < ncap) out[(long)cnt * 64] = j;
+                        if (cnt < ncap) out[(long)cnt * 64] = idm.x;
                         cnt++;
                     }
                 }
@@ -496,14 +499,14 @@ __global__ __launch_bounds__(256) void k_list(DevState d, int cur) {
                     double2 pa[4], pb[4];
 #pragma unroll
                     for (int u = 0; u < 4; u++) {
-                        idm[u] = ci[s + u];
+                        if (MULTI) idm[u] = ci[s + u];
                         pa[u] = cd[2 * (s + u)];
                         pb[u] = cd[2 * (s + u) + 1];
                     }
 #pragma unroll
-                    for (int u = 0; u < 4; u++) consider(idm[u], pa[u], pb[u]);
+                    for (int u = 0; u < 4; u++) consider(s + u, pa[u], pb[u], idm[u]);
                 }
-                for (; s < en; s++) consider(ci[s], cd[2 * s], cd[2 * s + 1]);
+                for (; s < en; s++) consider(s, cd[2 * s], cd[2 * s + 1], MULTI ? ci[s] : make_int2(0, 0));
             };
             for (int n = lo2; n <= hi2; n++) {
                 const double fz = n < b2 ? flo[2] : (n > b2 ? fhi[2] : 0.0);
