@@ -151,6 +151,17 @@ __global__ __launch_bounds__(256) void k_bin(DevState d, int cur) {
         if (in) {
             cell = ((box[2] - 1) * g.lim[1] + (box[1] - 1)) * g.lim[0] + (box[0] - 1);
             hb = h;
+        } else {
+            // outside the grid: no neighbours, only itself in corrected SPH
+            // (mcm_ll_neighbours.f90:109, 288-299)
+            int cnt = 0;
+            if (d.disctype == 1) {
+                nbr_list(d, i)[0] = i;
+                cnt = 1;
+            }
+            d.nreal[i] = cnt;
+            d.ntot[i] = cnt;
+            d.mindist[i] = 1.0e20;
         }
         d.cell_of[i] = cell;
     }
@@ -431,14 +442,19 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_ghost_fill(DevState d, int pi, in
 // and mindist unchanged.
 template <bool MULTI>
 __global__ __launch_bounds__(256) void k_list(DevState d, int cur) {
-    int i = xcd_block() * blockDim.x + threadIdx.x;
+    // threads take the binned particles in cell-sorted order, so that the
+    // lanes of a wavefront share cells and read the same candidates
+    const int t = xcd_block() * blockDim.x + threadIdx.x;
     Scalars* sc = d.sc;
-    if (i >= d.np || sc->abort) return;
+    if (sc->abort) return;
     const GridDev g = sc->grid;
+    const bool live = g.ncell > 0 && t < d.cell_start[g.ncell - 1] + d.cell_count[g.ncell - 1];
+    int cnt = 0;
+    if (live) {
+    const int i = d.sorted[t];
     const int L0 = g.lim[0], L1 = g.lim[1];
     const int ncap = d.ncap;
     int* out = nbr_list(d, i);
-    int cnt = 0;
     double mind = 1.0e20, mind_cp = 1.0e20;
     const int c = d.cell_of[i];
     int nreal;
@@ -466,7 +482,7 @@ __global__ __launch_bounds__(256) void k_list(DevState d, int cur) {
         }
         const double2* cd = reinterpret_cast<const double2*>(d.cand);
         const int2* ci = reinterpret_cast<const int2*>(d.cand_id);
-        const int spos_i = d.spos[i];
+        const int spos_i = t;
         for (int pass = 0; pass < (MULTI ? 2 : 1); pass++) {
             // candidate at sorted slot s; with one material its id is read
             // only when it is accepted (the self test uses the sorted slot)
@@ -587,9 +603,14 @@ __global__ __launch_bounds__(256) void k_list(DevState d, int cur) {
     d.nreal[i] = nreal;
     d.ntot[i] = cnt;
     d.mindist[i] = mind;
-    atomicMax(&sc->max_count, cnt);
-    if (cnt > ncap) {
-        atomicOr(&sc->error_flags, 1);
-        atomicExch(&sc->abort, 1);
+    }
+    // longest list of the wavefront, one atomic per wavefront
+    for (int off = 32; off > 0; off >>= 1) cnt = max(cnt, __shfl_xor(cnt, off, 64));
+    if ((threadIdx.x & 63) == 0 && cnt > 0) {
+        atomicMax(&sc->max_count, cnt);
+        if (cnt > d.ncap) {
+            atomicOr(&sc->error_flags, 1);
+            atomicExch(&sc->abort, 1);
+        }
     }
 }
