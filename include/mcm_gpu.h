@@ -57,9 +57,16 @@ typedef struct {
     int32_t reserved[2];
 } mcm_gpu_scalars;
 
-/* Create a context from a binary deck (format: mcmtop_amd/deck.py).
-   Replaces the deck reading of mcm_getinput.f90:90-128 for the GPU path. */
+/* Create a context from an input deck: the reference's version-2 text deck
+   (the cards read by mcm_getinput.f90:89-127: mcm_control_v2, mcm_matin,
+   mcm_nodein, mcm_inputvel, mcm_inp_symmetry_v2), or the binary deck of
+   mcmtop_amd/deck.py with the same content (told apart by its magic number).
+   Replaces the deck reading of mcm_getinput.f90 for the GPU path. */
 int mcm_gpu_create_from_deck(const char *deck_path, const mcm_gpu_options *opt, mcm_gpu_ctx **out);
+
+/* Convert a deck (text or binary) into the binary layout, values as written;
+   needs no GPU.  On failure a message is copied into msg[msg_len]. */
+int mcm_gpu_convert_deck(const char *in_path, const char *out_path, char *msg, int32_t msg_len);
 void mcm_gpu_destroy(mcm_gpu_ctx *ctx);
 const char *mcm_gpu_last_error(const mcm_gpu_ctx *ctx);
 

@@ -16,6 +16,7 @@
 
 #include "../../include/mcm_gpu.h"
 #include "mcm_wave.hip"
+#include "mcm_textdeck.h"
 
 namespace {
 
@@ -28,12 +29,16 @@ namespace {
         }                                                                                         \
     } while (0)
 
+// Deck content.  The readers fill it with the values as written (material
+// cards before the set-up routines, 1-based material ids); finish_deck then
+// applies the reference's defaults and material set-up.
 struct HostDeck {
     int np = 0, nummat = 0, disctype = 0, h_opt = 0, tcrit_opt = 0, contacttype = 0, krtype = 1;
     int maxnbr = 0, massopt = 0, init_h_opt = 0, init_rhoe = 0, mirr_opt = 0;
     int sym_flag[6] = {0, 0, 0, 0, 0, 0};
     double tssf = 0, itss = 0, endtime = 0, sym_x[6] = {0, 0, 0, 0, 0, 0};
     std::vector<MatParams> mat;
+    std::vector<double> mat_mass, mat_rho_min;   // material mass and rho_min cards
     std::vector<double> x, v, pm, ph, pr, pe;
     std::vector<int> pmat, dispbc;
 };
@@ -117,60 +122,38 @@ void setup_material(MatParams& m) {
     else if (m.model == 10) m.g = m.strinput[0];
 }
 
-int read_deck(mcm_gpu_ctx* ctx, const char* path) {
-    FILE* f = fopen(path, "rb");
-    if (!f) { ctx->err = std::string("cannot open deck ") + path; return MCM_GPU_ERR_IO; }
+// Defaults, material set-up and checks shared by both deck formats (mass
+// option 0 shares the material mass card equally).
+int finish_deck(mcm_gpu_ctx* ctx) {
     HostDeck& D = ctx->deck;
-    int32_t magic, ints[19];
-    double reals[9];
-    bool ok = rd(f, &magic, 1) && magic == kDeckMagic && rd(f, ints, 19) && rd(f, reals, 9);
-    if (!ok) { fclose(f); ctx->err = "not an MCM binary deck"; return MCM_GPU_ERR_IO; }
-    D.np = ints[0]; D.nummat = ints[1]; D.disctype = ints[2]; D.h_opt = ints[3]; D.tcrit_opt = ints[4];
-    D.contacttype = ints[5]; D.krtype = ints[6]; D.maxnbr = ints[7]; D.massopt = ints[8];
-    D.init_h_opt = ints[9]; D.init_rhoe = ints[10]; D.mirr_opt = ints[11];
-    for (int k = 0; k < 6; k++) D.sym_flag[k] = ints[12 + k];
-    D.tssf = reals[0]; D.itss = reals[1]; D.endtime = reals[2];
-    for (int k = 0; k < 6; k++) D.sym_x[k] = reals[3 + k];
-    if (D.tssf == 0.0) D.tssf = 0.8;   // mcm_control_v2.f90:193
-    if (D.np <= 0 || D.nummat <= 0 || D.nummat > MCM_MAX_MAT) {
-        fclose(f); ctx->err = "deck: bad particle or material count"; return MCM_GPU_ERR_ARG;
-    }
-    D.mat.assign(D.nummat, MatParams{});
-    std::vector<double> mat_mass(D.nummat, 0.0);
+    const std::vector<double>& mat_mass = D.mat_mass;
+    // control defaults, mcm_control_v2.f90:193-196
+    if (D.tssf == 0.0) D.tssf = 0.8;
+    if (D.krtype == 0) D.krtype = 1;
+    if (D.maxnbr == 0) D.maxnbr = 40;
     for (int m = 0; m < D.nummat; m++) {
-        int32_t mi[3];
-        double mr[7];
         MatParams& M = D.mat[m];
-        if (!(rd(f, mi, 3) && rd(f, mr, 7) && rd(f, M.strinput, 48) && rd(f, M.eosinput, 24))) {
-            fclose(f); ctx->err = "deck truncated in material cards"; return MCM_GPU_ERR_IO;
-        }
-        M.model = mi[0]; M.eos = mi[1]; M.visc_type = mi[2];
-        M.rho = mr[0]; M.av_q = mr[1]; M.av_l = mr[2]; M.h = mr[4];
-        M.rho_max = mr[6] == 0.0 ? 1.0E+10 : mr[6];
+        // material card defaults, mcm_matin.f90:46-56
+        if (M.visc_type == 0) M.visc_type = 1;
+        if (M.av_l == 0.0) M.av_l = 0.06;
+        if (M.av_q == 0.0) M.av_q = 1.5;
+        if (M.rho_max == 0.0) M.rho_max = 1.0E+10;
         M.pad_ = 0;
         M.efps_flag = 0; M.ispall = 0;
-        mat_mass[m] = mr[3];
-        setup_material(M);
         if (M.model == 9 || M.model == 10) {
             if (M.eos != 1 && M.eos != 4 && M.eos != 13 && M.eos != 28 && M.eos != 41) {
-                fclose(f); ctx->err = "unsupported equation of state"; return MCM_GPU_ERR_MODEL;
+                ctx->err = "unsupported equation of state"; return MCM_GPU_ERR_MODEL;
             }
         } else if (M.model == 1 || M.model == 3) {
-            if (M.eos != 0) { fclose(f); ctx->err = "models 1/3 take no EOS (mcm_matin.f90:72-79)"; return MCM_GPU_ERR_MODEL; }
+            if (M.eos != 0) { ctx->err = "models 1/3 take no EOS (mcm_matin.f90:72-79)"; return MCM_GPU_ERR_MODEL; }
         } else {
-            fclose(f); ctx->err = "material model not supported on the GPU path (1, 3, 9, 10 are)";
+            ctx->err = "material model not supported on the GPU path (1, 3, 9, 10 are)";
             return MCM_GPU_ERR_MODEL;
         }
-        if (M.visc_type != 1) { fclose(f); ctx->err = "only nodal viscosity (visc_type 1) exists"; return MCM_GPU_ERR_MODEL; }
+        setup_material(M);
+        if (M.visc_type != 1) { ctx->err = "only nodal viscosity (visc_type 1) exists"; return MCM_GPU_ERR_MODEL; }
     }
-    long n = D.np;
-    D.x.resize(3 * n); D.v.resize(3 * n); D.pmat.resize(n); D.dispbc.resize(n);
-    D.pm.resize(n); D.ph.resize(n); D.pr.resize(n); D.pe.resize(n);
-    ok = rd(f, D.x.data(), 3 * n) && rd(f, D.v.data(), 3 * n) && rd(f, D.pmat.data(), n) &&
-         rd(f, D.dispbc.data(), n) && rd(f, D.pm.data(), n) && rd(f, D.ph.data(), n) &&
-         rd(f, D.pr.data(), n) && rd(f, D.pe.data(), n);
-    fclose(f);
-    if (!ok) { ctx->err = "deck truncated in particle data"; return MCM_GPU_ERR_IO; }
+    const long n = D.np;
     for (long i = 0; i < n; i++) {
         D.pmat[i] -= 1;
         if (D.pmat[i] < 0 || D.pmat[i] >= D.nummat) { ctx->err = "particle with bad material id"; return MCM_GPU_ERR_ARG; }
@@ -185,6 +168,266 @@ int read_deck(mcm_gpu_ctx* ctx, const char* path) {
     if (D.krtype != 1 && D.krtype != 0) { ctx->err = "only the cubic B-spline kernel exists"; return MCM_GPU_ERR_ARG; }
     if (D.disctype != 0 && D.disctype != 1) { ctx->err = "disctype must be 0 or 1"; return MCM_GPU_ERR_ARG; }
     return MCM_GPU_OK;
+}
+
+// binary deck written by mcmtop_amd/deck.py (Deck.write_bin)
+int read_binary_deck(mcm_gpu_ctx* ctx, FILE* f) {
+    HostDeck& D = ctx->deck;
+    int32_t magic, ints[19];
+    double reals[9];
+    bool ok = rd(f, &magic, 1) && magic == kDeckMagic && rd(f, ints, 19) && rd(f, reals, 9);
+    if (!ok) { ctx->err = "not an MCM binary deck"; return MCM_GPU_ERR_IO; }
+    D.np = ints[0]; D.nummat = ints[1]; D.disctype = ints[2]; D.h_opt = ints[3]; D.tcrit_opt = ints[4];
+    D.contacttype = ints[5]; D.krtype = ints[6]; D.maxnbr = ints[7]; D.massopt = ints[8];
+    D.init_h_opt = ints[9]; D.init_rhoe = ints[10]; D.mirr_opt = ints[11];
+    for (int k = 0; k < 6; k++) D.sym_flag[k] = ints[12 + k];
+    D.tssf = reals[0]; D.itss = reals[1]; D.endtime = reals[2];
+    for (int k = 0; k < 6; k++) D.sym_x[k] = reals[3 + k];
+    if (D.np <= 0 || D.nummat <= 0 || D.nummat > MCM_MAX_MAT) {
+        ctx->err = "deck: bad particle or material count"; return MCM_GPU_ERR_ARG;
+    }
+    D.mat.assign(D.nummat, MatParams{});
+    D.mat_mass.assign(D.nummat, 0.0);
+    D.mat_rho_min.assign(D.nummat, 0.0);
+    for (int m = 0; m < D.nummat; m++) {
+        int32_t mi[3];
+        double mr[7];
+        MatParams& M = D.mat[m];
+        if (!(rd(f, mi, 3) && rd(f, mr, 7) && rd(f, M.strinput, 48) && rd(f, M.eosinput, 24))) {
+            ctx->err = "deck truncated in material cards"; return MCM_GPU_ERR_IO;
+        }
+        M.model = mi[0]; M.eos = mi[1]; M.visc_type = mi[2];
+        M.rho = mr[0]; M.av_q = mr[1]; M.av_l = mr[2]; M.h = mr[4];
+        M.rho_max = mr[6];
+        D.mat_mass[m] = mr[3];
+        D.mat_rho_min[m] = mr[5];
+    }
+    long n = D.np;
+    D.x.resize(3 * n); D.v.resize(3 * n); D.pmat.resize(n); D.dispbc.resize(n);
+    D.pm.resize(n); D.ph.resize(n); D.pr.resize(n); D.pe.resize(n);
+    ok = rd(f, D.x.data(), 3 * n) && rd(f, D.v.data(), 3 * n) && rd(f, D.pmat.data(), n) &&
+         rd(f, D.dispbc.data(), n) && rd(f, D.pm.data(), n) && rd(f, D.ph.data(), n) &&
+         rd(f, D.pr.data(), n) && rd(f, D.pe.data(), n);
+    if (!ok) { ctx->err = "deck truncated in particle data"; return MCM_GPU_ERR_IO; }
+    return MCM_GPU_OK;
+}
+
+// the reference's version-2 text deck (mcm_getinput.f90:89-127)
+int read_text_deck(mcm_gpu_ctx* ctx, FILE* f) {
+    using textdeck::Fmt;
+    HostDeck& D = ctx->deck;
+    textdeck::Cards C;
+    C.f = f;
+    std::string card;
+    auto bad = [&](const std::string& what, int code = MCM_GPU_ERR_IO) {
+        ctx->err = "text deck, line " + std::to_string(C.line) + ": " + what;
+        return code;
+    };
+    auto next = [&](const char* what) { return C.next(card) ? 0 : bad(std::string("missing ") + what); };
+    int rc;
+    // title and format version, (a78, i2)
+    if ((rc = next("title card"))) return rc;
+    { Fmt F(card); F.skip(78); long vern = F.i(2); if (!F.ok || vern != 2) return bad("only input format version 2 is read"); }
+    // control cards (mcm_control_v2.f90:86-124)
+    long axopt, thnode, ntrans, init_v_opt, veloc_opt, nlcur, nthp[3];
+    double drelax;
+    if ((rc = next("control card 1"))) return rc;
+    { Fmt F(card); axopt = F.i(5); D.disctype = F.i(5); D.nummat = F.i(10); D.np = F.i(10);
+      F.i(10); F.i(10); F.i(10); if (!F.ok) return bad("control card 1"); }
+    if ((rc = next("control card 2"))) return rc;
+    { Fmt F(card); D.endtime = F.e(10); D.tssf = F.e(10); D.itss = F.e(10); drelax = F.e(10);
+      if (!F.ok) return bad("control card 2"); }
+    if ((rc = next("control card 3"))) return rc;
+    { Fmt F(card); F.e(10); F.i(5); F.e(10); thnode = F.i(10); ntrans = F.i(5); F.i(10); F.i(5); F.i(5);
+      if (!F.ok) return bad("control card 3"); }
+    if ((rc = next("control card 4"))) return rc;
+    { Fmt F(card); F.i(5); init_v_opt = F.i(5); D.massopt = F.i(5); D.init_h_opt = F.i(5); D.init_rhoe = F.i(5);
+      if (!F.ok) return bad("control card 4"); }
+    if ((rc = next("control card 5"))) return rc;
+    { Fmt F(card); D.contacttype = F.i(5); D.tcrit_opt = F.i(5); veloc_opt = F.i(5); D.mirr_opt = F.i(5);
+      nlcur = F.i(5); for (int k = 0; k < 3; k++) nthp[k] = F.i(5); if (!F.ok) return bad("control card 5"); }
+    if ((rc = next("control card 6"))) return rc;
+    { Fmt F(card); D.h_opt = F.i(5); D.krtype = F.i(5); D.maxnbr = F.i(5); if (!F.ok) return bad("control card 6"); }
+    if ((rc = next("control card 7"))) return rc;
+    if (axopt != 3) return bad("only 3-D problems (axopt 3) run on the GPU path", MCM_GPU_ERR_ARG);
+    if (drelax != 0.0) return bad("dynamic relaxation is not on the GPU path", MCM_GPU_ERR_ARG);
+    if (veloc_opt != 0) return bad("XSPH (veloc_opt 1) is not on the GPU path", MCM_GPU_ERR_ARG);
+    if (nlcur != 0 || nthp[0] || nthp[1] || nthp[2]) return bad("load curves / base acceleration are not on the GPU path", MCM_GPU_ERR_ARG);
+    if (D.contacttype != 0) return bad("repulsive contact (contacttype > 0) is not on the GPU path", MCM_GPU_ERR_ARG);
+    if (D.np <= 0 || D.nummat <= 0 || D.nummat > MCM_MAX_MAT) return bad("bad particle or material count", MCM_GPU_ERR_ARG);
+    // material cards (mcm_matin.f90:38-203)
+    D.mat.assign(D.nummat, MatParams{});
+    D.mat_mass.assign(D.nummat, 0.0);
+    D.mat_rho_min.assign(D.nummat, 0.0);
+    std::vector<char> seen(D.nummat, 0);
+    for (int m = 0; m < D.nummat; m++) {
+        if ((rc = next("material control card"))) return rc;
+        long n, model, eos, visc;
+        double rho, av_q, av_l;
+        { Fmt F(card); n = F.i(5); model = F.i(5); rho = F.e(10); eos = F.i(5); F.i(5); F.e(10); visc = F.i(5);
+          av_q = F.e(10); av_l = F.e(10); if (!F.ok) return bad("material control card"); }
+        if (n < 1 || n > D.nummat || seen[n - 1]) return bad("material number out of range", MCM_GPU_ERR_ARG);
+        seen[n - 1] = 1;
+        MatParams& M = D.mat[n - 1];
+        M.model = (int)model; M.eos = (int)eos; M.rho = rho;
+        M.visc_type = (int)visc; M.av_l = av_l; M.av_q = av_q;
+        if ((rc = next("material options card"))) return rc;
+        { Fmt F(card); D.mat_mass[n - 1] = F.e(10); M.h = F.e(10); D.mat_rho_min[n - 1] = F.e(10);
+          M.rho_max = F.e(10); if (!F.ok) return bad("material options card"); }
+        if ((rc = next("material title card"))) return rc;
+        int per_card;
+        if (model == 1 || model == 3 || model == 4 || model == 9) per_card = 5;
+        else if (model == 10) per_card = 8;
+        else return bad("material model not implemented", MCM_GPU_ERR_MODEL);
+        for (int j = 0; j < 6; j++) {
+            if ((rc = next("material property card"))) return rc;
+            Fmt F(card);
+            for (int k = 0; k < per_card; k++) M.strinput[per_card * j + k] = F.e(10);
+            if (!F.ok) return bad("material property card");
+        }
+        if (eos != 0) {
+            if ((rc = next("eos title card"))) return rc;
+            // values per card: eos 1 reads 8 then 1 (mcm_matin.f90:177-185)
+            std::vector<int> cards;
+            if (eos == 1) cards = {8, 1};
+            else if (eos == 4 || eos == 41) cards = {8};
+            else if (eos == 13) cards = {5};
+            else if (eos == 28) cards = {2};
+            else return bad("eos type not implemented", MCM_GPU_ERR_MODEL);
+            int at = 0;
+            for (int cnt : cards) {
+                if ((rc = next("eos property card"))) return rc;
+                Fmt F(card);
+                for (int k = 0; k < cnt; k++) M.eosinput[at + k] = F.e(10);
+                if (!F.ok) return bad("eos property card");
+                at += cnt;
+            }
+        }
+    }
+    // particle cards (mcm_nodein.f90:34-125), (i8, i5, 3e20.0, i7)
+    const long np = D.np;
+    D.x.assign(3 * np, 0.0); D.v.assign(3 * np, 0.0); D.pmat.assign(np, 0); D.dispbc.assign(np, 0);
+    D.pm.assign(np, 0.0); D.ph.assign(np, 0.0); D.pr.assign(np, 0.0); D.pe.assign(np, 0.0);
+    for (long j = 0; j < np; j++) {
+        if ((rc = next("particle card"))) return rc;
+        Fmt F(card);
+        long id = F.i(8);
+        D.dispbc[j] = (int)F.i(5);
+        for (int k = 0; k < 3; k++) D.x[3 * j + k] = F.e(20);
+        D.pmat[j] = (int)F.i(7);
+        if (!F.ok) return bad("particle card");
+        if (id != j + 1) return bad("particle card out of order", MCM_GPU_ERR_ARG);
+        if (D.pmat[j] == 0) return bad("particle with zero material", MCM_GPU_ERR_ARG);
+    }
+    // additional particle data, (i8, 4e20.0) on an 80-column card
+    if (D.massopt == 1 || D.init_h_opt == 1 || D.init_rhoe == 1) {
+        for (long j = 0; j < np; j++) {
+            if ((rc = next("additional particle card"))) return rc;
+            Fmt F(card);
+            long id = F.i(8);
+            double v4[4];
+            for (int k = 0; k < 4; k++) v4[k] = F.e(20);
+            if (!F.ok) return bad("additional particle card");
+            if (id != j + 1) return bad("additional particle card out of order", MCM_GPU_ERR_ARG);
+            if (D.massopt == 1) D.pm[j] = v4[0];
+            if (D.init_h_opt == 1) D.ph[j] = v4[1];
+            if (D.init_rhoe == 1) { D.pr[j] = v4[2]; D.pe[j] = v4[3]; }
+        }
+    }
+    // time-history particles (10 ids a card) and pressure transducers: output
+    // only, read past
+    for (long k = 0; k < (thnode + 9) / 10; k++)
+        if ((rc = next("time history card"))) return rc;
+    for (long k = 0; k < ntrans; k++)
+        if ((rc = next("transducer card"))) return rc;
+    // initial velocities with the generation rule (mcm_Inputvel.f90:29-129),
+    // (i8, 3e10.0)
+    if (init_v_opt == 1) {
+        auto read_v = [&](long* nid, double* v) {
+            Fmt F(card);
+            *nid = F.i(8);
+            for (int k = 0; k < 3; k++) v[k] = F.e(10);
+            return F.ok;
+        };
+        long nid;
+        if ((rc = next("velocity card"))) return rc;
+        if (!read_v(&nid, &D.v[0])) return bad("velocity card");
+        if (nid != 1) return bad("first velocity card is not for particle 1", MCM_GPU_ERR_ARG);
+        bool gen = false;
+        for (long i = 1; i < np; i++) {
+            double* vi = &D.v[3 * i];
+            const double* vp = &D.v[3 * (i - 1)];
+            if (!gen) {
+                if ((rc = next("velocity card"))) return rc;
+                if (!read_v(&nid, vi)) return bad("velocity card");
+                if (nid > i + 1) {
+                    if (nid > np) return bad("velocity card id exceeds the particle count", MCM_GPU_ERR_ARG);
+                    for (int k = 0; k < 3; k++)
+                        if (vi[k] != vp[k]) return bad("velocity for generation is not constant", MCM_GPU_ERR_ARG);
+                    gen = true;
+                } else if (nid < i + 1) {
+                    return bad("velocity card id less than the previous card's", MCM_GPU_ERR_ARG);
+                }
+            } else {
+                for (int k = 0; k < 3; k++) vi[k] = vp[k];
+                if (i + 1 == nid) gen = false;
+            }
+        }
+    }
+    // boundary planes (mcm_inp_symmetry_v2.f90:23-98), (6i10) and (6e10.0)
+    for (int k = 0; k < 6; k++) { D.sym_flag[k] = 0; D.sym_x[k] = 0.0; }
+    if (D.mirr_opt == 1) {
+        if ((rc = next("boundary plane flag card"))) return rc;
+        { Fmt F(card); for (int k = 0; k < 6; k++) D.sym_flag[k] = (int)F.i(10); if (!F.ok) return bad("boundary plane flag card"); }
+        if ((rc = next("boundary plane coordinate card"))) return rc;
+        { Fmt F(card); for (int k = 0; k < 6; k++) D.sym_x[k] = F.e(10); if (!F.ok) return bad("boundary plane coordinate card"); }
+    }
+    return MCM_GPU_OK;
+}
+
+// A deck is the binary format when it starts with its magic number, else the
+// reference's text format.  Values as written (no defaults applied).
+int read_deck_raw(mcm_gpu_ctx* ctx, const char* path) {
+    FILE* f = fopen(path, "rb");
+    if (!f) { ctx->err = std::string("cannot open deck ") + path; return MCM_GPU_ERR_IO; }
+    int32_t magic = 0;
+    const bool binary = fread(&magic, 4, 1, f) == 1 && magic == kDeckMagic;
+    rewind(f);
+    const int rc = binary ? read_binary_deck(ctx, f) : read_text_deck(ctx, f);
+    fclose(f);
+    return rc;
+}
+
+int read_deck(mcm_gpu_ctx* ctx, const char* path) {
+    int rc = read_deck_raw(ctx, path);
+    return rc ? rc : finish_deck(ctx);
+}
+
+// the binary deck layout of Deck.write_bin (mcmtop_amd/deck.py)
+int write_binary_deck(const HostDeck& D, const char* path) {
+    FILE* f = fopen(path, "wb");
+    if (!f) return MCM_GPU_ERR_IO;
+    const int32_t magic = kDeckMagic;
+    int32_t ints[19] = {D.np, D.nummat, D.disctype, D.h_opt, D.tcrit_opt, D.contacttype, D.krtype, D.maxnbr,
+                        D.massopt, D.init_h_opt, D.init_rhoe, D.mirr_opt, 0, 0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < 6; k++) ints[12 + k] = D.sym_flag[k];
+    double reals[9] = {D.tssf, D.itss, D.endtime, 0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < 6; k++) reals[3 + k] = D.sym_x[k];
+    bool ok = fwrite(&magic, 4, 1, f) == 1 && fwrite(ints, 4, 19, f) == 19 && fwrite(reals, 8, 9, f) == 9;
+    for (int m = 0; m < D.nummat && ok; m++) {
+        const MatParams& M = D.mat[m];
+        int32_t mi[3] = {M.model, M.eos, M.visc_type};
+        double mr[7] = {M.rho, M.av_q, M.av_l, D.mat_mass[m], M.h, D.mat_rho_min[m], M.rho_max};
+        ok = fwrite(mi, 4, 3, f) == 3 && fwrite(mr, 8, 7, f) == 7 && fwrite(M.strinput, 8, 48, f) == 48 &&
+             fwrite(M.eosinput, 8, 24, f) == 24;
+    }
+    const size_t n = D.np;
+    ok = ok && fwrite(D.x.data(), 8, 3 * n, f) == 3 * n && fwrite(D.v.data(), 8, 3 * n, f) == 3 * n &&
+         fwrite(D.pmat.data(), 4, n, f) == n && fwrite(D.dispbc.data(), 4, n, f) == n &&
+         fwrite(D.pm.data(), 8, n, f) == n && fwrite(D.ph.data(), 8, n, f) == n && fwrite(D.pr.data(), 8, n, f) == n &&
+         fwrite(D.pe.data(), 8, n, f) == n;
+    ok = fclose(f) == 0 && ok;
+    return ok ? MCM_GPU_OK : MCM_GPU_ERR_IO;
 }
 
 // ------------------------------------------------------------- allocation
@@ -589,6 +832,21 @@ int check_sticky(mcm_gpu_ctx* ctx) {
 
 // =============================================================== C ABI
 extern "C" {
+
+int mcm_gpu_convert_deck(const char* in_path, const char* out_path, char* msg, int32_t msg_len) {
+    if (!in_path || !out_path) return MCM_GPU_ERR_ARG;
+    mcm_gpu_ctx tmp;
+    int rc = read_deck_raw(&tmp, in_path);
+    if (rc == 0) {
+        rc = write_binary_deck(tmp.deck, out_path);
+        if (rc) tmp.err = std::string("cannot write ") + out_path;
+    }
+    if (msg && msg_len > 0) {
+        strncpy(msg, tmp.err.c_str(), (size_t)msg_len - 1);
+        msg[msg_len - 1] = 0;
+    }
+    return rc;
+}
 
 int mcm_gpu_create_from_deck(const char* deck_path, const mcm_gpu_options* opt, mcm_gpu_ctx** out) {
     if (!out || !deck_path) return MCM_GPU_ERR_ARG;

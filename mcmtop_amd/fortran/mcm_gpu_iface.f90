@@ -34,6 +34,12 @@ module mcm_gpu_iface
       type(mcm_gpu_options), intent(in) :: opt
       type(c_ptr), intent(out) :: ctx
     end function
+    integer(c_int) function mcm_gpu_convert_deck(path_in, path_out, msg, msg_len) bind(c, name='mcm_gpu_convert_deck')
+      import :: c_int, c_char, c_int32_t
+      character(kind=c_char), dimension(*), intent(in) :: path_in, path_out
+      character(kind=c_char), dimension(*), intent(out) :: msg
+      integer(c_int32_t), value :: msg_len
+    end function
     subroutine mcm_gpu_destroy(ctx) bind(c, name='mcm_gpu_destroy')
       import :: c_ptr
       type(c_ptr), value :: ctx

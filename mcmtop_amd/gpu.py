@@ -30,7 +30,7 @@ SYMBOLS = (
     "mcm_gpu_dt_update", "mcm_gpu_momentum", "mcm_gpu_boundcond", "mcm_gpu_calctotale",
     "mcm_gpu_update_velocity", "mcm_gpu_move", "mcm_gpu_calccritts", "mcm_gpu_step",
     "mcm_gpu_get_scalars", "mcm_gpu_download", "mcm_gpu_download_neighbours", "mcm_gpu_dump",
-    "mcm_gpu_time_steps",
+    "mcm_gpu_time_steps", "mcm_gpu_convert_deck",
 )
 
 
@@ -82,6 +82,7 @@ def load_library(path: str = LIB_PATH):
     lib.mcm_gpu_download_neighbours.argtypes = [vp, i32p, i32p, i32p, ctypes.c_int64, i32p]
     lib.mcm_gpu_dump.argtypes = [vp, ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32]
     lib.mcm_gpu_time_steps.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
+    lib.mcm_gpu_convert_deck.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int32]
     for name in SYMBOLS:
         if name not in ("mcm_gpu_destroy", "mcm_gpu_last_error"):
             getattr(lib, name).restype = ctypes.c_int
@@ -89,8 +90,19 @@ def load_library(path: str = LIB_PATH):
     return lib
 
 
+def convert_deck(in_path: str, out_path: str) -> None:
+    """Text (reference version 2) or binary deck -> binary deck, values as
+    written.  Runs on the host; no GPU needed."""
+    lib = load_library()
+    msg = ctypes.create_string_buffer(512)
+    rc = lib.mcm_gpu_convert_deck(in_path.encode(), out_path.encode(), msg, 512)
+    if rc != 0:
+        raise McmGpuError("convert_deck failed: %s (%s)" % (ERRORS.get(rc, rc), msg.value.decode()))
+
+
 class Solver:
-    """One MCM run on one GPU, created from a binary deck (mcmtop_amd/deck.py)."""
+    """One MCM run on one GPU, created from an input deck: the reference's
+    version-2 text deck or the binary deck of mcmtop_amd/deck.py."""
 
     def __init__(self, deck_path: str, nbr_capacity: int = 0, ghost_capacity: int = 0, device: int = 0,
                  corrected_one_pass: bool = False, pair_mapping: int = 0):

@@ -32,3 +32,22 @@ def test_fortran_host_runs_the_gpu_step(mode, tmp_path):
     od, gd = o.state(15, 10), Dump(dump)
     for k in ("x", "v", "sigma", "e", "rho"):
         assert np.linalg.norm(gd.f(k) - od.f(k)) <= 1e-10 * np.linalg.norm(od.f(k)), k
+
+
+def test_fortran_host_reads_the_reference_text_deck(tmp_path):
+    """The host passes the reference's own version-2 input deck to the
+    library, as mcm_getinput reads it."""
+    import cases
+    if not os.path.exists(EXE):
+        pytest.fail("Fortran host not built (__graft_entry__.build())")
+    txt = str(tmp_path / "bar.mcm")
+    cases.build("corr_f3dm10_eos4_hopt1").write_mcm(txt)
+    dump = str(tmp_path / "f.bin")
+    out = subprocess.run([EXE, txt, "15", "fused", dump], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    o = Oracle(os.path.join(DATA, "corr_f3dm10_eos4_hopt1.bin"))
+    o.initial()
+    o.run(15)
+    od, gd = o.state(15, 10), Dump(dump)
+    for k in ("x", "v", "sigma", "e", "rho"):
+        assert np.array_equal(gd.f(k), od.f(k)), k

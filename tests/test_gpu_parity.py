@@ -287,3 +287,16 @@ def test_fused_state_is_bit_exact(name, tmp_path):
     bad = [k for k in STATE if not np.array_equal(gd.f(k), od.f(k))]
     assert not bad, {k: rel(gd.f(k), od.f(k)) for k in bad}
     assert gd.ptime == od.ptime
+
+
+@pytest.mark.parametrize("name", ["corr_f3dm10_eos4_hopt1", "quarter_f3dm1_xyz_sym_tcrit2", "two_materials_contact"])
+def test_text_deck_runs_like_the_binary_deck(name, tmp_path):
+    """A run created from the reference's text deck equals the run from the
+    binary deck of the same content, bit for bit."""
+    txt = str(tmp_path / (name + ".mcm"))
+    cases.build(name).write_mcm(txt)
+    n = cases.CASES[name][0]
+    a = _fused_dump(os.path.join(DATA, name + ".bin"), tmp_path, n, 1)
+    b = _fused_dump(txt, tmp_path, n, 2)
+    np.testing.assert_array_equal(a.rows, b.rows)
+    np.testing.assert_array_equal(a.nbr, b.nbr)

@@ -1,0 +1,101 @@
+"""The reference's version-2 text deck read by the C library (host code, no
+GPU): every golden deck written as text (Deck.write_mcm) converts back to
+exactly the bytes of its binary deck (Deck.write_bin), and the Fortran field
+rules of the reader (mcm_gttxsg.f90, E and I edit descriptors) hold."""
+import os
+
+import numpy as np
+import pytest
+
+import cases
+from mcmtop_amd import deck, gpu
+
+DATA = os.path.join(os.path.dirname(__file__), "golden", "data")
+
+
+@pytest.mark.parametrize("name", sorted(cases.CASES))
+def test_text_deck_converts_to_the_binary_deck(name, tmp_path):
+    dk = cases.build(name)
+    txt = str(tmp_path / "d.mcm")
+    out = str(tmp_path / "d.bin")
+    dk.write_mcm(txt)
+    gpu.convert_deck(txt, out)
+    assert open(out, "rb").read() == open(os.path.join(DATA, name + ".bin"), "rb").read()
+
+
+def test_binary_deck_converts_to_itself(tmp_path):
+    src = os.path.join(DATA, "corr_f3dm10_eos4_hopt1.bin")
+    out = str(tmp_path / "d.bin")
+    gpu.convert_deck(src, out)
+    assert open(out, "rb").read() == open(src, "rb").read()
+
+
+def _small_deck():
+    pts = deck.block(4.0e-4, 2, 2, 2)
+    m = deck.material(3, 8930.0, 8930.0 * 4.0e-4 ** 3 * len(pts), 1.2 * 4.0e-4,
+                      deck.f3dm3(117e9, 0.35, 400e6, 100e6))
+    return deck.make_deck(pts, (0.0, 0.0, -190.0), [m])
+
+
+def _read_back(tmp_path, text):
+    txt = str(tmp_path / "e.mcm")
+    out = str(tmp_path / "e.bin")
+    with open(txt, "w") as f:
+        f.write(text)
+    gpu.convert_deck(txt, out)
+    return deck.Deck.read_bin(out)
+
+
+def test_fortran_field_rules(tmp_path):
+    """Comments, blank fields, D and sign-only exponents, columns past 80."""
+    base = str(tmp_path / "b.mcm")
+    _small_deck().write_mcm(base)
+    lines = open(base).read().split("\n")
+    ref = _read_back(tmp_path, "\n".join(lines))
+    # comment records anywhere are skipped
+    mod = list(lines)
+    mod.insert(1, "* a comment card")
+    mod.insert(5, "$ another one")
+    assert _read_back(tmp_path, "\n".join(mod)).x.tolist() == ref.x.tolist()
+    # exponent spellings of the first particle's x: 'D', sign-only, blank-padded
+    first = next(k for k, l in enumerate(mod) if l.startswith("       1") and len(l) > 60)
+    card = mod[first]
+    v = float(card[13:33])
+    for text in (("%.12e" % v).replace("e", "D"), ("%.12e" % v).replace("e", ""), " %.12e " % v):
+        field = text.rjust(20)[:20]
+        mod2 = list(mod)
+        mod2[first] = card[:13] + field + card[33:]
+        got = _read_back(tmp_path, "\n".join(mod2)).x[0, 0]
+        assert got == float(("%.12e" % v)), (text, got)
+    # characters past column 80 are not part of the card
+    mod3 = list(mod)
+    mod3[first] = card.ljust(80) + "999999"
+    assert _read_back(tmp_path, "\n".join(mod3)).x.tolist() == ref.x.tolist()
+
+
+def test_velocity_generation_rule(tmp_path):
+    """mcm_Inputvel.f90:60-118: a card for a later particle with the same
+    velocity as the previous card fills the particles between."""
+    dk = _small_deck()
+    base = str(tmp_path / "b.mcm")
+    dk.write_mcm(base)
+    lines = open(base).read().rstrip("\n").split("\n")
+    n = dk.np_
+    vel_start = len(lines) - n
+    assert lines[vel_start].startswith("       1")
+    gen = lines[:vel_start] + [lines[vel_start], lines[vel_start + n - 1]]
+    got = _read_back(tmp_path, "\n".join(gen) + "\n")
+    np.testing.assert_array_equal(got.v, np.repeat(dk.v[:1], n, axis=0))
+
+
+def test_unsupported_options_are_rejected(tmp_path):
+    dk = _small_deck()
+    base = str(tmp_path / "b.mcm")
+    dk.write_mcm(base)
+    lines = open(base).read().split("\n")
+    # control card 5, third field: XSPH
+    lines[5] = lines[5][:10] + "%5d" % 1 + lines[5][15:]
+    bad = str(tmp_path / "x.mcm")
+    open(bad, "w").write("\n".join(lines))
+    with pytest.raises(gpu.McmGpuError, match="XSPH"):
+        gpu.convert_deck(bad, str(tmp_path / "x.bin"))
