@@ -10,6 +10,13 @@ ghosts, strain, update_h, constitutive + EOS + AV + energy, dt, momentum,
 boundary conditions, energy totals, velocity and position update), with
 inputs resident in HBM; dt/ptime advance on the device.
 
+Batched form (the "batched vs plain" part of the metric): 256 independent
+bars of uneven size (27 k to 200 k particles, every parameter drawn from one
+seed, mcmtop_amd.deck.batch_bars) stepped together in one launch set
+(mcm_gpu_batch_step), against the same bars stepped one at a time, each with
+its own captured step (mcm_gpu_step).  A sample of the runs is compared with
+its run alone after the same steps.
+
 Prints one JSON line (see the keys below).  --gpus N > 1 exits: the build
 runs on one GPU.
 """
@@ -106,6 +113,61 @@ def cpu_baseline(tmpdir: str, dx: float):
                       % (d.np_, steps, "flang -O2 build of the reference" if kind == "reference" else "C oracle")}
 
 
+def batched_leg(nruns: int, seed: int, steps: int, warmup: int, check_every: int):
+    """Throughput of nruns uneven bars stepped as one batch and stepped one
+    run at a time; equality of sampled runs between the two."""
+    import numpy as np
+    from mcmtop_amd import deck, gpu
+    t = time.perf_counter()
+    decks, params = deck.batch_bars(nruns, seed=seed)
+    sizes = np.array([p["np"] for p in params])
+    total = int(sizes.sum())
+    print("bench: %d batch decks, %d particles in %.1f s" % (nruns, total, time.perf_counter() - t),
+          file=sys.stderr, flush=True)
+    t = time.perf_counter()
+    b = gpu.Batch(decks)
+    b.initial()
+    print("bench: batch initialised in %.1f s" % (time.perf_counter() - t), file=sys.stderr, flush=True)
+    b.step(warmup)
+    t0 = time.perf_counter()
+    b.step(steps)
+    secs_b = time.perf_counter() - t0
+    ms_b = b.time_steps(2)
+    print("bench: batched %.2f ms/step" % (1e3 * secs_b / steps), file=sys.stderr, flush=True)
+    sample = list(range(0, nruns, check_every))
+    got = {k: (b.run(k).field(gpu.F_X), b.run(k).field(gpu.F_V), b.run(k).field(gpu.F_SIGMA)) for k in sample}
+    b.close()
+    secs_p = 0.0
+    worst = 0.0
+    for k, dk in enumerate(decks):
+        with gpu.Solver(dk) as s:
+            s.initial()
+            s.step(warmup)
+            t0 = time.perf_counter()
+            s.step(steps)
+            secs_p += time.perf_counter() - t0
+            if k in got:
+                # the batch ran 2 more steps (its timing call): catch up
+                s.step(2)
+                for a, ref in zip(got[k], (s.field(gpu.F_X), s.field(gpu.F_V), s.field(gpu.F_SIGMA))):
+                    nb = np.linalg.norm(ref)
+                    worst = max(worst, float(np.linalg.norm(a - ref) / nb) if nb > 0 else float(np.linalg.norm(a)))
+        if k % 32 == 31:
+            print("bench: plain runs %d/%d" % (k + 1, nruns), file=sys.stderr, flush=True)
+    v_b = total * steps / secs_b
+    v_p = total * steps / secs_p
+    return {"runs": nruns, "particles": total, "seed": seed,
+            "run_particles": {"min": int(sizes.min()), "median": float(np.median(sizes)), "max": int(sizes.max())},
+            "steps": steps, "warmup": warmup,
+            "value": v_b, "unit": "particle-steps/s", "ms_per_step": 1e3 * secs_b / steps,
+            "plain_value": v_p, "plain_ms_per_step": 1e3 * secs_p / steps, "speedup": v_b / v_p,
+            "checked_runs": len(sample), "max_rel_l2_vs_plain": worst,
+            "kernel_ms": {"neighbours": ms_b[0], "strain_pass": ms_b[1], "constitutive": ms_b[2],
+                          "momentum": ms_b[3], "step": ms_b[4]},
+            "workload": "taylor_bar_corrected_f3dm10_eqos4_hopt1 x %d, r U[2,5] mm, L/D U[3,8], dx U[0.05,0.2] mm, "
+                        "v0 U[100,250] m/s, sigma_y U[200,600] MPa, E U[70,210] GPa" % nruns}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,6 +175,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--dx", type=float, default=7.3e-5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch", type=int, default=256, help="runs of the batched leg (0: skip it)")
+    ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--batch-seed", type=int, default=3)
     ap.add_argument("--corrected-one-pass", action="store_true",
                     help="accumulate the corrected kernels' three passes in one (rounding differs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -167,7 +232,7 @@ def main():
     achieved = bytes_launch / t / 1e9
     traffic = measured_traffic(K["symbol"])
     res = {
-        "metric": "FP64 particle-steps/s (plain per-instance launch)",
+        "metric": "FP64 particle-steps/s, batched vs plain @\u22641e-7 L2; adjoint/forward time ratio",
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,
@@ -192,6 +257,9 @@ def main():
                               "frac": flops_launch / t / 1e12 / FP64_PEAK_TFLOPS}},
     }
     s.close()
+    if args.batch > 0:
+        res["batched"] = batched_leg(args.batch, args.batch_seed, args.batch_steps, 4, 16)
+    res["adjoint_forward_ratio"] = None
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(tmpdir, args.dx)
     print(json.dumps(res), flush=True)

@@ -22,6 +22,7 @@ extern "C" {
 #endif
 
 typedef struct mcm_gpu_ctx mcm_gpu_ctx;
+typedef struct mcm_gpu_batch mcm_gpu_batch;
 
 /* status codes */
 #define MCM_GPU_OK 0
@@ -63,6 +64,9 @@ typedef struct {
    mcmtop_amd/deck.py with the same content (told apart by its magic number).
    Replaces the deck reading of mcm_getinput.f90 for the GPU path. */
 int mcm_gpu_create_from_deck(const char *deck_path, const mcm_gpu_options *opt, mcm_gpu_ctx **out);
+/* The same from binary deck bytes held in memory (data[len], the layout of
+   Deck.write_bin / Deck.to_bytes in mcmtop_amd/deck.py). */
+int mcm_gpu_create_from_memory(const void *data, int64_t len, const mcm_gpu_options *opt, mcm_gpu_ctx **out);
 
 /* Convert a deck (text or binary) into the binary layout, values as written;
    needs no GPU.  On failure a message is copied into msg[msg_len]. */
@@ -134,6 +138,34 @@ int mcm_gpu_dump(mcm_gpu_ctx *ctx, const char *path, int32_t step, int32_t stage
    kernel (with boundcond, velocity and move; without the ghost-stress
    refresh and the energy fold), whole step. */
 int mcm_gpu_time_steps(mcm_gpu_ctx *ctx, int32_t nsteps, double *ms);
+
+/* Batched runs.  Many independent runs (each its own deck: particle count,
+   shape, materials, boundary planes) stepped together: every launch of the
+   fused step covers all runs, one workgroup map from launch workgroup to
+   (run, block), captured as one two-step hipGraph.  Each run's arithmetic,
+   list order and scalars (dt, ptime, energies, bounds) are its own, so every
+   run gives the same result as when stepped alone with mcm_gpu_step.  The
+   reference runs one problem per process; this has no reference counterpart
+   beyond running mcm_solution.f90:41-126 for each deck.
+   Options apply to every run.  A run whose capacity overflows during a
+   batched step grows and finishes that call's steps on its own. */
+int mcm_gpu_batch_create(const mcm_gpu_options *opt, mcm_gpu_batch **out);
+/* Add a run from a deck file (text or binary) or from binary deck bytes. */
+int mcm_gpu_batch_add_deck(mcm_gpu_batch *b, const char *deck_path);
+int mcm_gpu_batch_add_memory(mcm_gpu_batch *b, const void *data, int64_t len);
+int32_t mcm_gpu_batch_size(const mcm_gpu_batch *b);
+/* Run k of the batch, owned by the batch (mcm_gpu_destroy ignores it): the
+   state entry points (scalars, downloads, dumps) read it.  Stepping it on its
+   own is allowed when it returns to the batch's step parity. */
+mcm_gpu_ctx *mcm_gpu_batch_run(mcm_gpu_batch *b, int32_t k);
+const char *mcm_gpu_batch_last_error(const mcm_gpu_batch *b);
+/* mcm_gpu_initial for every run */
+int mcm_gpu_batch_initial(mcm_gpu_batch *b);
+/* nsteps fused steps of every run */
+int mcm_gpu_batch_step(mcm_gpu_batch *b, int32_t nsteps);
+/* Per-stage times of nsteps batched steps, as mcm_gpu_time_steps. */
+int mcm_gpu_batch_time_steps(mcm_gpu_batch *b, int32_t nsteps, double *ms);
+void mcm_gpu_batch_destroy(mcm_gpu_batch *b);
 
 #ifdef __cplusplus
 }

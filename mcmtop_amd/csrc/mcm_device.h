@@ -64,11 +64,18 @@ struct PlaneDev {
     int dim, side, code, pad_;
 };
 
+// One workgroup of a batched launch: the instance it works for, its block
+// number within that instance and the instance's block count in the launch.
+struct BlockEntry {
+    int inst, block, nblocks, pad_;
+};
+
 // Device pointers of one run.  Arrays marked [U] span the unified index
 // space of np real particles followed by the ghost slots.
 struct DevState {
     int np, npad, ngcap, nummat, disctype, h_opt, tcrit_opt, boundary, ncap;
     int nplanes, cellcap, need_s, has_model9;
+    int nblk;                  // workgroups of MCM_BLOCK threads over np (per-particle launches)
     int bcode[3][2];
     int btype[3];
     double bx[3][2];
@@ -104,7 +111,37 @@ struct DevState {
     int* yield;                // active-set mask of the plastic models
     double* red;               // per-block partial reductions
     double* defps_prev;        // f3dm10 increment carried for failed particles
+    // batched launch (null for a launch over one run): workgroup -> instance
+    // block, and the states of all instances
+    const BlockEntry* bmap;
+    const DevState* states;
 };
+
+// The workgroup of this launch as one run sees it: block b of n.  A launch
+// over one run uses the grid itself (XCD: renumbered by xcd_block) and the
+// state passed as the kernel argument; a batched launch looks its workgroup
+// up in the block map and reads the state of the instance it works for.
+// Both states are read through the constant address space, so their fields
+// stay scalar loads.  Every kernel that a batched step launches calls this
+// first.
+struct Blk {
+    int b, n;
+};
+typedef const __attribute__((address_space(4))) DevState ConstDevState;
+template <bool XCD>
+__device__ __forceinline__ const DevState& enter(const DevState& d0, Blk& bk) {
+    (void)d0;   // the kernel's first argument, read below through the kernarg segment
+    const int g = XCD ? xcd_block() : (int)blockIdx.x;
+    ConstDevState* p = (ConstDevState*)__builtin_amdgcn_kernarg_segment_ptr();
+    if (p->bmap == nullptr) {
+        bk = Blk{g, (int)gridDim.x};
+    } else {
+        const BlockEntry e = p->bmap[g];
+        bk = Blk{e.block, e.nblocks};
+        p = (ConstDevState*)p->states + e.inst;
+    }
+    return *(const DevState*)p;
+}
 
 // Neighbour lists are blocked by 64 particles: for each block of 64
 // consecutive particles, slot k of all 64 is one contiguous 256-byte row, so

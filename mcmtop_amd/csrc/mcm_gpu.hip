@@ -45,6 +45,8 @@ struct HostDeck {
 
 }  // namespace
 
+struct mcm_gpu_batch;
+
 struct mcm_gpu_ctx {
     HostDeck deck;
     DevState d{};
@@ -60,6 +62,8 @@ struct mcm_gpu_ctx {
     std::vector<void*> allocs;
     std::string err;
     int device = 0;
+    bool own_stream = true;            // false for a run of a batch (the batch's stream)
+    mcm_gpu_batch* batch = nullptr;    // the batch this run belongs to, if any
 };
 
 namespace {
@@ -522,31 +526,19 @@ int alloc_all(mcm_gpu_ctx* ctx) {
 // ----------------------------------------------------------------- launches
 int blocks_for(long n, int bs) { return (int)((n + bs - 1) / bs); }
 
-void launch_neighbours(mcm_gpu_ctx* ctx, int cur) {
-    DevState& d = ctx->d;
-    hipStream_t s = ctx->stream;
-    int gcl = std::min(blocks_for(d.cellcap, MCM_BLOCK), 4096);
-    int ntile = blocks_for(d.cellcap, SCAN_TILE);
-    const int ghost_fill_blocks = std::max(1, std::min(blocks_for(d.ngcap, MCM_BLOCK), 512));
-    k_grid_setup<<<1, 1, 0, s>>>(d);
-    k_clear_cells<<<gcl, MCM_BLOCK, 0, s>>>(d);
-    k_bin<<<ctx->nb, MCM_BLOCK, 0, s>>>(d, cur);
-    k_scan_tiles<<<ntile, 1024, 0, s>>>(d);
-    k_scan_top<<<1, 1024, 0, s>>>(d);
-    k_scan_apply<<<ntile, 1024, 0, s>>>(d);
-    k_scatter<<<ctx->nb, MCM_BLOCK, 0, s>>>(d);
-    k_sort_cells<<<gcl, MCM_BLOCK, 0, s>>>(d);
-    k_fill_cand<<<ctx->nb, MCM_BLOCK, 0, s>>>(d, cur);
-    for (int p = 0; p < d.nplanes; p++) {
-        k_ghost_scan<<<1, 1024, 0, s>>>(d, p);
-        k_ghost_fill<<<ghost_fill_blocks, MCM_BLOCK, 0, s>>>(d, p, cur);
-    }
-    if (ctx->pair_mapping == 1) k_list_wave<<<ctx->nb, MCM_BLOCK, 0, s>>>(d, cur);
-    else if (d.nummat > 1) k_list<true><<<ctx->nb, MCM_BLOCK, 0, s>>>(d, cur);
-    else k_list<false><<<ctx->nb, MCM_BLOCK, 0, s>>>(d, cur);
+// Launch sizes of the grid-stride kernels over cells and ghosts: bounded by
+// the capacity and by the particle count (a run holds at most a few cells
+// and ghosts per particle), so that the launches of a batch of small runs
+// stay small.
+int cell_blocks(const DevState& d) {
+    return std::max(1, std::min({blocks_for(d.cellcap, MCM_BLOCK), 4096, blocks_for(d.np, MCM_BLOCK)}));
 }
-
-int ghost_blocks(const DevState& d) { return std::max(1, std::min(blocks_for(d.ngcap, MCM_BLOCK), 1024)); }
+int tile_blocks(const DevState& d) {
+    return std::max(1, std::min(blocks_for(d.cellcap, SCAN_TILE), blocks_for(d.np, SCAN_TILE) + 1));
+}
+int ghost_blocks(const DevState& d) {
+    return std::max(1, std::min({blocks_for(d.ngcap, MCM_BLOCK), 1024, blocks_for(d.np, 2048)}));
+}
 
 // 0 plain SPH, 1 corrected in the reference's pass order, 2 corrected in one pass
 int corr_variant(const mcm_gpu_ctx* ctx) { return ctx->d.disctype == 1 ? (ctx->corr_one_pass ? 2 : 1) : 0; }
@@ -556,71 +548,157 @@ bool use_wave(const mcm_gpu_ctx* ctx) {
     return ctx->pair_mapping == 1 && ctx->d.ncap <= 64 * WAVE_SLOTS && corr_variant(ctx) != 2;
 }
 
-// energies (and, after a move, coordinate bounds) from the block partials
-void launch_c_fold(mcm_gpu_ctx* ctx, hipStream_t st, int with_bounds) {
-    DevState& d = ctx->d;
-    switch (d.nummat) {
-        case 1: k_c_fold<1><<<1, MCM_BLOCK, 0, st>>>(d, ctx->nb, with_bounds); break;
-        case 2: k_c_fold<2><<<1, MCM_BLOCK, 0, st>>>(d, ctx->nb, with_bounds); break;
-        case 3: k_c_fold<3><<<1, MCM_BLOCK, 0, st>>>(d, ctx->nb, with_bounds); break;
-        case 4: k_c_fold<4><<<1, MCM_BLOCK, 0, st>>>(d, ctx->nb, with_bounds); break;
-        case 5: k_c_fold<5><<<1, MCM_BLOCK, 0, st>>>(d, ctx->nb, with_bounds); break;
-        case 6: k_c_fold<6><<<1, MCM_BLOCK, 0, st>>>(d, ctx->nb, with_bounds); break;
-        case 7: k_c_fold<7><<<1, MCM_BLOCK, 0, st>>>(d, ctx->nb, with_bounds); break;
-        default: k_c_fold<8><<<1, MCM_BLOCK, 0, st>>>(d, ctx->nb, with_bounds); break;
+// list-build kernel of a run: 0 one material, 1 several, 2 wavefront per particle
+int list_variant(const mcm_gpu_ctx* c) { return c->pair_mapping == 1 ? 2 : (c->d.nummat > 1 ? 1 : 0); }
+
+// pair-pass kernels of a run: corr_variant, plus 3 for the wavefront mapping
+int pair_variant(const mcm_gpu_ctx* c) { return corr_variant(c) + (use_wave(c) ? 3 : 0); }
+
+// What a launch of the step spans, per run.  kind_blocks gives the
+// workgroups one run needs in a launch of that kind and kernel variant v;
+// 0 means the run takes no part in it.
+enum LaunchKind { L_ONE, L_PART, L_CELLS, L_TILES, L_GSCAN, L_GFILL, L_GSTRESS, L_LIST, L_PAIR, L_CFOLD, L_NKIND };
+constexpr int kNVariant = MCM_MAX_MAT + 1;
+
+int kind_blocks(const mcm_gpu_ctx* c, int kind, int v) {
+    const DevState& d = c->d;
+    switch (kind) {
+        case L_ONE: return v == 0 ? 1 : 0;
+        case L_PART: return v == 0 ? c->nb : 0;
+        case L_CELLS: return v == 0 ? cell_blocks(d) : 0;
+        case L_TILES: return v == 0 ? tile_blocks(d) : 0;
+        case L_GSCAN: return v < d.nplanes ? 1 : 0;
+        case L_GFILL: return v < d.nplanes ? std::min(ghost_blocks(d), 512) : 0;
+        case L_GSTRESS: return v == 0 && d.boundary ? ghost_blocks(d) : 0;
+        case L_LIST: return list_variant(c) == v ? c->nb : 0;
+        case L_PAIR: return pair_variant(c) == v ? c->nb : 0;
+        case L_CFOLD: return d.nummat == v ? 1 : 0;
     }
+    return 0;
+}
+
+}  // namespace
+
+// Several independent runs stepped together: every launch of the step
+// covers all of them, with a block map from workgroup to (run, block).
+struct mcm_gpu_batch {
+    std::vector<mcm_gpu_ctx*> inst;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    mcm_gpu_options opt{};               // options of every run
+    DevState* states = nullptr;          // device copy of every run's DevState
+    BlockEntry* entries = nullptr;       // the block maps of all launch kinds
+    size_t entries_cap = 0;
+    int map_off[L_NKIND][kNVariant] = {};
+    int map_n[L_NKIND][kNVariant] = {};
+    int max_planes = 0;
+    hipGraphExec_t graph2 = nullptr;     // two batched steps: parity 0 then 1
+    int cur = 0;
+    bool stale = true;                   // states and maps must be uploaded again
+    std::string err;
+};
+
+namespace {
+
+// Where a launch goes: one run, or every run of a batch.
+struct Target {
+    mcm_gpu_ctx* ctx;        // one run (null for a batch)
+    mcm_gpu_batch* batch;    // a batch (null for one run)
+    hipStream_t stream;
+    int planes() const { return batch ? batch->max_planes : ctx->d.nplanes; }
+};
+
+Target one(mcm_gpu_ctx* c) { return Target{c, nullptr, c->stream}; }
+Target all(mcm_gpu_batch* b) { return Target{nullptr, b, b->stream}; }
+
+// Launch kernel kern(d, args...) with threads per workgroup over every run of
+// the target that takes part in (kind, v).  A batched launch passes a state
+// that carries only the block map and the runs' states; the kernel's enter()
+// replaces it by its run's state.
+template <typename... P, typename... A>
+void launch(const Target& T, int kind, int v, int threads, void (*kern)(DevState, P...), A... args) {
+    if (T.batch == nullptr) {
+        const int n = kind_blocks(T.ctx, kind, v);
+        if (n > 0) kern<<<n, threads, 0, T.stream>>>(T.ctx->d, args...);
+        return;
+    }
+    const int n = T.batch->map_n[kind][v];
+    if (n == 0) return;
+    DevState d{};
+    d.bmap = T.batch->entries + T.batch->map_off[kind][v];
+    d.states = T.batch->states;
+    kern<<<n, threads, 0, T.stream>>>(d, args...);
+}
+
+void launch_neighbours(const Target& T, int cur) {
+    launch(T, L_ONE, 0, 1, k_grid_setup);
+    launch(T, L_CELLS, 0, MCM_BLOCK, k_clear_cells);
+    launch(T, L_PART, 0, MCM_BLOCK, k_bin, cur);
+    launch(T, L_TILES, 0, 1024, k_scan_tiles);
+    launch(T, L_ONE, 0, 1024, k_scan_top);
+    launch(T, L_TILES, 0, 1024, k_scan_apply);
+    launch(T, L_PART, 0, MCM_BLOCK, k_scatter);
+    launch(T, L_CELLS, 0, MCM_BLOCK, k_sort_cells);
+    launch(T, L_PART, 0, MCM_BLOCK, k_fill_cand, cur);
+    for (int p = 0; p < T.planes(); p++) {
+        launch(T, L_GSCAN, p, 1024, k_ghost_scan, p);
+        launch(T, L_GFILL, p, MCM_BLOCK, k_ghost_fill, p, cur);
+    }
+    launch(T, L_LIST, 0, MCM_BLOCK, k_list<false>, cur);
+    launch(T, L_LIST, 1, MCM_BLOCK, k_list<true>, cur);
+    launch(T, L_LIST, 2, MCM_BLOCK, k_list_wave, cur);
+}
+
+// energies (and, after a move, coordinate bounds) from the block partials
+void launch_c_fold(const Target& T, int with_bounds) {
+    launch(T, L_CFOLD, 1, MCM_BLOCK, k_c_fold<1>, with_bounds);
+    launch(T, L_CFOLD, 2, MCM_BLOCK, k_c_fold<2>, with_bounds);
+    launch(T, L_CFOLD, 3, MCM_BLOCK, k_c_fold<3>, with_bounds);
+    launch(T, L_CFOLD, 4, MCM_BLOCK, k_c_fold<4>, with_bounds);
+    launch(T, L_CFOLD, 5, MCM_BLOCK, k_c_fold<5>, with_bounds);
+    launch(T, L_CFOLD, 6, MCM_BLOCK, k_c_fold<6>, with_bounds);
+    launch(T, L_CFOLD, 7, MCM_BLOCK, k_c_fold<7>, with_bounds);
+    launch(T, L_CFOLD, 8, MCM_BLOCK, k_c_fold<8>, with_bounds);
 }
 
 // strain pass of the fused step: stages rod, spin, tracerod and x1
-void launch_strain(mcm_gpu_ctx* ctx, int cur) {
-    DevState& d = ctx->d;
-    if (use_wave(ctx)) {
-        if (corr_variant(ctx) == 1) k_strain_wave<1><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur);
-        else k_strain_wave<0><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur);
-        return;
-    }
-    switch (corr_variant(ctx)) {
-        case 1: k_strain_const<MODE_FSTRAIN, 1, false><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur); break;
-        case 2: k_strain_const<MODE_FSTRAIN, 2, false><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur); break;
-        default: k_strain_const<MODE_FSTRAIN, 0, false><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur); break;
-    }
+void launch_strain(const Target& T, int cur) {
+    launch(T, L_PAIR, 0, MCM_BLOCK, k_strain_const<MODE_FSTRAIN, 0, false>, cur);
+    launch(T, L_PAIR, 1, MCM_BLOCK, k_strain_const<MODE_FSTRAIN, 1, false>, cur);
+    launch(T, L_PAIR, 2, MCM_BLOCK, k_strain_const<MODE_FSTRAIN, 2, false>, cur);
+    launch(T, L_PAIR, 3, MCM_BLOCK, k_strain_wave<0>, cur);
+    launch(T, L_PAIR, 4, MCM_BLOCK, k_strain_wave<1>, cur);
 }
 
 // update_h + constitutive + critts of the fused step, then dt and ptime
-void launch_constitutive(mcm_gpu_ctx* ctx, int cur) {
-    DevState& d = ctx->d;
-    k_strain_const<MODE_FCONST, 0, false><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur);
-    k_strain_fold<MODE_FCONST><<<1, MCM_BLOCK, 0, ctx->stream>>>(d, ctx->nb);
+void launch_constitutive(const Target& T, int cur) {
+    launch(T, L_PART, 0, MCM_BLOCK, k_strain_const<MODE_FCONST, 0, false>, cur);
+    launch(T, L_ONE, 0, MCM_BLOCK, k_strain_fold<MODE_FCONST>);
 }
 
-// momentum + boundcond + energies + velocity + move of the fused step (the
-// ghosts' stress is refreshed first, the energy and bounds fold follows)
-void launch_momentum_kernel(mcm_gpu_ctx* ctx, int cur) {
-    DevState& d = ctx->d;
-    if (use_wave(ctx)) {
-        if (corr_variant(ctx) == 1) k_momentum_wave<1><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur);
-        else k_momentum_wave<0><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur);
-        return;
-    }
-    switch (corr_variant(ctx)) {
-        case 1: k_momentum<MODE_FUSED, 1, false><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur); break;
-        case 2: k_momentum<MODE_FUSED, 2, false><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur); break;
-        default: k_momentum<MODE_FUSED, 0, false><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, cur); break;
-    }
+void launch_ghost_stress(const Target& T, int cur) { launch(T, L_GSTRESS, 0, MCM_BLOCK, k_ghost_stress, cur); }
+
+// momentum + boundcond + energies + velocity + move of the fused step
+void launch_momentum_kernel(const Target& T, int cur) {
+    launch(T, L_PAIR, 0, MCM_BLOCK, k_momentum<MODE_FUSED, 0, false>, cur);
+    launch(T, L_PAIR, 1, MCM_BLOCK, k_momentum<MODE_FUSED, 1, false>, cur);
+    launch(T, L_PAIR, 2, MCM_BLOCK, k_momentum<MODE_FUSED, 2, false>, cur);
+    launch(T, L_PAIR, 3, MCM_BLOCK, k_momentum_wave<0>, cur);
+    launch(T, L_PAIR, 4, MCM_BLOCK, k_momentum_wave<1>, cur);
 }
 
-void launch_momentum(mcm_gpu_ctx* ctx, int cur) {
-    DevState& d = ctx->d;
-    if (d.boundary) k_ghost_stress<<<ghost_blocks(d), MCM_BLOCK, 0, ctx->stream>>>(d, cur);
-    launch_momentum_kernel(ctx, cur);
-    launch_c_fold(ctx, ctx->stream, 1);
+// the ghosts' stress is refreshed first, the energy and bounds fold follows
+void launch_momentum(const Target& T, int cur) {
+    launch_ghost_stress(T, cur);
+    launch_momentum_kernel(T, cur);
+    launch_c_fold(T, 1);
 }
 
-void launch_step(mcm_gpu_ctx* ctx, int cur) {
-    launch_neighbours(ctx, cur);
-    launch_strain(ctx, cur);
-    launch_constitutive(ctx, cur);
-    launch_momentum(ctx, cur);
+void launch_step(const Target& T, int cur) {
+    launch_neighbours(T, cur);
+    launch_strain(T, cur);
+    launch_constitutive(T, cur);
+    launch_momentum(T, cur);
 }
 
 int get_scalars_raw(mcm_gpu_ctx* ctx, Scalars* out) {
@@ -643,8 +721,8 @@ int build_graph(mcm_gpu_ctx* ctx) {
     drop_graph(ctx);
     hipGraph_t g;
     HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    launch_step(ctx, 0);
-    launch_step(ctx, 1);
+    launch_step(one(ctx), 0);
+    launch_step(one(ctx), 1);
     HIPCHK(hipStreamEndCapture(ctx->stream, &g));
     HIPCHK(hipGraphInstantiate(&ctx->graph2, g, nullptr, nullptr, 0));
     HIPCHK(hipGraphDestroy(g));
@@ -811,46 +889,15 @@ int initial(mcm_gpu_ctx* ctx) {
     rc = put_scalars_raw(ctx, s3);
     if (rc) return rc;
     k_calctotale<<<ctx->nb, MCM_BLOCK, 0, st>>>(d);
-    launch_c_fold(ctx, st, 0);
+    launch_c_fold(one(ctx), 0);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));
     return 0;
 }
 
-int check_sticky(mcm_gpu_ctx* ctx) {
-    Scalars s;
-    int rc = get_scalars_raw(ctx, &s);
-    if (rc) return rc;
-    if (s.abort) {
-        ctx->err = "capacity exceeded (flags " + std::to_string(s.error_flags) + ")";
-        return MCM_GPU_ERR_CAPACITY;
-    }
-    return 0;
-}
-
-}  // namespace
-
-// =============================================================== C ABI
-extern "C" {
-
-int mcm_gpu_convert_deck(const char* in_path, const char* out_path, char* msg, int32_t msg_len) {
-    if (!in_path || !out_path) return MCM_GPU_ERR_ARG;
-    mcm_gpu_ctx tmp;
-    int rc = read_deck_raw(&tmp, in_path);
-    if (rc == 0) {
-        rc = write_binary_deck(tmp.deck, out_path);
-        if (rc) tmp.err = std::string("cannot write ") + out_path;
-    }
-    if (msg && msg_len > 0) {
-        strncpy(msg, tmp.err.c_str(), (size_t)msg_len - 1);
-        msg[msg_len - 1] = 0;
-    }
-    return rc;
-}
-
-int mcm_gpu_create_from_deck(const char* deck_path, const mcm_gpu_options* opt, mcm_gpu_ctx** out) {
-    if (!out || !deck_path) return MCM_GPU_ERR_ARG;
-    *out = nullptr;
+// A context with its device and stream set (stream: the batch's stream for
+// a run of a batch, else a stream of its own).
+int new_run(const mcm_gpu_options* opt, hipStream_t stream, mcm_gpu_ctx** out) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return MCM_GPU_ERR_NODEVICE;
     mcm_gpu_ctx* ctx = new mcm_gpu_ctx();
@@ -860,9 +907,27 @@ int mcm_gpu_create_from_deck(const char* deck_path, const mcm_gpu_options* opt, 
     ctx->pair_mapping = opt ? opt->pair_mapping : 0;
     if (ctx->pair_mapping < 0 || ctx->pair_mapping > 1) { ctx->err = "pair_mapping must be 0 or 1"; return MCM_GPU_ERR_ARG; }
     HIPCHK(hipSetDevice(ctx->device));
-    HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-    int rc = read_deck(ctx, deck_path);
-    if (rc) return rc;
+    if (stream) {
+        ctx->stream = stream;
+        ctx->own_stream = false;
+    } else {
+        HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    }
+    return MCM_GPU_OK;
+}
+
+// binary deck bytes held in memory (the layout of Deck.write_bin)
+int read_deck_memory(mcm_gpu_ctx* ctx, const void* data, size_t len) {
+    FILE* f = fmemopen(const_cast<void*>(data), len, "rb");
+    if (!f) { ctx->err = "cannot read the deck buffer"; return MCM_GPU_ERR_IO; }
+    int rc = read_binary_deck(ctx, f);
+    fclose(f);
+    return rc ? rc : finish_deck(ctx);
+}
+
+// Device state of a run from its deck: sizes, boundary planes, capacities,
+// allocations.
+int setup_run(mcm_gpu_ctx* ctx, const mcm_gpu_options* opt) {
     HostDeck& D = ctx->deck;
     DevState& d = ctx->d;
     d.np = D.np;
@@ -920,19 +985,191 @@ int mcm_gpu_create_from_deck(const char* deck_path, const mcm_gpu_options* opt, 
         d.cellcap = (int)std::min(cells + 4096.0, 2.0e9);
     }
     ctx->nb = blocks_for(D.np, MCM_BLOCK);
-    rc = alloc_all(ctx);
+    d.nblk = ctx->nb;
+    int rc = alloc_all(ctx);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return MCM_GPU_OK;
 }
 
-void mcm_gpu_destroy(mcm_gpu_ctx* ctx) {
-    if (!ctx) return;
+void destroy_run(mcm_gpu_ctx* ctx) {
     drop_graph(ctx);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (void* p : ctx->allocs) (void)hipFree(p);
-    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->stream && ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
+}
+
+// Mean per-stage times of nsteps steps launched one at a time on T from
+// parity cur, with HIP events on T's stream: ms[0] neighbour search,
+// ms[1] strain kernel, ms[2] constitutive + fold, ms[3] momentum kernel,
+// ms[4] whole step.
+int time_stages(const Target& T, int cur, int nsteps, double* ms, std::string& err) {
+    hipEvent_t ev[7] = {};
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < 7 && !e; k++) e = hipEventCreate(&ev[k]);
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (int it = 0; it < nsteps && !e; it++) {
+        e = hipEventRecord(ev[0], T.stream);
+        launch_neighbours(T, cur);
+        if (!e) e = hipEventRecord(ev[1], T.stream);
+        launch_strain(T, cur);
+        if (!e) e = hipEventRecord(ev[2], T.stream);
+        launch_constitutive(T, cur);
+        if (!e) e = hipEventRecord(ev[3], T.stream);
+        launch_ghost_stress(T, cur);
+        if (!e) e = hipEventRecord(ev[4], T.stream);
+        launch_momentum_kernel(T, cur);
+        if (!e) e = hipEventRecord(ev[5], T.stream);
+        launch_c_fold(T, 1);
+        if (!e) e = hipEventRecord(ev[6], T.stream);
+        if (!e) e = hipEventSynchronize(ev[6]);
+        const int pairs[5][2] = {{0, 1}, {1, 2}, {2, 3}, {4, 5}, {0, 6}};
+        for (int k = 0; k < 5 && !e; k++) {
+            float t = 0.0f;
+            e = hipEventElapsedTime(&t, ev[pairs[k][0]], ev[pairs[k][1]]);
+            acc[k] += t;
+        }
+        cur ^= 1;
+    }
+    for (int k = 0; k < 5; k++) ms[k] = acc[k] / nsteps;
+    for (int k = 0; k < 7; k++)
+        if (ev[k]) (void)hipEventDestroy(ev[k]);
+    if (!e) e = hipGetLastError();
+    if (e) { err = std::string("timed steps: ") + hipGetErrorString(e); return MCM_GPU_ERR_HIP; }
+    return 0;
+}
+
+#define HIPCHK_B(call)                                                                            \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess) {                                                                   \
+            b->err = std::string(#call) + ": " + hipGetErrorString(e_);                           \
+            return MCM_GPU_ERR_HIP;                                                               \
+        }                                                                                         \
+    } while (0)
+
+void drop_batch_graph(mcm_gpu_batch* b) {
+    if (b->graph2) { (void)hipGraphExecDestroy(b->graph2); b->graph2 = nullptr; }
+}
+
+// Upload every run's DevState and the block maps of all launch kinds
+// (after runs were added or a run's capacities grew).
+int batch_upload(mcm_gpu_batch* b) {
+    drop_batch_graph(b);
+    HIPCHK_B(hipStreamSynchronize(b->stream));
+    const int nr = (int)b->inst.size();
+    std::vector<DevState> hs(nr);
+    std::vector<BlockEntry> ent;
+    b->max_planes = 0;
+    for (int i = 0; i < nr; i++) {
+        hs[i] = b->inst[i]->d;
+        b->max_planes = std::max(b->max_planes, hs[i].nplanes);
+    }
+    for (int kind = 0; kind < L_NKIND; kind++)
+        for (int v = 0; v < kNVariant; v++) {
+            const size_t off = ent.size();
+            for (int i = 0; i < nr; i++) {
+                const int n = kind_blocks(b->inst[i], kind, v);
+                for (int j = 0; j < n; j++) ent.push_back(BlockEntry{i, j, n, 0});
+            }
+            if (ent.size() > (size_t)INT32_MAX) { b->err = "batch too large for one launch"; return MCM_GPU_ERR_CAPACITY; }
+            b->map_off[kind][v] = (int)off;
+            b->map_n[kind][v] = (int)(ent.size() - off);
+        }
+    if (b->states) { (void)hipFree(b->states); b->states = nullptr; }
+    HIPCHK_B(hipMalloc(&b->states, sizeof(DevState) * nr));
+    if (ent.size() > b->entries_cap) {
+        if (b->entries) (void)hipFree(b->entries);
+        b->entries = nullptr;
+        b->entries_cap = ent.size() + ent.size() / 4 + 1024;
+        HIPCHK_B(hipMalloc(&b->entries, sizeof(BlockEntry) * b->entries_cap));
+    }
+    HIPCHK_B(hipMemcpyAsync(b->states, hs.data(), sizeof(DevState) * nr, hipMemcpyHostToDevice, b->stream));
+    if (!ent.empty())
+        HIPCHK_B(hipMemcpyAsync(b->entries, ent.data(), sizeof(BlockEntry) * ent.size(), hipMemcpyHostToDevice, b->stream));
+    HIPCHK_B(hipStreamSynchronize(b->stream));
+    b->stale = false;
+    return 0;
+}
+
+int batch_build_graph(mcm_gpu_batch* b) {
+    drop_batch_graph(b);
+    hipGraph_t g;
+    HIPCHK_B(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    launch_step(all(b), 0);
+    launch_step(all(b), 1);
+    HIPCHK_B(hipStreamEndCapture(b->stream, &g));
+    HIPCHK_B(hipGraphInstantiate(&b->graph2, g, nullptr, nullptr, 0));
+    HIPCHK_B(hipGraphDestroy(g));
+    return 0;
+}
+
+// scalar blocks of every run (one synchronisation)
+int batch_scalars(mcm_gpu_batch* b, std::vector<Scalars>& out) {
+    out.resize(b->inst.size());
+    for (size_t i = 0; i < b->inst.size(); i++)
+        HIPCHK_B(hipMemcpyAsync(&out[i], b->inst[i]->sc, sizeof(Scalars), hipMemcpyDeviceToHost, b->stream));
+    HIPCHK_B(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int check_sticky(mcm_gpu_ctx* ctx) {
+    Scalars s;
+    int rc = get_scalars_raw(ctx, &s);
+    if (rc) return rc;
+    if (s.abort) {
+        ctx->err = "capacity exceeded (flags " + std::to_string(s.error_flags) + ")";
+        return MCM_GPU_ERR_CAPACITY;
+    }
+    return 0;
+}
+
+}  // namespace
+
+// =============================================================== C ABI
+extern "C" {
+
+int mcm_gpu_convert_deck(const char* in_path, const char* out_path, char* msg, int32_t msg_len) {
+    if (!in_path || !out_path) return MCM_GPU_ERR_ARG;
+    mcm_gpu_ctx tmp;
+    int rc = read_deck_raw(&tmp, in_path);
+    if (rc == 0) {
+        rc = write_binary_deck(tmp.deck, out_path);
+        if (rc) tmp.err = std::string("cannot write ") + out_path;
+    }
+    if (msg && msg_len > 0) {
+        strncpy(msg, tmp.err.c_str(), (size_t)msg_len - 1);
+        msg[msg_len - 1] = 0;
+    }
+    return rc;
+}
+
+int mcm_gpu_create_from_deck(const char* deck_path, const mcm_gpu_options* opt, mcm_gpu_ctx** out) {
+    if (!out || !deck_path) return MCM_GPU_ERR_ARG;
+    *out = nullptr;
+    mcm_gpu_ctx* ctx = nullptr;
+    int rc = new_run(opt, nullptr, &ctx);
+    *out = ctx;
+    if (rc) return rc;
+    rc = read_deck(ctx, deck_path);
+    return rc ? rc : setup_run(ctx, opt);
+}
+
+int mcm_gpu_create_from_memory(const void* data, int64_t len, const mcm_gpu_options* opt, mcm_gpu_ctx** out) {
+    if (!out || !data || len <= 0) return MCM_GPU_ERR_ARG;
+    *out = nullptr;
+    mcm_gpu_ctx* ctx = nullptr;
+    int rc = new_run(opt, nullptr, &ctx);
+    *out = ctx;
+    if (rc) return rc;
+    rc = read_deck_memory(ctx, data, (size_t)len);
+    return rc ? rc : setup_run(ctx, opt);
+}
+
+void mcm_gpu_destroy(mcm_gpu_ctx* ctx) {
+    if (!ctx || ctx->batch) return;   // a run of a batch goes with the batch
+    destroy_run(ctx);
 }
 
 const char* mcm_gpu_last_error(const mcm_gpu_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
@@ -945,7 +1182,7 @@ int mcm_gpu_initial(mcm_gpu_ctx* ctx) {
 int mcm_gpu_neighbours(mcm_gpu_ctx* ctx) {
     if (!ctx || ctx->phase != 0) return MCM_GPU_ERR_ARG;
     for (int attempt = 0; attempt < 8; attempt++) {
-        launch_neighbours(ctx, ctx->cur);
+        launch_neighbours(one(ctx), ctx->cur);
         HIPCHK(hipGetLastError());
         Scalars s;
         int rc = get_scalars_raw(ctx, &s);
@@ -984,7 +1221,7 @@ int mcm_gpu_update_h(mcm_gpu_ctx* ctx) {
 int mcm_gpu_constitutive(mcm_gpu_ctx* ctx) {
     if (!ctx || ctx->phase != 1) return MCM_GPU_ERR_ARG;
     k_strain_const<MODE_CONST, 0, true><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(ctx->d, ctx->cur);
-    k_strain_fold<MODE_CONST><<<1, MCM_BLOCK, 0, ctx->stream>>>(ctx->d, ctx->nb);
+    k_strain_fold<MODE_CONST><<<1, MCM_BLOCK, 0, ctx->stream>>>(ctx->d);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -1019,7 +1256,7 @@ int mcm_gpu_boundcond(mcm_gpu_ctx* ctx) {
 int mcm_gpu_calctotale(mcm_gpu_ctx* ctx) {
     if (!ctx) return MCM_GPU_ERR_ARG;
     k_calctotale<<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(ctx->d);
-    launch_c_fold(ctx, ctx->stream, 0);
+    launch_c_fold(one(ctx), 0);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -1072,7 +1309,7 @@ int mcm_gpu_step(mcm_gpu_ctx* ctx, int32_t nsteps) {
                 HIPCHK(hipGraphLaunch(ctx->graph2, ctx->stream));
                 k += 2;
             } else {
-                launch_step(ctx, ctx->cur);
+                launch_step(one(ctx), ctx->cur);
                 HIPCHK(hipGetLastError());
                 ctx->cur ^= 1;
                 k += 1;
@@ -1293,37 +1530,143 @@ int mcm_gpu_dump(mcm_gpu_ctx* ctx, const char* path, int32_t step, int32_t stage
 
 int mcm_gpu_time_steps(mcm_gpu_ctx* ctx, int32_t nsteps, double* ms) {
     if (!ctx || !ms || nsteps <= 0 || ctx->phase != 0) return MCM_GPU_ERR_ARG;
-    DevState& d = ctx->d;
-    hipEvent_t ev[7];
-    for (int k = 0; k < 7; k++) HIPCHK(hipEventCreate(&ev[k]));
-    double acc[5] = {0, 0, 0, 0, 0};
-    for (int it = 0; it < nsteps; it++) {
-        const int cur = ctx->cur;
-        HIPCHK(hipEventRecord(ev[0], ctx->stream));
-        launch_neighbours(ctx, cur);
-        HIPCHK(hipEventRecord(ev[1], ctx->stream));
-        launch_strain(ctx, cur);
-        HIPCHK(hipEventRecord(ev[2], ctx->stream));
-        launch_constitutive(ctx, cur);
-        HIPCHK(hipEventRecord(ev[3], ctx->stream));
-        if (d.boundary) k_ghost_stress<<<ghost_blocks(d), MCM_BLOCK, 0, ctx->stream>>>(d, cur);
-        HIPCHK(hipEventRecord(ev[4], ctx->stream));
-        launch_momentum_kernel(ctx, cur);
-        HIPCHK(hipEventRecord(ev[5], ctx->stream));
-        launch_c_fold(ctx, ctx->stream, 1);
-        HIPCHK(hipEventRecord(ev[6], ctx->stream));
-        HIPCHK(hipEventSynchronize(ev[6]));
-        float t;
-        HIPCHK(hipEventElapsedTime(&t, ev[0], ev[1])); acc[0] += t;
-        HIPCHK(hipEventElapsedTime(&t, ev[1], ev[2])); acc[1] += t;
-        HIPCHK(hipEventElapsedTime(&t, ev[2], ev[3])); acc[2] += t;
-        HIPCHK(hipEventElapsedTime(&t, ev[4], ev[5])); acc[3] += t;
-        HIPCHK(hipEventElapsedTime(&t, ev[0], ev[6])); acc[4] += t;
-        ctx->cur ^= 1;
+    int rc = time_stages(one(ctx), ctx->cur, nsteps, ms, ctx->err);
+    ctx->cur ^= nsteps & 1;
+    return rc ? rc : check_sticky(ctx);
+}
+
+// ------------------------------------------------------------- batched runs
+int mcm_gpu_batch_create(const mcm_gpu_options* opt, mcm_gpu_batch** out) {
+    if (!out) return MCM_GPU_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return MCM_GPU_ERR_NODEVICE;
+    mcm_gpu_batch* b = new mcm_gpu_batch();
+    *out = b;
+    if (opt) b->opt = *opt;
+    b->device = b->opt.device;
+    HIPCHK_B(hipSetDevice(b->device));
+    HIPCHK_B(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    return MCM_GPU_OK;
+}
+
+// a run made from a deck file (path) or binary deck bytes (data, len)
+static int batch_add(mcm_gpu_batch* b, const char* path, const void* data, int64_t len) {
+    if (!b) return MCM_GPU_ERR_ARG;
+    mcm_gpu_ctx* ctx = nullptr;
+    int rc = new_run(&b->opt, b->stream, &ctx);
+    if (!rc) rc = path ? read_deck(ctx, path) : read_deck_memory(ctx, data, (size_t)len);
+    if (!rc) rc = setup_run(ctx, &b->opt);
+    if (rc) {
+        b->err = "run " + std::to_string(b->inst.size()) + ": " + (ctx ? ctx->err : std::string("no context"));
+        if (ctx) destroy_run(ctx);
+        return rc;
     }
-    for (int k = 0; k < 5; k++) ms[k] = acc[k] / nsteps;
-    for (int k = 0; k < 7; k++) (void)hipEventDestroy(ev[k]);
-    return check_sticky(ctx);
+    ctx->batch = b;
+    ctx->cur = b->cur;
+    b->inst.push_back(ctx);
+    b->stale = true;
+    return MCM_GPU_OK;
+}
+
+int mcm_gpu_batch_add_deck(mcm_gpu_batch* b, const char* deck_path) {
+    if (!deck_path) return MCM_GPU_ERR_ARG;
+    return batch_add(b, deck_path, nullptr, 0);
+}
+
+int mcm_gpu_batch_add_memory(mcm_gpu_batch* b, const void* data, int64_t len) {
+    if (!data || len <= 0) return MCM_GPU_ERR_ARG;
+    return batch_add(b, nullptr, data, len);
+}
+
+int32_t mcm_gpu_batch_size(const mcm_gpu_batch* b) { return b ? (int32_t)b->inst.size() : 0; }
+
+mcm_gpu_ctx* mcm_gpu_batch_run(mcm_gpu_batch* b, int32_t k) {
+    if (!b || k < 0 || k >= (int32_t)b->inst.size()) return nullptr;
+    return b->inst[k];
+}
+
+const char* mcm_gpu_batch_last_error(const mcm_gpu_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+int mcm_gpu_batch_initial(mcm_gpu_batch* b) {
+    if (!b || b->inst.empty()) return MCM_GPU_ERR_ARG;
+    for (size_t i = 0; i < b->inst.size(); i++) {
+        int rc = initial(b->inst[i]);
+        if (rc) { b->err = "run " + std::to_string(i) + ": " + b->inst[i]->err; return rc; }
+    }
+    b->cur = 0;
+    b->stale = true;
+    return MCM_GPU_OK;
+}
+
+int mcm_gpu_batch_step(mcm_gpu_batch* b, int32_t nsteps) {
+    if (!b || nsteps < 0 || b->inst.empty()) return MCM_GPU_ERR_ARG;
+    for (size_t i = 0; i < b->inst.size(); i++)
+        if (b->inst[i]->phase != 0 || b->inst[i]->cur != b->cur) {
+            b->err = "run " + std::to_string(i) + " is not at a step boundary of the batch";
+            return MCM_GPU_ERR_ARG;
+        }
+    std::vector<Scalars> s0, s1;
+    int rc = batch_scalars(b, s0);
+    if (rc) return rc;
+    if (b->stale && (rc = batch_upload(b))) return rc;
+    if (!b->graph2 && (rc = batch_build_graph(b))) return rc;
+    const int cur0 = b->cur;
+    for (int k = 0; k < nsteps;) {
+        if (b->cur == 0 && nsteps - k >= 2) {
+            HIPCHK_B(hipGraphLaunch(b->graph2, b->stream));
+            k += 2;
+        } else {
+            launch_step(all(b), b->cur);
+            HIPCHK_B(hipGetLastError());
+            b->cur ^= 1;
+            k += 1;
+        }
+    }
+    rc = batch_scalars(b, s1);
+    if (rc) return rc;
+    // A run that exceeded a capacity stopped at the start of that step (its
+    // step counter tells where); it grows and finishes the steps on its own.
+    for (size_t i = 0; i < b->inst.size(); i++) {
+        mcm_gpu_ctx* r = b->inst[i];
+        const int done = s1[i].timestep - s0[i].timestep;
+        r->cur = cur0 ^ (done & 1);
+        if (done < nsteps) {
+            rc = mcm_gpu_step(r, nsteps - done);
+            if (rc) { b->err = "run " + std::to_string(i) + ": " + r->err; return rc; }
+            b->stale = true;
+        }
+    }
+    return MCM_GPU_OK;
+}
+
+int mcm_gpu_batch_time_steps(mcm_gpu_batch* b, int32_t nsteps, double* ms) {
+    if (!b || !ms || nsteps <= 0 || b->inst.empty()) return MCM_GPU_ERR_ARG;
+    int rc = 0;
+    if (b->stale && (rc = batch_upload(b))) return rc;
+    rc = time_stages(all(b), b->cur, nsteps, ms, b->err);
+    b->cur ^= nsteps & 1;
+    for (mcm_gpu_ctx* r : b->inst) r->cur = b->cur;
+    if (rc) return rc;
+    std::vector<Scalars> s;
+    if ((rc = batch_scalars(b, s))) return rc;
+    for (size_t i = 0; i < s.size(); i++)
+        if (s[i].abort) {
+            b->err = "run " + std::to_string(i) + ": capacity exceeded during timed steps";
+            return MCM_GPU_ERR_CAPACITY;
+        }
+    return MCM_GPU_OK;
+}
+
+void mcm_gpu_batch_destroy(mcm_gpu_batch* b) {
+    if (!b) return;
+    drop_batch_graph(b);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (mcm_gpu_ctx* r : b->inst) destroy_run(r);
+    if (b->states) (void)hipFree(b->states);
+    if (b->entries) (void)hipFree(b->entries);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
 }
 
 }  // extern "C"

@@ -53,7 +53,9 @@ __device__ __forceinline__ int cell_index(double t) {
 // ------------------------------------------------------------ grid set-up
 // mcm_setuplist.f90:240-291, on the device from hmax and the last coordinate
 // bounds; also resets the per-step counters.
-__global__ void k_grid_setup(DevState d) {
+__global__ void k_grid_setup(DevState d_) {
+    Blk bk;
+    const DevState& d = enter<false>(d_, bk);
     Scalars* sc = d.sc;
     GridDev g;
     const double factor = 2.0;
@@ -113,9 +115,11 @@ __global__ void k_grid_setup(DevState d) {
     sc->hcap_bits = 0;
 }
 
-__global__ void k_clear_cells(DevState d) {
+__global__ void k_clear_cells(DevState d_) {
+    Blk bk;
+    const DevState& d = enter<false>(d_, bk);
     int ncell = d.sc->grid.ncell;
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += gridDim.x * blockDim.x) {
+    for (int c = bk.b * blockDim.x + threadIdx.x; c < ncell; c += bk.n * blockDim.x) {
         d.cell_count[c] = 0;
         d.gcount[0][c] = 0; d.gcount[1][c] = 0; d.gcount[2][c] = 0;
         d.gany[c] = 0;
@@ -124,8 +128,10 @@ __global__ void k_clear_cells(DevState d) {
 
 // histogram: cell of every real particle (mcm_setuplist.f90:320-345); also
 // lays down the particle's strain gather record for this step
-__global__ __launch_bounds__(256) void k_bin(DevState d, int cur) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_bin(DevState d_, int cur) {
+    Blk bk;
+    const DevState& d = enter<false>(d_, bk);
+    const int i = bk.b * blockDim.x + threadIdx.x;
     if (d.sc->abort) return;
     const bool live = i < d.np;
     double hb = 0.0;   // h of a binned particle: bounds every list partner's h
@@ -192,21 +198,27 @@ __global__ __launch_bounds__(256) void k_bin(DevState d, int cur) {
     }
 }
 
-// exclusive scan of cell_count -> cell_start, in three launches
+// exclusive scan of cell_count -> cell_start, in three launches; the tile
+// kernels take tiles b, b + n, ... of the grid's cells
 #define SCAN_ITEMS 4
 #define SCAN_TILE (1024 * SCAN_ITEMS)
-__global__ __launch_bounds__(1024) void k_scan_tiles(DevState d) {
-    int ncell = d.sc->grid.ncell;
-    int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
-    if (blockIdx.x * SCAN_TILE >= ncell) return;
-    int s = 0;
-    for (int k = 0; k < SCAN_ITEMS; k++) s += (base + k < ncell) ? d.cell_count[base + k] : 0;
-    int total;
-    block_excl_scan_1024(s, &total);
-    if (threadIdx.x == 0) d.scan_tmp[blockIdx.x] = total;
+__global__ __launch_bounds__(1024) void k_scan_tiles(DevState d_) {
+    Blk bk;
+    const DevState& d = enter<false>(d_, bk);
+    const int ncell = d.sc->grid.ncell;
+    for (int b = bk.b; b * SCAN_TILE < ncell; b += bk.n) {
+        const int base = b * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+        int s = 0;
+        for (int k = 0; k < SCAN_ITEMS; k++) s += (base + k < ncell) ? d.cell_count[base + k] : 0;
+        int total;
+        block_excl_scan_1024(s, &total);
+        if (threadIdx.x == 0) d.scan_tmp[b] = total;
+    }
 }
 
-__global__ __launch_bounds__(1024) void k_scan_top(DevState d) {
+__global__ __launch_bounds__(1024) void k_scan_top(DevState d_) {
+    Blk bk;
+    const DevState& d = enter<false>(d_, bk);
     int ntile = (d.sc->grid.ncell + SCAN_TILE - 1) / SCAN_TILE;
     __shared__ int carry;
     if (threadIdx.x == 0) carry = 0;
@@ -223,26 +235,31 @@ __global__ __launch_bounds__(1024) void k_scan_top(DevState d) {
     }
 }
 
-__global__ __launch_bounds__(1024) void k_scan_apply(DevState d) {
-    int ncell = d.sc->grid.ncell;
-    if (blockIdx.x * SCAN_TILE >= ncell) return;
-    int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
-    int vals[SCAN_ITEMS];
-    int s = 0;
-    for (int k = 0; k < SCAN_ITEMS; k++) {
-        vals[k] = (base + k < ncell) ? d.cell_count[base + k] : 0;
-        s += vals[k];
-    }
-    int total;
-    int ex = block_excl_scan_1024(s, &total) + d.scan_tmp[blockIdx.x];
-    for (int k = 0; k < SCAN_ITEMS; k++) {
-        if (base + k < ncell) d.cell_start[base + k] = ex;
-        ex += vals[k];
+__global__ __launch_bounds__(1024) void k_scan_apply(DevState d_) {
+    Blk bk;
+    const DevState& d = enter<false>(d_, bk);
+    const int ncell = d.sc->grid.ncell;
+    for (int b = bk.b; b * SCAN_TILE < ncell; b += bk.n) {
+        const int base = b * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+        int vals[SCAN_ITEMS];
+        int s = 0;
+        for (int k = 0; k < SCAN_ITEMS; k++) {
+            vals[k] = (base + k < ncell) ? d.cell_count[base + k] : 0;
+            s += vals[k];
+        }
+        int total;
+        int ex = block_excl_scan_1024(s, &total) + d.scan_tmp[b];
+        for (int k = 0; k < SCAN_ITEMS; k++) {
+            if (base + k < ncell) d.cell_start[base + k] = ex;
+            ex += vals[k];
+        }
     }
 }
 
-__global__ __launch_bounds__(256) void k_scatter(DevState d) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_scatter(DevState d_) {
+    Blk bk;
+    const DevState& d = enter<false>(d_, bk);
+    int i = bk.b * blockDim.x + threadIdx.x;
     if (i >= d.np || d.sc->abort) return;
     int c = d.cell_of[i];
     if (c >= 0) d.sorted[d.cell_start[c] + d.slot_of[i]] = i;
@@ -250,9 +267,11 @@ __global__ __launch_bounds__(256) void k_scatter(DevState d) {
 
 // order each cell by descending id (the walk order of the reference's pushed
 // lists), one thread per cell
-__global__ __launch_bounds__(256) void k_sort_cells(DevState d) {
+__global__ __launch_bounds__(256) void k_sort_cells(DevState d_) {
+    Blk bk;
+    const DevState& d = enter<false>(d_, bk);
     int ncell = d.sc->grid.ncell;
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += gridDim.x * blockDim.x) {
+    for (int c = bk.b * blockDim.x + threadIdx.x; c < ncell; c += bk.n * blockDim.x) {
         int n = d.cell_count[c];
         if (n < 2) continue;
         int* seg = d.sorted + d.cell_start[c];
@@ -267,8 +286,10 @@ __global__ __launch_bounds__(256) void k_sort_cells(DevState d) {
 
 // cell-sorted copies of x, h and the material of the binned particles, one
 // thread per sorted slot (the candidate records of the list build)
-__global__ __launch_bounds__(256) void k_fill_cand(DevState d, int cur) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_fill_cand(DevState d_, int cur) {
+    Blk bk;
+    const DevState& d = enter<false>(d_, bk);
+    const int s = bk.b * blockDim.x + threadIdx.x;
     const GridDev& g = d.sc->grid;
     if (d.sc->abort || g.ncell <= 0) return;
     const int nsorted = d.cell_start[g.ncell - 1] + d.cell_count[g.ncell - 1];
@@ -346,7 +367,9 @@ struct PlaneLayers {
 // id of the first ghost made from source cell r, in the reference's creation
 // order.  Records each target cell's run of ghosts and the plane's ghost id
 // range for k_ghost_fill.
-__global__ __launch_bounds__(1024) void k_ghost_scan(DevState d, int pi) {
+__global__ __launch_bounds__(1024) void k_ghost_scan(DevState d_, int pi) {
+    Blk bk;
+    const DevState& d = enter<false>(d_, bk);
     Scalars* sc = d.sc;
     if (sc->abort) return;
     const PlaneLayers PL(d.planes[pi], sc->grid);
@@ -395,13 +418,15 @@ __global__ __launch_bounds__(1024) void k_ghost_scan(DevState d, int pi) {
 // reference pushes them: a cell holding ghosts gives its ghosts of the z,
 // then y, then x planes, each run from its last ghost down; a cell of real
 // particles gives them in descending id (its sorted order).
-__global__ __launch_bounds__(MCM_BLOCK) void k_ghost_fill(DevState d, int pi, int cur) {
+__global__ __launch_bounds__(MCM_BLOCK) void k_ghost_fill(DevState d_, int pi, int cur) {
+    Blk bk;
+    const DevState& d = enter<false>(d_, bk);
     Scalars* sc = d.sc;
     if (sc->abort) return;
     const PlaneDev P = d.planes[pi];
     const PlaneLayers PL(P, sc->grid);
     const int lo = sc->gplane_lo, hi = sc->gplane_hi;
-    for (int g = lo + blockIdx.x * MCM_BLOCK + threadIdx.x; g < hi; g += gridDim.x * MCM_BLOCK) {
+    for (int g = lo + bk.b * MCM_BLOCK + threadIdx.x; g < hi; g += bk.n * MCM_BLOCK) {
         int a = 0, b = PL.nlayer - 1;   // last r with gbase[r] <= g
         while (a < b) {
             const int mid = (a + b + 1) >> 1;
@@ -441,10 +466,12 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_ghost_fill(DevState d, int pi, in
 // never skipped.  Skipping rejected candidates leaves the list, its order
 // and mindist unchanged.
 template <bool MULTI>
-__global__ __launch_bounds__(256) void k_list(DevState d, int cur) {
+__global__ __launch_bounds__(256) void k_list(DevState d_, int cur) {
     // threads take the binned particles in cell-sorted order, so that the
     // lanes of a wavefront share cells and read the same candidates
-    const int t = xcd_block() * blockDim.x + threadIdx.x;
+    Blk bk;
+    const DevState& d = enter<true>(d_, bk);
+    const int t = bk.b * blockDim.x + threadIdx.x;
     Scalars* sc = d.sc;
     if (sc->abort) return;
     const GridDev g = sc->grid;

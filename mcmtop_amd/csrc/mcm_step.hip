@@ -369,7 +369,10 @@ __device__ __forceinline__ void strain_block_partials(const DevState& d, int lb,
 // critts and hmax from the partials of nblk blocks; in the fused step dt and
 // ptime advance (mcm_solution.f90:58-62).  One workgroup.
 template <int MODE>
-__global__ __launch_bounds__(MCM_BLOCK) void k_strain_fold(DevState d, int nblk) {
+__global__ __launch_bounds__(MCM_BLOCK) void k_strain_fold(DevState d_) {
+    Blk bk;
+    const DevState& d = enter<false>(d_, bk);
+    const int nblk = d.nblk;
     Scalars* sc = d.sc;
     double c1 = fold<1>(d.red, 2, 0, nblk, 1.0e20);
     double h1 = fold<2>(d.red, 2, 1, nblk, 0.0);
@@ -388,8 +391,10 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_strain_fold(DevState d, int nblk)
 // Thread-per-particle strain (+ constitutive) kernel.  CORR: 0 plain SPH,
 // 1 corrected (reference order), 2 corrected (one pass).
 template <int MODE, int CORR, bool DIAG>
-__global__ __launch_bounds__(MCM_BLOCK) void k_strain_const(DevState d, int cur) {
-    const int lb = xcd_block();
+__global__ __launch_bounds__(MCM_BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void k_strain_const(DevState d_, int cur) {
+    Blk bk;
+    const DevState& d = enter<true>(d_, bk);
+    const int lb = bk.b;
     const int i = lb * MCM_BLOCK + threadIdx.x;
     const int nxt = cur ^ 1;
     Scalars* sc = d.sc;
@@ -477,11 +482,13 @@ __global__ void k_dt_update(DevState d) {
 // mcm_update_ghost_stress.f90:28-41: a ghost carries its parent's stress and
 // viscosity with every off-diagonal component negated, divided by the
 // ghost's own (start-of-step) density
-__global__ __launch_bounds__(MCM_BLOCK) void k_ghost_stress(DevState d, int cur) {
+__global__ __launch_bounds__(MCM_BLOCK) void k_ghost_stress(DevState d_, int cur) {
+    Blk bk;
+    const DevState& d = enter<false>(d_, bk);
     const int nxt = cur ^ 1;
     if (d.sc->abort) return;
     const int ngp = d.sc->ngp;
-    for (int g = blockIdx.x * MCM_BLOCK + threadIdx.x; g < ngp; g += gridDim.x * MCM_BLOCK) {
+    for (int g = bk.b * MCM_BLOCK + threadIdx.x; g < ngp; g += bk.n * MCM_BLOCK) {
         const int u = d.np + g, p = d.gpar[g];
         double rg = d.rbuf[nxt][u];
         double r2 = rg * rg;
@@ -721,7 +728,10 @@ __device__ void reduce_c_partials(const DevState& d, const CPart& c, bool with_b
 // thread folds all components of its blocks in one sweep (block order per
 // component, as fold() does), then the block reductions combine the threads.
 template <int NM>
-__global__ __launch_bounds__(MCM_BLOCK) void k_c_fold(DevState d, int nblk, int with_bounds) {
+__global__ __launch_bounds__(MCM_BLOCK) void k_c_fold(DevState d_, int with_bounds) {
+    Blk bk;
+    const DevState& d = enter<false>(d_, bk);
+    const int nblk = d.nblk;
     Scalars* sc = d.sc;
     constexpr int nm = NM;
     constexpr int nv = 6 + 5 * NM;
@@ -797,8 +807,10 @@ __device__ __forceinline__ void integrate_particle(const DevState& d, int i, dou
 
 // Thread-per-particle momentum kernel (CORR as k_strain_const).
 template <int MODE, int CORR, bool DIAG>
-__global__ __launch_bounds__(MCM_BLOCK) void k_momentum(DevState d, int cur) {
-    const int lb = xcd_block();
+__global__ __launch_bounds__(MCM_BLOCK) void k_momentum(DevState d_, int cur) {
+    Blk bk;
+    const DevState& d = enter<true>(d_, bk);
+    const int lb = bk.b;
     const int i = lb * MCM_BLOCK + threadIdx.x;
     Scalars* sc = d.sc;
     const bool live = i < d.np && !sc->abort;

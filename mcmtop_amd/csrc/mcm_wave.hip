@@ -94,11 +94,14 @@ __device__ __forceinline__ void keep_if_lane(int p, const double (&v)[S], double
 // where a row holds ghost-headed cells) are tested 64 at a time, and a
 // ballot with a prefix count packs the accepted ids into the list row in
 // walk order.
-__global__ __launch_bounds__(256) void k_list_wave(DevState d, int cur) {
+__global__ __launch_bounds__(256) void k_list_wave(DevState d_, int cur) {
+    Blk bk;
+    const DevState& d = enter<true>(d_, bk);
+    const int lb = bk.b;
     Scalars* sc = d.sc;
     if (sc->abort) return;
     const int lane = threadIdx.x & 63;
-    const int wbase = xcd_block() * 256 + wave_uniform(threadIdx.x >> 6) * 64;
+    const int wbase = lb * 256 + wave_uniform(threadIdx.x >> 6) * 64;
     const GridDev g = sc->grid;
     const int L0 = g.lim[0], L1 = g.lim[1], L2 = g.lim[2];
     const int ncap = d.ncap, np = d.np;
@@ -353,9 +356,11 @@ __device__ __forceinline__ void strain_wave(const DevState& d, double* buf, int 
 // constitutive update and the dt reduction thread per particle, so that its
 // register demand does not limit the occupancy of the gathers.
 template <int CORR>
-__global__ __launch_bounds__(256) void k_strain_wave(DevState d, int cur) {
+__global__ __launch_bounds__(256) void k_strain_wave(DevState d_, int cur) {
     __shared__ __align__(16) double lds[4][OSUM_WORDS];
-    const int lb = xcd_block();
+    Blk bk;
+    const DevState& d = enter<true>(d_, bk);
+    const int lb = bk.b;
     const int lane = threadIdx.x & 63;
     const int wid = wave_uniform(threadIdx.x >> 6);
     double* buf = lds[wid];
@@ -528,9 +533,11 @@ __device__ __forceinline__ void accel_wave(const DevState& d, double* buf, int i
 // Fused momentum + boundcond + energies + velocity + move; the last
 // workgroup reduces energies and coordinate bounds.
 template <int CORR>
-__global__ __launch_bounds__(256) void k_momentum_wave(DevState d, int cur) {
+__global__ __launch_bounds__(256) void k_momentum_wave(DevState d_, int cur) {
     __shared__ __align__(16) double lds[4][OSUM_WORDS];
-    const int lb = xcd_block();
+    Blk bk;
+    const DevState& d = enter<true>(d_, bk);
+    const int lb = bk.b;
     const int lane = threadIdx.x & 63;
     const int wid = wave_uniform(threadIdx.x >> 6);
     double* buf = lds[wid];

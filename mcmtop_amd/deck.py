@@ -21,6 +21,7 @@ doubles: the text field is the definition of the input.
 from __future__ import annotations
 
 import dataclasses
+import io
 import math
 import struct
 from typing import List, Optional
@@ -104,27 +105,35 @@ class Deck:
 
     # ------------------------------------------------------------------ binary
     def write_bin(self, path: str) -> None:
-        n = self.np_
         with open(path, "wb") as f:
-            f.write(struct.pack("<i", DECK_MAGIC))
-            ints = [n, len(self.materials), self.disctype, self.h_opt, self.tcrit_opt,
-                    self.contacttype, self.krtype, self.maxnbr, self.massopt,
-                    self.init_h_opt, self.init_rhoe, self.mirr_opt]
-            ints += [int(s) for s in self.sym_flag] + [0]
-            f.write(struct.pack("<%di" % len(ints), *ints))
-            f.write(struct.pack("<9d", self.tssf, self.itss, self.endtime, *[float(s) for s in self.sym_x]))
-            for m in self.materials:
-                f.write(struct.pack("<3i", m.model, m.eos, m.visc_type))
-                f.write(struct.pack("<7d", m.rho, m.av_q, m.av_l, m.mass, m.h, m.rho_min, m.rho_max))
-                f.write(np.asarray(m.card, dtype="<f8").tobytes())
-                f.write(np.asarray(m.eoscard, dtype="<f8").tobytes())
-            f.write(np.ascontiguousarray(self.x, dtype="<f8").tobytes())
-            f.write(np.ascontiguousarray(self.v, dtype="<f8").tobytes())
-            f.write(np.ascontiguousarray(self.mat, dtype="<i4").tobytes())
-            f.write(np.ascontiguousarray(self.dispbc, dtype="<i4").tobytes())
-            for arr in (self.pmass, self.ph, self.prho, self.pe):
-                a = np.zeros(n) if arr is None else np.asarray(arr, dtype="<f8")
-                f.write(a.tobytes())
+            f.write(self.to_bytes())
+
+    def to_bytes(self) -> bytes:
+        """The binary deck (little endian): magic, 19 ints, 9 reals, per material
+        3 ints + 7 reals + 48 property + 24 EOS reals, then x, v, material,
+        dispbc, mass, h, rho, e per particle."""
+        n = self.np_
+        f = io.BytesIO()
+        f.write(struct.pack("<i", DECK_MAGIC))
+        ints = [n, len(self.materials), self.disctype, self.h_opt, self.tcrit_opt,
+                self.contacttype, self.krtype, self.maxnbr, self.massopt,
+                self.init_h_opt, self.init_rhoe, self.mirr_opt]
+        ints += [int(s) for s in self.sym_flag] + [0]
+        f.write(struct.pack("<%di" % len(ints), *ints))
+        f.write(struct.pack("<9d", self.tssf, self.itss, self.endtime, *[float(s) for s in self.sym_x]))
+        for m in self.materials:
+            f.write(struct.pack("<3i", m.model, m.eos, m.visc_type))
+            f.write(struct.pack("<7d", m.rho, m.av_q, m.av_l, m.mass, m.h, m.rho_min, m.rho_max))
+            f.write(np.asarray(m.card, dtype="<f8").tobytes())
+            f.write(np.asarray(m.eoscard, dtype="<f8").tobytes())
+        f.write(np.ascontiguousarray(self.x, dtype="<f8").tobytes())
+        f.write(np.ascontiguousarray(self.v, dtype="<f8").tobytes())
+        f.write(np.ascontiguousarray(self.mat, dtype="<i4").tobytes())
+        f.write(np.ascontiguousarray(self.dispbc, dtype="<i4").tobytes())
+        for arr in (self.pmass, self.ph, self.prho, self.pe):
+            a = np.zeros(n) if arr is None else np.asarray(arr, dtype="<f8")
+            f.write(a.tobytes())
+        return f.getvalue()
 
     @staticmethod
     def read_bin(path: str) -> "Deck":
@@ -275,18 +284,23 @@ def taylor_bar(dx=4.0e-4, radius=3.2e-3, length=25.4e-3, v0=190.0, seed=1,
                jitter=0.01, model="f3dm3", corrected=False, h_opt=0,
                E=117e9, nu=0.35, sy=400e6, Et=100e6, rho=8930.0,
                av_l=1.0, av_q=2.0, tssf=0.3, maxnbr=80, sym=True,
-               C0=3940.0, S1=1.49, gamma0=2.0, a_eos=0.47, tcrit_opt=1) -> Deck:
+               C0=3940.0, S1=1.49, gamma0=2.0, a_eos=0.47, tcrit_opt=1, fit_text=True) -> Deck:
     """Taylor impact bar on a symmetry plane at z=0, impacting at -v0 along z.
 
     Defaults are the 12.8k-particle plain-SPH case: dx = 0.4 mm, r = 3.2 mm,
     L = 25.4 mm, h = 1.2 dx, f3dm3 (rho 8930, E 117 GPa, nu 0.35, sigma_y 400 MPa,
     E_tan 100 MPa, kinematic beta=0), nodal AV av_l=1 av_q=2, CFL 0.3, seeded
     U(+-0.01 dx) jitter from numpy default_rng(seed) in C order.
+
+    fit_text: round every coordinate to what its 20-column text field holds,
+    so that the text deck (write_mcm) carries exactly the same values; off
+    for decks that only travel in binary form (large batches).
     """
     pts = bar_lattice(dx, radius, length)
     rng = np.random.default_rng(seed)
     pts = pts + rng.uniform(-jitter * dx, jitter * dx, size=pts.shape)
-    pts = np.vectorize(lambda c: _fit(c, 20))(pts) if len(pts) < 200000 else _fit_array(pts, 20)
+    if fit_text:
+        pts = np.vectorize(lambda c: _fit(c, 20))(pts) if len(pts) < 200000 else _fit_array(pts, 20)
     n = len(pts)
     total_mass = rho * dx ** 3 * n
     h = 1.2 * dx
@@ -302,6 +316,38 @@ def taylor_bar(dx=4.0e-4, radius=3.2e-3, length=25.4e-3, v0=190.0, seed=1,
     return Deck(n, 1 if corrected else 0, h_opt, tcrit_opt, maxnbr, _fit(tssf, 10), 0.0, _fit(1.0, 10),
                 [mat], pts, v, np.ones(n, dtype=np.int32), np.zeros(n, dtype=np.int32),
                 sym_flag, np.zeros(6))
+
+
+def batch_bars(count=256, seed=3, nmin=5000, nmax=200000, corrected=True, h_opt=1, dx_range=(0.05e-3, 0.2e-3)):
+    """Independent Taylor bars of uneven size, every parameter from one
+    numpy default_rng(seed): radius U[2, 5] mm, L/D U[3, 8], dx U[dx_range]
+    (default [0.05, 0.2] mm), v0 U[100, 250] m/s, sigma_y U[200, 600] MPa,
+    E U[70, 210] GPa, drawn in that order per candidate, kept when the
+    particle count lies in [nmin, nmax] (rejection sampling); then a jitter
+    seed per bar.  f3dm10 + eqos4 as the scaled bar (nu 0.35, E_h 100 MPa,
+    copper Gruneisen).  Returns (decks, parameters)."""
+    rng = np.random.default_rng(seed)
+    decks, params = [], []
+    while len(decks) < count:
+        r = rng.uniform(2.0e-3, 5.0e-3)
+        ld = rng.uniform(3.0, 8.0)
+        dx = rng.uniform(*dx_range)
+        v0 = rng.uniform(100.0, 250.0)
+        sy = rng.uniform(200e6, 600e6)
+        E = rng.uniform(70e9, 210e9)
+        length = 2.0 * r * ld
+        est = math.pi * r * r * length / dx ** 3
+        if est < 0.7 * nmin or est > 1.4 * nmax:
+            continue
+        n = len(bar_lattice(dx, r, length))
+        if n < nmin or n > nmax:
+            continue
+        jseed = int(rng.integers(0, 2 ** 31 - 1))
+        decks.append(taylor_bar(dx=dx, radius=r, length=length, v0=v0, seed=jseed, model="f3dm10",
+                                corrected=corrected, h_opt=h_opt, E=E, sy=sy, fit_text=False))
+        params.append({"radius": r, "length": length, "dx": dx, "v0": v0, "sigma_y": sy, "E": E, "np": n,
+                       "seed": jseed})
+    return decks, params
 
 
 def _fit_array(a: np.ndarray, width: int) -> np.ndarray:

@@ -1,7 +1,8 @@
 ! mcm_gpu_run.f90 -- Fortran host driver of the GPU step.
 !
-! Usage: mcm_gpu_run <deck.bin> <nsteps> [fused|stages] [dump.bin]
-! Creates the device run from a binary deck, runs the initialisation of
+! Usage: mcm_gpu_run <deck> <nsteps> [fused|stages] [dump.bin]
+! Creates the device run from a deck (the reference's text deck or the
+! binary deck), runs the initialisation of
 ! mcm_initial.f90:33-101 on the device, then either the loop body of
 ! mcm_solution.f90:41-126 through the same-named stage subroutines
 ! ("stages", dt and ptime advanced on the device between mcm_constitutive

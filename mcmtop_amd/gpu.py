@@ -30,7 +30,10 @@ SYMBOLS = (
     "mcm_gpu_dt_update", "mcm_gpu_momentum", "mcm_gpu_boundcond", "mcm_gpu_calctotale",
     "mcm_gpu_update_velocity", "mcm_gpu_move", "mcm_gpu_calccritts", "mcm_gpu_step",
     "mcm_gpu_get_scalars", "mcm_gpu_download", "mcm_gpu_download_neighbours", "mcm_gpu_dump",
-    "mcm_gpu_time_steps", "mcm_gpu_convert_deck",
+    "mcm_gpu_time_steps", "mcm_gpu_convert_deck", "mcm_gpu_create_from_memory",
+    "mcm_gpu_batch_create", "mcm_gpu_batch_add_deck", "mcm_gpu_batch_add_memory", "mcm_gpu_batch_size",
+    "mcm_gpu_batch_run", "mcm_gpu_batch_last_error", "mcm_gpu_batch_initial", "mcm_gpu_batch_step",
+    "mcm_gpu_batch_time_steps", "mcm_gpu_batch_destroy",
 )
 
 
@@ -83,8 +86,26 @@ def load_library(path: str = LIB_PATH):
     lib.mcm_gpu_dump.argtypes = [vp, ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32]
     lib.mcm_gpu_time_steps.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
     lib.mcm_gpu_convert_deck.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int32]
+    lib.mcm_gpu_create_from_memory.argtypes = [ctypes.c_char_p, ctypes.c_int64, ctypes.POINTER(Options),
+                                               ctypes.POINTER(vp)]
+    lib.mcm_gpu_batch_create.argtypes = [ctypes.POINTER(Options), ctypes.POINTER(vp)]
+    lib.mcm_gpu_batch_add_deck.argtypes = [vp, ctypes.c_char_p]
+    lib.mcm_gpu_batch_add_memory.argtypes = [vp, ctypes.c_char_p, ctypes.c_int64]
+    lib.mcm_gpu_batch_size.argtypes = [vp]
+    lib.mcm_gpu_batch_size.restype = ctypes.c_int32
+    lib.mcm_gpu_batch_run.argtypes = [vp, ctypes.c_int32]
+    lib.mcm_gpu_batch_run.restype = vp
+    lib.mcm_gpu_batch_last_error.argtypes = [vp]
+    lib.mcm_gpu_batch_last_error.restype = ctypes.c_char_p
+    lib.mcm_gpu_batch_initial.argtypes = [vp]
+    lib.mcm_gpu_batch_step.argtypes = [vp, ctypes.c_int32]
+    lib.mcm_gpu_batch_time_steps.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
+    lib.mcm_gpu_batch_destroy.argtypes = [vp]
+    lib.mcm_gpu_batch_destroy.restype = None
+    special = ("mcm_gpu_destroy", "mcm_gpu_last_error", "mcm_gpu_batch_size", "mcm_gpu_batch_run",
+               "mcm_gpu_batch_last_error", "mcm_gpu_batch_destroy")
     for name in SYMBOLS:
-        if name not in ("mcm_gpu_destroy", "mcm_gpu_last_error"):
+        if name not in special:
             getattr(lib, name).restype = ctypes.c_int
     _LIB = lib
     return lib
@@ -100,19 +121,41 @@ def convert_deck(in_path: str, out_path: str) -> None:
         raise McmGpuError("convert_deck failed: %s (%s)" % (ERRORS.get(rc, rc), msg.value.decode()))
 
 
-class Solver:
-    """One MCM run on one GPU, created from an input deck: the reference's
-    version-2 text deck or the binary deck of mcmtop_amd/deck.py."""
+def _deck_bytes(deck) -> bytes:
+    """Binary deck bytes of a Deck object or of bytes as given."""
+    if isinstance(deck, (bytes, bytearray)):
+        return bytes(deck)
+    return deck.to_bytes()
 
-    def __init__(self, deck_path: str, nbr_capacity: int = 0, ghost_capacity: int = 0, device: int = 0,
+
+class Solver:
+    """One MCM run on one GPU, created from an input deck: a path to the
+    reference's version-2 text deck or to the binary deck of
+    mcmtop_amd/deck.py, or binary deck bytes / a Deck object."""
+
+    def __init__(self, deck, nbr_capacity: int = 0, ghost_capacity: int = 0, device: int = 0,
                  corrected_one_pass: bool = False, pair_mapping: int = 0):
         """pair_mapping: 0 one thread per particle, 1 one wavefront per particle (same results)."""
         self._lib = load_library()
+        self._owned = True
         opt = Options(nbr_capacity, ghost_capacity, device, int(bool(corrected_one_pass)), int(pair_mapping))
         h = ctypes.c_void_p()
-        rc = self._lib.mcm_gpu_create_from_deck(deck_path.encode(), ctypes.byref(opt), ctypes.byref(h))
+        if isinstance(deck, str):
+            rc = self._lib.mcm_gpu_create_from_deck(deck.encode(), ctypes.byref(opt), ctypes.byref(h))
+        else:
+            data = _deck_bytes(deck)
+            rc = self._lib.mcm_gpu_create_from_memory(data, len(data), ctypes.byref(opt), ctypes.byref(h))
         self._h = h
         self._check(rc, "create")
+
+    @classmethod
+    def _run_of(cls, lib, handle: int) -> "Solver":
+        """A run owned by a batch (closing it does nothing)."""
+        s = cls.__new__(cls)
+        s._lib = lib
+        s._owned = False
+        s._h = ctypes.c_void_p(handle)
+        return s
 
     def _check(self, rc: int, what: str):
         if rc != 0:
@@ -121,7 +164,8 @@ class Solver:
 
     def close(self):
         if getattr(self, "_h", None):
-            self._lib.mcm_gpu_destroy(self._h)
+            if self._owned:
+                self._lib.mcm_gpu_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -208,3 +252,69 @@ class Solver:
         ms = (ctypes.c_double * 5)()
         self._check(self._lib.mcm_gpu_time_steps(self._h, int(n), ms), "time_steps")
         return np.array(ms[:])
+
+
+class Batch:
+    """Independent runs stepped together in one launch set (mcm_gpu_batch_*):
+    each run has its own deck, particle count and scalars, and gives the
+    same result as when stepped alone."""
+
+    def __init__(self, decks=(), nbr_capacity: int = 0, ghost_capacity: int = 0, device: int = 0,
+                 corrected_one_pass: bool = False, pair_mapping: int = 0):
+        self._lib = load_library()
+        opt = Options(nbr_capacity, ghost_capacity, device, int(bool(corrected_one_pass)), int(pair_mapping))
+        h = ctypes.c_void_p()
+        rc = self._lib.mcm_gpu_batch_create(ctypes.byref(opt), ctypes.byref(h))
+        self._h = h
+        self._check(rc, "batch_create")
+        for d in decks:
+            self.add(d)
+
+    def _check(self, rc: int, what: str):
+        if rc != 0:
+            msg = self._lib.mcm_gpu_batch_last_error(self._h).decode() if self._h else ""
+            raise McmGpuError("%s failed: %s (%s)" % (what, ERRORS.get(rc, rc), msg))
+
+    def add(self, deck):
+        """Add a run: a deck path, binary deck bytes or a Deck object."""
+        if isinstance(deck, str):
+            rc = self._lib.mcm_gpu_batch_add_deck(self._h, deck.encode())
+        else:
+            data = _deck_bytes(deck)
+            rc = self._lib.mcm_gpu_batch_add_memory(self._h, data, len(data))
+        self._check(rc, "batch_add")
+
+    def __len__(self):
+        return int(self._lib.mcm_gpu_batch_size(self._h))
+
+    def run(self, k: int) -> Solver:
+        """Run k (owned by the batch): scalars, fields, lists, dumps."""
+        h = self._lib.mcm_gpu_batch_run(self._h, int(k))
+        if not h:
+            raise McmGpuError("batch has no run %d" % k)
+        return Solver._run_of(self._lib, h)
+
+    def initial(self):
+        self._check(self._lib.mcm_gpu_batch_initial(self._h), "batch_initial")
+
+    def step(self, n: int = 1):
+        self._check(self._lib.mcm_gpu_batch_step(self._h, int(n)), "batch_step")
+
+    def time_steps(self, n: int) -> np.ndarray:
+        ms = (ctypes.c_double * 5)()
+        self._check(self._lib.mcm_gpu_batch_time_steps(self._h, int(n), ms), "batch_time_steps")
+        return np.array(ms[:])
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.mcm_gpu_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
