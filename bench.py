@@ -113,9 +113,9 @@ def cpu_baseline(tmpdir: str, dx: float):
                       % (d.np_, steps, "flang -O2 build of the reference" if kind == "reference" else "C oracle")}
 
 
-def batched_leg(nruns: int, seed: int, steps: int, warmup: int, check_every: int):
-    """Throughput of nruns uneven bars stepped as one batch and stepped one
-    run at a time; equality of sampled runs between the two."""
+def batched_leg(nruns: int, seed: int, steps: int, warmup: int, check_every: int, plain: bool = True):
+    """Throughput of nruns uneven bars stepped as one batch and (plain) stepped
+    one run at a time; equality of sampled runs between the two."""
     import numpy as np
     from mcmtop_amd import deck, gpu
     t = time.perf_counter()
@@ -139,7 +139,7 @@ def batched_leg(nruns: int, seed: int, steps: int, warmup: int, check_every: int
     b.close()
     secs_p = 0.0
     worst = 0.0
-    for k, dk in enumerate(decks):
+    for k, dk in enumerate(decks if plain else []):
         with gpu.Solver(dk) as s:
             s.initial()
             s.step(warmup)
@@ -155,13 +155,14 @@ def batched_leg(nruns: int, seed: int, steps: int, warmup: int, check_every: int
         if k % 32 == 31:
             print("bench: plain runs %d/%d" % (k + 1, nruns), file=sys.stderr, flush=True)
     v_b = total * steps / secs_b
-    v_p = total * steps / secs_p
+    v_p = total * steps / secs_p if plain else None
     return {"runs": nruns, "particles": total, "seed": seed,
             "run_particles": {"min": int(sizes.min()), "median": float(np.median(sizes)), "max": int(sizes.max())},
             "steps": steps, "warmup": warmup,
             "value": v_b, "unit": "particle-steps/s", "ms_per_step": 1e3 * secs_b / steps,
-            "plain_value": v_p, "plain_ms_per_step": 1e3 * secs_p / steps, "speedup": v_b / v_p,
-            "checked_runs": len(sample), "max_rel_l2_vs_plain": worst,
+            "plain_value": v_p, "plain_ms_per_step": 1e3 * secs_p / steps if plain else None,
+            "speedup": v_b / v_p if plain else None,
+            "checked_runs": len(sample) if plain else 0, "max_rel_l2_vs_plain": worst if plain else None,
             "kernel_ms": {"neighbours": ms_b[0], "strain_pass": ms_b[1], "constitutive": ms_b[2],
                           "momentum": ms_b[3], "step": ms_b[4]},
             "workload": "taylor_bar_corrected_f3dm10_eqos4_hopt1 x %d, r U[2,5] mm, L/D U[3,8], dx U[0.05,0.2] mm, "
@@ -178,6 +179,8 @@ def main():
     ap.add_argument("--batch", type=int, default=256, help="runs of the batched leg (0: skip it)")
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--batch-seed", type=int, default=3)
+    ap.add_argument("--batch-plain", type=int, default=1,
+                    help="1: also step the batch's bars one at a time (plain_value, speedup, equality check)")
     ap.add_argument("--corrected-one-pass", action="store_true",
                     help="accumulate the corrected kernels' three passes in one (rounding differs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -258,7 +261,7 @@ def main():
     }
     s.close()
     if args.batch > 0:
-        res["batched"] = batched_leg(args.batch, args.batch_seed, args.batch_steps, 4, 16)
+        res["batched"] = batched_leg(args.batch, args.batch_seed, args.batch_steps, 4, 16, bool(args.batch_plain))
     res["adjoint_forward_ratio"] = None
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(tmpdir, args.dx)
