@@ -139,6 +139,26 @@ int mcm_gpu_dump(mcm_gpu_ctx *ctx, const char *path, int32_t step, int32_t stage
    refresh and the energy fold), whole step. */
 int mcm_gpu_time_steps(mcm_gpu_ctx *ctx, int32_t nsteps, double *ms);
 
+/* Output sync for the writers that stay on the host (mcm_state_output,
+   mcm_timehist: mcm_solution.f90:82-92).  Packs, on the device, the fields
+   they read into one staging buffer and copies it to host[nbytes] in one
+   transfer: per particle MCM_OUT_WORDS values x(3), v(3), p, sigma 11 22 33
+   12 13 23, e, efps, rho, h; precision 8 (double) or 4 (float, converted on
+   the device, as d3plot stores it). */
+#define MCM_OUT_WORDS 17
+int mcm_gpu_output_fields(mcm_gpu_ctx *ctx, int32_t precision, void *host, int64_t nbytes);
+/* Legacy ASCII VTK particle file in the layout of the reference's WriteVTK
+   (material 0) / WriteVTKmat (material m >= 1, particles of 1-based material
+   m) (mcm_VTK.f90:1-507), from np packed output records fields[np *
+   MCM_OUT_WORDS] and 1-based material ids (may be NULL for material 0).
+   Particle counts are written in full where the reference's I5/I6 fields
+   overflow (>= 100,000 particles).  Host only: needs no GPU. */
+int mcm_vtk_write(const char *path, int64_t np, const double *fields, const int32_t *material_ids,
+                  int32_t material);
+/* mcm_gpu_output_fields + mcm_vtk_write for a run (the state_opt 4 output,
+   mcm_state_output.f90:55-60). */
+int mcm_gpu_write_vtk(mcm_gpu_ctx *ctx, const char *path, int32_t material);
+
 /* Batched runs.  Many independent runs (each its own deck: particle count,
    shape, materials, boundary planes) stepped together: every launch of the
    fused step covers all runs, one workgroup map from launch workgroup to

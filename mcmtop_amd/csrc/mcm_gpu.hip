@@ -16,6 +16,7 @@
 
 #include "../../include/mcm_gpu.h"
 #include "mcm_wave.hip"
+#include "mcm_output.h"
 #include "mcm_textdeck.h"
 
 namespace {
@@ -62,6 +63,7 @@ struct mcm_gpu_ctx {
     std::vector<void*> allocs;
     std::string err;
     int device = 0;
+    void* out_buf = nullptr;           // staging buffer of the packed output fields (first use)
     bool own_stream = true;            // false for a run of a batch (the batch's stream)
     mcm_gpu_batch* batch = nullptr;    // the batch this run belongs to, if any
 };
@@ -1533,6 +1535,47 @@ int mcm_gpu_time_steps(mcm_gpu_ctx* ctx, int32_t nsteps, double* ms) {
     int rc = time_stages(one(ctx), ctx->cur, nsteps, ms, ctx->err);
     ctx->cur ^= nsteps & 1;
     return rc ? rc : check_sticky(ctx);
+}
+
+// ------------------------------------------------------------------ output
+static_assert(OUT_WORDS == MCM_OUT_WORDS && OUT_WORDS == vtkout::kWords, "packed output record size");
+
+int mcm_gpu_output_fields(mcm_gpu_ctx* ctx, int32_t precision, void* host, int64_t nbytes) {
+    if (!ctx || !host || (precision != 4 && precision != 8)) return MCM_GPU_ERR_ARG;
+    const size_t np = ctx->d.np;
+    const size_t bytes = np * OUT_WORDS * (size_t)precision;
+    if ((size_t)nbytes < bytes) { ctx->err = "output buffer too small"; return MCM_GPU_ERR_ARG; }
+    if (!ctx->out_buf) {
+        double* p = nullptr;
+        int rc = dalloc(ctx, &p, np * OUT_WORDS);
+        if (rc) return rc;
+        ctx->out_buf = p;
+    }
+    const int hb = ctx->phase == 1 ? ctx->cur ^ 1 : ctx->cur;
+    if (precision == 8) k_pack_output<double><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(ctx->d, hb, ctx->phase, (double*)ctx->out_buf);
+    else k_pack_output<float><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(ctx->d, hb, ctx->phase, (float*)ctx->out_buf);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(host, ctx->out_buf, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return MCM_GPU_OK;
+}
+
+int mcm_vtk_write(const char* path, int64_t np, const double* fields, const int32_t* material_ids, int32_t material) {
+    if (!path || np < 0 || (np > 0 && !fields) || material < 0 || (material > 0 && !material_ids)) return MCM_GPU_ERR_ARG;
+    return vtkout::write_vtk(path, (long)np, fields, material_ids, material) == 0 ? MCM_GPU_OK : MCM_GPU_ERR_IO;
+}
+
+int mcm_gpu_write_vtk(mcm_gpu_ctx* ctx, const char* path, int32_t material) {
+    if (!ctx || !path || material < 0 || material > ctx->d.nummat) return MCM_GPU_ERR_ARG;
+    const size_t np = ctx->d.np;
+    std::vector<double> f(np * OUT_WORDS);
+    int rc = mcm_gpu_output_fields(ctx, 8, f.data(), (int64_t)(f.size() * sizeof(double)));
+    if (rc) return rc;
+    std::vector<int32_t> mat(np);
+    for (size_t i = 0; i < np; i++) mat[i] = ctx->deck.pmat[i] + 1;
+    rc = mcm_vtk_write(path, (int64_t)np, f.data(), mat.data(), material);
+    if (rc) ctx->err = std::string("cannot write ") + path;
+    return rc;
 }
 
 // ------------------------------------------------------------- batched runs

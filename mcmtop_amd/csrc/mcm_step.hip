@@ -923,3 +923,33 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_calccritts_fold(DevState d, int n
     double c1 = fold<1>(d.red, 1, 0, nblk, 1.0e20);
     if (threadIdx.x == 0) d.sc->critts = c1;
 }
+
+// ------------------------------------------------------------------ output
+// The fields the reference's writers read, packed per particle for one
+// device-to-host copy (mcm_gpu_output_fields): x(3), v(3), p, sigma 11 22 33
+// 12 13 23, e, efps, rho, h, converted to T.  x is the position after the
+// strain pass between the strain and move stages (xsel), rho and h those of
+// buffer hb.
+#define OUT_WORDS 17
+template <typename T>
+__global__ __launch_bounds__(MCM_BLOCK) void k_pack_output(DevState d, int hb, int xsel, T* out) {
+    const int i = blockIdx.x * MCM_BLOCK + threadIdx.x;
+    if (i >= d.np) return;
+    double* const* x = xsel ? d.x1 : d.x0;
+    T* r = out + (long)i * OUT_WORDS;
+    for (int a = 0; a < 3; a++) {
+        r[a] = (T)x[a][i];
+        r[3 + a] = (T)d.v[a][i];
+    }
+    r[6] = (T)d.p[i];
+    r[7] = (T)d.sig[S11][i];
+    r[8] = (T)d.sig[S22][i];
+    r[9] = (T)d.sig[S33][i];
+    r[10] = (T)d.sig[S12][i];
+    r[11] = (T)d.sig[S13][i];
+    r[12] = (T)d.sig[S23][i];
+    r[13] = (T)d.e[i];
+    r[14] = (T)d.efps[i];
+    r[15] = (T)d.rbuf[hb][i];
+    r[16] = (T)d.hbuf[hb][i];
+}

@@ -114,6 +114,22 @@ module mcm_gpu_iface
       real(c_double), dimension(*), intent(out) :: host
       integer(c_int64_t), value :: n
     end function
+    ! output sync before the host's writers: 17 packed values per particle
+    ! (x, v, p, sigma 11 22 33 12 13 23, e, efps, rho, h), precision 8 or 4
+    integer(c_int) function mcm_gpu_output_fields(ctx, precision, host, nbytes) bind(c, name='mcm_gpu_output_fields')
+      import :: c_int, c_ptr, c_int32_t, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: precision
+      type(c_ptr), value :: host
+      integer(c_int64_t), value :: nbytes
+    end function
+    ! the state_opt 4 VTK file (material 0: all particles)
+    integer(c_int) function mcm_gpu_write_vtk(ctx, path, material) bind(c, name='mcm_gpu_write_vtk')
+      import :: c_int, c_ptr, c_char, c_int32_t
+      type(c_ptr), value :: ctx
+      character(kind=c_char), dimension(*), intent(in) :: path
+      integer(c_int32_t), value :: material
+    end function
     integer(c_int) function mcm_gpu_dump(ctx, path, step, stage) bind(c, name='mcm_gpu_dump')
       import :: c_int, c_ptr, c_char, c_int32_t
       type(c_ptr), value :: ctx

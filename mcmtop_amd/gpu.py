@@ -33,8 +33,11 @@ SYMBOLS = (
     "mcm_gpu_time_steps", "mcm_gpu_convert_deck", "mcm_gpu_create_from_memory",
     "mcm_gpu_batch_create", "mcm_gpu_batch_add_deck", "mcm_gpu_batch_add_memory", "mcm_gpu_batch_size",
     "mcm_gpu_batch_run", "mcm_gpu_batch_last_error", "mcm_gpu_batch_initial", "mcm_gpu_batch_step",
-    "mcm_gpu_batch_time_steps", "mcm_gpu_batch_destroy",
+    "mcm_gpu_batch_time_steps", "mcm_gpu_batch_destroy", "mcm_gpu_output_fields", "mcm_vtk_write",
+    "mcm_gpu_write_vtk",
 )
+
+OUT_WORDS = 17   # packed output record: x(3), v(3), p, sigma 11 22 33 12 13 23, e, efps, rho, h
 
 
 class McmGpuError(RuntimeError):
@@ -100,6 +103,10 @@ def load_library(path: str = LIB_PATH):
     lib.mcm_gpu_batch_initial.argtypes = [vp]
     lib.mcm_gpu_batch_step.argtypes = [vp, ctypes.c_int32]
     lib.mcm_gpu_batch_time_steps.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
+    lib.mcm_gpu_output_fields.argtypes = [vp, ctypes.c_int32, vp, ctypes.c_int64]
+    lib.mcm_vtk_write.argtypes = [ctypes.c_char_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_double),
+                                  ctypes.POINTER(ctypes.c_int32), ctypes.c_int32]
+    lib.mcm_gpu_write_vtk.argtypes = [vp, ctypes.c_char_p, ctypes.c_int32]
     lib.mcm_gpu_batch_destroy.argtypes = [vp]
     lib.mcm_gpu_batch_destroy.restype = None
     special = ("mcm_gpu_destroy", "mcm_gpu_last_error", "mcm_gpu_batch_size", "mcm_gpu_batch_run",
@@ -119,6 +126,21 @@ def convert_deck(in_path: str, out_path: str) -> None:
     rc = lib.mcm_gpu_convert_deck(in_path.encode(), out_path.encode(), msg, 512)
     if rc != 0:
         raise McmGpuError("convert_deck failed: %s (%s)" % (ERRORS.get(rc, rc), msg.value.decode()))
+
+
+def vtk_write(path: str, fields: np.ndarray, material_ids=None, material: int = 0) -> None:
+    """The reference's VTK particle file from packed output records
+    fields[np, OUT_WORDS] (host only, no GPU needed)."""
+    lib = load_library()
+    f = np.ascontiguousarray(fields, dtype=np.float64).reshape(-1, OUT_WORDS)
+    ids = None
+    if material_ids is not None:
+        ids = np.ascontiguousarray(material_ids, dtype=np.int32)
+    rc = lib.mcm_vtk_write(path.encode(), len(f), f.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                           ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)) if ids is not None else None,
+                           int(material))
+    if rc != 0:
+        raise McmGpuError("vtk_write failed: %s" % ERRORS.get(rc, rc))
 
 
 def _deck_bytes(deck) -> bytes:
@@ -241,6 +263,18 @@ class Solver:
                                                    ids.ctypes.data_as(i32p), ids.size, ctypes.byref(cap))
         self._check(rc, "download_neighbours")
         return counts, gcounts, ids
+
+    def output_fields(self, precision: int = 8) -> np.ndarray:
+        """Packed output records [np, OUT_WORDS] in float64 (precision 8) or
+        float32 (4, converted on the device), one transfer."""
+        n = self.scalars().np
+        out = np.zeros((n, OUT_WORDS), dtype=np.float64 if precision == 8 else np.float32)
+        self._check(self._lib.mcm_gpu_output_fields(self._h, int(precision), out.ctypes.data, out.nbytes),
+                    "output_fields")
+        return out
+
+    def write_vtk(self, path: str, material: int = 0):
+        self._check(self._lib.mcm_gpu_write_vtk(self._h, path.encode(), int(material)), "write_vtk")
 
     def dump(self, path: str, step: int = 0, stage: int = 0):
         self._check(self._lib.mcm_gpu_dump(self._h, path.encode(), step, stage), "dump")

@@ -13,7 +13,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def _declared():
     text = open(os.path.join(ROOT, "include", "mcm_gpu.h")).read()
-    return sorted(set(re.findall(r"\b(mcm_gpu_[a-z_0-9]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(mcm_(?:gpu|vtk)_[a-z_0-9]+)\s*\(", text)))
 
 
 def test_header_and_python_mirror_agree():
