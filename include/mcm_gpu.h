@@ -53,7 +53,9 @@ typedef struct {
     double coord_min[3], coord_max[3];
     int32_t np, ngp, max_count, timestep;
     int32_t error_flags;      /* bit 0: neighbour capacity, 1: ghost capacity, 2: cell capacity,
-                                 3: particle in a ghost-layer cell */
+                                 3: particle in a ghost-layer cell, 4: EOS warning,
+                                 5: penalty contact with ghost contact partners (undefined in
+                                 the reference; the step stops with MCM_GPU_ERR_ARG) */
     int32_t nummat;
     int32_t reserved[2];
 } mcm_gpu_scalars;
@@ -90,13 +92,19 @@ int mcm_gpu_update_h(mcm_gpu_ctx *ctx);
 int mcm_gpu_constitutive(mcm_gpu_ctx *ctx);
 /* dt/ptime update of mcm_solution.f90:58-62 (kept on the device) */
 int mcm_gpu_dt_update(mcm_gpu_ctx *ctx);
+/* mcm_rep_cont (mcm_rep_contact.f90:20-98), called by mcm_solution.f90:64
+   after the dt update when contacttype > 0: Monaghan's repulsive acceleration
+   from the other-material contact partners, added to the acceleration by
+   mcm_gpu_momentum.  Does nothing for runs without penalty contact. */
+int mcm_gpu_rep_cont(mcm_gpu_ctx *ctx);
 /* mcm_momentum (mcm_momentum.f90:22-43: ghost stress, old/corr acceleration) */
 int mcm_gpu_momentum(mcm_gpu_ctx *ctx);
 /* mcm_boundcond (mcm_boundcond.f90:23-62) */
 int mcm_gpu_boundcond(mcm_gpu_ctx *ctx);
 /* mcm_calctotale (mcm_calctotale.f90:27-60) */
 int mcm_gpu_calctotale(mcm_gpu_ctx *ctx);
-/* mcm_update_velocity (mcm_update_velocity.f90:29-56) */
+/* mcm_update_velocity (mcm_update_velocity.f90:29-87: velocity update, and
+   the XSPH smoothing when veloc_opt = 1) */
 int mcm_gpu_update_velocity(mcm_gpu_ctx *ctx);
 /* mcm_move (mcm_move.f90:25-55, with mcm_check_sym_pen.f90) */
 int mcm_gpu_move(mcm_gpu_ctx *ctx);

@@ -76,6 +76,8 @@ struct DevState {
     int np, npad, ngcap, nummat, disctype, h_opt, tcrit_opt, boundary, ncap;
     int nplanes, cellcap, need_s, has_model9;
     int nblk;                  // workgroups of MCM_BLOCK threads over np (per-particle launches)
+    int contacttype;           // > 0: penalty contact (separate contact lists, mcm_rep_cont)
+    int veloc_opt;             // 1: XSPH velocity smoothing
     int bcode[3][2];
     int btype[3];
     double bx[3][2];
@@ -105,6 +107,10 @@ struct DevState {
     int* gany;                 // [cellcap] 1 where a cell holds ghosts
     int* nbr;                  // [npad/64][ncap][64]: entry k of particle i at ((i/64)*ncap + k)*64 + i%64
     int *nreal, *ntot;
+    int* cont;                 // penalty contact lists, blocked like nbr (contacttype > 0)
+    int* ncont;                // [np] real contact partners
+    double* rep[3];            // [np] contact acceleration of mcm_rep_cont
+    double* sv[3];             // [np] XSPH smoothing of the velocity
     int* scan_tmp;
     // diagnostics / stage outputs
     double *rod[6], *spin[3], *tracerod, *a[3], *einc, *etry, *rhoold, *critts_p;
@@ -149,4 +155,7 @@ __device__ __forceinline__ const DevState& enter(const DevState& d0, Blk& bk) {
 // Entry k of particle i is nbr_list(d, i)[k * 64].
 __device__ __forceinline__ int* nbr_list(const DevState& d, int i) {
     return d.nbr + (long)(i >> 6) * d.ncap * 64 + (i & 63);
+}
+__device__ __forceinline__ int* cont_list(const DevState& d, int i) {
+    return d.cont + (long)(i >> 6) * d.ncap * 64 + (i & 63);
 }

@@ -36,6 +36,8 @@ namespace {
 struct HostDeck {
     int np = 0, nummat = 0, disctype = 0, h_opt = 0, tcrit_opt = 0, contacttype = 0, krtype = 1;
     int maxnbr = 0, massopt = 0, init_h_opt = 0, init_rhoe = 0, mirr_opt = 0;
+    int veloc_opt = 0;                     // 1: XSPH
+    int ncontmats = -1, cont_opt = 0;      // penalty contact cards (K and n in MatParams)
     int sym_flag[6] = {0, 0, 0, 0, 0, 0};
     double tssf = 0, itss = 0, endtime = 0, sym_x[6] = {0, 0, 0, 0, 0, 0};
     std::vector<MatParams> mat;
@@ -72,6 +74,7 @@ namespace {
 
 // ------------------------------------------------------------------- deck
 constexpr int32_t kDeckMagic = 0x4D434D31;
+constexpr int32_t kContactTag = 0x434F4E54;   // 'CONT': penalty contact section after the particles
 
 template <typename T>
 bool rd(FILE* f, T* p, size_t n) { return fread(p, sizeof(T), n, f) == n; }
@@ -170,7 +173,12 @@ int finish_deck(mcm_gpu_ctx* ctx) {
         for (long i = 0; i < n; i++) cnt[D.pmat[i]]++;
         for (long i = 0; i < n; i++) D.pm[i] = mat_mass[D.pmat[i]] / (double)cnt[D.pmat[i]];
     }
-    if (D.contacttype != 0) { ctx->err = "repulsive contact (contacttype > 0) is not on the GPU path"; return MCM_GPU_ERR_ARG; }
+    // mcm_contact_in.f90:40-46 sets K and n only for "every material pair"
+    if (D.contacttype > 0 && D.ncontmats != -1) {
+        ctx->err = "penalty contact needs ncontmats = -1 (the reference implements no other contact definition)";
+        return MCM_GPU_ERR_ARG;
+    }
+    D.veloc_opt = D.veloc_opt == 1 ? 1 : 0;   // mcm_update_velocity.f90:61 tests veloc_opt == 1
     if (D.krtype != 1 && D.krtype != 0) { ctx->err = "only the cubic B-spline kernel exists"; return MCM_GPU_ERR_ARG; }
     if (D.disctype != 0 && D.disctype != 1) { ctx->err = "disctype must be 0 or 1"; return MCM_GPU_ERR_ARG; }
     return MCM_GPU_OK;
@@ -186,6 +194,7 @@ int read_binary_deck(mcm_gpu_ctx* ctx, FILE* f) {
     D.np = ints[0]; D.nummat = ints[1]; D.disctype = ints[2]; D.h_opt = ints[3]; D.tcrit_opt = ints[4];
     D.contacttype = ints[5]; D.krtype = ints[6]; D.maxnbr = ints[7]; D.massopt = ints[8];
     D.init_h_opt = ints[9]; D.init_rhoe = ints[10]; D.mirr_opt = ints[11];
+    D.veloc_opt = ints[18];
     for (int k = 0; k < 6; k++) D.sym_flag[k] = ints[12 + k];
     D.tssf = reals[0]; D.itss = reals[1]; D.endtime = reals[2];
     for (int k = 0; k < 6; k++) D.sym_x[k] = reals[3 + k];
@@ -215,6 +224,18 @@ int read_binary_deck(mcm_gpu_ctx* ctx, FILE* f) {
          rd(f, D.dispbc.data(), n) && rd(f, D.pm.data(), n) && rd(f, D.ph.data(), n) &&
          rd(f, D.pr.data(), n) && rd(f, D.pe.data(), n);
     if (!ok) { ctx->err = "deck truncated in particle data"; return MCM_GPU_ERR_IO; }
+    if (D.contacttype > 0) {   // penalty contact section: tag, ncontmats, cont_opt, K and n per material
+        int32_t ct[3];
+        if (!(rd(f, ct, 3) && ct[0] == kContactTag)) { ctx->err = "contact deck without its contact section"; return MCM_GPU_ERR_IO; }
+        D.ncontmats = ct[1];
+        D.cont_opt = ct[2];
+        for (int m = 0; m < D.nummat; m++) {
+            double kn[2];
+            if (!rd(f, kn, 2)) { ctx->err = "deck truncated in contact data"; return MCM_GPU_ERR_IO; }
+            D.mat[m].k_cont = kn[0];
+            D.mat[m].n_cont = kn[1];
+        }
+    }
     return MCM_GPU_OK;
 }
 
@@ -250,16 +271,14 @@ int read_text_deck(mcm_gpu_ctx* ctx, FILE* f) {
     { Fmt F(card); F.i(5); init_v_opt = F.i(5); D.massopt = F.i(5); D.init_h_opt = F.i(5); D.init_rhoe = F.i(5);
       if (!F.ok) return bad("control card 4"); }
     if ((rc = next("control card 5"))) return rc;
-    { Fmt F(card); D.contacttype = F.i(5); D.tcrit_opt = F.i(5); veloc_opt = F.i(5); D.mirr_opt = F.i(5);
+    { Fmt F(card); D.contacttype = F.i(5); D.tcrit_opt = F.i(5); D.veloc_opt = veloc_opt = F.i(5); D.mirr_opt = F.i(5);
       nlcur = F.i(5); for (int k = 0; k < 3; k++) nthp[k] = F.i(5); if (!F.ok) return bad("control card 5"); }
     if ((rc = next("control card 6"))) return rc;
     { Fmt F(card); D.h_opt = F.i(5); D.krtype = F.i(5); D.maxnbr = F.i(5); if (!F.ok) return bad("control card 6"); }
     if ((rc = next("control card 7"))) return rc;
     if (axopt != 3) return bad("only 3-D problems (axopt 3) run on the GPU path", MCM_GPU_ERR_ARG);
     if (drelax != 0.0) return bad("dynamic relaxation is not on the GPU path", MCM_GPU_ERR_ARG);
-    if (veloc_opt != 0) return bad("XSPH (veloc_opt 1) is not on the GPU path", MCM_GPU_ERR_ARG);
     if (nlcur != 0 || nthp[0] || nthp[1] || nthp[2]) return bad("load curves / base acceleration are not on the GPU path", MCM_GPU_ERR_ARG);
-    if (D.contacttype != 0) return bad("repulsive contact (contacttype > 0) is not on the GPU path", MCM_GPU_ERR_ARG);
     if (D.np <= 0 || D.nummat <= 0 || D.nummat > MCM_MAX_MAT) return bad("bad particle or material count", MCM_GPU_ERR_ARG);
     // material cards (mcm_matin.f90:38-203)
     D.mat.assign(D.nummat, MatParams{});
@@ -388,6 +407,20 @@ int read_text_deck(mcm_gpu_ctx* ctx, FILE* f) {
         if ((rc = next("boundary plane coordinate card"))) return rc;
         { Fmt F(card); for (int k = 0; k < 6; k++) D.sym_x[k] = F.e(10); if (!F.ok) return bad("boundary plane coordinate card"); }
     }
+    // penalty contact cards (mcm_contact_in.f90:22-45): (2i5), then K and n
+    // per material as (2E10.5) when every material pair is in contact
+    if (D.contacttype > 0) {
+        if ((rc = next("contact card"))) return rc;
+        { Fmt F(card); D.ncontmats = (int)F.i(5); D.cont_opt = (int)F.i(5); if (!F.ok) return bad("contact card"); }
+        if (D.ncontmats == -1)
+            for (int m = 0; m < D.nummat; m++) {
+                if ((rc = next("contact material card"))) return rc;
+                Fmt F(card);
+                D.mat[m].k_cont = F.ed(10, 5);
+                D.mat[m].n_cont = F.ed(10, 5);
+                if (!F.ok) return bad("contact material card");
+            }
+    }
     return MCM_GPU_OK;
 }
 
@@ -415,7 +448,7 @@ int write_binary_deck(const HostDeck& D, const char* path) {
     if (!f) return MCM_GPU_ERR_IO;
     const int32_t magic = kDeckMagic;
     int32_t ints[19] = {D.np, D.nummat, D.disctype, D.h_opt, D.tcrit_opt, D.contacttype, D.krtype, D.maxnbr,
-                        D.massopt, D.init_h_opt, D.init_rhoe, D.mirr_opt, 0, 0, 0, 0, 0, 0, 0};
+                        D.massopt, D.init_h_opt, D.init_rhoe, D.mirr_opt, 0, 0, 0, 0, 0, 0, D.veloc_opt};
     for (int k = 0; k < 6; k++) ints[12 + k] = D.sym_flag[k];
     double reals[9] = {D.tssf, D.itss, D.endtime, 0, 0, 0, 0, 0, 0};
     for (int k = 0; k < 6; k++) reals[3 + k] = D.sym_x[k];
@@ -432,6 +465,14 @@ int write_binary_deck(const HostDeck& D, const char* path) {
          fwrite(D.pmat.data(), 4, n, f) == n && fwrite(D.dispbc.data(), 4, n, f) == n &&
          fwrite(D.pm.data(), 8, n, f) == n && fwrite(D.ph.data(), 8, n, f) == n && fwrite(D.pr.data(), 8, n, f) == n &&
          fwrite(D.pe.data(), 8, n, f) == n;
+    if (ok && D.contacttype > 0) {
+        const int32_t ct[3] = {kContactTag, D.ncontmats, D.cont_opt};
+        ok = fwrite(ct, 4, 3, f) == 3;
+        for (int m = 0; m < D.nummat && ok; m++) {
+            const double kn[2] = {D.mat[m].k_cont, D.mat[m].n_cont};
+            ok = fwrite(kn, 8, 2, f) == 2;
+        }
+    }
     ok = fclose(f) == 0 && ok;
     return ok ? MCM_GPU_OK : MCM_GPU_ERR_IO;
 }
@@ -480,11 +521,13 @@ int alloc_neighbour_arrays(mcm_gpu_ctx* ctx) {
     rc |= dalloc(ctx, &d.scan_tmp, d.cellcap / SCAN_TILE + 2);
     rc |= dalloc(ctx, &d.gbase, d.cellcap);
     rc |= dalloc(ctx, &d.gany, d.cellcap);
+    if (d.contacttype > 0) rc |= dalloc(ctx, &d.cont, (size_t)d.ncap * d.npad);
     return rc;
 }
 
 void free_neighbour_arrays(mcm_gpu_ctx* ctx) {
     DevState& d = ctx->d;
+    if (d.cont) { dfree(ctx, d.cont); d.cont = nullptr; }
     dfree(ctx, d.nbr); dfree(ctx, d.cell_count); dfree(ctx, d.cell_start); dfree(ctx, d.scan_tmp); dfree(ctx, d.gbase); dfree(ctx, d.gany);
     for (int k = 0; k < 3; k++) { dfree(ctx, d.gstart[k]); dfree(ctx, d.gcount[k]); }
 }
@@ -517,6 +560,12 @@ int alloc_all(mcm_gpu_ctx* ctx) {
     rc |= dalloc(ctx, &d.cand_id, 2 * n);
     rc |= dalloc(ctx, &d.spos, n);
     rc |= dalloc(ctx, &d.nreal, n); rc |= dalloc(ctx, &d.ntot, n);
+    if (d.contacttype > 0) {
+        rc |= dalloc(ctx, &d.ncont, n);
+        for (int k = 0; k < 3; k++) rc |= dalloc(ctx, &d.rep[k], n);
+    }
+    if (d.veloc_opt == 1)
+        for (int k = 0; k < 3; k++) rc |= dalloc(ctx, &d.sv[k], n);
     rc |= dalloc(ctx, &d.red, (size_t)ctx->nb * RED_C + 64);
     rc |= dalloc(ctx, &d.mat, d.nummat);
     rc |= dalloc(ctx, &d.sc, 1);
@@ -545,21 +594,30 @@ int ghost_blocks(const DevState& d) {
 // 0 plain SPH, 1 corrected in the reference's pass order, 2 corrected in one pass
 int corr_variant(const mcm_gpu_ctx* ctx) { return ctx->d.disctype == 1 ? (ctx->corr_one_pass ? 2 : 1) : 0; }
 
-// the wavefront-per-particle pair kernels hold up to 2 x 64 list entries
+// the wavefront-per-particle pair kernels hold up to 2 x 64 list entries; runs
+// with penalty contact or XSPH take the thread-per-particle kernels
+bool wave_capable(const mcm_gpu_ctx* ctx) { return ctx->d.contacttype == 0 && ctx->d.veloc_opt == 0; }
 bool use_wave(const mcm_gpu_ctx* ctx) {
-    return ctx->pair_mapping == 1 && ctx->d.ncap <= 64 * WAVE_SLOTS && corr_variant(ctx) != 2;
+    return ctx->pair_mapping == 1 && wave_capable(ctx) && ctx->d.ncap <= 64 * WAVE_SLOTS && corr_variant(ctx) != 2;
 }
 
 // list-build kernel of a run: 0 one material, 1 several, 2 wavefront per particle
-int list_variant(const mcm_gpu_ctx* c) { return c->pair_mapping == 1 ? 2 : (c->d.nummat > 1 ? 1 : 0); }
+int list_variant(const mcm_gpu_ctx* c) {
+    return c->pair_mapping == 1 && wave_capable(c) ? 2 : (c->d.nummat > 1 ? 1 : 0);
+}
 
-// pair-pass kernels of a run: corr_variant, plus 3 for the wavefront mapping
+// strain-pass kernel of a run: corr_variant, plus 3 for the wavefront mapping
 int pair_variant(const mcm_gpu_ctx* c) { return corr_variant(c) + (use_wave(c) ? 3 : 0); }
+
+// momentum kernel of a run: as the strain pass, or 5 + corr_variant when
+// XSPH moves the particles after the velocity update
+int mom_variant(const mcm_gpu_ctx* c) { return c->d.veloc_opt == 1 ? 5 + corr_variant(c) : pair_variant(c); }
 
 // What a launch of the step spans, per run.  kind_blocks gives the
 // workgroups one run needs in a launch of that kind and kernel variant v;
 // 0 means the run takes no part in it.
-enum LaunchKind { L_ONE, L_PART, L_CELLS, L_TILES, L_GSCAN, L_GFILL, L_GSTRESS, L_LIST, L_PAIR, L_CFOLD, L_NKIND };
+enum LaunchKind { L_ONE, L_PART, L_CELLS, L_TILES, L_GSCAN, L_GFILL, L_GSTRESS, L_LIST, L_PAIR, L_MOM, L_CFOLD,
+                  L_REP, L_XSPH, L_NKIND };
 constexpr int kNVariant = MCM_MAX_MAT + 1;
 
 int kind_blocks(const mcm_gpu_ctx* c, int kind, int v) {
@@ -574,7 +632,10 @@ int kind_blocks(const mcm_gpu_ctx* c, int kind, int v) {
         case L_GSTRESS: return v == 0 && d.boundary ? ghost_blocks(d) : 0;
         case L_LIST: return list_variant(c) == v ? c->nb : 0;
         case L_PAIR: return pair_variant(c) == v ? c->nb : 0;
+        case L_MOM: return mom_variant(c) == v ? c->nb : 0;
         case L_CFOLD: return d.nummat == v ? 1 : 0;
+        case L_REP: return v == 0 && d.contacttype > 0 ? c->nb : 0;
+        case L_XSPH: return v == 0 && d.veloc_opt == 1 ? c->nb : 0;
     }
     return 0;
 }
@@ -672,21 +733,29 @@ void launch_strain(const Target& T, int cur) {
     launch(T, L_PAIR, 4, MCM_BLOCK, k_strain_wave<1>, cur);
 }
 
-// update_h + constitutive + critts of the fused step, then dt and ptime
+// update_h + constitutive + critts of the fused step, then dt and ptime, then
+// the contact repulsion of runs with penalty contact (mcm_solution.f90:58-64)
 void launch_constitutive(const Target& T, int cur) {
     launch(T, L_PART, 0, MCM_BLOCK, k_strain_const<MODE_FCONST, 0, false>, cur);
     launch(T, L_ONE, 0, MCM_BLOCK, k_strain_fold<MODE_FCONST>);
+    launch(T, L_REP, 0, MCM_BLOCK, k_rep_cont, cur);
 }
 
 void launch_ghost_stress(const Target& T, int cur) { launch(T, L_GSTRESS, 0, MCM_BLOCK, k_ghost_stress, cur); }
 
-// momentum + boundcond + energies + velocity + move of the fused step
+// momentum + boundcond + energies + velocity + move of the fused step (with
+// XSPH: up to the velocity update, then the smoothing pass and the move)
 void launch_momentum_kernel(const Target& T, int cur) {
-    launch(T, L_PAIR, 0, MCM_BLOCK, k_momentum<MODE_FUSED, 0, false>, cur);
-    launch(T, L_PAIR, 1, MCM_BLOCK, k_momentum<MODE_FUSED, 1, false>, cur);
-    launch(T, L_PAIR, 2, MCM_BLOCK, k_momentum<MODE_FUSED, 2, false>, cur);
-    launch(T, L_PAIR, 3, MCM_BLOCK, k_momentum_wave<0>, cur);
-    launch(T, L_PAIR, 4, MCM_BLOCK, k_momentum_wave<1>, cur);
+    launch(T, L_MOM, 0, MCM_BLOCK, k_momentum<MODE_FUSED, 0, false>, cur);
+    launch(T, L_MOM, 1, MCM_BLOCK, k_momentum<MODE_FUSED, 1, false>, cur);
+    launch(T, L_MOM, 2, MCM_BLOCK, k_momentum<MODE_FUSED, 2, false>, cur);
+    launch(T, L_MOM, 3, MCM_BLOCK, k_momentum_wave<0>, cur);
+    launch(T, L_MOM, 4, MCM_BLOCK, k_momentum_wave<1>, cur);
+    launch(T, L_MOM, 5, MCM_BLOCK, k_momentum<MODE_FVEL, 0, false>, cur);
+    launch(T, L_MOM, 6, MCM_BLOCK, k_momentum<MODE_FVEL, 1, false>, cur);
+    launch(T, L_MOM, 7, MCM_BLOCK, k_momentum<MODE_FVEL, 2, false>, cur);
+    launch(T, L_XSPH, 0, MCM_BLOCK, k_xsph, cur);
+    launch(T, L_XSPH, 0, MCM_BLOCK, k_xsph_move, cur);
 }
 
 // the ghosts' stress is refreshed first, the energy and bounds fold follows
@@ -736,6 +805,11 @@ int build_graph(mcm_gpu_ctx* ctx) {
 int regrow(mcm_gpu_ctx* ctx, const Scalars& sc_host) {
     DevState& d = ctx->d;
     int flags = sc_host.error_flags;
+    if (flags & 32) {
+        ctx->err = "penalty contact: a particle has both real and ghost contact partners; the reference's repulsion "
+                   "is undefined there (mcm_get_j_cont_info.f90:44-48 reads the real contact list past its end)";
+        return MCM_GPU_ERR_ARG;
+    }
     drop_graph(ctx);
     HIPCHK(hipStreamSynchronize(ctx->stream));
     if (flags & 2) {   // ghosts: unified arrays grow, real part kept
@@ -938,6 +1012,8 @@ int setup_run(mcm_gpu_ctx* ctx, const mcm_gpu_options* opt) {
     d.disctype = D.disctype;
     d.h_opt = D.h_opt;
     d.tcrit_opt = D.tcrit_opt;
+    d.contacttype = D.contacttype > 0 ? D.contacttype : 0;
+    d.veloc_opt = D.veloc_opt;
     d.tssf = D.tssf;
     d.pi = acos(-1.0);
     d.ngcap = (opt && opt->ghost_capacity > 0) ? opt->ghost_capacity : D.np / 2 + 4096;
@@ -1228,6 +1304,13 @@ int mcm_gpu_constitutive(mcm_gpu_ctx* ctx) {
     return 0;
 }
 
+int mcm_gpu_rep_cont(mcm_gpu_ctx* ctx) {
+    if (!ctx || ctx->phase != 1) return MCM_GPU_ERR_ARG;
+    if (ctx->d.contacttype > 0) k_rep_cont<<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(ctx->d, ctx->cur);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 int mcm_gpu_dt_update(mcm_gpu_ctx* ctx) {
     if (!ctx) return MCM_GPU_ERR_ARG;
     k_dt_update<<<1, 1, 0, ctx->stream>>>(ctx->d);
@@ -1266,6 +1349,10 @@ int mcm_gpu_calctotale(mcm_gpu_ctx* ctx) {
 int mcm_gpu_update_velocity(mcm_gpu_ctx* ctx) {
     if (!ctx) return MCM_GPU_ERR_ARG;
     k_update_velocity<<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(ctx->d);
+    if (ctx->d.veloc_opt == 1) {   // XSPH, mcm_update_velocity.f90:61-87
+        k_xsph<<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(ctx->d, ctx->cur);
+        k_xsph_apply<<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(ctx->d);
+    }
     HIPCHK(hipGetLastError());
     return 0;
 }

@@ -168,6 +168,7 @@ __global__ __launch_bounds__(256) void k_bin(DevState d_, int cur) {
             d.nreal[i] = cnt;
             d.ntot[i] = cnt;
             d.mindist[i] = 1.0e20;
+            if (d.contacttype > 0) d.ncont[i] = 0;
         }
         d.cell_of[i] = cell;
     }
@@ -482,6 +483,11 @@ __global__ __launch_bounds__(256) void k_list(DevState d_, int cur) {
     const int L0 = g.lim[0], L1 = g.lim[1];
     const int ncap = d.ncap;
     int* out = nbr_list(d, i);
+    // penalty contact: other-material partners go to the contact list, ghosts
+    // of other materials are only counted (mcm_ll_neighbours.f90:162-174,
+    // 209-223; merged into the lists only for kernel contact, :239-267)
+    const bool penalty = MULTI && d.contacttype > 0;
+    int ccnt = 0, gccnt = 0;
     double mind = 1.0e20, mind_cp = 1.0e20;
     const int c = d.cell_of[i];
     int nreal;
@@ -528,8 +534,13 @@ __global__ __launch_bounds__(256) void k_list(DevState d_, int cur) {
                         cnt++;
                     } else if (MULTI && pass == 1 && !same) {
                         mind_cp = fmin(mind_cp, dist_sq);
-                        if (cnt < ncap) out[(long)cnt * 64] = idm.x;
-                        cnt++;
+                        if (penalty) {
+                            if (ccnt < ncap) cont_list(d, i)[(long)ccnt * 64] = idm.x;
+                            ccnt++;
+                        } else {
+                            if (cnt < ncap) out[(long)cnt * 64] = idm.x;
+                            cnt++;
+                        }
                     }
                 }
             };
@@ -610,8 +621,12 @@ __global__ __launch_bounds__(256) void k_list(DevState d_, int cur) {
                                             cnt++;
                                         } else if (MULTI && pass == 1 && !same) {
                                             mind_cp = fmin(mind_cp, dist_sq);
-                                            if (cnt < ncap) out[(long)cnt * 64] = u;
-                                            cnt++;
+                                            if (penalty) {
+                                                gccnt++;
+                                            } else {
+                                                if (cnt < ncap) out[(long)cnt * 64] = u;
+                                                cnt++;
+                                            }
                                         }
                                     }
                                 }
@@ -619,7 +634,14 @@ __global__ __launch_bounds__(256) void k_list(DevState d_, int cur) {
                         }
             }
         }
-        mind = sqrt(fmin(mind, mind_cp));
+        mind = sqrt(penalty ? mind : fmin(mind, mind_cp));
+        // a particle with real and ghost contact partners: the reference
+        // reads the ghost partners from the real contact list past its end
+        // (mcm_get_j_cont_info.f90:44-48), so its repulsion is undefined
+        if (penalty && ccnt > 0 && gccnt > 0) {
+            atomicOr(&sc->error_flags, 32);
+            atomicExch(&sc->abort, 1);
+        }
     } else {
         if (d.disctype == 1) {
             if (cnt < ncap) out[0] = i;
@@ -630,6 +652,8 @@ __global__ __launch_bounds__(256) void k_list(DevState d_, int cur) {
     d.nreal[i] = nreal;
     d.ntot[i] = cnt;
     d.mindist[i] = mind;
+    if (penalty) d.ncont[i] = ccnt;
+    cnt = max(cnt, ccnt);   // both lists share the capacity ncap
     }
     // longest list of the wavefront, one atomic per wavefront
     for (int off = 32; off > 0; off >>= 1) cnt = max(cnt, __shfl_xor(cnt, off, 64));

@@ -25,6 +25,7 @@ struct MatParams {
     double rho, g, av_l, av_q, h, rho_max;
     double strinput[48];
     double eosinput[24];
+    double k_cont, n_cont;   // penalty contact K and n (mcm_contact_in.f90:22-45)
 };
 
 // Per-particle state held in registers by the fused kernels.

@@ -28,7 +28,8 @@
 // stages rod, spin and tracerod, then update_h + constitutive + dt in a
 // kernel of its own, so that the constitutive code's register demand does
 // not limit the occupancy of the gathers.
-enum { MODE_FUSED = 0, MODE_STRAIN = 1, MODE_CONST = 2, MODE_MOMENTUM = 1, MODE_FCONST = 3, MODE_FSTRAIN = 4 };
+enum { MODE_FUSED = 0, MODE_STRAIN = 1, MODE_CONST = 2, MODE_MOMENTUM = 1, MODE_FCONST = 3, MODE_FSTRAIN = 4,
+       MODE_FVEL = 5 };   // MODE_FVEL: fused momentum up to the velocity update, the move follows XSPH
 
 // ------------------------------------------------------------- reductions
 // Block-wide reduction (OP 0 sum, 1 min, 2 max) in a fixed order: butterfly
@@ -506,10 +507,16 @@ __device__ __forceinline__ int sym_index(int r, int c) {
 }
 
 // mcm_oldacceleration.f90:60-89
+// a starts at 0, plus the contact repulsion when penalty contact is on
+// (mcm_oldacceleration.f90:43-56, mcm_corr_acceleration.f90:38-49)
+__device__ __forceinline__ double accel_start(const DevState& d, int i, int k) {
+    return d.contacttype > 0 ? 0.0 + d.rep[k][i] : 0.0;
+}
+
 __device__ __forceinline__ void accel_plain(const DevState& d, int i, const MRec& me, double acc[3]) {
     const int n = d.ntot[i];
     const int* lst = nbr_list(d, i);
-    acc[0] = 0.0; acc[1] = 0.0; acc[2] = 0.0;
+    acc[0] = accel_start(d, i, 0); acc[1] = accel_start(d, i, 1); acc[2] = accel_start(d, i, 2);
     NbrStream<MRec> st(lst, d.rm, n, i);
     for (int k = 0; k < n; k++) {
         int j;
@@ -590,7 +597,7 @@ __device__ __forceinline__ void accel_corr(const DevState& d, int i, const MRec&
         }
     }
     for (int nn = 0; nn < 3; nn++) {
-        double a = 0.0;
+        double a = accel_start(d, i, nn);
         for (int mm = 0; mm < 3; mm++) {
             int c = sym_index(nn, mm);
             for (int l = 0; l < 3; l++) a = a + massj * D[c][l] * invB[l + 3 * mm];
@@ -644,7 +651,7 @@ __device__ __forceinline__ void accel_corr_fast(const DevState& d, int i, const 
             int c = sym_index(nn, mm);
             for (int l = 0; l < 3; l++) sacc += (E1[c][l] * sw - E2[c] * sg[l]) / sw2 * invB[l + 3 * mm];
         }
-        acc[nn] = mlast * sacc;
+        acc[nn] = accel_start(d, i, nn) + mlast * sacc;
     }
 }
 
@@ -769,9 +776,10 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_c_fold(DevState d_, int with_boun
 
 // Fused epilogue of the momentum pass for particle i with acceleration acc:
 // mcm_boundcond, the particle's terms of mcm_calctotale (kinetic energy
-// from the previous |v|), mcm_update_velocity and mcm_move with the
-// symmetry reflection of mcm_check_sym_pen.
-template <bool DIAG>
+// from the previous |v|), mcm_update_velocity and (MOVE) mcm_move with the
+// symmetry reflection of mcm_check_sym_pen.  Without MOVE (XSPH on) the new
+// velocity is stored and k_xsph / k_xsph_move finish the step.
+template <bool DIAG, bool MOVE>
 __device__ __forceinline__ void integrate_particle(const DevState& d, int i, double acc[3], const double xi[3],
                                                    CPart& cp) {
     Scalars* sc = d.sc;
@@ -790,6 +798,12 @@ __device__ __forceinline__ void integrate_particle(const DevState& d, int i, dou
         va = va + v[a] * v[a];
     }
     d.vabs[i] = sqrt(va);
+    if (!MOVE) {
+        for (int a = 0; a < 3; a++) d.v[a][i] = v[a];
+        if (DIAG)
+            for (int a = 0; a < 3; a++) d.a[a][i] = acc[a];
+        return;
+    }
     // mcm_move.f90:32-55
     double x[3];
     for (int a = 0; a < 3; a++) {
@@ -826,10 +840,132 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_momentum(DevState d_, int cur) {
         if (MODE == MODE_MOMENTUM) {
             for (int a = 0; a < 3; a++) d.a[a][i] = acc[a];
         } else {
-            integrate_particle<DIAG>(d, i, acc, xi, cp);
+            integrate_particle<DIAG, MODE != MODE_FVEL>(d, i, acc, xi, cp);
         }
     }
-    if (MODE == MODE_FUSED) reduce_c_partials(d, cp, true, lb);
+    if (MODE == MODE_FUSED || MODE == MODE_FVEL) reduce_c_partials(d, cp, MODE == MODE_FUSED, lb);
+}
+
+// ------------------------------------------------------ optional pair passes
+// mcm_rep_contact.f90:20-98 (with mcm_calc_fij, :100-161): Monaghan's
+// repulsive acceleration of real particle i from its real contact partners
+// (other materials within 2h, listed separately under penalty contact),
+// after the constitutive update: x after the strain pass, the new h.
+__global__ __launch_bounds__(MCM_BLOCK) void k_rep_cont(DevState d_, int cur) {
+    Blk bk;
+    const DevState& d = enter<false>(d_, bk);
+    const int i = bk.b * MCM_BLOCK + threadIdx.x;
+    if (i >= d.np || d.sc->abort) return;
+    const int nxt = cur ^ 1;
+    double r[3] = {0.0, 0.0, 0.0};
+    const int nc = d.ncont[i];
+    if (nc > 0) {
+        const MatParams& mi = d.mat[d.pmat[i]];
+        const double ki = mi.k_cont, en = mi.n_cont;
+        const double xi[3] = {d.x1[0][i], d.x1[1][i], d.x1[2][i]};
+        const double hi = d.hbuf[nxt][i];
+        const int* lst = cont_list(d, i);
+        for (int k = 0; k < nc; k++) {
+            const int j = lst[(long)k * 64];
+            const double hj = d.hbuf[nxt][j], massj = d.mass[j];
+            const double kj = d.mat[d.pmat[j]].k_cont;
+            const double havg = 0.5 * (hi + hj);
+            // inverse of W at the particle spacing (B-spline, :57-66)
+            const double ratio = hj / havg;
+            double inv_w;
+            if (ratio < 1.0) {
+                inv_w = 1.0 / (((0.75 * ratio - 1.5) * ratio) * ratio + 1.0);
+            } else {
+                const double t = 2.0 - ratio;
+                inv_w = 1.0 / (t * (t * t));
+            }
+            double dwdx[3], rad;
+            kernel_grad(xi[0] - d.x1[0][j], xi[1] - d.x1[1][j], xi[2] - d.x1[2][j], havg, d.pi, dwdx, &rad);
+            const double z = rad / havg;
+            double fij;
+            if (z < 1.0) {
+                fij = inv_w * (((0.75 * z - 1.5) * z) * z + 1.0);
+            } else if (z < 2.0) {
+                const double t = 2.0 - z;
+                fij = 0.25 * inv_w * (t * (t * t));
+            } else {
+                fij = 0.0;
+            }
+            const double kij = ki + kj;
+            for (int l = 0; l < 3; l++) r[l] = r[l] - massj * (kij * pow(fij, en)) * dwdx[l];
+        }
+    }
+    for (int l = 0; l < 3; l++) d.rep[l][i] = r[l];
+}
+
+// XSPH smoothing of mcm_update_velocity.f90:61-81 for real particle i, from
+// the new velocities of its list (reals, self, then ghosts with their
+// start-of-step copies): sum of m_j / (rho_j + rho_i)/2 (v_j - v_i) W(x_i - x_j,
+// h_j) into sv.
+__global__ __launch_bounds__(MCM_BLOCK) void k_xsph(DevState d_, int cur) {
+    Blk bk;
+    const DevState& d = enter<true>(d_, bk);
+    const int i = bk.b * MCM_BLOCK + threadIdx.x;
+    if (i >= d.np || d.sc->abort) return;
+    const int nxt = cur ^ 1;
+    const double xi[3] = {d.x1[0][i], d.x1[1][i], d.x1[2][i]};
+    const double vi[3] = {d.v[0][i], d.v[1][i], d.v[2][i]};
+    const double rhoi = d.rbuf[nxt][i];
+    double s[3] = {0.0, 0.0, 0.0};
+    const int n = d.ntot[i];
+    const int* lst = nbr_list(d, i);
+    for (int k = 0; k < n; k++) {
+        const int u = lst[(long)k * 64];
+        double rad = 0.0;
+        for (int a = 0; a < 3; a++) {
+            const double xij = xi[a] - d.x1[a][u];
+            rad = rad + xij * xij;
+        }
+        rad = sqrt(rad);
+        const double w = kernel_w(rad, d.hbuf[nxt][u], d.pi);
+        const double volj = d.mass[u] / (0.5 * (d.rbuf[nxt][u] + rhoi));
+        for (int a = 0; a < 3; a++) s[a] = s[a] + volj * (d.v[a][u] - vi[a]) * w;
+    }
+    for (int a = 0; a < 3; a++) d.sv[a][i] = s[a];
+}
+
+// v += 0.1 sv once every particle's smoothing is known (:83-87)
+__global__ __launch_bounds__(MCM_BLOCK) void k_xsph_apply(DevState d) {
+    const int i = blockIdx.x * MCM_BLOCK + threadIdx.x;
+    if (i >= d.np || d.sc->abort) return;
+    for (int a = 0; a < 3; a++) d.v[a][i] = d.v[a][i] + 0.100 * d.sv[a][i];
+}
+
+// Fused step with XSPH: the smoothed velocity, then mcm_move and the
+// symmetry reflection; coordinate-bound partials of the block (its energy
+// partials were written by the momentum kernel).
+__global__ __launch_bounds__(MCM_BLOCK) void k_xsph_move(DevState d_, int cur) {
+    Blk bk;
+    const DevState& d = enter<true>(d_, bk);
+    const int lb = bk.b;
+    const int i = lb * MCM_BLOCK + threadIdx.x;
+    Scalars* sc = d.sc;
+    const bool live = i < d.np && !sc->abort;
+    double xb[3] = {0.0, 0.0, 0.0};   // bounds before the reflection, as mcm_move.f90:40-55
+    if (live) {
+        double v[3], x[3];
+        for (int a = 0; a < 3; a++) v[a] = d.v[a][i] + 0.100 * d.sv[a][i];
+        for (int a = 0; a < 3; a++) {
+            x[a] = d.x1[a][i] + v[a] * sc->dt;
+            xb[a] = x[a];
+        }
+        if (d.boundary) sym_pen(d, x, v);
+        for (int a = 0; a < 3; a++) {
+            d.v[a][i] = v[a];
+            d.x0[a][i] = x[a];
+        }
+    }
+    double* out = d.red + (long)lb * RED_C;
+    for (int a = 0; a < 3; a++) {
+        const double lo = block_reduce<1>(live ? xb[a] : 1.0e20);
+        const double hi = block_reduce<2>(live ? xb[a] : -1.0e20);
+        if (threadIdx.x == 0) { out[a] = lo; out[3 + a] = hi; }
+    }
 }
 
 // ---------------------------------------------- granular integration stages

@@ -13,9 +13,9 @@
 // I and E edit descriptors (blanks ignored, an empty field is zero, E input
 // accepts 'D' and sign-only exponents).
 //
-// Options the GPU path does not carry are rejected with an error: 1-D/2-D
-// problems, dynamic relaxation, XSPH, load curves, base acceleration and
-// penalty contact.
+// Penalty contact cards (mcm_contact_in.f90:22-45) follow the boundary
+// planes.  Options the GPU path does not carry are rejected with an error:
+// 1-D/2-D problems, dynamic relaxation, load curves and base acceleration.
 #pragma once
 
 #include <cctype>
@@ -71,11 +71,24 @@ inline bool fint(const std::string& card, int col, int w, long* out) {
     return true;
 }
 
-// Fortran Ew.0 input: mantissa with or without a point, exponent introduced by
-// E, D or by its sign alone
-inline bool freal(const std::string& card, int col, int w, double* out) {
+// Fortran Ew.d input: mantissa with or without a point, exponent introduced by
+// E, D or by its sign alone; a mantissa without a point has its last d
+// digits after the implied point (Ew.0: none)
+inline bool freal(const std::string& card, int col, int w, double* out, int d = 0) {
     std::string s = field(card, col, w);
     if (s.empty()) { *out = 0.0; return true; }
+    if (d > 0) {
+        size_t e = 1;   // end of the mantissa: first exponent letter or sign after the first character
+        while (e < s.size() && !strchr("eEdDqQ+-", s[e])) e++;
+        std::string mant = s.substr(0, e), rest = s.substr(e);
+        if (mant.find('.') == std::string::npos) {
+            std::string sign;
+            if (!mant.empty() && (mant[0] == '+' || mant[0] == '-')) { sign = mant.substr(0, 1); mant = mant.substr(1); }
+            while ((int)mant.size() <= d) mant = "0" + mant;
+            mant = sign + mant.substr(0, mant.size() - d) + "." + mant.substr(mant.size() - d);
+        }
+        s = mant + rest;
+    }
     std::string t;
     for (size_t k = 0; k < s.size(); k++) {
         char ch = s[k];
@@ -101,6 +114,7 @@ struct Fmt {
     explicit Fmt(const std::string& c) : card(c) {}
     long i(int w) { long v = 0; ok = ok && fint(card, col, w, &v); col += w; return v; }
     double e(int w) { double v = 0.0; ok = ok && freal(card, col, w, &v); col += w; return v; }
+    double ed(int w, int d) { double v = 0.0; ok = ok && freal(card, col, w, &v, d); col += w; return v; }
     void skip(int w) { col += w; }
 };
 

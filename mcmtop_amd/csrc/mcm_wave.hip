@@ -564,7 +564,7 @@ __global__ __launch_bounds__(256) void k_momentum_wave(DevState d_, int cur) {
         const MRec me = load_mrec(d.rm, i);
         const double xi[3] = {me.x, me.y, me.z};
         double acc[3] = {mine[0], mine[1], mine[2]};
-        integrate_particle<false>(d, i, acc, xi, cp);
+        integrate_particle<false, true>(d, i, acc, xi, cp);
     }
     reduce_c_partials(d, cp, true, lb);
 }

@@ -29,6 +29,7 @@ from typing import List, Optional
 import numpy as np
 
 DECK_MAGIC = int.from_bytes(b"MCM1", "big")
+CONTACT_TAG = int.from_bytes(b"CONT", "big")   # optional penalty-contact section after the particles
 
 
 def _fit(value: float, width: int) -> float:
@@ -98,6 +99,11 @@ class Deck:
     ph: Optional[np.ndarray] = None
     prho: Optional[np.ndarray] = None
     pe: Optional[np.ndarray] = None
+    veloc_opt: int = 0          # 1: XSPH velocity smoothing (mcm_update_velocity.f90:61-87)
+    ncontmats: int = -1         # penalty contact (contacttype > 0): -1 = every material pair
+    cont_opt: int = 0
+    contact_k: Optional[List[float]] = None   # per material K and n of mcm_rep_cont
+    contact_n: Optional[List[float]] = None
 
     @property
     def mirr_opt(self) -> int:
@@ -118,7 +124,7 @@ class Deck:
         ints = [n, len(self.materials), self.disctype, self.h_opt, self.tcrit_opt,
                 self.contacttype, self.krtype, self.maxnbr, self.massopt,
                 self.init_h_opt, self.init_rhoe, self.mirr_opt]
-        ints += [int(s) for s in self.sym_flag] + [0]
+        ints += [int(s) for s in self.sym_flag] + [self.veloc_opt]
         f.write(struct.pack("<%di" % len(ints), *ints))
         f.write(struct.pack("<9d", self.tssf, self.itss, self.endtime, *[float(s) for s in self.sym_x]))
         for m in self.materials:
@@ -133,6 +139,13 @@ class Deck:
         for arr in (self.pmass, self.ph, self.prho, self.pe):
             a = np.zeros(n) if arr is None else np.asarray(arr, dtype="<f8")
             f.write(a.tobytes())
+        if self.contacttype > 0:
+            # penalty contact cards (mcm_contact_in): tag, ncontmats, cont_opt, K and n per material
+            nm = len(self.materials)
+            f.write(struct.pack("<3i", CONTACT_TAG, self.ncontmats, self.cont_opt))
+            k = list(self.contact_k or [0.0] * nm)
+            c = list(self.contact_n or [0.0] * nm)
+            f.write(struct.pack("<%dd" % (2 * nm), *[v for pair in zip(k, c) for v in pair]))
         return f.getvalue()
 
     @staticmethod
@@ -173,9 +186,14 @@ class Deck:
         mat = arr("<i4", n)
         bc = arr("<i4", n)
         pm, ph, pr, pe = (arr("<f8", n) for _ in range(4))
-        return Deck(n, disctype, h_opt, tcrit, maxnbr, reals[0], reals[1], reals[2], mats, x, v,
-                    mat, bc, sym_flag, np.array(reals[3:9]), contact, krtype, massopt, ihopt,
-                    irhoe, pmass=pm, ph=ph, prho=pr, pe=pe)
+        dk = Deck(n, disctype, h_opt, tcrit, maxnbr, reals[0], reals[1], reals[2], mats, x, v,
+                  mat, bc, sym_flag, np.array(reals[3:9]), contact, krtype, massopt, ihopt,
+                  irhoe, pmass=pm, ph=ph, prho=pr, pe=pe, veloc_opt=ints[18])
+        if off + 12 <= len(buf) and struct.unpack_from("<i", buf, off)[0] == CONTACT_TAG:
+            _, dk.ncontmats, dk.cont_opt = take("<3i")
+            kn = take("<%dd" % (2 * nummat))
+            dk.contact_k, dk.contact_n = list(kn[0::2]), list(kn[1::2])
+        return dk
 
     # -------------------------------------------------------------------- text
     def write_mcm(self, path: str) -> None:
@@ -187,7 +205,8 @@ class Deck:
         lines.append(_fmt(self.endtime, 10) + _fmt(self.tssf, 10) + _fmt(self.itss, 10) + _fmt(0.0, 10))
         lines.append(_fmt(1.0e9, 10) + "%5d" % 0 + _fmt(1.0e9, 10) + "%10d%5d%10d%5d%5d" % (0, 0, 100, 0, 0))
         lines.append("%5d%5d%5d%5d%5d" % (1, 1, self.massopt, self.init_h_opt, self.init_rhoe))
-        lines.append("%5d%5d%5d%5d%5d%5d%5d%5d" % (self.contacttype, self.tcrit_opt, 0, self.mirr_opt, 0, 0, 0, 0))
+        lines.append("%5d%5d%5d%5d%5d%5d%5d%5d" % (self.contacttype, self.tcrit_opt, self.veloc_opt, self.mirr_opt,
+                                                    0, 0, 0, 0))
         lines.append("%5d%5d%5d" % (self.h_opt, self.krtype, self.maxnbr))
         lines.append("")
         for k, m in enumerate(self.materials, start=1):
@@ -227,6 +246,12 @@ class Deck:
         if self.mirr_opt:
             lines.append("".join("%10d" % s for s in self.sym_flag))
             lines.append("".join(_fmt(s, 10) for s in self.sym_x))
+        if self.contacttype > 0:
+            lines.append("%5d%5d" % (self.ncontmats, self.cont_opt))
+            if self.ncontmats == -1:
+                nm = len(self.materials)
+                for k, c in zip(self.contact_k or [0.0] * nm, self.contact_n or [0.0] * nm):
+                    lines.append(_fmt(k, 10) + _fmt(c, 10))
         with open(path, "w") as f:
             f.write("\n".join(lines) + "\n")
 
@@ -413,8 +438,10 @@ def block(dx, nx, ny, nz, origin=(0.0, 0.0, 0.0)):
 
 def make_deck(pts, vel, mats, matid=None, dispbc=None, disctype=0, h_opt=0, tcrit_opt=1, tssf=0.3,
               sym_flag=(0, 0, 0, 0, 0, 0), sym_x=(0, 0, 0, 0, 0, 0), seed=1, jitter_dx=0.0,
-              massopt=0, pmass=None) -> Deck:
-    """Generic deck from points, velocities and materials (1-based matid)."""
+              massopt=0, pmass=None, veloc_opt=0, contacttype=0, contact_k=None, contact_n=None) -> Deck:
+    """Generic deck from points, velocities and materials (1-based matid);
+    veloc_opt 1 turns on XSPH, contacttype > 0 penalty contact with K and n
+    per material."""
     pts = np.asarray(pts, dtype=float)
     n = len(pts)
     if jitter_dx:
@@ -426,7 +453,11 @@ def make_deck(pts, vel, mats, matid=None, dispbc=None, disctype=0, h_opt=0, tcri
     bc = np.zeros(n, np.int32) if dispbc is None else np.asarray(dispbc, np.int32)
     d = Deck(n, disctype, h_opt, tcrit_opt, 80, _fit(tssf, 10), 0.0, _fit(1.0, 10), list(mats), pts, v, matid,
              bc, np.asarray(sym_flag, np.int32), np.array([_fit(s, 10) if s else 0.0 for s in sym_x]),
-             massopt=massopt)
+             massopt=massopt, veloc_opt=veloc_opt)
+    d.contacttype = contacttype
+    if contacttype > 0:
+        d.contact_k = [_fit(k, 10) for k in contact_k]
+        d.contact_n = [_fit(c, 10) for c in contact_n]
     if massopt:
         d.pmass = np.array([_fit(m, 20) for m in pmass])
     return d

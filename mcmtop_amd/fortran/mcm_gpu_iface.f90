@@ -72,6 +72,10 @@ module mcm_gpu_iface
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
     end function
+    integer(c_int) function mcm_gpu_rep_cont(ctx) bind(c, name='mcm_gpu_rep_cont')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
     integer(c_int) function mcm_gpu_momentum(ctx) bind(c, name='mcm_gpu_momentum')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

@@ -33,6 +33,7 @@ program mcm_gpu_run
       call mcm_update_h
       call mcm_constitutive
       call mcm_gpu_check(mcm_gpu_dt_update(mcm_gpu_context), 'dt update')
+      call mcm_rep_cont        ! mcm_solution.f90:64 (does nothing without penalty contact)
       call mcm_momentum
       call mcm_boundcond
       call mcm_calctotale

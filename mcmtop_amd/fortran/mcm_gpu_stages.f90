@@ -23,6 +23,11 @@ subroutine mcm_constitutive
   call mcm_gpu_check(mcm_gpu_constitutive(mcm_gpu_context), 'mcm_constitutive')
 end subroutine
 
+subroutine mcm_rep_cont
+  use mcm_gpu_iface
+  call mcm_gpu_check(mcm_gpu_rep_cont(mcm_gpu_context), 'mcm_rep_cont')
+end subroutine
+
 subroutine mcm_momentum
   use mcm_gpu_iface
   call mcm_gpu_check(mcm_gpu_momentum(mcm_gpu_context), 'mcm_momentum')

@@ -34,7 +34,7 @@ SYMBOLS = (
     "mcm_gpu_batch_create", "mcm_gpu_batch_add_deck", "mcm_gpu_batch_add_memory", "mcm_gpu_batch_size",
     "mcm_gpu_batch_run", "mcm_gpu_batch_last_error", "mcm_gpu_batch_initial", "mcm_gpu_batch_step",
     "mcm_gpu_batch_time_steps", "mcm_gpu_batch_destroy", "mcm_gpu_output_fields", "mcm_vtk_write",
-    "mcm_gpu_write_vtk",
+    "mcm_gpu_write_vtk", "mcm_gpu_rep_cont",
 )
 
 OUT_WORDS = 17   # packed output record: x(3), v(3), p, sigma 11 22 33 12 13 23, e, efps, rho, h
@@ -77,7 +77,7 @@ def load_library(path: str = LIB_PATH):
     lib.mcm_gpu_destroy.restype = None
     lib.mcm_gpu_last_error.argtypes = [vp]
     lib.mcm_gpu_last_error.restype = ctypes.c_char_p
-    for name in ("initial", "neighbours", "strain", "update_h", "constitutive", "dt_update", "momentum",
+    for name in ("initial", "neighbours", "strain", "update_h", "constitutive", "dt_update", "rep_cont", "momentum",
                  "boundcond", "calctotale", "update_velocity", "move"):
         getattr(lib, "mcm_gpu_" + name).argtypes = [vp]
     lib.mcm_gpu_calccritts.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
@@ -211,6 +211,7 @@ class Solver:
     def update_h(self): self._stage("update_h")
     def constitutive(self): self._stage("constitutive")
     def dt_update(self): self._stage("dt_update")
+    def rep_cont(self): self._stage("rep_cont")
     def momentum(self): self._stage("momentum")
     def boundcond(self): self._stage("boundcond")
     def calctotale(self): self._stage("calctotale")
@@ -227,7 +228,7 @@ class Solver:
 
     def step_granular(self):
         """One step through the stage entry points, in mcm_solution's order."""
-        for name in ("neighbours", "strain", "update_h", "constitutive", "dt_update", "momentum",
+        for name in ("neighbours", "strain", "update_h", "constitutive", "dt_update", "rep_cont", "momentum",
                      "boundcond", "calctotale", "update_velocity", "move"):
             self._stage(name)
 

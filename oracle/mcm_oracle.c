@@ -50,6 +50,7 @@ static void alloc_particles(orc_state *st) {
                      &st->e, &st->etry, &st->einc, &st->c, &st->vabs, &st->tracerod, &st->pcut,
                      &st->p_cut, &st->efps, &st->fail, &st->mindist, &st->critts_p};
     for (size_t k = 0; k < sizeof(s1) / sizeof(s1[0]); k++) *s1[k] = xcalloc(n, 8);
+    st->rep = xcalloc(3 * n, 8);
     double **s9[] = {&st->sigma, &st->s, &st->alfa, &st->q, &st->qold, &st->rod, &st->spin};
     for (size_t k = 0; k < sizeof(s9) / sizeof(s9[0]); k++) *s9[k] = xcalloc(9 * n, 8);
     st->nnbr = xcalloc(n, sizeof(int)); st->g_nnbr = xcalloc(n, sizeof(int));
@@ -75,7 +76,7 @@ void orc_free(orc_state *st) {
                     st->g_nnbr, st->ncont, st->g_ncont, st->nbr, st->cont, st->gnbr, st->gcont, st->g_par,
                     st->g_mat, st->g_ll, st->g_x, st->g_xzero, st->g_v, st->g_mass, st->g_h, st->g_hold,
                     st->g_rho, st->g_sigma, st->g_q, st->mat, st->mat_ke, st->mat_ie, st->mat_mom,
-                    st->llgrid};
+                    st->llgrid, st->rep};
     for (size_t k = 0; k < sizeof(ptrs) / sizeof(ptrs[0]); k++) free(ptrs[k]);
     free(st);
 }
@@ -128,6 +129,7 @@ static void sets10(double *cm) {
 
 /* ------------------------------------------------------------------ deck */
 #define DECK_MAGIC 0x4D434D31
+#define CONTACT_TAG 0x434F4E54   /* 'CONT': penalty contact section after the particles */
 
 orc_state *orc_load_deck(const char *path, int ncap, int gcap) {
     FILE *f = fopen(path, "rb");
@@ -140,6 +142,7 @@ orc_state *orc_load_deck(const char *path, int ncap, int gcap) {
     st->np = ints[0]; st->nummat = ints[1]; st->disctype = ints[2]; st->h_opt = ints[3];
     st->tcrit_opt = ints[4]; st->contacttype = ints[5]; st->krtype = ints[6];
     st->massopt = ints[8]; st->init_h_opt = ints[9]; st->init_rhoe = ints[10];
+    st->veloc_opt = ints[18];
     int mirr_opt = ints[11];
     const int *flag = &ints[12];
     st->tssf = reals[0]; st->itss = reals[1]; st->endtime = reals[2];
@@ -188,6 +191,19 @@ orc_state *orc_load_deck(const char *path, int ncap, int gcap) {
     for (long i = 0; i < n; i++) st->dispbc[i] = ibuf[i];
     ok = ok && fread(pm, 8, n, f) == (size_t)n && fread(ph, 8, n, f) == (size_t)n &&
          fread(pr, 8, n, f) == (size_t)n && fread(pe, 8, n, f) == (size_t)n;
+    if (ok && st->contacttype > 0) {
+        /* penalty contact cards, mcm_contact_in.f90:22-45 (binary section after the particles) */
+        int32_t ct[3];
+        ok = fread(ct, 4, 3, f) == 3 && ct[0] == CONTACT_TAG;
+        if (ok) { st->ncontmats = ct[1]; st->cont_opt = ct[2]; }
+        for (int m = 0; ok && m < st->nummat; m++) {
+            double kn[2];
+            ok = fread(kn, 8, 2, f) == 2;
+            st->mat[m].k_cont = kn[0]; st->mat[m].n_cont = kn[1];
+        }
+        /* only "every material in contact" sets K and n; otherwise they are unset */
+        if (ok && st->ncontmats != -1) set_err(st, ORC_ERR_MODEL);
+    }
     fclose(f);
     if (!ok) { free(ibuf); free(pm); free(ph); free(pr); free(pe); orc_free(st); return NULL; }
     for (long i = 0; i < n; i++) {   /* mcm_nodein.f90:118-135 */
@@ -1253,6 +1269,8 @@ static void nbr_moment_info(const orc_state *st, int i, int k, double *xj, doubl
 static void oldacceleration(orc_state *st) {
     for (int i = 0; i < st->np; i++) {
         for (int d = 0; d < 3; d++) V3(st->a, i, d) = 0.0;
+        if (st->contacttype > 0)   /* :49-56 */
+            for (int d = 0; d < 3; d++) V3(st->a, i, d) = V3(st->a, i, d) + V3(st->rep, i, d);
         double rhoi = st->rho[i];
         const double *sgi = &st->sigma[9 * (long)i], *qi = &st->q[9 * (long)i];
         int nk = st->nnbr[i] + st->g_nnbr[i];
@@ -1276,6 +1294,8 @@ static void oldacceleration(orc_state *st) {
 static void corr_acceleration(orc_state *st) {
     for (int i = 0; i < st->np; i++) {
         for (int d = 0; d < 3; d++) V3(st->a, i, d) = 0.0;
+        if (st->contacttype > 0)   /* :45-49 */
+            for (int d = 0; d < 3; d++) V3(st->a, i, d) = V3(st->a, i, d) + V3(st->rep, i, d);
         const double *xi = &V3(st->x, i, 0);
         double rhoi = st->rho[i];
         const double *sgi = &st->sigma[9 * (long)i], *qi = &st->q[9 * (long)i];
@@ -1329,6 +1349,42 @@ static void corr_acceleration(orc_state *st) {
     }
 }
 
+/* mcm_rep_contact.f90:20-98 with mcm_calc_fij (:100-161) and
+   mcm_get_j_cont_info.f90:26-42: Monaghan's repulsive term over the real
+   contact partners (other materials within 2h), B-spline normalisation at
+   the partner's spacing, exponent n of particle i's material */
+int orc_rep_cont(orc_state *st) {
+    for (int i = 0; i < st->np; i++) {
+        for (int d = 0; d < 3; d++) V3(st->rep, i, d) = 0.0;
+        if (st->ncont[i] <= 0) continue;
+        if (st->g_ncont[i] > 0) { set_err(st, ORC_ERR_UNDEFINED); continue; }
+        const double ki = st->mat[st->pmat[i]].k_cont, n = st->mat[st->pmat[i]].n_cont;
+        const double *xi = &V3(st->x, i, 0);
+        for (int k = 0; k < st->ncont[i]; k++) {
+            int j = st->cont[(long)i * st->ncap + k];
+            const double kj = st->mat[st->pmat[j]].k_cont;
+            const double *xj = &V3(st->x, j, 0);
+            double massj = st->mass[j], hj = st->h[j];
+            double havg = 0.5 * (st->h[i] + hj);
+            double ratio = hj / havg, inv_w;
+            if (ratio < 1.0) inv_w = 1.0 / (((0.75 * ratio - 1.5) * ratio) * ratio + 1.0);
+            else { double t = 2.0 - ratio; inv_w = 1.0 / (t * (t * t)); }
+            double dwdx[3];
+            orc_kernel_gradw(xi, xj, havg, st->pi, dwdx);
+            double rad = 0.0;
+            for (int d = 0; d < 3; d++) { double xij = xi[d] - xj[d]; rad = rad + xij * xij; }
+            rad = sqrt(rad);
+            double z = rad / havg, fij;
+            if (z < 1.0) fij = inv_w * (((0.75 * z - 1.5) * z) * z + 1.0);
+            else if (z < 2.0) { double t = 2.0 - z; fij = 0.25 * inv_w * (t * (t * t)); }
+            else fij = 0.0;
+            double kij = ki + kj;
+            for (int l = 0; l < 3; l++) V3(st->rep, i, l) = V3(st->rep, i, l) - massj * (kij * pow(fij, n)) * dwdx[l];
+        }
+    }
+    return st->error;
+}
+
 /* mcm_momentum.f90:22-43 */
 int orc_momentum(orc_state *st) {
     if (st->boundary) update_ghost_stress(st);
@@ -1370,7 +1426,7 @@ int orc_calctotale(orc_state *st) {
     return st->error;
 }
 
-/* mcm_update_velocity.f90:29-56 (no dynamic relaxation, no XSPH) */
+/* mcm_update_velocity.f90:29-87 (no dynamic relaxation; XSPH when veloc_opt = 1) */
 int orc_update_velocity(orc_state *st) {
     double dtn = 0.5 * (st->dt + st->dtold);
     for (int i = 0; i < st->np; i++) {
@@ -1380,6 +1436,26 @@ int orc_update_velocity(orc_state *st) {
             vabs = vabs + V3(st->v, i, j) * V3(st->v, i, j);
         }
         st->vabs[i] = sqrt(vabs);
+    }
+    if (st->veloc_opt == 1) {
+        /* XSPH, :61-87: every particle's smoothing from the new velocities,
+           then all applied; the kernel at the partner's h, the volume over
+           the mean density; |v| keeps the unsmoothed value */
+        const double eps = 0.100;
+        double *sv = xcalloc(3 * (size_t)st->np, 8);
+        for (int i = 0; i < st->np; i++) {
+            int nk = st->nnbr[i] + st->g_nnbr[i];
+            for (int k = 0; k < nk; k++) {
+                double xj[3], vj[3], mj, rhoj, hj;
+                nbr_strain_info(st, i, k, xj, vj, &mj, &rhoj, &hj);
+                double w = orc_kernel_w(&V3(st->x, i, 0), xj, hj, st->pi);
+                double volj = mj / (0.5 * (rhoj + st->rho[i]));
+                for (int d = 0; d < 3; d++) V3(sv, i, d) = V3(sv, i, d) + volj * (vj[d] - V3(st->v, i, d)) * w;
+            }
+        }
+        for (int i = 0; i < st->np; i++)
+            for (int d = 0; d < 3; d++) V3(st->v, i, d) = V3(st->v, i, d) + eps * V3(sv, i, d);
+        free(sv);
     }
     return st->error;
 }
@@ -1595,6 +1671,7 @@ int orc_step(orc_state *st) {
     if (st->h_opt > 0) orc_update_h(st);
     orc_constitutive(st);
     orc_dt_update(st);
+    if (st->contacttype > 0) orc_rep_cont(st);   /* mcm_solution.f90:64 */
     orc_momentum(st);
     orc_boundcond(st);
     orc_calctotale(st);

@@ -16,7 +16,10 @@
  * Parity unpinned: branches restated here that no reference digest
  * exercises -- f3dm10 spall type 3 and type-1 pcut clamping of a failed
  * particle, the rho_max clamp of rhoupd, and drelax_scale in
- * update_velocity (never set by the decks).
+ * update_velocity (never set by the decks).  Penalty contact with ghost
+ * contacts is not restated: the reference reads its ghost contact partners
+ * from the real contact list past its end (mcm_get_j_cont_info.f90:44-48),
+ * so its result is undefined there; rep_cont stops with ORC_ERR_UNDEFINED.
  *
  * Conventions: 0-based particle and ghost indices; 3x3 tensors stored
  * column-major, element (r,c) at [9*i + r + 3*c]; vectors at [3*i + d].
@@ -33,12 +36,15 @@ typedef struct {
     double rho, g, av_l, av_q, mass, h, rho_min, rho_max;
     double strinput[48];
     double eosinput[24];
+    double k_cont, n_cont;      /* penalty contact K and n (mcm_contact_in.f90) */
 } orc_material;
 
 typedef struct {
     /* control */
     int np, nummat, disctype, h_opt, tcrit_opt, contacttype, krtype;
     int massopt, init_h_opt, init_rhoe, boundary;
+    int veloc_opt;              /* 1: XSPH velocity smoothing */
+    int ncontmats, cont_opt;    /* penalty contact cards */
     int boundary_code[3][2];   /* [dim][0=min,1=max]: 0 none, 1 symmetry, 2 periodic */
     int boundary_type[3];
     double boundary_x[3][2];
@@ -60,6 +66,7 @@ typedef struct {
     double *mass, *h, *hold, *h0, *rho, *rho0, *rhoold, *p, *e, *etry, *einc, *c, *vabs;
     double *tracerod, *pcut, *p_cut, *efps, *fail, *mindist, *critts_p;
     double *sigma, *s, *alfa, *q, *qold, *rod, *spin; /* 9 per particle */
+    double *rep;                /* 3 per particle: contact acceleration of mcm_rep_cont */
     /* neighbour lists (0-based ids), capacity ncap / gcap per particle */
     int ncap, gcap;
     int *nnbr, *g_nnbr, *ncont, *g_ncont;
@@ -82,6 +89,7 @@ typedef struct {
 #define ORC_ERR_CAPACITY -2
 #define ORC_ERR_MODEL -3
 #define ORC_ERR_IO -4
+#define ORC_ERR_UNDEFINED -5   /* penalty contact with ghost contacts: the reference reads an undefined list entry */
 
 orc_state *orc_load_deck(const char *path, int ncap, int gcap);
 void orc_free(orc_state *st);
@@ -91,6 +99,7 @@ int orc_strain(orc_state *st);
 int orc_update_h(orc_state *st);
 int orc_constitutive(orc_state *st);
 int orc_dt_update(orc_state *st);
+int orc_rep_cont(orc_state *st);
 int orc_momentum(orc_state *st);
 int orc_boundcond(orc_state *st);
 int orc_calctotale(orc_state *st);

@@ -32,7 +32,7 @@ def _lib():
     lib.orc_load_deck.restype = ctypes.c_void_p
     lib.orc_load_deck.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int]
     lib.orc_free.argtypes = [ctypes.c_void_p]
-    for name in ("orc_initial", "orc_step") + tuple("orc_" + s for s in STAGES):
+    for name in ("orc_initial", "orc_step", "orc_rep_cont") + tuple("orc_" + s for s in STAGES):
         getattr(lib, name).argtypes = [ctypes.c_void_p]
         getattr(lib, name).restype = ctypes.c_int
     lib.orc_run.argtypes = [ctypes.c_void_p, ctypes.c_int]
@@ -66,6 +66,10 @@ class Oracle:
         if not self.h:
             raise IOError("oracle could not load deck %s" % deck_path)
         self.step_no = 0
+        # penalty contact (deck int 6, contacttype > 0): mcm_solution calls
+        # mcm_rep_cont between the dt update and mcm_momentum
+        with open(deck_path, "rb") as f:
+            self.contact = int.from_bytes(f.read(28)[24:28], "little", signed=True) > 0
 
     def close(self):
         if self.h:

@@ -79,7 +79,7 @@ contains
 
  subroutine probe_read_deck(fname)
   character(len=*) :: fname
-  integer :: magic, u, m, n, j
+  integer :: magic, u, m, n, j, ios, tag
   integer :: mirr_opt, sym_flag(6), nst
   integer :: model, eos, visc
   real(kind=real_acc) :: rho, avq, avl, mass, h, rmin, rmax
@@ -103,7 +103,7 @@ contains
   mcm_thnode = 0
   mcm_num_transducer = 0
   mcm_status_interval = 100
-  mcm_veloc_opt = 0
+  mcm_veloc_opt = nst
   mcm_init_v_opt = 1
   mcm_nlcur = 0
   mcm_nthpx = 0; mcm_nthpy = 0; mcm_nthpz = 0
@@ -174,6 +174,16 @@ contains
   read(u) matin
   read(u) bcin
   read(u) pm, ph, pr, pe
+  ! optional penalty-contact section (mcm_contact_in.f90 cards): 'CONT',
+  ! ncontmats, cont_opt, then K and n per material
+  if (mcm_contacttype.gt.0) then
+    read(u, iostat=ios) tag
+    if (ios.ne.0 .or. tag.ne.1129270868) stop 'contact deck without contact section'
+    read(u) mcm_ncontmats, mcm_cont_opt
+    do m = 1, mcm_nummat
+      read(u) mcm_k_cont(m), mcm_n_cont(m)
+    enddo
+  endif
   close(u)
   do j = 1, mcm_np
     par(j)%x = xin(:,j)
@@ -274,6 +284,7 @@ contains
   mcm_dt = min(mcm_dt, 1.1_d*mcm_dtold)
   mcm_ptime = mcm_ptime + mcm_dt
   if (dump) call probe_dump(od, istep, 5)
+  if (mcm_contacttype.gt.0) call mcm_rep_cont
   call mcm_momentum
   if (dump) call probe_dump(od, istep, 6)
   call mcm_boundcond

@@ -97,6 +97,28 @@ def build(name: str) -> deck.Deck:
         v = np.zeros((len(pts), 3))
         v[:, 2] = np.where(pts[:, 2] > 3 * dx, 150.0, -150.0)
         return deck.make_deck(pts, v, [m], jitter_dx=0.01 * dx)
+    if name == "xsph_corr_f3dm10_eos4_sym":
+        # corrected bar on its symmetry plane with XSPH velocity smoothing: the
+        # smoothing sums run over reals, self and ghosts (ghost velocities are
+        # the mirrored start-of-step ones)
+        d = deck.taylor_bar(radius=0.8e-3, length=3.2e-3, model="f3dm10", corrected=True, h_opt=1)
+        d.veloc_opt = 1
+        return d
+    if name == "penalty_contact_f3dm1":
+        # two elastic blocks of different materials closing on each other with
+        # Monaghan's penalty contact (contacttype 1): the materials do not see
+        # each other through the kernel, only through the repulsion
+        a = deck.block(dx, 4, 4, 3)
+        b = deck.block(dx, 4, 4, 3, origin=(0.5 * dx, 0.5 * dx, 3.3 * dx))
+        pts = np.concatenate([a, b])
+        mid = np.concatenate([np.ones(len(a), np.int32), 2 * np.ones(len(b), np.int32)])
+        v = np.zeros((len(pts), 3))
+        v[: len(a), 2] = 60.0
+        v[len(a):, 2] = -60.0
+        m1 = deck.material(1, 7850.0, 7850.0 * dx ** 3 * len(a), h, deck.f3dm1(200e9, 0.3))
+        m2 = deck.material(1, 2700.0, 2700.0 * dx ** 3 * len(b), h, deck.f3dm1(70e9, 0.33))
+        return deck.make_deck(pts, v, [m1, m2], matid=mid, jitter_dx=0.01 * dx, contacttype=1,
+                              contact_k=[1.5e3, 2.5e3], contact_n=[4.0, 3.5])
     raise KeyError(name)
 
 
@@ -111,4 +133,6 @@ CASES = {
     "gas_f3dm10_eos13": (30, (2,)),
     "two_materials_contact": (30, (2,)),
     "f3dm10_spall2_failure": (30, (2,)),
+    "xsph_corr_f3dm10_eos4_sym": (30, (2,)),
+    "penalty_contact_f3dm1": (30, (2,)),
 }

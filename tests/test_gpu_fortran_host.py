@@ -18,10 +18,13 @@ DATA = os.path.join(ROOT, "tests", "golden", "data")
 
 
 @pytest.mark.parametrize("mode", ["stages", "fused"])
-def test_fortran_host_runs_the_gpu_step(mode, tmp_path):
+@pytest.mark.parametrize("name", ["corr_f3dm10_eos4_hopt1", "penalty_contact_f3dm1", "xsph_corr_f3dm10_eos4_sym"])
+def test_fortran_host_runs_the_gpu_step(name, mode, tmp_path):
+    """The loop of mcm_solution through the stage subroutines (mcm_rep_cont
+    included) and the fused step; penalty contact and XSPH decks too."""
     if not os.path.exists(EXE):
         pytest.fail("Fortran host not built (__graft_entry__.build())")
-    deck = os.path.join(DATA, "corr_f3dm10_eos4_hopt1.bin")
+    deck = os.path.join(DATA, name + ".bin")
     dump = str(tmp_path / "f.bin")
     out = subprocess.run([EXE, deck, "15", mode, dump], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stdout + out.stderr

@@ -93,9 +93,53 @@ def test_unsupported_options_are_rejected(tmp_path):
     base = str(tmp_path / "b.mcm")
     dk.write_mcm(base)
     lines = open(base).read().split("\n")
-    # control card 5, third field: XSPH
-    lines[5] = lines[5][:10] + "%5d" % 1 + lines[5][15:]
+    # control card 2, fourth field: dynamic relaxation
+    drelax = list(lines)
+    drelax[2] = drelax[2][:30] + "%10.3f" % 0.9 + drelax[2][40:]
     bad = str(tmp_path / "x.mcm")
-    open(bad, "w").write("\n".join(lines))
-    with pytest.raises(gpu.McmGpuError, match="XSPH"):
+    open(bad, "w").write("\n".join(drelax))
+    with pytest.raises(gpu.McmGpuError, match="dynamic relaxation"):
         gpu.convert_deck(bad, str(tmp_path / "x.bin"))
+    # control card 1, first field: a 2-D problem
+    twod = list(lines)
+    twod[1] = "%5d" % 2 + twod[1][5:]
+    open(bad, "w").write("\n".join(twod))
+    with pytest.raises(gpu.McmGpuError, match="3-D"):
+        gpu.convert_deck(bad, str(tmp_path / "x.bin"))
+
+
+# Contact material cards are read with (2E10.5) (mcm_contact_in.f90:34,110):
+# a mantissa without a point keeps its last 5 digits after the implied point.
+# Values as flang reads them (ROCm flang, read(card, '(2E10.5)')).
+E10_5 = [("    150000     15E+3", 1.5, 0.15), ("      -250   1.5    ", -2.50000000000000005e-03, 1.5),
+         ("    1.5D+3        4 ", 1500.0, 4.00000000000000033e-05),
+         ("         7  -12345-2", 6.99999999999999939e-05, -1.23449999999999992e-03),
+         ("       1 5    +15000", 1.49999999999999987e-04, 1.49999999999999994e-01),
+         ("                 .5 ", 0.0, 0.5), ("   123456789 0      ", 1.23456700000000001e+01, 8.89999999999999993e-03)]
+
+
+@pytest.mark.parametrize("card,k,n", E10_5)
+def test_contact_cards_use_implied_decimals(card, k, n, tmp_path):
+    dk = cases.build("penalty_contact_f3dm1")
+    base = str(tmp_path / "c.mcm")
+    dk.write_mcm(base)
+    lines = open(base).read().rstrip("\n").split("\n")
+    assert lines[-3].split() == ["-1", "0"]       # ncontmats, cont_opt
+    lines[-2] = card
+    lines[-1] = card
+    got = _read_back(tmp_path, "\n".join(lines) + "\n")
+    assert got.contacttype == 1 and got.ncontmats == -1
+    assert got.contact_k == [k, k] and got.contact_n == [n, n]
+
+
+def test_xsph_and_contact_decks_round_trip(tmp_path):
+    """veloc_opt and the contact section survive text -> binary -> Deck."""
+    for name in ("xsph_corr_f3dm10_eos4_sym", "penalty_contact_f3dm1"):
+        dk = cases.build(name)
+        txt = str(tmp_path / "r.mcm")
+        dk.write_mcm(txt)
+        out = str(tmp_path / "r.bin")
+        gpu.convert_deck(txt, out)
+        back = deck.Deck.read_bin(out)
+        assert back.veloc_opt == dk.veloc_opt and back.contacttype == dk.contacttype
+        assert back.contact_k == dk.contact_k and back.contact_n == dk.contact_n
