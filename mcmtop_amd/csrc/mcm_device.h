@@ -78,6 +78,8 @@ struct DevState {
     int nblk;                  // workgroups of MCM_BLOCK threads over np (per-particle launches)
     int contacttype;           // > 0: penalty contact (separate contact lists, mcm_rep_cont)
     int veloc_opt;             // 1: XSPH velocity smoothing
+    int drelax, baseaccel;     // dynamic relaxation; base acceleration
+    double drelax_scale, base_a[3];
     int bcode[3][2];
     int btype[3];
     double bx[3][2];

@@ -38,6 +38,9 @@ struct HostDeck {
     int maxnbr = 0, massopt = 0, init_h_opt = 0, init_rhoe = 0, mirr_opt = 0;
     int veloc_opt = 0;                     // 1: XSPH
     int ncontmats = -1, cont_opt = 0;      // penalty contact cards (K and n in MatParams)
+    double drelax_scale = 0.0;             // dynamic relaxation when 0.1 < scale < 1
+    int nthp[3] = {0, 0, 0};               // base acceleration on where 1
+    double base_a[3] = {0.0, 0.0, 0.0};
     int sym_flag[6] = {0, 0, 0, 0, 0, 0};
     double tssf = 0, itss = 0, endtime = 0, sym_x[6] = {0, 0, 0, 0, 0, 0};
     std::vector<MatParams> mat;
@@ -75,6 +78,7 @@ namespace {
 // ------------------------------------------------------------------- deck
 constexpr int32_t kDeckMagic = 0x4D434D31;
 constexpr int32_t kContactTag = 0x434F4E54;   // 'CONT': penalty contact section after the particles
+constexpr int32_t kOptsTag = 0x4F505453;      // 'OPTS': dynamic relaxation and base acceleration section
 
 template <typename T>
 bool rd(FILE* f, T* p, size_t n) { return fread(p, sizeof(T), n, f) == n; }
@@ -236,6 +240,13 @@ int read_binary_deck(mcm_gpu_ctx* ctx, FILE* f) {
             D.mat[m].n_cont = kn[1];
         }
     }
+    int32_t ot[4];
+    if (rd(f, ot, 4) && ot[0] == kOptsTag) {   // dynamic relaxation and base acceleration
+        double ov[4];
+        if (!rd(f, ov, 4)) { ctx->err = "deck truncated in its options section"; return MCM_GPU_ERR_IO; }
+        for (int k = 0; k < 3; k++) { D.nthp[k] = ot[1 + k]; D.base_a[k] = ov[1 + k]; }
+        D.drelax_scale = ov[0];
+    }
     return MCM_GPU_OK;
 }
 
@@ -277,8 +288,9 @@ int read_text_deck(mcm_gpu_ctx* ctx, FILE* f) {
     { Fmt F(card); D.h_opt = F.i(5); D.krtype = F.i(5); D.maxnbr = F.i(5); if (!F.ok) return bad("control card 6"); }
     if ((rc = next("control card 7"))) return rc;
     if (axopt != 3) return bad("only 3-D problems (axopt 3) run on the GPU path", MCM_GPU_ERR_ARG);
-    if (drelax != 0.0) return bad("dynamic relaxation is not on the GPU path", MCM_GPU_ERR_ARG);
-    if (nlcur != 0 || nthp[0] || nthp[1] || nthp[2]) return bad("load curves / base acceleration are not on the GPU path", MCM_GPU_ERR_ARG);
+    D.drelax_scale = drelax;
+    for (int k = 0; k < 3; k++) D.nthp[k] = (int)nthp[k];
+    (void)nlcur;   // load curves are read by nothing on the step's path
     if (D.np <= 0 || D.nummat <= 0 || D.nummat > MCM_MAX_MAT) return bad("bad particle or material count", MCM_GPU_ERR_ARG);
     // material cards (mcm_matin.f90:38-203)
     D.mat.assign(D.nummat, MatParams{});
@@ -421,6 +433,13 @@ int read_text_deck(mcm_gpu_ctx* ctx, FILE* f) {
                 if (!F.ok) return bad("contact material card");
             }
     }
+    // base acceleration card (mcm_baseaccel_in.f90:25-31): (3e20.0)
+    if (D.nthp[0] == 1 || D.nthp[1] == 1 || D.nthp[2] == 1) {
+        if ((rc = next("base acceleration card"))) return rc;
+        Fmt F(card);
+        for (int k = 0; k < 3; k++) D.base_a[k] = F.e(20);
+        if (!F.ok) return bad("base acceleration card");
+    }
     return MCM_GPU_OK;
 }
 
@@ -472,6 +491,11 @@ int write_binary_deck(const HostDeck& D, const char* path) {
             const double kn[2] = {D.mat[m].k_cont, D.mat[m].n_cont};
             ok = fwrite(kn, 8, 2, f) == 2;
         }
+    }
+    if (ok && (D.drelax_scale != 0.0 || D.nthp[0] || D.nthp[1] || D.nthp[2])) {
+        const int32_t ot[4] = {kOptsTag, D.nthp[0], D.nthp[1], D.nthp[2]};
+        const double ov[4] = {D.drelax_scale, D.base_a[0], D.base_a[1], D.base_a[2]};
+        ok = fwrite(ot, 4, 4, f) == 4 && fwrite(ov, 8, 4, f) == 4;
     }
     ok = fclose(f) == 0 && ok;
     return ok ? MCM_GPU_OK : MCM_GPU_ERR_IO;
@@ -1014,6 +1038,11 @@ int setup_run(mcm_gpu_ctx* ctx, const mcm_gpu_options* opt) {
     d.tcrit_opt = D.tcrit_opt;
     d.contacttype = D.contacttype > 0 ? D.contacttype : 0;
     d.veloc_opt = D.veloc_opt;
+    // mcm_control_v2.f90:223-243, mcm_baseaccel_in.f90:33-35
+    d.drelax = D.drelax_scale > 0.1 && D.drelax_scale < 1.0;
+    d.drelax_scale = D.drelax_scale;
+    d.baseaccel = D.nthp[0] == 1 || D.nthp[1] == 1 || D.nthp[2] == 1;
+    for (int k = 0; k < 3; k++) d.base_a[k] = D.nthp[k] == 1 ? D.base_a[k] : 0.0;
     d.tssf = D.tssf;
     d.pi = acos(-1.0);
     d.ngcap = (opt && opt->ghost_capacity > 0) ? opt->ghost_capacity : D.np / 2 + 4096;

@@ -783,6 +783,8 @@ template <bool DIAG, bool MOVE>
 __device__ __forceinline__ void integrate_particle(const DevState& d, int i, double acc[3], const double xi[3],
                                                    CPart& cp) {
     Scalars* sc = d.sc;
+    if (d.baseaccel)   // mcm_boundcond.f90:23-29
+        for (int a = 0; a < 3; a++) acc[a] = acc[a] + d.base_a[a];
     boundcond(d.dispbc[i], acc);
     const int mi = d.pmat[i];
     const double mass = d.mass[i];
@@ -794,7 +796,7 @@ __device__ __forceinline__ void integrate_particle(const DevState& d, int i, dou
     double dtn = 0.5 * (sc->dt + sc->dtold);
     double va = 0.0;
     for (int a = 0; a < 3; a++) {
-        v[a] = v[a] + acc[a] * dtn;
+        v[a] = (d.drelax ? d.drelax_scale * v[a] : v[a]) + acc[a] * dtn;   // :31-41 with dynamic relaxation
         va = va + v[a] * v[a];
     }
     d.vabs[i] = sqrt(va);
@@ -973,6 +975,8 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_boundcond(DevState d) {
     const int i = blockIdx.x * MCM_BLOCK + threadIdx.x;
     if (i >= d.np || d.sc->abort) return;
     double acc[3] = {d.a[0][i], d.a[1][i], d.a[2][i]};
+    if (d.baseaccel)   // mcm_boundcond.f90:23-29
+        for (int a = 0; a < 3; a++) acc[a] = acc[a] + d.base_a[a];
     boundcond(d.dispbc[i], acc);
     for (int a = 0; a < 3; a++) d.a[a][i] = acc[a];
 }
@@ -997,7 +1001,7 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_update_velocity(DevState d) {
     double dtn = 0.5 * (sc->dt + sc->dtold);
     double va = 0.0;
     for (int a = 0; a < 3; a++) {
-        double v = d.v[a][i] + d.a[a][i] * dtn;
+        double v = (d.drelax ? d.drelax_scale * d.v[a][i] : d.v[a][i]) + d.a[a][i] * dtn;
         d.v[a][i] = v;
         va = va + v * v;
     }

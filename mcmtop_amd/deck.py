@@ -30,6 +30,7 @@ import numpy as np
 
 DECK_MAGIC = int.from_bytes(b"MCM1", "big")
 CONTACT_TAG = int.from_bytes(b"CONT", "big")   # optional penalty-contact section after the particles
+OPTS_TAG = int.from_bytes(b"OPTS", "big")      # optional section: dynamic relaxation, base acceleration
 
 
 def _fit(value: float, width: int) -> float:
@@ -104,6 +105,9 @@ class Deck:
     cont_opt: int = 0
     contact_k: Optional[List[float]] = None   # per material K and n of mcm_rep_cont
     contact_n: Optional[List[float]] = None
+    drelax_scale: float = 0.0   # dynamic relaxation when 0.1 < scale < 1 (mcm_control_v2.f90:223-225)
+    nthp: tuple = (0, 0, 0)     # base acceleration on in x, y, z where 1 (mcm_control_v2.f90:241-243)
+    base_a: tuple = (0.0, 0.0, 0.0)
 
     @property
     def mirr_opt(self) -> int:
@@ -146,6 +150,9 @@ class Deck:
             k = list(self.contact_k or [0.0] * nm)
             c = list(self.contact_n or [0.0] * nm)
             f.write(struct.pack("<%dd" % (2 * nm), *[v for pair in zip(k, c) for v in pair]))
+        if self.drelax_scale != 0.0 or any(self.nthp):
+            f.write(struct.pack("<4i", OPTS_TAG, *[int(t) for t in self.nthp]))
+            f.write(struct.pack("<4d", self.drelax_scale, *[float(a) for a in self.base_a]))
         return f.getvalue()
 
     @staticmethod
@@ -193,6 +200,10 @@ class Deck:
             _, dk.ncontmats, dk.cont_opt = take("<3i")
             kn = take("<%dd" % (2 * nummat))
             dk.contact_k, dk.contact_n = list(kn[0::2]), list(kn[1::2])
+        if off + 16 <= len(buf) and struct.unpack_from("<i", buf, off)[0] == OPTS_TAG:
+            dk.nthp = tuple(take("<4i")[1:])
+            vals = take("<4d")
+            dk.drelax_scale, dk.base_a = vals[0], tuple(vals[1:])
         return dk
 
     # -------------------------------------------------------------------- text
@@ -202,11 +213,11 @@ class Deck:
         lines = []
         lines.append(self.title[:78].ljust(78) + "%2d" % 2)
         lines.append("%5d%5d%10d%10d%10d%10d%10d" % (3, self.disctype, len(self.materials), n, n, 0, 0))
-        lines.append(_fmt(self.endtime, 10) + _fmt(self.tssf, 10) + _fmt(self.itss, 10) + _fmt(0.0, 10))
+        lines.append(_fmt(self.endtime, 10) + _fmt(self.tssf, 10) + _fmt(self.itss, 10) + _fmt(self.drelax_scale, 10))
         lines.append(_fmt(1.0e9, 10) + "%5d" % 0 + _fmt(1.0e9, 10) + "%10d%5d%10d%5d%5d" % (0, 0, 100, 0, 0))
         lines.append("%5d%5d%5d%5d%5d" % (1, 1, self.massopt, self.init_h_opt, self.init_rhoe))
         lines.append("%5d%5d%5d%5d%5d%5d%5d%5d" % (self.contacttype, self.tcrit_opt, self.veloc_opt, self.mirr_opt,
-                                                    0, 0, 0, 0))
+                                                    0, *[int(t) for t in self.nthp]))
         lines.append("%5d%5d%5d" % (self.h_opt, self.krtype, self.maxnbr))
         lines.append("")
         for k, m in enumerate(self.materials, start=1):
@@ -252,6 +263,8 @@ class Deck:
                 nm = len(self.materials)
                 for k, c in zip(self.contact_k or [0.0] * nm, self.contact_n or [0.0] * nm):
                     lines.append(_fmt(k, 10) + _fmt(c, 10))
+        if any(int(t) == 1 for t in self.nthp):
+            lines.append("".join(_fmt(a, 20) for a in self.base_a))
         with open(path, "w") as f:
             f.write("\n".join(lines) + "\n")
 
@@ -438,7 +451,8 @@ def block(dx, nx, ny, nz, origin=(0.0, 0.0, 0.0)):
 
 def make_deck(pts, vel, mats, matid=None, dispbc=None, disctype=0, h_opt=0, tcrit_opt=1, tssf=0.3,
               sym_flag=(0, 0, 0, 0, 0, 0), sym_x=(0, 0, 0, 0, 0, 0), seed=1, jitter_dx=0.0,
-              massopt=0, pmass=None, veloc_opt=0, contacttype=0, contact_k=None, contact_n=None) -> Deck:
+              massopt=0, pmass=None, veloc_opt=0, contacttype=0, contact_k=None, contact_n=None,
+              drelax_scale=0.0, nthp=(0, 0, 0), base_a=(0.0, 0.0, 0.0)) -> Deck:
     """Generic deck from points, velocities and materials (1-based matid);
     veloc_opt 1 turns on XSPH, contacttype > 0 penalty contact with K and n
     per material."""
@@ -455,6 +469,9 @@ def make_deck(pts, vel, mats, matid=None, dispbc=None, disctype=0, h_opt=0, tcri
              bc, np.asarray(sym_flag, np.int32), np.array([_fit(s, 10) if s else 0.0 for s in sym_x]),
              massopt=massopt, veloc_opt=veloc_opt)
     d.contacttype = contacttype
+    d.drelax_scale = _fit(drelax_scale, 10) if drelax_scale else 0.0
+    d.nthp = tuple(int(t) for t in nthp)
+    d.base_a = tuple(_fit(a, 20) if a else 0.0 for a in base_a)
     if contacttype > 0:
         d.contact_k = [_fit(k, 10) for k in contact_k]
         d.contact_n = [_fit(c, 10) for c in contact_n]

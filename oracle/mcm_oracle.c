@@ -130,6 +130,7 @@ static void sets10(double *cm) {
 /* ------------------------------------------------------------------ deck */
 #define DECK_MAGIC 0x4D434D31
 #define CONTACT_TAG 0x434F4E54   /* 'CONT': penalty contact section after the particles */
+#define OPTS_TAG 0x4F505453      /* 'OPTS': dynamic relaxation and base acceleration section */
 
 orc_state *orc_load_deck(const char *path, int ncap, int gcap) {
     FILE *f = fopen(path, "rb");
@@ -203,6 +204,19 @@ orc_state *orc_load_deck(const char *path, int ncap, int gcap) {
         }
         /* only "every material in contact" sets K and n; otherwise they are unset */
         if (ok && st->ncontmats != -1) set_err(st, ORC_ERR_MODEL);
+    }
+    if (ok) {
+        /* dynamic relaxation (mcm_control_v2.f90:96,223-225) and base acceleration
+           (:112,241-243; mcm_baseaccel_in.f90:25-36) */
+        int32_t ot[4];
+        if (fread(ot, 4, 4, f) == 4 && ot[0] == OPTS_TAG) {
+            double ov[4];
+            ok = fread(ov, 8, 4, f) == 4;
+            st->drelax_scale = ov[0];
+            st->drelax = ov[0] > 0.1 && ov[0] < 1.0;
+            st->baseaccel = ot[1] == 1 || ot[2] == 1 || ot[3] == 1;
+            for (int d = 0; d < 3; d++) st->base_a[d] = ot[1 + d] == 1 ? ov[1 + d] : 0.0;
+        }
     }
     fclose(f);
     if (!ok) { free(ibuf); free(pm); free(ph); free(pr); free(pe); orc_free(st); return NULL; }
@@ -1396,6 +1410,9 @@ int orc_momentum(orc_state *st) {
 int orc_boundcond(orc_state *st) {
     static const int mask[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1},
                                    {1, 1, 0}, {0, 1, 1}, {1, 0, 1}, {1, 1, 1}};
+    if (st->baseaccel)   /* :23-29 */
+        for (int i = 0; i < st->np; i++)
+            for (int d = 0; d < 3; d++) V3(st->a, i, d) = V3(st->a, i, d) + st->base_a[d];
     for (int i = 0; i < st->np; i++) {
         int bc = st->dispbc[i];
         if (bc < 1 || bc > 7) continue;
@@ -1426,13 +1443,16 @@ int orc_calctotale(orc_state *st) {
     return st->error;
 }
 
-/* mcm_update_velocity.f90:29-87 (no dynamic relaxation; XSPH when veloc_opt = 1) */
+/* mcm_update_velocity.f90:29-87 (dynamic relaxation; XSPH when veloc_opt = 1) */
 int orc_update_velocity(orc_state *st) {
     double dtn = 0.5 * (st->dt + st->dtold);
     for (int i = 0; i < st->np; i++) {
         double vabs = 0.0;
         for (int j = 0; j < 3; j++) {
-            V3(st->v, i, j) = V3(st->v, i, j) + V3(st->a, i, j) * dtn;
+            if (st->drelax)   /* :31-41 */
+                V3(st->v, i, j) = st->drelax_scale * V3(st->v, i, j) + V3(st->a, i, j) * dtn;
+            else
+                V3(st->v, i, j) = V3(st->v, i, j) + V3(st->a, i, j) * dtn;
             vabs = vabs + V3(st->v, i, j) * V3(st->v, i, j);
         }
         st->vabs[i] = sqrt(vabs);

@@ -13,11 +13,8 @@
  * against closed forms and brute force).  Each function cites the
  * reference routine it follows (MCM/<file>.f90:<lines>).
  *
- * Parity unpinned: branches restated here that no reference digest
- * exercises -- f3dm10 spall type 3 and type-1 pcut clamping of a failed
- * particle, the rho_max clamp of rhoupd, and drelax_scale in
- * update_velocity (never set by the decks).  Penalty contact with ghost
- * contacts is not restated: the reference reads its ghost contact partners
+ * Every branch restated here is exercised by a reference digest.  Penalty
+ * contact with ghost contacts is not restated: the reference reads its ghost contact partners
  * from the real contact list past its end (mcm_get_j_cont_info.f90:44-48),
  * so its result is undefined there; rep_cont stops with ORC_ERR_UNDEFINED.
  *
@@ -45,6 +42,8 @@ typedef struct {
     int massopt, init_h_opt, init_rhoe, boundary;
     int veloc_opt;              /* 1: XSPH velocity smoothing */
     int ncontmats, cont_opt;    /* penalty contact cards */
+    int drelax, baseaccel;      /* dynamic relaxation; base acceleration */
+    double drelax_scale, base_a[3];
     int boundary_code[3][2];   /* [dim][0=min,1=max]: 0 none, 1 symmetry, 2 periodic */
     int boundary_type[3];
     double boundary_x[3][2];

@@ -184,6 +184,18 @@ contains
       read(u) mcm_k_cont(m), mcm_n_cont(m)
     enddo
   endif
+  ! optional 'OPTS' section: nthp(3), drelax_scale, base_a(3)
+  ! (mcm_control_v2.f90:96,112,223-243 and mcm_baseaccel_in.f90:25-36)
+  read(u, iostat=ios) tag
+  if (ios.eq.0 .and. tag.eq.1330664531) then
+    read(u) mcm_nthpx, mcm_nthpy, mcm_nthpz
+    read(u) mcm_drelax_scale, mcm_base_a
+    if (mcm_drelax_scale.gt.0.1_d.and.mcm_drelax_scale.lt.1.0_d) mcm_drelax = .true.
+    if (mcm_nthpx.eq.1.or.mcm_nthpy.eq.1.or.mcm_nthpz.eq.1) mcm_baseaccel = .true.
+    if (mcm_nthpx.ne.1) mcm_base_a(1) = 0.0_d
+    if (mcm_nthpy.ne.1) mcm_base_a(2) = 0.0_d
+    if (mcm_nthpz.ne.1) mcm_base_a(3) = 0.0_d
+  endif
   close(u)
   do j = 1, mcm_np
     par(j)%x = xin(:,j)

@@ -119,6 +119,43 @@ def build(name: str) -> deck.Deck:
         m2 = deck.material(1, 2700.0, 2700.0 * dx ** 3 * len(b), h, deck.f3dm1(70e9, 0.33))
         return deck.make_deck(pts, v, [m1, m2], matid=mid, jitter_dx=0.01 * dx, contacttype=1,
                               contact_k=[1.5e3, 2.5e3], contact_n=[4.0, 3.5])
+    if name == "drelax_baseaccel_f3dm1":
+        # elastic block on a held base layer under a base acceleration in z,
+        # with dynamic relaxation (velocity scale 0.9)
+        pts = deck.block(dx, 4, 4, 5)
+        rho = 7850.0
+        m = deck.material(1, rho, rho * dx ** 3 * len(pts), h, deck.f3dm1(200e9, 0.3))
+        bc = np.where(pts[:, 2] < dx, 7, 0).astype(np.int32)
+        return deck.make_deck(pts, (0.0, 0.0, 0.0), [m], dispbc=bc, jitter_dx=0.01 * dx,
+                              drelax_scale=0.9, nthp=(0, 0, 1), base_a=(0.0, 0.0, -2.0e9))
+    if name == "f3dm10_spall3_failure":
+        # spall type 3: the deviatoric stress and pcut switch off below the cut-off pressure
+        pts = deck.block(dx, 4, 4, 6)
+        rho = 7850.0
+        m = deck.material(10, rho, rho * dx ** 3 * len(pts), h,
+                          deck.f3dm10(80e9, 250e6, 1e9, pcut=-3e8, spall=3.0, epf=0.05), eos=4,
+                          eoscard={0: 4570.0, 1: 1.49, 4: 1.93})
+        v = np.zeros((len(pts), 3))
+        v[:, 2] = np.where(pts[:, 2] > 3 * dx, 150.0, -150.0)
+        return deck.make_deck(pts, v, [m], jitter_dx=0.01 * dx)
+    if name == "f3dm10_spall1_failure_pcut":
+        # spall type 1 (the default): particles fail (efps > epf) and the EOS
+        # clamps their tension at pcut
+        pts = deck.block(dx, 4, 4, 6)
+        rho = 7850.0
+        m = deck.material(10, rho, rho * dx ** 3 * len(pts), h,
+                          deck.f3dm10(80e9, 150e6, 2e8, pcut=-1e8, epf=0.005), eos=4,
+                          eoscard={0: 4570.0, 1: 1.49, 4: 1.93})
+        v = np.zeros((len(pts), 3))
+        v[:, 2] = np.where(pts[:, 2] > 3 * dx, 300.0, -300.0)
+        return deck.make_deck(pts, v, [m], jitter_dx=0.01 * dx)
+    if name == "rho_max_clamp_f3dm1":
+        # rhoupd's rho_max clamp: an elastic column striking its symmetry plane
+        # with rho_max just above the reference density
+        pts = deck.block(dx, 3, 3, 6)
+        rho = 2700.0
+        m = deck.material(1, rho, rho * dx ** 3 * len(pts), h, deck.f3dm1(70e9, 0.33), rho_max=2703.0)
+        return deck.make_deck(pts, (0.0, 0.0, -400.0), [m], jitter_dx=0.01 * dx, sym_flag=(0, 0, 0, 0, 1, 0))
     raise KeyError(name)
 
 
@@ -135,4 +172,8 @@ CASES = {
     "f3dm10_spall2_failure": (30, (2,)),
     "xsph_corr_f3dm10_eos4_sym": (30, (2,)),
     "penalty_contact_f3dm1": (30, (2,)),
+    "drelax_baseaccel_f3dm1": (30, (2,)),
+    "f3dm10_spall3_failure": (30, (2,)),
+    "f3dm10_spall1_failure_pcut": (30, (2,)),
+    "rho_max_clamp_f3dm1": (30, (2,)),
 }

@@ -93,19 +93,26 @@ def test_unsupported_options_are_rejected(tmp_path):
     base = str(tmp_path / "b.mcm")
     dk.write_mcm(base)
     lines = open(base).read().split("\n")
-    # control card 2, fourth field: dynamic relaxation
-    drelax = list(lines)
-    drelax[2] = drelax[2][:30] + "%10.3f" % 0.9 + drelax[2][40:]
     bad = str(tmp_path / "x.mcm")
-    open(bad, "w").write("\n".join(drelax))
-    with pytest.raises(gpu.McmGpuError, match="dynamic relaxation"):
-        gpu.convert_deck(bad, str(tmp_path / "x.bin"))
     # control card 1, first field: a 2-D problem
     twod = list(lines)
     twod[1] = "%5d" % 2 + twod[1][5:]
     open(bad, "w").write("\n".join(twod))
     with pytest.raises(gpu.McmGpuError, match="3-D"):
         gpu.convert_deck(bad, str(tmp_path / "x.bin"))
+
+
+def test_dynamic_relaxation_and_base_acceleration_cards(tmp_path):
+    """drelax_scale (control card 2), the nthp flags (control card 5) and the
+    base acceleration card survive text -> binary -> Deck."""
+    dk = cases.build("drelax_baseaccel_f3dm1")
+    txt = str(tmp_path / "d.mcm")
+    dk.write_mcm(txt)
+    out = str(tmp_path / "d.bin")
+    gpu.convert_deck(txt, out)
+    back = deck.Deck.read_bin(out)
+    assert back.drelax_scale == dk.drelax_scale == 0.9
+    assert back.nthp == (0, 0, 1) and back.base_a == dk.base_a
 
 
 # Contact material cards are read with (2E10.5) (mcm_contact_in.f90:34,110):
