@@ -13,9 +13,9 @@
 // I and E edit descriptors (blanks ignored, an empty field is zero, E input
 // accepts 'D' and sign-only exponents).
 //
-// Penalty contact cards (mcm_contact_in.f90:22-45) follow the boundary
-// planes.  Options the GPU path does not carry are rejected with an error:
-// 1-D/2-D problems, dynamic relaxation, load curves and base acceleration.
+// Penalty contact cards (mcm_contact_in.f90:22-45) and the base
+// acceleration card (mcm_baseaccel_in.f90:25-31) follow the boundary planes.
+// 1-D and 2-D problems are rejected with an error.
 #pragma once
 
 #include <cctype>
