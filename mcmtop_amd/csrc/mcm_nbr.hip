@@ -467,7 +467,7 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_ghost_fill(DevState d_, int pi, i
 // never skipped.  Skipping rejected candidates leaves the list, its order
 // and mindist unchanged.
 template <bool MULTI>
-__global__ __launch_bounds__(256) void k_list(DevState d_, int cur) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 3 : 5))) void k_list(DevState d_, int cur) {
     // threads take the binned particles in cell-sorted order, so that the
     // lanes of a wavefront share cells and read the same candidates
     Blk bk;
