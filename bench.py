@@ -129,20 +129,23 @@ def batched_leg(nruns: int, seed: int, steps: int, warmup: int, check_every: int
     b.initial()
     print("bench: batch initialised in %.1f s" % (time.perf_counter() - t), file=sys.stderr, flush=True)
     b.step(warmup)
+    mapping = b.tune(2)
+    tune_steps = 2 * (2 + 2)
     t0 = time.perf_counter()
     b.step(steps)
     secs_b = time.perf_counter() - t0
     ms_b = b.time_steps(2)
-    print("bench: batched %.2f ms/step" % (1e3 * secs_b / steps), file=sys.stderr, flush=True)
+    print("bench: batched %.2f ms/step (pair mapping %d)" % (1e3 * secs_b / steps, mapping),
+          file=sys.stderr, flush=True)
     sample = list(range(0, nruns, check_every))
     got = {k: (b.run(k).field(gpu.F_X), b.run(k).field(gpu.F_V), b.run(k).field(gpu.F_SIGMA)) for k in sample}
     b.close()
     secs_p = 0.0
     worst = 0.0
     for k, dk in enumerate(decks if plain else []):
-        with gpu.Solver(dk) as s:
+        with gpu.Solver(dk, pair_mapping=mapping) as s:
             s.initial()
-            s.step(warmup)
+            s.step(warmup + tune_steps)
             t0 = time.perf_counter()
             s.step(steps)
             secs_p += time.perf_counter() - t0
@@ -158,7 +161,7 @@ def batched_leg(nruns: int, seed: int, steps: int, warmup: int, check_every: int
     v_p = total * steps / secs_p if plain else None
     return {"runs": nruns, "particles": total, "seed": seed,
             "run_particles": {"min": int(sizes.min()), "median": float(np.median(sizes)), "max": int(sizes.max())},
-            "steps": steps, "warmup": warmup,
+            "steps": steps, "warmup": warmup, "pair_mapping": mapping,
             "value": v_b, "unit": "particle-steps/s", "ms_per_step": 1e3 * secs_b / steps,
             "plain_value": v_p, "plain_ms_per_step": 1e3 * secs_p / steps if plain else None,
             "speedup": v_b / v_p if plain else None,

@@ -193,6 +193,15 @@ int mcm_gpu_batch_initial(mcm_gpu_batch *b);
 int mcm_gpu_batch_step(mcm_gpu_batch *b, int32_t nsteps);
 /* Per-stage times of nsteps batched steps, as mcm_gpu_time_steps. */
 int mcm_gpu_batch_time_steps(mcm_gpu_batch *b, int32_t nsteps, double *ms);
+/* Autotuning of the batch's pair mapping: 2 + nsteps batched steps with
+   one thread per particle, then 2 + nsteps with one wavefront per particle
+   (runs that cannot use it keep the thread kernels), the last nsteps of
+   each timed; the faster mapping is kept and returned in *mapping.  Both
+   mappings give the same results, so the 2 (2 + nsteps) steps are ordinary
+   steps of every run.  The cell size is not
+   tuned: it is 2 hmax as in mcm_setuplist.f90:35, and it fixes the
+   reference's list order. */
+int mcm_gpu_batch_tune(mcm_gpu_batch *b, int32_t nsteps, int32_t *mapping);
 void mcm_gpu_batch_destroy(mcm_gpu_batch *b);
 
 #ifdef __cplusplus

@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <chrono>
 #include <string>
 #include <vector>
 
@@ -1814,6 +1815,28 @@ int mcm_gpu_batch_time_steps(mcm_gpu_batch* b, int32_t nsteps, double* ms) {
             b->err = "run " + std::to_string(i) + ": capacity exceeded during timed steps";
             return MCM_GPU_ERR_CAPACITY;
         }
+    return MCM_GPU_OK;
+}
+
+int mcm_gpu_batch_tune(mcm_gpu_batch* b, int32_t nsteps, int32_t* mapping) {
+    if (!b || nsteps <= 0 || b->inst.empty()) return MCM_GPU_ERR_ARG;
+    double best = 1.0e300;
+    int best_m = 0;
+    for (int m = 0; m <= 1; m++) {
+        b->opt.pair_mapping = m;
+        for (mcm_gpu_ctx* r : b->inst) r->pair_mapping = m;
+        b->stale = true;
+        int rc = mcm_gpu_batch_step(b, 2);          // block maps, graph, warm caches
+        if (rc) return rc;
+        const auto t0 = std::chrono::steady_clock::now();
+        if ((rc = mcm_gpu_batch_step(b, nsteps))) return rc;   // returns after the stream drained
+        const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (t < best) { best = t; best_m = m; }
+    }
+    b->opt.pair_mapping = best_m;
+    for (mcm_gpu_ctx* r : b->inst) r->pair_mapping = best_m;
+    b->stale = true;
+    if (mapping) *mapping = best_m;
     return MCM_GPU_OK;
 }
 

@@ -34,7 +34,7 @@ SYMBOLS = (
     "mcm_gpu_batch_create", "mcm_gpu_batch_add_deck", "mcm_gpu_batch_add_memory", "mcm_gpu_batch_size",
     "mcm_gpu_batch_run", "mcm_gpu_batch_last_error", "mcm_gpu_batch_initial", "mcm_gpu_batch_step",
     "mcm_gpu_batch_time_steps", "mcm_gpu_batch_destroy", "mcm_gpu_output_fields", "mcm_vtk_write",
-    "mcm_gpu_write_vtk", "mcm_gpu_rep_cont",
+    "mcm_gpu_write_vtk", "mcm_gpu_rep_cont", "mcm_gpu_batch_tune",
 )
 
 OUT_WORDS = 17   # packed output record: x(3), v(3), p, sigma 11 22 33 12 13 23, e, efps, rho, h
@@ -107,6 +107,7 @@ def load_library(path: str = LIB_PATH):
     lib.mcm_vtk_write.argtypes = [ctypes.c_char_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_double),
                                   ctypes.POINTER(ctypes.c_int32), ctypes.c_int32]
     lib.mcm_gpu_write_vtk.argtypes = [vp, ctypes.c_char_p, ctypes.c_int32]
+    lib.mcm_gpu_batch_tune.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
     lib.mcm_gpu_batch_destroy.argtypes = [vp]
     lib.mcm_gpu_batch_destroy.restype = None
     special = ("mcm_gpu_destroy", "mcm_gpu_last_error", "mcm_gpu_batch_size", "mcm_gpu_batch_run",
@@ -339,6 +340,13 @@ class Batch:
         ms = (ctypes.c_double * 5)()
         self._check(self._lib.mcm_gpu_batch_time_steps(self._h, int(n), ms), "batch_time_steps")
         return np.array(ms[:])
+
+    def tune(self, n: int = 2) -> int:
+        """Time n steps with each pair mapping and keep the faster (0 thread,
+        1 wavefront per particle); advances every run by 2 (n + 2) steps."""
+        m = ctypes.c_int32()
+        self._check(self._lib.mcm_gpu_batch_tune(self._h, int(n), ctypes.byref(m)), "batch_tune")
+        return m.value
 
     def close(self):
         if getattr(self, "_h", None):
