@@ -204,6 +204,43 @@ int mcm_gpu_batch_time_steps(mcm_gpu_batch *b, int32_t nsteps, double *ms);
 int mcm_gpu_batch_tune(mcm_gpu_batch *b, int32_t nsteps, int32_t *mapping);
 void mcm_gpu_batch_destroy(mcm_gpu_batch *b);
 
+/* ---- Differentiation of the step (reverse mode).  No counterpart in the
+   reference, which has no derivatives: these entry points serve material
+   identification, gradients of a displacement-history loss with respect
+   to material constants and initial velocities.
+
+   Scope: plain SPH (disctype 0), material model 10 (any EOS), h_opt 0,
+   tcrit_opt 0 or 1, no penalty contact, XSPH, dynamic relaxation or base
+   acceleration; other decks are rejected with MCM_GPU_ERR_ARG.
+   Parameters, per material: strinput(1:3) (f3dm10 G, sigma_y, Eh) and
+   eosinput(2) (S1 of eos4). */
+typedef struct mcm_gpu_adjoint mcm_gpu_adjoint;
+/* Set / read the 4 parameters of material mat (0-based); a new value takes
+   effect at the next mcm_gpu_initial (mcm_calcg: G of model 10 follows). */
+int mcm_gpu_set_material_params(mcm_gpu_ctx *ctx, int32_t mat, const double *params);
+int mcm_gpu_get_material_params(mcm_gpu_ctx *ctx, int32_t mat, double *params);
+/* Replace the deck's initial velocities (n = 3 np, particle-major, deck
+   order); takes effect at the next mcm_gpu_initial. */
+int mcm_gpu_set_initial_velocity(mcm_gpu_ctx *ctx, const double *v, int64_t n);
+/* nsteps fused steps from the run's current state (normally right after
+   mcm_gpu_initial), keeping a state checkpoint every ckpt_every steps on the
+   device and writing the positions of the nsample listed particles (distinct
+   0-based ids) after every step to hist (nsteps x nsample x 3, host).  The
+   run's state afterwards is the end of the recorded run. */
+int mcm_gpu_adj_forward(mcm_gpu_ctx *ctx, int32_t nsteps, int32_t ckpt_every, const int32_t *sample,
+                        int32_t nsample, double *hist, mcm_gpu_adjoint **out);
+/* Reverse sweep of a recorded run.  In: dL/dx and dL/dv of the final state
+   (3 np each, particle-major) and dL/dhist (nsteps x nsample x 3); any may
+   be NULL for zero.  Out: dL/dv0 (3 np) of the deck's initial velocities
+   and dL/dparams (4 per material).  Each segment of ckpt_every steps is
+   recomputed from its checkpoint and each step once more through the
+   granular stages; the run's state afterwards is unspecified (call
+   mcm_gpu_initial before stepping it again). */
+int mcm_gpu_adj_backward(mcm_gpu_adjoint *a, const double *xbar, const double *vbar, const double *histbar,
+                         double *v0bar, double *pbar);
+const char *mcm_gpu_adj_last_error(const mcm_gpu_adjoint *a);
+void mcm_gpu_adj_destroy(mcm_gpu_adjoint *a);
+
 #ifdef __cplusplus
 }
 #endif

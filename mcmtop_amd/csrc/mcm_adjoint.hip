@@ -1,0 +1,1107 @@
+// mcm_adjoint.hip -- reverse-mode derivative of the plain-SPH MCM step on gfx950.
+//
+// A forward run records a state snapshot every k steps.  The reverse sweep
+// walks the segments backwards.  Each segment is recomputed from its
+// checkpoint with the fused step, keeping the start-of-step state of every
+// step.  Then, for each step from last to first:
+//   1. the step is recomputed once more through the granular stage kernels,
+//      which leave every intermediate the reverse needs on the device (lists,
+//      ghosts, gather records, rod/spin, per-particle critts, acceleration);
+//   2. adjoint kernels map the adjoint of the end-of-step state to the adjoint
+//      of the start-of-step state, in the reverse order of the stages:
+//      move and velocity update, dt update, momentum pair gather, constitutive
+//      update, strain pair gather.
+//
+// Pair stages: the real-particle neighbour relation is symmetric (the list
+// criterion |x_i - x_j|^2 < (h_i + h_j)^2 at x^n is), so the contribution of
+// pair (i, j) of i's sum to j's adjoint is gathered by j over its own list;
+// every thread writes only its own particle and nothing is scattered.
+// Ghosts carry no list of their own: each step their real partners are
+// collected into per-ghost lists, sorted, and the ghost adjoints are mapped
+// to the real parents with the mirror signs of the ghost fill.
+//
+// Per-particle stages: the constitutive chain (rhoupd, stress_rot1, f3dm10,
+// sueos, nodalav, hieupd, eqos, stress_up, calccritts) is the forward's own
+// code (mcm_physics.h) instantiated with dual numbers.  Its local Jacobian is
+// taken column block by column block and applied transposed, so the yield
+// active set, the EOS branch and the pressure cut-off are those the forward
+// took.
+//
+// Scope: plain SPH (disctype 0), material model 10 with any EOS, fixed h
+// (h_opt 0), tcrit_opt 0 or 1, no penalty contact, XSPH, dynamic relaxation
+// or base acceleration.  With tcrit_opt 1 the critical step h / (c + |v|)
+// reads |v| of the previous velocity update, which equals |v^n| (the
+// reflections only flip signs), so it enters as a function of v^n.  Parameters: per material strinput(1:3) (f3dm10 G,
+// sigma_y, Eh) and eosinput(2) (S1 of eos4), and the deck's initial
+// velocities.
+#pragma once
+
+constexpr int ADJ_NPAR = 4;                   // strinput[0], strinput[1], strinput[2], eosinput[1]
+constexpr int ADJ_NIN = 20 + ADJ_NPAR;        // rod 6, spin 3, sigma 6, rho, e, efps, |v|, dt, parameters
+constexpr int ADJ_K = 4;                      // directions per dual pass
+enum { IN_ROD = 0, IN_SPIN = 6, IN_SIG = 9, IN_RHO = 15, IN_E = 16, IN_EFPS = 17, IN_VABS = 18, IN_DT = 19,
+       IN_PAR = 20 };
+
+// snapshot fields (np doubles each): everything a step reads of the real
+// particles' state
+enum {
+    SN_X = 0, SN_V = 3, SN_R = 6, SN_H = 8, SN_HOLD = 10, SN_SIG = 11, SN_E = 17, SN_EFPS = 18, SN_P = 19,
+    SN_C = 20, SN_Q = 21, SN_VABS = 22, SN_PCUT = 23, SN_PCUT2 = 24, SN_FAIL = 25, SN_DEFPS = 26,
+    SN_ALFA = 27, SN_SD = 33, SN_N = 39
+};
+
+// adjoint scalars
+enum { AS_DTB = 0, AS_DTP = 1, AS_CRIT = 2, AS_DTACC = 3, AS_ARGMIN = 4, AS_PAR = 8 };
+constexpr int AS_N = AS_PAR + ADJ_NPAR * MCM_MAX_MAT;
+
+// per-block partials: move (x.v, vbar.a, argmin), constitutive (dt, params), strain (dt)
+enum { RA_XV = 0, RA_VA = 1, RA_ARG = 2, RA_CDT = 3, RA_PAR = 4, RA_SDT = RA_PAR + ADJ_NPAR * MCM_MAX_MAT };
+constexpr int RA_N = RA_SDT + 1;
+
+constexpr int ADJ_GOF = 8;                    // ghosts of one real particle (at most 7 for 3 planes)
+
+struct AdjDev {
+    double *xb[3], *vb[3], *rb, *sb[6], *eb, *fb;   // adjoint of the state (real particles)
+    double *ab[3], *qb, *rbn;                       // adjoints local to a step
+    double* lb[9];                                  // adjoint of the velocity gradient g[n][m], at 3n + m
+    double *gxb[3], *gvb[3], *grb, *gAb[6];         // ghost adjoints [ngcap]
+    int *gt_cnt, *gt;                               // real partners of each ghost [ngcap], [ngcap][gtcap]
+    int *gof_cnt, *gof;                             // ghosts of each real particle [np], [np][ADJ_GOF]
+    int gtcap, ngcap;
+    const double* snap;                             // start-of-step snapshot of the step being reversed
+    int cur;                                        // its h / rho buffer parity
+    double* red;                                    // [nb][RA_N]
+    double* as;                                     // [AS_N]
+    int* err;                                       // bit 1: a transposed list overflowed
+};
+
+__device__ __forceinline__ double* snap_field(const DevState& d, int f) {
+    if (f < SN_V) return d.x0[f];
+    if (f < SN_R) return d.v[f - SN_V];
+    if (f < SN_H) return d.rbuf[f - SN_R];
+    if (f < SN_HOLD) return d.hbuf[f - SN_H];
+    if (f == SN_HOLD) return d.hold;
+    if (f < SN_E) return d.sig[f - SN_SIG];
+    switch (f) {
+        case SN_E: return d.e;
+        case SN_EFPS: return d.efps;
+        case SN_P: return d.p;
+        case SN_C: return d.c;
+        case SN_Q: return d.q;
+        case SN_VABS: return d.vabs;
+        case SN_PCUT: return d.pcut;
+        case SN_PCUT2: return d.p_cut;
+        case SN_FAIL: return d.fail;
+        case SN_DEFPS: return d.defps_prev;
+        default: break;
+    }
+    if (f < SN_SD) return d.alfa[f - SN_ALFA];
+    return d.sd[f - SN_SD];
+}
+
+__device__ __forceinline__ double snap_at(const AdjDev& a, int np, int f, int i) { return a.snap[(long)f * np + i]; }
+
+// save (1) or restore (0) the real-particle state; grid.y = field
+__global__ __launch_bounds__(MCM_BLOCK) void k_snap(DevState d, double* base, int save) {
+    const int f = blockIdx.y;
+    double* p = snap_field(d, f);
+    double* b = base + (long)f * d.np;
+    for (int i = blockIdx.x * MCM_BLOCK + threadIdx.x; i < d.np; i += gridDim.x * MCM_BLOCK) {
+        if (save) b[i] = p[i];
+        else p[i] = b[i];
+    }
+}
+
+// positions of the sampled particles after a step
+__global__ void k_adj_sample(DevState d, const int* sample, int ns, double* out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ns) return;
+    const int i = sample[k];
+    for (int c = 0; c < 3; c++) out[3 * k + c] = d.x0[c][i];
+}
+
+// xbar of the sampled particles += dL/dx of the recorded positions
+__global__ void k_adj_seed(AdjDev a, const int* sample, int ns, const double* seed) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ns) return;
+    const int i = sample[k];
+    for (int c = 0; c < 3; c++) a.xb[c][i] += seed[3 * k + c];
+}
+
+// ------------------------------------------------------------ kernel Hessian
+// out = H w with H = d(grad W)/d(xij) of the cubic B-spline (mcm_gradw.f90
+// differentiated): grad W = F(r) xij, F = dw / (r havg), so
+// H = F I + (dw' - dw / r) / (r^2 havg) xij xij^T, dw' = d(dw)/dr; zero at r = 0.
+__device__ __forceinline__ void kernel_hess_apply(const double xij[3], double rad, double havg, double pi,
+                                                  const double w[3], double out[3]) {
+    if (rad == 0.0) { out[0] = out[1] = out[2] = 0.0; return; }
+    const double const1 = 1.0 / (pi * havg * havg * havg);
+    const double const2 = const1 * 0.25;
+    const double z = rad / havg;
+    double dw, dwz;
+    if (z < 1.0) {
+        dw = const1 * ((2.25 * z - 3.0) * z);
+        dwz = const1 * (4.5 * z - 3.0);
+    } else if (z < 2.0) {
+        const double t = 2.0 - z;
+        dw = -(3.0 * const2 * (t * t));
+        dwz = 6.0 * const2 * t;
+    } else {
+        out[0] = out[1] = out[2] = 0.0;
+        return;
+    }
+    const double F = dw / (rad * havg);
+    const double C = (dwz / havg - dw / rad) / (rad * rad * havg);
+    const double xw = xij[0] * w[0] + xij[1] * w[1] + xij[2] * w[2];
+    for (int k = 0; k < 3; k++) out[k] = F * w[k] + C * xw * xij[k];
+}
+
+// ------------------------------------------------------- ghost transposes
+// real partners of each ghost, from the real particles' lists
+__global__ __launch_bounds__(MCM_BLOCK) void k_adj_gt_fill(DevState d, AdjDev a) {
+    const int i = blockIdx.x * MCM_BLOCK + threadIdx.x;
+    if (i >= d.np) return;
+    const int* lst = nbr_list(d, i);
+    for (int k = d.nreal[i]; k < d.ntot[i]; k++) {
+        const int g = lst[(long)k * 64] - d.np;
+        const int slot = atomicAdd(&a.gt_cnt[g], 1);
+        if (slot < a.gtcap) a.gt[(long)g * a.gtcap + slot] = i;
+        else atomicOr(a.err, 1);
+    }
+}
+
+// ghosts of each real particle
+__global__ __launch_bounds__(MCM_BLOCK) void k_adj_gof_fill(DevState d, AdjDev a) {
+    const int ngp = d.sc->ngp;
+    for (int g = blockIdx.x * MCM_BLOCK + threadIdx.x; g < ngp; g += gridDim.x * MCM_BLOCK) {
+        const int p = d.gpar[g];
+        const int slot = atomicAdd(&a.gof_cnt[p], 1);
+        if (slot < ADJ_GOF) a.gof[(long)p * ADJ_GOF + slot] = g;
+        else atomicOr(a.err, 1);
+    }
+}
+
+__device__ __forceinline__ void isort(int* v, int n, long stride) {
+    for (int k = 1; k < n; k++) {
+        const int x = v[k * stride];
+        int j = k - 1;
+        while (j >= 0 && v[j * stride] > x) {
+            v[(j + 1) * stride] = v[j * stride];
+            j--;
+        }
+        v[(j + 1) * stride] = x;
+    }
+}
+
+// ascending order, so that every sum over a transposed list has a fixed order
+__global__ __launch_bounds__(MCM_BLOCK) void k_adj_gt_sort(DevState d, AdjDev a) {
+    const int ngp = d.sc->ngp;
+    for (int g = blockIdx.x * MCM_BLOCK + threadIdx.x; g < ngp; g += gridDim.x * MCM_BLOCK)
+        isort(a.gt + (long)g * a.gtcap, min(a.gt_cnt[g], a.gtcap), 1);
+}
+
+__global__ __launch_bounds__(MCM_BLOCK) void k_adj_gof_sort(DevState d, AdjDev a) {
+    const int i = blockIdx.x * MCM_BLOCK + threadIdx.x;
+    if (i >= d.np) return;
+    isort(a.gof + (long)i * ADJ_GOF, min(a.gof_cnt[i], ADJ_GOF), 1);
+}
+
+// ------------------------------------------------- move and velocity update
+// Reverse of mcm_update_velocity + mcm_move + mcm_check_sym_pen for real
+// particle i: v' = v + a dtn, x' = x1 + v' dt', then the reflections.
+// xb, vb hold the adjoint of x', v' on entry and of x^n, v^n (partial) on
+// exit; ab gets the adjoint of the acceleration after mcm_boundcond.
+__global__ __launch_bounds__(MCM_BLOCK) void k_adj_move(DevState d, AdjDev a) {
+    const int i = blockIdx.x * MCM_BLOCK + threadIdx.x;
+    const Scalars* sc = d.sc;
+    double pxv = 0.0, pva = 0.0, parg = 1.0e300;
+    if (i < d.np) {
+        const double dt1 = sc->dt, dt0 = sc->dtold;
+        const double dtn = 0.5 * (dt1 + dt0);
+        const SRec me = load_srec(d.rs, i);
+        const double vn[3] = {me.vx, me.vy, me.vz};
+        double acc[3], vp[3], x[3], v[3], sx[3] = {1.0, 1.0, 1.0}, sv[3] = {1.0, 1.0, 1.0};
+        for (int k = 0; k < 3; k++) {
+            acc[k] = d.a[k][i];
+            vp[k] = vn[k] + acc[k] * dtn;
+            x[k] = d.x1[k][i] + vp[k] * dt1;
+            v[k] = vp[k];
+        }
+        if (d.boundary) {   // the reflections of sym_pen, in its order
+            for (int k = 0; k < 3; k++) {
+                const double b0 = d.bx[k][0], b1 = d.bx[k][1];
+                if (d.bcode[k][0] == 1 && x[k] < b0) { x[k] = 2.0 * b0 - x[k]; v[k] = -v[k]; sx[k] = -sx[k]; sv[k] = -sv[k]; }
+                if (d.bcode[k][1] == 1 && x[k] > b1) { x[k] = 2.0 * b1 - x[k]; v[k] = -v[k]; sx[k] = -sx[k]; sv[k] = -sv[k]; }
+                if (d.bcode[k][0] == 2) {
+                    const double span = b1 - b0;
+                    if (x[k] < b0) x[k] = x[k] + span;
+                    if (x[k] > b1) x[k] = x[k] - span;
+                }
+            }
+        }
+        double abar[3];
+        for (int k = 0; k < 3; k++) {
+            const double xpb = sx[k] * a.xb[k][i];
+            double vpb = sv[k] * a.vb[k][i] + dt1 * xpb;
+            pxv += xpb * vp[k];
+            pva += vpb * acc[k];
+            abar[k] = dtn * vpb;
+            a.xb[k][i] = xpb;
+            a.vb[k][i] = vpb;
+        }
+        boundcond(d.dispbc[i], abar);
+        for (int k = 0; k < 3; k++) a.ab[k][i] = abar[k];
+        if (d.critts_p[i] == sc->critts) parg = (double)i;
+    }
+    const double s1 = block_reduce<0>(pxv);
+    const double s2 = block_reduce<0>(pva);
+    const double s3 = block_reduce<1>(parg);
+    if (threadIdx.x == 0) {
+        double* r = a.red + (long)blockIdx.x * RA_N;
+        r[RA_XV] = s1; r[RA_VA] = s2; r[RA_ARG] = s3;
+    }
+}
+
+// dt' = min(tssf critts, 1.1 dt) (mcm_solution.f90:58-62) reversed, with the
+// adjoint of dt' from the move and the velocity update.  One workgroup.
+__global__ __launch_bounds__(MCM_BLOCK) void k_adj_fold_move(DevState d, AdjDev a, int nblk) {
+    const double xv = fold<0>(a.red, RA_N, RA_XV, nblk, 0.0);
+    const double va = fold<0>(a.red, RA_N, RA_VA, nblk, 0.0);
+    const double am = fold<1>(a.red, RA_N, RA_ARG, nblk, 1.0e300);
+    if (threadIdx.x != 0) return;
+    const Scalars* sc = d.sc;
+    const double dtp = a.as[AS_DTB] + xv + 0.5 * va;
+    double acc = 0.5 * va, crit = 0.0;
+    if (d.tssf * sc->critts <= 1.1 * sc->dtold) crit = d.tssf * dtp;
+    else acc = acc + 1.1 * dtp;
+    a.as[AS_DTP] = dtp;
+    a.as[AS_CRIT] = crit;
+    a.as[AS_DTACC] = acc;
+    a.as[AS_ARGMIN] = am;
+}
+
+// ------------------------------------------------------------- momentum
+// Reverse of mcm_oldacceleration.f90:60-89 for real particle t, with the
+// terms of t in its neighbours' sums gathered over t's own list:
+//   a_t = sum_u m_u (A_u + A_t) grad W(x_t - x_u),
+//   Abar_t += (m_u abar_t - m_t abar_u) (x) grad W, xbar_t += H (A_u + A_t) (m_u abar_t - m_t abar_u)
+// (abar_u = 0 for ghosts).  A = (sigma - q I) / rho^2 maps to sigma', q', rho'.
+__global__ __launch_bounds__(MCM_BLOCK) void k_adj_mom(DevState d, AdjDev a) {
+    const int t = blockIdx.x * MCM_BLOCK + threadIdx.x;
+    if (t >= d.np) return;
+    const int nxt = a.cur ^ 1;
+    const MRec me = load_mrec(d.rm, t);
+    const double at[3] = {a.ab[0][t], a.ab[1][t], a.ab[2][t]};
+    double Ab[6] = {0, 0, 0, 0, 0, 0}, xbt[3] = {0, 0, 0};
+    const int n = d.ntot[t];
+    const int* lst = nbr_list(d, t);
+    for (int k = 0; k < n; k++) {
+        const int u = lst[(long)k * 64];
+        const MRec r = load_mrec(d.rm, u);
+        const double havg = 0.5 * (me.h + r.h);
+        const double xij[3] = {me.x - r.x, me.y - r.y, me.z - r.z};
+        double g[3], rad;
+        kernel_grad(xij[0], xij[1], xij[2], havg, d.pi, g, &rad);
+        double w[3];
+        for (int c = 0; c < 3; c++) w[c] = r.m * at[c] - (u < d.np ? me.m * a.ab[c][u] : 0.0);
+        for (int nn = 0; nn < 3; nn++)
+            for (int mm = 0; mm < 3; mm++) Ab[sym_index(mm, nn)] += w[nn] * g[mm];
+        double bw[3];
+        for (int mm = 0; mm < 3; mm++) {
+            bw[mm] = 0.0;
+            for (int nn = 0; nn < 3; nn++) {
+                const int c = sym_index(mm, nn);
+                bw[mm] += (r.A[c] + me.A[c]) * w[nn];
+            }
+        }
+        double hw[3];
+        kernel_hess_apply(xij, rad, havg, d.pi, bw, hw);
+        for (int c = 0; c < 3; c++) xbt[c] += hw[c];
+    }
+    const double rho = d.rbuf[nxt][t], r2 = rho * rho;
+    double rbar = 0.0;
+    for (int c = 0; c < 6; c++) {
+        a.sb[c][t] += Ab[c] / r2;
+        rbar += Ab[c] * me.A[c];
+    }
+    a.qb[t] = -(Ab[0] + Ab[1] + Ab[2]) / r2;
+    a.rb[t] += -2.0 * rbar / rho;
+    for (int c = 0; c < 3; c++) a.xb[c][t] += xbt[c];
+}
+
+// the ghosts' share of the momentum reverse, gathered over their real partners
+__global__ __launch_bounds__(MCM_BLOCK) void k_adj_mom_ghost(DevState d, AdjDev a) {
+    const int ngp = d.sc->ngp;
+    for (int g = blockIdx.x * MCM_BLOCK + threadIdx.x; g < ngp; g += gridDim.x * MCM_BLOCK) {
+        const MRec me = load_mrec(d.rm, d.np + g);
+        double Ab[6] = {0, 0, 0, 0, 0, 0}, xbg[3] = {0, 0, 0};
+        const int cnt = min(a.gt_cnt[g], a.gtcap);
+        const int* lst = a.gt + (long)g * a.gtcap;
+        for (int k = 0; k < cnt; k++) {
+            const int i = lst[k];
+            const MRec r = load_mrec(d.rm, i);
+            const double havg = 0.5 * (r.h + me.h);
+            const double xij[3] = {r.x - me.x, r.y - me.y, r.z - me.z};
+            double gw[3], rad;
+            kernel_grad(xij[0], xij[1], xij[2], havg, d.pi, gw, &rad);
+            double w[3];
+            for (int c = 0; c < 3; c++) w[c] = me.m * a.ab[c][i];
+            for (int nn = 0; nn < 3; nn++)
+                for (int mm = 0; mm < 3; mm++) Ab[sym_index(mm, nn)] += w[nn] * gw[mm];
+            double bw[3];
+            for (int mm = 0; mm < 3; mm++) {
+                bw[mm] = 0.0;
+                for (int nn = 0; nn < 3; nn++) {
+                    const int c = sym_index(mm, nn);
+                    bw[mm] += (me.A[c] + r.A[c]) * w[nn];
+                }
+            }
+            double hw[3];
+            kernel_hess_apply(xij, rad, havg, d.pi, bw, hw);
+            for (int c = 0; c < 3; c++) xbg[c] -= hw[c];
+        }
+        for (int c = 0; c < 6; c++) a.gAb[c][g] = Ab[c];
+        for (int c = 0; c < 3; c++) a.gxb[c][g] = xbg[c];
+    }
+}
+
+// ghost momentum adjoints to the real parents: a ghost's A is its parent's
+// (sigma' - q') with the off-diagonals negated, over the ghost's own
+// start-of-step rho (mcm_update_ghost_stress.f90:28-41); its position is the
+// parent's x^n mirrored
+__global__ __launch_bounds__(MCM_BLOCK) void k_adj_gmap_mom(DevState d, AdjDev a) {
+    const int p = blockIdx.x * MCM_BLOCK + threadIdx.x;
+    if (p >= d.np) return;
+    const int cnt = min(a.gof_cnt[p], ADJ_GOF);
+    for (int k = 0; k < cnt; k++) {
+        const int g = a.gof[(long)p * ADJ_GOF + k], u = d.np + g;
+        const MRec me = load_mrec(d.rm, u);
+        const double rg = d.rbuf[a.cur][u], r2 = rg * rg;
+        double rbar = 0.0;
+        for (int c = 0; c < 6; c++) {
+            const double ab = a.gAb[c][g];
+            a.sb[c][p] += (c < 3 ? ab : -ab) / r2;
+            rbar += ab * me.A[c];
+        }
+        a.qb[p] += -(a.gAb[0][g] + a.gAb[1][g] + a.gAb[2][g]) / r2;
+        a.rbn[p] += -2.0 * rbar / rg;
+        const int sg = d.gsgn[g];
+        for (int c = 0; c < 3; c++) a.xb[c][p] += ((sg >> c) & 1 ? -a.gxb[c][g] : a.gxb[c][g]);
+    }
+}
+
+// ---------------------------------------------------------- constitutive
+// The material constants the model-10 chain reads (f3dm10: strinput(1:8);
+// the EOS routines: eosinput(1:8); rhoupd, nodalav, sueos: rho, h, rho_max,
+// av_l, av_q, g), as duals.
+template <int K>
+struct MatDual {
+    int model, eos, visc_type, ispall, efps_flag;
+    Dual<K> rho, g, av_l, av_q, h, rho_max;
+    Dual<K> strinput[8], eosinput[8];
+};
+
+template <int K>
+__device__ __forceinline__ MatDual<K> dual_mat(const MatParams& m) {
+    MatDual<K> r;
+    r.model = m.model; r.eos = m.eos; r.visc_type = m.visc_type; r.ispall = m.ispall; r.efps_flag = m.efps_flag;
+    r.rho = m.rho; r.g = m.g; r.av_l = m.av_l; r.av_q = m.av_q; r.h = m.h; r.rho_max = m.rho_max;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        r.strinput[k] = m.strinput[k];
+        r.eosinput[k] = m.eosinput[k];
+    }
+    return r;
+}
+
+template <int K>
+__device__ __forceinline__ void seed(Dual<K>& x, int j, int p0) {
+    if (j >= p0 && j < p0 + K) x.d[j - p0] = 1.0;
+}
+
+// Reverse of the constitutive update of real particle i (mcm_constitutive.f90
+// with mcm_rhoupd, mcm_stress_rot1, mcm_f3dm10, mcm_sueos, mcm_nodalav,
+// mcm_hieupd, mcm_eqos, mcm_stress_up and mcm_calccritts): the adjoint of
+// (sigma', rho', e', efps', q', critts) becomes the adjoint of (rod, spin,
+// sigma, rho, e, efps, dt, parameters) through the transposed local Jacobian.
+__global__ __launch_bounds__(MCM_BLOCK) void k_adj_const(DevState d, AdjDev a) {
+    typedef Dual<ADJ_K> D;
+    const int i = blockIdx.x * MCM_BLOCK + threadIdx.x;
+    const int np = d.np;
+    const int mi = i < np ? d.pmat[i] : 0;
+    double xbar[ADJ_NIN];
+    for (int j = 0; j < ADJ_NIN; j++) xbar[j] = 0.0;
+    if (i < np) {
+        const MatParams& m0 = d.mat[mi];
+        const double dtn = d.sc->dtold;   // the step's dt (before the dt update)
+        double ys[6];
+        for (int c = 0; c < 6; c++) ys[c] = a.sb[c][i];
+        const double yr = a.rb[i], ye = a.eb[i], yf = a.fb[i], yq = a.qb[i];
+        const double yc = (double)i == a.as[AS_ARGMIN] ? a.as[AS_CRIT] : 0.0;
+#pragma unroll
+        for (int p0 = 0; p0 < ADJ_NIN; p0 += ADJ_K) {
+            PStateT<D> s;
+            s.mass = d.mass[i];
+            s.rho0 = d.rho0[i];
+            s.h = snap_at(a, np, SN_H + a.cur, i);
+            s.hold = snap_at(a, np, SN_HOLD, i);
+            s.h0 = d.h0[i];
+            s.p = snap_at(a, np, SN_P, i);
+            s.c = snap_at(a, np, SN_C, i);
+            s.vabs = snap_at(a, np, SN_VABS, i);
+            seed(s.vabs, IN_VABS, p0);
+            s.pcut = snap_at(a, np, SN_PCUT, i);
+            s.p_cut = snap_at(a, np, SN_PCUT2, i);
+            s.fail = snap_at(a, np, SN_FAIL, i);
+            s.q = snap_at(a, np, SN_Q, i);
+            s.einc = 0.0;
+            s.etry = 0.0;
+            for (int c = 0; c < 6; c++) { s.rod[c] = d.rod[c][i]; seed(s.rod[c], IN_ROD + c, p0); }
+            for (int c = 0; c < 3; c++) { s.spin[c] = d.spin[c][i]; seed(s.spin[c], IN_SPIN + c, p0); }
+            for (int c = 0; c < 6; c++) {
+                s.sig[c] = snap_at(a, np, SN_SIG + c, i);
+                seed(s.sig[c], IN_SIG + c, p0);
+            }
+            s.rho = snap_at(a, np, SN_R + a.cur, i);
+            seed(s.rho, IN_RHO, p0);
+            s.e = snap_at(a, np, SN_E, i);
+            seed(s.e, IN_E, p0);
+            s.efps = snap_at(a, np, SN_EFPS, i);
+            seed(s.efps, IN_EFPS, p0);
+            D dt = dtn;
+            seed(dt, IN_DT, p0);
+            MatDual<ADJ_K> m = dual_mat<ADJ_K>(m0);
+            seed(m.strinput[0], IN_PAR + 0, p0);
+            seed(m.strinput[1], IN_PAR + 1, p0);
+            seed(m.strinput[2], IN_PAR + 2, p0);
+            seed(m.eosinput[1], IN_PAR + 3, p0);
+            m.g = m.strinput[0];   // mcm_calcg.f90: G of model 10
+            s.tracerod = s.rod[S11] + s.rod[S22] + s.rod[S33];
+            // the forward's chain (constitutive_particle, model 10)
+            rhoupd(s, m, dt);
+            stress_rot1(s, dt);
+            double dprev = snap_at(a, np, SN_DEFPS, i);
+            f3dm10(s, m, dt, dprev);
+            sueos(s, m);
+            nodalav(s, m);
+            hieupd(s, dt);
+            eqos(s, m, dt);
+            stress_up(s);
+            const D cr = critts(s, m, d.tcrit_opt, 0.0);
+#pragma unroll
+            for (int k = 0; k < ADJ_K; k++) {
+                const int j = p0 + k;
+                if (j >= ADJ_NIN) break;
+                double v = yr * s.rho.d[k] + ye * s.e.d[k] + yf * s.efps.d[k] + yq * s.q.d[k] + yc * cr.d[k];
+                for (int c = 0; c < 6; c++) v += ys[c] * s.sig[c].d[k];
+                xbar[j] += v;
+            }
+        }
+        // rod / spin -> velocity gradient g (rod_spin: rod_ab = (g_ab + g_ba) / 2,
+        // spin = (g_ab - g_ba) / 2 for (ab) = (01), (02), (12))
+        const double* rb_ = xbar + IN_ROD;
+        const double* wb = xbar + IN_SPIN;
+        a.lb[0][i] = rb_[S11];
+        a.lb[4][i] = rb_[S22];
+        a.lb[8][i] = rb_[S33];
+        a.lb[1][i] = 0.5 * rb_[S12] + 0.5 * wb[0];
+        a.lb[3][i] = 0.5 * rb_[S12] - 0.5 * wb[0];
+        a.lb[2][i] = 0.5 * rb_[S13] + 0.5 * wb[1];
+        a.lb[6][i] = 0.5 * rb_[S13] - 0.5 * wb[1];
+        a.lb[5][i] = 0.5 * rb_[S23] + 0.5 * wb[2];
+        a.lb[7][i] = 0.5 * rb_[S23] - 0.5 * wb[2];
+        for (int c = 0; c < 6; c++) a.sb[c][i] = xbar[IN_SIG + c];
+        a.rb[i] = xbar[IN_RHO];
+        a.eb[i] = xbar[IN_E];
+        a.fb[i] = xbar[IN_EFPS];
+        // |v^n| = sqrt(v . v) (mcm_update_velocity.f90:45-50)
+        const double va = snap_at(a, np, SN_VABS, i);
+        if (xbar[IN_VABS] != 0.0 && va > 0.0) {
+            const SRec me = load_srec(d.rs, i);
+            const double f = xbar[IN_VABS] / va;
+            a.vb[0][i] += f * me.vx;
+            a.vb[1][i] += f * me.vy;
+            a.vb[2][i] += f * me.vz;
+        }
+    }
+    const double sdt = block_reduce<0>(xbar[IN_DT]);
+    double* r = a.red + (long)blockIdx.x * RA_N;
+    if (threadIdx.x == 0) r[RA_CDT] = sdt;
+    for (int m = 0; m < d.nummat; m++)
+        for (int k = 0; k < ADJ_NPAR; k++) {
+            const double s = block_reduce<0>(i < np && mi == m ? xbar[IN_PAR + k] : 0.0);
+            if (threadIdx.x == 0) r[RA_PAR + ADJ_NPAR * m + k] = s;
+        }
+}
+
+// --------------------------------------------------------------- strain
+// Reverse of mcm_oldstrain.f90:34-64 with the half-step positions of
+// mcm_strain.f90:27-31 for real particle t, with t's terms in its real
+// neighbours' sums gathered over t's own list:
+//   g_t[n][m] = sum_u V_u (v_u - v_t)_n grad_m W(xh_t - xh_u), V = m / rho.
+__global__ __launch_bounds__(MCM_BLOCK) void k_adj_strain(DevState d, AdjDev a) {
+    const int t = blockIdx.x * MCM_BLOCK + threadIdx.x;
+    double pdt = 0.0;
+    if (t < d.np) {
+        const double dt = d.sc->dtold;
+        const SRec me = load_srec(d.rs, t);
+        const double vt[3] = {me.vx, me.vy, me.vz};
+        const double xh[3] = {me.x - 0.5 * me.vx * dt, me.y - 0.5 * me.vy * dt, me.z - 0.5 * me.vz * dt};
+        const double rho_t = d.rbuf[a.cur][t];
+        double Gt[9];
+        for (int c = 0; c < 9; c++) Gt[c] = a.lb[c][t];
+        double vbt[3] = {0, 0, 0}, xhb[3] = {0, 0, 0}, rbt = 0.0;
+        const int n = d.ntot[t];
+        const int* lst = nbr_list(d, t);
+        for (int k = 0; k < n; k++) {
+            const int u = lst[(long)k * 64];
+            const SRec r = load_srec(d.rs, u);
+            const double sh = u < d.np ? dt : 0.0;
+            const double xj[3] = {r.x - 0.5 * r.vx * sh, r.y - 0.5 * r.vy * sh, r.z - 0.5 * r.vz * sh};
+            const double havg = 0.5 * (me.h + r.h);
+            const double xij[3] = {xh[0] - xj[0], xh[1] - xj[1], xh[2] - xj[2]};
+            double g[3], rad;
+            kernel_grad(xij[0], xij[1], xij[2], havg, d.pi, g, &rad);
+            const double dv[3] = {r.vx - vt[0], r.vy - vt[1], r.vz - vt[2]};
+            // own terms
+            double w[3];
+            for (int mm = 0; mm < 3; mm++) {
+                w[mm] = Gt[mm] * dv[0] + Gt[3 + mm] * dv[1] + Gt[6 + mm] * dv[2];
+            }
+            for (int nn = 0; nn < 3; nn++)
+                vbt[nn] -= r.vol * (Gt[3 * nn] * g[0] + Gt[3 * nn + 1] * g[1] + Gt[3 * nn + 2] * g[2]);
+            double hw[3];
+            kernel_hess_apply(xij, rad, havg, d.pi, w, hw);
+            for (int c = 0; c < 3; c++) xhb[c] += r.vol * hw[c];
+            // t's terms in the sum of real neighbour u
+            if (u < d.np) {
+                double Gu[9];
+                for (int c = 0; c < 9; c++) Gu[c] = a.lb[c][u];
+                double w2[3];
+                for (int mm = 0; mm < 3; mm++) w2[mm] = -(Gu[mm] * dv[0] + Gu[3 + mm] * dv[1] + Gu[6 + mm] * dv[2]);
+                for (int nn = 0; nn < 3; nn++)
+                    vbt[nn] -= me.vol * (Gu[3 * nn] * g[0] + Gu[3 * nn + 1] * g[1] + Gu[3 * nn + 2] * g[2]);
+                rbt += me.vol / rho_t * (w2[0] * g[0] + w2[1] * g[1] + w2[2] * g[2]);
+                kernel_hess_apply(xij, rad, havg, d.pi, w2, hw);
+                for (int c = 0; c < 3; c++) xhb[c] -= me.vol * hw[c];
+            }
+        }
+        // xh = x - v dt / 2
+        for (int c = 0; c < 3; c++) {
+            a.xb[c][t] += xhb[c];
+            a.vb[c][t] += vbt[c] - 0.5 * dt * xhb[c];
+            pdt += -0.5 * vt[c] * xhb[c];
+        }
+        a.rbn[t] += rbt;
+    }
+    const double s = block_reduce<0>(pdt);
+    if (threadIdx.x == 0) a.red[(long)blockIdx.x * RA_N + RA_SDT] = s;
+}
+
+// the ghosts' share of the strain reverse (ghost positions are not shifted)
+__global__ __launch_bounds__(MCM_BLOCK) void k_adj_strain_ghost(DevState d, AdjDev a) {
+    const int ngp = d.sc->ngp;
+    const double dt = d.sc->dtold;
+    for (int g = blockIdx.x * MCM_BLOCK + threadIdx.x; g < ngp; g += gridDim.x * MCM_BLOCK) {
+        const int u = d.np + g;
+        const SRec me = load_srec(d.rs, u);
+        const double rho_g = d.rbuf[a.cur][u];
+        double vbg[3] = {0, 0, 0}, xbg[3] = {0, 0, 0}, rbg = 0.0;
+        const int cnt = min(a.gt_cnt[g], a.gtcap);
+        const int* lst = a.gt + (long)g * a.gtcap;
+        for (int k = 0; k < cnt; k++) {
+            const int i = lst[k];
+            const SRec r = load_srec(d.rs, i);
+            const double xh[3] = {r.x - 0.5 * r.vx * dt, r.y - 0.5 * r.vy * dt, r.z - 0.5 * r.vz * dt};
+            const double havg = 0.5 * (r.h + me.h);
+            const double xij[3] = {xh[0] - me.x, xh[1] - me.y, xh[2] - me.z};
+            double gw[3], rad;
+            kernel_grad(xij[0], xij[1], xij[2], havg, d.pi, gw, &rad);
+            double Gi[9];
+            for (int c = 0; c < 9; c++) Gi[c] = a.lb[c][i];
+            const double dv[3] = {me.vx - r.vx, me.vy - r.vy, me.vz - r.vz};
+            double w[3];
+            for (int mm = 0; mm < 3; mm++) w[mm] = Gi[mm] * dv[0] + Gi[3 + mm] * dv[1] + Gi[6 + mm] * dv[2];
+            for (int nn = 0; nn < 3; nn++)
+                vbg[nn] += me.vol * (Gi[3 * nn] * gw[0] + Gi[3 * nn + 1] * gw[1] + Gi[3 * nn + 2] * gw[2]);
+            rbg += -(me.vol / rho_g) * (w[0] * gw[0] + w[1] * gw[1] + w[2] * gw[2]);
+            double hw[3];
+            kernel_hess_apply(xij, rad, havg, d.pi, w, hw);
+            for (int c = 0; c < 3; c++) xbg[c] -= me.vol * hw[c];
+        }
+        for (int c = 0; c < 3; c++) { a.gvb[c][g] = vbg[c]; a.gxb[c][g] = xbg[c]; }
+        a.grb[g] = rbg;
+    }
+}
+
+// ghost strain adjoints to the real parents: x, v mirrored, rho copied
+// (mcm_ghost_setup.f90:330-341)
+__global__ __launch_bounds__(MCM_BLOCK) void k_adj_gmap_strain(DevState d, AdjDev a) {
+    const int p = blockIdx.x * MCM_BLOCK + threadIdx.x;
+    if (p >= d.np) return;
+    const int cnt = min(a.gof_cnt[p], ADJ_GOF);
+    for (int k = 0; k < cnt; k++) {
+        const int g = a.gof[(long)p * ADJ_GOF + k];
+        const int sg = d.gsgn[g];
+        for (int c = 0; c < 3; c++) {
+            const double s = (sg >> c) & 1 ? -1.0 : 1.0;
+            a.xb[c][p] += s * a.gxb[c][g];
+            a.vb[c][p] += s * a.gvb[c][g];
+        }
+        a.rbn[p] += a.grb[g];
+    }
+    a.rb[p] += a.rbn[p];
+    a.rbn[p] = 0.0;
+}
+
+// adjoint of the step's dt and of the parameters.  One workgroup.
+__global__ __launch_bounds__(MCM_BLOCK) void k_adj_fold_end(DevState d, AdjDev a, int nblk) {
+    const double cdt = fold<0>(a.red, RA_N, RA_CDT, nblk, 0.0);
+    const double sdt = fold<0>(a.red, RA_N, RA_SDT, nblk, 0.0);
+    double par[ADJ_NPAR * MCM_MAX_MAT];
+    for (int m = 0; m < d.nummat; m++)
+        for (int k = 0; k < ADJ_NPAR; k++)
+            par[ADJ_NPAR * m + k] = fold<0>(a.red, RA_N, RA_PAR + ADJ_NPAR * m + k, nblk, 0.0);
+    if (threadIdx.x != 0) return;
+    a.as[AS_DTB] = a.as[AS_DTACC] + cdt + sdt;
+    for (int m = 0; m < d.nummat; m++)
+        for (int k = 0; k < ADJ_NPAR; k++) a.as[AS_PAR + ADJ_NPAR * m + k] += par[ADJ_NPAR * m + k];
+}
+
+
+// ================================================================== host
+struct mcm_gpu_adjoint {
+    mcm_gpu_ctx* ctx = nullptr;
+    int nsteps = 0, k = 1, nseg = 0, nsample = 0;
+    long np = 0;
+    double* ckpt = nullptr;           // [nseg][SN_N][np]: state at the start of each segment
+    Scalars* ckpt_sc = nullptr;       // [nseg]
+    std::vector<int> ckpt_cur;
+    double* seg = nullptr;            // [k][SN_N][np]: the steps of the segment being reversed (k > 1)
+    Scalars* seg_sc = nullptr;
+    std::vector<int> seg_cur;
+    int* d_sample = nullptr;
+    double* d_hist = nullptr;         // [nsteps][nsample][3]: positions, then their adjoints
+    AdjDev a{};
+    std::vector<void*> allocs, gallocs;   // run-long and ghost-sized allocations
+    std::string err;
+};
+
+namespace {
+
+#define HIPCHK_A(call)                                                                            \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess) {                                                                   \
+            A->err = std::string(#call) + ": " + hipGetErrorString(e_);                           \
+            return MCM_GPU_ERR_HIP;                                                               \
+        }                                                                                         \
+    } while (0)
+
+template <typename T>
+int adj_alloc(mcm_gpu_adjoint* A, std::vector<void*>& list, T** p, size_t n) {
+    void* q = nullptr;
+    if (n == 0) n = 1;
+    HIPCHK_A(hipMalloc(&q, n * sizeof(T)));
+    HIPCHK_A(hipMemsetAsync(q, 0, n * sizeof(T), A->ctx->stream));
+    list.push_back(q);
+    *p = (T*)q;
+    return 0;
+}
+
+const size_t kSnapWords = SN_N;
+
+double* slot_ptr(mcm_gpu_adjoint* A, bool ck, int s) {
+    return (ck ? A->ckpt : A->seg) + (size_t)s * kSnapWords * A->np;
+}
+
+int adj_save(mcm_gpu_adjoint* A, bool ck, int s) {
+    mcm_gpu_ctx* ctx = A->ctx;
+    dim3 grid((unsigned)std::min<long>(blocks_for(A->np, MCM_BLOCK), 1024), SN_N);
+    k_snap<<<grid, MCM_BLOCK, 0, ctx->stream>>>(ctx->d, slot_ptr(A, ck, s), 1);
+    HIPCHK_A(hipMemcpyAsync((ck ? A->ckpt_sc : A->seg_sc) + s, ctx->sc, sizeof(Scalars), hipMemcpyDeviceToDevice,
+                            ctx->stream));
+    (ck ? A->ckpt_cur : A->seg_cur)[s] = ctx->cur;
+    HIPCHK_A(hipGetLastError());
+    return 0;
+}
+
+int adj_restore(mcm_gpu_adjoint* A, bool ck, int s) {
+    mcm_gpu_ctx* ctx = A->ctx;
+    dim3 grid((unsigned)std::min<long>(blocks_for(A->np, MCM_BLOCK), 1024), SN_N);
+    k_snap<<<grid, MCM_BLOCK, 0, ctx->stream>>>(ctx->d, slot_ptr(A, ck, s), 0);
+    HIPCHK_A(hipMemcpyAsync(ctx->sc, (ck ? A->ckpt_sc : A->seg_sc) + s, sizeof(Scalars), hipMemcpyDeviceToDevice,
+                            ctx->stream));
+    ctx->cur = (ck ? A->ckpt_cur : A->seg_cur)[s];
+    ctx->phase = 0;
+    HIPCHK_A(hipGetLastError());
+    return 0;
+}
+
+// nsteps fused steps from the current state, the sampled positions after each
+// step written to hist (device) when given; a step that outgrows a capacity
+// makes the run grow and the whole call start again from its first state,
+// which is saved in slot (ck, s) first.
+int adj_steps(mcm_gpu_adjoint* A, int nsteps, double* hist, bool ck, int s, bool save_each) {
+    mcm_gpu_ctx* ctx = A->ctx;
+    int rc = 0;
+    for (int attempt = 0; attempt < 16; attempt++) {
+        Scalars s0;
+        if ((rc = get_scalars_raw(ctx, &s0))) { A->err = ctx->err; return rc; }
+        for (int n = 0; n < nsteps; n++) {
+            if (save_each && (rc = adj_save(A, false, n))) return rc;
+            launch_step(one(ctx), ctx->cur);
+            ctx->cur ^= 1;
+            if (hist) k_adj_sample<<<blocks_for(A->nsample, 64), 64, 0, ctx->stream>>>(
+                ctx->d, A->d_sample, A->nsample, hist + (size_t)n * 3 * A->nsample);
+        }
+        HIPCHK_A(hipGetLastError());
+        Scalars s1;
+        if ((rc = get_scalars_raw(ctx, &s1))) { A->err = ctx->err; return rc; }
+        if (!s1.abort && s1.timestep - s0.timestep == nsteps) return MCM_GPU_OK;
+        if ((rc = regrow(ctx, s1))) { A->err = ctx->err; return rc; }
+        if ((rc = adj_restore(A, ck, s))) return rc;
+    }
+    A->err = "steps did not fit after regrowing";
+    return MCM_GPU_ERR_CAPACITY;
+}
+
+// ghost-sized adjoint buffers for the run's current capacities
+int adj_ghost_buffers(mcm_gpu_adjoint* A) {
+    const DevState& d = A->ctx->d;
+    AdjDev& a = A->a;
+    if (a.ngcap >= d.ngcap && a.gtcap >= d.ncap && !A->gallocs.empty()) return 0;
+    for (void* p : A->gallocs) (void)hipFree(p);
+    A->gallocs.clear();
+    a.ngcap = d.ngcap;
+    a.gtcap = std::max(a.gtcap, d.ncap);
+    int rc = 0;
+    for (int c = 0; c < 3; c++) {
+        rc |= adj_alloc(A, A->gallocs, &a.gxb[c], a.ngcap);
+        rc |= adj_alloc(A, A->gallocs, &a.gvb[c], a.ngcap);
+    }
+    rc |= adj_alloc(A, A->gallocs, &a.grb, a.ngcap);
+    for (int c = 0; c < 6; c++) rc |= adj_alloc(A, A->gallocs, &a.gAb[c], a.ngcap);
+    rc |= adj_alloc(A, A->gallocs, &a.gt_cnt, a.ngcap);
+    rc |= adj_alloc(A, A->gallocs, &a.gt, (size_t)a.ngcap * a.gtcap);
+    return rc;
+}
+
+// the reference's call sequence of one step through the granular stages,
+// which leave the step's intermediates on the device
+int adj_granular_step(mcm_gpu_adjoint* A) {
+    mcm_gpu_ctx* ctx = A->ctx;
+    int rc = 0;
+    if (!rc) rc = mcm_gpu_neighbours(ctx);
+    if (!rc) rc = mcm_gpu_strain(ctx);
+    if (!rc) rc = mcm_gpu_update_h(ctx);
+    if (!rc) rc = mcm_gpu_constitutive(ctx);
+    if (!rc) rc = mcm_gpu_dt_update(ctx);
+    if (!rc) rc = mcm_gpu_momentum(ctx);
+    if (!rc) rc = mcm_gpu_boundcond(ctx);
+    if (!rc) rc = mcm_gpu_update_velocity(ctx);
+    if (!rc) rc = mcm_gpu_move(ctx);
+    if (rc) A->err = ctx->err;
+    return rc;
+}
+
+// reverse of the step just recomputed (start-of-step state in slot (ck, s))
+int adj_reverse_step(mcm_gpu_adjoint* A, bool ck, int s) {
+    mcm_gpu_ctx* ctx = A->ctx;
+    const DevState& d = ctx->d;
+    AdjDev& a = A->a;
+    hipStream_t st = ctx->stream;
+    int rc = adj_ghost_buffers(A);
+    if (rc) return rc;
+    a.snap = slot_ptr(A, ck, s);
+    a.cur = (ck ? A->ckpt_cur : A->seg_cur)[s];
+    const int nb = ctx->nb, gb = ghost_blocks(d);
+    for (int attempt = 0; attempt < 8; attempt++) {
+        HIPCHK_A(hipMemsetAsync(a.gt_cnt, 0, sizeof(int) * a.ngcap, st));
+        HIPCHK_A(hipMemsetAsync(a.gof_cnt, 0, sizeof(int) * A->np, st));
+        HIPCHK_A(hipMemsetAsync(a.err, 0, sizeof(int), st));
+        k_adj_gt_fill<<<nb, MCM_BLOCK, 0, st>>>(d, a);
+        k_adj_gof_fill<<<gb, MCM_BLOCK, 0, st>>>(d, a);
+        int herr = 0;
+        HIPCHK_A(hipMemcpyAsync(&herr, a.err, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK_A(hipStreamSynchronize(st));
+        if (!herr) break;
+        if (attempt == 7) { A->err = "transposed ghost lists overflowed"; return MCM_GPU_ERR_CAPACITY; }
+        a.gtcap = a.gtcap * 2;   // more real partners of one ghost than slots
+        for (void* p : A->gallocs) (void)hipFree(p);
+        A->gallocs.clear();
+        if ((rc = adj_ghost_buffers(A))) return rc;
+    }
+    k_adj_gt_sort<<<gb, MCM_BLOCK, 0, st>>>(d, a);
+    k_adj_gof_sort<<<nb, MCM_BLOCK, 0, st>>>(d, a);
+    k_adj_move<<<nb, MCM_BLOCK, 0, st>>>(d, a);
+    k_adj_fold_move<<<1, MCM_BLOCK, 0, st>>>(d, a, nb);
+    k_adj_mom<<<nb, MCM_BLOCK, 0, st>>>(d, a);
+    k_adj_mom_ghost<<<gb, MCM_BLOCK, 0, st>>>(d, a);
+    k_adj_gmap_mom<<<nb, MCM_BLOCK, 0, st>>>(d, a);
+    k_adj_const<<<nb, MCM_BLOCK, 0, st>>>(d, a);
+    k_adj_strain<<<nb, MCM_BLOCK, 0, st>>>(d, a);
+    k_adj_strain_ghost<<<gb, MCM_BLOCK, 0, st>>>(d, a);
+    k_adj_gmap_strain<<<nb, MCM_BLOCK, 0, st>>>(d, a);
+    k_adj_fold_end<<<1, MCM_BLOCK, 0, st>>>(d, a, nb);
+    HIPCHK_A(hipGetLastError());
+    return 0;
+}
+
+// Parameter adjoints from the initial state (host; mcm_initial.f90:33-101):
+// sigma_0 = -p_0 I, rho_0 and e_0 per particle, and dt_0 = tssf min critts_0
+// (critts of the argmin particle, which with tcrit_opt 1 also reads |v_0|).
+// The initial state's derivatives come from init_particle with dual material
+// constants; the last dual direction is |v_0|.
+void adj_initial_params(mcm_gpu_adjoint* A, const std::vector<double> sb[6], const std::vector<double>& rb,
+                        const std::vector<double>& eb, double dtb, double* pbar, double* v0bar,
+                        const std::vector<double>& v0) {
+    typedef Dual<ADJ_NPAR + 1> D;
+    mcm_gpu_ctx* ctx = A->ctx;
+    const HostDeck& Dk = ctx->deck;
+    double cmin = 1.0e300;
+    long imin = -1;
+    D cbest;
+    for (long i = 0; i < A->np; i++) {
+        const int m = Dk.pmat[i];
+        const MatParams& M = Dk.mat[m];
+        MatParamsT<D> Md;
+        Md.model = M.model; Md.eos = M.eos; Md.visc_type = M.visc_type; Md.ispall = M.ispall;
+        Md.efps_flag = M.efps_flag; Md.pad_ = 0;
+        Md.rho = M.rho; Md.g = M.g; Md.av_l = M.av_l; Md.av_q = M.av_q; Md.h = M.h; Md.rho_max = M.rho_max;
+        for (int k = 0; k < 48; k++) Md.strinput[k] = M.strinput[k];
+        for (int k = 0; k < 24; k++) Md.eosinput[k] = M.eosinput[k];
+        Md.strinput[0].d[0] = 1.0;
+        Md.strinput[1].d[1] = 1.0;
+        Md.strinput[2].d[2] = 1.0;
+        Md.eosinput[1].d[3] = 1.0;
+        Md.g = Md.strinput[0];
+        PStateT<D> s{};
+        init_particle(s, Md, Dk, i, Dk.pm[i], M.strinput[3]);
+        s.tracerod = 0.0;
+        s.h = Dk.init_h_opt == 1 ? Dk.ph[i] : M.h;   // mcm_init_h.f90:22-48
+        double va = 0.0;
+        for (int c = 0; c < 3; c++) va = va + v0[3 * i + c] * v0[3 * i + c];
+        s.vabs = std::sqrt(va);
+        s.vabs.d[ADJ_NPAR] = 1.0;
+        const D cr = critts(s, Md, Dk.tcrit_opt, 0.0);
+        const double sp = -(sb[0][i] + sb[1][i] + sb[2][i]);
+        for (int k = 0; k < ADJ_NPAR; k++)
+            pbar[ADJ_NPAR * m + k] += sp * s.p.d[k] + rb[i] * s.rho.d[k] + eb[i] * s.e.d[k];
+        if (cr.v < cmin) { cmin = cr.v; imin = i; cbest = cr; }
+    }
+    const bool itss_binds = Dk.itss != 0.0 && Dk.itss < cmin * Dk.tssf;
+    if (imin >= 0 && !itss_binds) {
+        const int m = Dk.pmat[imin];
+        for (int k = 0; k < ADJ_NPAR; k++) pbar[ADJ_NPAR * m + k] += dtb * Dk.tssf * cbest.d[k];
+        const double va = std::sqrt(v0[3 * imin] * v0[3 * imin] + v0[3 * imin + 1] * v0[3 * imin + 1] +
+                                                         v0[3 * imin + 2] * v0[3 * imin + 2]);
+        if (va > 0.0)
+            for (int c = 0; c < 3; c++)
+                v0bar[3 * imin + c] += dtb * Dk.tssf * cbest.d[ADJ_NPAR] * v0[3 * imin + c] / va;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int mcm_gpu_set_material_params(mcm_gpu_ctx* ctx, int32_t mat, const double* params) {
+    if (!ctx || !params || mat < 0 || mat >= ctx->deck.nummat) return MCM_GPU_ERR_ARG;
+    MatParams& M = ctx->deck.mat[mat];
+    M.strinput[0] = params[0];
+    M.strinput[1] = params[1];
+    M.strinput[2] = params[2];
+    M.eosinput[1] = params[3];
+    if (M.model == 10) M.g = M.strinput[0];   // mcm_calcg.f90
+    return MCM_GPU_OK;
+}
+
+int mcm_gpu_get_material_params(mcm_gpu_ctx* ctx, int32_t mat, double* params) {
+    if (!ctx || !params || mat < 0 || mat >= ctx->deck.nummat) return MCM_GPU_ERR_ARG;
+    const MatParams& M = ctx->deck.mat[mat];
+    params[0] = M.strinput[0];
+    params[1] = M.strinput[1];
+    params[2] = M.strinput[2];
+    params[3] = M.eosinput[1];
+    return MCM_GPU_OK;
+}
+
+int mcm_gpu_set_initial_velocity(mcm_gpu_ctx* ctx, const double* v, int64_t n) {
+    if (!ctx || !v || n != 3 * (int64_t)ctx->deck.np) return MCM_GPU_ERR_ARG;
+    ctx->deck.v.assign(v, v + n);
+    return MCM_GPU_OK;
+}
+
+int mcm_gpu_adj_forward(mcm_gpu_ctx* ctx, int32_t nsteps, int32_t ckpt_every, const int32_t* sample,
+                        int32_t nsample, double* hist, mcm_gpu_adjoint** out) {
+    if (!ctx || !out || nsteps <= 0 || ckpt_every <= 0 || nsample < 0 || (nsample > 0 && (!sample || !hist)))
+        return MCM_GPU_ERR_ARG;
+    *out = nullptr;
+    const DevState& d = ctx->d;
+    std::string why;
+    if (ctx->phase != 0 || ctx->batch) why = "the run must be at a step boundary and not part of a batch";
+    else if (d.disctype != 0) why = "the adjoint covers plain SPH (disctype 0)";
+    else if (d.h_opt != 0) why = "the adjoint covers fixed h (h_opt 0)";
+    else if (d.tcrit_opt != 0 && d.tcrit_opt != 1) why = "the adjoint covers tcrit_opt 0 and 1";
+    else if (d.contacttype > 0 || d.veloc_opt == 1 || d.drelax || d.baseaccel)
+        why = "the adjoint does not cover penalty contact, XSPH, dynamic relaxation or base acceleration";
+    for (int m = 0; m < d.nummat && why.empty(); m++)
+        if (ctx->deck.mat[m].model != 10) why = "the adjoint covers material model 10";
+    std::vector<char> seen(d.np, 0);
+    for (int k = 0; k < nsample && why.empty(); k++) {
+        if (sample[k] < 0 || sample[k] >= d.np || seen[sample[k]]) why = "sampled particles must be distinct ids below np";
+        else seen[sample[k]] = 1;
+    }
+    if (!why.empty()) { ctx->err = why; return MCM_GPU_ERR_ARG; }
+    mcm_gpu_adjoint* A = new mcm_gpu_adjoint();
+    *out = A;
+    A->ctx = ctx;
+    A->nsteps = nsteps;
+    A->k = ckpt_every;
+    A->nseg = (nsteps + ckpt_every - 1) / ckpt_every;
+    A->nsample = nsample;
+    A->np = d.np;
+    A->ckpt_cur.assign(A->nseg, 0);
+    A->seg_cur.assign(ckpt_every, 0);
+    const size_t slot = kSnapWords * A->np;
+    int rc = 0;
+    rc |= adj_alloc(A, A->allocs, &A->ckpt, slot * A->nseg);
+    rc |= adj_alloc(A, A->allocs, &A->ckpt_sc, A->nseg);
+    if (ckpt_every > 1) {
+        rc |= adj_alloc(A, A->allocs, &A->seg, slot * ckpt_every);
+        rc |= adj_alloc(A, A->allocs, &A->seg_sc, ckpt_every);
+    }
+    rc |= adj_alloc(A, A->allocs, &A->d_sample, std::max(nsample, 1));
+    rc |= adj_alloc(A, A->allocs, &A->d_hist, (size_t)nsteps * 3 * std::max(nsample, 1));
+    AdjDev& a = A->a;
+    const size_t n = A->np;
+    for (int c = 0; c < 3; c++) {
+        rc |= adj_alloc(A, A->allocs, &a.xb[c], n);
+        rc |= adj_alloc(A, A->allocs, &a.vb[c], n);
+        rc |= adj_alloc(A, A->allocs, &a.ab[c], n);
+    }
+    for (int c = 0; c < 6; c++) rc |= adj_alloc(A, A->allocs, &a.sb[c], n);
+    for (int c = 0; c < 9; c++) rc |= adj_alloc(A, A->allocs, &a.lb[c], n);
+    rc |= adj_alloc(A, A->allocs, &a.rb, n);
+    rc |= adj_alloc(A, A->allocs, &a.eb, n);
+    rc |= adj_alloc(A, A->allocs, &a.fb, n);
+    rc |= adj_alloc(A, A->allocs, &a.qb, n);
+    rc |= adj_alloc(A, A->allocs, &a.rbn, n);
+    rc |= adj_alloc(A, A->allocs, &a.gof_cnt, n);
+    rc |= adj_alloc(A, A->allocs, &a.gof, n * ADJ_GOF);
+    rc |= adj_alloc(A, A->allocs, &a.red, (size_t)ctx->nb * RA_N + 64);
+    rc |= adj_alloc(A, A->allocs, &a.as, AS_N);
+    rc |= adj_alloc(A, A->allocs, &a.err, 1);
+    if (rc) return rc;
+    a.gtcap = 0;
+    a.ngcap = 0;
+    if (nsample > 0)
+        HIPCHK_A(hipMemcpyAsync(A->d_sample, sample, sizeof(int) * nsample, hipMemcpyHostToDevice, ctx->stream));
+    for (int sgi = 0; sgi < A->nseg; sgi++) {
+        const int s0 = sgi * A->k, s1 = std::min(s0 + A->k, nsteps);
+        if ((rc = adj_save(A, true, sgi))) return rc;
+        if ((rc = adj_steps(A, s1 - s0, nsample > 0 ? A->d_hist + (size_t)s0 * 3 * nsample : nullptr, true, sgi,
+                            false)))
+            return rc;
+    }
+    if (nsample > 0)
+        HIPCHK_A(hipMemcpyAsync(hist, A->d_hist, sizeof(double) * 3 * nsample * (size_t)nsteps,
+                                hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK_A(hipStreamSynchronize(ctx->stream));
+    return MCM_GPU_OK;
+}
+
+int mcm_gpu_adj_backward(mcm_gpu_adjoint* A, const double* xbar, const double* vbar, const double* histbar,
+                         double* v0bar, double* pbar) {
+    if (!A || !v0bar || !pbar) return MCM_GPU_ERR_ARG;
+    mcm_gpu_ctx* ctx = A->ctx;
+    hipStream_t st = ctx->stream;
+    AdjDev& a = A->a;
+    const long np = A->np;
+    int rc = 0;
+    // adjoint of the end state (the run's stream is non-blocking: plain
+    // copies are ordered with it by synchronising it)
+    HIPCHK_A(hipStreamSynchronize(st));
+    std::vector<double> h(np);
+    for (int c = 0; c < 3; c++) {
+        for (long i = 0; i < np; i++) h[i] = xbar ? xbar[3 * i + c] : 0.0;
+        HIPCHK_A(hipMemcpy(a.xb[c], h.data(), sizeof(double) * np, hipMemcpyHostToDevice));
+        for (long i = 0; i < np; i++) h[i] = vbar ? vbar[3 * i + c] : 0.0;
+        HIPCHK_A(hipMemcpy(a.vb[c], h.data(), sizeof(double) * np, hipMemcpyHostToDevice));
+    }
+    double* zero[] = {a.rb, a.eb, a.fb, a.qb, a.rbn, a.sb[0], a.sb[1], a.sb[2], a.sb[3], a.sb[4], a.sb[5]};
+    for (double* z : zero) HIPCHK_A(hipMemsetAsync(z, 0, sizeof(double) * np, st));
+    HIPCHK_A(hipMemsetAsync(a.as, 0, sizeof(double) * AS_N, st));
+    const bool seeds = histbar && A->nsample > 0;
+    if (seeds)
+        HIPCHK_A(hipMemcpyAsync(A->d_hist, histbar, sizeof(double) * 3 * A->nsample * (size_t)A->nsteps,
+                                hipMemcpyHostToDevice, st));
+    for (int sgi = A->nseg - 1; sgi >= 0; sgi--) {
+        const int s0 = sgi * A->k, s1 = std::min(s0 + A->k, A->nsteps);
+        const bool ck = A->k == 1;
+        if (!ck) {   // recompute the segment, keeping the start of every step
+            if ((rc = adj_restore(A, true, sgi))) return rc;
+            if ((rc = adj_steps(A, s1 - s0 - 1, nullptr, true, sgi, true))) return rc;
+            if ((rc = adj_save(A, false, s1 - s0 - 1))) return rc;
+        }
+        for (int n = s1 - 1; n >= s0; n--) {
+            const int s = ck ? n : n - s0;
+            if ((rc = adj_restore(A, ck, s))) return rc;
+            if (seeds)
+                k_adj_seed<<<blocks_for(A->nsample, 64), 64, 0, st>>>(a, A->d_sample, A->nsample,
+                                                                      A->d_hist + (size_t)n * 3 * A->nsample);
+            if ((rc = adj_granular_step(A))) return rc;
+            if ((rc = adj_reverse_step(A, ck, s))) return rc;
+        }
+    }
+    // results
+    HIPCHK_A(hipStreamSynchronize(st));
+    std::vector<double> vb[3], sbh[6], rbh(np), ebh(np);
+    for (int c = 0; c < 3; c++) {
+        vb[c].resize(np);
+        HIPCHK_A(hipMemcpy(vb[c].data(), a.vb[c], sizeof(double) * np, hipMemcpyDeviceToHost));
+    }
+    for (int c = 0; c < 6; c++) {
+        sbh[c].resize(np);
+        HIPCHK_A(hipMemcpy(sbh[c].data(), a.sb[c], sizeof(double) * np, hipMemcpyDeviceToHost));
+    }
+    HIPCHK_A(hipMemcpy(rbh.data(), a.rb, sizeof(double) * np, hipMemcpyDeviceToHost));
+    HIPCHK_A(hipMemcpy(ebh.data(), a.eb, sizeof(double) * np, hipMemcpyDeviceToHost));
+    std::vector<double> as(AS_N);
+    HIPCHK_A(hipMemcpy(as.data(), a.as, sizeof(double) * AS_N, hipMemcpyDeviceToHost));
+    const HostDeck& Dk = ctx->deck;
+    static const int mask[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {0, 1, 1}, {1, 0, 1}, {1, 1, 1}};
+    // v_0 as the initialisation leaves it (mcm_init_boundcond.f90:25-54 zeroes
+    // fixed components)
+    std::vector<double> v0(3 * np);
+    for (long i = 0; i < np; i++) {
+        const int bc = Dk.dispbc[i];
+        for (int c = 0; c < 3; c++) {
+            const bool fixed = bc >= 1 && bc <= 7 && mask[bc][c];
+            v0[3 * i + c] = fixed ? 0.0 : Dk.v[3 * i + c];
+            v0bar[3 * i + c] = vb[c][i];
+        }
+    }
+    const int nm = ctx->d.nummat;
+    for (int k = 0; k < ADJ_NPAR * nm; k++) pbar[k] = as[AS_PAR + k];
+    adj_initial_params(A, sbh, rbh, ebh, as[AS_DTB], pbar, v0bar, v0);
+    for (long i = 0; i < np; i++) {
+        const int bc = Dk.dispbc[i];
+        for (int c = 0; c < 3; c++)
+            if (bc >= 1 && bc <= 7 && mask[bc][c]) v0bar[3 * i + c] = 0.0;
+    }
+    return MCM_GPU_OK;
+}
+
+const char* mcm_gpu_adj_last_error(const mcm_gpu_adjoint* A) { return A ? A->err.c_str() : "null adjoint"; }
+
+void mcm_gpu_adj_destroy(mcm_gpu_adjoint* A) {
+    if (!A) return;
+    if (A->ctx && A->ctx->stream) (void)hipStreamSynchronize(A->ctx->stream);
+    for (void* p : A->allocs) (void)hipFree(p);
+    for (void* p : A->gallocs) (void)hipFree(p);
+    delete A;
+}
+
+}  // extern "C"

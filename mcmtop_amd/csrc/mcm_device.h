@@ -95,6 +95,8 @@ struct DevState {
     int* pmat;                 // [U] 0-based material
     int* dispbc;
     int* gpar;                 // [ngcap] parent of each ghost (0-based real id)
+    int* gsgn;                 // [ngcap] bit k set: the ghost's x_k and v_k are its parent's mirrored
+                               // an odd number of times (product of the plane multipliers is -1)
     double *hold, *h0, *rho0, *p, *e, *c, *vabs, *pcut, *p_cut, *efps, *fail, *q, *mindist;
     double *sig[6], *sd[6], *alfa[6];
     double* rs;                // [U][8] strain gather record {x, v, m/rho, h} (start of step)

@@ -570,6 +570,7 @@ int alloc_all(mcm_gpu_ctx* ctx) {
     rc |= dalloc(ctx, &d.rs, 8 * u);
     rc |= dalloc(ctx, &d.rm, 12 * u);
     rc |= dalloc(ctx, &d.gpar, d.ngcap);
+    rc |= dalloc(ctx, &d.gsgn, d.ngcap);
     rc |= dalloc(ctx, &d.dispbc, n);
     double** per[] = {&d.hold, &d.h0, &d.rho0, &d.p, &d.e, &d.c, &d.vabs, &d.pcut, &d.p_cut, &d.efps, &d.fail,
                       &d.q, &d.mindist, &d.tracerod, &d.einc, &d.etry, &d.rhoold, &d.critts_p, &d.defps_prev};
@@ -850,6 +851,7 @@ int regrow(mcm_gpu_ctx* ctx, const Scalars& sc_host) {
         rc |= dregrow(ctx, &d.rs, 8 * u, 8 * keep);
         rc |= dregrow(ctx, &d.rm, 12 * u, 12 * keep);
         rc |= dregrow(ctx, &d.gpar, ng_new, 0);
+        rc |= dregrow(ctx, &d.gsgn, ng_new, 0);
         if (rc) return rc;
         d.ngcap = ng_new;
     }
@@ -867,12 +869,59 @@ int regrow(mcm_gpu_ctx* ctx, const Scalars& sc_host) {
 }
 
 // ---------------------------------------------------------- initialisation
+// Density, energy, pressure and sound speed of particle i at the start
+// (mcm_calcrhopc.f90:31-95 with mcm_eoscalc.f90:27-113).  T is double, or a
+// dual number when the adjoint differentiates the initial state with
+// respect to material constants.
+template <class T, class M>
+void init_particle(PStateT<T>& s, const M& Mt, const HostDeck& D, long i, double mass, double pcut) {
+    using std::sqrt;
+    s.mass = mass;
+    s.rho0 = Mt.rho;
+    s.rho = D.init_rhoe == 1 ? D.pr[i] : 0.0;
+    s.e = D.init_rhoe == 1 ? D.pe[i] : 0.0;
+    s.pcut = pcut;
+    if (Mt.model == 1 || Mt.model == 3) {
+        if (D.init_rhoe != 1) { s.rho = s.rho0; s.e = 0.0; }
+        s.p = 0.0;
+        s.c = sqrt(Mt.strinput[1] / s.rho);
+    } else {
+        const auto* ei = Mt.eosinput;
+        if (D.init_rhoe != 1) {
+            switch (Mt.eos) {
+                case 1: s.rho = s.rho0 / ei[8]; s.e = ei[7] * s.mass / s.rho; break;
+                case 4: s.rho = s.rho0 / ei[7]; s.e = ei[6] * s.mass / s.rho; break;
+                case 13: s.rho = s.rho0 / ei[3]; s.e = ei[2] * s.mass / s.rho; break;
+                case 28: s.rho = s.rho0 / ei[3]; s.e = ei[2] * s.mass; break;
+                case 41: s.rho = s.rho0 / ei[5]; s.e = ei[2] * ei[3] * s.mass / s.rho; break;
+            }
+        }
+        eos_p(s, Mt);
+        sueos(s, Mt);
+        if (Mt.model == 9 && s.p < Mt.strinput[0]) s.p = Mt.strinput[0];
+    }
+}
+
 // mcm_initial.f90:33-101: the per-particle part runs once on the host, the
 // first neighbour search, critts and energy totals on the device.
 int initial(mcm_gpu_ctx* ctx) {
     HostDeck& D = ctx->deck;
     DevState& d = ctx->d;
     const long n = D.np;
+    {   // state that the upload below does not set starts at zero, as in a new run
+        double* zero[] = {d.p_cut, d.efps, d.q, d.mindist, d.tracerod, d.einc, d.etry, d.critts_p, d.defps_prev};
+        for (double* z : zero) HIPCHK(hipMemsetAsync(z, 0, n * sizeof(double), ctx->stream));
+        for (int k = 0; k < 6; k++) {
+            HIPCHK(hipMemsetAsync(d.sd[k], 0, n * sizeof(double), ctx->stream));
+            HIPCHK(hipMemsetAsync(d.alfa[k], 0, n * sizeof(double), ctx->stream));
+            HIPCHK(hipMemsetAsync(d.rod[k], 0, n * sizeof(double), ctx->stream));
+        }
+        for (int k = 0; k < 3; k++) {
+            HIPCHK(hipMemsetAsync(d.spin[k], 0, n * sizeof(double), ctx->stream));
+            HIPCHK(hipMemsetAsync(d.a[k], 0, n * sizeof(double), ctx->stream));
+        }
+        HIPCHK(hipMemsetAsync(d.yield, 0, n * sizeof(int), ctx->stream));
+    }
     static const int mask[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {0, 1, 1}, {1, 0, 1}, {1, 1, 1}};
     std::vector<double> v = D.v;
     for (long i = 0; i < n; i++) {   // mcm_init_boundcond.f90:25-54
@@ -908,32 +957,8 @@ int initial(mcm_gpu_ctx* ctx) {
             if (hx[k][i] < sc.coord_min[k]) sc.coord_min[k] = hx[k][i];
             if (hx[k][i] > sc.coord_max[k]) sc.coord_max[k] = hx[k][i];
         }
-        // mcm_calcrhopc.f90:31-95 with mcm_eoscalc.f90:27-113
         PState s{};
-        s.mass = mass[i];
-        s.rho0 = M.rho;
-        s.rho = D.init_rhoe == 1 ? D.pr[i] : 0.0;
-        s.e = D.init_rhoe == 1 ? D.pe[i] : 0.0;
-        s.pcut = pcut[i];
-        if (M.model == 1 || M.model == 3) {
-            if (D.init_rhoe != 1) { s.rho = s.rho0; s.e = 0.0; }
-            s.p = 0.0;
-            s.c = std::sqrt(M.strinput[1] / s.rho);
-        } else {
-            const double* ei = M.eosinput;
-            if (D.init_rhoe != 1) {
-                switch (M.eos) {
-                    case 1: s.rho = s.rho0 / ei[8]; s.e = ei[7] * s.mass / s.rho; break;
-                    case 4: s.rho = s.rho0 / ei[7]; s.e = ei[6] * s.mass / s.rho; break;
-                    case 13: s.rho = s.rho0 / ei[3]; s.e = ei[2] * s.mass / s.rho; break;
-                    case 28: s.rho = s.rho0 / ei[3]; s.e = ei[2] * s.mass; break;
-                    case 41: s.rho = s.rho0 / ei[5]; s.e = ei[2] * ei[3] * s.mass / s.rho; break;
-                }
-            }
-            eos_p(s, M);
-            sueos(s, M);
-            if (M.model == 9 && s.p < M.strinput[0]) s.p = M.strinput[0];
-        }
+        init_particle(s, M, D, i, mass[i], pcut[i]);
         rho[i] = s.rho; rho0[i] = s.rho0; p[i] = s.p; e[i] = s.e; c[i] = s.c;
         for (int k = 0; k < 3; k++) sig[k][i] = -s.p;   // mcm_initsigma.f90:35-43
         h[i] = D.init_h_opt == 1 ? D.ph[i] : M.h;         // mcm_init_h.f90:22-48
@@ -1852,3 +1877,5 @@ void mcm_gpu_batch_destroy(mcm_gpu_batch* b) {
 }
 
 }  // extern "C"
+
+#include "mcm_adjoint.hip"

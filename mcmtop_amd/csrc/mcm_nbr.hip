@@ -321,6 +321,8 @@ __device__ __forceinline__ void make_ghost(const DevState& d, int cur, const Pla
     d.rbuf[0][u] = rv; d.rbuf[1][u] = rv;
     d.pmat[u] = d.pmat[us];
     d.gpar[gid] = from_real ? src : d.gpar[src];
+    const int pm = (P.mult[0] < 0.0 ? 1 : 0) | (P.mult[1] < 0.0 ? 2 : 0) | (P.mult[2] < 0.0 ? 4 : 0);
+    d.gsgn[gid] = (from_real ? 0 : d.gsgn[src]) ^ pm;
     // gather records of the ghost: start-of-step x, v, m, rho and h are
     // frozen for the whole step (mcm_ghost_setup.f90:330-341)
     double2* rs = reinterpret_cast<double2*>(d.rs) + 4 * (long)u;

@@ -14,27 +14,35 @@
 #include <math.h>
 #include <stdint.h>
 
+#include "mcm_dual.h"
+
 #define MCM_HD __host__ __device__ __forceinline__
 
 enum { S11 = 0, S22 = 1, S33 = 2, S12 = 3, S13 = 4, S23 = 5 };
 
-// Material constants (mcm_database.f90:235-257 plus derived values)
-struct MatParams {
+// Material constants (mcm_database.f90:235-257 plus derived values).  The
+// scalar type is double in the step; the adjoint instantiates the physics
+// below with dual numbers (mcm_dual.h) to take local derivatives.
+template <class T>
+struct MatParamsT {
     int model, eos, visc_type, ispall, efps_flag;
     int pad_;
-    double rho, g, av_l, av_q, h, rho_max;
-    double strinput[48];
-    double eosinput[24];
-    double k_cont, n_cont;   // penalty contact K and n (mcm_contact_in.f90:22-45)
+    T rho, g, av_l, av_q, h, rho_max;
+    T strinput[48];
+    T eosinput[24];
+    T k_cont, n_cont;   // penalty contact K and n (mcm_contact_in.f90:22-45)
 };
+typedef MatParamsT<double> MatParams;
 
 // Per-particle state held in registers by the fused kernels.
-struct PState {
-    double mass, rho, rhoold, rho0, h, hold, h0;
-    double p, e, etry, einc, c, vabs, pcut, p_cut, efps, fail, q, critts;
-    double sig[6], s[6], alfa[6];
-    double rod[6], spin[3], tracerod;
+template <class T>
+struct PStateT {
+    T mass, rho, rhoold, rho0, h, hold, h0;
+    T p, e, etry, einc, c, vabs, pcut, p_cut, efps, fail, q, critts;
+    T sig[6], s[6], alfa[6];
+    T rod[6], spin[3], tracerod;
 };
+typedef PStateT<double> PState;
 
 MCM_HD double fsign1(double x) { return signbit(x) ? -1.0 : 1.0; }
 MCM_HD double dmax(double a, double b) { return a > b ? a : b; }
@@ -205,7 +213,8 @@ MCM_HD bool update_h(PState& s, int h_opt, double dt) {
 
 // ----------------------------------------------------------------- rhoupd
 // mcm_rhoupd.f90:24-71 (the out-of-range warnings are not reproduced)
-MCM_HD void rhoupd(PState& s, const MatParams& m, double dt) {
+template <class T, class M>
+MCM_HD void rhoupd(PStateT<T>& s, const M& m, T dt) {
     const double othird = 1.0 / 3.0;
     s.rhoold = s.rho;
     s.rho = s.rho * (1.0 - s.tracerod * dt);
@@ -225,48 +234,49 @@ MCM_HD void rhoupd(PState& s, const MatParams& m, double dt) {
 // ------------------------------------------------------------ stress_rot1
 // mcm_stress_rot1.f90:22-184 (iorder = 3); the qq accumulation multiplies a
 // zero matrix in the reference and is not carried.
-MCM_HD void stress_rot1(PState& s, double dt) {
+template <class T>
+MCM_HD void stress_rot1(PStateT<T>& s, T dt) {
     const double etol = (double)1.0e-3f;   // single-precision literal of the reference
     const double z1 = 1.0 / 3.0, z2 = 2.0 / 15.0, z3 = 17.0 / 315.0, z4 = 62.0 / 2835.0;
-    double s11 = s.sig[S11], s22 = s.sig[S22], s33 = s.sig[S33];
-    double s12 = s.sig[S12], s23 = s.sig[S23], s31 = s.sig[S13];
-    double dt2 = 0.5 * dt;
-    double wzz = -(dt2 * s.spin[0]);
-    double wyy = dt2 * s.spin[1];
-    double wxx = -(dt2 * s.spin[2]);
-    double wxx2 = wxx * wxx, wyy2 = wyy * wyy, wzz2 = wzz * wzz;
-    double qm2 = wxx2 + wyy2 + wzz2;
-    double qm = sqrt(qm2);
-    double dd = tan(qm * 0.5);
-    double aa, cc;
+    T s11 = s.sig[S11], s22 = s.sig[S22], s33 = s.sig[S33];
+    T s12 = s.sig[S12], s23 = s.sig[S23], s31 = s.sig[S13];
+    T dt2 = 0.5 * dt;
+    T wzz = -(dt2 * s.spin[0]);
+    T wyy = dt2 * s.spin[1];
+    T wxx = -(dt2 * s.spin[2]);
+    T wxx2 = wxx * wxx, wyy2 = wyy * wyy, wzz2 = wzz * wzz;
+    T qm2 = wxx2 + wyy2 + wzz2;
+    T qm = sqrt(qm2);
+    T dd = tan(qm * 0.5);
+    T aa, cc;
     if (qm < etol) {
         aa = qm2 * 0.25;
         cc = 0.5 * (1.0 + aa * (z1 + aa * (z2 + aa * (z3 + aa * z4))));
     } else {
         cc = dd / qm;
     }
-    double bb = 2.0 * cc / (1.0 + dd * dd);
+    T bb = 2.0 * cc / (1.0 + dd * dd);
     aa = bb * cc;
-    double Q00 = 1.0 - aa * (wyy2 + wzz2);
-    double Q11 = 1.0 - aa * (wxx2 + wzz2);
-    double Q22 = 1.0 - aa * (wxx2 + wyy2);
-    double Q01 = aa * wxx * wyy;
-    double Q02 = aa * wxx * wzz;
-    double Q12 = aa * wyy * wzz;
-    double Q10 = Q01, Q20 = Q02, Q21 = Q12;
-    double bwxx = bb * wxx, bwyy = bb * wyy, bwzz = bb * wzz;
+    T Q00 = 1.0 - aa * (wyy2 + wzz2);
+    T Q11 = 1.0 - aa * (wxx2 + wzz2);
+    T Q22 = 1.0 - aa * (wxx2 + wyy2);
+    T Q01 = aa * wxx * wyy;
+    T Q02 = aa * wxx * wzz;
+    T Q12 = aa * wyy * wzz;
+    T Q10 = Q01, Q20 = Q02, Q21 = Q12;
+    T bwxx = bb * wxx, bwyy = bb * wyy, bwzz = bb * wzz;
     Q01 = Q01 - bwzz; Q10 = Q10 + bwzz;
     Q02 = Q02 + bwyy; Q20 = Q20 - bwyy;
     Q12 = Q12 - bwxx; Q21 = Q21 + bwxx;
-    double qs00 = Q00 * s11 + Q01 * s12 + Q02 * s31;
-    double qs01 = Q00 * s12 + Q01 * s22 + Q02 * s23;
-    double qs02 = Q00 * s31 + Q01 * s23 + Q02 * s33;
-    double qs10 = Q10 * s11 + Q11 * s12 + Q12 * s31;
-    double qs11 = Q10 * s12 + Q11 * s22 + Q12 * s23;
-    double qs12 = Q10 * s31 + Q11 * s23 + Q12 * s33;
-    double qs20 = Q20 * s11 + Q21 * s12 + Q22 * s31;
-    double qs21 = Q20 * s12 + Q21 * s22 + Q22 * s23;
-    double qs22 = Q20 * s31 + Q21 * s23 + Q22 * s33;
+    T qs00 = Q00 * s11 + Q01 * s12 + Q02 * s31;
+    T qs01 = Q00 * s12 + Q01 * s22 + Q02 * s23;
+    T qs02 = Q00 * s31 + Q01 * s23 + Q02 * s33;
+    T qs10 = Q10 * s11 + Q11 * s12 + Q12 * s31;
+    T qs11 = Q10 * s12 + Q11 * s22 + Q12 * s23;
+    T qs12 = Q10 * s31 + Q11 * s23 + Q12 * s33;
+    T qs20 = Q20 * s11 + Q21 * s12 + Q22 * s31;
+    T qs21 = Q20 * s12 + Q21 * s22 + Q22 * s23;
+    T qs22 = Q20 * s31 + Q21 * s23 + Q22 * s33;
     s.sig[S11] = qs00 * Q00 + qs01 * Q01 + qs02 * Q02;
     s.sig[S22] = qs10 * Q10 + qs11 * Q11 + qs12 * Q12;
     s.sig[S33] = qs20 * Q20 + qs21 * Q21 + qs22 * Q22;
@@ -276,18 +286,20 @@ MCM_HD void stress_rot1(PState& s, double dt) {
 }
 
 // energy increment before the stress update (mcm_f3dm3.f90:54-56); q is isotropic
-MCM_HD double einc_pre(const PState& s, double dt) {
-    const double* r = s.rod;
-    const double* g = s.sig;
+template <class T>
+MCM_HD T einc_pre(const PStateT<T>& s, T dt) {
+    const T* r = s.rod;
+    const T* g = s.sig;
     return (r[S11] * (g[S11] + s.q) + r[S22] * (g[S22] + s.q) + r[S33] * (g[S33] + s.q) +
             2.0 * (r[S12] * (g[S12] + 0.0) + r[S13] * (g[S13] + 0.0) + r[S23] * (g[S23] + 0.0))) *
            dt;
 }
 
 // energy increment after the stress update (mcm_f3dm3.f90:154-156)
-MCM_HD double einc_post(const PState& s, double dt) {
-    const double* r = s.rod;
-    const double* g = s.sig;
+template <class T>
+MCM_HD T einc_post(const PStateT<T>& s, T dt) {
+    const T* r = s.rod;
+    const T* g = s.sig;
     return 0.5 * ((r[S11] * g[S11] + r[S22] * g[S22] + r[S33] * g[S33] +
                    2.0 * (r[S12] * g[S12] + r[S13] * g[S13] + r[S23] * g[S23])) *
                       dt +
@@ -297,14 +309,15 @@ MCM_HD double einc_post(const PState& s, double dt) {
 #define CM5(a, b) m.strinput[((a) - 1) + 5 * ((b) - 1)]
 
 // mcm_f3dm1.f90:27-71 (elastic)
-MCM_HD void f3dm1(PState& s, const MatParams& m, double dt) {
+template <class T, class M>
+MCM_HD void f3dm1(PStateT<T>& s, const M& m, T dt) {
     const double othird = 1.0 / 3.0;
-    double ym = CM5(1, 1), pr = CM5(1, 2), gg = CM5(5, 4);
-    double gdt = dt * gg;
-    double gd2 = 2.0 * gdt;
-    double blk = dt * ym / (1.0 - 2.0 * pr);
-    double davg = othird * s.tracerod;
-    double p_incr = blk * davg;
+    T ym = CM5(1, 1), pr = CM5(1, 2), gg = CM5(5, 4);
+    T gdt = dt * gg;
+    T gd2 = 2.0 * gdt;
+    T blk = dt * ym / (1.0 - 2.0 * pr);
+    T davg = othird * s.tracerod;
+    T p_incr = blk * davg;
     s.einc = einc_pre(s, dt);
     s.sig[S11] = s.sig[S11] + p_incr + gd2 * (s.rod[S11] - davg);
     s.sig[S22] = s.sig[S22] + p_incr + gd2 * (s.rod[S22] - davg);
@@ -318,16 +331,17 @@ MCM_HD void f3dm1(PState& s, const MatParams& m, double dt) {
 
 // mcm_f3dm3.f90:31-156 (elastic-plastic with kinematic/isotropic hardening);
 // returns the yield active-set flag (scle)
-MCM_HD int f3dm3(PState& s, const MatParams& m, double dt) {
+template <class T, class M>
+MCM_HD int f3dm3(PStateT<T>& s, const M& m, T dt) {
     const double othird = 1.0 / 3.0;
-    double qh = CM5(2, 4), qb = CM5(1, 5), qs = CM5(1, 3), ym = CM5(1, 1), pr = CM5(1, 2);
-    double efps_f = CM5(2, 6);
-    double gg = 2.0 * CM5(5, 4);
-    double gdt = dt * gg;
-    double qbqh = qb * qh;
-    double blk = dt * ym / (1.0 - 2.0 * pr);
-    double davg = othird * s.tracerod;
-    double p_incr = blk * davg;
+    T qh = CM5(2, 4), qb = CM5(1, 5), qs = CM5(1, 3), ym = CM5(1, 1), pr = CM5(1, 2);
+    T efps_f = CM5(2, 6);
+    T gg = 2.0 * CM5(5, 4);
+    T gdt = dt * gg;
+    T qbqh = qb * qh;
+    T blk = dt * ym / (1.0 - 2.0 * pr);
+    T davg = othird * s.tracerod;
+    T p_incr = blk * davg;
     s.p = s.p - p_incr;
     s.einc = einc_pre(s, dt);
     s.sig[S11] = s.sig[S11] + p_incr + gdt * (s.rod[S11] - davg);
@@ -336,25 +350,25 @@ MCM_HD int f3dm3(PState& s, const MatParams& m, double dt) {
     s.sig[S12] = s.sig[S12] + gdt * s.rod[S12];
     s.sig[S13] = s.sig[S13] + gdt * s.rod[S13];
     s.sig[S23] = s.sig[S23] + gdt * s.rod[S23];
-    double k = qs + qbqh * s.efps;
-    double d11 = s.p + s.sig[S11] - s.alfa[S11];
-    double d22 = s.p + s.sig[S22] - s.alfa[S22];
-    double d33 = s.p + s.sig[S33] - s.alfa[S33];
-    double d12 = s.sig[S12] - s.alfa[S12];
-    double d13 = s.sig[S13] - s.alfa[S13];
-    double d23 = s.sig[S23] - s.alfa[S23];
+    T k = qs + qbqh * s.efps;
+    T d11 = s.p + s.sig[S11] - s.alfa[S11];
+    T d22 = s.p + s.sig[S22] - s.alfa[S22];
+    T d33 = s.p + s.sig[S33] - s.alfa[S33];
+    T d12 = s.sig[S12] - s.alfa[S12];
+    T d13 = s.sig[S13] - s.alfa[S13];
+    T d23 = s.sig[S23] - s.alfa[S23];
     s.s[S11] = d11; s.s[S22] = d22; s.s[S33] = d33; s.s[S12] = d12; s.s[S13] = d13; s.s[S23] = d23;
-    double j2 = d23 * d23 + d13 * d13 + d12 * d12 - d11 * d22 - d22 * d33 - d11 * d33;
-    double k2 = 3.0 * j2 - k * k;
-    double scle = 0.5 * (1.0 + fsign1(k2));
-    double fac1 = 1.0 / (1.5 * gg + qh);
-    double fac2 = 1.5 * gg;
-    double fac3 = (1.0 - qb) * qh;
-    double j1 = sqrt(3.0 * fabs(j2)) + 1.0 - scle;
-    double defps = scle * fac1 * (j1 - k);
+    T j2 = d23 * d23 + d13 * d13 + d12 * d12 - d11 * d22 - d22 * d33 - d11 * d33;
+    T k2 = 3.0 * j2 - k * k;
+    T scle = 0.5 * (1.0 + fsign1(k2));
+    T fac1 = 1.0 / (1.5 * gg + qh);
+    T fac2 = 1.5 * gg;
+    T fac3 = (1.0 - qb) * qh;
+    T j1 = sqrt(3.0 * fabs(j2)) + 1.0 - scle;
+    T defps = scle * fac1 * (j1 - k);
     s.efps = s.efps + defps;
-    double des = scle * fac2 * defps / j1;
-    double depn = scle * fac3 * defps / j1;
+    T des = scle * fac2 * defps / j1;
+    T depn = scle * fac3 * defps / j1;
     s.sig[S11] = s.sig[S11] - des * d11;
     s.sig[S22] = s.sig[S22] - des * d22;
     s.sig[S33] = s.sig[S33] - des * d33;
@@ -368,7 +382,7 @@ MCM_HD int f3dm3(PState& s, const MatParams& m, double dt) {
     s.alfa[S13] = s.alfa[S13] + depn * d13;
     s.alfa[S23] = s.alfa[S23] + depn * d23;
     if (m.efps_flag == 1 && s.efps > efps_f) {
-        double press = othird * (s.sig[S11] + s.sig[S22] + s.sig[S33]);
+        T press = othird * (s.sig[S11] + s.sig[S22] + s.sig[S33]);
         s.sig[S11] = press; s.sig[S22] = press; s.sig[S33] = press;
         s.sig[S12] = 0.0; s.sig[S13] = 0.0; s.sig[S23] = 0.0;
     }
@@ -397,17 +411,18 @@ MCM_HD void f3dm9(PState& s, const MatParams& m, int matid1) {
 // mcm_f3dm10.f90:28-157 (elastic-plastic-hydrodynamic).  defps_prev stands
 // in for the plastic-strain increment the reference reads uninitialised when
 // a particle has failed (:82-89); returns the yield active-set flag.
-MCM_HD int f3dm10(PState& s, const MatParams& m, double dt, double& defps_prev) {
+template <class T, class M>
+MCM_HD int f3dm10(PStateT<T>& s, const M& m, T dt, double& defps_prev) {
     const double othird = 1.0 / 3.0;
-    const double* cm = m.strinput;
-    double gg = cm[0], qs = cm[1], qh = cm[2], a1 = cm[4], a2 = cm[5];
+    const auto* cm = m.strinput;
+    T gg = cm[0], qs = cm[1], qh = cm[2], a1 = cm[4], a2 = cm[5];
     int ispall = m.ispall;
-    double epf = cm[7];
-    double g2 = 2.0 * dt * gg;
-    double davg = othird * s.tracerod;
+    T epf = cm[7];
+    T g2 = 2.0 * dt * gg;
+    T davg = othird * s.tracerod;
     s.p = -(othird * (s.sig[S11] + s.sig[S22] + s.sig[S33]));
     s.p_cut = s.pcut;
-    double ak = qs + qh * s.efps + (a1 + a2 * s.p) * dmax(0.0, s.p);
+    T ak = qs + qh * s.efps + (a1 + a2 * s.p) * dmax(0.0, s.p);
     s.s[S11] = s.sig[S11] + s.p + g2 * (s.rod[S11] + davg);
     s.s[S22] = s.sig[S22] + s.p + g2 * (s.rod[S22] + davg);
     s.s[S33] = s.sig[S33] + s.p + g2 * (s.rod[S33] + davg);
@@ -415,29 +430,29 @@ MCM_HD int f3dm10(PState& s, const MatParams& m, double dt, double& defps_prev) 
     s.s[S13] = s.sig[S13] + 0.5 * g2 * s.rod[S13];
     s.s[S23] = s.sig[S23] + 0.5 * g2 * s.rod[S23];
     if (ispall == 3) {
-        double deltap = s.p - s.p_cut;
-        double scle = 0.5 * (1.0 + fsign1(deltap));
+        T deltap = s.p - s.p_cut;
+        T scle = 0.5 * (1.0 + fsign1(deltap));
         s.pcut = scle * s.p_cut;
         for (int k = 0; k < 6; k++) s.s[k] = scle * s.s[k];
         s.p_cut = s.pcut;
     }
-    double* d = s.s;
-    double aj2 = 0.5 * (d[S11] * d[S11] + d[S22] * d[S22] + d[S33] * d[S33]) + d[S12] * d[S12] +
+    T* d = s.s;
+    T aj2 = 0.5 * (d[S11] * d[S11] + d[S22] * d[S22] + d[S33] * d[S33]) + d[S12] * d[S12] +
                  d[S13] * d[S13] + d[S23] * d[S23];
-    double sj2 = sqrt(3.0 * aj2);
-    double defps = defps_prev;
+    T sj2 = sqrt(3.0 * aj2);
+    T defps = defps_prev;
     int active = 0;
     if (s.fail == 1.0) {
-        double thrg = othird / gg;
+        T thrg = othird / gg;
         defps = ddim(sj2, ak) / ((double)1.e-30f + ddim(1.0, -(thrg * qh)));
         defps = thrg * dmin(sj2, defps);
         s.efps = s.efps + defps;
         active = sj2 > ak ? 1 : 0;
     }
-    defps_prev = defps;
+    defps_prev = dval(defps);
     ak = dmax(ak + qh * defps, 0.0);
-    double seps = (double)1.e-30f * gg;
-    double scle = (ak + seps) / (dmax(ak, sj2) + seps);
+    T seps = (double)1.e-30f * gg;
+    T scle = (ak + seps) / (dmax(ak, sj2) + seps);
     for (int k = 0; k < 6; k++) d[k] = scle * d[k];
     if (epf != 0.0 && s.efps > epf) {
         s.fail = 0.0;
@@ -448,12 +463,12 @@ MCM_HD int f3dm10(PState& s, const MatParams& m, double dt, double& defps_prev) 
               d[S23] * d[S23] + (double)1.e-12f;
         sj2 = d[S11] * d[S23] * d[S23] + d[S22] * d[S13] * d[S13] + d[S33] * d[S12] * d[S12] -
               d[S11] * d[S22] * d[S33] - 2.0 * d[S12] * d[S13] * d[S23];
-        double akt = -(sqrt(27.0 / aj2) * sj2 * 0.5 / aj2);
+        T akt = -(sqrt(27.0 / aj2) * sj2 * 0.5 / aj2);
         akt = copysign(dmin(fabs(akt), 1.0), akt);
-        double ywh = acos(akt) * othird;
+        T ywh = acos(akt) * othird;
         sj2 = 2.0 * sqrt(aj2 * othird) * cos(ywh);
-        double davg1 = s.p - sj2 - s.p_cut;
-        double sc2 = 0.5 * (1.0 + fsign1(davg1));
+        T davg1 = s.p - sj2 - s.p_cut;
+        T sc2 = 0.5 * (1.0 + fsign1(davg1));
         s.pcut = sc2 * s.p_cut;
         for (int k = 0; k < 6; k++) d[k] = sc2 * d[k];
         s.p_cut = s.pcut;
@@ -462,17 +477,19 @@ MCM_HD int f3dm10(PState& s, const MatParams& m, double dt, double& defps_prev) 
 }
 
 // mcm_nodalav.f90:32-47 (isotropic nodal viscosity; visc_type 1 only)
-MCM_HD void nodalav(PState& s, const MatParams& m) {
+template <class T, class M>
+MCM_HD void nodalav(PStateT<T>& s, const M& m) {
     s.q = 0.0;
     if (s.tracerod < 0.0) {
-        double ht = s.h * s.tracerod;
+        T ht = s.h * s.tracerod;
         s.q = m.av_l * s.rho * s.c * s.h * fabs(s.tracerod) + m.av_q * s.rho * (ht * ht);
     }
 }
 
 // mcm_lieupd.f90:26-33 (q:D over the column-major flattening; q is diagonal)
-MCM_HD void lieupd(PState& s, double dt) {
-    double qpm = 0.0;
+template <class T>
+MCM_HD void lieupd(PStateT<T>& s, T dt) {
+    T qpm = 0.0;
     qpm = qpm + s.q * s.rod[S11];
     qpm = qpm + s.q * s.rod[S22];
     qpm = qpm + s.q * s.rod[S33];
@@ -480,109 +497,111 @@ MCM_HD void lieupd(PState& s, double dt) {
 }
 
 // mcm_hieupd.f90:28-64 (trial energy)
-MCM_HD void hieupd(PState& s, double dt) {
+template <class T>
+MCM_HD void hieupd(PStateT<T>& s, T dt) {
     const double othird = 1.0 / 3.0;
-    double pold = -(othird * (s.sig[S11] + s.sig[S22] + s.sig[S33]));
-    double trace_q = othird * (s.q + s.q + s.q);
-    double volold = s.mass / s.rhoold;
-    double volnew = s.mass / s.rho;
+    T pold = -(othird * (s.sig[S11] + s.sig[S22] + s.sig[S33]));
+    T trace_q = othird * (s.q + s.q + s.q);
+    T volold = s.mass / s.rhoold;
+    T volnew = s.mass / s.rho;
     // column-major sum over (k,l): 11 21 31 12 22 32 13 23 33
-    double e11 = s.rod[S11] * (0.5 * (s.sig[S11] + pold + s.s[S11]) - s.q + trace_q);
-    double e22 = s.rod[S22] * (0.5 * (s.sig[S22] + pold + s.s[S22]) - s.q + trace_q);
-    double e33 = s.rod[S33] * (0.5 * (s.sig[S33] + pold + s.s[S33]) - s.q + trace_q);
-    double e12 = s.rod[S12] * (0.5 * (s.sig[S12] + s.s[S12]) - 0.0);
-    double e13 = s.rod[S13] * (0.5 * (s.sig[S13] + s.s[S13]) - 0.0);
-    double e23 = s.rod[S23] * (0.5 * (s.sig[S23] + s.s[S23]) - 0.0);
-    double vavg = 0.5 * (volnew + volold);
-    double eincr = 0.0;
+    T e11 = s.rod[S11] * (0.5 * (s.sig[S11] + pold + s.s[S11]) - s.q + trace_q);
+    T e22 = s.rod[S22] * (0.5 * (s.sig[S22] + pold + s.s[S22]) - s.q + trace_q);
+    T e33 = s.rod[S33] * (0.5 * (s.sig[S33] + pold + s.s[S33]) - s.q + trace_q);
+    T e12 = s.rod[S12] * (0.5 * (s.sig[S12] + s.s[S12]) - 0.0);
+    T e13 = s.rod[S13] * (0.5 * (s.sig[S13] + s.s[S13]) - 0.0);
+    T e23 = s.rod[S23] * (0.5 * (s.sig[S23] + s.s[S23]) - 0.0);
+    T vavg = 0.5 * (volnew + volold);
+    T eincr = 0.0;
     eincr = eincr + e11; eincr = eincr + e12; eincr = eincr + e13;
     eincr = eincr + e12; eincr = eincr + e22; eincr = eincr + e23;
     eincr = eincr + e13; eincr = eincr + e23; eincr = eincr + e33;
     eincr = vavg * eincr;
-    double dvol = volnew - volold;
-    double mean_incr = dvol * (0.5 * pold + trace_q);
+    T dvol = volnew - volold;
+    T mean_incr = dvol * (0.5 * pold + trace_q);
     s.etry = s.e + dt * eincr - mean_incr;
 }
 
 // Sound speed (mcm_sueos.f90:24-46 -> mcm_eos{1,4,13,28,41}c.f90); returns
 // nonzero when eos41 meets a negative c^2 (the reference stops there)
-MCM_HD int sueos(PState& s, const MatParams& m) {
-    const double* ei = m.eosinput;
-    double rho = s.rho, rho0 = s.rho0, mass = s.mass;
+template <class T, class M>
+MCM_HD int sueos(PStateT<T>& s, const M& m) {
+    const auto* ei = m.eosinput;
+    T rho = s.rho, rho0 = s.rho0, mass = s.mass;
     switch (m.eos) {
         case 1: {
-            double c1 = ei[1], c2 = ei[2], c3 = ei[3], c4 = ei[4], c5 = ei[5], c6 = ei[6];
-            double mu = rho / rho0 - 1.0;
-            double mu_t = dmax(mu, 0.0);
-            double e1 = 0.0;
-            double dmudrho = 1.0 / rho0;
-            double termone = c1 * dmudrho + (c2 * (dmudrho * dmudrho)) + (3.0 * c3 * dmudrho) * mu * mu;
-            double termtwo = (c5 * dmudrho + c6 * dmudrho * dmudrho) * e1;
-            double dpdrho = termone + termtwo;
-            double dpde = c4 + c5 * mu + c6 * (mu_t * mu_t);
-            double mr = mass / rho;
-            double c_sq = ((4.0 / 3.0) * m.g / rho0 + dpdrho + s.p / rho0 * (mr * mr) * dpde);
+            T c1 = ei[1], c2 = ei[2], c3 = ei[3], c4 = ei[4], c5 = ei[5], c6 = ei[6];
+            T mu = rho / rho0 - 1.0;
+            T mu_t = dmax(mu, 0.0);
+            T e1 = 0.0;
+            T dmudrho = 1.0 / rho0;
+            T termone = c1 * dmudrho + (c2 * (dmudrho * dmudrho)) + (3.0 * c3 * dmudrho) * mu * mu;
+            T termtwo = (c5 * dmudrho + c6 * dmudrho * dmudrho) * e1;
+            T dpdrho = termone + termtwo;
+            T dpde = c4 + c5 * mu + c6 * (mu_t * mu_t);
+            T mr = mass / rho;
+            T c_sq = ((4.0 / 3.0) * m.g / rho0 + dpdrho + s.p / rho0 * (mr * mr) * dpde);
             s.c = c_sq > 0.0 ? sqrt(c_sq) : sqrt(fabs(c_sq));
             return 0;
         }
         case 4: {
-            double c1 = ei[0], s1 = ei[1], s2 = ei[2], s3 = ei[3], gamma = ei[4], a1 = ei[5];
-            double e1 = s.e * rho0 / mass;
-            double mu = rho / rho0 - 1.0;
-            double dpdrho;
+            T c1 = ei[0], s1 = ei[1], s2 = ei[2], s3 = ei[3], gamma = ei[4], a1 = ei[5];
+            T e1 = s.e * rho0 / mass;
+            T mu = rho / rho0 - 1.0;
+            T dpdrho;
             if (mu > 0.0) {
-                double termone = mu * (1.0 - (gamma / 2.0));
-                double termtwo = (a1 / 2.0) * mu * mu;
-                double eosnum = mu * rho0 * (c1 * c1) * (1.0 + termone - termtwo);
-                double mp1 = mu + 1.0;
-                double mu11 = mu / mp1;
-                double mu21 = (mu * mu) / mp1;
-                double mu22 = (mu * mu) / (mp1 * mp1);
-                double mu32 = (mu * mu * mu) / (mp1 * mp1);
-                double mu33 = (mu * mu * mu) / (mp1 * mp1 * mp1);
-                double eosden = 1.0 - (s1 - 1.0) * mu - s2 * mu21 - s3 * mu32;
-                double termthree = rho0 * (c1 * c1) * (1.0 + 2.0 * termone - 3.0 * termtwo);
-                double termfour = 1.0 - s1 - 2.0 * s2 * mu11 + s2 * mu22 - 3.0 * s3 * mu22 + 2.0 * s3 * mu33;
-                double ed2 = eosden * eosden;
-                double ed3 = eosden * ed2;
+                T termone = mu * (1.0 - (gamma / 2.0));
+                T termtwo = (a1 / 2.0) * mu * mu;
+                T eosnum = mu * rho0 * (c1 * c1) * (1.0 + termone - termtwo);
+                T mp1 = mu + 1.0;
+                T mu11 = mu / mp1;
+                T mu21 = (mu * mu) / mp1;
+                T mu22 = (mu * mu) / (mp1 * mp1);
+                T mu32 = (mu * mu * mu) / (mp1 * mp1);
+                T mu33 = (mu * mu * mu) / (mp1 * mp1 * mp1);
+                T eosden = 1.0 - (s1 - 1.0) * mu - s2 * mu21 - s3 * mu32;
+                T termthree = rho0 * (c1 * c1) * (1.0 + 2.0 * termone - 3.0 * termtwo);
+                T termfour = 1.0 - s1 - 2.0 * s2 * mu11 + s2 * mu22 - 3.0 * s3 * mu22 + 2.0 * s3 * mu33;
+                T ed2 = eosden * eosden;
+                T ed3 = eosden * ed2;
                 dpdrho = (termthree / ed2 - 2.0 * eosnum * termfour / ed3) / rho0 + a1 * e1 / rho0;
             } else {
                 dpdrho = c1 * c1 + a1 * e1 / rho0;
             }
-            double dpde = gamma + a1 * mu;
-            double mr = mass / rho;
-            double c_sq = ((4.0 / 3.0) * m.g / rho0 + dpdrho + s.p / rho0 * (mr * mr) * dpde);
+            T dpde = gamma + a1 * mu;
+            T mr = mass / rho;
+            T c_sq = ((4.0 / 3.0) * m.g / rho0 + dpdrho + s.p / rho0 * (mr * mr) * dpde);
             s.c = c_sq > 0.0 ? sqrt(c_sq) : sqrt(fabs(c_sq));
             return 0;
         }
         case 13: {
-            double gamma = ei[0];
-            double e1 = s.e / mass;
+            T gamma = ei[0];
+            T e1 = s.e / mass;
             s.c = sqrt((gamma - 1.0) * gamma * e1);
             return 0;
         }
         case 28: s.c = sqrt(ei[0] * ei[1] / rho0); return 0;
         case 41: {
-            double cc = ei[0], sl = ei[1], gamma0 = ei[4];
-            double e1 = s.e / mass;
-            double ao = rho0 * (cc * cc);
-            double mu = rho / rho0 - 1.0;
-            double gamma = gamma0 * rho0 / rho;
-            double bo = ao * (1.0 + 2.0 * (sl - 1.0));
-            double slm = sl - 1.0;
-            double co = ao * (2.0 * (sl - 1.0) + 3.0 * (slm * slm));
-            double f = gamma * mu;
-            double a1 = ao * (1.0 - f);
-            double b1 = bo * (2.0 - 1.5 * f);
-            double c1 = co * (3.0 - 2.0 * f);
-            double dpdrho, c2;
+            T cc = ei[0], sl = ei[1], gamma0 = ei[4];
+            T e1 = s.e / mass;
+            T ao = rho0 * (cc * cc);
+            T mu = rho / rho0 - 1.0;
+            T gamma = gamma0 * rho0 / rho;
+            T bo = ao * (1.0 + 2.0 * (sl - 1.0));
+            T slm = sl - 1.0;
+            T co = ao * (2.0 * (sl - 1.0) + 3.0 * (slm * slm));
+            T f = gamma * mu;
+            T a1 = ao * (1.0 - f);
+            T b1 = bo * (2.0 - 1.5 * f);
+            T c1 = co * (3.0 - 2.0 * f);
+            T dpdrho, c2;
             if (mu > 0.0) dpdrho = (a1 + b1 * mu + c1 * (mu * mu)) / rho0 + gamma * e1;
             else dpdrho = ao * (1.0 - f) / rho0 + gamma * e1;
             if (s.p == s.pcut) {
                 c2 = cc * cc;
             } else {
-                double dpdi = gamma * rho;
-                double didrho = s.p / (rho * rho);
+                T dpdi = gamma * rho;
+                T didrho = s.p / (rho * rho);
                 dpdrho = dpdrho + dpdi * didrho;
                 c2 = dpdrho;
             }
@@ -594,19 +613,20 @@ MCM_HD int sueos(PState& s, const MatParams& m) {
 }
 
 // Pressure and energy (mcm_eqos.f90:24-55 -> mcm_eqos{1,4,13,28,41}.f90)
-MCM_HD void eqos(PState& s, const MatParams& m, double dt) {
-    const double* ei = m.eosinput;
-    double rho = s.rho, rho0 = s.rho0, mass = s.mass;
+template <class T, class M>
+MCM_HD void eqos(PStateT<T>& s, const M& m, T dt) {
+    const auto* ei = m.eosinput;
+    T rho = s.rho, rho0 = s.rho0, mass = s.mass;
     switch (m.eos) {
         case 1: {
-            double c0 = ei[0], c1 = ei[1], c2 = ei[2], c3 = ei[3], c4 = ei[4], c5 = ei[5], c6 = ei[6];
-            double mu = rho / rho0 - 1.0;
-            double mu_t = dmax(mu, 0.0);
-            double dvol = mass / rho - mass / s.rhoold;
-            double vol0 = mass / rho0;
-            double e1try = s.etry / vol0;
-            double termone = c0 + (c1 + c3 * (mu * mu)) * mu + c2 * (mu_t * mu_t);
-            double termtwo = c4 + c5 * mu + c6 * (mu_t * mu_t);
+            T c0 = ei[0], c1 = ei[1], c2 = ei[2], c3 = ei[3], c4 = ei[4], c5 = ei[5], c6 = ei[6];
+            T mu = rho / rho0 - 1.0;
+            T mu_t = dmax(mu, 0.0);
+            T dvol = mass / rho - mass / s.rhoold;
+            T vol0 = mass / rho0;
+            T e1try = s.etry / vol0;
+            T termone = c0 + (c1 + c3 * (mu * mu)) * mu + c2 * (mu_t * mu_t);
+            T termtwo = c4 + c5 * mu + c6 * (mu_t * mu_t);
             s.p = (termone + termtwo * e1try) / (1.0 + termtwo * (dvol / vol0));
             if (s.p < s.pcut) {
                 s.p = s.pcut;
@@ -617,19 +637,19 @@ MCM_HD void eqos(PState& s, const MatParams& m, double dt) {
             break;
         }
         case 4: {
-            double c1 = ei[0], s1 = ei[1], s2 = ei[2], s3 = ei[3], gamma = ei[4], a1 = ei[5];
-            double mu = rho / rho0 - 1.0;
-            double dvol = mass / rho - mass / s.rhoold;
-            double vol0 = mass / rho0;
-            double e1try = s.etry / vol0;
+            T c1 = ei[0], s1 = ei[1], s2 = ei[2], s3 = ei[3], gamma = ei[4], a1 = ei[5];
+            T mu = rho / rho0 - 1.0;
+            T dvol = mass / rho - mass / s.rhoold;
+            T vol0 = mass / rho0;
+            T e1try = s.etry / vol0;
             if (mu > 0.0) {
-                double termone = mu * (1.0 - (gamma / 2.0));
-                double termtwo = (a1 / 2.0) * mu * mu;
-                double eosnum = mu * rho0 * (c1 * c1) * (1.0 + termone - termtwo);
-                double mp1 = mu + 1.0;
-                double mu21 = (mu * mu) / mp1;
-                double mu32 = (mu * mu * mu) / (mp1 * mp1);
-                double eosden = 1.0 - (s1 - 1.0) * mu - s2 * mu21 - s3 * mu32;
+                T termone = mu * (1.0 - (gamma / 2.0));
+                T termtwo = (a1 / 2.0) * mu * mu;
+                T eosnum = mu * rho0 * (c1 * c1) * (1.0 + termone - termtwo);
+                T mp1 = mu + 1.0;
+                T mu21 = (mu * mu) / mp1;
+                T mu32 = (mu * mu * mu) / (mp1 * mp1);
+                T eosden = 1.0 - (s1 - 1.0) * mu - s2 * mu21 - s3 * mu32;
                 s.p = (eosnum / (eosden * eosden) + (gamma + a1 * mu) * e1try) /
                       (1.0 + 0.5 * (gamma + a1 * mu) * dvol / vol0);
             } else {
@@ -645,36 +665,36 @@ MCM_HD void eqos(PState& s, const MatParams& m, double dt) {
             break;
         }
         case 13: {
-            double gamma_m1 = ei[0] - 1.0;
-            double e1try = s.etry / mass;
-            double dvol = mass / rho - mass / s.rhoold;
+            T gamma_m1 = ei[0] - 1.0;
+            T e1try = s.etry / mass;
+            T dvol = mass / rho - mass / s.rhoold;
             s.p = (rho * gamma_m1 * e1try) / (1.0 + 0.5 * rho * gamma_m1 * dvol / mass);
             s.e = s.etry - 0.5 * dvol * s.p;
             break;
         }
         case 28: {
-            double b = ei[0], gamma = ei[1];
-            double dvol = mass / rho - mass / s.rhoold;
+            T b = ei[0], gamma = ei[1];
+            T dvol = mass / rho - mass / s.rhoold;
             s.p = b * (pow(rho / rho0, gamma) - 1.0);
             s.e = s.etry - 0.5 * dvol * s.p;
             break;
         }
         case 41: {
-            double cc = ei[0], sl = ei[1], gama0 = ei[4];
-            double etry1 = s.etry / mass;
-            double ao = rho0 * (cc * cc);
-            double mu = rho / rho0 - 1.0;
-            double dvol = mass / rho - mass / s.rhoold;
-            double vol0 = mass / rho0;
-            double gama = gama0 * rho0 / rho;
-            double slm = sl - 1.0;
-            double bo = ao * (1.0 + 2.0 * (sl - 1.0));
-            double co = ao * (2.0 * (sl - 1.0) + 3.0 * (slm * slm));
-            double f = gama * mu;
-            double ph;
+            T cc = ei[0], sl = ei[1], gama0 = ei[4];
+            T etry1 = s.etry / mass;
+            T ao = rho0 * (cc * cc);
+            T mu = rho / rho0 - 1.0;
+            T dvol = mass / rho - mass / s.rhoold;
+            T vol0 = mass / rho0;
+            T gama = gama0 * rho0 / rho;
+            T slm = sl - 1.0;
+            T bo = ao * (1.0 + 2.0 * (sl - 1.0));
+            T co = ao * (2.0 * (sl - 1.0) + 3.0 * (slm * slm));
+            T f = gama * mu;
+            T ph;
             if (mu > 0.0) ph = ao * mu + bo * (mu * mu) + co * (mu * (mu * mu));
             else ph = ao * mu;
-            double pn = (ph * (1.0 - 0.5 * f) + rho * gama * etry1) / (1.0 + 0.5 * rho * gama * dvol / vol0);
+            T pn = (ph * (1.0 - 0.5 * f) + rho * gama * etry1) / (1.0 + 0.5 * rho * gama * dvol / vol0);
             if (pn < s.pcut) {
                 pn = s.pcut;
                 s.rho = s.rho / (1.0 - s.tracerod * dt);
@@ -689,7 +709,8 @@ MCM_HD void eqos(PState& s, const MatParams& m, double dt) {
 }
 
 // mcm_stress_up.f90:28-36
-MCM_HD void stress_up(PState& s) {
+template <class T>
+MCM_HD void stress_up(PStateT<T>& s) {
     s.sig[S11] = s.s[S11] - s.p;
     s.sig[S22] = s.s[S22] - s.p;
     s.sig[S33] = s.s[S33] - s.p;
@@ -699,12 +720,13 @@ MCM_HD void stress_up(PState& s) {
 }
 
 // per-particle critical timestep (mcm_calccritts.f90:40-78)
-MCM_HD double critts(const PState& s, const MatParams& m, int tcrit_opt, double mindist) {
+template <class T, class M>
+MCM_HD T critts(const PStateT<T>& s, const M& m, int tcrit_opt, double mindist) {
     if (tcrit_opt == 0) {
-        double mr = s.mass / s.rho;
-        double veae = pow(mr, 1.0 / 3.0);
-        double ae = pow(mr, 2.0 / 3.0);
-        double qu = (m.av_l * s.c + m.av_q * s.mass / s.rho * s.tracerod) / ae;
+        T mr = s.mass / s.rho;
+        T veae = pow(mr, 1.0 / 3.0);
+        T ae = pow(mr, 2.0 / 3.0);
+        T qu = (m.av_l * s.c + m.av_q * s.mass / s.rho * s.tracerod) / ae;
         return veae / (qu + sqrt(qu * qu + s.c * s.c));
     }
     if (tcrit_opt == 1) return s.h / (s.c + s.vabs);
@@ -739,28 +761,29 @@ MCM_HD int constitutive_one(PState& s, const MatParams& m, int matid1, double dt
 }
 
 // eos{1,4,13,28,41}p.f90 (initial pressure), used by the initialisation only
-MCM_HD void eos_p(PState& s, const MatParams& m) {
-    const double* ei = m.eosinput;
-    double rho = s.rho, rho0 = s.rho0;
+template <class T, class M>
+MCM_HD void eos_p(PStateT<T>& s, const M& m) {
+    const auto* ei = m.eosinput;
+    T rho = s.rho, rho0 = s.rho0;
     switch (m.eos) {
         case 1: {
-            double mu = rho / rho0 - 1.0, mu_t = dmax(mu, 0.0), e1 = 0.0;
+            T mu = rho / rho0 - 1.0, mu_t = dmax(mu, 0.0), e1 = 0.0;
             s.p = ei[0] + (ei[1] + ei[3] * (mu * mu)) * mu + ei[2] * (mu_t * mu_t) +
                   (ei[4] + ei[5] * mu + ei[6] * (mu_t * mu_t)) * e1;
             break;
         }
         case 4: {
-            double c1 = ei[0], s1 = ei[1], s2 = ei[2], s3 = ei[3], gamma = ei[4], a1 = ei[5];
-            double e1 = s.e * rho0 / s.mass;
-            double mu = rho / rho0 - 1.0;
+            T c1 = ei[0], s1 = ei[1], s2 = ei[2], s3 = ei[3], gamma = ei[4], a1 = ei[5];
+            T e1 = s.e * rho0 / s.mass;
+            T mu = rho / rho0 - 1.0;
             if (mu > 0.0) {
-                double termone = mu * (1.0 - (gamma / 2.0));
-                double termtwo = (a1 / 2.0) * mu * mu;
-                double eosnum = mu * rho0 * (c1 * c1) * (1.0 + termone - termtwo);
-                double mp1 = mu + 1.0;
-                double mu21 = (mu * mu) / mp1;
-                double mu32 = (mu * mu * mu) / (mp1 * mp1);
-                double eosden = 1.0 - (s1 - 1.0) * mu - s2 * mu21 - s3 * mu32;
+                T termone = mu * (1.0 - (gamma / 2.0));
+                T termtwo = (a1 / 2.0) * mu * mu;
+                T eosnum = mu * rho0 * (c1 * c1) * (1.0 + termone - termtwo);
+                T mp1 = mu + 1.0;
+                T mu21 = (mu * mu) / mp1;
+                T mu32 = (mu * mu * mu) / (mp1 * mp1);
+                T eosden = 1.0 - (s1 - 1.0) * mu - s2 * mu21 - s3 * mu32;
                 s.p = eosnum / (eosden * eosden) + (gamma + a1 * mu) * e1;
             } else {
                 s.p = rho0 * (c1 * c1) * mu + (gamma + a1 * mu) * e1;
@@ -770,18 +793,18 @@ MCM_HD void eos_p(PState& s, const MatParams& m) {
         case 13: s.p = rho * (ei[0] - 1.0) * (s.e / s.mass); break;
         case 28: s.p = ei[0] * (pow(rho / rho0, ei[1]) - 1.0); break;
         case 41: {
-            double cc = ei[0], sl = ei[1], gamma = ei[4];
-            double e1 = s.e / s.mass;
-            double ao = rho0 * (cc * cc);
-            double mu = rho / rho0 - 1.0;
-            double slm = sl - 1.0;
-            double bo = ao * (1.0 + 2.0 * (sl - 1.0));
-            double co = ao * (2.0 * (sl - 1.0) + 3.0 * (slm * slm));
+            T cc = ei[0], sl = ei[1], gamma = ei[4];
+            T e1 = s.e / s.mass;
+            T ao = rho0 * (cc * cc);
+            T mu = rho / rho0 - 1.0;
+            T slm = sl - 1.0;
+            T bo = ao * (1.0 + 2.0 * (sl - 1.0));
+            T co = ao * (2.0 * (sl - 1.0) + 3.0 * (slm * slm));
             gamma = gamma * rho0 / rho;
-            double f = gamma * mu, ph;
+            T f = gamma * mu, ph;
             if (mu > 0.0) ph = ao * mu + bo * (mu * mu) + co * (mu * (mu * mu));
             else ph = ao * mu;
-            double pn = ph * (1.0 - 0.5 * f) + rho * gamma * e1;
+            T pn = ph * (1.0 - 0.5 * f) + rho * gamma * e1;
             if (pn < s.pcut) pn = s.pcut;
             s.p = pn;
             break;

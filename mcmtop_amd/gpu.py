@@ -35,7 +35,11 @@ SYMBOLS = (
     "mcm_gpu_batch_run", "mcm_gpu_batch_last_error", "mcm_gpu_batch_initial", "mcm_gpu_batch_step",
     "mcm_gpu_batch_time_steps", "mcm_gpu_batch_destroy", "mcm_gpu_output_fields", "mcm_vtk_write",
     "mcm_gpu_write_vtk", "mcm_gpu_rep_cont", "mcm_gpu_batch_tune",
+    "mcm_gpu_set_material_params", "mcm_gpu_get_material_params", "mcm_gpu_set_initial_velocity",
+    "mcm_gpu_adj_forward", "mcm_gpu_adj_backward", "mcm_gpu_adj_last_error", "mcm_gpu_adj_destroy",
 )
+
+N_PARAMS = 4   # per material: strinput(1:3) (f3dm10 G, sigma_y, Eh) and eosinput(2) (eos4 S1)
 
 OUT_WORDS = 17   # packed output record: x(3), v(3), p, sigma 11 22 33 12 13 23, e, efps, rho, h
 
@@ -110,8 +114,20 @@ def load_library(path: str = LIB_PATH):
     lib.mcm_gpu_batch_tune.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
     lib.mcm_gpu_batch_destroy.argtypes = [vp]
     lib.mcm_gpu_batch_destroy.restype = None
+    dp = ctypes.POINTER(ctypes.c_double)
+    lib.mcm_gpu_set_material_params.argtypes = [vp, ctypes.c_int32, dp]
+    lib.mcm_gpu_get_material_params.argtypes = [vp, ctypes.c_int32, dp]
+    lib.mcm_gpu_set_initial_velocity.argtypes = [vp, dp, ctypes.c_int64]
+    lib.mcm_gpu_adj_forward.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, i32p, ctypes.c_int32, dp,
+                                        ctypes.POINTER(vp)]
+    lib.mcm_gpu_adj_backward.argtypes = [vp, dp, dp, dp, dp, dp]
+    lib.mcm_gpu_adj_last_error.argtypes = [vp]
+    lib.mcm_gpu_adj_last_error.restype = ctypes.c_char_p
+    lib.mcm_gpu_adj_destroy.argtypes = [vp]
+    lib.mcm_gpu_adj_destroy.restype = None
     special = ("mcm_gpu_destroy", "mcm_gpu_last_error", "mcm_gpu_batch_size", "mcm_gpu_batch_run",
-               "mcm_gpu_batch_last_error", "mcm_gpu_batch_destroy")
+               "mcm_gpu_batch_last_error", "mcm_gpu_batch_destroy", "mcm_gpu_adj_last_error",
+               "mcm_gpu_adj_destroy")
     for name in SYMBOLS:
         if name not in special:
             getattr(lib, name).restype = ctypes.c_int
@@ -288,6 +304,95 @@ class Solver:
         ms = (ctypes.c_double * 5)()
         self._check(self._lib.mcm_gpu_time_steps(self._h, int(n), ms), "time_steps")
         return np.array(ms[:])
+
+    # ---- differentiation (plain SPH, material model 10)
+    def set_material_params(self, mat: int, params) -> None:
+        """strinput(1:3) and eosinput(2) of material mat (0-based): f3dm10 G,
+        sigma_y, Eh and eos4 S1; takes effect at the next initial()."""
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        assert p.size == N_PARAMS
+        self._check(self._lib.mcm_gpu_set_material_params(self._h, int(mat), p.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_double))), "set_material_params")
+
+    def material_params(self, mat: int = 0) -> np.ndarray:
+        p = np.zeros(N_PARAMS)
+        self._check(self._lib.mcm_gpu_get_material_params(self._h, int(mat), p.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_double))), "get_material_params")
+        return p
+
+    def set_initial_velocity(self, v0) -> None:
+        """The deck's initial velocities, (np, 3); takes effect at the next initial()."""
+        v = np.ascontiguousarray(v0, dtype=np.float64).reshape(-1)
+        self._check(self._lib.mcm_gpu_set_initial_velocity(self._h, v.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_double)), v.size), "set_initial_velocity")
+
+    def record(self, nsteps: int, ckpt_every: int = 50, sample=()) -> "Recording":
+        """nsteps fused steps from the current state (normally right after
+        initial()), with a state checkpoint every ckpt_every steps and the
+        positions of the sampled particles after every step, for a later
+        reverse sweep (Recording.backward)."""
+        smp = np.ascontiguousarray(sample, dtype=np.int32)
+        hist = np.zeros((int(nsteps), len(smp), 3))
+        h = ctypes.c_void_p()
+        i32p = ctypes.POINTER(ctypes.c_int32)
+        rc = self._lib.mcm_gpu_adj_forward(self._h, int(nsteps), int(ckpt_every), smp.ctypes.data_as(i32p),
+                                           len(smp), hist.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                           ctypes.byref(h))
+        rec = Recording(self, h, int(nsteps), len(smp), hist)
+        if rc != 0:
+            msg = self._lib.mcm_gpu_adj_last_error(h).decode() if h else \
+                self._lib.mcm_gpu_last_error(self._h).decode()
+            rec.close()
+            raise McmGpuError("record failed: %s (%s)" % (ERRORS.get(rc, rc), msg))
+        return rec
+
+
+class Recording:
+    """A recorded forward run (mcm_gpu_adj_forward): the sampled positions
+    after every step in ``hist`` (nsteps, nsample, 3) and a reverse sweep."""
+
+    def __init__(self, solver: Solver, handle, nsteps: int, nsample: int, hist: np.ndarray):
+        self._s = solver
+        self._lib = solver._lib
+        self._h = handle
+        self.nsteps = nsteps
+        self.nsample = nsample
+        self.hist = hist
+
+    def backward(self, xbar=None, vbar=None, histbar=None):
+        """Adjoint of the recorded run: dL/dx and dL/dv of the final state
+        (np, 3) and dL/dhist (nsteps, nsample, 3), any of them None for zero.
+        Returns (dL/dv0 (np, 3) of the deck's initial velocities,
+        dL/dparams (nummat, 4))."""
+        n = self._s.scalars().np
+        nm = self._s.scalars().nummat
+        dp = ctypes.POINTER(ctypes.c_double)
+
+        def arr(a, shape):
+            if a is None:
+                return None, None
+            b = np.ascontiguousarray(a, dtype=np.float64)
+            assert b.shape == shape, (b.shape, shape)
+            return b, b.ctypes.data_as(dp)
+
+        xb, xp = arr(xbar, (n, 3))
+        vb, vp = arr(vbar, (n, 3))
+        hb, hp = arr(histbar, (self.nsteps, self.nsample, 3))
+        v0 = np.zeros((n, 3))
+        pb = np.zeros((nm, N_PARAMS))
+        rc = self._lib.mcm_gpu_adj_backward(self._h, xp, vp, hp, v0.ctypes.data_as(dp), pb.ctypes.data_as(dp))
+        if rc != 0:
+            raise McmGpuError("backward failed: %s (%s)" % (ERRORS.get(rc, rc),
+                                                            self._lib.mcm_gpu_adj_last_error(self._h).decode()))
+        return v0, pb
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.mcm_gpu_adj_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
 
 
 class Batch:

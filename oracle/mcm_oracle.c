@@ -1771,3 +1771,7 @@ int orc_dump(orc_state *st, const char *path, int step, int stage) {
     fclose(f);
     return ORC_OK;
 }
+
+void orc_get_x(const orc_state *st, double *out) {
+    for (long k = 0; k < 3L * st->np; k++) out[k] = st->x[k];
+}

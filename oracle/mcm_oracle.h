@@ -108,6 +108,10 @@ int orc_step(orc_state *st);
 int orc_run(orc_state *st, int nsteps);
 int orc_dump(orc_state *st, const char *path, int step, int stage);
 
+/* positions (3 per particle) of the current state, for finite-difference
+   checks of the GPU adjoint */
+void orc_get_x(const orc_state *st, double *out);
+
 /* closed-form pieces exposed for unit tests */
 double orc_kernel_w(const double *xi, const double *xj, double havg, double pi);
 void orc_kernel_gradw(const double *xi, const double *xj, double havg, double pi, double *dwdx);

@@ -44,6 +44,8 @@ def _lib():
     lib.orc_kernel_w.restype = ctypes.c_double
     lib.orc_kernel_gradw.argtypes = [dbl3, dbl3, ctypes.c_double, ctypes.c_double, dbl3]
     lib.orc_inverse3.argtypes = [dbl3, dbl3]
+    lib.orc_get_x.argtypes = [ctypes.c_void_p, dbl3]
+    lib.orc_get_x.restype = None
     return lib
 
 
@@ -92,6 +94,13 @@ class Oracle:
     def run(self, n: int) -> int:
         self.step_no += n
         return self._l.orc_run(self.h, n)
+
+    def positions(self, np_: int):
+        """(np, 3) positions of the current state."""
+        import numpy
+        out = numpy.zeros((np_, 3))
+        self._l.orc_get_x(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return out
 
     def dump(self, path: str, step: int = 0, stage: int = 0) -> None:
         rc = self._l.orc_dump(self.h, path.encode(), step, stage)

@@ -1,0 +1,175 @@
+"""Reverse-mode derivative of the plain-SPH step (mcm_gpu_adj_*), checked
+against central finite differences of the CPU oracle, which restates the
+reference step (tests/test_oracle_golden.py pins it to the reference's own
+state dumps).
+
+Case: an f3dm10 + eqos4 Taylor bar (the differentiation workload's material
+and EOS, 800 particles at dx = 0.8 mm), 30 steps, loss = identification loss
+of mcmtop_amd.adjoint (final length, mushroom radius, 8 sampled surface
+displacement histories) against a target from hidden parameters.  The
+gradient is taken at parameters away from the hidden ones, with respect to
+the four model constants (G, sigma_y, Eh, S1) and to single initial-velocity
+components.
+
+The initial velocities carry a seeded 0.5 % jitter.  With a uniform start
+every particle has the same critical step, dt = tssf min(critts) is then not
+differentiable with respect to one particle's velocity (a central difference
+sees half of a one-sided slope) and only the parameter gradients, which move
+all tied particles alike, would be comparable.  Eh barely changes the loss in
+30 steps: its central difference is taken with a relative step of 1e-5 (at
+1e-6 rounding of the loss difference dominates) and held to 1e-4.
+"""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from mcmtop_amd import adjoint, deck, gpu
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+NSTEPS = 30
+DX = 0.8e-3
+RADIUS = 3.2e-3
+NU = 0.35
+FD_REL = (1.0e-6, 1.0e-6, 1.0e-5, 1.0e-6)   # relative steps of the central differences (G, sigma_y, Eh, S1)
+TOL = (1.0e-5, 1.0e-5, 1.0e-4, 1.0e-5)      # gradient agreement (relative)
+FD_V = 1.0e-3                               # m/s, step of the initial-velocity differences
+
+
+def _bar():
+    dk = deck.taylor_bar(dx=DX, radius=RADIUS, length=12.8e-3, model="f3dm10", tcrit_opt=1, seed=5)
+    dk.v = dk.v * (1.0 + np.random.default_rng(7).uniform(-0.005, 0.005, size=dk.v.shape))
+    return dk
+
+
+def _model_params(dk):
+    m = dk.materials[0]
+    return np.array([m.card[0], m.card[1], m.card[2], m.eoscard[1]])
+
+
+def _with(dk, params=None, v0=None):
+    import copy
+    d2 = copy.deepcopy(dk)
+    if params is not None:
+        d2.materials[0].card[0:3] = params[0:3]
+        d2.materials[0].eoscard[1] = params[3]
+    if v0 is not None:
+        d2.v = np.array(v0, dtype=np.float64).reshape(-1, 3)
+    return d2
+
+
+def _loss_np(x, hist, x0s, target):
+    length = x[:, 2].max() - x[:, 2].min()
+    radius = np.sqrt(x[:, 0] ** 2 + x[:, 1] ** 2).max()
+    disp = hist - x0s[None]
+    return (((length - target["length"]) / target["length"]) ** 2 + ((radius - target["radius"]) / target["radius"]) ** 2
+            + ((disp - target["disp"]) ** 2).sum() / (target["disp"] ** 2).sum())
+
+
+def _oracle_loss(dk, sample, target, tmp_path):
+    p = str(tmp_path / "fd.bin")
+    dk.write_bin(p)
+    o = Oracle(p)
+    assert o.initial() == 0
+    n = dk.np_
+    hist = np.zeros((NSTEPS, len(sample), 3))
+    for k in range(NSTEPS):
+        assert o.step() == 0
+        hist[k] = o.positions(n)[sample]
+    x = o.positions(n)
+    o.close()
+    return _loss_np(x, hist, dk.x[sample], target)
+
+
+@pytest.fixture(scope="module")
+def problem():
+    dk = _bar()
+    sample = adjoint.surface_sample(dk.x, RADIUS, DX, 8, seed=2)
+    hidden = _model_params(dk)
+    # evaluation point away from the hidden parameters
+    p_eval = hidden * np.array([1.05, 0.9, 1.2, 1.03])
+    with gpu.Solver(dk) as s:
+        s.set_material_params(0, hidden)
+        s.initial()
+        rec = s.record(NSTEPS, NSTEPS, sample)
+        x = s.field(gpu.F_X)
+        hist = rec.hist.copy()
+        rec.close()
+    target = {"length": x[:, 2].max() - x[:, 2].min(), "radius": np.sqrt(x[:, 0] ** 2 + x[:, 1] ** 2).max(),
+              "disp": hist - dk.x[sample][None]}
+    return dk, sample, p_eval, target
+
+
+def _gpu_grad(dk, sample, params, target, ckpt):
+    tt = {k: torch.as_tensor(v) for k, v in target.items()}
+    with gpu.Solver(dk) as s:
+        p = torch.tensor(params.reshape(1, 4), dtype=torch.float64, requires_grad=True)
+        v = torch.tensor(dk.v, dtype=torch.float64, requires_grad=True)
+        x, hist = adjoint.StepRun.apply(p, v, s, NSTEPS, ckpt, sample)
+        L = adjoint.identification_loss(x, hist, torch.from_numpy(dk.x[sample]), tt)
+        L.backward()
+        return L.item(), p.grad.numpy()[0].copy(), v.grad.numpy().copy()
+
+
+def test_parameter_gradients_match_oracle_differences(problem, tmp_path):
+    dk, sample, p_eval, target = problem
+    L, gp, gv = _gpu_grad(dk, sample, p_eval, target, 7)
+    print("loss %.12e, nonfinite v0 gradients %d" % (L, int(np.sum(~np.isfinite(gv)))))
+    L_orc = _oracle_loss(_with(dk, p_eval), sample, target, tmp_path)
+    assert abs(L - L_orc) <= 1e-9 * abs(L_orc)
+    fds = []
+    for k in range(4):
+        h = FD_REL[k] * abs(p_eval[k])
+        pp, pm = p_eval.copy(), p_eval.copy()
+        pp[k] += h
+        pm[k] -= h
+        fd = (_oracle_loss(_with(dk, pp), sample, target, tmp_path) -
+              _oracle_loss(_with(dk, pm), sample, target, tmp_path)) / (pp[k] - pm[k])
+        print("param %d: adjoint %.12e  fd %.12e  rel %.2e" % (k, gp[k], fd, abs(gp[k] - fd) / abs(fd)))
+        fds.append(fd)
+    for k in range(4):
+        assert abs(gp[k] - fds[k]) <= TOL[k] * abs(fds[k]), (k, gp[k], fds[k])
+
+
+def test_initial_velocity_gradients_match_oracle_differences(problem, tmp_path):
+    dk, sample, p_eval, target = problem
+    _, _, gv = _gpu_grad(dk, sample, p_eval, target, 7)
+    # the largest components, and two sampled particles' z velocity
+    assert np.all(np.isfinite(gv))
+    flat = np.argsort(-np.abs(gv).ravel())[:3].tolist() + [3 * int(sample[0]) + 2, 3 * int(sample[3]) + 0]
+    scale = np.abs(gv).max()
+    res = []
+    for f in flat:
+        i, c = divmod(f, 3)
+        h = FD_V
+        vp, vm = dk.v.copy(), dk.v.copy()
+        vp[i, c] += h
+        vm[i, c] -= h
+        p_dk = _with(dk, p_eval)
+        fd = (_oracle_loss(_with(p_dk, v0=vp), sample, target, tmp_path) -
+              _oracle_loss(_with(p_dk, v0=vm), sample, target, tmp_path)) / (vp[i, c] - vm[i, c])
+        print("v0[%d,%d]: adjoint %.12e  fd %.12e" % (i, c, gv[i, c], fd))
+        res.append((i, c, gv[i, c], fd))
+    for i, c, g, fd in res:
+        assert abs(g - fd) <= 1.0e-5 * max(abs(fd), 1e-3 * scale), (i, c, g, fd)
+
+
+def test_checkpoint_interval_does_not_change_the_gradient(problem):
+    dk, sample, p_eval, target = problem
+    ref = _gpu_grad(dk, sample, p_eval, target, 1)
+    for ck in (7, NSTEPS):
+        got = _gpu_grad(dk, sample, p_eval, target, ck)
+        assert got[0] == ref[0]
+        np.testing.assert_array_equal(got[1], ref[1])
+        np.testing.assert_array_equal(got[2], ref[2])
+
+
+def test_unsupported_decks_are_rejected():
+    dk = deck.taylor_bar(dx=DX, radius=RADIUS, length=12.8e-3, model="f3dm10", tcrit_opt=1, corrected=True)
+    with gpu.Solver(dk) as s:
+        s.initial()
+        with pytest.raises(gpu.McmGpuError, match="plain SPH"):
+            s.record(4, 2, [0])
