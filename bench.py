@@ -172,6 +172,83 @@ def batched_leg(nruns: int, seed: int, steps: int, warmup: int, check_every: int
                         "v0 U[100,250] m/s, sigma_y U[200,600] MPa, E U[70,210] GPa" % nruns}
 
 
+def adjoint_leg(steps: int, ckpt: int, dx: float, nsample: int, fd: bool):
+    """Differentiation workload: gradients of the Taylor-bar identification
+    loss (final length, mushroom radius, sampled surface displacement
+    histories) with respect to (E, sigma_y, E_tan, S1) and the initial
+    velocities, by the device reverse sweep; times of the plain forward, the
+    recorded forward and the reverse sweep; central differences of the GPU
+    forward for the four parameters and two velocity components."""
+    import numpy as np
+    import torch
+    from mcmtop_amd import adjoint, deck, gpu
+    radius, nu = 3.2e-3, 0.35
+    dk = deck.taylor_bar(dx=dx, radius=radius, length=25.4e-3, model="f3dm10", tcrit_opt=1, seed=1,
+                         fit_text=False)
+    # seeded 0.5 % velocity jitter: with a uniform start all critical steps tie
+    # and dt = tssf min(critts) has no derivative per particle velocity
+    dk.v = dk.v * (1.0 + np.random.default_rng(7).uniform(-0.005, 0.005, size=dk.v.shape))
+    sample = adjoint.surface_sample(dk.x, radius, dx, nsample, seed=2)
+    hidden = np.array([117e9, 400e6, 100e6, 1.49])
+    theta = hidden * np.array([1.05, 0.9, 1.2, 1.03])
+    ident = adjoint.Identification(dk, steps, ckpt, sample, nu)
+    ident.make_target(hidden)
+    s = ident.solver
+    # plain forward (the fused step, no recording)
+    s.set_material_params(0, adjoint.physical_to_model(torch.tensor(theta), nu).numpy())
+    s.set_initial_velocity(dk.v)
+    s.initial()
+    s.step(2)
+    s.initial()
+    t0 = time.perf_counter()
+    s.step(steps)
+    t_fwd = time.perf_counter() - t0
+    # recorded forward + reverse sweep, timed separately
+    th = torch.tensor(theta, dtype=torch.float64, requires_grad=True)
+    v0 = torch.from_numpy(dk.v.copy()).requires_grad_(True)
+    t0 = time.perf_counter()
+    x, hist = ident.run(th, v0)
+    t_rec = time.perf_counter() - t0
+    L = adjoint.identification_loss(x, hist, ident.x0s, ident.target)
+    t0 = time.perf_counter()
+    L.backward()
+    t_bwd = time.perf_counter() - t0
+    g_theta = th.grad.numpy().copy()
+    g_v = v0.grad.numpy().copy()
+    print("bench: adjoint %d particles, forward %.3f s, recorded %.3f s, reverse %.3f s"
+          % (dk.np_, t_fwd, t_rec, t_bwd), file=sys.stderr, flush=True)
+    out = {"workload": "taylor_bar_plain_f3dm10_eqos4_tcrit1", "particles": dk.np_, "dx_mm": dx * 1e3,
+           "steps": steps, "ckpt_every": ckpt, "sampled": len(sample), "loss": L.item(),
+           "forward_s": t_fwd, "recorded_forward_s": t_rec, "reverse_s": t_bwd,
+           "ratio_reverse": t_bwd / t_fwd, "ratio_recorded_plus_reverse": (t_rec + t_bwd) / t_fwd,
+           "grad_theta": {"E": g_theta[0], "sigma_y": g_theta[1], "E_tan": g_theta[2], "S1": g_theta[3]},
+           "grad_v0_norm": float(np.linalg.norm(g_v))}
+    if fd:
+        def loss_at(th_, v_):
+            with torch.no_grad():
+                xx, hh = ident.run(torch.tensor(th_), torch.from_numpy(v_))
+                return adjoint.identification_loss(xx, hh, ident.x0s, ident.target).item()
+        rel = {}
+        for k, name in enumerate(("E", "sigma_y", "E_tan", "S1")):
+            h = (1e-5 if name == "E_tan" else 1e-6) * theta[k]
+            tp, tm = theta.copy(), theta.copy()
+            tp[k] += h
+            tm[k] -= h
+            f = (loss_at(tp, dk.v) - loss_at(tm, dk.v)) / (tp[k] - tm[k])
+            rel[name] = abs(g_theta[k] - f) / abs(f)
+        for f_ in np.argsort(-np.abs(g_v).ravel())[:2]:
+            i, c = divmod(int(f_), 3)
+            vp, vm = dk.v.copy(), dk.v.copy()
+            vp[i, c] += 1e-3
+            vm[i, c] -= 1e-3
+            f = (loss_at(theta, vp) - loss_at(theta, vm)) / 2e-3
+            rel["v0[%d,%d]" % (i, c)] = abs(g_v[i, c] - f) / abs(f)
+        out["fd_rel_error"] = rel
+        out["fd"] = "central differences of the GPU forward; relative step 1e-6 (E_tan 1e-5), velocity step 1e-3 m/s"
+    ident.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -184,6 +261,10 @@ def main():
     ap.add_argument("--batch-seed", type=int, default=3)
     ap.add_argument("--batch-plain", type=int, default=1,
                     help="1: also step the batch's bars one at a time (plain_value, speedup, equality check)")
+    ap.add_argument("--adjoint-steps", type=int, default=800, help="steps of the differentiation leg (0: skip it)")
+    ap.add_argument("--adjoint-ckpt", type=int, default=50)
+    ap.add_argument("--adjoint-dx", type=float, default=2.54e-4)
+    ap.add_argument("--adjoint-fd", type=int, default=1, help="1: check the gradients by central differences")
     ap.add_argument("--corrected-one-pass", action="store_true",
                     help="accumulate the corrected kernels' three passes in one (rounding differs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -266,6 +347,9 @@ def main():
     if args.batch > 0:
         res["batched"] = batched_leg(args.batch, args.batch_seed, args.batch_steps, 4, 16, bool(args.batch_plain))
     res["adjoint_forward_ratio"] = None
+    if args.adjoint_steps > 0:
+        res["adjoint"] = adjoint_leg(args.adjoint_steps, args.adjoint_ckpt, args.adjoint_dx, 32, bool(args.adjoint_fd))
+        res["adjoint_forward_ratio"] = res["adjoint"]["ratio_reverse"]
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(tmpdir, args.dx)
     print(json.dumps(res), flush=True)

@@ -2,15 +2,15 @@
 //
 // A forward run records a state snapshot every k steps.  The reverse sweep
 // walks the segments backwards.  Each segment is recomputed from its
-// checkpoint with the fused step, keeping the start-of-step state of every
-// step.  Then, for each step from last to first:
-//   1. the step is recomputed once more through the granular stage kernels,
-//      which leave every intermediate the reverse needs on the device (lists,
-//      ghosts, gather records, rod/spin, per-particle critts, acceleration);
-//   2. adjoint kernels map the adjoint of the end-of-step state to the adjoint
-//      of the start-of-step state, in the reverse order of the stages:
-//      move and velocity update, dt update, momentum pair gather, constitutive
-//      update, strain pair gather.
+// checkpoint through the granular stage kernels, which leave every
+// intermediate the reverse needs on the device (lists, ghosts, gather
+// records, rod/spin, per-particle critts, acceleration); the start-of-step
+// state and those intermediates of every step of the segment are kept in
+// HBM.  Then, for each step from last to first, its intermediates are put
+// back and the adjoint kernels map the adjoint of the end-of-step state to
+// the adjoint of the start-of-step state, in the reverse order of the
+// stages: move and velocity update, dt update, momentum pair gather,
+// constitutive update, strain pair gather.
 //
 // Pair stages: the real-particle neighbour relation is symmetric (the list
 // criterion |x_i - x_j|^2 < (h_i + h_j)^2 at x^n is), so the contribution of
@@ -62,10 +62,12 @@ constexpr int ADJ_GOF = 8;                    // ghosts of one real particle (at
 
 struct AdjDev {
     double *xb[3], *vb[3], *rb, *sb[6], *eb, *fb;   // adjoint of the state (real particles)
-    double *ab[3], *qb, *rbn;                       // adjoints local to a step
-    double* lb[9];                                  // adjoint of the velocity gradient g[n][m], at 3n + m
+    double *abp, *qb, *rbn, *pdt;                   // adjoints local to a step; abp: acceleration [np][4]
+    double* lbp;                                    // adjoint of the velocity gradient [np][10], g[n][m] at 3n + m
+    double* xin;                                    // [ADJ_NIN][np] input adjoints of the constitutive reverse
     double *gxb[3], *gvb[3], *grb, *gAb[6];         // ghost adjoints [ngcap]
-    int *gt_cnt, *gt;                               // real partners of each ghost [ngcap], [ngcap][gtcap]
+    int *gt_cnt, *gt, *gts;                         // real partners of each ghost [ngcap], [ngcap][gtcap],
+                                                    // unsorted and sorted
     int *gof_cnt, *gof;                             // ghosts of each real particle [np], [np][ADJ_GOF]
     int gtcap, ngcap;
     const double* snap;                             // start-of-step snapshot of the step being reversed
@@ -101,6 +103,19 @@ __device__ __forceinline__ double* snap_field(const DevState& d, int f) {
 
 __device__ __forceinline__ double snap_at(const AdjDev& a, int np, int f, int i) { return a.snap[(long)f * np + i]; }
 
+// packed per-particle adjoints read by the pair gathers (one aligned record
+// per neighbour)
+__device__ __forceinline__ void load_ab(const double* abp, int i, double v[3]) {
+    const double2* p = reinterpret_cast<const double2*>(abp + 4L * i);
+    const double2 x = p[0], y = p[1];
+    v[0] = x.x; v[1] = x.y; v[2] = y.x;
+}
+__device__ __forceinline__ void load_g9(const double* lbp, int i, double G[9]) {
+    const double2* p = reinterpret_cast<const double2*>(lbp + 10L * i);
+    const double2 a = p[0], b = p[1], c = p[2], e = p[3], f = p[4];
+    G[0] = a.x; G[1] = a.y; G[2] = b.x; G[3] = b.y; G[4] = c.x; G[5] = c.y; G[6] = e.x; G[7] = e.y; G[8] = f.x;
+}
+
 // save (1) or restore (0) the real-particle state; grid.y = field
 __global__ __launch_bounds__(MCM_BLOCK) void k_snap(DevState d, double* base, int save) {
     const int f = blockIdx.y;
@@ -129,31 +144,40 @@ __global__ void k_adj_seed(AdjDev a, const int* sample, int ns, const double* se
 }
 
 // ------------------------------------------------------------ kernel Hessian
-// out = H w with H = d(grad W)/d(xij) of the cubic B-spline (mcm_gradw.f90
-// differentiated): grad W = F(r) xij, F = dw / (r havg), so
-// H = F I + (dw' - dw / r) / (r^2 havg) xij xij^T, dw' = d(dw)/dr; zero at r = 0.
-__device__ __forceinline__ void kernel_hess_apply(const double xij[3], double rad, double havg, double pi,
-                                                  const double w[3], double out[3]) {
-    if (rad == 0.0) { out[0] = out[1] = out[2] = 0.0; return; }
-    const double const1 = 1.0 / (pi * havg * havg * havg);
-    const double const2 = const1 * 0.25;
-    const double z = rad / havg;
-    double dw, dwz;
-    if (z < 1.0) {
-        dw = const1 * ((2.25 * z - 3.0) * z);
-        dwz = const1 * (4.5 * z - 3.0);
-    } else if (z < 2.0) {
-        const double t = 2.0 - z;
-        dw = -(3.0 * const2 * (t * t));
-        dwz = 6.0 * const2 * t;
-    } else {
-        out[0] = out[1] = out[2] = 0.0;
-        return;
+// grad W of the cubic B-spline (mcm_gradw.f90) and its derivative for one
+// pair, with two reciprocals (1/havg, 1/r): grad W = F xij with
+// F = dw / (r havg), and H = d(grad W)/d(xij) = F I + C xij xij^T with
+// C = (dw' - dw / r) / (r^2 havg), dw' = d(dw)/dr; zero at r = 0.  The reverse only
+// needs the forward's gradient to rounding, not bit for bit.
+struct KGH {
+    double g[3], F, C;
+};
+__device__ __forceinline__ void kernel_gh(const double xij[3], double havg, double ipi, KGH& k) {
+    const double r2 = xij[0] * xij[0] + xij[1] * xij[1] + xij[2] * xij[2];
+    k.F = 0.0;
+    k.C = 0.0;
+    if (r2 > 0.0) {
+        const double rad = sqrt(r2);
+        const double ih = 1.0 / havg, ir = 1.0 / rad;
+        const double z = rad * ih;
+        const double c1 = ih * ih * ih * ipi;
+        double dw = 0.0, dwz = 0.0;
+        if (z < 1.0) {
+            dw = c1 * ((2.25 * z - 3.0) * z);
+            dwz = c1 * (4.5 * z - 3.0);
+        } else if (z < 2.0) {
+            const double t = 2.0 - z;
+            dw = -(0.75 * c1 * (t * t));
+            dwz = 1.5 * c1 * t;
+        }
+        k.F = dw * ir * ih;
+        k.C = (dwz * ih - dw * ir) * ir * ir * ih;
     }
-    const double F = dw / (rad * havg);
-    const double C = (dwz / havg - dw / rad) / (rad * rad * havg);
+    for (int c = 0; c < 3; c++) k.g[c] = k.F * xij[c];
+}
+__device__ __forceinline__ void hess_apply(const KGH& k, const double xij[3], const double w[3], double out[3]) {
     const double xw = xij[0] * w[0] + xij[1] * w[1] + xij[2] * w[2];
-    for (int k = 0; k < 3; k++) out[k] = F * w[k] + C * xw * xij[k];
+    for (int c = 0; c < 3; c++) out[c] = k.F * w[c] + k.C * xw * xij[c];
 }
 
 // ------------------------------------------------------- ghost transposes
@@ -193,11 +217,24 @@ __device__ __forceinline__ void isort(int* v, int n, long stride) {
     }
 }
 
-// ascending order, so that every sum over a transposed list has a fixed order
+// ascending order, so that every sum over a transposed list has a fixed
+// order: one wavefront per ghost, each entry goes to the slot given by the
+// number of smaller ids in the list (the ids of one list are distinct)
 __global__ __launch_bounds__(MCM_BLOCK) void k_adj_gt_sort(DevState d, AdjDev a) {
     const int ngp = d.sc->ngp;
-    for (int g = blockIdx.x * MCM_BLOCK + threadIdx.x; g < ngp; g += gridDim.x * MCM_BLOCK)
-        isort(a.gt + (long)g * a.gtcap, min(a.gt_cnt[g], a.gtcap), 1);
+    const int lane = threadIdx.x & 63, wpb = MCM_BLOCK / 64;
+    for (int g = blockIdx.x * wpb + (threadIdx.x >> 6); g < ngp; g += gridDim.x * wpb) {
+        const int* src = a.gt + (long)g * a.gtcap;
+        int* dst = a.gts + (long)g * a.gtcap;
+        const int cnt = min(a.gt_cnt[g], a.gtcap);
+        for (int k0 = 0; k0 < cnt; k0 += 64) {
+            const int k = k0 + lane;
+            const int v = k < cnt ? src[k] : 0;
+            int rank = 0;
+            for (int j = 0; j < cnt; j++) rank += src[j] < v ? 1 : 0;
+            if (k < cnt) dst[rank] = v;
+        }
+    }
 }
 
 __global__ __launch_bounds__(MCM_BLOCK) void k_adj_gof_sort(DevState d, AdjDev a) {
@@ -221,10 +258,12 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_adj_move(DevState d, AdjDev a) {
         const SRec me = load_srec(d.rs, i);
         const double vn[3] = {me.vx, me.vy, me.vz};
         double acc[3], vp[3], x[3], v[3], sx[3] = {1.0, 1.0, 1.0}, sv[3] = {1.0, 1.0, 1.0};
+        const MRec mr = load_mrec(d.rm, i);   // position after the strain pass (x1)
+        const double x1[3] = {mr.x, mr.y, mr.z};
         for (int k = 0; k < 3; k++) {
             acc[k] = d.a[k][i];
             vp[k] = vn[k] + acc[k] * dtn;
-            x[k] = d.x1[k][i] + vp[k] * dt1;
+            x[k] = x1[k] + vp[k] * dt1;
             v[k] = vp[k];
         }
         if (d.boundary) {   // the reflections of sym_pen, in its order
@@ -250,7 +289,9 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_adj_move(DevState d, AdjDev a) {
             a.vb[k][i] = vpb;
         }
         boundcond(d.dispbc[i], abar);
-        for (int k = 0; k < 3; k++) a.ab[k][i] = abar[k];
+        double2* ap = reinterpret_cast<double2*>(a.abp + 4L * i);
+        ap[0] = make_double2(abar[0], abar[1]);
+        ap[1] = make_double2(abar[2], 0.0);
         if (d.critts_p[i] == sc->critts) parg = (double)i;
     }
     const double s1 = block_reduce<0>(pxv);
@@ -286,24 +327,31 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_adj_fold_move(DevState d, AdjDev 
 //   a_t = sum_u m_u (A_u + A_t) grad W(x_t - x_u),
 //   Abar_t += (m_u abar_t - m_t abar_u) (x) grad W, xbar_t += H (A_u + A_t) (m_u abar_t - m_t abar_u)
 // (abar_u = 0 for ghosts).  A = (sigma - q I) / rho^2 maps to sigma', q', rho'.
+// One wavefront per particle: lane l takes list entries l, l + 64, ...; the
+// sums are combined across lanes in a fixed butterfly order.
 __global__ __launch_bounds__(MCM_BLOCK) void k_adj_mom(DevState d, AdjDev a) {
-    const int t = blockIdx.x * MCM_BLOCK + threadIdx.x;
-    if (t >= d.np) return;
+    const double ipi = 1.0 / d.pi;
+    const int lane = threadIdx.x & 63;
+    const int t = blockIdx.x * (MCM_BLOCK / 64) + (threadIdx.x >> 6);
+    if (t >= d.np) return;   // the whole wavefront
     const int nxt = a.cur ^ 1;
     const MRec me = load_mrec(d.rm, t);
-    const double at[3] = {a.ab[0][t], a.ab[1][t], a.ab[2][t]};
+    double at[3];
+    load_ab(a.abp, t, at);
     double Ab[6] = {0, 0, 0, 0, 0, 0}, xbt[3] = {0, 0, 0};
     const int n = d.ntot[t];
     const int* lst = nbr_list(d, t);
-    for (int k = 0; k < n; k++) {
+    for (int k = lane; k < n; k += 64) {
         const int u = lst[(long)k * 64];
         const MRec r = load_mrec(d.rm, u);
         const double havg = 0.5 * (me.h + r.h);
         const double xij[3] = {me.x - r.x, me.y - r.y, me.z - r.z};
-        double g[3], rad;
-        kernel_grad(xij[0], xij[1], xij[2], havg, d.pi, g, &rad);
-        double w[3];
-        for (int c = 0; c < 3; c++) w[c] = r.m * at[c] - (u < d.np ? me.m * a.ab[c][u] : 0.0);
+        KGH kg;
+        kernel_gh(xij, havg, ipi, kg);
+        const double* g = kg.g;
+        double w[3], au[3] = {0.0, 0.0, 0.0};
+        if (u < d.np) load_ab(a.abp, u, au);
+        for (int c = 0; c < 3; c++) w[c] = r.m * at[c] - me.m * au[c];
         for (int nn = 0; nn < 3; nn++)
             for (int mm = 0; mm < 3; mm++) Ab[sym_index(mm, nn)] += w[nn] * g[mm];
         double bw[3];
@@ -315,9 +363,12 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_adj_mom(DevState d, AdjDev a) {
             }
         }
         double hw[3];
-        kernel_hess_apply(xij, rad, havg, d.pi, bw, hw);
+        hess_apply(kg, xij, bw, hw);
         for (int c = 0; c < 3; c++) xbt[c] += hw[c];
     }
+    for (int c = 0; c < 6; c++) Ab[c] = wave_sum(Ab[c]);
+    for (int c = 0; c < 3; c++) xbt[c] = wave_sum(xbt[c]);
+    if (lane != 0) return;
     const double rho = d.rbuf[nxt][t], r2 = rho * rho;
     double rbar = 0.0;
     for (int c = 0; c < 6; c++) {
@@ -331,21 +382,25 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_adj_mom(DevState d, AdjDev a) {
 
 // the ghosts' share of the momentum reverse, gathered over their real partners
 __global__ __launch_bounds__(MCM_BLOCK) void k_adj_mom_ghost(DevState d, AdjDev a) {
+    const double ipi = 1.0 / d.pi;
     const int ngp = d.sc->ngp;
-    for (int g = blockIdx.x * MCM_BLOCK + threadIdx.x; g < ngp; g += gridDim.x * MCM_BLOCK) {
+    const int lane = threadIdx.x & 63, wpb = MCM_BLOCK / 64;
+    for (int g = blockIdx.x * wpb + (threadIdx.x >> 6); g < ngp; g += gridDim.x * wpb) {
         const MRec me = load_mrec(d.rm, d.np + g);
         double Ab[6] = {0, 0, 0, 0, 0, 0}, xbg[3] = {0, 0, 0};
         const int cnt = min(a.gt_cnt[g], a.gtcap);
-        const int* lst = a.gt + (long)g * a.gtcap;
-        for (int k = 0; k < cnt; k++) {
+        const int* lst = a.gts + (long)g * a.gtcap;
+        for (int k = lane; k < cnt; k += 64) {
             const int i = lst[k];
             const MRec r = load_mrec(d.rm, i);
             const double havg = 0.5 * (r.h + me.h);
             const double xij[3] = {r.x - me.x, r.y - me.y, r.z - me.z};
-            double gw[3], rad;
-            kernel_grad(xij[0], xij[1], xij[2], havg, d.pi, gw, &rad);
-            double w[3];
-            for (int c = 0; c < 3; c++) w[c] = me.m * a.ab[c][i];
+            KGH kg;
+            kernel_gh(xij, havg, ipi, kg);
+            const double* gw = kg.g;
+            double w[3], ai[3];
+            load_ab(a.abp, i, ai);
+            for (int c = 0; c < 3; c++) w[c] = me.m * ai[c];
             for (int nn = 0; nn < 3; nn++)
                 for (int mm = 0; mm < 3; mm++) Ab[sym_index(mm, nn)] += w[nn] * gw[mm];
             double bw[3];
@@ -357,11 +412,15 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_adj_mom_ghost(DevState d, AdjDev 
                 }
             }
             double hw[3];
-            kernel_hess_apply(xij, rad, havg, d.pi, bw, hw);
+            hess_apply(kg, xij, bw, hw);
             for (int c = 0; c < 3; c++) xbg[c] -= hw[c];
         }
-        for (int c = 0; c < 6; c++) a.gAb[c][g] = Ab[c];
-        for (int c = 0; c < 3; c++) a.gxb[c][g] = xbg[c];
+        for (int c = 0; c < 6; c++) Ab[c] = wave_sum(Ab[c]);
+        for (int c = 0; c < 3; c++) xbg[c] = wave_sum(xbg[c]);
+        if (lane == 0) {
+            for (int c = 0; c < 6; c++) a.gAb[c][g] = Ab[c];
+            for (int c = 0; c < 3; c++) a.gxb[c][g] = xbg[c];
+        }
     }
 }
 
@@ -419,117 +478,155 @@ __device__ __forceinline__ void seed(Dual<K>& x, int j, int p0) {
     if (j >= p0 && j < p0 + K) x.d[j - p0] = 1.0;
 }
 
-// Reverse of the constitutive update of real particle i (mcm_constitutive.f90
-// with mcm_rhoupd, mcm_stress_rot1, mcm_f3dm10, mcm_sueos, mcm_nodalav,
-// mcm_hieupd, mcm_eqos, mcm_stress_up and mcm_calccritts): the adjoint of
-// (sigma', rho', e', efps', q', critts) becomes the adjoint of (rod, spin,
-// sigma, rho, e, efps, dt, parameters) through the transposed local Jacobian.
-__global__ __launch_bounds__(MCM_BLOCK) void k_adj_const(DevState d, AdjDev a) {
+constexpr int ADJ_NPASS = (ADJ_NIN + ADJ_K - 1) / ADJ_K;
+
+// One dual pass of the reverse of the constitutive update of real particle i
+// (mcm_constitutive.f90 with mcm_rhoupd, mcm_stress_rot1, mcm_f3dm10,
+// mcm_sueos, mcm_nodalav, mcm_hieupd, mcm_eqos, mcm_stress_up and
+// mcm_calccritts): the forward's chain with the inputs P0 .. P0 + K - 1 seeded
+// gives K columns of the local Jacobian of (sigma', rho', e', efps', q',
+// critts) with respect to (rod, spin, sigma, rho, e, efps, |v|, dt,
+// parameters); out[k] = adjoint of input P0 + k.
+template <int P0>
+__device__ __forceinline__ void const_pass(const DevState& d, const AdjDev& a, int i, double out[ADJ_K]) {
     typedef Dual<ADJ_K> D;
+    const int np = d.np;
+    const MatParams& m0 = d.mat[d.pmat[i]];
+    const double dtn = d.sc->dtold;   // the step's dt (before the dt update)
+    PStateT<D> s;
+    s.mass = d.mass[i];
+    s.rho0 = d.rho0[i];
+    s.h = snap_at(a, np, SN_H + a.cur, i);
+    s.hold = snap_at(a, np, SN_HOLD, i);
+    s.h0 = d.h0[i];
+    s.p = snap_at(a, np, SN_P, i);
+    s.c = snap_at(a, np, SN_C, i);
+    s.vabs = snap_at(a, np, SN_VABS, i);
+    seed(s.vabs, IN_VABS, P0);
+    s.pcut = snap_at(a, np, SN_PCUT, i);
+    s.p_cut = snap_at(a, np, SN_PCUT2, i);
+    s.fail = snap_at(a, np, SN_FAIL, i);
+    s.q = snap_at(a, np, SN_Q, i);
+    s.einc = 0.0;
+    s.etry = 0.0;
+#pragma unroll
+    for (int c = 0; c < 6; c++) { s.rod[c] = d.rod[c][i]; seed(s.rod[c], IN_ROD + c, P0); }
+#pragma unroll
+    for (int c = 0; c < 3; c++) { s.spin[c] = d.spin[c][i]; seed(s.spin[c], IN_SPIN + c, P0); }
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+        s.sig[c] = snap_at(a, np, SN_SIG + c, i);
+        seed(s.sig[c], IN_SIG + c, P0);
+    }
+    s.rho = snap_at(a, np, SN_R + a.cur, i);
+    seed(s.rho, IN_RHO, P0);
+    s.e = snap_at(a, np, SN_E, i);
+    seed(s.e, IN_E, P0);
+    s.efps = snap_at(a, np, SN_EFPS, i);
+    seed(s.efps, IN_EFPS, P0);
+    D dt = dtn;
+    seed(dt, IN_DT, P0);
+    MatDual<ADJ_K> m = dual_mat<ADJ_K>(m0);
+    seed(m.strinput[0], IN_PAR + 0, P0);
+    seed(m.strinput[1], IN_PAR + 1, P0);
+    seed(m.strinput[2], IN_PAR + 2, P0);
+    seed(m.eosinput[1], IN_PAR + 3, P0);
+    m.g = m.strinput[0];   // mcm_calcg.f90: G of model 10
+    s.tracerod = s.rod[S11] + s.rod[S22] + s.rod[S33];
+    // the forward's chain (constitutive_particle, model 10)
+    rhoupd(s, m, dt);
+    stress_rot1(s, dt);
+    double dprev = snap_at(a, np, SN_DEFPS, i);
+    f3dm10(s, m, dt, dprev);
+    sueos(s, m);
+    nodalav(s, m);
+    hieupd(s, dt);
+    eqos(s, m, dt);
+    stress_up(s);
+    const D cr = critts(s, m, d.tcrit_opt, 0.0);
+    double ys[6];
+#pragma unroll
+    for (int c = 0; c < 6; c++) ys[c] = a.sb[c][i];
+    const double yr = a.rb[i], ye = a.eb[i], yf = a.fb[i], yq = a.qb[i];
+    const double yc = (double)i == a.as[AS_ARGMIN] ? a.as[AS_CRIT] : 0.0;
+#pragma unroll
+    for (int k = 0; k < ADJ_K; k++) {
+        double v = yr * s.rho.d[k] + ye * s.e.d[k] + yf * s.efps.d[k] + yq * s.q.d[k] + yc * cr.d[k];
+#pragma unroll
+        for (int c = 0; c < 6; c++) v += ys[c] * s.sig[c].d[k];
+        out[k] = v;
+    }
+}
+
+// The passes of the constitutive reverse, one thread per (particle, pass):
+// grid.y = pass; the input adjoints go to the staging array xin[input][np].
+__global__ __launch_bounds__(MCM_BLOCK) void k_adj_const(DevState d, AdjDev a) {
+    const int i = blockIdx.x * MCM_BLOCK + threadIdx.x;
+    if (i >= d.np) return;
+    double out[ADJ_K];
+    switch (blockIdx.y) {
+        case 0: const_pass<0 * ADJ_K>(d, a, i, out); break;
+        case 1: const_pass<1 * ADJ_K>(d, a, i, out); break;
+        case 2: const_pass<2 * ADJ_K>(d, a, i, out); break;
+        case 3: const_pass<3 * ADJ_K>(d, a, i, out); break;
+        case 4: const_pass<4 * ADJ_K>(d, a, i, out); break;
+        default: const_pass<5 * ADJ_K>(d, a, i, out); break;
+    }
+    static_assert(ADJ_NPASS == 6, "one case per pass");
+#pragma unroll
+    for (int k = 0; k < ADJ_K; k++) {
+        const int j = blockIdx.y * ADJ_K + k;
+        if (j < ADJ_NIN) a.xin[(long)j * d.np + i] = out[k];
+    }
+}
+
+// Input adjoints of the constitutive reverse to the state adjoints: rod /
+// spin -> velocity gradient g (rod_spin: rod_ab = (g_ab + g_ba) / 2,
+// spin = (g_ab - g_ba) / 2 for (ab) = (01), (02), (12)), |v^n| -> v^n
+// (mcm_update_velocity.f90:45-50), per-block partials of dt and the
+// parameters.
+__global__ __launch_bounds__(MCM_BLOCK) void k_adj_const_fin(DevState d, AdjDev a) {
     const int i = blockIdx.x * MCM_BLOCK + threadIdx.x;
     const int np = d.np;
     const int mi = i < np ? d.pmat[i] : 0;
-    double xbar[ADJ_NIN];
-    for (int j = 0; j < ADJ_NIN; j++) xbar[j] = 0.0;
+    double x[ADJ_NIN];
+#pragma unroll
+    for (int j = 0; j < ADJ_NIN; j++) x[j] = i < np ? a.xin[(long)j * np + i] : 0.0;
     if (i < np) {
-        const MatParams& m0 = d.mat[mi];
-        const double dtn = d.sc->dtold;   // the step's dt (before the dt update)
-        double ys[6];
-        for (int c = 0; c < 6; c++) ys[c] = a.sb[c][i];
-        const double yr = a.rb[i], ye = a.eb[i], yf = a.fb[i], yq = a.qb[i];
-        const double yc = (double)i == a.as[AS_ARGMIN] ? a.as[AS_CRIT] : 0.0;
-#pragma unroll
-        for (int p0 = 0; p0 < ADJ_NIN; p0 += ADJ_K) {
-            PStateT<D> s;
-            s.mass = d.mass[i];
-            s.rho0 = d.rho0[i];
-            s.h = snap_at(a, np, SN_H + a.cur, i);
-            s.hold = snap_at(a, np, SN_HOLD, i);
-            s.h0 = d.h0[i];
-            s.p = snap_at(a, np, SN_P, i);
-            s.c = snap_at(a, np, SN_C, i);
-            s.vabs = snap_at(a, np, SN_VABS, i);
-            seed(s.vabs, IN_VABS, p0);
-            s.pcut = snap_at(a, np, SN_PCUT, i);
-            s.p_cut = snap_at(a, np, SN_PCUT2, i);
-            s.fail = snap_at(a, np, SN_FAIL, i);
-            s.q = snap_at(a, np, SN_Q, i);
-            s.einc = 0.0;
-            s.etry = 0.0;
-            for (int c = 0; c < 6; c++) { s.rod[c] = d.rod[c][i]; seed(s.rod[c], IN_ROD + c, p0); }
-            for (int c = 0; c < 3; c++) { s.spin[c] = d.spin[c][i]; seed(s.spin[c], IN_SPIN + c, p0); }
-            for (int c = 0; c < 6; c++) {
-                s.sig[c] = snap_at(a, np, SN_SIG + c, i);
-                seed(s.sig[c], IN_SIG + c, p0);
-            }
-            s.rho = snap_at(a, np, SN_R + a.cur, i);
-            seed(s.rho, IN_RHO, p0);
-            s.e = snap_at(a, np, SN_E, i);
-            seed(s.e, IN_E, p0);
-            s.efps = snap_at(a, np, SN_EFPS, i);
-            seed(s.efps, IN_EFPS, p0);
-            D dt = dtn;
-            seed(dt, IN_DT, p0);
-            MatDual<ADJ_K> m = dual_mat<ADJ_K>(m0);
-            seed(m.strinput[0], IN_PAR + 0, p0);
-            seed(m.strinput[1], IN_PAR + 1, p0);
-            seed(m.strinput[2], IN_PAR + 2, p0);
-            seed(m.eosinput[1], IN_PAR + 3, p0);
-            m.g = m.strinput[0];   // mcm_calcg.f90: G of model 10
-            s.tracerod = s.rod[S11] + s.rod[S22] + s.rod[S33];
-            // the forward's chain (constitutive_particle, model 10)
-            rhoupd(s, m, dt);
-            stress_rot1(s, dt);
-            double dprev = snap_at(a, np, SN_DEFPS, i);
-            f3dm10(s, m, dt, dprev);
-            sueos(s, m);
-            nodalav(s, m);
-            hieupd(s, dt);
-            eqos(s, m, dt);
-            stress_up(s);
-            const D cr = critts(s, m, d.tcrit_opt, 0.0);
-#pragma unroll
-            for (int k = 0; k < ADJ_K; k++) {
-                const int j = p0 + k;
-                if (j >= ADJ_NIN) break;
-                double v = yr * s.rho.d[k] + ye * s.e.d[k] + yf * s.efps.d[k] + yq * s.q.d[k] + yc * cr.d[k];
-                for (int c = 0; c < 6; c++) v += ys[c] * s.sig[c].d[k];
-                xbar[j] += v;
-            }
-        }
-        // rod / spin -> velocity gradient g (rod_spin: rod_ab = (g_ab + g_ba) / 2,
-        // spin = (g_ab - g_ba) / 2 for (ab) = (01), (02), (12))
-        const double* rb_ = xbar + IN_ROD;
-        const double* wb = xbar + IN_SPIN;
-        a.lb[0][i] = rb_[S11];
-        a.lb[4][i] = rb_[S22];
-        a.lb[8][i] = rb_[S33];
-        a.lb[1][i] = 0.5 * rb_[S12] + 0.5 * wb[0];
-        a.lb[3][i] = 0.5 * rb_[S12] - 0.5 * wb[0];
-        a.lb[2][i] = 0.5 * rb_[S13] + 0.5 * wb[1];
-        a.lb[6][i] = 0.5 * rb_[S13] - 0.5 * wb[1];
-        a.lb[5][i] = 0.5 * rb_[S23] + 0.5 * wb[2];
-        a.lb[7][i] = 0.5 * rb_[S23] - 0.5 * wb[2];
-        for (int c = 0; c < 6; c++) a.sb[c][i] = xbar[IN_SIG + c];
-        a.rb[i] = xbar[IN_RHO];
-        a.eb[i] = xbar[IN_E];
-        a.fb[i] = xbar[IN_EFPS];
-        // |v^n| = sqrt(v . v) (mcm_update_velocity.f90:45-50)
+        const double* rb_ = x + IN_ROD;
+        const double* wb = x + IN_SPIN;
+        double G[10];
+        G[0] = rb_[S11];
+        G[4] = rb_[S22];
+        G[8] = rb_[S33];
+        G[1] = 0.5 * rb_[S12] + 0.5 * wb[0];
+        G[3] = 0.5 * rb_[S12] - 0.5 * wb[0];
+        G[2] = 0.5 * rb_[S13] + 0.5 * wb[1];
+        G[6] = 0.5 * rb_[S13] - 0.5 * wb[1];
+        G[5] = 0.5 * rb_[S23] + 0.5 * wb[2];
+        G[7] = 0.5 * rb_[S23] - 0.5 * wb[2];
+        G[9] = 0.0;
+        double2* gp = reinterpret_cast<double2*>(a.lbp + 10L * i);
+        for (int c = 0; c < 5; c++) gp[c] = make_double2(G[2 * c], G[2 * c + 1]);
+        for (int c = 0; c < 6; c++) a.sb[c][i] = x[IN_SIG + c];
+        a.rb[i] = x[IN_RHO];
+        a.eb[i] = x[IN_E];
+        a.fb[i] = x[IN_EFPS];
         const double va = snap_at(a, np, SN_VABS, i);
-        if (xbar[IN_VABS] != 0.0 && va > 0.0) {
+        if (x[IN_VABS] != 0.0 && va > 0.0) {
             const SRec me = load_srec(d.rs, i);
-            const double f = xbar[IN_VABS] / va;
+            const double f = x[IN_VABS] / va;
             a.vb[0][i] += f * me.vx;
             a.vb[1][i] += f * me.vy;
             a.vb[2][i] += f * me.vz;
         }
     }
-    const double sdt = block_reduce<0>(xbar[IN_DT]);
+    const double sdt = block_reduce<0>(x[IN_DT]);
     double* r = a.red + (long)blockIdx.x * RA_N;
     if (threadIdx.x == 0) r[RA_CDT] = sdt;
     for (int m = 0; m < d.nummat; m++)
         for (int k = 0; k < ADJ_NPAR; k++) {
-            const double s = block_reduce<0>(i < np && mi == m ? xbar[IN_PAR + k] : 0.0);
+            const double s = block_reduce<0>(i < np && mi == m ? x[IN_PAR + k] : 0.0);
             if (threadIdx.x == 0) r[RA_PAR + ADJ_NPAR * m + k] = s;
         }
 }
@@ -539,86 +636,96 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_adj_const(DevState d, AdjDev a) {
 // mcm_strain.f90:27-31 for real particle t, with t's terms in its real
 // neighbours' sums gathered over t's own list:
 //   g_t[n][m] = sum_u V_u (v_u - v_t)_n grad_m W(xh_t - xh_u), V = m / rho.
+// One wavefront per particle, as k_adj_mom; the particle's share of the dt
+// adjoint goes to pdt (reduced by k_adj_gmap_strain).
 __global__ __launch_bounds__(MCM_BLOCK) void k_adj_strain(DevState d, AdjDev a) {
-    const int t = blockIdx.x * MCM_BLOCK + threadIdx.x;
-    double pdt = 0.0;
-    if (t < d.np) {
-        const double dt = d.sc->dtold;
-        const SRec me = load_srec(d.rs, t);
-        const double vt[3] = {me.vx, me.vy, me.vz};
-        const double xh[3] = {me.x - 0.5 * me.vx * dt, me.y - 0.5 * me.vy * dt, me.z - 0.5 * me.vz * dt};
-        const double rho_t = d.rbuf[a.cur][t];
-        double Gt[9];
-        for (int c = 0; c < 9; c++) Gt[c] = a.lb[c][t];
-        double vbt[3] = {0, 0, 0}, xhb[3] = {0, 0, 0}, rbt = 0.0;
-        const int n = d.ntot[t];
-        const int* lst = nbr_list(d, t);
-        for (int k = 0; k < n; k++) {
-            const int u = lst[(long)k * 64];
-            const SRec r = load_srec(d.rs, u);
-            const double sh = u < d.np ? dt : 0.0;
-            const double xj[3] = {r.x - 0.5 * r.vx * sh, r.y - 0.5 * r.vy * sh, r.z - 0.5 * r.vz * sh};
-            const double havg = 0.5 * (me.h + r.h);
-            const double xij[3] = {xh[0] - xj[0], xh[1] - xj[1], xh[2] - xj[2]};
-            double g[3], rad;
-            kernel_grad(xij[0], xij[1], xij[2], havg, d.pi, g, &rad);
-            const double dv[3] = {r.vx - vt[0], r.vy - vt[1], r.vz - vt[2]};
-            // own terms
-            double w[3];
-            for (int mm = 0; mm < 3; mm++) {
-                w[mm] = Gt[mm] * dv[0] + Gt[3 + mm] * dv[1] + Gt[6 + mm] * dv[2];
-            }
+    const double ipi = 1.0 / d.pi;
+    const int lane = threadIdx.x & 63;
+    const int t = blockIdx.x * (MCM_BLOCK / 64) + (threadIdx.x >> 6);
+    if (t >= d.np) return;   // the whole wavefront
+    const double dt = d.sc->dtold;
+    const SRec me = load_srec(d.rs, t);
+    const double vt[3] = {me.vx, me.vy, me.vz};
+    const double xh[3] = {me.x - 0.5 * me.vx * dt, me.y - 0.5 * me.vy * dt, me.z - 0.5 * me.vz * dt};
+    const double rho_t = d.rbuf[a.cur][t];
+    double Gt[9];
+    load_g9(a.lbp, t, Gt);
+    double vbt[3] = {0, 0, 0}, xhb[3] = {0, 0, 0}, rbt = 0.0;
+    const int n = d.ntot[t];
+    const int* lst = nbr_list(d, t);
+    for (int k = lane; k < n; k += 64) {
+        const int u = lst[(long)k * 64];
+        const SRec r = load_srec(d.rs, u);
+        const double sh = u < d.np ? dt : 0.0;
+        const double xj[3] = {r.x - 0.5 * r.vx * sh, r.y - 0.5 * r.vy * sh, r.z - 0.5 * r.vz * sh};
+        const double havg = 0.5 * (me.h + r.h);
+        const double xij[3] = {xh[0] - xj[0], xh[1] - xj[1], xh[2] - xj[2]};
+        KGH kg;
+        kernel_gh(xij, havg, ipi, kg);
+        const double* g = kg.g;
+        const double dv[3] = {r.vx - vt[0], r.vy - vt[1], r.vz - vt[2]};
+        // own terms
+        double w[3];
+        for (int mm = 0; mm < 3; mm++) w[mm] = Gt[mm] * dv[0] + Gt[3 + mm] * dv[1] + Gt[6 + mm] * dv[2];
+        for (int nn = 0; nn < 3; nn++)
+            vbt[nn] -= r.vol * (Gt[3 * nn] * g[0] + Gt[3 * nn + 1] * g[1] + Gt[3 * nn + 2] * g[2]);
+        double hw[3];
+        hess_apply(kg, xij, w, hw);
+        for (int c = 0; c < 3; c++) xhb[c] += r.vol * hw[c];
+        // t's terms in the sum of real neighbour u
+        if (u < d.np) {
+            double Gu[9];
+            load_g9(a.lbp, u, Gu);
+            double w2[3];
+            for (int mm = 0; mm < 3; mm++) w2[mm] = -(Gu[mm] * dv[0] + Gu[3 + mm] * dv[1] + Gu[6 + mm] * dv[2]);
             for (int nn = 0; nn < 3; nn++)
-                vbt[nn] -= r.vol * (Gt[3 * nn] * g[0] + Gt[3 * nn + 1] * g[1] + Gt[3 * nn + 2] * g[2]);
-            double hw[3];
-            kernel_hess_apply(xij, rad, havg, d.pi, w, hw);
-            for (int c = 0; c < 3; c++) xhb[c] += r.vol * hw[c];
-            // t's terms in the sum of real neighbour u
-            if (u < d.np) {
-                double Gu[9];
-                for (int c = 0; c < 9; c++) Gu[c] = a.lb[c][u];
-                double w2[3];
-                for (int mm = 0; mm < 3; mm++) w2[mm] = -(Gu[mm] * dv[0] + Gu[3 + mm] * dv[1] + Gu[6 + mm] * dv[2]);
-                for (int nn = 0; nn < 3; nn++)
-                    vbt[nn] -= me.vol * (Gu[3 * nn] * g[0] + Gu[3 * nn + 1] * g[1] + Gu[3 * nn + 2] * g[2]);
-                rbt += me.vol / rho_t * (w2[0] * g[0] + w2[1] * g[1] + w2[2] * g[2]);
-                kernel_hess_apply(xij, rad, havg, d.pi, w2, hw);
-                for (int c = 0; c < 3; c++) xhb[c] -= me.vol * hw[c];
-            }
+                vbt[nn] -= me.vol * (Gu[3 * nn] * g[0] + Gu[3 * nn + 1] * g[1] + Gu[3 * nn + 2] * g[2]);
+            rbt += me.vol / rho_t * (w2[0] * g[0] + w2[1] * g[1] + w2[2] * g[2]);
+            hess_apply(kg, xij, w2, hw);
+            for (int c = 0; c < 3; c++) xhb[c] -= me.vol * hw[c];
         }
-        // xh = x - v dt / 2
-        for (int c = 0; c < 3; c++) {
-            a.xb[c][t] += xhb[c];
-            a.vb[c][t] += vbt[c] - 0.5 * dt * xhb[c];
-            pdt += -0.5 * vt[c] * xhb[c];
-        }
-        a.rbn[t] += rbt;
     }
-    const double s = block_reduce<0>(pdt);
-    if (threadIdx.x == 0) a.red[(long)blockIdx.x * RA_N + RA_SDT] = s;
+    for (int c = 0; c < 3; c++) {
+        vbt[c] = wave_sum(vbt[c]);
+        xhb[c] = wave_sum(xhb[c]);
+    }
+    rbt = wave_sum(rbt);
+    if (lane != 0) return;
+    // xh = x - v dt / 2
+    double pdt = 0.0;
+    for (int c = 0; c < 3; c++) {
+        a.xb[c][t] += xhb[c];
+        a.vb[c][t] += vbt[c] - 0.5 * dt * xhb[c];
+        pdt += -0.5 * vt[c] * xhb[c];
+    }
+    a.rbn[t] += rbt;
+    a.pdt[t] = pdt;
 }
 
 // the ghosts' share of the strain reverse (ghost positions are not shifted)
 __global__ __launch_bounds__(MCM_BLOCK) void k_adj_strain_ghost(DevState d, AdjDev a) {
+    const double ipi = 1.0 / d.pi;
     const int ngp = d.sc->ngp;
     const double dt = d.sc->dtold;
-    for (int g = blockIdx.x * MCM_BLOCK + threadIdx.x; g < ngp; g += gridDim.x * MCM_BLOCK) {
+    const int lane = threadIdx.x & 63, wpb = MCM_BLOCK / 64;
+    for (int g = blockIdx.x * wpb + (threadIdx.x >> 6); g < ngp; g += gridDim.x * wpb) {
         const int u = d.np + g;
         const SRec me = load_srec(d.rs, u);
         const double rho_g = d.rbuf[a.cur][u];
         double vbg[3] = {0, 0, 0}, xbg[3] = {0, 0, 0}, rbg = 0.0;
         const int cnt = min(a.gt_cnt[g], a.gtcap);
-        const int* lst = a.gt + (long)g * a.gtcap;
-        for (int k = 0; k < cnt; k++) {
+        const int* lst = a.gts + (long)g * a.gtcap;
+        for (int k = lane; k < cnt; k += 64) {
             const int i = lst[k];
             const SRec r = load_srec(d.rs, i);
             const double xh[3] = {r.x - 0.5 * r.vx * dt, r.y - 0.5 * r.vy * dt, r.z - 0.5 * r.vz * dt};
             const double havg = 0.5 * (r.h + me.h);
             const double xij[3] = {xh[0] - me.x, xh[1] - me.y, xh[2] - me.z};
-            double gw[3], rad;
-            kernel_grad(xij[0], xij[1], xij[2], havg, d.pi, gw, &rad);
+            KGH kg;
+            kernel_gh(xij, havg, ipi, kg);
+            const double* gw = kg.g;
             double Gi[9];
-            for (int c = 0; c < 9; c++) Gi[c] = a.lb[c][i];
+            load_g9(a.lbp, i, Gi);
             const double dv[3] = {me.vx - r.vx, me.vy - r.vy, me.vz - r.vz};
             double w[3];
             for (int mm = 0; mm < 3; mm++) w[mm] = Gi[mm] * dv[0] + Gi[3 + mm] * dv[1] + Gi[6 + mm] * dv[2];
@@ -626,32 +733,51 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_adj_strain_ghost(DevState d, AdjD
                 vbg[nn] += me.vol * (Gi[3 * nn] * gw[0] + Gi[3 * nn + 1] * gw[1] + Gi[3 * nn + 2] * gw[2]);
             rbg += -(me.vol / rho_g) * (w[0] * gw[0] + w[1] * gw[1] + w[2] * gw[2]);
             double hw[3];
-            kernel_hess_apply(xij, rad, havg, d.pi, w, hw);
+            hess_apply(kg, xij, w, hw);
             for (int c = 0; c < 3; c++) xbg[c] -= me.vol * hw[c];
         }
-        for (int c = 0; c < 3; c++) { a.gvb[c][g] = vbg[c]; a.gxb[c][g] = xbg[c]; }
-        a.grb[g] = rbg;
+        for (int c = 0; c < 3; c++) {
+            vbg[c] = wave_sum(vbg[c]);
+            xbg[c] = wave_sum(xbg[c]);
+        }
+        rbg = wave_sum(rbg);
+        if (lane == 0) {
+            for (int c = 0; c < 3; c++) { a.gvb[c][g] = vbg[c]; a.gxb[c][g] = xbg[c]; }
+            a.grb[g] = rbg;
+        }
     }
 }
 
 // ghost strain adjoints to the real parents: x, v mirrored, rho copied
 // (mcm_ghost_setup.f90:330-341)
+// (and the block partials of the strain pass's dt adjoint)
 __global__ __launch_bounds__(MCM_BLOCK) void k_adj_gmap_strain(DevState d, AdjDev a) {
     const int p = blockIdx.x * MCM_BLOCK + threadIdx.x;
-    if (p >= d.np) return;
-    const int cnt = min(a.gof_cnt[p], ADJ_GOF);
-    for (int k = 0; k < cnt; k++) {
-        const int g = a.gof[(long)p * ADJ_GOF + k];
-        const int sg = d.gsgn[g];
-        for (int c = 0; c < 3; c++) {
-            const double s = (sg >> c) & 1 ? -1.0 : 1.0;
-            a.xb[c][p] += s * a.gxb[c][g];
-            a.vb[c][p] += s * a.gvb[c][g];
+    double pdt = 0.0;
+    if (p < d.np) {
+        const int cnt = min(a.gof_cnt[p], ADJ_GOF);
+        for (int k = 0; k < cnt; k++) {
+            const int g = a.gof[(long)p * ADJ_GOF + k];
+            const int sg = d.gsgn[g];
+            for (int c = 0; c < 3; c++) {
+                const double s = (sg >> c) & 1 ? -1.0 : 1.0;
+                a.xb[c][p] += s * a.gxb[c][g];
+                a.vb[c][p] += s * a.gvb[c][g];
+            }
+            a.rbn[p] += a.grb[g];
         }
-        a.rbn[p] += a.grb[g];
+        a.rb[p] += a.rbn[p];
+        a.rbn[p] = 0.0;
+        pdt = a.pdt[p];
     }
-    a.rb[p] += a.rbn[p];
-    a.rbn[p] = 0.0;
+    const double sdt = block_reduce<0>(pdt);
+    if (threadIdx.x == 0) a.red[(long)blockIdx.x * RA_N + RA_SDT] = sdt;
+}
+
+// a recomputed step that stopped on a capacity (cannot happen: the forward ran
+// the same states at the same or smaller capacities) is reported at the end
+__global__ void k_adj_check(DevState d, AdjDev a) {
+    if (d.sc->abort) atomicOr(a.err, 2);
 }
 
 // adjoint of the step's dt and of the parameters.  One workgroup.
@@ -669,6 +795,76 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_adj_fold_end(DevState d, AdjDev a
 }
 
 
+// The intermediates of one recomputed step that the adjoint kernels read:
+// gather records, both density buffers (over reals and the step's ghosts),
+// rod, spin, acceleration and critts of the reals; neighbour lists and
+// counts, ghost parents and mirror signs.  Field f of a slot: doubles at
+// dbase + off, ints at ibase + off.
+struct InterLayout {
+    long ucap, np, nbrlen, ngcap;
+};
+constexpr int INTER_ND = 17, INTER_NI = 5;
+
+__device__ __forceinline__ void inter_field(const DevState& d, const InterLayout& L, int f, int ngp, double*& dp,
+                                            int*& ip, long& len, long& off) {
+    const long u = L.np + ngp;
+    dp = nullptr;
+    ip = nullptr;
+    switch (f) {
+        case 0: dp = d.rs; len = 8 * u; off = 0; return;
+        case 1: dp = d.rm; len = 12 * u; off = 8 * L.ucap; return;
+        case 2: dp = d.rbuf[0]; len = u; off = 20 * L.ucap; return;
+        case 3: dp = d.rbuf[1]; len = u; off = 21 * L.ucap; return;
+        default: break;
+    }
+    if (f < INTER_ND) {
+        const int k = f - 4;   // rod 6, spin 3, a 3, critts
+        dp = k < 6 ? d.rod[k] : (k < 9 ? d.spin[k - 6] : (k < 12 ? d.a[k - 9] : d.critts_p));
+        len = L.np;
+        off = 22 * L.ucap + (long)k * L.np;
+        return;
+    }
+    switch (f - INTER_ND) {
+        case 0: ip = d.nbr; len = L.nbrlen; off = 0; return;   // rows below max_count only (k_inter)
+        case 1: ip = d.nreal; len = L.np; off = L.nbrlen; return;
+        case 2: ip = d.ntot; len = L.np; off = L.nbrlen + L.np; return;
+        case 3: ip = d.gpar; len = ngp; off = L.nbrlen + 2 * L.np; return;
+        default: ip = d.gsgn; len = ngp; off = L.nbrlen + 2 * L.np + L.ngcap; return;
+    }
+}
+
+// save (1) or restore (0) the intermediates; grid.y = field.  On restore the
+// step's scalars (ngp) are restored first.
+__global__ __launch_bounds__(MCM_BLOCK) void k_inter(DevState d, InterLayout L, double* dbase, int* ibase,
+                                                     int save) {
+    double* dp;
+    int* ip;
+    long len, off;
+    inter_field(d, L, blockIdx.y, d.sc->ngp, dp, ip, len, off);
+    if (blockIdx.y == INTER_ND) {
+        // the blocked lists [npad/64][ncap][64]: rows k < the step's largest count
+        const long rows = min(d.sc->max_count, d.ncap), nblk64 = L.np / 64 + 1;
+        const long span = rows * 64;
+        const long n = nblk64 * span;
+        for (long i = blockIdx.x * (long)MCM_BLOCK + threadIdx.x; i < n; i += (long)gridDim.x * MCM_BLOCK) {
+            const long b = i / span, q = b * d.ncap * 64 + (i - b * span);
+            if (q >= L.nbrlen) continue;
+            if (save) ibase[off + q] = ip[q];
+            else ip[q] = ibase[off + q];
+        }
+        return;
+    }
+    for (long i = blockIdx.x * (long)MCM_BLOCK + threadIdx.x; i < len; i += (long)gridDim.x * MCM_BLOCK) {
+        if (dp) {
+            if (save) dbase[off + i] = dp[i];
+            else dp[i] = dbase[off + i];
+        } else {
+            if (save) ibase[off + i] = ip[i];
+            else ip[i] = ibase[off + i];
+        }
+    }
+}
+
 // ================================================================== host
 struct mcm_gpu_adjoint {
     mcm_gpu_ctx* ctx = nullptr;
@@ -677,9 +873,15 @@ struct mcm_gpu_adjoint {
     double* ckpt = nullptr;           // [nseg][SN_N][np]: state at the start of each segment
     Scalars* ckpt_sc = nullptr;       // [nseg]
     std::vector<int> ckpt_cur;
-    double* seg = nullptr;            // [k][SN_N][np]: the steps of the segment being reversed (k > 1)
+    double* seg = nullptr;            // [k][SN_N][np]: start of every step of the segment being reversed
     Scalars* seg_sc = nullptr;
     std::vector<int> seg_cur;
+    InterLayout lay{};                // intermediates of every step of the segment
+    double* inter_d = nullptr;        // [k][INTER doubles]
+    int* inter_i = nullptr;           // [k][INTER ints]
+    Scalars* inter_sc = nullptr;      // [k]: scalars after the step (dt, dtold, critts, ngp)
+    size_t inter_dn = 0, inter_in = 0;
+    std::vector<void*> iallocs;       // intermediate slots (sized by the run's capacities)
     int* d_sample = nullptr;
     double* d_hist = nullptr;         // [nsteps][nsample][3]: positions, then their adjoints
     AdjDev a{};
@@ -784,15 +986,51 @@ int adj_ghost_buffers(mcm_gpu_adjoint* A) {
     for (int c = 0; c < 6; c++) rc |= adj_alloc(A, A->gallocs, &a.gAb[c], a.ngcap);
     rc |= adj_alloc(A, A->gallocs, &a.gt_cnt, a.ngcap);
     rc |= adj_alloc(A, A->gallocs, &a.gt, (size_t)a.ngcap * a.gtcap);
+    rc |= adj_alloc(A, A->gallocs, &a.gts, (size_t)a.ngcap * a.gtcap);
     return rc;
 }
 
+// intermediate slots for k steps at the run's current capacities
+int adj_inter_buffers(mcm_gpu_adjoint* A) {
+    const DevState& d = A->ctx->d;
+    const long ucap = (long)d.np + d.ngcap, nbrlen = (long)d.npad * d.ncap;
+    if (!A->iallocs.empty() && A->lay.ucap == ucap && A->lay.nbrlen == nbrlen) return 0;
+    for (void* p : A->iallocs) (void)hipFree(p);
+    A->iallocs.clear();
+    A->lay = InterLayout{ucap, (long)d.np, nbrlen, (long)d.ngcap};
+    A->inter_dn = (size_t)(22 * ucap + 13 * (long)d.np);
+    A->inter_in = (size_t)(nbrlen + 2 * (long)d.np + 2 * (long)d.ngcap);
+    int rc = 0;
+    rc |= adj_alloc(A, A->iallocs, &A->inter_d, A->inter_dn * A->k);
+    rc |= adj_alloc(A, A->iallocs, &A->inter_i, A->inter_in * A->k);
+    rc |= adj_alloc(A, A->iallocs, &A->inter_sc, A->k);
+    return rc;
+}
+
+int adj_inter(mcm_gpu_adjoint* A, int s, bool save) {
+    mcm_gpu_ctx* ctx = A->ctx;
+    const long big = std::max(A->lay.nbrlen, 22 * A->lay.ucap);
+    dim3 grid((unsigned)std::min<long>(blocks_for(big / 8 + 1, MCM_BLOCK), 1024), INTER_ND + INTER_NI);
+    if (save) {
+        HIPCHK_A(hipMemcpyAsync(A->inter_sc + s, ctx->sc, sizeof(Scalars), hipMemcpyDeviceToDevice, ctx->stream));
+    } else {
+        HIPCHK_A(hipMemcpyAsync(ctx->sc, A->inter_sc + s, sizeof(Scalars), hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    k_inter<<<grid, MCM_BLOCK, 0, ctx->stream>>>(ctx->d, A->lay, A->inter_d + (size_t)s * A->inter_dn,
+                                                 A->inter_i + (size_t)s * A->inter_in, save ? 1 : 0);
+    HIPCHK_A(hipGetLastError());
+    return 0;
+}
+
 // the reference's call sequence of one step through the granular stages,
-// which leave the step's intermediates on the device
+// which leave the step's intermediates on the device; the neighbour search
+// is launched without the capacity round trip (k_adj_check reports it)
 int adj_granular_step(mcm_gpu_adjoint* A) {
     mcm_gpu_ctx* ctx = A->ctx;
     int rc = 0;
-    if (!rc) rc = mcm_gpu_neighbours(ctx);
+    launch_neighbours(one(ctx), ctx->cur);
+    if (hipGetLastError() != hipSuccess) { A->err = "neighbour launch failed"; return MCM_GPU_ERR_HIP; }
+    k_adj_check<<<1, 1, 0, ctx->stream>>>(ctx->d, A->a);
     if (!rc) rc = mcm_gpu_strain(ctx);
     if (!rc) rc = mcm_gpu_update_h(ctx);
     if (!rc) rc = mcm_gpu_constitutive(ctx);
@@ -816,32 +1054,24 @@ int adj_reverse_step(mcm_gpu_adjoint* A, bool ck, int s) {
     a.snap = slot_ptr(A, ck, s);
     a.cur = (ck ? A->ckpt_cur : A->seg_cur)[s];
     const int nb = ctx->nb, gb = ghost_blocks(d);
-    for (int attempt = 0; attempt < 8; attempt++) {
-        HIPCHK_A(hipMemsetAsync(a.gt_cnt, 0, sizeof(int) * a.ngcap, st));
-        HIPCHK_A(hipMemsetAsync(a.gof_cnt, 0, sizeof(int) * A->np, st));
-        HIPCHK_A(hipMemsetAsync(a.err, 0, sizeof(int), st));
-        k_adj_gt_fill<<<nb, MCM_BLOCK, 0, st>>>(d, a);
-        k_adj_gof_fill<<<gb, MCM_BLOCK, 0, st>>>(d, a);
-        int herr = 0;
-        HIPCHK_A(hipMemcpyAsync(&herr, a.err, sizeof(int), hipMemcpyDeviceToHost, st));
-        HIPCHK_A(hipStreamSynchronize(st));
-        if (!herr) break;
-        if (attempt == 7) { A->err = "transposed ghost lists overflowed"; return MCM_GPU_ERR_CAPACITY; }
-        a.gtcap = a.gtcap * 2;   // more real partners of one ghost than slots
-        for (void* p : A->gallocs) (void)hipFree(p);
-        A->gallocs.clear();
-        if ((rc = adj_ghost_buffers(A))) return rc;
-    }
-    k_adj_gt_sort<<<gb, MCM_BLOCK, 0, st>>>(d, a);
+    const int wb = blocks_for(A->np, MCM_BLOCK / 64);                                    // wave per particle
+    const int wg = std::max(1, std::min(blocks_for(d.ngcap, MCM_BLOCK / 64), 4096));   // wave per ghost
+    // transposed lists; an overflow sets err bit 1 (checked after the sweep)
+    HIPCHK_A(hipMemsetAsync(a.gt_cnt, 0, sizeof(int) * a.ngcap, st));
+    HIPCHK_A(hipMemsetAsync(a.gof_cnt, 0, sizeof(int) * A->np, st));
+    k_adj_gt_fill<<<nb, MCM_BLOCK, 0, st>>>(d, a);
+    k_adj_gof_fill<<<gb, MCM_BLOCK, 0, st>>>(d, a);
+    k_adj_gt_sort<<<wg, MCM_BLOCK, 0, st>>>(d, a);
     k_adj_gof_sort<<<nb, MCM_BLOCK, 0, st>>>(d, a);
     k_adj_move<<<nb, MCM_BLOCK, 0, st>>>(d, a);
     k_adj_fold_move<<<1, MCM_BLOCK, 0, st>>>(d, a, nb);
-    k_adj_mom<<<nb, MCM_BLOCK, 0, st>>>(d, a);
-    k_adj_mom_ghost<<<gb, MCM_BLOCK, 0, st>>>(d, a);
+    k_adj_mom<<<wb, MCM_BLOCK, 0, st>>>(d, a);
+    k_adj_mom_ghost<<<wg, MCM_BLOCK, 0, st>>>(d, a);
     k_adj_gmap_mom<<<nb, MCM_BLOCK, 0, st>>>(d, a);
-    k_adj_const<<<nb, MCM_BLOCK, 0, st>>>(d, a);
-    k_adj_strain<<<nb, MCM_BLOCK, 0, st>>>(d, a);
-    k_adj_strain_ghost<<<gb, MCM_BLOCK, 0, st>>>(d, a);
+    k_adj_const<<<dim3(nb, ADJ_NPASS), MCM_BLOCK, 0, st>>>(d, a);
+    k_adj_const_fin<<<nb, MCM_BLOCK, 0, st>>>(d, a);
+    k_adj_strain<<<wb, MCM_BLOCK, 0, st>>>(d, a);
+    k_adj_strain_ghost<<<wg, MCM_BLOCK, 0, st>>>(d, a);
     k_adj_gmap_strain<<<nb, MCM_BLOCK, 0, st>>>(d, a);
     k_adj_fold_end<<<1, MCM_BLOCK, 0, st>>>(d, a, nb);
     HIPCHK_A(hipGetLastError());
@@ -901,6 +1131,9 @@ void adj_initial_params(mcm_gpu_adjoint* A, const std::vector<double> sb[6], con
                 v0bar[3 * imin + c] += dtb * Dk.tssf * cbest.d[ADJ_NPAR] * v0[3 * imin + c] / va;
     }
 }
+
+static int adj_sweep(mcm_gpu_adjoint* A, const double* xbar, const double* vbar, const double* histbar,
+                     double* v0bar, double* pbar, int* flags);
 
 }  // namespace
 
@@ -968,10 +1201,8 @@ int mcm_gpu_adj_forward(mcm_gpu_ctx* ctx, int32_t nsteps, int32_t ckpt_every, co
     int rc = 0;
     rc |= adj_alloc(A, A->allocs, &A->ckpt, slot * A->nseg);
     rc |= adj_alloc(A, A->allocs, &A->ckpt_sc, A->nseg);
-    if (ckpt_every > 1) {
-        rc |= adj_alloc(A, A->allocs, &A->seg, slot * ckpt_every);
-        rc |= adj_alloc(A, A->allocs, &A->seg_sc, ckpt_every);
-    }
+    rc |= adj_alloc(A, A->allocs, &A->seg, slot * ckpt_every);
+    rc |= adj_alloc(A, A->allocs, &A->seg_sc, ckpt_every);
     rc |= adj_alloc(A, A->allocs, &A->d_sample, std::max(nsample, 1));
     rc |= adj_alloc(A, A->allocs, &A->d_hist, (size_t)nsteps * 3 * std::max(nsample, 1));
     AdjDev& a = A->a;
@@ -979,15 +1210,17 @@ int mcm_gpu_adj_forward(mcm_gpu_ctx* ctx, int32_t nsteps, int32_t ckpt_every, co
     for (int c = 0; c < 3; c++) {
         rc |= adj_alloc(A, A->allocs, &a.xb[c], n);
         rc |= adj_alloc(A, A->allocs, &a.vb[c], n);
-        rc |= adj_alloc(A, A->allocs, &a.ab[c], n);
     }
+    rc |= adj_alloc(A, A->allocs, &a.abp, 4 * n);
     for (int c = 0; c < 6; c++) rc |= adj_alloc(A, A->allocs, &a.sb[c], n);
-    for (int c = 0; c < 9; c++) rc |= adj_alloc(A, A->allocs, &a.lb[c], n);
+    rc |= adj_alloc(A, A->allocs, &a.lbp, 10 * n);
+    rc |= adj_alloc(A, A->allocs, &a.xin, n * ADJ_NIN);
     rc |= adj_alloc(A, A->allocs, &a.rb, n);
     rc |= adj_alloc(A, A->allocs, &a.eb, n);
     rc |= adj_alloc(A, A->allocs, &a.fb, n);
     rc |= adj_alloc(A, A->allocs, &a.qb, n);
     rc |= adj_alloc(A, A->allocs, &a.rbn, n);
+    rc |= adj_alloc(A, A->allocs, &a.pdt, n);
     rc |= adj_alloc(A, A->allocs, &a.gof_cnt, n);
     rc |= adj_alloc(A, A->allocs, &a.gof, n * ADJ_GOF);
     rc |= adj_alloc(A, A->allocs, &a.red, (size_t)ctx->nb * RA_N + 64);
@@ -1015,6 +1248,27 @@ int mcm_gpu_adj_forward(mcm_gpu_ctx* ctx, int32_t nsteps, int32_t ckpt_every, co
 int mcm_gpu_adj_backward(mcm_gpu_adjoint* A, const double* xbar, const double* vbar, const double* histbar,
                          double* v0bar, double* pbar) {
     if (!A || !v0bar || !pbar) return MCM_GPU_ERR_ARG;
+    for (int attempt = 0; attempt < 4; attempt++) {
+        int flags = 0;
+        const int rc = adj_sweep(A, xbar, vbar, histbar, v0bar, pbar, &flags);
+        if (rc) return rc;
+        if (flags & 2) { A->err = "a recomputed step outgrew a capacity"; return MCM_GPU_ERR_CAPACITY; }
+        if (!(flags & 1)) return MCM_GPU_OK;
+        // more real partners of one ghost than transposed slots: sweep again
+        A->a.gtcap *= 2;
+        for (void* p : A->gallocs) (void)hipFree(p);
+        A->gallocs.clear();
+    }
+    A->err = "transposed ghost lists overflowed";
+    return MCM_GPU_ERR_CAPACITY;
+}
+
+}  // extern "C"
+
+namespace {
+
+static int adj_sweep(mcm_gpu_adjoint* A, const double* xbar, const double* vbar, const double* histbar, double* v0bar,
+              double* pbar, int* flags) {
     mcm_gpu_ctx* ctx = A->ctx;
     hipStream_t st = ctx->stream;
     AdjDev& a = A->a;
@@ -1033,30 +1287,35 @@ int mcm_gpu_adj_backward(mcm_gpu_adjoint* A, const double* xbar, const double* v
     double* zero[] = {a.rb, a.eb, a.fb, a.qb, a.rbn, a.sb[0], a.sb[1], a.sb[2], a.sb[3], a.sb[4], a.sb[5]};
     for (double* z : zero) HIPCHK_A(hipMemsetAsync(z, 0, sizeof(double) * np, st));
     HIPCHK_A(hipMemsetAsync(a.as, 0, sizeof(double) * AS_N, st));
+    HIPCHK_A(hipMemsetAsync(a.err, 0, sizeof(int), st));
     const bool seeds = histbar && A->nsample > 0;
     if (seeds)
         HIPCHK_A(hipMemcpyAsync(A->d_hist, histbar, sizeof(double) * 3 * A->nsample * (size_t)A->nsteps,
                                 hipMemcpyHostToDevice, st));
+    if ((rc = adj_inter_buffers(A))) return rc;
+    if ((rc = adj_ghost_buffers(A))) return rc;
     for (int sgi = A->nseg - 1; sgi >= 0; sgi--) {
         const int s0 = sgi * A->k, s1 = std::min(s0 + A->k, A->nsteps);
-        const bool ck = A->k == 1;
-        if (!ck) {   // recompute the segment, keeping the start of every step
-            if ((rc = adj_restore(A, true, sgi))) return rc;
-            if ((rc = adj_steps(A, s1 - s0 - 1, nullptr, true, sgi, true))) return rc;
-            if ((rc = adj_save(A, false, s1 - s0 - 1))) return rc;
+        // recompute the segment through the granular stages, keeping the
+        // start of every step and the step's intermediates
+        if ((rc = adj_restore(A, true, sgi))) return rc;
+        for (int n = s0; n < s1; n++) {
+            if ((rc = adj_save(A, false, n - s0))) return rc;
+            if ((rc = adj_granular_step(A))) return rc;
+            if ((rc = adj_inter(A, n - s0, true))) return rc;
         }
         for (int n = s1 - 1; n >= s0; n--) {
-            const int s = ck ? n : n - s0;
-            if ((rc = adj_restore(A, ck, s))) return rc;
+            if ((rc = adj_inter(A, n - s0, false))) return rc;
             if (seeds)
                 k_adj_seed<<<blocks_for(A->nsample, 64), 64, 0, st>>>(a, A->d_sample, A->nsample,
                                                                       A->d_hist + (size_t)n * 3 * A->nsample);
-            if ((rc = adj_granular_step(A))) return rc;
-            if ((rc = adj_reverse_step(A, ck, s))) return rc;
+            if ((rc = adj_reverse_step(A, false, n - s0))) return rc;
         }
     }
     // results
+    HIPCHK_A(hipMemcpyAsync(flags, a.err, sizeof(int), hipMemcpyDeviceToHost, st));
     HIPCHK_A(hipStreamSynchronize(st));
+    if (*flags) return MCM_GPU_OK;
     std::vector<double> vb[3], sbh[6], rbh(np), ebh(np);
     for (int c = 0; c < 3; c++) {
         vb[c].resize(np);
@@ -1094,6 +1353,10 @@ int mcm_gpu_adj_backward(mcm_gpu_adjoint* A, const double* xbar, const double* v
     return MCM_GPU_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
 const char* mcm_gpu_adj_last_error(const mcm_gpu_adjoint* A) { return A ? A->err.c_str() : "null adjoint"; }
 
 void mcm_gpu_adj_destroy(mcm_gpu_adjoint* A) {
@@ -1101,6 +1364,7 @@ void mcm_gpu_adj_destroy(mcm_gpu_adjoint* A) {
     if (A->ctx && A->ctx->stream) (void)hipStreamSynchronize(A->ctx->stream);
     for (void* p : A->allocs) (void)hipFree(p);
     for (void* p : A->gallocs) (void)hipFree(p);
+    for (void* p : A->iallocs) (void)hipFree(p);
     delete A;
 }
 
