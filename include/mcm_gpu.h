@@ -222,20 +222,22 @@ int mcm_gpu_get_material_params(mcm_gpu_ctx *ctx, int32_t mat, double *params);
 /* Replace the deck's initial velocities (n = 3 np, particle-major, deck
    order); takes effect at the next mcm_gpu_initial. */
 int mcm_gpu_set_initial_velocity(mcm_gpu_ctx *ctx, const double *v, int64_t n);
-/* nsteps fused steps from the run's current state (normally right after
+/* nsteps steps from the run's current state (normally right after
    mcm_gpu_initial), keeping a state checkpoint every ckpt_every steps on the
    device and writing the positions of the nsample listed particles (distinct
-   0-based ids) after every step to hist (nsteps x nsample x 3, host).  The
-   run's state afterwards is the end of the recorded run. */
+   0-based ids) after every step to hist (nsteps x nsample x 3, host).  With
+   ckpt_every 1 every step's state and intermediates are kept (about 60 MB
+   per step per 50k particles) and the reverse sweep recomputes nothing;
+   otherwise the steps are fused and each segment is recomputed once in the
+   reverse sweep.  The run's state afterwards is the end of the recorded run;
+   do not step the run again before mcm_gpu_adj_backward. */
 int mcm_gpu_adj_forward(mcm_gpu_ctx *ctx, int32_t nsteps, int32_t ckpt_every, const int32_t *sample,
                         int32_t nsample, double *hist, mcm_gpu_adjoint **out);
 /* Reverse sweep of a recorded run.  In: dL/dx and dL/dv of the final state
    (3 np each, particle-major) and dL/dhist (nsteps x nsample x 3); any may
    be NULL for zero.  Out: dL/dv0 (3 np) of the deck's initial velocities
-   and dL/dparams (4 per material).  Each segment of ckpt_every steps is
-   recomputed from its checkpoint and each step once more through the
-   granular stages; the run's state afterwards is unspecified (call
-   mcm_gpu_initial before stepping it again). */
+   and dL/dparams (4 per material).  The run's state afterwards is
+   unspecified (call mcm_gpu_initial before stepping it again). */
 int mcm_gpu_adj_backward(mcm_gpu_adjoint *a, const double *xbar, const double *vbar, const double *histbar,
                          double *v0bar, double *pbar);
 const char *mcm_gpu_adj_last_error(const mcm_gpu_adjoint *a);

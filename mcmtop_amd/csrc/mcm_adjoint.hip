@@ -1,6 +1,8 @@
 // mcm_adjoint.hip -- reverse-mode derivative of the plain-SPH MCM step on gfx950.
 //
-// A forward run records a state snapshot every k steps.  The reverse sweep
+// A forward run records a state snapshot every k steps (k = 1: the
+// recording runs the granular stages and keeps every step's start state and
+// intermediates, and the reverse sweep recomputes nothing).  The reverse sweep
 // walks the segments backwards.  Each segment is recomputed from its
 // checkpoint through the granular stage kernels, which leave every
 // intermediate the reverse needs on the device (lists, ghosts, gather
@@ -869,6 +871,8 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_inter(DevState d, InterLayout L, 
 struct mcm_gpu_adjoint {
     mcm_gpu_ctx* ctx = nullptr;
     int nsteps = 0, k = 1, nseg = 0, nsample = 0;
+    bool store = false;               // k == 1: every step's state and intermediates kept from the recording
+    int ninter = 0;                   // intermediate slots (k, or nsteps when store)
     long np = 0;
     double* ckpt = nullptr;           // [nseg][SN_N][np]: state at the start of each segment
     Scalars* ckpt_sc = nullptr;       // [nseg]
@@ -1001,9 +1005,9 @@ int adj_inter_buffers(mcm_gpu_adjoint* A) {
     A->inter_dn = (size_t)(22 * ucap + 13 * (long)d.np);
     A->inter_in = (size_t)(nbrlen + 2 * (long)d.np + 2 * (long)d.ngcap);
     int rc = 0;
-    rc |= adj_alloc(A, A->iallocs, &A->inter_d, A->inter_dn * A->k);
-    rc |= adj_alloc(A, A->iallocs, &A->inter_i, A->inter_in * A->k);
-    rc |= adj_alloc(A, A->iallocs, &A->inter_sc, A->k);
+    rc |= adj_alloc(A, A->iallocs, &A->inter_d, A->inter_dn * A->ninter);
+    rc |= adj_alloc(A, A->iallocs, &A->inter_i, A->inter_in * A->ninter);
+    rc |= adj_alloc(A, A->iallocs, &A->inter_sc, A->ninter);
     return rc;
 }
 
@@ -1022,15 +1026,10 @@ int adj_inter(mcm_gpu_adjoint* A, int s, bool save) {
     return 0;
 }
 
-// the reference's call sequence of one step through the granular stages,
-// which leave the step's intermediates on the device; the neighbour search
-// is launched without the capacity round trip (k_adj_check reports it)
-int adj_granular_step(mcm_gpu_adjoint* A) {
+// the stages after the neighbour search, in the reference's call sequence
+int adj_granular_rest(mcm_gpu_adjoint* A) {
     mcm_gpu_ctx* ctx = A->ctx;
     int rc = 0;
-    launch_neighbours(one(ctx), ctx->cur);
-    if (hipGetLastError() != hipSuccess) { A->err = "neighbour launch failed"; return MCM_GPU_ERR_HIP; }
-    k_adj_check<<<1, 1, 0, ctx->stream>>>(ctx->d, A->a);
     if (!rc) rc = mcm_gpu_strain(ctx);
     if (!rc) rc = mcm_gpu_update_h(ctx);
     if (!rc) rc = mcm_gpu_constitutive(ctx);
@@ -1041,6 +1040,49 @@ int adj_granular_step(mcm_gpu_adjoint* A) {
     if (!rc) rc = mcm_gpu_move(ctx);
     if (rc) A->err = ctx->err;
     return rc;
+}
+
+// one step through the granular stages, which leave the step's intermediates
+// on the device; the neighbour search is launched without the capacity round
+// trip (a recomputed step cannot outgrow them; k_adj_check reports it)
+int adj_granular_step(mcm_gpu_adjoint* A) {
+    mcm_gpu_ctx* ctx = A->ctx;
+    launch_neighbours(one(ctx), ctx->cur);
+    if (hipGetLastError() != hipSuccess) { A->err = "neighbour launch failed"; return MCM_GPU_ERR_HIP; }
+    k_adj_check<<<1, 1, 0, ctx->stream>>>(ctx->d, A->a);
+    return adj_granular_rest(A);
+}
+
+// recording with every step kept: each step runs through the granular stages
+// (the neighbour search with its capacity check) and its start state and
+// intermediates are saved; a capacity that grows changes the slot layout, so
+// the recording starts again from the initial state
+int adj_record_store(mcm_gpu_adjoint* A) {
+    mcm_gpu_ctx* ctx = A->ctx;
+    const DevState& d = ctx->d;
+    int rc = 0;
+    for (int attempt = 0; attempt < 8; attempt++) {
+        if (attempt > 0 && (rc = adj_restore(A, true, 0))) return rc;
+        if ((rc = adj_inter_buffers(A))) return rc;
+        bool grown = false;
+        for (int n = 0; n < A->nsteps && !grown; n++) {
+            if ((rc = adj_save(A, true, n))) return rc;
+            const int ncap = d.ncap, ngcap = d.ngcap, cellcap = d.cellcap;
+            if ((rc = mcm_gpu_neighbours(ctx))) { A->err = ctx->err; return rc; }
+            if (d.ncap != ncap || d.ngcap != ngcap || d.cellcap != cellcap) { grown = true; break; }
+            if ((rc = adj_granular_rest(A))) return rc;
+            if ((rc = adj_inter(A, n, true))) return rc;
+            if (A->nsample > 0)
+                k_adj_sample<<<blocks_for(A->nsample, 64), 64, 0, ctx->stream>>>(
+                    ctx->d, A->d_sample, A->nsample, A->d_hist + (size_t)n * 3 * A->nsample);
+        }
+        if (!grown) {
+            HIPCHK_A(hipGetLastError());
+            return 0;
+        }
+    }
+    A->err = "recording did not fit after regrowing";
+    return MCM_GPU_ERR_CAPACITY;
 }
 
 // reverse of the step just recomputed (start-of-step state in slot (ck, s))
@@ -1193,6 +1235,8 @@ int mcm_gpu_adj_forward(mcm_gpu_ctx* ctx, int32_t nsteps, int32_t ckpt_every, co
     A->nsteps = nsteps;
     A->k = ckpt_every;
     A->nseg = (nsteps + ckpt_every - 1) / ckpt_every;
+    A->store = ckpt_every == 1;
+    A->ninter = A->store ? nsteps : ckpt_every;
     A->nsample = nsample;
     A->np = d.np;
     A->ckpt_cur.assign(A->nseg, 0);
@@ -1201,7 +1245,7 @@ int mcm_gpu_adj_forward(mcm_gpu_ctx* ctx, int32_t nsteps, int32_t ckpt_every, co
     int rc = 0;
     rc |= adj_alloc(A, A->allocs, &A->ckpt, slot * A->nseg);
     rc |= adj_alloc(A, A->allocs, &A->ckpt_sc, A->nseg);
-    rc |= adj_alloc(A, A->allocs, &A->seg, slot * ckpt_every);
+    rc |= adj_alloc(A, A->allocs, &A->seg, A->store ? 1 : slot * ckpt_every);
     rc |= adj_alloc(A, A->allocs, &A->seg_sc, ckpt_every);
     rc |= adj_alloc(A, A->allocs, &A->d_sample, std::max(nsample, 1));
     rc |= adj_alloc(A, A->allocs, &A->d_hist, (size_t)nsteps * 3 * std::max(nsample, 1));
@@ -1231,12 +1275,16 @@ int mcm_gpu_adj_forward(mcm_gpu_ctx* ctx, int32_t nsteps, int32_t ckpt_every, co
     a.ngcap = 0;
     if (nsample > 0)
         HIPCHK_A(hipMemcpyAsync(A->d_sample, sample, sizeof(int) * nsample, hipMemcpyHostToDevice, ctx->stream));
-    for (int sgi = 0; sgi < A->nseg; sgi++) {
-        const int s0 = sgi * A->k, s1 = std::min(s0 + A->k, nsteps);
-        if ((rc = adj_save(A, true, sgi))) return rc;
-        if ((rc = adj_steps(A, s1 - s0, nsample > 0 ? A->d_hist + (size_t)s0 * 3 * nsample : nullptr, true, sgi,
-                            false)))
-            return rc;
+    if (A->store) {
+        if ((rc = adj_record_store(A))) return rc;
+    } else {
+        for (int sgi = 0; sgi < A->nseg; sgi++) {
+            const int s0 = sgi * A->k, s1 = std::min(s0 + A->k, nsteps);
+            if ((rc = adj_save(A, true, sgi))) return rc;
+            if ((rc = adj_steps(A, s1 - s0, nsample > 0 ? A->d_hist + (size_t)s0 * 3 * nsample : nullptr, true,
+                                sgi, false)))
+                return rc;
+        }
     }
     if (nsample > 0)
         HIPCHK_A(hipMemcpyAsync(hist, A->d_hist, sizeof(double) * 3 * nsample * (size_t)nsteps,
@@ -1292,9 +1340,22 @@ static int adj_sweep(mcm_gpu_adjoint* A, const double* xbar, const double* vbar,
     if (seeds)
         HIPCHK_A(hipMemcpyAsync(A->d_hist, histbar, sizeof(double) * 3 * A->nsample * (size_t)A->nsteps,
                                 hipMemcpyHostToDevice, st));
+    if (A->store && (A->lay.ucap != (long)ctx->d.np + ctx->d.ngcap || A->lay.nbrlen != (long)ctx->d.npad * ctx->d.ncap)) {
+        A->err = "the run's capacities changed after the recording";
+        return MCM_GPU_ERR_ARG;
+    }
     if ((rc = adj_inter_buffers(A))) return rc;
     if ((rc = adj_ghost_buffers(A))) return rc;
-    for (int sgi = A->nseg - 1; sgi >= 0; sgi--) {
+    if (A->store) {   // every step's intermediates were kept by the recording
+        for (int n = A->nsteps - 1; n >= 0; n--) {
+            if ((rc = adj_inter(A, n, false))) return rc;
+            if (seeds)
+                k_adj_seed<<<blocks_for(A->nsample, 64), 64, 0, st>>>(a, A->d_sample, A->nsample,
+                                                                      A->d_hist + (size_t)n * 3 * A->nsample);
+            if ((rc = adj_reverse_step(A, true, n))) return rc;
+        }
+    }
+    for (int sgi = A->store ? -1 : A->nseg - 1; sgi >= 0; sgi--) {
         const int s0 = sgi * A->k, s1 = std::min(s0 + A->k, A->nsteps);
         // recompute the segment through the granular stages, keeping the
         // start of every step and the step's intermediates
