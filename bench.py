@@ -238,14 +238,17 @@ def adjoint_leg(steps: int, ckpt: int, dx: float, nsample: int, fd: bool):
             with torch.no_grad():
                 xx, hh = ident.run(torch.tensor(th_), torch.from_numpy(v_))
                 return adjoint.identification_loss(xx, hh, ident.x0s, ident.target).item()
-        rel = {}
+        rel, spread = {}, {}
         for k, name in enumerate(("E", "sigma_y", "E_tan", "S1")):
-            h = (1e-5 if name == "E_tan" else 1e-6) * theta[k]
-            tp, tm = theta.copy(), theta.copy()
-            tp[k] += h
-            tm[k] -= h
-            f = (loss_at(tp, dk.v) - loss_at(tm, dk.v)) / (tp[k] - tm[k])
-            rel[name] = abs(g_theta[k] - f) / abs(f)
+            fds = {}
+            for step in (1e-5, 1e-6, 1e-7):
+                h = step * theta[k]
+                tp, tm = theta.copy(), theta.copy()
+                tp[k] += h
+                tm[k] -= h
+                fds[step] = (loss_at(tp, dk.v) - loss_at(tm, dk.v)) / (tp[k] - tm[k])
+            rel[name] = abs(g_theta[k] - fds[1e-6]) / abs(fds[1e-6])
+            spread[name] = (max(fds.values()) - min(fds.values())) / abs(fds[1e-6])
         for f_ in np.argsort(-np.abs(g_v).ravel())[:2]:
             i, c = divmod(int(f_), 3)
             vp, vm = dk.v.copy(), dk.v.copy()
@@ -254,7 +257,10 @@ def adjoint_leg(steps: int, ckpt: int, dx: float, nsample: int, fd: bool):
             f = (loss_at(theta, vp) - loss_at(theta, vm)) / 2e-3
             rel["v0[%d,%d]" % (i, c)] = abs(g_v[i, c] - f) / abs(f)
         out["fd_rel_error"] = rel
-        out["fd"] = "central differences of the GPU forward; relative step 1e-6 (E_tan 1e-5), velocity step 1e-3 m/s"
+        out["fd_spread"] = spread
+        out["fd"] = ("central differences of the GPU forward, relative step 1e-6, velocity step 1e-3 m/s; "
+                     "fd_spread: relative spread of the differences at steps 1e-5, 1e-6, 1e-7 (kinks of the "
+                     "800-step loss: yield transitions, min/max switches)")
     ident.close()
     return out
 

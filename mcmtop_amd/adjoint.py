@@ -45,10 +45,16 @@ class StepRun(torch.autograd.Function):
             solver.set_material_params(m, p[m])
         solver.set_initial_velocity(v0.detach().cpu().numpy())
         solver.initial()
-        rec = solver.record(nsteps, ckpt_every, sample)
-        ctx.rec = rec
+        need = any(ctx.needs_input_grad[:2])
+        # without a gradient to take, one checkpoint and nothing kept
+        rec = solver.record(nsteps, ckpt_every if need else nsteps, sample)
         x = torch.from_numpy(solver.field(gpu.F_X))
-        return x, torch.from_numpy(rec.hist.copy())
+        hist = torch.from_numpy(rec.hist.copy())
+        if need:
+            ctx.rec = rec
+        else:
+            rec.close()
+        return x, hist
 
     @staticmethod
     def backward(ctx, gx, ghist):
@@ -122,7 +128,7 @@ class Identification:
         v = torch.from_numpy(self.v0 if v0 is None else np.asarray(v0, dtype=np.float64)).clone().requires_grad_(True)
         L = self.loss(th, v)
         L.backward()
-        return float(L), th.grad.numpy().copy(), v.grad.numpy().copy()
+        return L.item(), th.grad.numpy().copy(), v.grad.numpy().copy()
 
     def close(self):
         self.solver.close()
