@@ -265,6 +265,56 @@ def adjoint_leg(steps: int, ckpt: int, dx: float, nsample: int, fd: bool):
     return out
 
 
+def ident_leg(ninst: int, steps: int, iters: int, threads: int, seed: int):
+    """Batched identification, reduced: ninst independent bars of uneven
+    size and material (deck.batch_bars: plain SPH, f3dm10 + eqos4, fixed h),
+    each with its own hidden (E, sigma_y) and S1 = 1.49, fitted from seeded
+    start points 10-20 % off by Adam on the logarithms over (E, sigma_y, S1),
+    every gradient by the device reverse sweep with every step kept; the
+    instances run concurrently (one host thread and HIP stream each).  One
+    iteration is also timed with the instances one after another."""
+    import numpy as np
+    from mcmtop_amd import adjoint, deck
+    decks, params = deck.batch_bars(ninst, seed=seed, nmin=15000, nmax=40000, corrected=False, h_opt=0,
+                                    dx_range=(0.15e-3, 0.3e-3))
+    rng = np.random.default_rng(seed + 1)
+    idents, starts, hidden = [], [], []
+    for dk, pr in zip(decks, params):
+        dk.v = dk.v * (1.0 + rng.uniform(-0.005, 0.005, size=dk.v.shape))
+        sample = adjoint.surface_sample(dk.x, pr["radius"], pr["dx"], 32, seed=int(rng.integers(1 << 30)))
+        idn = adjoint.Identification(dk, steps, 1, sample, 0.35)
+        th = np.array([pr["E"], pr["sigma_y"], 100e6, 1.49])
+        idn.make_target(th)
+        idents.append(idn)
+        hidden.append(th)
+        starts.append(th * np.array([rng.uniform(0.8, 1.2), rng.uniform(0.8, 1.2), 1.0, rng.uniform(0.9, 1.1)]))
+    adjoint.fit_many(idents, starts, 1, threads=threads)    # buffers allocated once per run
+    t0 = time.perf_counter()
+    adjoint.fit_many(idents, starts, 1, threads=1)
+    t_seq = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    res = adjoint.fit_many(idents, starts, iters, lr=0.03, threads=threads)
+    t_conc = (time.perf_counter() - t0) / iters
+    for idn in idents:
+        idn.close()
+    total = int(sum(p["np"] for p in params))
+    ratio = [r["loss"][-1] / r["loss"][0] for r in res]
+    names = {"E": 0, "sigma_y": 1, "S1": 3}
+    perr = {nm: {"start_median": float(np.median([abs(st[k] / h[k] - 1) for st, h in zip(starts, hidden)])),
+                 "after_median": float(np.median([abs(r["final"][k] / h[k] - 1) for r, h in zip(res, hidden)]))}
+            for nm, k in names.items()}
+    print("bench: identification %d bars (%d particles), %d steps: %.2f s per iteration concurrent, %.2f s one "
+          "after another" % (ninst, total, steps, t_conc, t_seq), file=sys.stderr, flush=True)
+    return {"instances": ninst, "particles": total, "steps": steps, "iterations": iters, "threads": threads,
+            "s_per_iteration_concurrent": t_conc, "s_per_iteration_one_by_one": t_seq,
+            "speedup": t_seq / t_conc,
+            "gradient_particle_steps_per_s": total * steps / t_conc,
+            "loss_ratio_after": {"median": float(np.median(ratio)), "max": float(np.max(ratio))},
+            "param_rel_error": perr,
+            "workload": "batch_bars(%d, seed=%d), 15k-40k particles, plain SPH, f3dm10 + eqos4, tcrit_opt 1, "
+                        "fit (E, sigma_y, S1) by Adam (lr 0.03) on the logarithms" % (ninst, seed)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -281,6 +331,10 @@ def main():
     ap.add_argument("--adjoint-ckpt", type=int, default=50)
     ap.add_argument("--adjoint-dx", type=float, default=2.54e-4)
     ap.add_argument("--adjoint-fd", type=int, default=1, help="1: check the gradients by central differences")
+    ap.add_argument("--ident", type=int, default=16, help="bars of the batched identification leg (0: skip it)")
+    ap.add_argument("--ident-steps", type=int, default=200)
+    ap.add_argument("--ident-iters", type=int, default=10)
+    ap.add_argument("--ident-threads", type=int, default=8)
     ap.add_argument("--corrected-one-pass", action="store_true",
                     help="accumulate the corrected kernels' three passes in one (rounding differs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -363,6 +417,8 @@ def main():
     if args.batch > 0:
         res["batched"] = batched_leg(args.batch, args.batch_seed, args.batch_steps, 4, 16, bool(args.batch_plain))
     res["adjoint_forward_ratio"] = None
+    if args.ident > 0:
+        res["identification"] = ident_leg(args.ident, args.ident_steps, args.ident_iters, args.ident_threads, 5)
     if args.adjoint_steps > 0:
         res["adjoint"] = adjoint_leg(args.adjoint_steps, args.adjoint_ckpt, args.adjoint_dx, 32, bool(args.adjoint_fd))
         res["adjoint_forward_ratio"] = res["adjoint"]["ratio_reverse"]

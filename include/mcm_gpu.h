@@ -230,7 +230,10 @@ int mcm_gpu_set_initial_velocity(mcm_gpu_ctx *ctx, const double *v, int64_t n);
    per step per 50k particles) and the reverse sweep recomputes nothing;
    otherwise the steps are fused and each segment is recomputed once in the
    reverse sweep.  The run's state afterwards is the end of the recorded run;
-   do not step the run again before mcm_gpu_adj_backward. */
+   do not step the run again before mcm_gpu_adj_backward.  *out is NULL, or
+   a recording of this run to replace: with the same nsteps, ckpt_every,
+   nsample its device buffers are reused (no allocation or free), otherwise
+   it is destroyed. */
 int mcm_gpu_adj_forward(mcm_gpu_ctx *ctx, int32_t nsteps, int32_t ckpt_every, const int32_t *sample,
                         int32_t nsample, double *hist, mcm_gpu_adjoint **out);
 /* Reverse sweep of a recorded run.  In: dL/dx and dL/dv of the final state

@@ -132,3 +132,51 @@ class Identification:
 
     def close(self):
         self.solver.close()
+
+
+class Adam:
+    """Adam steps on a parameter vector (numpy)."""
+
+    def __init__(self, n: int, lr: float, b1: float = 0.9, b2: float = 0.999, eps: float = 1e-12):
+        self.lr, self.b1, self.b2, self.eps = lr, b1, b2, eps
+        self.m = np.zeros(n)
+        self.v = np.zeros(n)
+        self.t = 0
+
+    def step(self, x: np.ndarray, g: np.ndarray) -> np.ndarray:
+        self.t += 1
+        self.m = self.b1 * self.m + (1 - self.b1) * g
+        self.v = self.b2 * self.v + (1 - self.b2) * g * g
+        mh = self.m / (1 - self.b1 ** self.t)
+        vh = self.v / (1 - self.b2 ** self.t)
+        return x - self.lr * mh / (np.sqrt(vh) + self.eps)
+
+
+def fit(ident: Identification, theta0, iterations: int, lr: float = 0.05, fitted=(True, True, False, True)):
+    """Gradient descent (Adam on the logarithms) of the identification loss
+    over the fitted parameters of (E, sigma_y, E_tan, S1), from theta0.
+    Returns {"loss": [...], "theta": [...]} per iteration (the loss and
+    parameters at which each gradient was taken) and the final parameters."""
+    mask = np.asarray(fitted, dtype=bool)
+    theta = np.asarray(theta0, dtype=np.float64).copy()
+    opt = Adam(int(mask.sum()), lr)
+    hist = {"loss": [], "theta": []}
+    for _ in range(int(iterations)):
+        L, g, _ = ident.value_and_grad(theta)
+        hist["loss"].append(L)
+        hist["theta"].append(theta.copy())
+        u = np.log(theta[mask])
+        u = opt.step(u, g[mask] * theta[mask])   # dL/dlog(theta) = theta dL/dtheta
+        theta[mask] = np.exp(u)
+    hist["final"] = theta
+    return hist
+
+
+def fit_many(idents, theta0s, iterations: int, lr: float = 0.05, threads: int = 8, fitted=(True, True, False, True)):
+    """fit() for several independent instances at once: one host thread per
+    instance, each run on its own HIP stream, so the GPU overlaps the
+    instances' kernels (the C ABI releases the interpreter during its calls)."""
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=max(1, int(threads))) as ex:
+        futs = [ex.submit(fit, idn, th, iterations, lr, fitted) for idn, th in zip(idents, theta0s)]
+        return [f.result() for f in futs]

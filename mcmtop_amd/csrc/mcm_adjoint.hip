@@ -1177,6 +1177,30 @@ void adj_initial_params(mcm_gpu_adjoint* A, const std::vector<double> sb[6], con
 static int adj_sweep(mcm_gpu_adjoint* A, const double* xbar, const double* vbar, const double* histbar,
                      double* v0bar, double* pbar, int* flags);
 
+// the recording itself (buffers allocated): steps from the run's current
+// state, checkpoints, sampled positions to hist
+int adj_record(mcm_gpu_adjoint* A, double* hist) {
+    mcm_gpu_ctx* ctx = A->ctx;
+    const int nsteps = A->nsteps, nsample = A->nsample;
+    int rc = 0;
+    if (A->store) {
+        if ((rc = adj_record_store(A))) return rc;
+    } else {
+        for (int sgi = 0; sgi < A->nseg; sgi++) {
+            const int s0 = sgi * A->k, s1 = std::min(s0 + A->k, nsteps);
+            if ((rc = adj_save(A, true, sgi))) return rc;
+            if ((rc = adj_steps(A, s1 - s0, nsample > 0 ? A->d_hist + (size_t)s0 * 3 * nsample : nullptr, true,
+                                sgi, false)))
+                return rc;
+        }
+    }
+    if (nsample > 0)
+        HIPCHK_A(hipMemcpyAsync(hist, A->d_hist, sizeof(double) * 3 * nsample * (size_t)nsteps,
+                                hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK_A(hipStreamSynchronize(ctx->stream));
+    return MCM_GPU_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1212,7 +1236,8 @@ int mcm_gpu_adj_forward(mcm_gpu_ctx* ctx, int32_t nsteps, int32_t ckpt_every, co
                         int32_t nsample, double* hist, mcm_gpu_adjoint** out) {
     if (!ctx || !out || nsteps <= 0 || ckpt_every <= 0 || nsample < 0 || (nsample > 0 && (!sample || !hist)))
         return MCM_GPU_ERR_ARG;
-    *out = nullptr;
+    mcm_gpu_adjoint* prev = *out;   // a recording of this run whose buffers may be reused, or NULL
+    if (prev && prev->ctx != ctx) { ctx->err = "the recording to reuse belongs to another run"; return MCM_GPU_ERR_ARG; }
     const DevState& d = ctx->d;
     std::string why;
     if (ctx->phase != 0 || ctx->batch) why = "the run must be at a step boundary and not part of a batch";
@@ -1229,6 +1254,17 @@ int mcm_gpu_adj_forward(mcm_gpu_ctx* ctx, int32_t nsteps, int32_t ckpt_every, co
         else seen[sample[k]] = 1;
     }
     if (!why.empty()) { ctx->err = why; return MCM_GPU_ERR_ARG; }
+    int rc = 0;
+    if (prev && prev->nsteps == nsteps && prev->k == ckpt_every && prev->nsample == nsample && prev->np == d.np) {
+        // same shape: the device buffers are reused (no allocation, no free,
+        // which would synchronise the device)
+        mcm_gpu_adjoint* A = prev;
+        A->err.clear();
+        if (nsample > 0)
+            HIPCHK_A(hipMemcpyAsync(A->d_sample, sample, sizeof(int) * nsample, hipMemcpyHostToDevice, ctx->stream));
+        return adj_record(A, hist);
+    }
+    if (prev) mcm_gpu_adj_destroy(prev);
     mcm_gpu_adjoint* A = new mcm_gpu_adjoint();
     *out = A;
     A->ctx = ctx;
@@ -1242,7 +1278,6 @@ int mcm_gpu_adj_forward(mcm_gpu_ctx* ctx, int32_t nsteps, int32_t ckpt_every, co
     A->ckpt_cur.assign(A->nseg, 0);
     A->seg_cur.assign(ckpt_every, 0);
     const size_t slot = kSnapWords * A->np;
-    int rc = 0;
     rc |= adj_alloc(A, A->allocs, &A->ckpt, slot * A->nseg);
     rc |= adj_alloc(A, A->allocs, &A->ckpt_sc, A->nseg);
     rc |= adj_alloc(A, A->allocs, &A->seg, A->store ? 1 : slot * ckpt_every);
@@ -1275,22 +1310,7 @@ int mcm_gpu_adj_forward(mcm_gpu_ctx* ctx, int32_t nsteps, int32_t ckpt_every, co
     a.ngcap = 0;
     if (nsample > 0)
         HIPCHK_A(hipMemcpyAsync(A->d_sample, sample, sizeof(int) * nsample, hipMemcpyHostToDevice, ctx->stream));
-    if (A->store) {
-        if ((rc = adj_record_store(A))) return rc;
-    } else {
-        for (int sgi = 0; sgi < A->nseg; sgi++) {
-            const int s0 = sgi * A->k, s1 = std::min(s0 + A->k, nsteps);
-            if ((rc = adj_save(A, true, sgi))) return rc;
-            if ((rc = adj_steps(A, s1 - s0, nsample > 0 ? A->d_hist + (size_t)s0 * 3 * nsample : nullptr, true,
-                                sgi, false)))
-                return rc;
-        }
-    }
-    if (nsample > 0)
-        HIPCHK_A(hipMemcpyAsync(hist, A->d_hist, sizeof(double) * 3 * nsample * (size_t)nsteps,
-                                hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK_A(hipStreamSynchronize(ctx->stream));
-    return MCM_GPU_OK;
+    return adj_record(A, hist);
 }
 
 int mcm_gpu_adj_backward(mcm_gpu_adjoint* A, const double* xbar, const double* vbar, const double* histbar,

@@ -202,6 +202,10 @@ class Solver:
             raise McmGpuError("%s failed: %s (%s)" % (what, ERRORS.get(rc, rc), msg))
 
     def close(self):
+        spare = getattr(self, "_adj_spare", None)
+        if spare:
+            self._lib.mcm_gpu_adj_destroy(spare)
+            self._adj_spare = None
         if getattr(self, "_h", None):
             if self._owned:
                 self._lib.mcm_gpu_destroy(self._h)
@@ -333,18 +337,20 @@ class Solver:
         reverse sweep (Recording.backward)."""
         smp = np.ascontiguousarray(sample, dtype=np.int32)
         hist = np.zeros((int(nsteps), len(smp), 3))
-        h = ctypes.c_void_p()
+        # a closed recording of this run is reused (its buffers, when the shape matches)
+        h = getattr(self, "_adj_spare", None) or ctypes.c_void_p()
+        self._adj_spare = None
         i32p = ctypes.POINTER(ctypes.c_int32)
         rc = self._lib.mcm_gpu_adj_forward(self._h, int(nsteps), int(ckpt_every), smp.ctypes.data_as(i32p),
                                            len(smp), hist.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                                            ctypes.byref(h))
-        rec = Recording(self, h, int(nsteps), len(smp), hist)
         if rc != 0:
             msg = self._lib.mcm_gpu_adj_last_error(h).decode() if h else \
                 self._lib.mcm_gpu_last_error(self._h).decode()
-            rec.close()
+            if h:
+                self._lib.mcm_gpu_adj_destroy(h)
             raise McmGpuError("record failed: %s (%s)" % (ERRORS.get(rc, rc), msg))
-        return rec
+        return Recording(self, h, int(nsteps), len(smp), hist)
 
 
 class Recording:
@@ -387,8 +393,14 @@ class Recording:
         return v0, pb
 
     def close(self):
+        """Release the recording: its buffers go back to the run for the next
+        record() (or are freed when the run is closed)."""
         if getattr(self, "_h", None):
-            self._lib.mcm_gpu_adj_destroy(self._h)
+            s = self._s
+            if getattr(s, "_h", None) and not getattr(s, "_adj_spare", None):
+                s._adj_spare = self._h
+            else:
+                self._lib.mcm_gpu_adj_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -173,3 +173,32 @@ def test_unsupported_decks_are_rejected():
         s.initial()
         with pytest.raises(gpu.McmGpuError, match="plain SPH"):
             s.record(4, 2, [0])
+
+
+def _instance(seed, length):
+    dk = deck.taylor_bar(dx=DX, radius=RADIUS, length=length, model="f3dm10", tcrit_opt=1, seed=seed)
+    dk.v = dk.v * (1.0 + np.random.default_rng(seed + 100).uniform(-0.005, 0.005, size=dk.v.shape))
+    sample = adjoint.surface_sample(dk.x, RADIUS, DX, 8, seed=seed)
+    ident = adjoint.Identification(dk, NSTEPS, 1, sample, NU)
+    ident.make_target([117e9, 400e6, 100e6, 1.49])
+    return ident
+
+
+def test_identification_fits_run_concurrently_and_reduce_the_loss():
+    """Two bars of different length fitted at once (one thread and stream
+    each) from parameters 10-15 % off: the loss falls, and each instance's
+    iterates are bit-identical to the same fit run alone."""
+    starts = [np.array([117e9 * 1.1, 400e6 * 0.87, 100e6, 1.49 * 1.12]),
+              np.array([117e9 * 0.9, 400e6 * 1.15, 100e6, 1.49 * 0.9])]
+    idents = [_instance(11, 12.8e-3), _instance(12, 9.6e-3)]
+    try:
+        together = adjoint.fit_many(idents, starts, 8, lr=0.05, threads=2)
+        alone = [adjoint.fit(idn, th, 8, lr=0.05) for idn, th in zip(idents, starts)]
+    finally:
+        for idn in idents:
+            idn.close()
+    for a, b in zip(together, alone):
+        np.testing.assert_array_equal(np.array(a["loss"]), np.array(b["loss"]))
+        np.testing.assert_array_equal(a["final"], b["final"])
+        print("loss %.3e -> %.3e" % (a["loss"][0], a["loss"][-1]))
+        assert a["loss"][-1] < 0.2 * a["loss"][0]
