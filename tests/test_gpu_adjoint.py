@@ -202,3 +202,87 @@ def test_identification_fits_run_concurrently_and_reduce_the_loss():
         np.testing.assert_array_equal(a["final"], b["final"])
         print("loss %.3e -> %.3e" % (a["loss"][0], a["loss"][-1]))
         assert a["loss"][-1] < 0.2 * a["loss"][0]
+
+
+# ---- planes: corner ghosts made from ghosts, periodic ghosts
+def _quarter_deck():
+    import cases
+    dx = 0.4e-3
+    pts = cases._quarter_bar(dx, 1.2e-3, 7)
+    rho = 8930.0
+    m = deck.material(10, rho, rho * dx ** 3 * len(pts), 1.2 * dx, deck.f3dm10(45e9, 400e6, 100e6), eos=4,
+                      eoscard={0: 3940.0, 1: 1.49, 4: 2.0, 5: 0.47})
+    rng = np.random.default_rng(21)
+    v = np.zeros((len(pts), 3))
+    v[:, 2] = -150.0
+    v += rng.uniform(-3.0, 3.0, size=v.shape)
+    return deck.make_deck(pts, v, [m], tcrit_opt=1, jitter_dx=0.01 * dx, sym_flag=(1, 0, 1, 0, 1, 0))
+
+
+def _periodic_deck():
+    dx = 0.4e-3
+    nx = 5
+    pts = deck.block(dx, nx, 4, 5)
+    rho = 2700.0
+    m = deck.material(10, rho, rho * dx ** 3 * len(pts), 1.2 * dx, deck.f3dm10(26e9, 300e6, 500e6), eos=1,
+                      eoscard={1: 70e9, 2: 30e9, 8: 1.0})
+    rng = np.random.default_rng(22)
+    v = np.zeros((len(pts), 3))
+    v[:, 2] = -100.0
+    v[:, 0] = 20.0 * np.sin(2 * np.pi * pts[:, 2] / (5 * dx))
+    v += rng.uniform(-2.0, 2.0, size=v.shape)
+    return deck.make_deck(pts, v, [m], jitter_dx=0.01 * dx, tcrit_opt=1,
+                          sym_flag=(2, 2, 0, 0, 1, 0), sym_x=(0.0, nx * dx, 0, 0, 0, 0))
+
+
+def _oracle_functional(dk, w, nsteps, tmp_path):
+    p = str(tmp_path / "fdp.bin")
+    dk.write_bin(p)
+    o = Oracle(p)
+    assert o.initial() == 0
+    for _ in range(nsteps):
+        assert o.step() == 0
+    x = o.positions(dk.np_)
+    o.close()
+    return float((w * x).sum())
+
+
+@pytest.mark.parametrize("which", ["corner", "periodic"])
+def test_gradients_with_mirrored_and_periodic_ghosts(which, tmp_path):
+    """L = sum w . x after 20 steps (random weights): parameter and
+    initial-velocity gradients through ghost-of-ghost corners (three
+    symmetry planes) and periodic ghosts against central differences of the
+    oracle."""
+    nsteps = 20
+    dk = _quarter_deck() if which == "corner" else _periodic_deck()
+    w = np.random.default_rng(5).standard_normal((dk.np_, 3))
+    p0 = _model_params(dk)
+    with gpu.Solver(dk) as s:
+        s.initial()
+        rec = s.record(nsteps, 7, [])
+        v0b, pb = rec.backward(xbar=w)
+        rec.close()
+    gp = pb[0]
+    for k in range(4):
+        if p0[k] == 0.0:
+            continue
+        h = 1e-6 * abs(p0[k])
+        pp, pm = p0.copy(), p0.copy()
+        pp[k] += h
+        pm[k] -= h
+        fd = (_oracle_functional(_with(dk, pp), w, nsteps, tmp_path) -
+              _oracle_functional(_with(dk, pm), w, nsteps, tmp_path)) / (pp[k] - pm[k])
+        print("%s param %d: adjoint %.10e fd %.10e" % (which, k, gp[k], fd))
+        assert abs(gp[k] - fd) <= 1e-5 * abs(fd) + 1e-9 * np.abs(gp).max(), (k, gp[k], fd)
+    # the particle nearest the planes' common corner (or edge), and the largest components
+    near = int(np.argmin(dk.x.sum(axis=1)))
+    comps = [(near, 0), (near, 1), (near, 2)] + [divmod(int(f), 3) for f in np.argsort(-np.abs(v0b).ravel())[:2]]
+    scale = np.abs(v0b).max()
+    for i, c in comps:
+        vp, vm = dk.v.copy(), dk.v.copy()
+        vp[i, c] += 1e-3
+        vm[i, c] -= 1e-3
+        fd = (_oracle_functional(_with(dk, v0=vp), w, nsteps, tmp_path) -
+              _oracle_functional(_with(dk, v0=vm), w, nsteps, tmp_path)) / 2e-3
+        print("%s v0[%d,%d]: adjoint %.10e fd %.10e" % (which, i, c, v0b[i, c], fd))
+        assert abs(v0b[i, c] - fd) <= 1e-5 * max(abs(fd), 1e-3 * scale), (i, c, v0b[i, c], fd)
