@@ -209,14 +209,16 @@ void mcm_gpu_batch_destroy(mcm_gpu_batch *b);
    identification, gradients of a displacement-history loss with respect
    to material constants and initial velocities.
 
-   Scope: plain SPH (disctype 0), material model 10 (any EOS), h_opt 0,
-   tcrit_opt 0 or 1, no penalty contact, XSPH, dynamic relaxation or base
-   acceleration; other decks are rejected with MCM_GPU_ERR_ARG.
-   Parameters, per material: strinput(1:3) (f3dm10 G, sigma_y, Eh) and
-   eosinput(2) (S1 of eos4). */
+   Scope: plain SPH (disctype 0), material models 10 (any EOS) and 1,
+   h_opt 0, tcrit_opt 0 or 1, no penalty contact, XSPH, dynamic relaxation
+   or base acceleration; other decks are rejected with MCM_GPU_ERR_ARG.
+   Parameters, 4 per material: model 10 strinput(1:3) (f3dm10 G, sigma_y,
+   Eh) and eosinput(2) (S1 of eos4); model 1 E and nu (then 2 unused). */
 typedef struct mcm_gpu_adjoint mcm_gpu_adjoint;
-/* Set / read the 4 parameters of material mat (0-based); a new value takes
-   effect at the next mcm_gpu_initial (mcm_calcg: G of model 10 follows). */
+/* Set / read the 4 parameters of material mat (0-based); the constants the
+   reference's set-up routines derive from them follow (mcm_sets1,
+   mcm_sets10, mcm_calcg); a new value takes effect at the next
+   mcm_gpu_initial. */
 int mcm_gpu_set_material_params(mcm_gpu_ctx *ctx, int32_t mat, const double *params);
 int mcm_gpu_get_material_params(mcm_gpu_ctx *ctx, int32_t mat, double *params);
 /* Replace the deck's initial velocities (n = 3 np, particle-major, deck

@@ -29,12 +29,13 @@
 // active set, the EOS branch and the pressure cut-off are those the forward
 // took.
 //
-// Scope: plain SPH (disctype 0), material model 10 with any EOS, fixed h
+// Scope: plain SPH (disctype 0), material models 10 (any EOS) and 1, fixed h
 // (h_opt 0), tcrit_opt 0 or 1, no penalty contact, XSPH, dynamic relaxation
 // or base acceleration.  With tcrit_opt 1 the critical step h / (c + |v|)
 // reads |v| of the previous velocity update, which equals |v^n| (the
-// reflections only flip signs), so it enters as a function of v^n.  Parameters: per material strinput(1:3) (f3dm10 G,
-// sigma_y, Eh) and eosinput(2) (S1 of eos4), and the deck's initial
+// reflections only flip signs), so it enters as a function of v^n.
+// Parameters, per material: strinput(1:3) and eosinput(2) for model 10 (G,
+// sigma_y, Eh, S1 of eos4), E and nu for model 1; and the deck's initial
 // velocities.
 #pragma once
 
@@ -452,14 +453,14 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_adj_gmap_mom(DevState d, AdjDev a
 }
 
 // ---------------------------------------------------------- constitutive
-// The material constants the model-10 chain reads (f3dm10: strinput(1:8);
-// the EOS routines: eosinput(1:8); rhoupd, nodalav, sueos: rho, h, rho_max,
-// av_l, av_q, g), as duals.
+// The material constants the chains read (f3dm10: strinput(1:8); f3dm1:
+// strinput(1), (6), (20); the EOS routines: eosinput(1:8); rhoupd, nodalav,
+// sueos: rho, h, rho_max, av_l, av_q, g), as duals.
 template <int K>
 struct MatDual {
     int model, eos, visc_type, ispall, efps_flag;
     Dual<K> rho, g, av_l, av_q, h, rho_max;
-    Dual<K> strinput[8], eosinput[8];
+    Dual<K> strinput[20], eosinput[8];
 };
 
 template <int K>
@@ -468,10 +469,9 @@ __device__ __forceinline__ MatDual<K> dual_mat(const MatParams& m) {
     r.model = m.model; r.eos = m.eos; r.visc_type = m.visc_type; r.ispall = m.ispall; r.efps_flag = m.efps_flag;
     r.rho = m.rho; r.g = m.g; r.av_l = m.av_l; r.av_q = m.av_q; r.h = m.h; r.rho_max = m.rho_max;
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
-        r.strinput[k] = m.strinput[k];
-        r.eosinput[k] = m.eosinput[k];
-    }
+    for (int k = 0; k < 20; k++) r.strinput[k] = m.strinput[k];
+#pragma unroll
+    for (int k = 0; k < 8; k++) r.eosinput[k] = m.eosinput[k];
     return r;
 }
 
@@ -481,6 +481,20 @@ __device__ __forceinline__ void seed(Dual<K>& x, int j, int p0) {
 }
 
 constexpr int ADJ_NPASS = (ADJ_NIN + ADJ_K - 1) / ADJ_K;
+
+// model 1: the set-up constants from E and nu (mcm_sets1.f90:23-51, as
+// setup_material derives them): G = q2 at strinput(20) and lambda + 2G = q3
+// at strinput(2)
+template <class T, class M>
+MCM_HD void model1_constants(M& m, const T& E, const T& nu) {
+    const T q1 = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+    const T q2 = E * 0.5 / (1.0 + nu);
+    const T q3 = q1 + 2.0 * q2;
+    m.strinput[0] = E;
+    m.strinput[5] = nu;
+    m.strinput[19] = q2;
+    m.strinput[1] = q3;
+}
 
 // One dual pass of the reverse of the constitutive update of real particle i
 // (mcm_constitutive.f90 with mcm_rhoupd, mcm_stress_rot1, mcm_f3dm10,
@@ -529,22 +543,40 @@ __device__ __forceinline__ void const_pass(const DevState& d, const AdjDev& a, i
     D dt = dtn;
     seed(dt, IN_DT, P0);
     MatDual<ADJ_K> m = dual_mat<ADJ_K>(m0);
-    seed(m.strinput[0], IN_PAR + 0, P0);
-    seed(m.strinput[1], IN_PAR + 1, P0);
-    seed(m.strinput[2], IN_PAR + 2, P0);
-    seed(m.eosinput[1], IN_PAR + 3, P0);
-    m.g = m.strinput[0];   // mcm_calcg.f90: G of model 10
     s.tracerod = s.rod[S11] + s.rod[S22] + s.rod[S33];
-    // the forward's chain (constitutive_particle, model 10)
-    rhoupd(s, m, dt);
-    stress_rot1(s, dt);
-    double dprev = snap_at(a, np, SN_DEFPS, i);
-    f3dm10(s, m, dt, dprev);
-    sueos(s, m);
-    nodalav(s, m);
-    hieupd(s, dt);
-    eqos(s, m, dt);
-    stress_up(s);
+    if (m0.model == 10) {
+        seed(m.strinput[0], IN_PAR + 0, P0);
+        seed(m.strinput[1], IN_PAR + 1, P0);
+        seed(m.strinput[2], IN_PAR + 2, P0);
+        seed(m.eosinput[1], IN_PAR + 3, P0);
+        m.g = m.strinput[0];   // mcm_calcg.f90: G of model 10
+        // the forward's chain (constitutive_particle, model 10)
+        rhoupd(s, m, dt);
+        stress_rot1(s, dt);
+        double dprev = snap_at(a, np, SN_DEFPS, i);
+        f3dm10(s, m, dt, dprev);
+        sueos(s, m);
+        nodalav(s, m);
+        hieupd(s, dt);
+        eqos(s, m, dt);
+        stress_up(s);
+    } else {
+        // model 1 (constitutive_particle, model 1): parameters E and nu; the
+        // sound speed sqrt((lambda + 2G) / rho) is fixed at the initialisation
+        // (mcm_calcrhopc.f90:51), so its derivative follows lambda + 2G
+        D E = m0.strinput[0], nu = m0.strinput[5];
+        seed(E, IN_PAR + 0, P0);
+        seed(nu, IN_PAR + 1, P0);
+        model1_constants(m, E, nu);
+        const double c0 = s.c.v, q3 = m0.strinput[1];
+#pragma unroll
+        for (int k = 0; k < ADJ_K; k++) s.c.d[k] = c0 / (2.0 * q3) * m.strinput[1].d[k];
+        rhoupd(s, m, dt);
+        stress_rot1(s, dt);
+        f3dm1(s, m, dt);
+        nodalav(s, m);
+        lieupd(s, dt);
+    }
     const D cr = critts(s, m, d.tcrit_opt, 0.0);
     double ys[6];
 #pragma unroll
@@ -1143,11 +1175,19 @@ void adj_initial_params(mcm_gpu_adjoint* A, const std::vector<double> sb[6], con
         Md.rho = M.rho; Md.g = M.g; Md.av_l = M.av_l; Md.av_q = M.av_q; Md.h = M.h; Md.rho_max = M.rho_max;
         for (int k = 0; k < 48; k++) Md.strinput[k] = M.strinput[k];
         for (int k = 0; k < 24; k++) Md.eosinput[k] = M.eosinput[k];
-        Md.strinput[0].d[0] = 1.0;
-        Md.strinput[1].d[1] = 1.0;
-        Md.strinput[2].d[2] = 1.0;
-        Md.eosinput[1].d[3] = 1.0;
-        Md.g = Md.strinput[0];
+        if (M.model == 10) {
+            Md.strinput[0].d[0] = 1.0;
+            Md.strinput[1].d[1] = 1.0;
+            Md.strinput[2].d[2] = 1.0;
+            Md.eosinput[1].d[3] = 1.0;
+            Md.g = Md.strinput[0];
+        } else {   // model 1: E, nu and the constants derived from them
+            const MatParams& R = Dk.mat_raw[m];
+            D E = R.strinput[0], nu = R.strinput[5];
+            E.d[0] = 1.0;
+            nu.d[1] = 1.0;
+            model1_constants(Md, E, nu);
+        }
         PStateT<D> s{};
         init_particle(s, Md, Dk, i, Dk.pm[i], M.strinput[3]);
         s.tracerod = 0.0;
@@ -1207,22 +1247,39 @@ extern "C" {
 
 int mcm_gpu_set_material_params(mcm_gpu_ctx* ctx, int32_t mat, const double* params) {
     if (!ctx || !params || mat < 0 || mat >= ctx->deck.nummat) return MCM_GPU_ERR_ARG;
-    MatParams& M = ctx->deck.mat[mat];
-    M.strinput[0] = params[0];
-    M.strinput[1] = params[1];
-    M.strinput[2] = params[2];
-    M.eosinput[1] = params[3];
-    if (M.model == 10) M.g = M.strinput[0];   // mcm_calcg.f90
+    MatParams R = ctx->deck.mat_raw[mat];
+    if (R.model == 10) {
+        R.strinput[0] = params[0];
+        R.strinput[1] = params[1];
+        R.strinput[2] = params[2];
+        R.eosinput[1] = params[3];
+    } else if (R.model == 1) {
+        R.strinput[0] = params[0];   // E (card 1)
+        R.strinput[5] = params[1];   // nu (card 2)
+    } else {
+        ctx->err = "material parameters are defined for models 1 and 10";
+        return MCM_GPU_ERR_MODEL;
+    }
+    ctx->deck.mat_raw[mat] = R;
+    setup_material(R);   // derived constants as the reference's set-up routines leave them
+    ctx->deck.mat[mat] = R;
     return MCM_GPU_OK;
 }
 
 int mcm_gpu_get_material_params(mcm_gpu_ctx* ctx, int32_t mat, double* params) {
     if (!ctx || !params || mat < 0 || mat >= ctx->deck.nummat) return MCM_GPU_ERR_ARG;
-    const MatParams& M = ctx->deck.mat[mat];
-    params[0] = M.strinput[0];
-    params[1] = M.strinput[1];
-    params[2] = M.strinput[2];
-    params[3] = M.eosinput[1];
+    const MatParams& R = ctx->deck.mat_raw[mat];
+    if (R.model == 1) {
+        params[0] = R.strinput[0];
+        params[1] = R.strinput[5];
+        params[2] = 0.0;
+        params[3] = 0.0;
+    } else {
+        params[0] = R.strinput[0];
+        params[1] = R.strinput[1];
+        params[2] = R.strinput[2];
+        params[3] = R.eosinput[1];
+    }
     return MCM_GPU_OK;
 }
 
@@ -1247,7 +1304,8 @@ int mcm_gpu_adj_forward(mcm_gpu_ctx* ctx, int32_t nsteps, int32_t ckpt_every, co
     else if (d.contacttype > 0 || d.veloc_opt == 1 || d.drelax || d.baseaccel)
         why = "the adjoint does not cover penalty contact, XSPH, dynamic relaxation or base acceleration";
     for (int m = 0; m < d.nummat && why.empty(); m++)
-        if (ctx->deck.mat[m].model != 10) why = "the adjoint covers material model 10";
+        if (ctx->deck.mat[m].model != 10 && ctx->deck.mat[m].model != 1)
+            why = "the adjoint covers material models 1 and 10";
     std::vector<char> seen(d.np, 0);
     for (int k = 0; k < nsample && why.empty(); k++) {
         if (sample[k] < 0 || sample[k] >= d.np || seen[sample[k]]) why = "sampled particles must be distinct ids below np";

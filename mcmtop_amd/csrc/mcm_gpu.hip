@@ -45,6 +45,7 @@ struct HostDeck {
     int sym_flag[6] = {0, 0, 0, 0, 0, 0};
     double tssf = 0, itss = 0, endtime = 0, sym_x[6] = {0, 0, 0, 0, 0, 0};
     std::vector<MatParams> mat;
+    std::vector<MatParams> mat_raw;              // material cards before the set-up routines (defaults applied)
     std::vector<double> mat_mass, mat_rho_min;   // material mass and rho_min cards
     std::vector<double> x, v, pm, ph, pr, pe;
     std::vector<int> pmat, dispbc;
@@ -145,6 +146,7 @@ int finish_deck(mcm_gpu_ctx* ctx) {
     if (D.tssf == 0.0) D.tssf = 0.8;
     if (D.krtype == 0) D.krtype = 1;
     if (D.maxnbr == 0) D.maxnbr = 40;
+    D.mat_raw.clear();
     for (int m = 0; m < D.nummat; m++) {
         MatParams& M = D.mat[m];
         // material card defaults, mcm_matin.f90:46-56
@@ -164,6 +166,7 @@ int finish_deck(mcm_gpu_ctx* ctx) {
             ctx->err = "material model not supported on the GPU path (1, 3, 9, 10 are)";
             return MCM_GPU_ERR_MODEL;
         }
+        D.mat_raw.push_back(M);
         setup_material(M);
         if (M.visc_type != 1) { ctx->err = "only nodal viscosity (visc_type 1) exists"; return MCM_GPU_ERR_MODEL; }
     }

@@ -39,7 +39,7 @@ SYMBOLS = (
     "mcm_gpu_adj_forward", "mcm_gpu_adj_backward", "mcm_gpu_adj_last_error", "mcm_gpu_adj_destroy",
 )
 
-N_PARAMS = 4   # per material: strinput(1:3) (f3dm10 G, sigma_y, Eh) and eosinput(2) (eos4 S1)
+N_PARAMS = 4   # per material: model 10 strinput(1:3) (G, sigma_y, Eh) and eosinput(2) (eos4 S1); model 1 E, nu
 
 OUT_WORDS = 17   # packed output record: x(3), v(3), p, sigma 11 22 33 12 13 23, e, efps, rho, h
 
@@ -311,8 +311,10 @@ class Solver:
 
     # ---- differentiation (plain SPH, material model 10)
     def set_material_params(self, mat: int, params) -> None:
-        """strinput(1:3) and eosinput(2) of material mat (0-based): f3dm10 G,
-        sigma_y, Eh and eos4 S1; takes effect at the next initial()."""
+        """The 4 parameters of material mat (0-based): model 10 G, sigma_y,
+        Eh (strinput(1:3)) and S1 (eosinput(2)); model 1 E, nu (then 2
+        unused).  Derived constants follow; takes effect at the next
+        initial()."""
         p = np.ascontiguousarray(params, dtype=np.float64)
         assert p.size == N_PARAMS
         self._check(self._lib.mcm_gpu_set_material_params(self._h, int(mat), p.ctypes.data_as(

@@ -286,3 +286,55 @@ def test_gradients_with_mirrored_and_periodic_ghosts(which, tmp_path):
               _oracle_functional(_with(dk, v0=vm), w, nsteps, tmp_path)) / 2e-3
         print("%s v0[%d,%d]: adjoint %.10e fd %.10e" % (which, i, c, v0b[i, c], fd))
         assert abs(v0b[i, c] - fd) <= 1e-5 * max(abs(fd), 1e-3 * scale), (i, c, v0b[i, c], fd)
+
+
+def _two_material_deck():
+    """f3dm10 + eqos4 bar with an f3dm1 (elastic) cap, in kernel contact
+    (contacttype 0: one list for both materials), on a symmetry plane."""
+    dx = 0.4e-3
+    pts = deck.block(dx, 5, 5, 9)
+    rho1, rho2 = 8930.0, 7850.0
+    cap = pts[:, 2] > 6 * dx
+    n1, n2 = int((~cap).sum()), int(cap.sum())
+    m1 = deck.material(10, rho1, rho1 * dx ** 3 * n1, 1.2 * dx, deck.f3dm10(45e9, 400e6, 100e6), eos=4,
+                       eoscard={0: 3940.0, 1: 1.49, 4: 2.0, 5: 0.47})
+    m2 = deck.material(1, rho2, rho2 * dx ** 3 * n2, 1.2 * dx, deck.f3dm1(200e9, 0.3))
+    rng = np.random.default_rng(31)
+    v = np.zeros((len(pts), 3))
+    v[:, 2] = -150.0
+    v += rng.uniform(-3.0, 3.0, size=v.shape)
+    return deck.make_deck(pts, v, [m1, m2], matid=np.where(cap, 2, 1), tcrit_opt=1, jitter_dx=0.01 * dx,
+                          sym_flag=(0, 0, 0, 0, 1, 0))
+
+
+def test_two_materials_with_an_elastic_part(tmp_path):
+    """Model 10 and model 1 in one run: the gradient with respect to G,
+    sigma_y, Eh, S1 of the first material and E, nu of the second (whose
+    constants and frozen sound speed derive from them) against central
+    differences of the oracle; L = sum w . x after 20 steps."""
+    import copy
+    nsteps = 20
+    dk = _two_material_deck()
+    w = np.random.default_rng(6).standard_normal((dk.np_, 3))
+    with gpu.Solver(dk) as s:
+        p_all = np.array([s.material_params(0), s.material_params(1)])
+        s.initial()
+        rec = s.record(nsteps, 5, [])
+        _, pb = rec.backward(xbar=w)
+        rec.close()
+    np.testing.assert_array_equal(p_all[1], [200e9, 0.3, 0.0, 0.0])
+    checks = [(0, k, (0, k) if k < 3 else None) for k in range(4)] + [(1, 0, (1, 0)), (1, 1, (1, 5))]
+    for mat, k, where in checks:
+        base = p_all[mat][k]
+        h = 1e-6 * abs(base)
+        vals = []
+        for sgn in (1.0, -1.0):
+            d2 = copy.deepcopy(dk)
+            if where is None:
+                d2.materials[0].eoscard[1] = base + sgn * h
+            else:
+                d2.materials[where[0]].card[where[1]] = base + sgn * h
+            vals.append(_oracle_functional(d2, w, nsteps, tmp_path))
+        fd = (vals[0] - vals[1]) / (2 * h)
+        print("material %d param %d: adjoint %.10e fd %.10e" % (mat, k, pb[mat][k], fd))
+        assert abs(pb[mat][k] - fd) <= 1e-5 * abs(fd), (mat, k, pb[mat][k], fd)
