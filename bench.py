@@ -315,6 +315,69 @@ def ident_leg(ninst: int, steps: int, iters: int, threads: int, seed: int):
                         "fit (E, sigma_y, S1) by Adam (lr 0.03) on the logarithms" % (ninst, seed)}
 
 
+def with_elastic_cap(dk, dx: float, frac: float = 0.1):
+    """The top frac of a bar's length as a second material, f3dm1 steel
+    (E 200 GPa, nu 0.3): the elastic sub-part of the identification
+    configuration, in kernel contact with the bar."""
+    import numpy as np
+    from mcmtop_amd import deck
+    z = dk.x[:, 2]
+    cap = z > z.min() + (1.0 - frac) * (z.max() - z.min())
+    n2 = int(cap.sum())
+    m1 = dk.materials[0]
+    m1.mass = m1.mass * (dk.np_ - n2) / dk.np_
+    rho2 = 7850.0
+    m2 = deck.material(1, rho2, rho2 * dx ** 3 * n2, m1.h, deck.f3dm1(200e9, 0.3), av_q=m1.av_q, av_l=m1.av_l)
+    dk.materials = [m1, m2]
+    dk.mat = np.where(cap, 2, 1).astype(np.int32)
+    return dk
+
+
+def ident_config5(ninst: int, steps: int, ckpt: int, iters: int, threads: int, seed: int):
+    """Batched identification at the configuration's size: ninst bars of
+    about 100k particles (uneven radius, length, dx, v0, E, sigma_y), each an
+    f3dm10 + eqos4 body with an f3dm1 cap, steps with a checkpoint every
+    ckpt steps; (E, sigma_y, S1) of the body fitted by Adam from seeded
+    start points.  Time per iteration over all instances (concurrent, each
+    instance's buffers released when its fit ends)."""
+    import numpy as np
+    from mcmtop_amd import adjoint, deck
+    t0 = time.perf_counter()
+    decks, params = deck.batch_bars(ninst, seed=seed, nmin=90000, nmax=110000, corrected=False, h_opt=0,
+                                    dx_range=(0.1e-3, 0.3e-3))
+    rng = np.random.default_rng(seed + 1)
+    idents, starts = [], []
+    for dk, pr in zip(decks, params):
+        dk = with_elastic_cap(dk, pr["dx"])
+        dk.v = dk.v * (1.0 + rng.uniform(-0.005, 0.005, size=dk.v.shape))
+        sample = adjoint.surface_sample(dk.x, pr["radius"], pr["dx"], 32, seed=int(rng.integers(1 << 30)))
+        idn = adjoint.Identification(dk, steps, ckpt, sample, 0.35)
+        th = np.array([pr["E"], pr["sigma_y"], 100e6, 1.49])
+        idents.append((idn, th))
+        starts.append(th * np.array([rng.uniform(0.8, 1.2), rng.uniform(0.8, 1.2), 1.0, rng.uniform(0.9, 1.1)]))
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        list(ex.map(lambda it: it[0].make_target(it[1]), idents))
+    t_setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    res = adjoint.fit_many([i for i, _ in idents], starts, iters, lr=0.03, threads=threads, release=True)
+    t_it = (time.perf_counter() - t0) / iters
+    for idn, _ in idents:
+        idn.close()
+    total = int(sum(p["np"] for p in params))
+    ratio = [r["loss"][-1] / r["loss"][0] for r in res]
+    print("bench: identification at size, %d bars (%d particles), %d steps: %.1f s per iteration"
+          % (ninst, total, steps, t_it), file=sys.stderr, flush=True)
+    return {"instances": ninst, "particles": total, "steps": steps, "ckpt_every": ckpt, "iterations": iters,
+            "threads": threads, "setup_s": t_setup, "s_per_iteration": t_it,
+            "projected_200_iterations_min": 200 * t_it / 60.0,
+            "gradient_particle_steps_per_s": total * steps / t_it,
+            "loss_ratio_after": {"median": float(np.median(ratio)), "max": float(np.max(ratio))},
+            "workload": "batch_bars(%d, seed=%d) at 90k-110k particles with an f3dm1 cap (top 10 %%), plain SPH, "
+                        "f3dm10 + eqos4 body, tcrit_opt 1; (E, sigma_y, S1) of the body by Adam (lr 0.03)"
+                        % (ninst, seed)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -335,6 +398,10 @@ def main():
     ap.add_argument("--ident-steps", type=int, default=200)
     ap.add_argument("--ident-iters", type=int, default=10)
     ap.add_argument("--ident-threads", type=int, default=8)
+    ap.add_argument("--ident-config5", type=int, default=0,
+                    help="bars of the identification leg at the configuration's size (100k particles, 600 steps, "
+                         "checkpoint every 40; 0: skip it)")
+    ap.add_argument("--ident-config5-iters", type=int, default=2)
     ap.add_argument("--corrected-one-pass", action="store_true",
                     help="accumulate the corrected kernels' three passes in one (rounding differs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -417,6 +484,9 @@ def main():
     if args.batch > 0:
         res["batched"] = batched_leg(args.batch, args.batch_seed, args.batch_steps, 4, 16, bool(args.batch_plain))
     res["adjoint_forward_ratio"] = None
+    if args.ident_config5 > 0:
+        res["identification_at_size"] = ident_config5(args.ident_config5, 600, 40, args.ident_config5_iters,
+                                                      args.ident_threads, 9)
     if args.ident > 0:
         res["identification"] = ident_leg(args.ident, args.ident_steps, args.ident_iters, args.ident_threads, 5)
     if args.adjoint_steps > 0:

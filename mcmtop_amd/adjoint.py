@@ -172,11 +172,21 @@ def fit(ident: Identification, theta0, iterations: int, lr: float = 0.05, fitted
     return hist
 
 
-def fit_many(idents, theta0s, iterations: int, lr: float = 0.05, threads: int = 8, fitted=(True, True, False, True)):
+def fit_many(idents, theta0s, iterations: int, lr: float = 0.05, threads: int = 8, fitted=(True, True, False, True),
+             release: bool = False):
     """fit() for several independent instances at once: one host thread per
     instance, each run on its own HIP stream, so the GPU overlaps the
-    instances' kernels (the C ABI releases the interpreter during its calls)."""
+    instances' kernels (the C ABI releases the interpreter during its calls).
+    release: free each instance's recording buffers when its fit ends, so
+    that device memory holds those of at most ``threads`` instances."""
     from concurrent.futures import ThreadPoolExecutor
+
+    def one(idn, th):
+        out = fit(idn, th, iterations, lr, fitted)
+        if release:
+            idn.solver.release_recordings()
+        return out
+
     with ThreadPoolExecutor(max_workers=max(1, int(threads))) as ex:
-        futs = [ex.submit(fit, idn, th, iterations, lr, fitted) for idn, th in zip(idents, theta0s)]
+        futs = [ex.submit(one, idn, th) for idn, th in zip(idents, theta0s)]
         return [f.result() for f in futs]

@@ -910,6 +910,7 @@ struct mcm_gpu_adjoint {
     Scalars* ckpt_sc = nullptr;       // [nseg]
     std::vector<int> ckpt_cur;
     double* seg = nullptr;            // [k][SN_N][np]: start of every step of the segment being reversed
+                                      // (allocated by the first reverse sweep)
     Scalars* seg_sc = nullptr;
     std::vector<int> seg_cur;
     InterLayout lay{};                // intermediates of every step of the segment
@@ -978,16 +979,15 @@ int adj_restore(mcm_gpu_adjoint* A, bool ck, int s) {
 
 // nsteps fused steps from the current state, the sampled positions after each
 // step written to hist (device) when given; a step that outgrows a capacity
-// makes the run grow and the whole call start again from its first state,
-// which is saved in slot (ck, s) first.
-int adj_steps(mcm_gpu_adjoint* A, int nsteps, double* hist, bool ck, int s, bool save_each) {
+// makes the run grow and the call start again from checkpoint s, which holds
+// its first state.
+int adj_steps(mcm_gpu_adjoint* A, int nsteps, double* hist, int s) {
     mcm_gpu_ctx* ctx = A->ctx;
     int rc = 0;
     for (int attempt = 0; attempt < 16; attempt++) {
         Scalars s0;
         if ((rc = get_scalars_raw(ctx, &s0))) { A->err = ctx->err; return rc; }
         for (int n = 0; n < nsteps; n++) {
-            if (save_each && (rc = adj_save(A, false, n))) return rc;
             launch_step(one(ctx), ctx->cur);
             ctx->cur ^= 1;
             if (hist) k_adj_sample<<<blocks_for(A->nsample, 64), 64, 0, ctx->stream>>>(
@@ -998,7 +998,7 @@ int adj_steps(mcm_gpu_adjoint* A, int nsteps, double* hist, bool ck, int s, bool
         if ((rc = get_scalars_raw(ctx, &s1))) { A->err = ctx->err; return rc; }
         if (!s1.abort && s1.timestep - s0.timestep == nsteps) return MCM_GPU_OK;
         if ((rc = regrow(ctx, s1))) { A->err = ctx->err; return rc; }
-        if ((rc = adj_restore(A, ck, s))) return rc;
+        if ((rc = adj_restore(A, true, s))) return rc;
     }
     A->err = "steps did not fit after regrowing";
     return MCM_GPU_ERR_CAPACITY;
@@ -1229,8 +1229,7 @@ int adj_record(mcm_gpu_adjoint* A, double* hist) {
         for (int sgi = 0; sgi < A->nseg; sgi++) {
             const int s0 = sgi * A->k, s1 = std::min(s0 + A->k, nsteps);
             if ((rc = adj_save(A, true, sgi))) return rc;
-            if ((rc = adj_steps(A, s1 - s0, nsample > 0 ? A->d_hist + (size_t)s0 * 3 * nsample : nullptr, true,
-                                sgi, false)))
+            if ((rc = adj_steps(A, s1 - s0, nsample > 0 ? A->d_hist + (size_t)s0 * 3 * nsample : nullptr, sgi)))
                 return rc;
         }
     }
@@ -1338,8 +1337,6 @@ int mcm_gpu_adj_forward(mcm_gpu_ctx* ctx, int32_t nsteps, int32_t ckpt_every, co
     const size_t slot = kSnapWords * A->np;
     rc |= adj_alloc(A, A->allocs, &A->ckpt, slot * A->nseg);
     rc |= adj_alloc(A, A->allocs, &A->ckpt_sc, A->nseg);
-    rc |= adj_alloc(A, A->allocs, &A->seg, A->store ? 1 : slot * ckpt_every);
-    rc |= adj_alloc(A, A->allocs, &A->seg_sc, ckpt_every);
     rc |= adj_alloc(A, A->allocs, &A->d_sample, std::max(nsample, 1));
     rc |= adj_alloc(A, A->allocs, &A->d_hist, (size_t)nsteps * 3 * std::max(nsample, 1));
     AdjDev& a = A->a;
@@ -1421,6 +1418,11 @@ static int adj_sweep(mcm_gpu_adjoint* A, const double* xbar, const double* vbar,
     if (A->store && (A->lay.ucap != (long)ctx->d.np + ctx->d.ngcap || A->lay.nbrlen != (long)ctx->d.npad * ctx->d.ncap)) {
         A->err = "the run's capacities changed after the recording";
         return MCM_GPU_ERR_ARG;
+    }
+    if (!A->store && !A->seg) {   // the segment's start states (first reverse sweep)
+        rc |= adj_alloc(A, A->allocs, &A->seg, kSnapWords * A->np * A->k);
+        rc |= adj_alloc(A, A->allocs, &A->seg_sc, A->k);
+        if (rc) return rc;
     }
     if ((rc = adj_inter_buffers(A))) return rc;
     if ((rc = adj_ghost_buffers(A))) return rc;

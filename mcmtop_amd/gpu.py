@@ -332,6 +332,13 @@ class Solver:
         self._check(self._lib.mcm_gpu_set_initial_velocity(self._h, v.ctypes.data_as(
             ctypes.POINTER(ctypes.c_double)), v.size), "set_initial_velocity")
 
+    def release_recordings(self) -> None:
+        """Free the device buffers kept from the last recording."""
+        spare = getattr(self, "_adj_spare", None)
+        if spare:
+            self._lib.mcm_gpu_adj_destroy(spare)
+            self._adj_spare = None
+
     def record(self, nsteps: int, ckpt_every: int = 50, sample=()) -> "Recording":
         """nsteps fused steps from the current state (normally right after
         initial()), with a state checkpoint every ckpt_every steps and the
