@@ -103,6 +103,7 @@ struct DevState {
     double* rm;                // [U][12] momentum gather record {x, h, m, m/rho, (sigma - q)/rho^2}
     // neighbour search
     int *cell_of, *slot_of, *cell_count, *cell_start, *sorted;
+    int* unsorted;             // [np] cell runs in histogram slot order (k_scatter), sorted by k_sort_cells
     double* cand;              // [np][4] cell-sorted {x, y, z, h} of the candidates
     int* cand_id;              // [np][2] cell-sorted {particle id, material}
     int* spos;                 // [np] sorted slot of each binned particle

@@ -585,6 +585,7 @@ int alloc_all(mcm_gpu_ctx* ctx) {
     for (int k = 0; k < 3; k++) { rc |= dalloc(ctx, &d.spin[k], n); rc |= dalloc(ctx, &d.a[k], n); }
     rc |= dalloc(ctx, &d.yield, n);
     rc |= dalloc(ctx, &d.cell_of, n); rc |= dalloc(ctx, &d.slot_of, n); rc |= dalloc(ctx, &d.sorted, n);
+    rc |= dalloc(ctx, &d.unsorted, n);
     rc |= dalloc(ctx, &d.cand, 4 * n);
     rc |= dalloc(ctx, &d.cand_id, 2 * n);
     rc |= dalloc(ctx, &d.spos, n);
@@ -730,7 +731,7 @@ void launch_neighbours(const Target& T, int cur) {
     launch(T, L_ONE, 0, 1024, k_scan_top);
     launch(T, L_TILES, 0, 1024, k_scan_apply);
     launch(T, L_PART, 0, MCM_BLOCK, k_scatter);
-    launch(T, L_CELLS, 0, MCM_BLOCK, k_sort_cells);
+    launch(T, L_PART, 0, MCM_BLOCK, k_sort_cells);
     launch(T, L_PART, 0, MCM_BLOCK, k_fill_cand, cur);
     for (int p = 0; p < T.planes(); p++) {
         launch(T, L_GSCAN, p, 1024, k_ghost_scan, p);

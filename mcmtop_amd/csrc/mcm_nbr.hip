@@ -257,32 +257,31 @@ __global__ __launch_bounds__(1024) void k_scan_apply(DevState d_) {
     }
 }
 
+// each binned particle into its cell's run, in histogram slot order
 __global__ __launch_bounds__(256) void k_scatter(DevState d_) {
     Blk bk;
     const DevState& d = enter<false>(d_, bk);
     int i = bk.b * blockDim.x + threadIdx.x;
     if (i >= d.np || d.sc->abort) return;
     int c = d.cell_of[i];
-    if (c >= 0) d.sorted[d.cell_start[c] + d.slot_of[i]] = i;
+    if (c >= 0) d.unsorted[d.cell_start[c] + d.slot_of[i]] = i;
 }
 
 // order each cell by descending id (the walk order of the reference's pushed
-// lists), one thread per cell
+// lists), one thread per particle: its place in the cell's run is the number
+// of larger ids in the cell (ids are distinct)
 __global__ __launch_bounds__(256) void k_sort_cells(DevState d_) {
     Blk bk;
     const DevState& d = enter<false>(d_, bk);
-    int ncell = d.sc->grid.ncell;
-    for (int c = bk.b * blockDim.x + threadIdx.x; c < ncell; c += bk.n * blockDim.x) {
-        int n = d.cell_count[c];
-        if (n < 2) continue;
-        int* seg = d.sorted + d.cell_start[c];
-        for (int a = 1; a < n; a++) {
-            int key = seg[a];
-            int b = a - 1;
-            while (b >= 0 && seg[b] < key) { seg[b + 1] = seg[b]; b--; }
-            seg[b + 1] = key;
-        }
-    }
+    const int i = bk.b * blockDim.x + threadIdx.x;
+    if (i >= d.np || d.sc->abort) return;
+    const int c = d.cell_of[i];
+    if (c < 0) return;
+    const int s0 = d.cell_start[c], n = d.cell_count[c];
+    const int* run = d.unsorted + s0;
+    int rank = 0;
+    for (int k = 0; k < n; k++) rank += run[k] > i ? 1 : 0;
+    d.sorted[s0 + rank] = i;
 }
 
 // cell-sorted copies of x, h and the material of the binned particles, one
