@@ -678,9 +678,9 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_adj_strain(DevState d, AdjDev a) 
     const int t = blockIdx.x * (MCM_BLOCK / 64) + (threadIdx.x >> 6);
     if (t >= d.np) return;   // the whole wavefront
     const double dt = d.sc->dtold;
-    const SRec me = load_srec(d.rs, t);
+    const SRec me = load_srec(d.rs, t);   // records hold the strain pass's positions
     const double vt[3] = {me.vx, me.vy, me.vz};
-    const double xh[3] = {me.x - 0.5 * me.vx * dt, me.y - 0.5 * me.vy * dt, me.z - 0.5 * me.vz * dt};
+    const double xh[3] = {me.x, me.y, me.z};
     const double rho_t = d.rbuf[a.cur][t];
     double Gt[9];
     load_g9(a.lbp, t, Gt);
@@ -690,8 +690,7 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_adj_strain(DevState d, AdjDev a) 
     for (int k = lane; k < n; k += 64) {
         const int u = lst[(long)k * 64];
         const SRec r = load_srec(d.rs, u);
-        const double sh = u < d.np ? dt : 0.0;
-        const double xj[3] = {r.x - 0.5 * r.vx * sh, r.y - 0.5 * r.vy * sh, r.z - 0.5 * r.vz * sh};
+        const double xj[3] = {r.x, r.y, r.z};
         const double havg = 0.5 * (me.h + r.h);
         const double xij[3] = {xh[0] - xj[0], xh[1] - xj[1], xh[2] - xj[2]};
         KGH kg;
@@ -740,7 +739,6 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_adj_strain(DevState d, AdjDev a) 
 __global__ __launch_bounds__(MCM_BLOCK) void k_adj_strain_ghost(DevState d, AdjDev a) {
     const double ipi = 1.0 / d.pi;
     const int ngp = d.sc->ngp;
-    const double dt = d.sc->dtold;
     const int lane = threadIdx.x & 63, wpb = MCM_BLOCK / 64;
     for (int g = blockIdx.x * wpb + (threadIdx.x >> 6); g < ngp; g += gridDim.x * wpb) {
         const int u = d.np + g;
@@ -752,7 +750,7 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_adj_strain_ghost(DevState d, AdjD
         for (int k = lane; k < cnt; k += 64) {
             const int i = lst[k];
             const SRec r = load_srec(d.rs, i);
-            const double xh[3] = {r.x - 0.5 * r.vx * dt, r.y - 0.5 * r.vy * dt, r.z - 0.5 * r.vz * dt};
+            const double xh[3] = {r.x, r.y, r.z};
             const double havg = 0.5 * (r.h + me.h);
             const double xij[3] = {xh[0] - me.x, xh[1] - me.y, xh[2] - me.z};
             KGH kg;

@@ -1018,6 +1018,7 @@ int initial(mcm_gpu_ctx* ctx) {
     else s3.dt = s3.critts * D.tssf;
     rc = put_scalars_raw(ctx, s3);
     if (rc) return rc;
+    k_rs_shift<<<ctx->nb, MCM_BLOCK, 0, st>>>(d);
     k_calctotale<<<ctx->nb, MCM_BLOCK, 0, st>>>(d);
     launch_c_fold(one(ctx), 0);
     HIPCHK(hipGetLastError());

@@ -126,8 +126,24 @@ __global__ void k_clear_cells(DevState d_) {
     }
 }
 
+// The strain records' half-step positions of the real particles with the
+// current dt.  mcm_initial sets the first dt after its neighbour search, so
+// it refreshes them; a strain pass on those lists then shifts by that dt.
+__global__ __launch_bounds__(MCM_BLOCK) void k_rs_shift(DevState d) {
+    const int i = blockIdx.x * MCM_BLOCK + threadIdx.x;
+    if (i >= d.np) return;
+    const double dt = d.sc->dt;
+    double2* r = reinterpret_cast<double2*>(d.rs) + 4 * (long)i;
+    const double vx = d.v[0][i], vy = d.v[1][i], vz = d.v[2][i];
+    r[0] = make_double2(d.x0[0][i] - 0.5 * vx * dt, d.x0[1][i] - 0.5 * vy * dt);
+    r[1] = make_double2(d.x0[2][i] - 0.5 * vz * dt, vx);
+}
+
 // histogram: cell of every real particle (mcm_setuplist.f90:320-345); also
-// lays down the particle's strain gather record for this step
+// lays down the particle's strain gather record for this step, whose position
+// is already the half-step position x - v dt / 2 of the strain pass
+// (mcm_strain.f90:27-31) with this step's dt, the expression and rounding of
+// the pair loops' own shift
 __global__ __launch_bounds__(256) void k_bin(DevState d_, int cur) {
     Blk bk;
     const DevState& d = enter<false>(d_, bk);
@@ -142,9 +158,10 @@ __global__ __launch_bounds__(256) void k_bin(DevState d_, int cur) {
         const double h = d.hbuf[cur][i];
         {
             double2* r = reinterpret_cast<double2*>(d.rs) + 4 * (long)i;
+            const double dt = d.sc->dt;
             double vx = d.v[0][i], vy = d.v[1][i], vz = d.v[2][i];
-            r[0] = make_double2(x[0], x[1]);
-            r[1] = make_double2(x[2], vx);
+            r[0] = make_double2(x[0] - 0.5 * vx * dt, x[1] - 0.5 * vy * dt);
+            r[1] = make_double2(x[2] - 0.5 * vz * dt, vx);
             r[2] = make_double2(vy, vz);
             r[3] = make_double2(d.mass[i] / d.rbuf[cur][i], h);
         }
@@ -323,10 +340,13 @@ __device__ __forceinline__ void make_ghost(const DevState& d, int cur, const Pla
     const int pm = (P.mult[0] < 0.0 ? 1 : 0) | (P.mult[1] < 0.0 ? 2 : 0) | (P.mult[2] < 0.0 ? 4 : 0);
     d.gsgn[gid] = (from_real ? 0 : d.gsgn[src]) ^ pm;
     // gather records of the ghost: start-of-step x, v, m, rho and h are
-    // frozen for the whole step (mcm_ghost_setup.f90:330-341)
+    // frozen for the whole step (mcm_ghost_setup.f90:330-341); the strain
+    // pass does not shift ghosts (mcm_strain.f90:27-31), which is a shift
+    // with a zero step, x - v 0 / 2, rounded as for a real particle
+    const double sh = 0.0;
     double2* rs = reinterpret_cast<double2*>(d.rs) + 4 * (long)u;
-    rs[0] = make_double2(d.x0[0][u], d.x0[1][u]);
-    rs[1] = make_double2(d.x0[2][u], d.v[0][u]);
+    rs[0] = make_double2(d.x0[0][u] - 0.5 * d.v[0][u] * sh, d.x0[1][u] - 0.5 * d.v[1][u] * sh);
+    rs[1] = make_double2(d.x0[2][u] - 0.5 * d.v[2][u] * sh, d.v[0][u]);
     rs[2] = make_double2(d.v[1][u], d.v[2][u]);
     rs[3] = make_double2(mv / rv, hv);
     double2* rm = reinterpret_cast<double2*>(d.rm) + 6 * (long)u;

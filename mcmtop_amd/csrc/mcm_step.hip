@@ -88,7 +88,9 @@ __device__ __forceinline__ void load_state(const DevState& d, int i, int cur, PS
 
 // -------------------------------------------------------------- strain
 // Neighbour records: the strain passes read rs[u] = {x, y, z, vx, vy, vz,
-// m/rho, h} (8 doubles, 64 B) and the momentum passes rm[u] = {x, y, z, h,
+// m/rho, h} (8 doubles, 64 B), whose position is the strain pass's position
+// of u: x - v dt / 2 for a real particle, the unshifted x of a ghost
+// (mcm_strain.f90:27-31; written by the list build), and the momentum passes rm[u] = {x, y, z, h,
 // m, m/rho, A11, A22, A33, A12, A13, A23} (12 doubles, 96 B), one aligned
 // record per neighbour instead of one load per state array.
 struct SRec { double x, y, z, vx, vy, vz, vol, h; };
@@ -142,21 +144,19 @@ struct NbrStream {
 
 // plain strain rate, mcm_oldstrain.f90:34-64, positions half a step back
 // for real particles only (mcm_strain.f90:27-31)
-__device__ __forceinline__ void strain_plain(const DevState& d, int i, double dt, const double xi[3],
+__device__ __forceinline__ void strain_plain(const DevState& d, int i, const double xi[3],
                                              const double vi[3], double hi, double g[3][3]) {
     const int n = d.ntot[i];
     const int* lst = nbr_list(d, i);
     for (int a = 0; a < 3; a++)
         for (int b = 0; b < 3; b++) g[a][b] = 0.0;
     NbrStream<SRec> st(lst, d.rs, n, i);
+#pragma unroll 2
     for (int k = 0; k < n; k++) {
         int j;
         SRec r;
         st.next(j, r);
-        double sh = j < d.np ? dt : 0.0;
-        double xj0 = r.x - 0.5 * r.vx * sh;
-        double xj1 = r.y - 0.5 * r.vy * sh;
-        double xj2 = r.z - 0.5 * r.vz * sh;
+        double xj0 = r.x, xj1 = r.y, xj2 = r.z;
         double havg = 0.5 * (hi + r.h);
         double dwdx[3], rad;
         kernel_grad(xi[0] - xj0, xi[1] - xj1, xi[2] - xj2, havg, d.pi, dwdx, &rad);
@@ -176,11 +176,11 @@ __device__ __forceinline__ void strain_plain(const DevState& d, int i, double dt
 
 // kernel value and gradient of one pair of the corrected passes, centred at
 // a (xij = a - b), with the neighbour's half-step position
-__device__ __forceinline__ void corr_pair(const SRec& r, double sh, const double xi[3], double hi, double pi,
+__device__ __forceinline__ void corr_pair(const SRec& r, const double xi[3], double hi, double pi,
                                           double xj[3], double& w, double dwdx[3]) {
-    xj[0] = r.x - 0.5 * r.vx * sh;
-    xj[1] = r.y - 0.5 * r.vy * sh;
-    xj[2] = r.z - 0.5 * r.vz * sh;
+    xj[0] = r.x;
+    xj[1] = r.y;
+    xj[2] = r.z;
     double havg = 0.5 * (hi + r.h);
     double rad;
     kernel_grad(xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2], havg, pi, dwdx, &rad);
@@ -189,18 +189,19 @@ __device__ __forceinline__ void corr_pair(const SRec& r, double sh, const double
 
 // corrected strain rate, mcm_corr_strain.f90:37-146, in the reference's three
 // passes and evaluation order
-__device__ __forceinline__ void strain_corr(const DevState& d, int i, double dt, const double xi[3], double hi,
+__device__ __forceinline__ void strain_corr(const DevState& d, int i, const double xi[3], double hi,
                                             double f[3][3]) {
     const int n = d.ntot[i];
     const int* lst = nbr_list(d, i);
     double sw = 0.0, sg[3] = {0.0, 0.0, 0.0};
     NbrStream<SRec> s1(lst, d.rs, n, i);
+#pragma unroll 2
     for (int k = 0; k < n; k++) {
         int j;
         SRec r;
         s1.next(j, r);
         double xj[3], w, dwdx[3];
-        corr_pair(r, j < d.np ? dt : 0.0, xi, hi, d.pi, xj, w, dwdx);
+        corr_pair(r, xi, hi, d.pi, xj, w, dwdx);
         sw = sw + r.vol * w;
         for (int l = 0; l < 3; l++) sg[l] = sg[l] + r.vol * dwdx[l];
     }
@@ -208,12 +209,13 @@ __device__ __forceinline__ void strain_corr(const DevState& d, int i, double dt,
     const Divisor isw2(sw2);
     double B[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, invB[9];
     NbrStream<SRec> s2(lst, d.rs, n, i);
+#pragma unroll 2
     for (int k = 0; k < n; k++) {
         int j;
         SRec r;
         s2.next(j, r);
         double xj[3], w, dwdx[3], cg[3];
-        corr_pair(r, j < d.np ? dt : 0.0, xi, hi, d.pi, xj, w, dwdx);
+        corr_pair(r, xi, hi, d.pi, xj, w, dwdx);
         isw2.div3(dwdx[0] * sw - w * sg[0], dwdx[1] * sw - w * sg[1], dwdx[2] * sw - w * sg[2], cg);
         for (int m = 0; m < 3; m++)
             for (int nn = 0; nn < 3; nn++) B[nn + 3 * m] = B[nn + 3 * m] + r.vol * (xj[nn] - xi[nn]) * cg[m];
@@ -227,7 +229,7 @@ __device__ __forceinline__ void strain_corr(const DevState& d, int i, double dt,
         SRec r;
         s3.next(j, r);
         double xj[3], w, dwdx[3], cg[3], g[3];
-        corr_pair(r, j < d.np ? dt : 0.0, xi, hi, d.pi, xj, w, dwdx);
+        corr_pair(r, xi, hi, d.pi, xj, w, dwdx);
         isw2.div3(dwdx[0] * sw - w * sg[0], dwdx[1] * sw - w * sg[1], dwdx[2] * sw - w * sg[2], cg);
         for (int q = 0; q < 3; q++) g[q] = invB[q] * cg[0] + invB[q + 3] * cg[1] + invB[q + 6] * cg[2];
         double vj[3] = {r.vx, r.vy, r.vz};
@@ -239,7 +241,7 @@ __device__ __forceinline__ void strain_corr(const DevState& d, int i, double dt,
 // corrected strain rate in one pass: the correction is linear in the pass-1
 // sums, so B and the contracted gradient follow from single-pass moments
 // (same result up to rounding; the fast variant)
-__device__ __forceinline__ void strain_corr_fast(const DevState& d, int i, double dt, const double xi[3],
+__device__ __forceinline__ void strain_corr_fast(const DevState& d, int i, const double xi[3],
                                                  double hi, double f[3][3]) {
     const int n = d.ntot[i];
     const int* lst = nbr_list(d, i);
@@ -250,7 +252,7 @@ __device__ __forceinline__ void strain_corr_fast(const DevState& d, int i, doubl
         SRec r;
         st.next(j, r);
         double xj[3], w, dwdx[3];
-        corr_pair(r, j < d.np ? dt : 0.0, xi, hi, d.pi, xj, w, dwdx);
+        corr_pair(r, xi, hi, d.pi, xj, w, dwdx);
         double vw = r.vol * w;
         double vj[3] = {r.vx, r.vy, r.vz};
         sw += vw;
@@ -415,9 +417,9 @@ __global__ __launch_bounds__(MCM_BLOCK) __attribute__((amdgpu_waves_per_eu(3))) 
             }
             double hi = d.hbuf[cur][i];
             double g[3][3];
-            if (CORR == 1) strain_corr(d, i, dt, xi, hi, g);
-            else if (CORR == 2) strain_corr_fast(d, i, dt, xi, hi, g);
-            else strain_plain(d, i, dt, xi, vi, hi, g);
+            if (CORR == 1) strain_corr(d, i, xi, hi, g);
+            else if (CORR == 2) strain_corr_fast(d, i, xi, hi, g);
+            else strain_plain(d, i, xi, vi, hi, g);
             rod_spin(g, s);
             x1z = strain_finish(d, i, dt, xi, vi);
             if (MODE == MODE_STRAIN) {
@@ -518,6 +520,7 @@ __device__ __forceinline__ void accel_plain(const DevState& d, int i, const MRec
     const int* lst = nbr_list(d, i);
     acc[0] = accel_start(d, i, 0); acc[1] = accel_start(d, i, 1); acc[2] = accel_start(d, i, 2);
     NbrStream<MRec> st(lst, d.rm, n, i);
+#pragma unroll 2
     for (int k = 0; k < n; k++) {
         int j;
         MRec r;
@@ -555,6 +558,7 @@ __device__ __forceinline__ void accel_corr(const DevState& d, int i, const MRec&
     double sw = 0.0, sg[3] = {0.0, 0.0, 0.0}, massj = 0.0;
     int j;
     NbrStream<MRec> s1(lst, d.rm, n, i);
+#pragma unroll 2
     for (int k = 0; k < n; k++) {
         MRec r;
         s1.next(j, r);
@@ -568,6 +572,7 @@ __device__ __forceinline__ void accel_corr(const DevState& d, int i, const MRec&
     const Divisor isw2(sw2);
     double B[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, invB[9];
     NbrStream<MRec> s2(lst, d.rm, n, i);
+#pragma unroll 2
     for (int k = 0; k < n; k++) {
         MRec r;
         s2.next(j, r);
@@ -582,6 +587,7 @@ __device__ __forceinline__ void accel_corr(const DevState& d, int i, const MRec&
     // deltasig(m,n,l) is symmetric in (m,n): 6 x 3 accumulators
     double D[6][3] = {{0.0}};
     NbrStream<MRec> s3(lst, d.rm, n, i);
+#pragma unroll 2
     for (int k = 0; k < n; k++) {
         MRec r;
         s3.next(j, r);

@@ -254,13 +254,12 @@ struct SPair {
     double xj[3], vj[3], vol, w, dwdx[3], cg[3];
 };
 
-__device__ __forceinline__ void strain_pair(const DevState& d, int j, double dt, const double xi[3], double hi,
+__device__ __forceinline__ void strain_pair(const DevState& d, int j, const double xi[3], double hi,
                                             SPair& q) {
-    SRec r = load_srec(d.rs, j);
-    const double sh = j < d.np ? dt : 0.0;
-    q.xj[0] = r.x - 0.5 * r.vx * sh;
-    q.xj[1] = r.y - 0.5 * r.vy * sh;
-    q.xj[2] = r.z - 0.5 * r.vz * sh;
+    SRec r = load_srec(d.rs, j);   // position: the strain pass's (half step back for real j)
+    q.xj[0] = r.x;
+    q.xj[1] = r.y;
+    q.xj[2] = r.z;
     q.vj[0] = r.vx; q.vj[1] = r.vy; q.vj[2] = r.vz;
     q.vol = r.vol;
     const double havg = 0.5 * (hi + r.h);
@@ -272,7 +271,7 @@ __device__ __forceinline__ void strain_pair(const DevState& d, int j, double dt,
 // velocity gradient of particle i (mcm_oldstrain.f90:34-64 when CORR == 0,
 // mcm_corr_strain.f90:37-146 when CORR == 1), returned in every lane
 template <int CORR>
-__device__ __forceinline__ void strain_wave(const DevState& d, double* buf, int i, int n, double dt,
+__device__ __forceinline__ void strain_wave(const DevState& d, double* buf, int i, int n,
                                             const double xi[3], const double vi[3], double hi, double (&gout)[9]) {
     const int lane = threadIdx.x & 63;
     const int* row = nbr_list(d, i);
@@ -283,7 +282,7 @@ __device__ __forceinline__ void strain_wave(const DevState& d, double* buf, int 
         if (s > 0 && !two) break;
         const int k = lane + 64 * s;
         const int j = k < n ? row[(long)k * 64] : i;
-        strain_pair(d, j, dt, xi, hi, q[s]);
+        strain_pair(d, j, xi, hi, q[s]);
     }
     if (CORR == 0) {
         double t[WAVE_SLOTS][9];
@@ -382,7 +381,7 @@ __global__ __launch_bounds__(256) void k_strain_wave(DevState d_, int cur) {
         }
         const double hi = d.hbuf[cur][i];
         double gsum[9];
-        strain_wave<CORR>(d, buf, i, n, dt, xi, vi, hi, gsum);
+        strain_wave<CORR>(d, buf, i, n, xi, vi, hi, gsum);
         keep_if_lane<9>(p, gsum, mine);
     }
     const int i = wbase + lane;
