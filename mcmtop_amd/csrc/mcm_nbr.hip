@@ -567,21 +567,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 3 :
             };
             auto walk = [&](int st, int en) {
                 // candidates four at a time: the loads of a group are issued
-                // together, the tests run in walk order
-                int s = st;
-                for (; s + 4 <= en; s += 4) {
+                // together (past the run's end they re-read its last
+                // candidate), the tests run in walk order
+                for (int s = st; s < en; s += 4) {
                     int2 idm[4];
                     double2 pa[4], pb[4];
 #pragma unroll
                     for (int u = 0; u < 4; u++) {
-                        if (MULTI) idm[u] = ci[s + u];
-                        pa[u] = cd[2 * (s + u)];
-                        pb[u] = cd[2 * (s + u) + 1];
+                        const int q = min(s + u, en - 1);
+                        if (MULTI) idm[u] = ci[q];
+                        pa[u] = cd[2 * q];
+                        pb[u] = cd[2 * q + 1];
                     }
 #pragma unroll
-                    for (int u = 0; u < 4; u++) consider(s + u, pa[u], pb[u], idm[u]);
+                    for (int u = 0; u < 4; u++)
+                        if (s + u < en) consider(s + u, pa[u], pb[u], idm[u]);
                 }
-                for (; s < en; s++) consider(s, cd[2 * s], cd[2 * s + 1], MULTI ? ci[s] : make_int2(0, 0));
             };
             for (int n = lo2; n <= hi2; n++) {
                 const double fz = n < b2 ? flo[2] : (n > b2 ? fhi[2] : 0.0);
