@@ -483,10 +483,11 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_ghost_fill(DevState d_, int pi, i
 // accepted only when its distance is below h_i + h_j <= h_i + hcap (hcap:
 // largest h of the binned particles), so the real particles of a cell whose
 // box lies at least that far away can never be accepted: such rows and
-// x-run end cells are skipped (with a relative margin of 1e-9 that covers the
-// rounding of the box faces).  Ghosts are not binned by position and are
-// never skipped.  Skipping rejected candidates leaves the list, its order
-// and mindist unchanged.
+// x-run end cells are skipped when every lane of the wavefront can skip them
+// (with a relative margin of 1e-9 that covers the rounding of the box
+// faces).  Ghosts are not binned by position and are never skipped.
+// Skipping rejected candidates leaves the list, its order and mindist
+// unchanged.
 template <bool MULTI>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 3 : 5))) void k_list(DevState d_, int cur) {
     // threads take the binned particles in cell-sorted order, so that the
@@ -589,11 +590,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 3 :
                 for (int m = lo1; m <= hi1; m++) {
                     const double fy = m < b1 ? flo[1] : (m > b1 ? fhi[1] : 0.0);
                     const double dyz = fz * fz + fy * fy;
-                    if (dyz >= reach2) continue;
-                    // x-run of the row, without end cells out of reach
+                    // rows and x-run end cells out of reach are skipped when
+                    // they are for every lane of the wavefront, so that the
+                    // lanes of a cell keep walking the same candidates together
+                    if (__all(dyz >= reach2)) continue;
                     int l0 = lo0, l1 = hi0;
-                    if (l0 < b0 && dyz + flo[0] * flo[0] >= reach2) l0 = b0;
-                    if (l1 > b0 && dyz + fhi[0] * fhi[0] >= reach2) l1 = b0;
+                    if (__all(l0 == b0 || dyz + flo[0] * flo[0] >= reach2)) l0 = b0;
+                    if (__all(l1 == b0 || dyz + fhi[0] * fhi[0] >= reach2)) l1 = b0;
                     // the run is contiguous in the sorted order; a row holding a
                     // ghost-headed cell is walked cell by cell so that that
                     // cell's real particles stay hidden
