@@ -62,6 +62,17 @@ enum { RA_XV = 0, RA_VA = 1, RA_ARG = 2, RA_CDT = 3, RA_PAR = 4, RA_SDT = RA_PAR
 constexpr int RA_N = RA_SDT + 1;
 
 constexpr int ADJ_GOF = 8;                    // ghosts of one real particle (at most 7 for 3 planes)
+// lanes per particle of the reverse pair gathers (a power of two up to 64):
+// lane l of a particle's group takes list entries l, l + kLpp, ... and the
+// group's partial sums are folded by a butterfly in a fixed order.  Eight
+// lanes were 1.3 times as fast as a wavefront per particle on a
+// 50,000-particle bar (the list's rows are 256 bytes apart, and a group of
+// eight keeps eight particles' gathers in flight per wavefront).
+constexpr int kLpp = 8;
+__device__ __forceinline__ double group_sum(double v) {
+    for (int off = kLpp / 2; off > 0; off >>= 1) v = v + __shfl_xor(v, off, 64);
+    return v;
+}
 
 struct AdjDev {
     double *xb[3], *vb[3], *rb, *sb[6], *eb, *fb;   // adjoint of the state (real particles)
@@ -330,13 +341,13 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_adj_fold_move(DevState d, AdjDev 
 //   a_t = sum_u m_u (A_u + A_t) grad W(x_t - x_u),
 //   Abar_t += (m_u abar_t - m_t abar_u) (x) grad W, xbar_t += H (A_u + A_t) (m_u abar_t - m_t abar_u)
 // (abar_u = 0 for ghosts).  A = (sigma - q I) / rho^2 maps to sigma', q', rho'.
-// One wavefront per particle: lane l takes list entries l, l + 64, ...; the
-// sums are combined across lanes in a fixed butterfly order.
+// kLpp lanes per particle: lane l takes list entries l, l + kLpp, ...; the
+// sums are combined across the group's lanes in a fixed butterfly order.
 __global__ __launch_bounds__(MCM_BLOCK) void k_adj_mom(DevState d, AdjDev a) {
     const double ipi = 1.0 / d.pi;
-    const int lane = threadIdx.x & 63;
-    const int t = blockIdx.x * (MCM_BLOCK / 64) + (threadIdx.x >> 6);
-    if (t >= d.np) return;   // the whole wavefront
+    const int lane = threadIdx.x & (kLpp - 1);
+    const int t = blockIdx.x * (MCM_BLOCK / kLpp) + threadIdx.x / kLpp;
+    if (t >= d.np) return;   // the particle's whole group
     const int nxt = a.cur ^ 1;
     const MRec me = load_mrec(d.rm, t);
     double at[3];
@@ -344,7 +355,7 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_adj_mom(DevState d, AdjDev a) {
     double Ab[6] = {0, 0, 0, 0, 0, 0}, xbt[3] = {0, 0, 0};
     const int n = d.ntot[t];
     const int* lst = nbr_list(d, t);
-    for (int k = lane; k < n; k += 64) {
+    for (int k = lane; k < n; k += kLpp) {
         const int u = lst[(long)k * 64];
         const MRec r = load_mrec(d.rm, u);
         const double havg = 0.5 * (me.h + r.h);
@@ -369,8 +380,8 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_adj_mom(DevState d, AdjDev a) {
         hess_apply(kg, xij, bw, hw);
         for (int c = 0; c < 3; c++) xbt[c] += hw[c];
     }
-    for (int c = 0; c < 6; c++) Ab[c] = wave_sum(Ab[c]);
-    for (int c = 0; c < 3; c++) xbt[c] = wave_sum(xbt[c]);
+    for (int c = 0; c < 6; c++) Ab[c] = group_sum(Ab[c]);
+    for (int c = 0; c < 3; c++) xbt[c] = group_sum(xbt[c]);
     if (lane != 0) return;
     const double rho = d.rbuf[nxt][t], r2 = rho * rho;
     double rbar = 0.0;
@@ -670,13 +681,13 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_adj_const_fin(DevState d, AdjDev 
 // mcm_strain.f90:27-31 for real particle t, with t's terms in its real
 // neighbours' sums gathered over t's own list:
 //   g_t[n][m] = sum_u V_u (v_u - v_t)_n grad_m W(xh_t - xh_u), V = m / rho.
-// One wavefront per particle, as k_adj_mom; the particle's share of the dt
+// kLpp lanes per particle, as k_adj_mom; the particle's share of the dt
 // adjoint goes to pdt (reduced by k_adj_gmap_strain).
 __global__ __launch_bounds__(MCM_BLOCK) void k_adj_strain(DevState d, AdjDev a) {
     const double ipi = 1.0 / d.pi;
-    const int lane = threadIdx.x & 63;
-    const int t = blockIdx.x * (MCM_BLOCK / 64) + (threadIdx.x >> 6);
-    if (t >= d.np) return;   // the whole wavefront
+    const int lane = threadIdx.x & (kLpp - 1);
+    const int t = blockIdx.x * (MCM_BLOCK / kLpp) + threadIdx.x / kLpp;
+    if (t >= d.np) return;   // the particle's whole group
     const double dt = d.sc->dtold;
     const SRec me = load_srec(d.rs, t);   // records hold the strain pass's positions
     const double vt[3] = {me.vx, me.vy, me.vz};
@@ -687,7 +698,7 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_adj_strain(DevState d, AdjDev a) 
     double vbt[3] = {0, 0, 0}, xhb[3] = {0, 0, 0}, rbt = 0.0;
     const int n = d.ntot[t];
     const int* lst = nbr_list(d, t);
-    for (int k = lane; k < n; k += 64) {
+    for (int k = lane; k < n; k += kLpp) {
         const int u = lst[(long)k * 64];
         const SRec r = load_srec(d.rs, u);
         const double xj[3] = {r.x, r.y, r.z};
@@ -719,10 +730,10 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_adj_strain(DevState d, AdjDev a) 
         }
     }
     for (int c = 0; c < 3; c++) {
-        vbt[c] = wave_sum(vbt[c]);
-        xhb[c] = wave_sum(xhb[c]);
+        vbt[c] = group_sum(vbt[c]);
+        xhb[c] = group_sum(xhb[c]);
     }
-    rbt = wave_sum(rbt);
+    rbt = group_sum(rbt);
     if (lane != 0) return;
     // xh = x - v dt / 2
     double pdt = 0.0;
@@ -1126,7 +1137,7 @@ int adj_reverse_step(mcm_gpu_adjoint* A, bool ck, int s) {
     a.snap = slot_ptr(A, ck, s);
     a.cur = (ck ? A->ckpt_cur : A->seg_cur)[s];
     const int nb = ctx->nb, gb = ghost_blocks(d);
-    const int wb = blocks_for(A->np, MCM_BLOCK / 64);                                    // wave per particle
+    const int wb = blocks_for(A->np, MCM_BLOCK / kLpp);                                  // kLpp lanes per particle
     const int wg = std::max(1, std::min(blocks_for(d.ngcap, MCM_BLOCK / 64), 4096));   // wave per ghost
     // transposed lists; an overflow sets err bit 1 (checked after the sweep)
     HIPCHK_A(hipMemsetAsync(a.gt_cnt, 0, sizeof(int) * a.ngcap, st));
