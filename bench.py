@@ -238,17 +238,20 @@ def adjoint_leg(steps: int, ckpt: int, dx: float, nsample: int, fd: bool):
             with torch.no_grad():
                 xx, hh = ident.run(torch.tensor(th_), torch.from_numpy(v_))
                 return adjoint.identification_loss(xx, hh, ident.x0s, ident.target).item()
-        rel, spread = {}, {}
-        for k, name in enumerate(("E", "sigma_y", "E_tan", "S1")):
-            fds = {}
-            for step in (1e-5, 1e-6, 1e-7):
-                h = step * theta[k]
-                tp, tm = theta.copy(), theta.copy()
-                tp[k] += h
-                tm[k] -= h
-                fds[step] = (loss_at(tp, dk.v) - loss_at(tm, dk.v)) / (tp[k] - tm[k])
-            rel[name] = abs(g_theta[k] - fds[1e-6]) / abs(fds[1e-6])
-            spread[name] = (max(fds.values()) - min(fds.values())) / abs(fds[1e-6])
+        def param_fd(g):
+            rel_, spread_ = {}, {}
+            for k, name in enumerate(("E", "sigma_y", "E_tan", "S1")):
+                fds = {}
+                for step in (1e-5, 1e-6, 1e-7):
+                    h = step * theta[k]
+                    tp, tm = theta.copy(), theta.copy()
+                    tp[k] += h
+                    tm[k] -= h
+                    fds[step] = (loss_at(tp, dk.v) - loss_at(tm, dk.v)) / (tp[k] - tm[k])
+                rel_[name] = abs(g[k] - fds[1e-6]) / abs(fds[1e-6])
+                spread_[name] = (max(fds.values()) - min(fds.values())) / abs(fds[1e-6])
+            return rel_, spread_
+        rel, spread = param_fd(g_theta)
         for f_ in np.argsort(-np.abs(g_v).ravel())[:2]:
             i, c = divmod(int(f_), 3)
             vp, vm = dk.v.copy(), dk.v.copy()
@@ -261,6 +264,15 @@ def adjoint_leg(steps: int, ckpt: int, dx: float, nsample: int, fd: bool):
         out["fd"] = ("central differences of the GPU forward, relative step 1e-6, velocity step 1e-3 m/s; "
                      "fd_spread: relative spread of the differences at steps 1e-5, 1e-6, 1e-7 (kinks of the "
                      "800-step loss: yield transitions, min/max switches)")
+        # the same bar and loss over a shorter horizon, whose loss has fewer kinks
+        short = min(100, steps)
+        ident.nsteps, ident.ckpt_every = short, 1
+        ident.make_target(hidden)
+        th = torch.tensor(theta, dtype=torch.float64, requires_grad=True)
+        x, hist = ident.run(th, torch.from_numpy(dk.v.copy()))
+        adjoint.identification_loss(x, hist, ident.x0s, ident.target).backward()
+        rel_s, spread_s = param_fd(th.grad.numpy().copy())
+        out["fd_short"] = {"steps": short, "rel_error": rel_s, "spread": spread_s}
     ident.close()
     return out
 
