@@ -99,7 +99,8 @@ struct DevState {
                                // an odd number of times (product of the plane multipliers is -1)
     double *hold, *h0, *rho0, *p, *e, *c, *vabs, *pcut, *p_cut, *efps, *fail, *q, *mindist;
     double *sig[6], *sd[6], *alfa[6];
-    double* rs;                // [U][8] strain gather record {x, v, m/rho, h} (start of step)
+    double* rs;                // [U][8] strain gather record {x, v, m/rho, h}: x the strain pass's
+                               // position (x - v dt / 2 for real particles), the rest start of step
     double* rm;                // [U][12] momentum gather record {x, h, m, m/rho, (sigma - q)/rho^2}
     // neighbour search
     int *cell_of, *slot_of, *cell_count, *cell_start, *sorted;
