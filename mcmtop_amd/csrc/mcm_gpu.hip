@@ -1374,6 +1374,9 @@ int mcm_gpu_rep_cont(mcm_gpu_ctx* ctx) {
 int mcm_gpu_dt_update(mcm_gpu_ctx* ctx) {
     if (!ctx) return MCM_GPU_ERR_ARG;
     k_dt_update<<<1, 1, 0, ctx->stream>>>(ctx->d);
+    // before a step's strain pass (the reference calls it after the
+    // constitutive update): the strain records' half step follows the new dt
+    if (ctx->phase == 0) k_rs_shift<<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(ctx->d);
     HIPCHK(hipGetLastError());
     return 0;
 }

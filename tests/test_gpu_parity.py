@@ -326,3 +326,24 @@ def test_first_step_may_reuse_the_initial_search(name, tmp_path):
     a, b = first_step(True), first_step(False)
     _assert_same_state(a, b)
     assert np.array_equal(a.nbr, b.nbr) and np.array_equal(a.gnbr, b.gnbr)
+
+
+@pytest.mark.parametrize("name", ["plain_f3dm3", "corr_f3dm10_eos4_hopt1"])
+def test_dt_update_between_search_and_strain(name, tmp_path):
+    """A dt update between the neighbour search and the strain pass (off the
+    reference's order) moves the strain records' half step with it: the
+    strain pass then equals the one after a search made with the new dt."""
+    path = os.path.join(DATA, name + ".bin")
+
+    def strain_after(order, tag):
+        with gpu.Solver(path) as s:
+            s.initial()
+            for st in order:
+                s.stage(st)
+            return gdump(s, tmp_path, 1, tag)
+
+    a = strain_after(("neighbours", "dt_update", "strain"), 1)
+    b = strain_after(("dt_update", "neighbours", "strain"), 2)
+    for k in ("x", "rod", "spin", "tracerod"):
+        np.testing.assert_array_equal(a.f(k), b.f(k), err_msg=k)
+    assert a.dt == b.dt and np.array_equal(a.nbr, b.nbr)
