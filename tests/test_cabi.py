@@ -33,3 +33,14 @@ def test_missing_library_fails_loudly(tmp_path):
         gpu._LIB = None
         gpu.load_library(str(tmp_path / "absent.so"))
     gpu._LIB = None
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """The step runs on one GPU: bench.py --gpus 2 exits with a message
+    before it touches a device."""
+    import subprocess
+    import sys
+    out = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 2
+    assert "one GPU" in out.stderr and out.stdout == ""
