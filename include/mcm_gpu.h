@@ -81,7 +81,16 @@ const char *mcm_gpu_last_error(const mcm_gpu_ctx *ctx);
    init_neighb, calccritts, first dt, calctotale, initold). */
 int mcm_gpu_initial(mcm_gpu_ctx *ctx);
 
-/* Stage entry points, one per reference subroutine called by mcm_solution:
+/* Stage entry points, one per reference subroutine called by mcm_solution.
+   They must be called in the order of the loop body (mcm_solution.f90:41-116):
+   neighbours, strain, update_h, constitutive, dt update (mcm_gpu_dt_update or
+   mcm_gpu_set_time), rep_cont, momentum, boundcond, calctotale,
+   update_velocity, move.  Each stage reads what the stages before it wrote,
+   so a call out of that order, a second call of a stage in one step, a strain
+   pass after a move without a new search, or any stage call on a run that
+   belongs to a batch returns MCM_GPU_ERR_ARG with a message and changes
+   nothing.  calctotale and calccritts are reductions and may be called at any
+   point.  update_h may be skipped (h_opt 0), as may rep_cont and boundcond.
    mcm_neighbours (mcm_neighbours.f90:29-49: setuplist + ghost_setup + ll_neighbours) */
 int mcm_gpu_neighbours(mcm_gpu_ctx *ctx);
 /* mcm_strain (mcm_strain.f90:27-60: half-step shift, oldstrain / corr_strain) */
@@ -90,8 +99,19 @@ int mcm_gpu_strain(mcm_gpu_ctx *ctx);
 int mcm_gpu_update_h(mcm_gpu_ctx *ctx);
 /* mcm_constitutive (mcm_constitutive.f90:26-87: rhoupd, per-particle models, calccritts) */
 int mcm_gpu_constitutive(mcm_gpu_ctx *ctx);
-/* dt/ptime update of mcm_solution.f90:58-62 (kept on the device) */
+/* dt/ptime update of mcm_solution.f90:58-62 (kept on the device).  Once per
+   step, after the constitutive update (the reference's place) or at the step
+   start before the strain pass; there, after a search, it also rewrites the
+   strain records' half-step positions with the new dt. */
 int mcm_gpu_dt_update(mcm_gpu_ctx *ctx);
+/* The same update with the values a host computed itself: a host that keeps
+   the dt lines of mcm_solution.f90:58-62 (reading critts from
+   mcm_gpu_get_scalars after mcm_gpu_constitutive) hands over its mcm_dt,
+   mcm_dtold and mcm_ptime here, at a step start (where, after a search, the
+   strain records take the new dt) or between the constitutive update and the
+   momentum stage.  The values are absolute, so repeated calls are allowed.
+   dt, dtold >= 0 and all three finite, else MCM_GPU_ERR_ARG. */
+int mcm_gpu_set_time(mcm_gpu_ctx *ctx, double dt, double dtold, double ptime);
 /* mcm_rep_cont (mcm_rep_contact.f90:20-98), called by mcm_solution.f90:64
    after the dt update when contacttype > 0: Monaghan's repulsive acceleration
    from the other-material contact partners, added to the acceleration by
@@ -103,6 +123,10 @@ int mcm_gpu_momentum(mcm_gpu_ctx *ctx);
 int mcm_gpu_boundcond(mcm_gpu_ctx *ctx);
 /* mcm_calctotale (mcm_calctotale.f90:27-60) */
 int mcm_gpu_calctotale(mcm_gpu_ctx *ctx);
+/* The per-material totals of the last mcm_gpu_calctotale (or fused step):
+   out[5 m .. 5 m + 4] = ke, ie, mom(3) of material m + 1, the reference's
+   mcm_mat(m+1)%ke, %ie, %mom (mcm_calctotale.f90:33-52); n >= 5 nummat. */
+int mcm_gpu_get_energies(mcm_gpu_ctx *ctx, double *out, int32_t n);
 /* mcm_update_velocity (mcm_update_velocity.f90:29-87: velocity update, and
    the XSPH smoothing when veloc_opt = 1) */
 int mcm_gpu_update_velocity(mcm_gpu_ctx *ctx);
@@ -132,9 +156,31 @@ int mcm_gpu_get_scalars(mcm_gpu_ctx *ctx, mcm_gpu_scalars *out);
 #define MCM_F_VABS 10
 #define MCM_F_MASS 11
 int mcm_gpu_download(mcm_gpu_ctx *ctx, int32_t field, double *host, int64_t n);
+/* The per-particle state as the host's particle records hold it
+   (mcm_database.f90:159-216: the par(:) array), for hosts that keep their own
+   copy: one row of MCM_STATE_WORDS doubles per particle, in deck order, in
+   the order of the state dumps: x(3), v(3), a(3), rho, rhoold, h, hold, p, e,
+   etry, einc, c, vabs, tracerod, pcut, p_cut, efps, fail, mindist, critts,
+   mass, rho0, sigma(9), s(9), alfa(9), q(9), rod(9), spin(9) (3x3 tensors
+   column-major), h0, xzero(3).
+   mcm_gpu_download_state copies the current state into rows[np *
+   MCM_STATE_WORDS] (rows may be NULL) and the scalars into *sc (may be
+   NULL); it synchronises the stream.  Called before the host's writers read
+   par(:) (mcm_state_output, mcm_timehist: mcm_solution.f90:82-92).
+   mcm_gpu_upload_state replaces the state of the real particles at a step
+   boundary with the host's (symmetric tensors from their upper triangle, q
+   from q(1,1), spin from W12 W13 W23), and the scalars dt, dtold, ptime,
+   critts, hmax, coord_min/max and timestep with *sc: a host that ran the
+   initialisation (mcm_initial.f90:33-101) itself uploads once, after which
+   the stages continue from its state.  The next strain pass needs a search
+   first.  np must be the run's count; hmax > 0. */
+#define MCM_STATE_WORDS 86
+int mcm_gpu_download_state(mcm_gpu_ctx *ctx, double *rows, int64_t np, mcm_gpu_scalars *sc);
+int mcm_gpu_upload_state(mcm_gpu_ctx *ctx, const double *rows, int64_t np, const mcm_gpu_scalars *sc);
 /* Neighbour lists in reference order: counts[np] (real incl. self),
    gcounts[np] (ghost), ids[np*cap] row-major with 0-based ids (ghosts as
-   np + ghost index); returns the capacity used per row in *cap. */
+   np + ghost index); returns the capacity used per row in *cap.  ids may be
+   NULL for the counts alone (the host's par%nnbr and par%g_nnbr). */
 int mcm_gpu_download_neighbours(mcm_gpu_ctx *ctx, int32_t *counts, int32_t *gcounts, int32_t *ids,
                                 int64_t ids_len, int32_t *cap);
 /* Full state dump in the layout of the reference probe (oracle/ref_probe.f90). */

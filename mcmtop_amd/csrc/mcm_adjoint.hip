@@ -982,6 +982,8 @@ int adj_restore(mcm_gpu_adjoint* A, bool ck, int s) {
                             ctx->stream));
     ctx->cur = (ck ? A->ckpt_cur : A->seg_cur)[s];
     ctx->phase = 0;
+    ctx->done = 0;
+    ctx->fresh = false;
     HIPCHK_A(hipGetLastError());
     return 0;
 }
@@ -1305,7 +1307,8 @@ int mcm_gpu_adj_forward(mcm_gpu_ctx* ctx, int32_t nsteps, int32_t ckpt_every, co
     if (prev && prev->ctx != ctx) { ctx->err = "the recording to reuse belongs to another run"; return MCM_GPU_ERR_ARG; }
     const DevState& d = ctx->d;
     std::string why;
-    if (ctx->phase != 0 || ctx->batch) why = "the run must be at a step boundary and not part of a batch";
+    if (ctx->phase != 0 || ctx->done != 0 || ctx->batch)
+        why = "the run must be at a step boundary and not part of a batch";
     else if (d.disctype != 0) why = "the adjoint covers plain SPH (disctype 0)";
     else if (d.h_opt != 0) why = "the adjoint covers fixed h (h_opt 0)";
     else if (d.tcrit_opt != 0 && d.tcrit_opt != 1) why = "the adjoint covers tcrit_opt 0 and 1";
@@ -1320,6 +1323,7 @@ int mcm_gpu_adj_forward(mcm_gpu_ctx* ctx, int32_t nsteps, int32_t ckpt_every, co
         else seen[sample[k]] = 1;
     }
     if (!why.empty()) { ctx->err = why; return MCM_GPU_ERR_ARG; }
+    ctx->fresh = false;   // the recorded steps are fused steps, each with its own search
     int rc = 0;
     if (prev && prev->nsteps == nsteps && prev->k == ckpt_every && prev->nsample == nsample && prev->np == d.np) {
         // same shape: the device buffers are reused (no allocation, no free,

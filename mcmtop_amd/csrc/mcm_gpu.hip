@@ -63,6 +63,9 @@ struct mcm_gpu_ctx {
     hipGraphExec_t graph2 = nullptr;   // two fused steps: parity 0 then 1
     int cur = 0;                       // parity of the current h / rho buffers
     int phase = 0;                     // 0: step start, 1: after strain (x in x1, h/rho in next)
+    bool fresh = false;                // strain records hold this step's search (set by the search and
+                                       // the initialisation, cleared by a move or a fused step)
+    unsigned done = 0;                 // stages of the current step already run (ST_* bits)
     int nb = 0;                        // blocks over np
     int corr_one_pass = 0;             // corrected SPH: one-pass moments instead of the reference's three passes
     int pair_mapping = 0;              // 0 thread per particle, 1 wavefront per particle (lists up to 128)
@@ -814,6 +817,29 @@ int put_scalars_raw(mcm_gpu_ctx* ctx, const Scalars& in) {
     return 0;
 }
 
+// Stages of one loop body of mcm_solution.f90:41-116, one bit each in
+// mcm_gpu_ctx::done once run in the current step (cleared by the move).
+enum : unsigned {
+    ST_STRAIN = 1u, ST_UPDATE_H = 2u, ST_CONST = 4u, ST_DT = 8u, ST_REP = 16u, ST_MOMENTUM = 32u,
+    ST_BOUNDCOND = 64u, ST_VELOCITY = 128u
+};
+
+// Order check of a stage entry point: every stage reads what the stages
+// before it in the loop body wrote, so a call out of that order (or twice in
+// one step, or on a run that a batch steps) is refused with MCM_GPU_ERR_ARG
+// and a message instead of computing from stale data.
+int stage_check(mcm_gpu_ctx* ctx, const char* stage, bool ok, const char* needs) {
+    if (ctx->batch) {
+        ctx->err = std::string(stage) + ": the run belongs to a batch; step it with mcm_gpu_batch_step or mcm_gpu_step";
+        return MCM_GPU_ERR_ARG;
+    }
+    if (!ok) {
+        ctx->err = std::string(stage) + " called out of order: " + needs;
+        return MCM_GPU_ERR_ARG;
+    }
+    return 0;
+}
+
 void drop_graph(mcm_gpu_ctx* ctx) {
     if (ctx->graph2) { (void)hipGraphExecDestroy(ctx->graph2); ctx->graph2 = nullptr; }
 }
@@ -904,6 +930,154 @@ void init_particle(PStateT<T>& s, const M& Mt, const HostDeck& D, long i, double
         sueos(s, Mt);
         if (Mt.model == 9 && s.p < Mt.strinput[0]) s.p = Mt.strinput[0];
     }
+}
+
+// Per-particle state as rows of MCM_STATE_WORDS values, the fields of the
+// reference's particle record (mcm_database.f90:159-216) in the order of the
+// state dumps: x(3), v(3), a(3), rho, rhoold, h, hold, p, e, etry, einc, c,
+// vabs, tracerod, pcut, p_cut, efps, fail, mindist, critts, mass, rho0,
+// sigma(9), s(9), alfa(9), q(9), rod(9), spin(9) (3x3 column-major), h0,
+// xzero(3).  ghost (when given) receives the ghosts' x, v, mass, h (8 per
+// ghost) followed by their rho.
+int gather_rows(mcm_gpu_ctx* ctx, const Scalars& s, std::vector<double>& rows, std::vector<double>* ghost) {
+    DevState& d = ctx->d;
+    const size_t np = d.np;
+    const int hb = ctx->phase == 1 ? ctx->cur ^ 1 : ctx->cur;
+    hipError_t err = hipSuccess;
+    auto get = [&](const double* src, size_t n) {
+        std::vector<double> v(n);
+        if (!err) err = hipMemcpy(v.data(), src, n * sizeof(double), hipMemcpyDeviceToHost);
+        return v;
+    };
+    const size_t ng = ghost ? (size_t)s.ngp : 0;
+    std::vector<double> X[3], V[3], Acc[3], SG[6], SD[6], AL[6], ROD[6], SP[3];
+    for (int k = 0; k < 3; k++) {
+        X[k] = get(ctx->phase == 1 ? d.x1[k] : d.x0[k], np + ng);
+        V[k] = get(d.v[k], np + ng);
+        Acc[k] = get(d.a[k], np);
+        SP[k] = get(d.spin[k], np);
+    }
+    for (int k = 0; k < 6; k++) {
+        SG[k] = get(d.sig[k], np); SD[k] = get(d.sd[k], np); AL[k] = get(d.alfa[k], np); ROD[k] = get(d.rod[k], np);
+    }
+    auto rho = get(d.rbuf[hb], np + ng), h = get(d.hbuf[hb], np + ng), mass = get(d.mass, np + ng);
+    auto rhoold = get(d.rhoold, np), hold = get(d.hold, np), p = get(d.p, np), e = get(d.e, np);
+    auto etry = get(d.etry, np), einc = get(d.einc, np), c = get(d.c, np), vabs = get(d.vabs, np);
+    auto tr = get(d.tracerod, np), pcut = get(d.pcut, np), p_cut = get(d.p_cut, np), efps = get(d.efps, np);
+    auto fail = get(d.fail, np), mind = get(d.mindist, np), crit = get(d.critts_p, np), rho0 = get(d.rho0, np);
+    auto h0 = get(d.h0, np), q = get(d.q, np);
+    if (err) { ctx->err = std::string("state download: ") + hipGetErrorString(err); return MCM_GPU_ERR_HIP; }
+    static const int map[9] = {S11, S12, S13, S12, S22, S23, S13, S23, S33};
+    rows.assign(np * MCM_STATE_WORDS, 0.0);
+    for (size_t i = 0; i < np; i++) {
+        double* row = &rows[i * MCM_STATE_WORDS];
+        for (int k = 0; k < 3; k++) { row[k] = X[k][i]; row[3 + k] = V[k][i]; row[6 + k] = Acc[k][i]; }
+        row[9] = rho[i]; row[10] = rhoold[i]; row[11] = h[i]; row[12] = hold[i]; row[13] = p[i]; row[14] = e[i];
+        row[15] = etry[i]; row[16] = einc[i]; row[17] = c[i]; row[18] = vabs[i]; row[19] = tr[i]; row[20] = pcut[i];
+        row[21] = p_cut[i]; row[22] = efps[i]; row[23] = fail[i]; row[24] = mind[i]; row[25] = crit[i];
+        row[26] = mass[i]; row[27] = rho0[i];
+        for (int k = 0; k < 9; k++) {
+            int m = map[k];
+            row[28 + k] = SG[m][i]; row[37 + k] = SD[m][i]; row[46 + k] = AL[m][i];
+            row[55 + k] = (k == 0 || k == 4 || k == 8) ? q[i] : 0.0;
+            row[64 + k] = ROD[m][i];
+        }
+        // spin: column-major (r,c) -> W12 at (0,1)=k3, W13 at (0,2)=k6, W23 at (1,2)=k7
+        double w12 = SP[0][i], w13 = SP[1][i], w23 = SP[2][i];
+        double sp[9] = {0.0, -w12, -w13, w12, 0.0, -w23, w13, w23, 0.0};
+        for (int k = 0; k < 9; k++) row[73 + k] = sp[k];
+        row[82] = h0[i];
+        for (int k = 0; k < 3; k++) row[83 + k] = ctx->xzero[3 * i + k];
+    }
+    if (ghost) {
+        ghost->assign(9 * ng, 0.0);
+        for (size_t g = 0; g < ng; g++) {
+            size_t u = np + g;
+            double r8[8] = {X[0][u], X[1][u], X[2][u], V[0][u], V[1][u], V[2][u], mass[u], h[u]};
+            for (int k = 0; k < 8; k++) (*ghost)[8 * g + k] = r8[k];
+            (*ghost)[8 * ng + g] = rho[u];
+        }
+    }
+    return 0;
+}
+
+// The inverse of gather_rows for the real particles at a step boundary (h
+// and rho into both parity buffers), plus the scalars the next step starts
+// from.  The strain records are laid down by the next search.
+int scatter_rows(mcm_gpu_ctx* ctx, const double* rows, const mcm_gpu_scalars& sc) {
+    DevState& d = ctx->d;
+    const size_t np = d.np;
+    hipStream_t st = ctx->stream;
+    hipError_t err = hipSuccess;
+    std::vector<std::vector<double>> keep;   // host columns alive until the copies are done
+    auto put = [&](double* dst, int word) {
+        if (err) return;
+        keep.emplace_back(np);
+        std::vector<double>& c = keep.back();
+        for (size_t i = 0; i < np; i++) c[i] = rows[i * MCM_STATE_WORDS + word];
+        err = hipMemcpyAsync(dst, c.data(), np * sizeof(double), hipMemcpyHostToDevice, st);
+    };
+    for (int k = 0; k < 3; k++) { put(d.x0[k], k); put(d.v[k], 3 + k); put(d.a[k], 6 + k); }
+    for (int b = 0; b < 2; b++) { put(d.rbuf[b], 9); put(d.hbuf[b], 11); }
+    put(d.rhoold, 10); put(d.hold, 12); put(d.p, 13); put(d.e, 14); put(d.etry, 15); put(d.einc, 16);
+    put(d.c, 17); put(d.vabs, 18); put(d.tracerod, 19); put(d.pcut, 20); put(d.p_cut, 21); put(d.efps, 22);
+    put(d.fail, 23); put(d.mindist, 24); put(d.critts_p, 25); put(d.mass, 26); put(d.rho0, 27);
+    // symmetric tensors from their upper triangle (column-major (r,c) at r + 3c)
+    static const int upper[6] = {0, 4, 8, 3, 6, 7};   // 11 22 33 12 13 23
+    for (int k = 0; k < 6; k++) {
+        put(d.sig[k], 28 + upper[k]); put(d.sd[k], 37 + upper[k]); put(d.alfa[k], 46 + upper[k]);
+        put(d.rod[k], 64 + upper[k]);
+    }
+    put(d.q, 55);                                          // q is isotropic: q(1,1)
+    put(d.spin[0], 73 + 3); put(d.spin[1], 73 + 6); put(d.spin[2], 73 + 7);   // W12 W13 W23
+    put(d.h0, 82);
+    // material ids, displacement codes and the material table come from the deck
+    const HostDeck& D = ctx->deck;
+    if (!err) err = hipMemcpyAsync(d.pmat, D.pmat.data(), np * sizeof(int), hipMemcpyHostToDevice, st);
+    if (!err) err = hipMemcpyAsync(d.dispbc, D.dispbc.data(), np * sizeof(int), hipMemcpyHostToDevice, st);
+    if (!err) err = hipMemcpyAsync(d.mat, D.mat.data(), D.nummat * sizeof(MatParams), hipMemcpyHostToDevice, st);
+    if (!err) err = hipMemsetAsync(d.defps_prev, 0, np * sizeof(double), st);
+    if (!err) err = hipMemsetAsync(d.yield, 0, np * sizeof(int), st);
+    if (!err) err = hipStreamSynchronize(st);
+    if (err) { ctx->err = std::string("state upload: ") + hipGetErrorString(err); return MCM_GPU_ERR_HIP; }
+    ctx->xzero.resize(3 * np);
+    for (size_t i = 0; i < np; i++)
+        for (int k = 0; k < 3; k++) ctx->xzero[3 * i + k] = rows[i * MCM_STATE_WORDS + 83 + k];
+    Scalars s{};
+    int rc = get_scalars_raw(ctx, &s);
+    if (rc) return rc;
+    s.dt = sc.dt; s.dtold = sc.dtold; s.ptime = sc.ptime; s.critts = sc.critts; s.hmax = sc.hmax;
+    for (int k = 0; k < 3; k++) { s.coord_min[k] = sc.coord_min[k]; s.coord_max[k] = sc.coord_max[k]; }
+    s.timestep = sc.timestep;
+    s.abort = 0;
+    s.error_flags = 0;
+    if ((rc = put_scalars_raw(ctx, s))) return rc;
+    ctx->phase = 0;
+    ctx->done = 0;
+    ctx->fresh = false;
+    return 0;
+}
+
+// The neighbour search of mcm_neighbours.f90:29-49 (setuplist, ghost_setup,
+// ll_neighbours); a capacity that overflows grows and the search is repeated
+// (mcm_ll_neighbours.f90:44-311 stops instead).  Lays down this step's strain
+// records.
+int search(mcm_gpu_ctx* ctx) {
+    for (int attempt = 0; attempt < 8; attempt++) {
+        launch_neighbours(one(ctx), ctx->cur);
+        HIPCHK(hipGetLastError());
+        Scalars s;
+        int rc = get_scalars_raw(ctx, &s);
+        if (rc) return rc;
+        if (!s.abort) {
+            ctx->fresh = true;
+            return MCM_GPU_OK;
+        }
+        rc = regrow(ctx, s);
+        if (rc) return rc;
+    }
+    ctx->err = "neighbour search did not fit after regrowing";
+    return MCM_GPU_ERR_CAPACITY;
 }
 
 // mcm_initial.f90:33-101: the per-particle part runs once on the host, the
@@ -1005,8 +1179,9 @@ int initial(mcm_gpu_ctx* ctx) {
     if (rc) return rc;
     ctx->cur = 0;
     ctx->phase = 0;
+    ctx->done = 0;
     // mcm_init_neighb -> calccritts -> dt -> calctotale
-    rc = mcm_gpu_neighbours(ctx);
+    rc = search(ctx);
     if (rc) return rc;
     k_calccritts<<<ctx->nb, MCM_BLOCK, 0, st>>>(d, 0);
     k_calccritts_fold<<<1, MCM_BLOCK, 0, st>>>(d, ctx->nb);
@@ -1318,23 +1493,17 @@ int mcm_gpu_initial(mcm_gpu_ctx* ctx) {
 }
 
 int mcm_gpu_neighbours(mcm_gpu_ctx* ctx) {
-    if (!ctx || ctx->phase != 0) return MCM_GPU_ERR_ARG;
-    for (int attempt = 0; attempt < 8; attempt++) {
-        launch_neighbours(one(ctx), ctx->cur);
-        HIPCHK(hipGetLastError());
-        Scalars s;
-        int rc = get_scalars_raw(ctx, &s);
-        if (rc) return rc;
-        if (!s.abort) return MCM_GPU_OK;
-        rc = regrow(ctx, s);   // mcm_ll_neighbours.f90:44-311 regrow, done right
-        if (rc) return rc;
-    }
-    ctx->err = "neighbour search did not fit after regrowing";
-    return MCM_GPU_ERR_CAPACITY;
+    if (!ctx) return MCM_GPU_ERR_ARG;
+    int rc = stage_check(ctx, "mcm_gpu_neighbours", ctx->phase == 0 && !(ctx->done & ST_STRAIN),
+                         "the search starts a step (after mcm_gpu_initial or mcm_gpu_move)");
+    return rc ? rc : search(ctx);
 }
 
 int mcm_gpu_strain(mcm_gpu_ctx* ctx) {
-    if (!ctx || ctx->phase != 0) return MCM_GPU_ERR_ARG;
+    if (!ctx) return MCM_GPU_ERR_ARG;
+    int rc = stage_check(ctx, "mcm_gpu_strain", ctx->phase == 0 && ctx->fresh,
+                         "the strain pass needs this step's neighbour search (mcm_gpu_neighbours after a move)");
+    if (rc) return rc;
     DevState& d = ctx->d;
     switch (corr_variant(ctx)) {
         case 1: k_strain_const<MODE_STRAIN, 1, true><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, ctx->cur); break;
@@ -1343,46 +1512,100 @@ int mcm_gpu_strain(mcm_gpu_ctx* ctx) {
     }
     HIPCHK(hipGetLastError());
     ctx->phase = 1;
+    ctx->done |= ST_STRAIN;
     return 0;
 }
 
 int mcm_gpu_update_h(mcm_gpu_ctx* ctx) {
-    if (!ctx || ctx->phase != 1) return MCM_GPU_ERR_ARG;
+    if (!ctx) return MCM_GPU_ERR_ARG;
+    int rc = stage_check(ctx, "mcm_gpu_update_h", ctx->phase == 1 && !(ctx->done & (ST_UPDATE_H | ST_CONST)),
+                         "update_h runs once, after the strain pass and before the constitutive update");
+    if (rc) return rc;
     if (ctx->d.h_opt > 0) {
         k_update_h<<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(ctx->d, ctx->cur);
         k_update_h_fold<<<1, MCM_BLOCK, 0, ctx->stream>>>(ctx->d, ctx->nb);
     }
     HIPCHK(hipGetLastError());
+    ctx->done |= ST_UPDATE_H;
     return 0;
 }
 
 int mcm_gpu_constitutive(mcm_gpu_ctx* ctx) {
-    if (!ctx || ctx->phase != 1) return MCM_GPU_ERR_ARG;
+    if (!ctx) return MCM_GPU_ERR_ARG;
+    int rc = stage_check(ctx, "mcm_gpu_constitutive", ctx->phase == 1 && !(ctx->done & ST_CONST),
+                         "the constitutive update runs once, after the strain pass");
+    if (rc) return rc;
     k_strain_const<MODE_CONST, 0, true><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(ctx->d, ctx->cur);
     k_strain_fold<MODE_CONST><<<1, MCM_BLOCK, 0, ctx->stream>>>(ctx->d);
     HIPCHK(hipGetLastError());
+    ctx->done |= ST_CONST;
     return 0;
 }
 
 int mcm_gpu_rep_cont(mcm_gpu_ctx* ctx) {
-    if (!ctx || ctx->phase != 1) return MCM_GPU_ERR_ARG;
+    if (!ctx) return MCM_GPU_ERR_ARG;
+    int rc = stage_check(ctx, "mcm_gpu_rep_cont",
+                         ctx->phase == 1 && (ctx->done & ST_DT) && !(ctx->done & (ST_REP | ST_MOMENTUM)),
+                         "the contact repulsion runs once, after the dt update and before the momentum stage");
+    if (rc) return rc;
     if (ctx->d.contacttype > 0) k_rep_cont<<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(ctx->d, ctx->cur);
     HIPCHK(hipGetLastError());
+    ctx->done |= ST_REP;
+    return 0;
+}
+
+// dt update: after the constitutive update (the reference's place), or at a
+// step start before the strain pass
+static int time_check(mcm_gpu_ctx* ctx, const char* stage) {
+    return stage_check(ctx, stage, !(ctx->done & ST_DT) && (ctx->phase == 0 || (ctx->done & ST_CONST)),
+                       "the dt update runs once per step, after the constitutive update or before the strain pass");
+}
+
+// after a new dt at a step start, the strain records of this step's search
+// take their half step with it (a later search lays them down itself)
+static int time_done(mcm_gpu_ctx* ctx) {
+    if (ctx->phase == 0 && ctx->fresh) k_rs_shift<<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(ctx->d);
+    HIPCHK(hipGetLastError());
+    ctx->done |= ST_DT;
     return 0;
 }
 
 int mcm_gpu_dt_update(mcm_gpu_ctx* ctx) {
     if (!ctx) return MCM_GPU_ERR_ARG;
+    int rc = time_check(ctx, "mcm_gpu_dt_update");
+    if (rc) return rc;
     k_dt_update<<<1, 1, 0, ctx->stream>>>(ctx->d);
-    // before a step's strain pass (the reference calls it after the
-    // constitutive update): the strain records' half step follows the new dt
-    if (ctx->phase == 0) k_rs_shift<<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(ctx->d);
-    HIPCHK(hipGetLastError());
-    return 0;
+    return time_done(ctx);
+}
+
+int mcm_gpu_set_time(mcm_gpu_ctx* ctx, double dt, double dtold, double ptime) {
+    if (!ctx) return MCM_GPU_ERR_ARG;
+    if (!(dt >= 0.0) || !(dtold >= 0.0) || !std::isfinite(dt) || !std::isfinite(dtold) || !std::isfinite(ptime)) {
+        ctx->err = "mcm_gpu_set_time: dt and dtold must be finite and not negative, ptime finite";
+        return MCM_GPU_ERR_ARG;
+    }
+    // absolute values, so a repeated call changes nothing; after the momentum
+    // stage has used dt it would split the step between two values
+    int rc = stage_check(ctx, "mcm_gpu_set_time",
+                         ctx->phase == 0 || ((ctx->done & ST_CONST) && !(ctx->done & ST_MOMENTUM)),
+                         "the host's dt is taken at a step start or between the constitutive update and the "
+                         "momentum stage");
+    if (rc) return rc;
+    Scalars s;
+    if ((rc = get_scalars_raw(ctx, &s))) return rc;
+    s.dt = dt;
+    s.dtold = dtold;
+    s.ptime = ptime;
+    if ((rc = put_scalars_raw(ctx, s))) return rc;
+    return time_done(ctx);
 }
 
 int mcm_gpu_momentum(mcm_gpu_ctx* ctx) {
-    if (!ctx || ctx->phase != 1) return MCM_GPU_ERR_ARG;
+    if (!ctx) return MCM_GPU_ERR_ARG;
+    int rc = stage_check(ctx, "mcm_gpu_momentum",
+                         ctx->phase == 1 && (ctx->done & ST_CONST) && (ctx->done & ST_DT) && !(ctx->done & ST_MOMENTUM),
+                         "the momentum stage runs once, after the constitutive update and the dt update");
+    if (rc) return rc;
     DevState& d = ctx->d;
     if (d.boundary) k_ghost_stress<<<ghost_blocks(d), MCM_BLOCK, 0, ctx->stream>>>(d, ctx->cur);
     switch (corr_variant(ctx)) {
@@ -1391,18 +1614,27 @@ int mcm_gpu_momentum(mcm_gpu_ctx* ctx) {
         default: k_momentum<MODE_MOMENTUM, 0, true><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, ctx->cur); break;
     }
     HIPCHK(hipGetLastError());
+    ctx->done |= ST_MOMENTUM;
     return 0;
 }
 
 int mcm_gpu_boundcond(mcm_gpu_ctx* ctx) {
     if (!ctx) return MCM_GPU_ERR_ARG;
+    int rc = stage_check(ctx, "mcm_gpu_boundcond",
+                         (ctx->done & ST_MOMENTUM) && !(ctx->done & (ST_BOUNDCOND | ST_VELOCITY)),
+                         "the boundary conditions apply once, to the momentum stage's acceleration");
+    if (rc) return rc;
     k_boundcond<<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(ctx->d);
     HIPCHK(hipGetLastError());
+    ctx->done |= ST_BOUNDCOND;
     return 0;
 }
 
 int mcm_gpu_calctotale(mcm_gpu_ctx* ctx) {
     if (!ctx) return MCM_GPU_ERR_ARG;
+    // a reduction of the current state: any point of the step
+    int rc = stage_check(ctx, "mcm_gpu_calctotale", true, "");
+    if (rc) return rc;
     k_calctotale<<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(ctx->d);
     launch_c_fold(one(ctx), 0);
     HIPCHK(hipGetLastError());
@@ -1411,40 +1643,59 @@ int mcm_gpu_calctotale(mcm_gpu_ctx* ctx) {
 
 int mcm_gpu_update_velocity(mcm_gpu_ctx* ctx) {
     if (!ctx) return MCM_GPU_ERR_ARG;
+    int rc = stage_check(ctx, "mcm_gpu_update_velocity", (ctx->done & ST_MOMENTUM) && !(ctx->done & ST_VELOCITY),
+                         "the velocity update runs once, after the momentum stage");
+    if (rc) return rc;
     k_update_velocity<<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(ctx->d);
     if (ctx->d.veloc_opt == 1) {   // XSPH, mcm_update_velocity.f90:61-87
         k_xsph<<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(ctx->d, ctx->cur);
         k_xsph_apply<<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(ctx->d);
     }
     HIPCHK(hipGetLastError());
+    ctx->done |= ST_VELOCITY;
     return 0;
 }
 
 int mcm_gpu_move(mcm_gpu_ctx* ctx) {
-    if (!ctx || ctx->phase != 1) return MCM_GPU_ERR_ARG;
+    if (!ctx) return MCM_GPU_ERR_ARG;
+    int rc = stage_check(ctx, "mcm_gpu_move", ctx->phase == 1 && (ctx->done & ST_VELOCITY),
+                         "the move ends a step, after the velocity update");
+    if (rc) return rc;
     k_move<<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(ctx->d);
     k_move_fold<<<1, MCM_BLOCK, 0, ctx->stream>>>(ctx->d, ctx->nb);
     HIPCHK(hipGetLastError());
     ctx->cur ^= 1;
     ctx->phase = 0;
+    ctx->done = 0;
+    ctx->fresh = false;   // positions moved: the next strain pass needs a new search
     return 0;
 }
 
 int mcm_gpu_calccritts(mcm_gpu_ctx* ctx, double* critts_out) {
     if (!ctx) return MCM_GPU_ERR_ARG;
+    int rc = stage_check(ctx, "mcm_gpu_calccritts", true, "");
+    if (rc) return rc;
     int hb = ctx->phase == 1 ? ctx->cur ^ 1 : ctx->cur;
     k_calccritts<<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(ctx->d, hb);
     k_calccritts_fold<<<1, MCM_BLOCK, 0, ctx->stream>>>(ctx->d, ctx->nb);
     HIPCHK(hipGetLastError());
     Scalars s;
-    int rc = get_scalars_raw(ctx, &s);
+    rc = get_scalars_raw(ctx, &s);
     if (rc) return rc;
     if (critts_out) *critts_out = s.critts;
     return 0;
 }
 
 int mcm_gpu_step(mcm_gpu_ctx* ctx, int32_t nsteps) {
-    if (!ctx || ctx->phase != 0 || nsteps < 0) return MCM_GPU_ERR_ARG;
+    if (!ctx || nsteps < 0) return MCM_GPU_ERR_ARG;
+    if (ctx->phase != 0 || ctx->done != 0) {
+        ctx->err = "mcm_gpu_step: the run is inside a step begun with the stage entry points";
+        return MCM_GPU_ERR_ARG;
+    }
+    // the fused steps search again themselves; a run of a batch may grow its
+    // capacities here, so the batch takes its states afresh
+    ctx->fresh = false;
+    if (ctx->batch) ctx->batch->stale = true;
     Scalars s0;
     int rc = get_scalars_raw(ctx, &s0);
     if (rc) return rc;
@@ -1555,20 +1806,35 @@ int mcm_gpu_download(mcm_gpu_ctx* ctx, int32_t field, double* host, int64_t n) {
 
 int mcm_gpu_download_neighbours(mcm_gpu_ctx* ctx, int32_t* counts, int32_t* gcounts, int32_t* ids, int64_t ids_len,
                                 int32_t* cap) {
-    if (!ctx || !counts || !gcounts || !ids || !cap) return MCM_GPU_ERR_ARG;
+    if (!ctx || !counts || !gcounts || !cap) return MCM_GPU_ERR_ARG;
     DevState& d = ctx->d;
     const size_t np = d.np;
-    std::vector<int> nr(np), nt(np), lst((size_t)d.ncap * d.npad);
+    std::vector<int> nr(np), nt(np), lst(ids ? (size_t)d.ncap * d.npad : 0);
     HIPCHK(hipMemcpyAsync(nr.data(), d.nreal, np * 4, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipMemcpyAsync(nt.data(), d.ntot, np * 4, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipMemcpyAsync(lst.data(), d.nbr, lst.size() * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (ids) HIPCHK(hipMemcpyAsync(lst.data(), d.nbr, lst.size() * 4, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     *cap = d.ncap;
-    if (ids_len < (int64_t)np * d.ncap) return MCM_GPU_ERR_ARG;
+    if (ids && ids_len < (int64_t)np * d.ncap) return MCM_GPU_ERR_ARG;
     for (size_t i = 0; i < np; i++) {
         counts[i] = nr[i];
         gcounts[i] = nt[i] - nr[i];
-        for (int k = 0; k < nt[i] && k < d.ncap; k++) ids[i * d.ncap + k] = lst[((i >> 6) * d.ncap + k) * 64 + (i & 63)];
+        if (ids)
+            for (int k = 0; k < nt[i] && k < d.ncap; k++)
+                ids[i * d.ncap + k] = lst[((i >> 6) * d.ncap + k) * 64 + (i & 63)];
+    }
+    return 0;
+}
+
+int mcm_gpu_get_energies(mcm_gpu_ctx* ctx, double* out, int32_t n) {
+    if (!ctx || !out || n < 5 * ctx->d.nummat) return MCM_GPU_ERR_ARG;
+    Scalars s;
+    int rc = get_scalars_raw(ctx, &s);
+    if (rc) return rc;
+    for (int m = 0; m < ctx->d.nummat; m++) {
+        out[5 * m] = s.mat_ke[m];
+        out[5 * m + 1] = s.mat_ie[m];
+        for (int k = 0; k < 3; k++) out[5 * m + 2 + k] = s.mat_mom[m][k];
     }
     return 0;
 }
@@ -1581,34 +1847,14 @@ int mcm_gpu_dump(mcm_gpu_ctx* ctx, const char* path, int32_t step, int32_t stage
     Scalars s;
     int rc = get_scalars_raw(ctx, &s);
     if (rc) return rc;
-    const int hb = ctx->phase == 1 ? ctx->cur ^ 1 : ctx->cur;
-    auto get = [&](const double* src, size_t n) {
-        std::vector<double> v(n);
-        (void)hipMemcpy(v.data(), src, n * sizeof(double), hipMemcpyDeviceToHost);
-        return v;
-    };
     auto geti = [&](const int* src, size_t n) {
         std::vector<int> v(n);
         (void)hipMemcpy(v.data(), src, n * sizeof(int), hipMemcpyDeviceToHost);
         return v;
     };
     const size_t ng = s.ngp;
-    std::vector<double> X[3], V[3], Acc[3], SG[6], SD[6], AL[6], ROD[6], SP[3];
-    for (int k = 0; k < 3; k++) {
-        X[k] = get(ctx->phase == 1 ? d.x1[k] : d.x0[k], np + ng);
-        V[k] = get(d.v[k], np + ng);
-        Acc[k] = get(d.a[k], np);
-        SP[k] = get(d.spin[k], np);
-    }
-    for (int k = 0; k < 6; k++) {
-        SG[k] = get(d.sig[k], np); SD[k] = get(d.sd[k], np); AL[k] = get(d.alfa[k], np); ROD[k] = get(d.rod[k], np);
-    }
-    auto rho = get(d.rbuf[hb], np + ng), h = get(d.hbuf[hb], np + ng), mass = get(d.mass, np + ng);
-    auto rhoold = get(d.rhoold, np), hold = get(d.hold, np), p = get(d.p, np), e = get(d.e, np);
-    auto etry = get(d.etry, np), einc = get(d.einc, np), c = get(d.c, np), vabs = get(d.vabs, np);
-    auto tr = get(d.tracerod, np), pcut = get(d.pcut, np), p_cut = get(d.p_cut, np), efps = get(d.efps, np);
-    auto fail = get(d.fail, np), mind = get(d.mindist, np), crit = get(d.critts_p, np), rho0 = get(d.rho0, np);
-    auto h0 = get(d.h0, np), q = get(d.q, np);
+    std::vector<double> rows, ghost;
+    if ((rc = gather_rows(ctx, s, rows, &ghost))) return rc;
     auto nreal = geti(d.nreal, np), ntot = geti(d.ntot, np), cell_of = geti(d.cell_of, np);
     auto lst = geti(d.nbr, (size_t)d.ncap * d.npad);
     auto gpar = geti(d.gpar, ng), pmat = geti(d.pmat + np, ng);
@@ -1627,28 +1873,7 @@ int mcm_gpu_dump(mcm_gpu_ctx* ctx, const char* path, int32_t step, int32_t stage
         double mm[5] = {s.mat_ke[m], s.mat_ie[m], s.mat_mom[m][0], s.mat_mom[m][1], s.mat_mom[m][2]};
         fwrite(mm, 8, 5, f);
     }
-    static const int map[9] = {S11, S12, S13, S12, S22, S23, S13, S23, S33};
-    double row[86];
-    for (size_t i = 0; i < np; i++) {
-        for (int k = 0; k < 3; k++) { row[k] = X[k][i]; row[3 + k] = V[k][i]; row[6 + k] = Acc[k][i]; }
-        row[9] = rho[i]; row[10] = rhoold[i]; row[11] = h[i]; row[12] = hold[i]; row[13] = p[i]; row[14] = e[i];
-        row[15] = etry[i]; row[16] = einc[i]; row[17] = c[i]; row[18] = vabs[i]; row[19] = tr[i]; row[20] = pcut[i];
-        row[21] = p_cut[i]; row[22] = efps[i]; row[23] = fail[i]; row[24] = mind[i]; row[25] = crit[i];
-        row[26] = mass[i]; row[27] = rho0[i];
-        for (int k = 0; k < 9; k++) {
-            int m = map[k];
-            row[28 + k] = SG[m][i]; row[37 + k] = SD[m][i]; row[46 + k] = AL[m][i];
-            row[55 + k] = (k == 0 || k == 4 || k == 8) ? q[i] : 0.0;
-            row[64 + k] = ROD[m][i];
-        }
-        // spin: column-major (r,c) -> W12 at (0,1)=k3, W13 at (0,2)=k6, W23 at (1,2)=k7
-        double w12 = SP[0][i], w13 = SP[1][i], w23 = SP[2][i];
-        double sp[9] = {0.0, -w12, -w13, w12, 0.0, -w23, w13, w23, 0.0};
-        for (int k = 0; k < 9; k++) row[73 + k] = sp[k];
-        row[82] = h0[i];
-        for (int k = 0; k < 3; k++) row[83 + k] = ctx->xzero[3 * i + k];
-        fwrite(row, 8, 86, f);
-    }
+    fwrite(rows.data(), 8, rows.size(), f);
     std::vector<int32_t> nn(np), gn(np), llp(np);
     for (size_t i = 0; i < np; i++) { nn[i] = nreal[i]; gn[i] = ntot[i] - nreal[i]; llp[i] = cell_of[i] >= 0 ? 0 : -1; }
     fwrite(nn.data(), 4, np, f);
@@ -1667,12 +1892,8 @@ int mcm_gpu_dump(mcm_gpu_ctx* ctx, const char* path, int32_t step, int32_t stage
             }
         for (size_t g = 0; g < ng; g++) { int32_t v = gpar[g] + 1; fwrite(&v, 4, 1, f); }
         for (size_t g = 0; g < ng; g++) { int32_t v = pmat[g] + 1; fwrite(&v, 4, 1, f); }
-        for (size_t g = 0; g < ng; g++) {
-            size_t u = np + g;
-            double r8[8] = {X[0][u], X[1][u], X[2][u], V[0][u], V[1][u], V[2][u], mass[u], h[u]};
-            fwrite(r8, 8, 8, f);
-        }
-        for (size_t g = 0; g < ng; g++) fwrite(&rho[np + g], 8, 1, f);
+        fwrite(ghost.data(), 8, 8 * ng, f);
+        for (size_t g = 0; g < ng; g++) fwrite(&ghost[8 * ng + g], 8, 1, f);
         double z = 0.0;
         for (size_t g = 0; g < ng; g++) fwrite(&z, 8, 1, f);
     }
@@ -1680,8 +1901,45 @@ int mcm_gpu_dump(mcm_gpu_ctx* ctx, const char* path, int32_t step, int32_t stage
     return 0;
 }
 
+int mcm_gpu_download_state(mcm_gpu_ctx* ctx, double* rows, int64_t np, mcm_gpu_scalars* sc) {
+    if (!ctx || (!rows && !sc)) return MCM_GPU_ERR_ARG;
+    if (rows && np != ctx->d.np) {
+        ctx->err = "mcm_gpu_download_state: np must be the run's particle count";
+        return MCM_GPU_ERR_ARG;
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    Scalars s;
+    int rc = get_scalars_raw(ctx, &s);
+    if (rc) return rc;
+    if (rows) {
+        std::vector<double> r;
+        if ((rc = gather_rows(ctx, s, r, nullptr))) return rc;
+        memcpy(rows, r.data(), r.size() * sizeof(double));
+    }
+    if (sc) rc = mcm_gpu_get_scalars(ctx, sc);
+    return rc;
+}
+
+int mcm_gpu_upload_state(mcm_gpu_ctx* ctx, const double* rows, int64_t np, const mcm_gpu_scalars* sc) {
+    if (!ctx || !rows || !sc) return MCM_GPU_ERR_ARG;
+    if (np != ctx->d.np) {
+        ctx->err = "mcm_gpu_upload_state: np must be the run's particle count";
+        return MCM_GPU_ERR_ARG;
+    }
+    int rc = stage_check(ctx, "mcm_gpu_upload_state", ctx->phase == 0 && ctx->done == 0,
+                         "the state is replaced at a step boundary");
+    if (rc) return rc;
+    if (!(sc->hmax > 0.0) || !(sc->dt >= 0.0) || !(sc->dtold >= 0.0)) {
+        ctx->err = "mcm_gpu_upload_state: hmax must be positive and dt, dtold not negative";
+        return MCM_GPU_ERR_ARG;
+    }
+    return scatter_rows(ctx, rows, *sc);
+}
+
 int mcm_gpu_time_steps(mcm_gpu_ctx* ctx, int32_t nsteps, double* ms) {
-    if (!ctx || !ms || nsteps <= 0 || ctx->phase != 0) return MCM_GPU_ERR_ARG;
+    if (!ctx || !ms || nsteps <= 0 || ctx->phase != 0 || ctx->done != 0) return MCM_GPU_ERR_ARG;
+    if (ctx->batch) return stage_check(ctx, "mcm_gpu_time_steps", false, "");
+    ctx->fresh = false;
     int rc = time_stages(one(ctx), ctx->cur, nsteps, ms, ctx->err);
     ctx->cur ^= nsteps & 1;
     return rc ? rc : check_sticky(ctx);

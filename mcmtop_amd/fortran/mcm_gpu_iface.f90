@@ -72,6 +72,42 @@ module mcm_gpu_iface
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
     end function
+    integer(c_int) function mcm_gpu_set_time(ctx, dt, dtold, ptime) bind(c, name='mcm_gpu_set_time')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: dt, dtold, ptime
+    end function
+    integer(c_int) function mcm_gpu_get_energies(ctx, en, n) bind(c, name='mcm_gpu_get_energies')
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      real(c_double), dimension(*), intent(out) :: en
+      integer(c_int32_t), value :: n
+    end function
+    ! per-particle state rows (MCM_STATE_WORDS = 86 per particle) and scalars
+    integer(c_int) function mcm_gpu_upload_state(ctx, rows, np, s) bind(c, name='mcm_gpu_upload_state')
+      import :: c_int, c_ptr, c_double, c_int64_t, mcm_gpu_scalars
+      type(c_ptr), value :: ctx
+      real(c_double), dimension(*), intent(in) :: rows
+      integer(c_int64_t), value :: np
+      type(mcm_gpu_scalars), intent(in) :: s
+    end function
+    integer(c_int) function mcm_gpu_download_state(ctx, rows, np, s) bind(c, name='mcm_gpu_download_state')
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: ctx
+      real(c_double), dimension(*), intent(out) :: rows
+      integer(c_int64_t), value :: np
+      type(c_ptr), value :: s
+    end function
+    ! mcm_gpu_download_neighbours with ids = NULL: the list lengths alone
+    integer(c_int) function mcm_gpu_download_counts(ctx, counts, gcounts, ids, ids_len, cap) &
+        bind(c, name='mcm_gpu_download_neighbours')
+      import :: c_int, c_ptr, c_int32_t, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), dimension(*), intent(out) :: counts, gcounts
+      type(c_ptr), value :: ids
+      integer(c_int64_t), value :: ids_len
+      integer(c_int32_t), intent(out) :: cap
+    end function
     integer(c_int) function mcm_gpu_rep_cont(ctx) bind(c, name='mcm_gpu_rep_cont')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

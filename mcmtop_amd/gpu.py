@@ -37,7 +37,10 @@ SYMBOLS = (
     "mcm_gpu_write_vtk", "mcm_gpu_rep_cont", "mcm_gpu_batch_tune",
     "mcm_gpu_set_material_params", "mcm_gpu_get_material_params", "mcm_gpu_set_initial_velocity",
     "mcm_gpu_adj_forward", "mcm_gpu_adj_backward", "mcm_gpu_adj_last_error", "mcm_gpu_adj_destroy",
+    "mcm_gpu_set_time", "mcm_gpu_download_state", "mcm_gpu_upload_state", "mcm_gpu_get_energies",
 )
+
+STATE_WORDS = 86   # row of the per-particle state (include/mcm_gpu.h, the dump row layout)
 
 N_PARAMS = 4   # per material: model 10 strinput(1:3) (G, sigma_y, Eh) and eosinput(2) (eos4 S1); model 1 E, nu
 
@@ -125,6 +128,10 @@ def load_library(path: str = LIB_PATH):
     lib.mcm_gpu_adj_last_error.restype = ctypes.c_char_p
     lib.mcm_gpu_adj_destroy.argtypes = [vp]
     lib.mcm_gpu_adj_destroy.restype = None
+    lib.mcm_gpu_set_time.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+    lib.mcm_gpu_download_state.argtypes = [vp, dp, ctypes.c_int64, ctypes.POINTER(Scalars)]
+    lib.mcm_gpu_upload_state.argtypes = [vp, dp, ctypes.c_int64, ctypes.POINTER(Scalars)]
+    lib.mcm_gpu_get_energies.argtypes = [vp, dp, ctypes.c_int32]
     special = ("mcm_gpu_destroy", "mcm_gpu_last_error", "mcm_gpu_batch_size", "mcm_gpu_batch_run",
                "mcm_gpu_batch_last_error", "mcm_gpu_batch_destroy", "mcm_gpu_adj_last_error",
                "mcm_gpu_adj_destroy")
@@ -252,6 +259,24 @@ class Solver:
         for name in ("neighbours", "strain", "update_h", "constitutive", "dt_update", "rep_cont", "momentum",
                      "boundcond", "calctotale", "update_velocity", "move"):
             self._stage(name)
+
+    def set_time(self, dt: float, dtold: float, ptime: float):
+        """The host's dt, dtold and ptime (mcm_solution.f90:58-62 on the host)."""
+        self._check(self._lib.mcm_gpu_set_time(self._h, float(dt), float(dtold), float(ptime)), "set_time")
+
+    def download_state(self):
+        """(rows[np, STATE_WORDS], Scalars): the state as the host's particle records hold it."""
+        n = self.scalars().np
+        rows = np.zeros((n, STATE_WORDS))
+        s = Scalars()
+        self._check(self._lib.mcm_gpu_download_state(self._h, rows.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                     n, ctypes.byref(s)), "download_state")
+        return rows, s
+
+    def upload_state(self, rows, sc: Scalars):
+        r = np.ascontiguousarray(rows, dtype=np.float64)
+        self._check(self._lib.mcm_gpu_upload_state(self._h, r.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                   r.shape[0], ctypes.byref(sc)), "upload_state")
 
     def step(self, n: int = 1):
         """n fused steps (device-side dt, captured hipGraph)."""
@@ -458,6 +483,24 @@ class Batch:
 
     def initial(self):
         self._check(self._lib.mcm_gpu_batch_initial(self._h), "batch_initial")
+
+    def set_time(self, dt: float, dtold: float, ptime: float):
+        """The host's dt, dtold and ptime (mcm_solution.f90:58-62 on the host)."""
+        self._check(self._lib.mcm_gpu_set_time(self._h, float(dt), float(dtold), float(ptime)), "set_time")
+
+    def download_state(self):
+        """(rows[np, STATE_WORDS], Scalars): the state as the host's particle records hold it."""
+        n = self.scalars().np
+        rows = np.zeros((n, STATE_WORDS))
+        s = Scalars()
+        self._check(self._lib.mcm_gpu_download_state(self._h, rows.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                     n, ctypes.byref(s)), "download_state")
+        return rows, s
+
+    def upload_state(self, rows, sc: Scalars):
+        r = np.ascontiguousarray(rows, dtype=np.float64)
+        self._check(self._lib.mcm_gpu_upload_state(self._h, r.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                   r.shape[0], ctypes.byref(sc)), "upload_state")
 
     def step(self, n: int = 1):
         self._check(self._lib.mcm_gpu_batch_step(self._h, int(n)), "batch_step")

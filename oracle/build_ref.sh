@@ -59,6 +59,51 @@ for f in $FILES; do
   OBJS="$OBJS $obj"
 done
 "$FC" -shared -fPIC -Wl,-z,lazy $OBJS -o "$OUT/libmcmref.so"
-"$FC" -O2 -I "$OUT/mod" -module-dir "$OUT/mod" "$HERE/ref_probe.f90" -o "$OUT/ref_probe" \
+"$FC" -O2 -fPIC -I "$OUT/mod" -module-dir "$OUT/mod" -c "$HERE/ref_deck.f90" -o "$OUT/obj/ref_deck.o"
+"$FC" -O2 -I "$OUT/mod" -module-dir "$OUT/mod" "$HERE/ref_probe.f90" "$OUT/obj/ref_deck.o" -o "$OUT/ref_probe" \
   -L"$OUT" -lmcmref -Wl,--allow-shlib-undefined -Wl,-z,lazy -Wl,-rpath,'$ORIGIN'
 echo "build_ref: built $OUT/ref_probe"
+
+# The reference's own time loop and initialisation with its writers
+# (libmcmloop.so), driven by dropin_main.f90 twice:
+#   ref_solution: with every step routine of the reference (libmcmref.so);
+#   mcm_dropin:   with the GPU wrappers of mcmtop_amd/fortran/mcm_gpu_dropin.f90
+#                 in place of the step routines they replace (libmcmrest.so
+#                 holds the reference's objects minus those), on
+#                 mcmtop_amd/libmcm_gpu.so.
+LOOP="mcm_solution mcm_initial mcm_init_state_output mcm_state_output mcm_inittimehist mcm_timehist
+mcm_geometry mcm_write_ensight_3d mcm_write_res mcm_write_scalar mcm_write_vector mcm_write_tensor"
+REPLACED=" mcm_setuplist mcm_init_boundary mcm_ghost_setup mcm_ll_neighbours mcm_strain mcm_update_h
+mcm_constitutive mcm_rhoupd mcm_calccritts mcm_rep_contact mcm_momentum mcm_boundcond mcm_calctotale
+mcm_update_velocity mcm_move "
+LOBJS=""
+for f in $LOOP; do
+  src="$REF/$f.f90"
+  obj="$OUT/obj/$f.o"
+  if [ ! -f "$obj" ] || [ "$src" -nt "$obj" ]; then
+    "$FC" -O2 -fPIC -I "$OUT/mod" -module-dir "$OUT/mod" -c "$src" -o "$obj"
+  fi
+  LOBJS="$LOBJS $obj"
+done
+"$FC" -shared -fPIC -Wl,-z,lazy $LOBJS -o "$OUT/libmcmloop.so"
+REPLACED=" $(echo $REPLACED) "
+ROBJS=""
+for f in $FILES; do
+  case "$REPLACED" in *" $f "*) continue ;; esac
+  ROBJS="$ROBJS $OUT/obj/$f.o"
+done
+"$FC" -shared -fPIC -Wl,-z,lazy $ROBJS -o "$OUT/libmcmrest.so"
+"$FC" -O2 -I "$OUT/mod" -module-dir "$OUT/mod" "$HERE/dropin_main.f90" "$OUT/obj/ref_deck.o" -o "$OUT/ref_solution" \
+  -L"$OUT" -lmcmloop -lmcmref -Wl,--allow-shlib-undefined -Wl,-z,lazy -Wl,-rpath,'$ORIGIN'
+echo "build_ref: built $OUT/ref_solution"
+GPULIB="$HERE/../mcmtop_amd"
+FDIR="$HERE/../mcmtop_amd/fortran"
+if [ -f "$GPULIB/libmcm_gpu.so" ]; then
+  "$FC" -O2 -fPIC -I "$OUT/mod" -module-dir "$OUT/mod" -c "$FDIR/mcm_gpu_iface.f90" -o "$OUT/obj/mcm_gpu_iface.o"
+  "$FC" -O2 -fPIC -I "$OUT/mod" -module-dir "$OUT/mod" -c "$FDIR/mcm_gpu_dropin.f90" -o "$OUT/obj/mcm_gpu_dropin.o"
+  "$FC" -O2 -I "$OUT/mod" -module-dir "$OUT/mod" -rdynamic "$HERE/dropin_main.f90" "$OUT/obj/ref_deck.o" \
+    "$OUT/obj/mcm_gpu_iface.o" "$OUT/obj/mcm_gpu_dropin.o" -o "$OUT/mcm_dropin" \
+    -L"$OUT" -lmcmloop -lmcmrest -L"$GPULIB" -lmcm_gpu -Wl,--allow-shlib-undefined -Wl,-z,lazy \
+    -Wl,-rpath,'$ORIGIN' -Wl,-rpath,'$ORIGIN/../../mcmtop_amd'
+  echo "build_ref: built $OUT/mcm_dropin"
+fi

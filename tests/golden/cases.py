@@ -156,6 +156,52 @@ def build(name: str) -> deck.Deck:
         rho = 2700.0
         m = deck.material(1, rho, rho * dx ** 3 * len(pts), h, deck.f3dm1(70e9, 0.33), rho_max=2703.0)
         return deck.make_deck(pts, (0.0, 0.0, -400.0), [m], jitter_dx=0.01 * dx, sym_flag=(0, 0, 0, 0, 1, 0))
+    if name == "f3dm3_efps_failure":
+        # f3dm3 with its efps failure test (mcm_f3dm3.f90:137-150): two halves
+        # of a copper block driven into each other; particles whose efps passes
+        # efps_fail keep only the mean stress
+        pts = deck.block(dx, 3, 3, 6)
+        rho = 8930.0
+        m = deck.material(3, rho, rho * dx ** 3 * len(pts), h,
+                          deck.f3dm3(117e9, 0.35, 400e6, 100e6, beta=0.3, fail_flag=1, efps_fail=0.05))
+        v = np.zeros((len(pts), 3))
+        v[:, 2] = np.where(pts[:, 2] > 3 * dx, -400.0, 400.0)
+        return deck.make_deck(pts, v, [m], jitter_dx=0.01 * dx)
+    if name == "fluid_f3dm9_mu0_eos1":
+        # inviscid fluid: with mu = 0 the deviatoric stress is zero
+        # (mcm_f3dm9.f90:30-33); polynomial EOS, a block squeezed in z and
+        # stretched in x
+        pts = deck.block(dx, 4, 4, 4)
+        rho = 1000.0
+        m = deck.material(9, rho, rho * dx ** 3 * len(pts), h, deck.f3dm9(-1.0e5, 0.0), eos=1,
+                          eoscard={1: 2.2e9, 8: 1.0}, av_q=1.5, av_l=0.06)
+        c = pts - pts.mean(axis=0)
+        v = np.stack([2.0e4 * c[:, 0], np.zeros(len(pts)), -4.0e4 * c[:, 2]], axis=1)
+        return deck.make_deck(pts, v, [m], jitter_dx=0.01 * dx)
+    if name == "eos4_negative_c2":
+        # hot Gruneisen material with a negative volume coefficient a: the
+        # sound speed squared of mcm_eos4c.f90:72-89 is negative and the
+        # reference goes on with sqrt(|c^2|)
+        pts = deck.block(dx, 4, 4, 4)
+        rho = 1000.0
+        m = deck.material(10, rho, rho * dx ** 3 * len(pts), h, deck.f3dm10(1.0, 1.0e30, 0.0), eos=4,
+                          eoscard={0: 1000.0, 1: 1.0, 4: 0.0, 5: -1.0, 6: 1.0e10, 7: 1.0})
+        v = (pts - pts.mean(axis=0)) * 2.0e3
+        return deck.make_deck(pts, v, [m], jitter_dx=0.01 * dx)
+    if name == "corr_collide_f3dm10_eos4_hopt1":
+        # corrected SPH with h_opt 1 and no planes: two f3dm10 + eqos4 copper
+        # blocks closing on each other.  Without ghosts the reference's own
+        # time loop (mcm_solution.f90) runs it unchanged.
+        a = deck.block(dx, 4, 4, 3)
+        b = deck.block(dx, 4, 4, 3, origin=(0.3 * dx, 0.2 * dx, 3.2 * dx))
+        pts = np.concatenate([a, b])
+        rho = 8930.0
+        m = deck.material(10, rho, rho * dx ** 3 * len(pts), h, deck.f3dm10(43.3e9, 400e6, 100.1e6), eos=4,
+                          eoscard={0: 3940.0, 1: 1.49, 4: 2.0, 5: 0.47})
+        v = np.zeros((len(pts), 3))
+        v[: len(a), 2] = 150.0
+        v[len(a):, 2] = -150.0
+        return deck.make_deck(pts, v, [m], disctype=1, h_opt=1, jitter_dx=0.01 * dx)
     raise KeyError(name)
 
 
@@ -176,4 +222,8 @@ CASES = {
     "f3dm10_spall3_failure": (30, (2,)),
     "f3dm10_spall1_failure_pcut": (30, (2,)),
     "rho_max_clamp_f3dm1": (30, (2,)),
+    "f3dm3_efps_failure": (30, (2,)),
+    "fluid_f3dm9_mu0_eos1": (30, (2,)),
+    "eos4_negative_c2": (30, (2,)),
+    "corr_collide_f3dm10_eos4_hopt1": (30, (2,)),
 }

@@ -56,9 +56,11 @@ def main():
         dk.write_bin(deck_path)
         with tempfile.TemporaryDirectory() as tmp:
             steps = sorted(set(stage_steps) | {nsteps})
-            subprocess.run([probe, deck_path, tmp, str(nsteps), "1"] + [str(s) for s in steps], check=True,
-                           stdout=subprocess.DEVNULL)
-            rec = {"np": dk.np_, "nsteps": nsteps, "stage_steps": list(stage_steps), "dumps": {}}
+            run = subprocess.run([probe, deck_path, tmp, str(nsteps), "1"] + [str(s) for s in steps], check=True,
+                                 stdout=subprocess.PIPE, text=True)
+            rec = {"np": dk.np_, "nsteps": nsteps, "stage_steps": list(stage_steps), "dumps": {},
+                   # the reference's own warnings: branches it reports taking
+                   "warnings": {"complex_sound_speed": run.stdout.count("Complex speed of sound")}}
             rec["dumps"]["0_0"] = digest(os.path.join(tmp, "dump_000000_00.bin"))
             for s in steps:
                 stages = range(1, 11) if s in stage_steps else (10,)

@@ -64,3 +64,32 @@ def test_deck_generator_is_deterministic(name, tmp_path):
     p = tmp_path / "d.bin"
     cases.build(name).write_bin(str(p))
     assert p.read_bytes() == open(os.path.join(DATA, name + ".bin"), "rb").read()
+
+
+def test_decks_reach_their_branches():
+    """The decks added for branches no other deck runs do run them, on the
+    oracle state that the digests pin to the reference's."""
+    def final(name):
+        o = Oracle(os.path.join(DATA, name + ".bin"))
+        assert o.initial() == 0
+        assert o.run(DIGESTS[name]["nsteps"]) == 0
+        return o.state(DIGESTS[name]["nsteps"], 10)
+
+    # f3dm3 failure (mcm_f3dm3.f90:137-150): particles past efps_fail keep
+    # only the mean stress, the others keep a deviatoric part
+    d = final("f3dm3_efps_failure")
+    sig, efps = d.f("sigma").reshape(-1, 3, 3), d.f("efps")
+    failed = efps > 0.05
+    assert 0 < failed.sum() < len(efps)
+    off = np.abs(sig[:, [0, 0, 1], [1, 2, 2]]).max(axis=1)
+    spread = np.ptp(np.diagonal(sig, axis1=1, axis2=2), axis=1)
+    assert np.all(off[failed] == 0.0) and np.all(spread[failed] == 0.0)
+    assert np.all(spread[~failed] > 0.0)
+    # f3dm9 with mu = 0 (mcm_f3dm9.f90:30-33): no deviatoric stress anywhere
+    d = final("fluid_f3dm9_mu0_eos1")
+    assert np.all(d.f("s") == 0.0)
+    assert np.abs(d.f("rod")).max() > 0.0
+    # eos4 sound speed with c^2 <= 0 (mcm_eos4c.f90:76-89): the reference
+    # reports taking that branch
+    assert DIGESTS["eos4_negative_c2"]["warnings"]["complex_sound_speed"] > 0
+    assert all(DIGESTS[k]["warnings"]["complex_sound_speed"] == 0 for k in DIGESTS if k != "eos4_negative_c2")
