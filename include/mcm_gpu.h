@@ -210,7 +210,8 @@ int mcm_gpu_output_fields(mcm_gpu_ctx *ctx, int32_t precision, void *host, int64
 int mcm_vtk_write(const char *path, int64_t np, const double *fields, const int32_t *material_ids,
                   int32_t material);
 /* mcm_gpu_output_fields + mcm_vtk_write for a run (the state_opt 4 output,
-   mcm_state_output.f90:55-60). */
+   mcm_state_output.f90:55-60); a material id without particles writes a
+   file without points, as the reference does. */
 int mcm_gpu_write_vtk(mcm_gpu_ctx *ctx, const char *path, int32_t material);
 
 /* Batched runs.  Many independent runs (each its own deck: particle count,
@@ -292,6 +293,8 @@ int mcm_gpu_adj_forward(mcm_gpu_ctx *ctx, int32_t nsteps, int32_t ckpt_every, co
 int mcm_gpu_adj_backward(mcm_gpu_adjoint *a, const double *xbar, const double *vbar, const double *histbar,
                          double *v0bar, double *pbar);
 const char *mcm_gpu_adj_last_error(const mcm_gpu_adjoint *a);
+/* Release a recording.  mcm_gpu_destroy releases the recordings of its run
+   as well, after which their handles are no longer valid. */
 void mcm_gpu_adj_destroy(mcm_gpu_adjoint *a);
 
 #ifdef __cplusplus

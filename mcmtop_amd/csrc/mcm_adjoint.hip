@@ -1093,6 +1093,7 @@ int adj_granular_step(mcm_gpu_adjoint* A) {
     launch_neighbours(one(ctx), ctx->cur);
     if (hipGetLastError() != hipSuccess) { A->err = "neighbour launch failed"; return MCM_GPU_ERR_HIP; }
     k_adj_check<<<1, 1, 0, ctx->stream>>>(ctx->d, A->a);
+    ctx->fresh = true;   // this step's strain records are laid down
     return adj_granular_rest(A);
 }
 
@@ -1338,6 +1339,7 @@ int mcm_gpu_adj_forward(mcm_gpu_ctx* ctx, int32_t nsteps, int32_t ckpt_every, co
     mcm_gpu_adjoint* A = new mcm_gpu_adjoint();
     *out = A;
     A->ctx = ctx;
+    ctx->recs.push_back(A);
     A->nsteps = nsteps;
     A->k = ckpt_every;
     A->nseg = (nsteps + ckpt_every - 1) / ckpt_every;
@@ -1515,7 +1517,11 @@ const char* mcm_gpu_adj_last_error(const mcm_gpu_adjoint* A) { return A ? A->err
 
 void mcm_gpu_adj_destroy(mcm_gpu_adjoint* A) {
     if (!A) return;
-    if (A->ctx && A->ctx->stream) (void)hipStreamSynchronize(A->ctx->stream);
+    if (A->ctx) {
+        std::vector<mcm_gpu_adjoint*>& r = A->ctx->recs;
+        r.erase(std::remove(r.begin(), r.end(), A), r.end());
+        if (A->ctx->stream) (void)hipStreamSynchronize(A->ctx->stream);
+    }
     for (void* p : A->allocs) (void)hipFree(p);
     for (void* p : A->gallocs) (void)hipFree(p);
     for (void* p : A->iallocs) (void)hipFree(p);

@@ -76,6 +76,7 @@ struct mcm_gpu_ctx {
     void* out_buf = nullptr;           // staging buffer of the packed output fields (first use)
     bool own_stream = true;            // false for a run of a batch (the batch's stream)
     mcm_gpu_batch* batch = nullptr;    // the batch this run belongs to, if any
+    std::vector<mcm_gpu_adjoint*> recs; // recordings of this run (released with it)
 };
 
 namespace {
@@ -166,7 +167,11 @@ int finish_deck(mcm_gpu_ctx* ctx) {
         } else if (M.model == 1 || M.model == 3) {
             if (M.eos != 0) { ctx->err = "models 1/3 take no EOS (mcm_matin.f90:72-79)"; return MCM_GPU_ERR_MODEL; }
         } else {
-            ctx->err = "material model not supported on the GPU path (1, 3, 9, 10 are)";
+            // model 4 (f3dm25): the reference's yield test reads the local capa before
+            // any assignment (mcm_f3dm25.f90:205-206), so its result is undefined
+            ctx->err = M.model == 4 ? "material model 4 (f3dm25) has no defined result: the reference reads an "
+                                      "unset local in its yield test (mcm_f3dm25.f90:205-206)"
+                                    : "material model not supported on the GPU path (1, 3, 9, 10 are)";
             return MCM_GPU_ERR_MODEL;
         }
         D.mat_raw.push_back(M);
@@ -1306,6 +1311,8 @@ int setup_run(mcm_gpu_ctx* ctx, const mcm_gpu_options* opt) {
 }
 
 void destroy_run(mcm_gpu_ctx* ctx) {
+    // a recording holds device buffers and a pointer to its run: release it first
+    while (!ctx->recs.empty()) mcm_gpu_adj_destroy(ctx->recs.back());
     drop_graph(ctx);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (void* p : ctx->allocs) (void)hipFree(p);
@@ -1974,7 +1981,9 @@ int mcm_vtk_write(const char* path, int64_t np, const double* fields, const int3
 }
 
 int mcm_gpu_write_vtk(mcm_gpu_ctx* ctx, const char* path, int32_t material) {
-    if (!ctx || !path || material < 0 || material > ctx->d.nummat) return MCM_GPU_ERR_ARG;
+    // a material without particles gives a file without points, as the
+    // reference's WriteVTKmat(2) does for a one-material run
+    if (!ctx || !path || material < 0) return MCM_GPU_ERR_ARG;
     const size_t np = ctx->d.np;
     std::vector<double> f(np * OUT_WORDS);
     int rc = mcm_gpu_output_fields(ctx, 8, f.data(), (int64_t)(f.size() * sizeof(double)));

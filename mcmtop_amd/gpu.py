@@ -12,6 +12,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -213,6 +214,9 @@ class Solver:
         if spare:
             self._lib.mcm_gpu_adj_destroy(spare)
             self._adj_spare = None
+        # live recordings are released with the run (mcm_gpu_destroy)
+        for rec in list(getattr(self, "_recs", ())):
+            rec._h = None
         if getattr(self, "_h", None):
             if self._owned:
                 self._lib.mcm_gpu_destroy(self._h)
@@ -259,6 +263,14 @@ class Solver:
         for name in ("neighbours", "strain", "update_h", "constitutive", "dt_update", "rep_cont", "momentum",
                      "boundcond", "calctotale", "update_velocity", "move"):
             self._stage(name)
+
+    def energies(self) -> np.ndarray:
+        """(nummat, 5): ke, ie, mom(3) per material from the last calctotale or fused step."""
+        nm = self.scalars().nummat
+        out = np.zeros(5 * nm)
+        self._check(self._lib.mcm_gpu_get_energies(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 5 * nm),
+                    "get_energies")
+        return out.reshape(nm, 5)
 
     def set_time(self, dt: float, dtold: float, ptime: float):
         """The host's dt, dtold and ptime (mcm_solution.f90:58-62 on the host)."""
@@ -384,7 +396,11 @@ class Solver:
             if h:
                 self._lib.mcm_gpu_adj_destroy(h)
             raise McmGpuError("record failed: %s (%s)" % (ERRORS.get(rc, rc), msg))
-        return Recording(self, h, int(nsteps), len(smp), hist)
+        rec = Recording(self, h, int(nsteps), len(smp), hist)
+        if not hasattr(self, "_recs"):
+            self._recs = weakref.WeakSet()
+        self._recs.add(rec)
+        return rec
 
 
 class Recording:
@@ -483,6 +499,14 @@ class Batch:
 
     def initial(self):
         self._check(self._lib.mcm_gpu_batch_initial(self._h), "batch_initial")
+
+    def energies(self) -> np.ndarray:
+        """(nummat, 5): ke, ie, mom(3) per material from the last calctotale or fused step."""
+        nm = self.scalars().nummat
+        out = np.zeros(5 * nm)
+        self._check(self._lib.mcm_gpu_get_energies(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 5 * nm),
+                    "get_energies")
+        return out.reshape(nm, 5)
 
     def set_time(self, dt: float, dtold: float, ptime: float):
         """The host's dt, dtold and ptime (mcm_solution.f90:58-62 on the host)."""

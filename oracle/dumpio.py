@@ -1,7 +1,8 @@
 """Reader for the state dumps written by oracle/ref_probe.f90 and by the CPU
 oracle (same layout).  TEST INFRASTRUCTURE ONLY.
 
-Layout (little-endian): see the comment above ``probe_dump`` in ref_probe.f90.
+Layout (little-endian): see the comment above ``probe_dump`` in ref_deck.f90;
+``probe_dump_rows`` writes the same file up to the particle rows.
 Particle rows hold NF=86 doubles; the column names are ``FIELDS`` below.
 Neighbour ids are 1-based as in the reference (real ids into the particle
 array, ghost ids into the ghost array).
@@ -11,6 +12,7 @@ from __future__ import annotations
 import numpy as np
 
 DUMP_MAGIC = int.from_bytes(b"MCMD", "big")
+ROWS_MAGIC = int.from_bytes(b"MCMR", "big")   # state rows only (probe_dump_rows in ref_deck.f90)
 NF = 86
 
 FIELDS = {
@@ -30,7 +32,7 @@ class Dump:
         buf = open(path, "rb").read()
         off = 0
         hdr = np.frombuffer(buf, "<i4", 6, off); off += 24
-        if hdr[0] != DUMP_MAGIC:
+        if hdr[0] not in (DUMP_MAGIC, ROWS_MAGIC):
             raise ValueError("bad dump magic in %s" % path)
         self.step, self.stage, self.np, self.ngp, self.nummat = (int(v) for v in hdr[1:])
         sc = np.frombuffer(buf, "<f8", 8, off); off += 64
@@ -45,6 +47,9 @@ class Dump:
         self.mat_ke = m[:, 0].copy(); self.mat_ie = m[:, 1].copy(); self.mat_mom = m[:, 2:5].copy()
         n = self.np
         self.rows = np.frombuffer(buf, "<f8", n * NF, off).reshape(n, NF).copy(); off += 8 * n * NF
+        if hdr[0] == ROWS_MAGIC:
+            self.nnbr = self.g_nnbr = self.llpointer = None
+            return
         self.nnbr = np.frombuffer(buf, "<i4", n, off).copy(); off += 4 * n
         self.g_nnbr = np.frombuffer(buf, "<i4", n, off).copy(); off += 4 * n
         self.llpointer = np.frombuffer(buf, "<i4", n, off).copy(); off += 4 * n

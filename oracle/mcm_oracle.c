@@ -22,7 +22,11 @@
 #define V3(a, i, d) ((a)[3 * (long)(i) + (d)])
 #define CM(st, m, a, b) ((st)->mat[m].strinput[((a) - 1) + 5 * ((b) - 1)])
 
+/* The pair loops (list build, strain, acceleration, contact) run their
+   particles on OpenMP threads: each particle's sums stay in its own thread,
+   in list order, so the results do not depend on the thread count. */
 static void set_err(orc_state *st, int code) {
+#pragma omp critical(orc_error)
     if (st->error == 0) st->error = code;
 }
 
@@ -461,6 +465,7 @@ static void ghost_setup(orc_state *st) {
 /* mcm_ll_neighbours.f90:44-299 (lists rebuilt from scratch; capacity errors
    are reported, the reference's two-pass regrow is not reproduced) */
 static void ll_neighbours(orc_state *st, int ip_flag) {
+#pragma omp parallel for schedule(dynamic, 256) if (st->np >= 16384)
     for (int i = 0; i < st->np; i++) {
         st->nnbr[i] = 0; st->ncont[i] = 0; st->g_nnbr[i] = 0; st->g_ncont[i] = 0;
         st->mindist[i] = 1.0e20;
@@ -585,6 +590,7 @@ static void rod_spin_from(orc_state *st, int i, const double gv[3][3]) {
 
 /* mcm_oldstrain.f90:34-64 */
 static void oldstrain(orc_state *st) {
+#pragma omp parallel for schedule(dynamic, 256) if (st->np >= 16384)
     for (int i = 0; i < st->np; i++) {
         double gv[3][3] = {{0}};
         double vi[3] = {V3(st->v, i, 0), V3(st->v, i, 1), V3(st->v, i, 2)};
@@ -609,6 +615,7 @@ static void corr_grad(const double *dwdx, double w, double sum_w, const double *
 
 /* mcm_corr_strain.f90:37-146 (three neighbour passes, kernel centred at i) */
 static void corr_strain(orc_state *st) {
+#pragma omp parallel for schedule(dynamic, 256) if (st->np >= 16384)
     for (int i = 0; i < st->np; i++) {
         const double *xi = &V3(st->x, i, 0);
         int nk = st->nnbr[i] + st->g_nnbr[i];
@@ -1281,6 +1288,7 @@ static void nbr_moment_info(const orc_state *st, int i, int k, double *xj, doubl
 
 /* mcm_oldacceleration.f90:36-90 */
 static void oldacceleration(orc_state *st) {
+#pragma omp parallel for schedule(dynamic, 256) if (st->np >= 16384)
     for (int i = 0; i < st->np; i++) {
         for (int d = 0; d < 3; d++) V3(st->a, i, d) = 0.0;
         if (st->contacttype > 0)   /* :49-56 */
@@ -1306,6 +1314,7 @@ static void oldacceleration(orc_state *st) {
 /* mcm_corr_acceleration.f90:37-163: kernel centred at j, self term zeroed at
    k == nnbr, mass of the last neighbour visited, B^-1 contracted transposed */
 static void corr_acceleration(orc_state *st) {
+#pragma omp parallel for schedule(dynamic, 256) if (st->np >= 16384)
     for (int i = 0; i < st->np; i++) {
         for (int d = 0; d < 3; d++) V3(st->a, i, d) = 0.0;
         if (st->contacttype > 0)   /* :45-49 */
@@ -1368,6 +1377,7 @@ static void corr_acceleration(orc_state *st) {
    contact partners (other materials within 2h), B-spline normalisation at
    the partner's spacing, exponent n of particle i's material */
 int orc_rep_cont(orc_state *st) {
+#pragma omp parallel for schedule(dynamic, 256) if (st->np >= 16384)
     for (int i = 0; i < st->np; i++) {
         for (int d = 0; d < 3; d++) V3(st->rep, i, d) = 0.0;
         if (st->ncont[i] <= 0) continue;
@@ -1708,6 +1718,23 @@ int orc_run(orc_state *st, int nsteps) {
 
 /* ----------------------------------------------------------------- dump */
 /* Same layout as ref_probe.f90 probe_dump (1-based ids). */
+/* particle row of the dump layout (oracle/dumpio.py FIELDS) */
+static void state_row(const orc_state *st, int i, double *row) {
+    for (int d = 0; d < 3; d++) { row[d] = V3(st->x, i, d); row[3 + d] = V3(st->v, i, d); row[6 + d] = V3(st->a, i, d); }
+    row[9] = st->rho[i]; row[10] = st->rhoold[i]; row[11] = st->h[i]; row[12] = st->hold[i];
+    row[13] = st->p[i]; row[14] = st->e[i]; row[15] = st->etry[i]; row[16] = st->einc[i];
+    row[17] = st->c[i]; row[18] = st->vabs[i]; row[19] = st->tracerod[i]; row[20] = st->pcut[i];
+    row[21] = st->p_cut[i]; row[22] = st->efps[i]; row[23] = st->fail[i]; row[24] = st->mindist[i];
+    row[25] = st->critts_p[i]; row[26] = st->mass[i]; row[27] = st->rho0[i];
+    for (int k = 0; k < 9; k++) {
+        row[28 + k] = st->sigma[9 * (long)i + k]; row[37 + k] = st->s[9 * (long)i + k];
+        row[46 + k] = st->alfa[9 * (long)i + k]; row[55 + k] = st->q[9 * (long)i + k];
+        row[64 + k] = st->rod[9 * (long)i + k]; row[73 + k] = st->spin[9 * (long)i + k];
+    }
+    row[82] = st->h0[i];
+    for (int d = 0; d < 3; d++) row[83 + d] = V3(st->xzero, i, d);
+}
+
 int orc_dump(orc_state *st, const char *path, int step, int stage) {
     FILE *f = fopen(path, "wb");
     if (!f) return ORC_ERR_IO;
@@ -1726,19 +1753,7 @@ int orc_dump(orc_state *st, const char *path, int step, int stage) {
     }
     double row[86];
     for (int i = 0; i < st->np; i++) {
-        for (int d = 0; d < 3; d++) { row[d] = V3(st->x, i, d); row[3 + d] = V3(st->v, i, d); row[6 + d] = V3(st->a, i, d); }
-        row[9] = st->rho[i]; row[10] = st->rhoold[i]; row[11] = st->h[i]; row[12] = st->hold[i];
-        row[13] = st->p[i]; row[14] = st->e[i]; row[15] = st->etry[i]; row[16] = st->einc[i];
-        row[17] = st->c[i]; row[18] = st->vabs[i]; row[19] = st->tracerod[i]; row[20] = st->pcut[i];
-        row[21] = st->p_cut[i]; row[22] = st->efps[i]; row[23] = st->fail[i]; row[24] = st->mindist[i];
-        row[25] = st->critts_p[i]; row[26] = st->mass[i]; row[27] = st->rho0[i];
-        for (int k = 0; k < 9; k++) {
-            row[28 + k] = st->sigma[9 * (long)i + k]; row[37 + k] = st->s[9 * (long)i + k];
-            row[46 + k] = st->alfa[9 * (long)i + k]; row[55 + k] = st->q[9 * (long)i + k];
-            row[64 + k] = st->rod[9 * (long)i + k]; row[73 + k] = st->spin[9 * (long)i + k];
-        }
-        row[82] = st->h0[i];
-        for (int d = 0; d < 3; d++) row[83 + d] = V3(st->xzero, i, d);
+        state_row(st, i, row);
         fwrite(row, 8, 86, f);
     }
     fwrite(st->nnbr, 4, st->np, f);
@@ -1770,6 +1785,36 @@ int orc_dump(orc_state *st, const char *path, int step, int stage) {
     }
     fclose(f);
     return ORC_OK;
+}
+
+/* In-memory views for runs too large to dump: rows[np * 86] in the dump's
+   row layout; lists as ids[np * cap] row-major, 0-based, ghosts as
+   np + ghost id (real part first, then ghosts, in list order), with the
+   real (incl. self) and ghost counts; scalars dt, dtold, ptime, critts,
+   hmax, kinetice, internale, totale, ngp. */
+void orc_get_rows(const orc_state *st, double *rows) {
+#pragma omp parallel for schedule(static) if (st->np >= 16384)
+    for (int i = 0; i < st->np; i++) state_row(st, i, rows + 86 * (long)i);
+}
+
+int orc_get_lists(const orc_state *st, int32_t *counts, int32_t *gcounts, int32_t *ids, int64_t cap) {
+    for (int i = 0; i < st->np; i++)
+        if (st->nnbr[i] + st->g_nnbr[i] > cap) return ORC_ERR_CAPACITY;
+#pragma omp parallel for schedule(static) if (st->np >= 16384)
+    for (int i = 0; i < st->np; i++) {
+        counts[i] = st->nnbr[i];
+        gcounts[i] = st->g_nnbr[i];
+        int32_t *row = ids + (long)i * cap;
+        for (int k = 0; k < st->nnbr[i]; k++) row[k] = st->nbr[(long)i * st->ncap + k];
+        for (int k = 0; k < st->g_nnbr[i]; k++) row[st->nnbr[i] + k] = st->np + st->gnbr[(long)i * st->gcap + k];
+    }
+    return ORC_OK;
+}
+
+void orc_get_scalars(const orc_state *st, double *out) {
+    const double v[9] = {st->dt, st->dtold, st->ptime, st->critts, st->hmax, st->kinetice, st->internale,
+                         st->totale, (double)st->ngp};
+    for (int k = 0; k < 9; k++) out[k] = v[k];
 }
 
 void orc_get_x(const orc_state *st, double *out) {

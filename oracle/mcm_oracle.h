@@ -24,6 +24,8 @@
 #ifndef MCM_ORACLE_H
 #define MCM_ORACLE_H
 
+#include <stdint.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -111,6 +113,9 @@ int orc_dump(orc_state *st, const char *path, int step, int stage);
 /* positions (3 per particle) of the current state, for finite-difference
    checks of the GPU adjoint */
 void orc_get_x(const orc_state *st, double *out);
+void orc_get_rows(const orc_state *st, double *rows);
+int orc_get_lists(const orc_state *st, int32_t *counts, int32_t *gcounts, int32_t *ids, int64_t cap);
+void orc_get_scalars(const orc_state *st, double *out);
 
 /* closed-form pieces exposed for unit tests */
 double orc_kernel_w(const double *xi, const double *xj, double havg, double pi);

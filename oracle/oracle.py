@@ -46,6 +46,13 @@ def _lib():
     lib.orc_inverse3.argtypes = [dbl3, dbl3]
     lib.orc_get_x.argtypes = [ctypes.c_void_p, dbl3]
     lib.orc_get_x.restype = None
+    lib.orc_get_rows.argtypes = [ctypes.c_void_p, dbl3]
+    lib.orc_get_rows.restype = None
+    i32p = ctypes.POINTER(ctypes.c_int32)
+    lib.orc_get_lists.argtypes = [ctypes.c_void_p, i32p, i32p, i32p, ctypes.c_int64]
+    lib.orc_get_lists.restype = ctypes.c_int
+    lib.orc_get_scalars.argtypes = [ctypes.c_void_p, dbl3]
+    lib.orc_get_scalars.restype = None
     return lib
 
 
@@ -101,6 +108,33 @@ class Oracle:
         out = numpy.zeros((np_, 3))
         self._l.orc_get_x(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
         return out
+
+    def rows(self, np_: int):
+        """(np, 86) particle rows of the current state (dump row layout), in memory."""
+        import numpy
+        out = numpy.zeros((np_, 86))
+        self._l.orc_get_rows(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return out
+
+    def lists(self, np_: int, cap: int):
+        """(counts, gcounts, ids[np, cap]): the lists in order, 0-based, ghosts as np + ghost id."""
+        import numpy
+        c = numpy.zeros(np_, numpy.int32)
+        g = numpy.zeros(np_, numpy.int32)
+        ids = numpy.full((np_, cap), -1, numpy.int32)
+        i32p = ctypes.POINTER(ctypes.c_int32)
+        rc = self._l.orc_get_lists(self.h, c.ctypes.data_as(i32p), g.ctypes.data_as(i32p), ids.ctypes.data_as(i32p),
+                                   cap)
+        if rc != 0:
+            raise ValueError("oracle lists longer than %d" % cap)
+        return c, g, ids
+
+    def scalars(self) -> dict:
+        import numpy
+        v = numpy.zeros(9)
+        self._l.orc_get_scalars(self.h, v.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        keys = ("dt", "dtold", "ptime", "critts", "hmax", "kinetice", "internale", "totale", "ngp")
+        return dict(zip(keys, v.tolist()))
 
     def dump(self, path: str, step: int = 0, stage: int = 0) -> None:
         rc = self._l.orc_dump(self.h, path.encode(), step, stage)

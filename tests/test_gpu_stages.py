@@ -1,0 +1,143 @@
+"""The stage entry points as a host drives them: their order rules, the
+host-side dt (mcm_gpu_set_time) and the state transfer of the host's
+particle records (mcm_gpu_download_state / mcm_gpu_upload_state)."""
+import os
+
+import numpy as np
+import pytest
+
+import cases
+from dumpio import Dump
+from mcmtop_amd import gpu
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+DATA = os.path.join(os.path.dirname(__file__), "golden", "data")
+STATE = ("x", "v", "rho", "h", "p", "e", "sigma", "alfa", "efps", "q", "c", "vabs", "pcut", "fail")
+
+
+def _dump(s, tmp_path, tag, step=0, stage=0):
+    p = str(tmp_path / ("%s.bin" % tag))
+    s.dump(p, step, stage)
+    return Dump(p)
+
+
+def _refused(s, stage):
+    with pytest.raises(gpu.McmGpuError) as e:
+        s.stage(stage)
+    return str(e.value)
+
+
+def test_out_of_order_stage_calls_are_refused(tmp_path):
+    """A strain pass after a move without a new search, a second dt update,
+    stages that need the momentum stage before it, and a move before the
+    velocity update are refused, and change nothing: the run continued in
+    order still equals the oracle bit for bit."""
+    path = os.path.join(DATA, "plain_f3dm3.bin")
+    with gpu.Solver(path) as s:
+        s.initial()
+        s.step_granular()
+        assert "out of order" in _refused(s, "strain")          # positions moved, records stale
+        assert "out of order" in _refused(s, "update_h")
+        assert "out of order" in _refused(s, "momentum")
+        s.neighbours()
+        s.strain()
+        assert "out of order" in _refused(s, "strain")
+        assert "out of order" in _refused(s, "momentum")        # before the constitutive update
+        assert "out of order" in _refused(s, "update_velocity")
+        s.update_h()
+        assert "out of order" in _refused(s, "update_h")
+        s.constitutive()
+        assert "out of order" in _refused(s, "constitutive")
+        s.dt_update()
+        assert "out of order" in _refused(s, "dt_update")       # would advance ptime twice
+        assert "out of order" in _refused(s, "boundcond")
+        s.momentum()
+        assert "out of order" in _refused(s, "momentum")
+        assert "out of order" in _refused(s, "move")
+        s.boundcond()
+        s.calctotale()
+        s.update_velocity()
+        assert "out of order" in _refused(s, "boundcond")
+        assert "out of order" in _refused(s, "update_velocity")
+        s.move()
+        with pytest.raises(gpu.McmGpuError):
+            s.set_time(-1.0, 0.0, 0.0)
+        for _ in range(3):
+            s.step_granular()
+        got = _dump(s, tmp_path, "g", 5, 10)
+    o = Oracle(path)
+    o.initial()
+    o.run(5)
+    want = o.state(5, 10)
+    for k in STATE:
+        np.testing.assert_array_equal(got.f(k), want.f(k), err_msg=k)
+    assert got.ptime == want.ptime
+
+
+def test_runs_of_a_batch_refuse_stage_calls():
+    path = os.path.join(DATA, "plain_f3dm3.bin")
+    with gpu.Batch([path]) as b:
+        b.initial()
+        r = b.run(0)
+        for st in ("neighbours", "strain", "dt_update", "calctotale"):
+            assert "belongs to a batch" in _refused(r, st)
+        b.step(2)
+        assert r.scalars().timestep == 3
+
+
+@pytest.mark.parametrize("name", ["plain_f3dm3", "corr_f3dm10_eos4_hopt1", "penalty_contact_f3dm1"])
+def test_host_side_dt(name, tmp_path):
+    """A host that keeps mcm_solution.f90:58-62 (dt from the device's critts
+    after mcm_constitutive, handed back with mcm_gpu_set_time) steps like the
+    device-side dt update, bit for bit."""
+    path = os.path.join(DATA, name + ".bin")
+    tssf = cases.build(name).tssf
+    nsteps = 8
+    with gpu.Solver(path) as s:
+        s.initial()
+        for _ in range(nsteps):
+            for st in ("neighbours", "strain", "update_h", "constitutive"):
+                s.stage(st)
+            sc = s.scalars()
+            dtold = sc.dt
+            dt = min(tssf * sc.critts, 1.1 * dtold)
+            s.set_time(dt, dtold, sc.ptime + dt)
+            s.set_time(dt, dtold, sc.ptime + dt)     # absolute values: repeating changes nothing
+            for st in ("rep_cont", "momentum", "boundcond", "calctotale", "update_velocity", "move"):
+                s.stage(st)
+        got = _dump(s, tmp_path, "h", nsteps, 10)
+    o = Oracle(path)
+    o.initial()
+    o.run(nsteps)
+    want = o.state(nsteps, 10)
+    tol = 0.0 if name != "penalty_contact_f3dm1" else 1e-10   # penalty contact calls pow
+    for k in STATE:
+        d = np.linalg.norm(got.f(k) - want.f(k))
+        assert d <= tol * np.linalg.norm(want.f(k)), k
+    assert abs(got.ptime - want.ptime) <= tol * want.ptime
+
+
+@pytest.mark.parametrize("name", ["corr_f3dm10_eos4_hopt1", "two_materials_contact", "xsph_corr_f3dm10_eos4_sym"])
+def test_state_round_trip(name, tmp_path):
+    """The host's copy of the state (download), uploaded into a second run of
+    the same deck, steps on exactly as the first run does."""
+    path = os.path.join(DATA, name + ".bin")
+    with gpu.Solver(path) as a, gpu.Solver(path) as b:
+        a.initial()
+        a.step(5)
+        rows, sc = a.download_state()
+        assert rows.shape == (sc.np, gpu.STATE_WORDS)
+        b.upload_state(rows, sc)
+        with pytest.raises(gpu.McmGpuError):
+            b.strain()                      # the records of an uploaded state need a search first
+        a.step(10)
+        b.step(10)
+        da, db = _dump(a, tmp_path, "a", 15, 10), _dump(b, tmp_path, "b", 15, 10)
+        rows_b, _ = b.download_state()
+    np.testing.assert_array_equal(da.rows, db.rows)
+    np.testing.assert_array_equal(da.nnbr, db.nnbr)
+    np.testing.assert_array_equal(da.nbr, db.nbr)
+    assert da.ptime == db.ptime and da.kinetice == db.kinetice
+    np.testing.assert_array_equal(rows_b, db.rows)

@@ -93,3 +93,30 @@ def test_decks_reach_their_branches():
     # reports taking that branch
     assert DIGESTS["eos4_negative_c2"]["warnings"]["complex_sound_speed"] > 0
     assert all(DIGESTS[k]["warnings"]["complex_sound_speed"] == 0 for k in DIGESTS if k != "eos4_negative_c2")
+
+
+@pytest.mark.parametrize("name", sorted(json.load(open(os.path.join(DATA, "dropin.json")))))
+def test_reference_loop_matches_the_oracle(name, tmp_path):
+    """The reference's own unchanged time loop (oracle/_ref/ref_solution,
+    where the reference build exists) writes the files recorded in
+    dropin.json and ends in the oracle's state after the last step's
+    mcm_calctotale, bit for bit."""
+    import shutil
+    import subprocess
+    exe = os.path.join(os.path.dirname(DATA), "..", "..", "oracle", "_ref", "ref_solution")
+    if not os.path.exists(exe):
+        pytest.skip("reference build not present")
+    rec = json.load(open(os.path.join(DATA, "dropin.json")))[name]
+    shutil.copy(os.path.join(DATA, name + ".bin"), tmp_path / "deck.bin")
+    out = subprocess.run([exe, "deck.bin"] + rec["args"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
+    for f, digest in rec["files"].items():
+        assert hashlib.sha256((tmp_path / f).read_bytes()).hexdigest() == digest, f
+    o = Oracle(os.path.join(DATA, name + ".bin"))
+    assert o.initial() == 0
+    assert o.run(rec["nsteps"] - 1) == 0
+    for st in STAGES[:8]:
+        assert o.stage(st) == 0
+    want, got = o.state(rec["nsteps"], 8), Dump(str(tmp_path / "final.bin"))
+    assert np.array_equal(got.rows, want.rows)
+    assert got.ptime == want.ptime and got.kinetice == want.kinetice and got.internale == want.internale
