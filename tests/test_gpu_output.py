@@ -59,8 +59,12 @@ def test_material_files_of_a_two_material_run(tmp_path):
         assert mat == 2
         for m in (0, 1, 2):
             s.write_vtk(str(tmp_path / ("m%d.vtk" % m)), m)
+        # a material without particles gives a file without points, as the
+        # reference's WriteVTKmat(2) does for a one-material run
+        s.write_vtk(str(tmp_path / "m3.vtk"), 3)
         with pytest.raises(gpu.McmGpuError):
-            s.write_vtk(str(tmp_path / "m3.vtk"), 3)
+            s.write_vtk(str(tmp_path / "mneg.vtk"), -1)
+    assert "POINTS 0 double" in open(tmp_path / "m3.vtk").read()
     from mcmtop_amd import deck
     ids = deck.Deck.read_bin(path).mat
     for m in (0, 1, 2):
