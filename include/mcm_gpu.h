@@ -123,6 +123,25 @@ int mcm_gpu_momentum(mcm_gpu_ctx *ctx);
 int mcm_gpu_boundcond(mcm_gpu_ctx *ctx);
 /* mcm_calctotale (mcm_calctotale.f90:27-60) */
 int mcm_gpu_calctotale(mcm_gpu_ctx *ctx);
+/* No counterpart in the reference: for callers that cannot allocate device
+   memory (code under HIP graph capture, serving loops, harnesses that forbid
+   hipMalloc).  The energy totals of mcm_gpu_calctotale (mcm_calctotale.f90:
+   27-60: per material the kinetic energy from |v|, the internal energy and
+   the momentum sum m|v_k|) over particle arrays the caller holds in device
+   memory: mass, vabs, e, vx, vy, vz (np doubles each) and mat (np 0-based
+   material ids; ids outside [0, nummat) count nowhere).  out (device, 5 nummat
+   + 3 doubles) receives ke, ie, mom(3) per material, then kinetice,
+   internale, totale.  Enqueued on stream (a hipStream_t, NULL for the
+   default stream); no allocation, no synchronisation.  The block partials
+   live in a fixed device array of the library (about 6 MB, held by every GPU
+   the library runs on until the process ends): np <= MCM_NOALLOC_MAX_NP and
+   1 <= nummat <= 8, else MCM_GPU_ERR_ARG before anything is launched.  Calls
+   that overlap on one GPU share that array and race.  Same kernels and block
+   order as mcm_gpu_calctotale, so the same sums bit for bit. */
+#define MCM_NOALLOC_MAX_NP (1 << 22)
+int mcm_gpu_calctotale_noalloc(const double *mass, const double *vabs, const double *e, const double *vx,
+                               const double *vy, const double *vz, const int32_t *mat, int64_t np, int32_t nummat,
+                               double *out, void *stream);
 /* The per-material totals of the last mcm_gpu_calctotale (or fused step):
    out[5 m .. 5 m + 4] = ke, ie, mom(3) of material m + 1, the reference's
    mcm_mat(m+1)%ke, %ie, %mom (mcm_calctotale.f90:33-52); n >= 5 nummat. */

@@ -2153,4 +2153,5 @@ void mcm_gpu_batch_destroy(mcm_gpu_batch* b) {
 
 }  // extern "C"
 
+#include "mcm_noalloc.hip"
 #include "mcm_adjoint.hip"

@@ -39,6 +39,7 @@ SYMBOLS = (
     "mcm_gpu_set_material_params", "mcm_gpu_get_material_params", "mcm_gpu_set_initial_velocity",
     "mcm_gpu_adj_forward", "mcm_gpu_adj_backward", "mcm_gpu_adj_last_error", "mcm_gpu_adj_destroy",
     "mcm_gpu_set_time", "mcm_gpu_download_state", "mcm_gpu_upload_state", "mcm_gpu_get_energies",
+    "mcm_gpu_calctotale_noalloc",
 )
 
 STATE_WORDS = 86   # row of the per-particle state (include/mcm_gpu.h, the dump row layout)
@@ -133,6 +134,7 @@ def load_library(path: str = LIB_PATH):
     lib.mcm_gpu_download_state.argtypes = [vp, dp, ctypes.c_int64, ctypes.POINTER(Scalars)]
     lib.mcm_gpu_upload_state.argtypes = [vp, dp, ctypes.c_int64, ctypes.POINTER(Scalars)]
     lib.mcm_gpu_get_energies.argtypes = [vp, dp, ctypes.c_int32]
+    lib.mcm_gpu_calctotale_noalloc.argtypes = [vp] * 7 + [ctypes.c_int64, ctypes.c_int32, vp, vp]
     special = ("mcm_gpu_destroy", "mcm_gpu_last_error", "mcm_gpu_batch_size", "mcm_gpu_batch_run",
                "mcm_gpu_batch_last_error", "mcm_gpu_batch_destroy", "mcm_gpu_adj_last_error",
                "mcm_gpu_adj_destroy")
