@@ -503,7 +503,8 @@ def main():
         res["identification"] = ident_leg(args.ident, args.ident_steps, args.ident_iters, args.ident_threads, 5)
     if args.adjoint_steps > 0:
         res["adjoint"] = adjoint_leg(args.adjoint_steps, args.adjoint_ckpt, args.adjoint_dx, 32, bool(args.adjoint_fd))
-        res["adjoint_forward_ratio"] = res["adjoint"]["ratio_reverse"]
+        # the whole gradient: the recorded forward plus the reverse sweep, over one plain forward
+        res["adjoint_forward_ratio"] = res["adjoint"]["ratio_recorded_plus_reverse"]
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(tmpdir, args.dx)
     print(json.dumps(res), flush=True)

@@ -1085,16 +1085,28 @@ int adj_granular_rest(mcm_gpu_adjoint* A) {
     return rc;
 }
 
-// one step through the granular stages, which leave the step's intermediates
-// on the device; the neighbour search is launched without the capacity round
-// trip (a recomputed step cannot outgrow them; k_adj_check reports it)
-int adj_granular_step(mcm_gpu_adjoint* A) {
+// one recomputed step of a segment: the fused step's kernels with their
+// diagnostics kept (rod, spin, critts, the acceleration after the boundary
+// conditions, rhoold, etry, einc), which leave on the device every
+// intermediate the reverse step reads, as the granular stages do, in four
+// pair and particle launches instead of eleven (plain SPH, fixed h, no
+// contact or XSPH: the adjoint's scope)
+int adj_fused_diag_step(mcm_gpu_adjoint* A) {
     mcm_gpu_ctx* ctx = A->ctx;
-    launch_neighbours(one(ctx), ctx->cur);
-    if (hipGetLastError() != hipSuccess) { A->err = "neighbour launch failed"; return MCM_GPU_ERR_HIP; }
-    k_adj_check<<<1, 1, 0, ctx->stream>>>(ctx->d, A->a);
-    ctx->fresh = true;   // this step's strain records are laid down
-    return adj_granular_rest(A);
+    const DevState& d = ctx->d;
+    hipStream_t st = ctx->stream;
+    const int cur = ctx->cur;
+    launch_neighbours(one(ctx), cur);
+    k_adj_check<<<1, 1, 0, st>>>(ctx->d, A->a);
+    k_strain_const<MODE_FSTRAIN, 0, false><<<ctx->nb, MCM_BLOCK, 0, st>>>(ctx->d, cur);
+    k_strain_const<MODE_FCONST, 0, true><<<ctx->nb, MCM_BLOCK, 0, st>>>(ctx->d, cur);
+    k_strain_fold<MODE_FCONST><<<1, MCM_BLOCK, 0, st>>>(ctx->d);
+    if (d.boundary) k_ghost_stress<<<ghost_blocks(d), MCM_BLOCK, 0, st>>>(ctx->d, cur);
+    k_momentum<MODE_FUSED, 0, true><<<ctx->nb, MCM_BLOCK, 0, st>>>(ctx->d, cur);
+    launch_c_fold(one(ctx), 1);
+    HIPCHK_A(hipGetLastError());
+    ctx->cur ^= 1;
+    return 0;
 }
 
 // recording with every step kept: each step runs through the granular stages
@@ -1452,12 +1464,12 @@ static int adj_sweep(mcm_gpu_adjoint* A, const double* xbar, const double* vbar,
     }
     for (int sgi = A->store ? -1 : A->nseg - 1; sgi >= 0; sgi--) {
         const int s0 = sgi * A->k, s1 = std::min(s0 + A->k, A->nsteps);
-        // recompute the segment through the granular stages, keeping the
-        // start of every step and the step's intermediates
+        // recompute the segment with the fused kernels, keeping the start of
+        // every step and the step's intermediates
         if ((rc = adj_restore(A, true, sgi))) return rc;
         for (int n = s0; n < s1; n++) {
             if ((rc = adj_save(A, false, n - s0))) return rc;
-            if ((rc = adj_granular_step(A))) return rc;
+            if ((rc = adj_fused_diag_step(A))) return rc;
             if ((rc = adj_inter(A, n - s0, true))) return rc;
         }
         for (int n = s1 - 1; n >= s0; n--) {

@@ -48,7 +48,9 @@ mcm_f3dm1 mcm_f3dm3 mcm_f3dm9 mcm_f3dm10 mcm_f3dm25
 mcm_lieupd mcm_hieupd mcm_sueos mcm_eqos mcm_eqos1 mcm_eqos4 mcm_eqos13 mcm_eqos28 mcm_eqos41
 mcm_stress_up mcm_momentum mcm_update_ghost_stress mcm_oldacceleration mcm_corr_acceleration
 mcm_get_j_moment_info mcm_boundcond mcm_update_velocity mcm_move mcm_check_sym_pen
-mcm_rep_contact mcm_get_j_cont_info"
+mcm_rep_contact mcm_get_j_cont_info
+mcm_gttxsg mcm_matin mcm_nodein mcm_nodehist mcm_Inputvel mcm_inp_symmetry_v2 mcm_contactype mcm_baseaccel_in
+mcm_termin mcm_transducers_in"
 OBJS=""
 for f in $FILES; do
   src="$REF/$f.f90"

@@ -8,8 +8,11 @@
 ! mcm_database module state as the reference deck reader would
 ! (mcm_control_v2.f90:29-235, mcm_matin.f90:35-208, mcm_nodein.f90:30-61,
 ! mcm_inputvel.f90:28-131, mcm_inp_symmetry_v2.f90:111-137), calling the
-! reference's own material set-up routines; probe_dump writes the state dump
-! read by oracle/dumpio.py.
+! reference's own material set-up routines.  probe_read_text_deck reads the
+! reference's version-2 text deck with the reference's own card routines
+! (mcm_getinput.f90:89-127: mcm_matin, mcm_nodein, mcm_inputvel,
+! mcm_inp_symmetry_v2, mcm_contact_in, mcm_baseaccel_in), restating only the
+! control cards.  probe_dump writes the state dump read by oracle/dumpio.py.
 module ref_deck
 use mcm_database
 implicit none
@@ -166,6 +169,77 @@ contains
     if (mcm_boundary_code(2,j).gt.0) mcm_boundary_type(j) = mcm_boundary_type(j) + 2
   enddo
  end subroutine probe_read_deck
+
+ ! The version-2 text deck through the reference's own card routines, in the
+ ! order of mcm_getinput.f90:89-127.  Only mcm_control_v2 is restated (its
+ ! defaults, the six control cards with the reference's formats, the seventh
+ ! card skipped, the derived values; mcm_control_v2.f90:29-235): the routine
+ ! itself also opens the run's log and prints the version banner through
+ ! Intel's time/ctime (mcm_print_version.f90:54-55), which flang lacks.
+ subroutine probe_read_text_deck(fname)
+  character(len=*) :: fname
+  character(len=80) :: txts
+  integer :: lcount, vern, mirr_opt, keep_files
+  open(unit=101, file=fname, status='old')
+  mcm_openfiles(1) = 1
+  mcm_boundary_type = 1
+  mcm_boundary_code = 0
+  call mcm_gttxsg(txts, lcount)
+  read(unit=txts, fmt='(a78,i2)') mcm_title, vern
+  if (vern .ne. 2) stop 'text deck: input format version 2 expected'
+  pi = acos(-1.0_d)
+  mcm_disctype = 0; mcm_max_np = 0; mcm_itss = 0.0_d; mcm_state_opt = 0; mcm_thnode = 0
+  mcm_num_transducer = 0; mcm_status_interval = 100; mcm_restart_interval = 0; mcm_run_restart = 0
+  keep_files = 0; mcm_veloc_opt = 0; mcm_massopt = 0; mcm_init_h_opt = 0; mcm_init_rhoe = 0
+  mcm_contacttype = 0; mcm_tcrit_opt = 0; mcm_boundary = .false.; mirr_opt = 0; mcm_nlcur = 0
+  mcm_h_opt = 0; mcm_krtype = 1; mcm_maxnbr = 40; mcm_nthpx = 0; mcm_nthpy = 0; mcm_nthpz = 0
+  mcm_drelax = .false.
+  call mcm_gttxsg(txts, lcount)
+  read(unit=txts, fmt='(2i5,5i10)') mcm_axopt, mcm_disctype, mcm_nummat, mcm_np, mcm_max_np, mcm_nstressp, &
+       mcm_nvelocp
+  call mcm_gttxsg(txts, lcount)
+  read(unit=txts, fmt='(4e10.0)') mcm_endtime, mcm_tssf, mcm_itss, mcm_drelax_scale
+  call mcm_gttxsg(txts, lcount)
+  read(unit=txts, fmt='(e10.0,i5,e10.0,i10,i5,i10,2i5)') mcm_stpltime, mcm_state_opt, mcm_thpltime, mcm_thnode, &
+       mcm_num_transducer, mcm_status_interval, mcm_restart_interval, mcm_run_restart
+  call mcm_gttxsg(txts, lcount)
+  read(unit=txts, fmt='(5i5)') keep_files, mcm_init_v_opt, mcm_massopt, mcm_init_h_opt, mcm_init_rhoe
+  call mcm_gttxsg(txts, lcount)
+  read(unit=txts, fmt='(10i5)') mcm_contacttype, mcm_tcrit_opt, mcm_veloc_opt, mirr_opt, mcm_nlcur, mcm_nthpx, &
+       mcm_nthpy, mcm_nthpz
+  call mcm_gttxsg(txts, lcount)
+  read(unit=txts, fmt='(3i5)') mcm_h_opt, mcm_krtype, mcm_maxnbr
+  call mcm_gttxsg(txts, lcount)
+  if (mcm_tssf .eq. 0.0_d) mcm_tssf = 0.8_d
+  if (mcm_krtype .eq. 0) mcm_krtype = 1
+  if (mcm_maxnbr .eq. 0) mcm_maxnbr = 40
+  if (mcm_status_interval .lt. 1) mcm_status_interval = 100
+  if (mirr_opt .eq. 1) mcm_boundary = .true.
+  mcm_maxcont = mcm_maxnbr
+  mcm_g_maxnbr = mcm_maxnbr/2
+  mcm_g_maxcont = mcm_g_maxnbr
+  if (mcm_max_np .lt. mcm_np) mcm_max_np = mcm_np
+  if (mcm_axopt .lt. 1 .or. mcm_axopt .gt. 3) stop 'text deck: axis option 1, 2 or 3 expected'
+  mcm_ndim = mcm_axopt
+  if (mcm_drelax_scale .gt. 0.1_d .and. mcm_drelax_scale .lt. 1.0_d) mcm_drelax = .true.
+  mcm_ssp = 1; mcm_svp = 1; mcm_esp = mcm_np; mcm_evp = mcm_np
+  if (mcm_nthpx .eq. 1 .or. mcm_nthpy .eq. 1 .or. mcm_nthpz .eq. 1) mcm_baseaccel = .true.
+  mcm_ptime = 0.0_d
+  mcm_timestep = 0
+  ! mcm_getinput.f90:112-127, the reference's own routines
+  call mcm_allocate_memory
+  call mcm_initialise_database
+  call mcm_matin
+  call mcm_nodein
+  if (mcm_thnode .gt. 0) call mcm_nodehist
+  if (mcm_num_transducer .gt. 0) call mcm_transducers_in
+  call mcm_inputvel
+  if (mcm_boundary) call mcm_inp_symmetry_v2
+  if (mcm_contacttype .gt. 0) call mcm_contact_in
+  if (mcm_baseaccel) call mcm_baseaccel_in
+  close(101)
+  mcm_openfiles(1) = 0
+ end subroutine probe_read_text_deck
 
  ! Dump layout (little-endian stream):
  !   int32  magic 'MCMD', step, stage, np, ngp, nummat

@@ -20,7 +20,7 @@
 !   4. writes raw little-endian dumps of the per-particle state after selected
 !      stages (or, in timing mode, only the wall-clock of the step loop).
 !
-! Usage: ref_probe <deck.bin> <outdir> <nsteps> <mode> [dump_steps...]
+! Usage: ref_probe <deck.bin | deck.mcm> <outdir> <nsteps> <mode> [dump_steps...]
 !   mode = 0 : timing only (no dumps, prints wall seconds for the step loop)
 !   mode = 1 : dump every stage of the listed steps (and the init state)
 !   mode = 2 : dump only the end-of-step state (after mcm_move) of listed steps
@@ -31,7 +31,7 @@ use mcm_database
 use ref_deck
 implicit none
 character(len=512) :: deckfile, outdir, arg
-integer :: nsteps, mode, nlist, istep, k, i
+integer :: nsteps, mode, nlist, istep, k, i, n
 integer, allocatable :: dumplist(:)
 integer(8) :: t0, t1, rate
 logical :: dumpthis
@@ -51,7 +51,14 @@ open(unit=13, file=trim(outdir)//'/ref.log', status='replace', form='formatted')
 mcm_openfiles = 0
 mcm_openfiles(13) = 1
 
-call probe_read_deck(trim(deckfile))
+! a name ending in .mcm is the reference's text deck, read by its own card
+! routines; anything else is the binary deck
+n = len_trim(deckfile)
+if (n .gt. 4 .and. deckfile(max(1,n-3):n) .eq. '.mcm') then
+  call probe_read_text_deck(trim(deckfile))
+else
+  call probe_read_deck(trim(deckfile))
+endif
 call probe_initial()
 if (mode.ne.0) call probe_dump(trim(outdir), 0, 0)
 

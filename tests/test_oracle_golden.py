@@ -120,3 +120,29 @@ def test_reference_loop_matches_the_oracle(name, tmp_path):
     want, got = o.state(rec["nsteps"], 8), Dump(str(tmp_path / "final.bin"))
     assert np.array_equal(got.rows, want.rows)
     assert got.ptime == want.ptime and got.kinetice == want.kinetice and got.internale == want.internale
+
+
+@pytest.mark.parametrize("name", sorted(cases.CASES))
+def test_reference_card_routines_read_the_text_deck_alike(name, tmp_path):
+    """The reference's own card routines (mcm_matin, mcm_nodein,
+    mcm_inputvel, mcm_inp_symmetry_v2, mcm_contact_in, mcm_baseaccel_in,
+    compiled in place; only mcm_control_v2 restated) read the deck's text
+    form (Deck.write_mcm) into exactly the state the binary deck gives: every
+    dump of the case equals the digests of tests/golden/data/digests.json.
+    With test_textdeck.py (the library's reader turns the same text into the
+    binary deck's bytes) this pins the text reader to the reference's."""
+    import subprocess
+    probe = os.path.join(os.path.dirname(DATA), "..", "..", "oracle", "_ref", "ref_probe")
+    if not os.path.exists(probe):
+        pytest.skip("reference build not present")
+    rec = DIGESTS[name]
+    txt = str(tmp_path / "deck.mcm")
+    cases.build(name).write_mcm(txt)
+    steps = sorted(set(rec["stage_steps"]) | {rec["nsteps"]})
+    out = subprocess.run([probe, txt, str(tmp_path), str(rec["nsteps"]), "1"] + [str(s) for s in steps],
+                         capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
+    for key, want in rec["dumps"].items():
+        step, stage = (int(v) for v in key.split("_"))
+        raw = open(tmp_path / ("dump_%06d_%02d.bin" % (step, stage)), "rb").read()
+        assert hashlib.sha256(raw).hexdigest() == want["sha256"], key
