@@ -50,11 +50,25 @@ FP64_PEAK_TFLOPS = 78.6
 # Flops (add, mul, div, sqrt = 1; no FMA contraction): per list entry over the
 # reference's three passes, and per particle (inverse of B, contractions,
 # integration): strain 242 / 81, momentum 209 / 163.
+# Algorithmic bytes (per particle, per list entry) and flops (per list
+# entry, per particle) of the kernels that can dominate a step; keys
+# (kernel, corrected passes).
 KERNELS = {
-    "strain": {"name": "k_strain_const<MODE_FSTRAIN, corrected>", "symbol": "void k_strain_const<4, 1, false>(DevState, int)",
-               "bytes": (188.0, 4.0), "flops": (242.0, 81.0)},
-    "momentum": {"name": "k_momentum<MODE_FUSED, corrected>", "symbol": "void k_momentum<0, 1, false>(DevState, int)",
-                 "bytes": (208.0, 4.0), "flops": (209.0, 163.0)},
+    ("strain", 3): {"name": "k_strain_const<MODE_FSTRAIN, corrected>",
+                    "symbol": "void k_strain_const<4, 1, false>(DevState, int)",
+                    "bytes": (188.0, 4.0), "flops": (242.0, 81.0)},
+    ("momentum", 3): {"name": "k_momentum<MODE_FUSED, corrected>", "symbol": "void k_momentum<0, 1, false>(DevState, int)",
+                      "bytes": (208.0, 4.0), "flops": (209.0, 163.0)},
+    ("strain", 1): {"name": "k_strain_const<MODE_FSTRAIN, corrected one-pass>",
+                    "symbol": "void k_strain_const<4, 2, false>(DevState, int)",
+                    "bytes": (188.0, 4.0), "flops": (100.0, 150.0)},
+    ("momentum", 1): {"name": "k_momentum<MODE_FUSED, corrected one-pass>",
+                      "symbol": "void k_momentum<0, 2, false>(DevState, int)",
+                      "bytes": (208.0, 4.0), "flops": (130.0, 260.0)},
+    ("list", 1): {"name": "k_list<one material>", "symbol": "void k_list<false>(DevState, int)",
+                  "bytes": (56.0, 4.0), "flops": None},
+    ("list", 3): {"name": "k_list<one material>", "symbol": "void k_list<false>(DevState, int)",
+                  "bytes": (56.0, 4.0), "flops": None},
 }
 PROFILE = os.path.join(HERE, "profiles", "step_kernels_taylor_bar_corrected_2.1M.md")
 
@@ -111,6 +125,34 @@ def cpu_baseline(tmpdir: str, dx: float):
     return {"value": d.np_ * steps / secs, "unit": "particle-steps/s", "cores": 1, "kind": kind,
             "sample": "same bar section and dx (0.073 mm), 1.0 mm long (%d particles), %d steps, %s"
                       % (d.np_, steps, "flang -O2 build of the reference" if kind == "reference" else "C oracle")}
+
+
+def kernel_ms(ms):
+    return {"neighbours": ms[0], "list_build": ms[5], "strain_pass": ms[1], "constitutive": ms[2], "momentum": ms[3],
+            "step": ms[4]}
+
+
+def timed_run(deck_path: str, passes: int, steps: int, warmup: int):
+    """warmup untimed steps, then `steps` timed steps of the bar (corrected SPH
+    in `passes` passes) between two synchronisations of the run's stream,
+    then per-kernel times of 10 more steps with HIP events."""
+    from mcmtop_amd import gpu
+    with gpu.Solver(deck_path, corrected_one_pass=passes == 1) as s:
+        s.initial()
+        s.step(warmup)
+        # timed region: the step call enqueues on the context stream and
+        # returns after synchronising it
+        t0 = time.perf_counter()
+        s.step(steps)
+        t1 = time.perf_counter()
+        sc = s.scalars()
+        ms = s.time_steps(10)
+        counts, gcounts, _ = s.neighbour_lists()
+        nbar = float((counts + gcounts).mean())
+    secs = t1 - t0
+    print("bench: %d-pass corrected, %.3f ms/step" % (passes, 1e3 * secs / steps), file=sys.stderr, flush=True)
+    return {"value": sc.np * steps / secs, "secs": secs, "ms_per_step": 1e3 * secs / steps, "ms": ms, "nbar": nbar,
+            "sc": sc}
 
 
 def batched_leg(nruns: int, seed: int, steps: int, warmup: int, check_every: int, plain: bool = True):
@@ -414,8 +456,11 @@ def main():
                     help="bars of the identification leg at the configuration's size (100k particles, 600 steps, "
                          "checkpoint every 40; 0: skip it)")
     ap.add_argument("--ident-config5-iters", type=int, default=2)
-    ap.add_argument("--corrected-one-pass", action="store_true",
-                    help="accumulate the corrected kernels' three passes in one (rounding differs)")
+    ap.add_argument("--passes", type=int, default=1, choices=(1, 3),
+                    help="corrected SPH of the headline run: 1, single-pass moments (FP64, within the "
+                         "target's accuracy bar, tests/test_gpu_one_pass.py); 3, the reference's three "
+                         "passes (bit-exact). The other mode is timed beside it.")
+    ap.add_argument("--other-steps", type=int, default=100, help="timed steps of the other mode (0: skip it)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device copies of MIB MiB (a known-size kernel for scaling the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -440,33 +485,31 @@ def main():
     t = time.perf_counter()
     d = build_deck(deck_path, args.dx, 25.4e-3)
     print("bench: deck %d particles in %.1f s" % (d.np_, time.perf_counter() - t), file=sys.stderr, flush=True)
-    s = gpu.Solver(deck_path, corrected_one_pass=args.corrected_one_pass)
-    s.initial()
-    s.step(args.warmup)
-    sc = s.scalars()
-    # timed region: the step call enqueues on the context stream and returns
-    # after synchronising it
-    t0 = time.perf_counter()
-    s.step(args.steps)
-    t1 = time.perf_counter()
-    sc1 = s.scalars()
-    secs = t1 - t0
+    head = timed_run(deck_path, args.passes, args.steps, args.warmup)
     n = d.np_
-    value = n * args.steps / secs
-    # per-kernel durations with HIP events on the context stream
-    ms = s.time_steps(10)
-    counts, gcounts, _ = s.neighbour_lists()
-    nbar = float((counts + gcounts).mean())
-    # the dominant pair kernel: roofline against HBM (algorithmic bytes) and
-    # its FP64 rate (algorithmic flops)
-    kms = {"strain": ms[1], "momentum": ms[3]}
+    value = head["value"]
+    ms, nbar, sc1 = head["ms"], head["nbar"], head["sc"]
+    # the dominant kernel: roofline against HBM (algorithmic bytes) and, for
+    # the pair kernels, its FP64 rate (algorithmic flops)
+    kms = {"list": ms[5], "strain": ms[1], "momentum": ms[3]}
     dom = max(kms, key=kms.get)
-    K = KERNELS[dom]
+    K = KERNELS[(dom, args.passes)]
     t = kms[dom] * 1e-3
     bytes_launch = n * (K["bytes"][0] + K["bytes"][1] * nbar)
-    flops_launch = n * (K["flops"][0] * nbar + K["flops"][1])
     achieved = bytes_launch / t / 1e9
     traffic = measured_traffic(K["symbol"])
+    fp64 = None
+    if K["flops"]:
+        flops_launch = n * (K["flops"][0] * nbar + K["flops"][1])
+        fp64 = {"achieved": flops_launch / t / 1e12, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": flops_launch / t / 1e12 / FP64_PEAK_TFLOPS}
+    other = None
+    if args.other_steps > 0:
+        o = timed_run(deck_path, 4 - args.passes, args.other_steps, args.warmup)
+        other = {"corrected_passes": 4 - args.passes, "value": o["value"], "ms_per_step": o["ms_per_step"],
+                 "steps": args.other_steps, "kernel_ms": kernel_ms(o["ms"]),
+                 "accuracy": "bit-exact (the reference's three passes)" if args.passes == 1 else
+                             "single-pass moments, within the 1e-10 / 1e-7 / 1e-9 bar"}
     res = {
         "metric": "FP64 particle-steps/s, batched vs plain @\u22641e-7 L2; adjoint/forward time ratio",
         "value": value,
@@ -474,25 +517,25 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * secs / args.steps,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic Taylor-bar deck (seeded lattice + jitter)",
         "config": {"workload": "taylor_bar_corrected_f3dm10_eqos4_hopt1", "particles": n, "dx_mm": args.dx * 1e3,
-                   "corrected_passes": 1 if args.corrected_one_pass else 3,
+                   "corrected_passes": args.passes,
+                   "accuracy": ("single-pass moments: per-step fields within 1e-10, 1000-step L2 within 1e-7, "
+                                "energies within 1e-9 of the reference (tests/test_gpu_one_pass.py)")
+                   if args.passes == 1 else "bit-exact with the reference",
                    "mean_neighbours": round(nbar, 2), "ghosts": sc1.ngp, "ptime_end": sc1.ptime,
-                   "pair_interactions_per_s": n * nbar * args.steps / secs,
-                   "kernel_ms": {"neighbours": ms[0], "strain_pass": ms[1], "constitutive": ms[2],
-                                 "momentum": ms[3], "step": ms[4]}},
+                   "pair_interactions_per_s": n * nbar * args.steps / head["secs"],
+                   "kernel_ms": kernel_ms(ms)},
         "roofline": {"bound": "hbm", "kernel": K["name"], "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic, "algorithmic_bytes": bytes_launch, "launch_ms": kms[dom],
-                     "fp64": {"achieved": flops_launch / t / 1e12, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                              "frac": flops_launch / t / 1e12 / FP64_PEAK_TFLOPS}},
+                     "traffic": traffic, "algorithmic_bytes": bytes_launch, "launch_ms": kms[dom], "fp64": fp64},
+        "other_mode": other,
     }
-    s.close()
     if args.batch > 0:
         res["batched"] = batched_leg(args.batch, args.batch_seed, args.batch_steps, 4, 16, bool(args.batch_plain))
     res["adjoint_forward_ratio"] = None

@@ -212,6 +212,11 @@ int mcm_gpu_dump(mcm_gpu_ctx *ctx, const char *path, int32_t step, int32_t stage
    refresh and the energy fold), whole step. */
 int mcm_gpu_time_steps(mcm_gpu_ctx *ctx, int32_t nsteps, double *ms);
 
+/* As mcm_gpu_time_steps into ms[n] (n >= 5), with ms[5] (n >= 6) the list
+   build kernel alone (the part of ms[0] after the counting sort and the
+   ghosts). */
+int mcm_gpu_time_kernels(mcm_gpu_ctx *ctx, int32_t nsteps, double *ms, int32_t n);
+
 /* Output sync for the writers that stay on the host (mcm_state_output,
    mcm_timehist: mcm_solution.f90:82-92).  Packs, on the device, the fields
    they read into one staging buffer and copies it to host[nbytes] in one

@@ -199,7 +199,7 @@ __device__ __forceinline__ void hess_apply(const KGH& k, const double xij[3], co
 __global__ __launch_bounds__(MCM_BLOCK) void k_adj_gt_fill(DevState d, AdjDev a) {
     const int i = blockIdx.x * MCM_BLOCK + threadIdx.x;
     if (i >= d.np) return;
-    const int* lst = nbr_list(d, i);
+    const int* lst = nbr_row(d, d.spos[i]);
     for (int k = d.nreal[i]; k < d.ntot[i]; k++) {
         const int g = lst[(long)k * 64] - d.np;
         const int slot = atomicAdd(&a.gt_cnt[g], 1);
@@ -269,10 +269,10 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_adj_move(DevState d, AdjDev a) {
     if (i < d.np) {
         const double dt1 = sc->dt, dt0 = sc->dtold;
         const double dtn = 0.5 * (dt1 + dt0);
-        const SRec me = load_srec(d.rs, i);
+        const SRec me = load_srec(d, i);
         const double vn[3] = {me.vx, me.vy, me.vz};
         double acc[3], vp[3], x[3], v[3], sx[3] = {1.0, 1.0, 1.0}, sv[3] = {1.0, 1.0, 1.0};
-        const MRec mr = load_mrec(d.rm, i);   // position after the strain pass (x1)
+        const MRec mr = load_mrec(d, i);   // position after the strain pass (x1)
         const double x1[3] = {mr.x, mr.y, mr.z};
         for (int k = 0; k < 3; k++) {
             acc[k] = d.a[k][i];
@@ -349,15 +349,15 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_adj_mom(DevState d, AdjDev a) {
     const int t = blockIdx.x * (MCM_BLOCK / kLpp) + threadIdx.x / kLpp;
     if (t >= d.np) return;   // the particle's whole group
     const int nxt = a.cur ^ 1;
-    const MRec me = load_mrec(d.rm, t);
+    const MRec me = load_mrec(d, t);
     double at[3];
     load_ab(a.abp, t, at);
     double Ab[6] = {0, 0, 0, 0, 0, 0}, xbt[3] = {0, 0, 0};
     const int n = d.ntot[t];
-    const int* lst = nbr_list(d, t);
+    const int* lst = nbr_row(d, d.spos[t]);
     for (int k = lane; k < n; k += kLpp) {
         const int u = lst[(long)k * 64];
-        const MRec r = load_mrec(d.rm, u);
+        const MRec r = load_mrec(d, u);
         const double havg = 0.5 * (me.h + r.h);
         const double xij[3] = {me.x - r.x, me.y - r.y, me.z - r.z};
         KGH kg;
@@ -400,13 +400,13 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_adj_mom_ghost(DevState d, AdjDev 
     const int ngp = d.sc->ngp;
     const int lane = threadIdx.x & 63, wpb = MCM_BLOCK / 64;
     for (int g = blockIdx.x * wpb + (threadIdx.x >> 6); g < ngp; g += gridDim.x * wpb) {
-        const MRec me = load_mrec(d.rm, d.np + g);
+        const MRec me = load_mrec(d, d.np + g);
         double Ab[6] = {0, 0, 0, 0, 0, 0}, xbg[3] = {0, 0, 0};
         const int cnt = min(a.gt_cnt[g], a.gtcap);
         const int* lst = a.gts + (long)g * a.gtcap;
         for (int k = lane; k < cnt; k += 64) {
             const int i = lst[k];
-            const MRec r = load_mrec(d.rm, i);
+            const MRec r = load_mrec(d, i);
             const double havg = 0.5 * (r.h + me.h);
             const double xij[3] = {r.x - me.x, r.y - me.y, r.z - me.z};
             KGH kg;
@@ -448,7 +448,7 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_adj_gmap_mom(DevState d, AdjDev a
     const int cnt = min(a.gof_cnt[p], ADJ_GOF);
     for (int k = 0; k < cnt; k++) {
         const int g = a.gof[(long)p * ADJ_GOF + k], u = d.np + g;
-        const MRec me = load_mrec(d.rm, u);
+        const MRec me = load_mrec(d, u);
         const double rg = d.rbuf[a.cur][u], r2 = rg * rg;
         double rbar = 0.0;
         for (int c = 0; c < 6; c++) {
@@ -659,7 +659,7 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_adj_const_fin(DevState d, AdjDev 
         a.fb[i] = x[IN_EFPS];
         const double va = snap_at(a, np, SN_VABS, i);
         if (x[IN_VABS] != 0.0 && va > 0.0) {
-            const SRec me = load_srec(d.rs, i);
+            const SRec me = load_srec(d, i);
             const double f = x[IN_VABS] / va;
             a.vb[0][i] += f * me.vx;
             a.vb[1][i] += f * me.vy;
@@ -689,7 +689,7 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_adj_strain(DevState d, AdjDev a) 
     const int t = blockIdx.x * (MCM_BLOCK / kLpp) + threadIdx.x / kLpp;
     if (t >= d.np) return;   // the particle's whole group
     const double dt = d.sc->dtold;
-    const SRec me = load_srec(d.rs, t);   // records hold the strain pass's positions
+    const SRec me = load_srec(d, t);   // records hold the strain pass's positions
     const double vt[3] = {me.vx, me.vy, me.vz};
     const double xh[3] = {me.x, me.y, me.z};
     const double rho_t = d.rbuf[a.cur][t];
@@ -697,10 +697,10 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_adj_strain(DevState d, AdjDev a) 
     load_g9(a.lbp, t, Gt);
     double vbt[3] = {0, 0, 0}, xhb[3] = {0, 0, 0}, rbt = 0.0;
     const int n = d.ntot[t];
-    const int* lst = nbr_list(d, t);
+    const int* lst = nbr_row(d, d.spos[t]);
     for (int k = lane; k < n; k += kLpp) {
         const int u = lst[(long)k * 64];
-        const SRec r = load_srec(d.rs, u);
+        const SRec r = load_srec(d, u);
         const double xj[3] = {r.x, r.y, r.z};
         const double havg = 0.5 * (me.h + r.h);
         const double xij[3] = {xh[0] - xj[0], xh[1] - xj[1], xh[2] - xj[2]};
@@ -753,14 +753,14 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_adj_strain_ghost(DevState d, AdjD
     const int lane = threadIdx.x & 63, wpb = MCM_BLOCK / 64;
     for (int g = blockIdx.x * wpb + (threadIdx.x >> 6); g < ngp; g += gridDim.x * wpb) {
         const int u = d.np + g;
-        const SRec me = load_srec(d.rs, u);
+        const SRec me = load_srec(d, u);
         const double rho_g = d.rbuf[a.cur][u];
         double vbg[3] = {0, 0, 0}, xbg[3] = {0, 0, 0}, rbg = 0.0;
         const int cnt = min(a.gt_cnt[g], a.gtcap);
         const int* lst = a.gts + (long)g * a.gtcap;
         for (int k = lane; k < cnt; k += 64) {
             const int i = lst[k];
-            const SRec r = load_srec(d.rs, i);
+            const SRec r = load_srec(d, i);
             const double xh[3] = {r.x, r.y, r.z};
             const double havg = 0.5 * (r.h + me.h);
             const double xij[3] = {xh[0] - me.x, xh[1] - me.y, xh[2] - me.z};
@@ -841,12 +841,15 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_adj_fold_end(DevState d, AdjDev a
 // The intermediates of one recomputed step that the adjoint kernels read:
 // gather records, both density buffers (over reals and the step's ghosts),
 // rod, spin, acceleration and critts of the reals; neighbour lists and
-// counts, ghost parents and mirror signs.  Field f of a slot: doubles at
+// counts, ghost parents and mirror signs, the lists' positions (spos).
+// Field f of a slot: doubles at
 // dbase + off, ints at ibase + off.
 struct InterLayout {
     long ucap, np, nbrlen, ngcap;
 };
-constexpr int INTER_ND = 17, INTER_NI = 5;
+constexpr int INTER_ND = 17, INTER_NI = 6;
+// doubles per unified entry: the two records, then both density buffers
+constexpr long INTER_RS = 2 * RS_SLOTS, INTER_RM = 2 * RM_SLOTS, INTER_UCAP = INTER_RS + INTER_RM + 2;
 
 __device__ __forceinline__ void inter_field(const DevState& d, const InterLayout& L, int f, int ngp, double*& dp,
                                             int*& ip, long& len, long& off) {
@@ -854,17 +857,17 @@ __device__ __forceinline__ void inter_field(const DevState& d, const InterLayout
     dp = nullptr;
     ip = nullptr;
     switch (f) {
-        case 0: dp = d.rs; len = 8 * u; off = 0; return;
-        case 1: dp = d.rm; len = 12 * u; off = 8 * L.ucap; return;
-        case 2: dp = d.rbuf[0]; len = u; off = 20 * L.ucap; return;
-        case 3: dp = d.rbuf[1]; len = u; off = 21 * L.ucap; return;
+        case 0: dp = d.rs; len = INTER_RS * u; off = 0; return;
+        case 1: dp = d.rm; len = INTER_RM * u; off = INTER_RS * L.ucap; return;
+        case 2: dp = d.rbuf[0]; len = u; off = (INTER_RS + INTER_RM) * L.ucap; return;
+        case 3: dp = d.rbuf[1]; len = u; off = (INTER_RS + INTER_RM + 1) * L.ucap; return;
         default: break;
     }
     if (f < INTER_ND) {
         const int k = f - 4;   // rod 6, spin 3, a 3, critts
         dp = k < 6 ? d.rod[k] : (k < 9 ? d.spin[k - 6] : (k < 12 ? d.a[k - 9] : d.critts_p));
         len = L.np;
-        off = 22 * L.ucap + (long)k * L.np;
+        off = INTER_UCAP * L.ucap + (long)k * L.np;
         return;
     }
     switch (f - INTER_ND) {
@@ -872,7 +875,8 @@ __device__ __forceinline__ void inter_field(const DevState& d, const InterLayout
         case 1: ip = d.nreal; len = L.np; off = L.nbrlen; return;
         case 2: ip = d.ntot; len = L.np; off = L.nbrlen + L.np; return;
         case 3: ip = d.gpar; len = ngp; off = L.nbrlen + 2 * L.np; return;
-        default: ip = d.gsgn; len = ngp; off = L.nbrlen + 2 * L.np + L.ngcap; return;
+        case 4: ip = d.gsgn; len = ngp; off = L.nbrlen + 2 * L.np + L.ngcap; return;
+        default: ip = d.spos; len = L.np; off = L.nbrlen + 2 * L.np + 2 * L.ngcap; return;   // list positions
     }
 }
 
@@ -1045,8 +1049,8 @@ int adj_inter_buffers(mcm_gpu_adjoint* A) {
     for (void* p : A->iallocs) (void)hipFree(p);
     A->iallocs.clear();
     A->lay = InterLayout{ucap, (long)d.np, nbrlen, (long)d.ngcap};
-    A->inter_dn = (size_t)(22 * ucap + 13 * (long)d.np);
-    A->inter_in = (size_t)(nbrlen + 2 * (long)d.np + 2 * (long)d.ngcap);
+    A->inter_dn = (size_t)(INTER_UCAP * ucap + 13 * (long)d.np);
+    A->inter_in = (size_t)(nbrlen + 3 * (long)d.np + 2 * (long)d.ngcap);
     int rc = 0;
     rc |= adj_alloc(A, A->iallocs, &A->inter_d, A->inter_dn * A->ninter);
     rc |= adj_alloc(A, A->iallocs, &A->inter_i, A->inter_in * A->ninter);
@@ -1056,7 +1060,7 @@ int adj_inter_buffers(mcm_gpu_adjoint* A) {
 
 int adj_inter(mcm_gpu_adjoint* A, int s, bool save) {
     mcm_gpu_ctx* ctx = A->ctx;
-    const long big = std::max(A->lay.nbrlen, 22 * A->lay.ucap);
+    const long big = std::max(A->lay.nbrlen, INTER_UCAP * A->lay.ucap);
     dim3 grid((unsigned)std::min<long>(blocks_for(big / 8 + 1, MCM_BLOCK), 1024), INTER_ND + INTER_NI);
     if (save) {
         HIPCHK_A(hipMemcpyAsync(A->inter_sc + s, ctx->sc, sizeof(Scalars), hipMemcpyDeviceToDevice, ctx->stream));

@@ -7,6 +7,12 @@
 #define MCM_MAX_MAT 8
 #define MCM_BLOCK 256
 
+// Bits of DevState::range, set by the kernels that write the pair records
+// when a coordinate or h lies outside the ranges of the pair loops' fast
+// path (fast_coord, fast_h), cleared when a neighbour search starts:
+// RANGE_RS for the strain records rs, RANGE_RM for the momentum records rm.
+enum { RANGE_RS = 1, RANGE_RM = 2 };
+
 // butterfly reductions over the 64 lanes of a wavefront (all lanes active)
 __device__ __forceinline__ double wave_min(double v) {
     for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
@@ -50,6 +56,7 @@ struct Scalars {
     double mat_ke[MCM_MAX_MAT], mat_ie[MCM_MAX_MAT], mat_mom[MCM_MAX_MAT][3];
     GridDev grid;
     int ngp;            // ghosts generated this step
+    int nout;           // real particles outside the search grid
     int gplane_lo, gplane_hi; // ghost ids made by the plane being generated
     unsigned long long hcap_bits;  // bit pattern of the largest h among the binned particles
     int max_count;      // largest neighbour count seen by the last search
@@ -122,6 +129,7 @@ struct DevState {
     double *rod[6], *spin[3], *tracerod, *a[3], *einc, *etry, *rhoold, *critts_p;
     int* yield;                // active-set mask of the plastic models
     double* red;               // per-block partial reductions
+    int* range;                // RANGE_* bits: records outside the pair loops' fast-path ranges
     double* defps_prev;        // f3dm10 increment carried for failed particles
     // batched launch (null for a launch over one run): workgroup -> instance
     // block, and the states of all instances
@@ -155,13 +163,38 @@ __device__ __forceinline__ const DevState& enter(const DevState& d0, Blk& bk) {
     return *(const DevState*)p;
 }
 
-// Neighbour lists are blocked by 64 particles: for each block of 64
-// consecutive particles, slot k of all 64 is one contiguous 256-byte row, so
-// a wavefront of threads reading slot k of its particles reads one row.
-// Entry k of particle i is nbr_list(d, i)[k * 64].
-__device__ __forceinline__ int* nbr_list(const DevState& d, int i) {
-    return d.nbr + (long)(i >> 6) * d.ncap * 64 + (i & 63);
+// Neighbour lists are stored by position in the cell-sorted order of the
+// search (sorted[t] is the particle at position t and spos[i] the position
+// of particle i; particles outside the grid take the last positions), in
+// blocks of 64 positions: for each block, slot k of all 64 is one
+// contiguous 256-byte row, so a wavefront of threads reading slot k of its
+// positions reads one row.  The pair kernels take the particles in this
+// order: the lanes of a wavefront then lie in a few adjacent cells and
+// gather neighbours from fewer cache lines than particles taken in id
+// order.  Entry k of the particle at position t is nbr_row(d, t)[k * 64].
+__device__ __forceinline__ int* nbr_row(const DevState& d, int t) {
+    return d.nbr + (long)(t >> 6) * d.ncap * 64 + (t & 63);
 }
+// Pair gather records, one aligned record per entry of the unified index
+// space, in 16-byte slots:
+//   rs[u] (64 bytes): (x, y), (z, h), (m/rho, vx), (vy, vz)
+//   rm[u] (96 bytes): (x, y), (z, h), (m, m/rho), (A11, A22), (A33, A12), (A13, A23)
+// The pair loops' gathers are bound by the vector L1 path (the lanes of a
+// wavefront read 64 different neighbours per step).  Measured on the
+// 2.1 M-particle bar: whole records beat one array per slot (more lines per
+// neighbour, slower once the bar deforms), and the 96-byte momentum record
+// beats one padded to a 128-byte line (fewer L2 requests, but no two
+// lanes then share a line).
+enum { RS_SLOTS = 4, RM_SLOTS = 6 };
+__device__ __forceinline__ double2* rs_at(const DevState& d, int p, long u) {
+    return reinterpret_cast<double2*>(d.rs) + RS_SLOTS * u + p;
+}
+__device__ __forceinline__ double2* rm_at(const DevState& d, int p, long u) {
+    return reinterpret_cast<double2*>(d.rm) + RM_SLOTS * u + p;
+}
+// the first double of a slot
+__device__ __forceinline__ double* lo_of(double2* p) { return reinterpret_cast<double*>(p); }
+
 __device__ __forceinline__ int* cont_list(const DevState& d, int i) {
     return d.cont + (long)(i >> 6) * d.ncap * 64 + (i & 63);
 }

@@ -578,8 +578,8 @@ int alloc_all(mcm_gpu_ctx* ctx) {
     rc |= dalloc(ctx, &d.mass, u);
     for (int k = 0; k < 2; k++) { rc |= dalloc(ctx, &d.hbuf[k], u); rc |= dalloc(ctx, &d.rbuf[k], u); }
     rc |= dalloc(ctx, &d.pmat, u);
-    rc |= dalloc(ctx, &d.rs, 8 * u);
-    rc |= dalloc(ctx, &d.rm, 12 * u);
+    rc |= dalloc(ctx, &d.rs, 2 * RS_SLOTS * u);
+    rc |= dalloc(ctx, &d.rm, 2 * RM_SLOTS * u);
     rc |= dalloc(ctx, &d.gpar, d.ngcap);
     rc |= dalloc(ctx, &d.gsgn, d.ngcap);
     rc |= dalloc(ctx, &d.dispbc, n);
@@ -608,6 +608,8 @@ int alloc_all(mcm_gpu_ctx* ctx) {
     rc |= dalloc(ctx, &d.mat, d.nummat);
     rc |= dalloc(ctx, &d.sc, 1);
     ctx->sc = d.sc;
+    rc |= dalloc(ctx, &d.range, 1);
+    if (!rc) HIPCHK(hipMemsetAsync(d.range, 0xff, sizeof(int), ctx->stream));   // no fast path before a search
     rc |= alloc_neighbour_arrays(ctx);
     return rc;
 }
@@ -731,7 +733,8 @@ void launch(const Target& T, int kind, int v, int threads, void (*kern)(DevState
     kern<<<n, threads, 0, T.stream>>>(d, args...);
 }
 
-void launch_neighbours(const Target& T, int cur) {
+// the search up to the lists: grid, counting sort, candidate records, ghosts
+void launch_search(const Target& T, int cur) {
     launch(T, L_ONE, 0, 1, k_grid_setup);
     launch(T, L_CELLS, 0, MCM_BLOCK, k_clear_cells);
     launch(T, L_PART, 0, MCM_BLOCK, k_bin, cur);
@@ -745,9 +748,18 @@ void launch_neighbours(const Target& T, int cur) {
         launch(T, L_GSCAN, p, 1024, k_ghost_scan, p);
         launch(T, L_GFILL, p, MCM_BLOCK, k_ghost_fill, p, cur);
     }
+}
+
+// the list build
+void launch_lists(const Target& T, int cur) {
     launch(T, L_LIST, 0, MCM_BLOCK, k_list<false>, cur);
     launch(T, L_LIST, 1, MCM_BLOCK, k_list<true>, cur);
     launch(T, L_LIST, 2, MCM_BLOCK, k_list_wave, cur);
+}
+
+void launch_neighbours(const Target& T, int cur) {
+    launch_search(T, cur);
+    launch_lists(T, cur);
 }
 
 // energies (and, after a move, coordinate bounds) from the block partials
@@ -883,8 +895,8 @@ int regrow(mcm_gpu_ctx* ctx, const Scalars& sc_host) {
         rc |= dregrow(ctx, &d.mass, u, keep);
         for (int k = 0; k < 2; k++) { rc |= dregrow(ctx, &d.hbuf[k], u, keep); rc |= dregrow(ctx, &d.rbuf[k], u, keep); }
         rc |= dregrow(ctx, &d.pmat, u, keep);
-        rc |= dregrow(ctx, &d.rs, 8 * u, 8 * keep);
-        rc |= dregrow(ctx, &d.rm, 12 * u, 12 * keep);
+        rc |= dregrow(ctx, &d.rs, 2 * RS_SLOTS * u, 2 * RS_SLOTS * keep);
+        rc |= dregrow(ctx, &d.rm, 2 * RM_SLOTS * u, 2 * RM_SLOTS * keep);
         rc |= dregrow(ctx, &d.gpar, ng_new, 0);
         rc |= dregrow(ctx, &d.gsgn, ng_new, 0);
         if (rc) return rc;
@@ -1324,14 +1336,18 @@ void destroy_run(mcm_gpu_ctx* ctx) {
 // parity cur, with HIP events on T's stream: ms[0] neighbour search,
 // ms[1] strain kernel, ms[2] constitutive + fold, ms[3] momentum kernel,
 // ms[4] whole step.
+// ms[0..5]: neighbour search with its lists, strain pass, constitutive
+// update, momentum kernel, whole step, the list build alone
 int time_stages(const Target& T, int cur, int nsteps, double* ms, std::string& err) {
-    hipEvent_t ev[7] = {};
+    hipEvent_t ev[8] = {};
     hipError_t e = hipSuccess;
-    for (int k = 0; k < 7 && !e; k++) e = hipEventCreate(&ev[k]);
-    double acc[5] = {0, 0, 0, 0, 0};
+    for (int k = 0; k < 8 && !e; k++) e = hipEventCreate(&ev[k]);
+    double acc[6] = {0, 0, 0, 0, 0, 0};
     for (int it = 0; it < nsteps && !e; it++) {
         e = hipEventRecord(ev[0], T.stream);
-        launch_neighbours(T, cur);
+        launch_search(T, cur);
+        if (!e) e = hipEventRecord(ev[7], T.stream);
+        launch_lists(T, cur);
         if (!e) e = hipEventRecord(ev[1], T.stream);
         launch_strain(T, cur);
         if (!e) e = hipEventRecord(ev[2], T.stream);
@@ -1344,16 +1360,16 @@ int time_stages(const Target& T, int cur, int nsteps, double* ms, std::string& e
         launch_c_fold(T, 1);
         if (!e) e = hipEventRecord(ev[6], T.stream);
         if (!e) e = hipEventSynchronize(ev[6]);
-        const int pairs[5][2] = {{0, 1}, {1, 2}, {2, 3}, {4, 5}, {0, 6}};
-        for (int k = 0; k < 5 && !e; k++) {
+        const int pairs[6][2] = {{0, 1}, {1, 2}, {2, 3}, {4, 5}, {0, 6}, {7, 1}};
+        for (int k = 0; k < 6 && !e; k++) {
             float t = 0.0f;
             e = hipEventElapsedTime(&t, ev[pairs[k][0]], ev[pairs[k][1]]);
             acc[k] += t;
         }
         cur ^= 1;
     }
-    for (int k = 0; k < 5; k++) ms[k] = acc[k] / nsteps;
-    for (int k = 0; k < 7; k++)
+    for (int k = 0; k < 6; k++) ms[k] = acc[k] / nsteps;
+    for (int k = 0; k < 8; k++)
         if (ev[k]) (void)hipEventDestroy(ev[k]);
     if (!e) e = hipGetLastError();
     if (e) { err = std::string("timed steps: ") + hipGetErrorString(e); return MCM_GPU_ERR_HIP; }
@@ -1816,9 +1832,10 @@ int mcm_gpu_download_neighbours(mcm_gpu_ctx* ctx, int32_t* counts, int32_t* gcou
     if (!ctx || !counts || !gcounts || !cap) return MCM_GPU_ERR_ARG;
     DevState& d = ctx->d;
     const size_t np = d.np;
-    std::vector<int> nr(np), nt(np), lst(ids ? (size_t)d.ncap * d.npad : 0);
+    std::vector<int> nr(np), nt(np), pos(ids ? np : 0), lst(ids ? (size_t)d.ncap * d.npad : 0);
     HIPCHK(hipMemcpyAsync(nr.data(), d.nreal, np * 4, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipMemcpyAsync(nt.data(), d.ntot, np * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (ids) HIPCHK(hipMemcpyAsync(pos.data(), d.spos, np * 4, hipMemcpyDeviceToHost, ctx->stream));
     if (ids) HIPCHK(hipMemcpyAsync(lst.data(), d.nbr, lst.size() * 4, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     *cap = d.ncap;
@@ -1826,9 +1843,10 @@ int mcm_gpu_download_neighbours(mcm_gpu_ctx* ctx, int32_t* counts, int32_t* gcou
     for (size_t i = 0; i < np; i++) {
         counts[i] = nr[i];
         gcounts[i] = nt[i] - nr[i];
-        if (ids)
-            for (int k = 0; k < nt[i] && k < d.ncap; k++)
-                ids[i * d.ncap + k] = lst[((i >> 6) * d.ncap + k) * 64 + (i & 63)];
+        if (ids) {
+            const size_t t = pos[i];   // lists are stored by sorted position
+            for (int k = 0; k < nt[i] && k < d.ncap; k++) ids[i * d.ncap + k] = lst[((t >> 6) * d.ncap + k) * 64 + (t & 63)];
+        }
     }
     return 0;
 }
@@ -1862,7 +1880,7 @@ int mcm_gpu_dump(mcm_gpu_ctx* ctx, const char* path, int32_t step, int32_t stage
     const size_t ng = s.ngp;
     std::vector<double> rows, ghost;
     if ((rc = gather_rows(ctx, s, rows, &ghost))) return rc;
-    auto nreal = geti(d.nreal, np), ntot = geti(d.ntot, np), cell_of = geti(d.cell_of, np);
+    auto nreal = geti(d.nreal, np), ntot = geti(d.ntot, np), cell_of = geti(d.cell_of, np), spos = geti(d.spos, np);
     auto lst = geti(d.nbr, (size_t)d.ncap * d.npad);
     auto gpar = geti(d.gpar, ng), pmat = geti(d.pmat + np, ng);
     FILE* f = fopen(path, "wb");
@@ -1886,15 +1904,20 @@ int mcm_gpu_dump(mcm_gpu_ctx* ctx, const char* path, int32_t step, int32_t stage
     fwrite(nn.data(), 4, np, f);
     fwrite(gn.data(), 4, np, f);
     fwrite(llp.data(), 4, np, f);
+    // entry k of particle i (lists are stored by sorted position)
+    auto entry = [&](size_t i, int k) {
+        const size_t t = spos[i];
+        return lst[((t >> 6) * d.ncap + k) * 64 + (t & 63)];
+    };
     for (size_t i = 0; i < np; i++)
         for (int k = 0; k < nreal[i] && k < d.ncap; k++) {
-            int32_t v = lst[((i >> 6) * d.ncap + k) * 64 + (i & 63)] + 1;
+            int32_t v = entry(i, k) + 1;
             fwrite(&v, 4, 1, f);
         }
     if (d.boundary) {
         for (size_t i = 0; i < np; i++)
             for (int k = nreal[i]; k < ntot[i] && k < d.ncap; k++) {
-                int32_t v = lst[((i >> 6) * d.ncap + k) * 64 + (i & 63)] - (int32_t)np + 1;
+                int32_t v = entry(i, k) - (int32_t)np + 1;
                 fwrite(&v, 4, 1, f);
             }
         for (size_t g = 0; g < ng; g++) { int32_t v = gpar[g] + 1; fwrite(&v, 4, 1, f); }
@@ -1943,14 +1966,18 @@ int mcm_gpu_upload_state(mcm_gpu_ctx* ctx, const double* rows, int64_t np, const
     return scatter_rows(ctx, rows, *sc);
 }
 
-int mcm_gpu_time_steps(mcm_gpu_ctx* ctx, int32_t nsteps, double* ms) {
-    if (!ctx || !ms || nsteps <= 0 || ctx->phase != 0 || ctx->done != 0) return MCM_GPU_ERR_ARG;
+int mcm_gpu_time_kernels(mcm_gpu_ctx* ctx, int32_t nsteps, double* ms, int32_t n) {
+    if (!ctx || !ms || nsteps <= 0 || n < 5 || ctx->phase != 0 || ctx->done != 0) return MCM_GPU_ERR_ARG;
     if (ctx->batch) return stage_check(ctx, "mcm_gpu_time_steps", false, "");
     ctx->fresh = false;
-    int rc = time_stages(one(ctx), ctx->cur, nsteps, ms, ctx->err);
+    double t[6];
+    int rc = time_stages(one(ctx), ctx->cur, nsteps, t, ctx->err);
     ctx->cur ^= nsteps & 1;
+    for (int k = 0; k < std::min(n, 6); k++) ms[k] = t[k];
     return rc ? rc : check_sticky(ctx);
 }
+
+int mcm_gpu_time_steps(mcm_gpu_ctx* ctx, int32_t nsteps, double* ms) { return mcm_gpu_time_kernels(ctx, nsteps, ms, 5); }
 
 // ------------------------------------------------------------------ output
 static_assert(OUT_WORDS == MCM_OUT_WORDS && OUT_WORDS == vtkout::kWords, "packed output record size");
@@ -2104,7 +2131,9 @@ int mcm_gpu_batch_time_steps(mcm_gpu_batch* b, int32_t nsteps, double* ms) {
     if (!b || !ms || nsteps <= 0 || b->inst.empty()) return MCM_GPU_ERR_ARG;
     int rc = 0;
     if (b->stale && (rc = batch_upload(b))) return rc;
-    rc = time_stages(all(b), b->cur, nsteps, ms, b->err);
+    double t[6];
+    rc = time_stages(all(b), b->cur, nsteps, t, b->err);
+    for (int k = 0; k < 5; k++) ms[k] = t[k];
     b->cur ^= nsteps & 1;
     for (mcm_gpu_ctx* r : b->inst) r->cur = b->cur;
     if (rc) return rc;

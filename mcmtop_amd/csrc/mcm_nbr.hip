@@ -57,6 +57,8 @@ __global__ void k_grid_setup(DevState d_) {
     Blk bk;
     const DevState& d = enter<false>(d_, bk);
     Scalars* sc = d.sc;
+    d.range[0] = 0;   // the search rewrites every pair record; its writers flag them again
+    sc->nout = 0;
     GridDev g;
     const double factor = 2.0;
     for (int i = 0; i < 3; i++) {
@@ -133,10 +135,11 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_rs_shift(DevState d) {
     const int i = blockIdx.x * MCM_BLOCK + threadIdx.x;
     if (i >= d.np) return;
     const double dt = d.sc->dt;
-    double2* r = reinterpret_cast<double2*>(d.rs) + 4 * (long)i;
     const double vx = d.v[0][i], vy = d.v[1][i], vz = d.v[2][i];
-    r[0] = make_double2(d.x0[0][i] - 0.5 * vx * dt, d.x0[1][i] - 0.5 * vy * dt);
-    r[1] = make_double2(d.x0[2][i] - 0.5 * vz * dt, vx);
+    const double s0 = d.x0[0][i] - 0.5 * vx * dt, s1 = d.x0[1][i] - 0.5 * vy * dt, s2 = d.x0[2][i] - 0.5 * vz * dt;
+    *rs_at(d, 0, i) = make_double2(s0, s1);
+    *lo_of(rs_at(d, 1, i)) = s2;
+    if (!(fast_coord(s0) & fast_coord(s1) & fast_coord(s2))) atomicOr(d.range, (int)RANGE_RS);
 }
 
 // histogram: cell of every real particle (mcm_setuplist.f90:320-345); also
@@ -157,13 +160,14 @@ __global__ __launch_bounds__(256) void k_bin(DevState d_, int cur) {
         const double x[3] = {d.x0[0][i], d.x0[1][i], d.x0[2][i]};
         const double h = d.hbuf[cur][i];
         {
-            double2* r = reinterpret_cast<double2*>(d.rs) + 4 * (long)i;
             const double dt = d.sc->dt;
             double vx = d.v[0][i], vy = d.v[1][i], vz = d.v[2][i];
-            r[0] = make_double2(x[0] - 0.5 * vx * dt, x[1] - 0.5 * vy * dt);
-            r[1] = make_double2(x[2] - 0.5 * vz * dt, vx);
-            r[2] = make_double2(vy, vz);
-            r[3] = make_double2(d.mass[i] / d.rbuf[cur][i], h);
+            const double s0 = x[0] - 0.5 * vx * dt, s1 = x[1] - 0.5 * vy * dt, s2 = x[2] - 0.5 * vz * dt;
+            *rs_at(d, 0, i) = make_double2(s0, s1);
+            *rs_at(d, 1, i) = make_double2(s2, h);
+            *rs_at(d, 2, i) = make_double2(d.mass[i] / d.rbuf[cur][i], vx);
+            *rs_at(d, 3, i) = make_double2(vy, vz);
+            if (!(fast_coord(s0) & fast_coord(s1) & fast_coord(s2) & fast_h(h))) atomicOr(d.range, (int)RANGE_RS);
         }
         int box[3];
         bool in = true;
@@ -176,10 +180,14 @@ __global__ __launch_bounds__(256) void k_bin(DevState d_, int cur) {
             hb = h;
         } else {
             // outside the grid: no neighbours, only itself in corrected SPH
-            // (mcm_ll_neighbours.f90:109, 288-299)
+            // (mcm_ll_neighbours.f90:109, 288-299); the last positions of the
+            // sorted order
+            const int t = d.np - 1 - atomicAdd(&d.sc->nout, 1);
+            d.sorted[t] = i;
+            d.spos[i] = t;
             int cnt = 0;
             if (d.disctype == 1) {
-                nbr_list(d, i)[0] = i;
+                nbr_row(d, t)[0] = i;
                 cnt = 1;
             }
             d.nreal[i] = cnt;
@@ -344,15 +352,20 @@ __device__ __forceinline__ void make_ghost(const DevState& d, int cur, const Pla
     // pass does not shift ghosts (mcm_strain.f90:27-31), which is a shift
     // with a zero step, x - v 0 / 2, rounded as for a real particle
     const double sh = 0.0;
-    double2* rs = reinterpret_cast<double2*>(d.rs) + 4 * (long)u;
-    rs[0] = make_double2(d.x0[0][u] - 0.5 * d.v[0][u] * sh, d.x0[1][u] - 0.5 * d.v[1][u] * sh);
-    rs[1] = make_double2(d.x0[2][u] - 0.5 * d.v[2][u] * sh, d.v[0][u]);
-    rs[2] = make_double2(d.v[1][u], d.v[2][u]);
-    rs[3] = make_double2(mv / rv, hv);
-    double2* rm = reinterpret_cast<double2*>(d.rm) + 6 * (long)u;
-    rm[0] = make_double2(d.x0[0][u], d.x0[1][u]);
-    rm[1] = make_double2(d.x0[2][u], hv);
-    rm[2] = make_double2(mv, mv / rv);
+    const double s0 = d.x0[0][u] - 0.5 * d.v[0][u] * sh, s1 = d.x0[1][u] - 0.5 * d.v[1][u] * sh,
+                 s2 = d.x0[2][u] - 0.5 * d.v[2][u] * sh;
+    *rs_at(d, 0, u) = make_double2(s0, s1);
+    *rs_at(d, 1, u) = make_double2(s2, hv);
+    *rs_at(d, 2, u) = make_double2(mv / rv, d.v[0][u]);
+    *rs_at(d, 3, u) = make_double2(d.v[1][u], d.v[2][u]);
+    const double m0 = d.x0[0][u], m1 = d.x0[1][u], m2 = d.x0[2][u];
+    *rm_at(d, 0, u) = make_double2(m0, m1);
+    *rm_at(d, 1, u) = make_double2(m2, hv);
+    *rm_at(d, 2, u) = make_double2(mv, mv / rv);
+    const int okh = fast_h(hv);
+    const int bad = ((fast_coord(s0) & fast_coord(s1) & fast_coord(s2) & okh) ? 0 : (int)RANGE_RS) |
+                    ((fast_coord(m0) & fast_coord(m1) & fast_coord(m2) & okh) ? 0 : (int)RANGE_RM);
+    if (bad) atomicOr(d.range, bad);
 }
 
 // Layer geometry of one plane (mcm_ghost_setup.f90:38-43 and siblings):
@@ -476,7 +489,7 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_ghost_fill(DevState d_, int pi, i
 
 // ------------------------------------------------------------ list build
 // mcm_ll_neighbours.f90:96-299 for one real particle, one thread per
-// particle (list layout: nbr_list in mcm_device.h).  MULTI: more than one
+// particle (list layout: nbr_row in mcm_device.h).  MULTI: more than one
 // material, so same-material and other-material passes.
 //
 // Real candidates are read from cell runs of the sorted order.  A pair is
@@ -504,7 +517,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 3 :
     const int i = d.sorted[t];
     const int L0 = g.lim[0], L1 = g.lim[1];
     const int ncap = d.ncap;
-    int* out = nbr_list(d, i);
+    int* out = nbr_row(d, t);
     // penalty contact: other-material partners go to the contact list, ghosts
     // of other materials are only counted (mcm_ll_neighbours.f90:162-174,
     // 209-223; merged into the lists only for kernel contact, :239-267)

@@ -73,9 +73,13 @@ struct Divisor {
         r = fma(y, e, y);
 #endif
     }
+    // The residual a - b q is formed as -(b q - a): as exact as fma(-b, q, a)
+    // and equal to it but for the sign of a zero residual, which leaves a
+    // nonzero quotient unchanged and makes a = -0 give -0, as a / b does for
+    // b > 0 (fma(-b, q, a) would give +0 there).
     MCM_HD double quot(double a) const {
         const double q = a * r;
-        const double rem = fma(-b, q, a);
+        const double rem = -fma(b, q, -a);
         return fma(rem, r, q);
     }
     MCM_HD double div(double a) const {
@@ -147,6 +151,131 @@ MCM_HD void kernel_grad(double xij0, double xij1, double xij2, double havg, doub
     dwdx[0] = __builtin_unpredictable(zero) ? 0.0 : g[0];
     dwdx[1] = __builtin_unpredictable(zero) ? 0.0 : g[1];
     dwdx[2] = __builtin_unpredictable(zero) ? 0.0 : g[2];
+}
+
+// ---------------------------------------------------- pair loops, fast path
+// kernel_grad and kernel_w above test every quotient's operands, and a pair
+// whose test fails (the self pair, rad = 0; a ghost with a zero coordinate
+// difference) sends the wavefront through full divisions.  The fast path
+// below drops the per-pair tests: it is taken for a whole launch when every
+// pair record's coordinates are 0 or of magnitude in [2^-30, 2^30] and every
+// h lies in [2^-30, 2^30] (fast_coord / fast_h, checked where the records
+// are written).  From those ranges, for every pair:
+//   xij in {0} u [2^-82, 2^31] (both coordinates multiples of 2^-82), rad^2 in
+//   {0} u [2^-164, 2^64] (no scaling step in the square root), rad in {0} u
+//   [2^-82, 2^32], pi havg^3 in [2^-89, 2^92], z = rad / havg in {0} u
+//   [2^-112, 2^62], dW/dr in {0} u [2^-205, 2^91], its products with xij in
+//   {0} u [2^-287, 2^122] and rad havg in {0} u [2^-112, 2^62];
+// every quotient then has operands in Divisor's range (or a zero numerator,
+// or rad = 0, whose gradient is replaced by 0), so each result equals the
+// checked path's bit for bit.  W lies in {0} u [2^-250, 2^89] and each
+// gradient component in {0} u [2^-349, 2^234] (used by fast_sums).
+// (1 when x is in range, else 0; int so that several tests combine without
+// branches)
+MCM_HD int fast_coord(double x) {
+    const double a = fabs(x);
+    return (int)(x == 0.0) | ((int)(a >= 0x1p-30) & (int)(a <= 0x1p30));
+}
+MCM_HD int fast_h(double h) { return (int)(h >= 0x1p-30) & (int)(h <= 0x1p30); }
+
+#if defined(__HIP_DEVICE_COMPILE__)
+// sqrt(x) for x = 0 or x >= 2^-767: the device square root (reciprocal
+// square root, then the same Newton steps) without its scaling of tiny
+// arguments, which multiplies by 2^0 in that range
+__device__ __forceinline__ double sqrt_noscale(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y;
+    double h = y * 0.5;
+    const double r = fma(-h, g, 0.5);
+    g = fma(g, r, g);
+    double e = fma(-g, g, x);
+    h = fma(h, r, h);
+    g = fma(e, h, g);
+    e = fma(-g, g, x);
+    g = fma(e, h, g);
+    return x == 0.0 ? x : g;
+}
+
+// W and grad W of one pair (kernel_w and kernel_grad, one evaluation of
+// rad, 1/(pi havg^3) and z), fast path
+__device__ __forceinline__ void pair_wgrad_fast(double xij0, double xij1, double xij2, double havg, double pi,
+                                                double& w, double dwdx[3]) {
+    double rad2 = 0.0;
+    rad2 = rad2 + xij0 * xij0;
+    rad2 = rad2 + xij1 * xij1;
+    rad2 = rad2 + xij2 * xij2;
+    const double rad = sqrt_noscale(rad2);
+    const double const1 = Divisor(pi * havg * havg * havg).quot(1.0);
+    const double const2 = const1 * 0.25;   // const1 / 4.0 in kernel_w: the same value
+    const double z = Divisor(havg).quot(rad);
+    const double t = 2.0 - z;
+    // selects, not branches, as in kernel_w / kernel_dw
+    const double d1 = const1 * ((2.25 * z - 3.0) * z);
+    const double d2 = -(3.0 * const2 * (t * t));
+    const double d12 = __builtin_unpredictable(z < 2.0) ? d2 : 0.0;
+    const double dw = __builtin_unpredictable(z < 1.0) ? d1 : d12;
+    const double w1 = const1 * (((0.75 * z - 1.5) * z) * z + 1.0);
+    const double w2 = const2 * (t * (t * t));
+    const double w12 = __builtin_unpredictable(z < 2.0) ? w2 : 0.0;
+    w = __builtin_unpredictable(z < 1.0) ? w1 : w12;
+    const Divisor rh(rad * havg);
+    const double g0 = rh.quot(dw * xij0), g1 = rh.quot(dw * xij1), g2 = rh.quot(dw * xij2);
+    const bool zero = rad == 0.0;
+    dwdx[0] = __builtin_unpredictable(zero) ? 0.0 : g0;
+    dwdx[1] = __builtin_unpredictable(zero) ? 0.0 : g1;
+    dwdx[2] = __builtin_unpredictable(zero) ? 0.0 : g2;
+}
+
+__device__ __forceinline__ void pair_grad_fast(double xij0, double xij1, double xij2, double havg, double pi,
+                                               double dwdx[3]) {
+    double w;
+    pair_wgrad_fast(xij0, xij1, xij2, havg, pi, w, dwdx);
+}
+#endif
+
+// W and grad W of one pair: the fast path, or the checked functions
+template <bool FAST>
+__device__ __forceinline__ void pair_wgrad(double xij0, double xij1, double xij2, double havg, double pi, double& w,
+                                           double dwdx[3]) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (FAST) {
+        pair_wgrad_fast(xij0, xij1, xij2, havg, pi, w, dwdx);
+        return;
+    }
+#endif
+    double rad;
+    kernel_grad(xij0, xij1, xij2, havg, pi, dwdx, &rad);
+    w = kernel_w(rad, havg, pi);
+}
+
+template <bool FAST>
+__device__ __forceinline__ void pair_grad(double xij0, double xij1, double xij2, double havg, double pi,
+                                          double dwdx[3]) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (FAST) {
+        pair_grad_fast(xij0, xij1, xij2, havg, pi, dwdx);
+        return;
+    }
+#endif
+    double rad;
+    kernel_grad(xij0, xij1, xij2, havg, pi, dwdx, &rad);
+}
+
+// The corrected passes' quotients (grad W sw - W sg) / sw^2 on the fast path:
+// with sw in [2^-100, 2^100] and each sg component 0 or of magnitude in
+// [2^-200, 2^200] (tested once per particle), the two products are 0 or of
+// magnitude >= 2^-450, so the numerator is 0 or in [2^-502, 2^335] (both
+// products are multiples of 2^-502), and sw^2 lies in [2^-200, 2^200]: no
+// operand is below 2^-969, the exponents differ by less than 768 and the
+// quotient is normal, so the division's scaling and fix-up steps change
+// nothing and Divisor::quot gives a / b bit for bit.
+MCM_HD bool fast_sums(double sw, const double sg[3]) {
+    int ok = (int)(sw >= 0x1p-100) & (int)(sw <= 0x1p100);
+    for (int l = 0; l < 3; l++) {
+        const double a = fabs(sg[l]);
+        ok &= (int)(sg[l] == 0.0) | ((int)(a >= 0x1p-200) & (int)(a <= 0x1p200));
+    }
+    return ok != 0;
 }
 
 // 3x3 cofactor inverse, identity when |det| < 1e-8 (mcm_math_func.f90:50-79).

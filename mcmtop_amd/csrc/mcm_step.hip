@@ -87,25 +87,27 @@ __device__ __forceinline__ void load_state(const DevState& d, int i, int cur, PS
 }
 
 // -------------------------------------------------------------- strain
-// Neighbour records: the strain passes read rs[u] = {x, y, z, vx, vy, vz,
-// m/rho, h} (8 doubles, 64 B), whose position is the strain pass's position
-// of u: x - v dt / 2 for a real particle, the unshifted x of a ghost
-// (mcm_strain.f90:27-31; written by the list build), and the momentum passes rm[u] = {x, y, z, h,
-// m, m/rho, A11, A22, A33, A12, A13, A23} (12 doubles, 96 B), one aligned
-// record per neighbour instead of one load per state array.
+// Neighbour records (rs_at, rm_at): the strain passes read rs[u]
+// = {x, y, z, vx, vy, vz, m/rho, h}, whose position is the strain pass's
+// position of u: x - v dt / 2 for a real particle, the unshifted x of a
+// ghost (mcm_strain.f90:27-31; written by the list build), and the momentum
+// passes rm[u] = {x, y, z, h, m, m/rho, A11, A22, A33, A12, A13, A23}.  A
+// pass loads only the planes it uses (the others are dead loads).
 struct SRec { double x, y, z, vx, vy, vz, vol, h; };
 struct MRec { double x, y, z, h, m, vol, A[6]; };
 
 __device__ __forceinline__ SRec load_srec(const double* rs, int j) {
-    const double2* r = reinterpret_cast<const double2*>(rs) + 4 * (long)j;
-    double2 a = r[0], b = r[1], c = r[2], e = r[3];
-    return SRec{a.x, a.y, b.x, b.y, c.x, c.y, e.x, e.y};
+    const double2* r = reinterpret_cast<const double2*>(rs) + RS_SLOTS * (long)j;
+    const double2 a = r[0], b = r[1], c = r[2], e = r[3];
+    return SRec{a.x, a.y, b.x, c.y, e.x, e.y, c.x, b.y};
 }
 __device__ __forceinline__ MRec load_mrec(const double* rm, int j) {
-    const double2* r = reinterpret_cast<const double2*>(rm) + 6 * (long)j;
-    double2 a = r[0], b = r[1], c = r[2], e = r[3], f = r[4], g = r[5];
-    return MRec{a.x, a.y, b.x, b.y, c.x, c.y, {e.x, e.y, f.x, f.y, g.x, g.y}};
+    const double2* r = reinterpret_cast<const double2*>(rm) + RM_SLOTS * (long)j;
+    const double2 a = r[0], b = r[1], c = r[2], e = r[3], f = r[4], h = r[5];
+    return MRec{a.x, a.y, b.x, b.y, c.x, c.y, {e.x, e.y, f.x, f.y, h.x, h.y}};
 }
+__device__ __forceinline__ SRec load_srec(const DevState& d, int j) { return load_srec(d.rs, j); }
+__device__ __forceinline__ MRec load_mrec(const DevState& d, int j) { return load_mrec(d.rm, j); }
 
 template <typename Rec> __device__ __forceinline__ Rec load_rec(const double* base, int j);
 template <> __device__ __forceinline__ SRec load_rec<SRec>(const double* base, int j) { return load_srec(base, j); }
@@ -144,10 +146,11 @@ struct NbrStream {
 
 // plain strain rate, mcm_oldstrain.f90:34-64, positions half a step back
 // for real particles only (mcm_strain.f90:27-31)
-__device__ __forceinline__ void strain_plain(const DevState& d, int i, const double xi[3],
+template <bool FAST>
+__device__ __forceinline__ void strain_plain(const DevState& d, int i, int t, const double xi[3],
                                              const double vi[3], double hi, double g[3][3]) {
     const int n = d.ntot[i];
-    const int* lst = nbr_list(d, i);
+    const int* lst = nbr_row(d, t);
     for (int a = 0; a < 3; a++)
         for (int b = 0; b < 3; b++) g[a][b] = 0.0;
     NbrStream<SRec> st(lst, d.rs, n, i);
@@ -158,8 +161,8 @@ __device__ __forceinline__ void strain_plain(const DevState& d, int i, const dou
         st.next(j, r);
         double xj0 = r.x, xj1 = r.y, xj2 = r.z;
         double havg = 0.5 * (hi + r.h);
-        double dwdx[3], rad;
-        kernel_grad(xi[0] - xj0, xi[1] - xj1, xi[2] - xj2, havg, d.pi, dwdx, &rad);
+        double dwdx[3];
+        pair_grad<FAST>(xi[0] - xj0, xi[1] - xj1, xi[2] - xj2, havg, d.pi, dwdx);
         double dv0 = r.vx - vi[0], dv1 = r.vy - vi[1], dv2 = r.vz - vi[2];
         double volj = r.vol;
         g[0][0] = g[0][0] + volj * dv0 * dwdx[0];
@@ -176,23 +179,37 @@ __device__ __forceinline__ void strain_plain(const DevState& d, int i, const dou
 
 // kernel value and gradient of one pair of the corrected passes, centred at
 // a (xij = a - b), with the neighbour's half-step position
+template <bool FAST>
 __device__ __forceinline__ void corr_pair(const SRec& r, const double xi[3], double hi, double pi,
                                           double xj[3], double& w, double dwdx[3]) {
     xj[0] = r.x;
     xj[1] = r.y;
     xj[2] = r.z;
     double havg = 0.5 * (hi + r.h);
-    double rad;
-    kernel_grad(xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2], havg, pi, dwdx, &rad);
-    w = kernel_w(rad, havg, pi);
+    pair_wgrad<FAST>(xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2], havg, pi, w, dwdx);
+}
+
+// (grad W sw - W sg) / sw^2 of one pair: unchecked quotients when the
+// particle's sums allow them on the fast path (fast_sums), else div3
+__device__ __forceinline__ void corr_grad(const Divisor& isw2, bool unchecked, double sw, const double sg[3],
+                                          double w, const double dwdx[3], double cg[3]) {
+    const double n0 = dwdx[0] * sw - w * sg[0], n1 = dwdx[1] * sw - w * sg[1], n2 = dwdx[2] * sw - w * sg[2];
+    if (unchecked) {
+        cg[0] = isw2.quot(n0);
+        cg[1] = isw2.quot(n1);
+        cg[2] = isw2.quot(n2);
+    } else {
+        isw2.div3(n0, n1, n2, cg);
+    }
 }
 
 // corrected strain rate, mcm_corr_strain.f90:37-146, in the reference's three
 // passes and evaluation order
-__device__ __forceinline__ void strain_corr(const DevState& d, int i, const double xi[3], double hi,
+template <bool FAST>
+__device__ __forceinline__ void strain_corr(const DevState& d, int i, int t, const double xi[3], double hi,
                                             double f[3][3]) {
     const int n = d.ntot[i];
-    const int* lst = nbr_list(d, i);
+    const int* lst = nbr_row(d, t);
     double sw = 0.0, sg[3] = {0.0, 0.0, 0.0};
     NbrStream<SRec> s1(lst, d.rs, n, i);
 #pragma unroll 2
@@ -201,12 +218,13 @@ __device__ __forceinline__ void strain_corr(const DevState& d, int i, const doub
         SRec r;
         s1.next(j, r);
         double xj[3], w, dwdx[3];
-        corr_pair(r, xi, hi, d.pi, xj, w, dwdx);
+        corr_pair<FAST>(r, xi, hi, d.pi, xj, w, dwdx);
         sw = sw + r.vol * w;
         for (int l = 0; l < 3; l++) sg[l] = sg[l] + r.vol * dwdx[l];
     }
     const double sw2 = sw * sw;
     const Divisor isw2(sw2);
+    const bool unchecked = FAST && __all(fast_sums(sw, sg));
     double B[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, invB[9];
     NbrStream<SRec> s2(lst, d.rs, n, i);
 #pragma unroll 2
@@ -215,8 +233,8 @@ __device__ __forceinline__ void strain_corr(const DevState& d, int i, const doub
         SRec r;
         s2.next(j, r);
         double xj[3], w, dwdx[3], cg[3];
-        corr_pair(r, xi, hi, d.pi, xj, w, dwdx);
-        isw2.div3(dwdx[0] * sw - w * sg[0], dwdx[1] * sw - w * sg[1], dwdx[2] * sw - w * sg[2], cg);
+        corr_pair<FAST>(r, xi, hi, d.pi, xj, w, dwdx);
+        corr_grad(isw2, unchecked, sw, sg, w, dwdx, cg);
         for (int m = 0; m < 3; m++)
             for (int nn = 0; nn < 3; nn++) B[nn + 3 * m] = B[nn + 3 * m] + r.vol * (xj[nn] - xi[nn]) * cg[m];
     }
@@ -229,8 +247,8 @@ __device__ __forceinline__ void strain_corr(const DevState& d, int i, const doub
         SRec r;
         s3.next(j, r);
         double xj[3], w, dwdx[3], cg[3], g[3];
-        corr_pair(r, xi, hi, d.pi, xj, w, dwdx);
-        isw2.div3(dwdx[0] * sw - w * sg[0], dwdx[1] * sw - w * sg[1], dwdx[2] * sw - w * sg[2], cg);
+        corr_pair<FAST>(r, xi, hi, d.pi, xj, w, dwdx);
+        corr_grad(isw2, unchecked, sw, sg, w, dwdx, cg);
         for (int q = 0; q < 3; q++) g[q] = invB[q] * cg[0] + invB[q + 3] * cg[1] + invB[q + 6] * cg[2];
         double vj[3] = {r.vx, r.vy, r.vz};
         for (int nn = 0; nn < 3; nn++)
@@ -241,10 +259,11 @@ __device__ __forceinline__ void strain_corr(const DevState& d, int i, const doub
 // corrected strain rate in one pass: the correction is linear in the pass-1
 // sums, so B and the contracted gradient follow from single-pass moments
 // (same result up to rounding; the fast variant)
-__device__ __forceinline__ void strain_corr_fast(const DevState& d, int i, const double xi[3],
+template <bool FAST>
+__device__ __forceinline__ void strain_corr_fast(const DevState& d, int i, int t, const double xi[3],
                                                  double hi, double f[3][3]) {
     const int n = d.ntot[i];
-    const int* lst = nbr_list(d, i);
+    const int* lst = nbr_row(d, t);
     double sw = 0.0, sg[3] = {0, 0, 0}, P[3][3] = {{0}}, Pw[3] = {0, 0, 0}, Q[3][3] = {{0}}, Qw[3] = {0, 0, 0};
     NbrStream<SRec> st(lst, d.rs, n, i);
     for (int k = 0; k < n; k++) {
@@ -252,7 +271,7 @@ __device__ __forceinline__ void strain_corr_fast(const DevState& d, int i, const
         SRec r;
         st.next(j, r);
         double xj[3], w, dwdx[3];
-        corr_pair(r, xi, hi, d.pi, xj, w, dwdx);
+        corr_pair<FAST>(r, xi, hi, d.pi, xj, w, dwdx);
         double vw = r.vol * w;
         double vj[3] = {r.vx, r.vy, r.vz};
         sw += vw;
@@ -291,8 +310,8 @@ __device__ __forceinline__ double strain_finish(const DevState& d, int i, double
         x1[a] = xi[a] + 0.5 * vi[a] * dt;
         d.x1[a][i] = x1[a];
     }
-    double2* rm = reinterpret_cast<double2*>(d.rm) + 6 * (long)i;
-    rm[0] = make_double2(x1[0], x1[1]);
+    *rm_at(d, 0, i) = make_double2(x1[0], x1[1]);
+    if (!(fast_coord(x1[0]) & fast_coord(x1[1]) & fast_coord(x1[2]))) atomicOr(d.range, (int)RANGE_RM);
     return x1[2];
 }
 
@@ -339,12 +358,12 @@ __device__ __forceinline__ void constitutive_particle(const DevState& d, int i, 
     // momentum record: x after the strain pass, new h, mass, volume and
     // (sigma - q)/rho^2 (mcm_oldacceleration.f90:79-80, per particle)
     double r2 = s.rho * s.rho;
-    double2* rm = reinterpret_cast<double2*>(d.rm) + 6 * (long)i;
-    rm[1] = make_double2(x1z, s.h);
-    rm[2] = make_double2(s.mass, s.mass / s.rho);
-    rm[3] = make_double2((s.sig[S11] - s.q) / r2, (s.sig[S22] - s.q) / r2);
-    rm[4] = make_double2((s.sig[S33] - s.q) / r2, (s.sig[S12] - 0.0) / r2);
-    rm[5] = make_double2((s.sig[S13] - 0.0) / r2, (s.sig[S23] - 0.0) / r2);
+    *rm_at(d, 1, i) = make_double2(x1z, s.h);
+    if (!fast_h(s.h)) atomicOr(d.range, (int)RANGE_RM);
+    *rm_at(d, 2, i) = make_double2(s.mass, s.mass / s.rho);
+    *rm_at(d, 3, i) = make_double2((s.sig[S11] - s.q) / r2, (s.sig[S22] - s.q) / r2);
+    *rm_at(d, 4, i) = make_double2((s.sig[S33] - s.q) / r2, (s.sig[S12] - 0.0) / r2);
+    *rm_at(d, 5, i) = make_double2((s.sig[S13] - 0.0) / r2, (s.sig[S23] - 0.0) / r2);
     if (DIAG) {
         for (int k = 0; k < 6; k++) d.rod[k][i] = s.rod[k];
         for (int k = 0; k < 3; k++) d.spin[k][i] = s.spin[k];
@@ -357,15 +376,77 @@ __device__ __forceinline__ void constitutive_particle(const DevState& d, int i, 
     }
 }
 
-// Block min of critts and max of the updated h into the block's partials
-// (folded by k_strain_fold).  Every thread of the block calls it.
+// energy and coordinate-bound partials: [cmin 3][cmax 3][ke nm][ie nm][mom 3nm]
+#define RED_C (6 + 5 * MCM_MAX_MAT)
+
+// One thread's share of the step's end-of-step reductions: its particle's
+// new position (coordinate bounds) and its terms of mcm_calctotale.f90:33-51
+// for its material mi (mi < 0: no particle).
+struct CPart {
+    double x[3];
+    double ke, ie, mom[3];
+    int mi;
+};
+
+__device__ __forceinline__ void init_c_part(CPart& c) {
+    for (int a = 0; a < 3; a++) { c.x[a] = 0.0; c.mom[a] = 0.0; }
+    c.ke = 0.0; c.ie = 0.0; c.mi = -1;
+}
+
+__device__ __forceinline__ void energy_terms(CPart& c, int mi, double mass, double vabs, double e, const double v[3]) {
+    c.mi = mi;
+    c.ke = 0.5 * mass * (vabs * vabs);
+    c.ie = e;
+    for (int a = 0; a < 3; a++) c.mom[a] = mass * fabs(v[a]);
+}
+
+// Block partials of every block (folded by k_c_fold): the coordinate
+// bounds [cmin 3][cmax 3] (sentinels without bounds) ...
+__device__ void reduce_bound_partials(const DevState& d, const CPart& c, bool with_bounds, int lb) {
+    double* out = d.red + (long)lb * RED_C;
+    const bool has = c.mi >= 0;
+    for (int a = 0; a < 3; a++) {
+        double lo = block_reduce<1>(has && with_bounds ? c.x[a] : 1.0e20);
+        double hi = block_reduce<2>(has && with_bounds ? c.x[a] : -1.0e20);
+        if (threadIdx.x == 0) { out[a] = lo; out[3 + a] = hi; }
+    }
+}
+
+// ... and the energies [ke nm][ie nm][mom 3nm].  The fused step takes them
+// in the constitutive pass, in particle order (their terms use e after the
+// update and v, |v| before the momentum pass), so that the sums do not
+// depend on the momentum pass's particle order.  Every thread of the block
+// calls it.
+__device__ void reduce_energy_partials(const DevState& d, const CPart& c, int lb) {
+    const int nm = d.nummat;
+    double* out = d.red + (long)lb * RED_C;
+    for (int m = 0; m < nm; m++) {
+        const bool mine = c.mi == m;
+        double ke = block_reduce<0>(mine ? c.ke : 0.0);
+        double ie = block_reduce<0>(mine ? c.ie : 0.0);
+        if (threadIdx.x == 0) { out[6 + m] = ke; out[6 + nm + m] = ie; }
+        for (int a = 0; a < 3; a++) {
+            double mo = block_reduce<0>(mine ? c.mom[a] : 0.0);
+            if (threadIdx.x == 0) out[6 + 2 * nm + 3 * m + a] = mo;
+        }
+    }
+}
+
+__device__ void reduce_c_partials(const DevState& d, const CPart& c, bool with_bounds, int lb) {
+    reduce_bound_partials(d, c, with_bounds, lb);
+    reduce_energy_partials(d, c, lb);
+}
+
+// Block min of critts and max of the updated h into the first two slots of
+// the block's partials (folded by k_strain_fold, before the momentum pass
+// writes its coordinate bounds there).  Every thread of the block calls it.
 __device__ __forceinline__ void strain_block_partials(const DevState& d, int lb, double crit, double hm, int eos_err) {
     if (eos_err) atomicOr(&d.sc->error_flags, 16);
     double cb = block_reduce<1>(crit);
     double hb = block_reduce<2>(hm);
     if (threadIdx.x == 0) {
-        d.red[2 * lb] = cb;
-        d.red[2 * lb + 1] = hb;
+        d.red[(long)lb * RED_C] = cb;
+        d.red[(long)lb * RED_C + 1] = hb;
     }
 }
 
@@ -377,8 +458,8 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_strain_fold(DevState d_) {
     const DevState& d = enter<false>(d_, bk);
     const int nblk = d.nblk;
     Scalars* sc = d.sc;
-    double c1 = fold<1>(d.red, 2, 0, nblk, 1.0e20);
-    double h1 = fold<2>(d.red, 2, 1, nblk, 0.0);
+    double c1 = fold<1>(d.red, RED_C, 0, nblk, 1.0e20);
+    double h1 = fold<2>(d.red, RED_C, 1, nblk, 0.0);
     if (threadIdx.x == 0 && !sc->abort) {
         sc->critts = c1;
         if ((MODE == MODE_FUSED || MODE == MODE_FCONST) && d.h_opt > 0) sc->hmax = h1;
@@ -394,17 +475,42 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_strain_fold(DevState d_) {
 // Thread-per-particle strain (+ constitutive) kernel.  CORR: 0 plain SPH,
 // 1 corrected (reference order), 2 corrected (one pass).
 template <int MODE, int CORR, bool DIAG>
-__global__ __launch_bounds__(MCM_BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void k_strain_const(DevState d_, int cur) {
+// Waves per SIMD of the pair kernels (A/B-timed on the 2.1 M-particle bar).
+// The gathers are bound by the vector L1 path, and fewer resident waves
+// evict fewer of each other's neighbour lines: the one-pass corrected
+// kernels (28 and 40 moment sums) run fastest at 2 and 1 waves, the
+// three-pass ones at 3 and 2.
+#ifndef MCM_WPE_STRAIN1
+#define MCM_WPE_STRAIN1 2
+#endif
+#ifndef MCM_WPE_MOM1
+#define MCM_WPE_MOM1 1
+#endif
+#ifndef MCM_WPE_STRAIN3
+#define MCM_WPE_STRAIN3 3
+#endif
+#ifndef MCM_WPE_MOM3
+#define MCM_WPE_MOM3 2
+#endif
+__global__ __launch_bounds__(MCM_BLOCK) __attribute__((amdgpu_waves_per_eu(
+    MODE == MODE_FSTRAIN ? (CORR == 2 ? MCM_WPE_STRAIN1 : MCM_WPE_STRAIN3) : 3))) void k_strain_const(DevState d_,
+                                                                                                       int cur) {
     Blk bk;
     const DevState& d = enter<true>(d_, bk);
     const int lb = bk.b;
-    const int i = lb * MCM_BLOCK + threadIdx.x;
     const int nxt = cur ^ 1;
     Scalars* sc = d.sc;
-    const bool live = i < d.np && !sc->abort;
+    // the strain pass takes the particles in the search's sorted order (the
+    // order of the lists, nbr_row); the constitutive pass in id order
+    constexpr bool pairs = MODE == MODE_FUSED || MODE == MODE_STRAIN || MODE == MODE_FSTRAIN;
+    const int t = lb * MCM_BLOCK + threadIdx.x;
+    const bool live = t < d.np && !sc->abort;
+    const int i = pairs && live ? d.sorted[t] : t;
     const double dt = sc->dt;
     double crit = 1.0e20, hm = 0.0;
     int eos_err = 0;
+    CPart ce;   // energy terms (MODE_FCONST)
+    init_c_part(ce);
     if (live) {
         PState s;
         double x1z = 0.0;
@@ -417,14 +523,23 @@ __global__ __launch_bounds__(MCM_BLOCK) __attribute__((amdgpu_waves_per_eu(3))) 
             }
             double hi = d.hbuf[cur][i];
             double g[3][3];
-            if (CORR == 1) strain_corr(d, i, xi, hi, g);
-            else if (CORR == 2) strain_corr_fast(d, i, xi, hi, g);
-            else strain_plain(d, i, xi, vi, hi, g);
+            // records within the fast-path ranges (flag written by the search)
+            const bool fast = (d.range[0] & RANGE_RS) == 0;
+            if (CORR == 1) {
+                if (fast) strain_corr<true>(d, i, t, xi, hi, g);
+                else strain_corr<false>(d, i, t, xi, hi, g);
+            } else if (CORR == 2) {
+                if (fast) strain_corr_fast<true>(d, i, t, xi, hi, g);
+                else strain_corr_fast<false>(d, i, t, xi, hi, g);
+            } else {
+                if (fast) strain_plain<true>(d, i, t, xi, vi, hi, g);
+                else strain_plain<false>(d, i, t, xi, vi, hi, g);
+            }
             rod_spin(g, s);
             x1z = strain_finish(d, i, dt, xi, vi);
             if (MODE == MODE_STRAIN) {
-                double2* rm = reinterpret_cast<double2*>(d.rm) + 6 * (long)i;
-                rm[1] = make_double2(x1z, hi);
+                *rm_at(d, 1, i) = make_double2(x1z, hi);
+                if (!fast_h(hi)) atomicOr(d.range, (int)RANGE_RM);
                 for (int k = 0; k < 6; k++) d.rod[k][i] = s.rod[k];
                 for (int k = 0; k < 3; k++) d.spin[k][i] = s.spin[k];
                 d.tracerod[i] = s.tracerod;
@@ -444,9 +559,14 @@ __global__ __launch_bounds__(MCM_BLOCK) __attribute__((amdgpu_waves_per_eu(3))) 
         }
         if (MODE != MODE_STRAIN && MODE != MODE_FSTRAIN)
             constitutive_particle<MODE, DIAG>(d, i, cur, dt, s, x1z, crit, hm, eos_err);
+        if (MODE == MODE_FCONST) {   // mcm_calctotale.f90:33-51 of the fused step
+            const double v[3] = {d.v[0][i], d.v[1][i], d.v[2][i]};
+            energy_terms(ce, d.pmat[i], s.mass, s.vabs, s.e, v);
+        }
     }
     if (MODE == MODE_STRAIN || MODE == MODE_FSTRAIN) return;
     strain_block_partials(d, lb, crit, hm, eos_err);
+    if (MODE == MODE_FCONST) reduce_energy_partials(d, ce, lb);
 }
 
 // mcm_update_h.f90:25-54 as a stage of its own (granular path)
@@ -496,10 +616,9 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_ghost_stress(DevState d_, int cur
         double rg = d.rbuf[nxt][u];
         double r2 = rg * rg;
         double q = d.q[p];
-        double2* rm = reinterpret_cast<double2*>(d.rm) + 6 * (long)u;
-        rm[3] = make_double2((d.sig[S11][p] - q) / r2, (d.sig[S22][p] - q) / r2);
-        rm[4] = make_double2((d.sig[S33][p] - q) / r2, -d.sig[S12][p] / r2);
-        rm[5] = make_double2(-d.sig[S13][p] / r2, -d.sig[S23][p] / r2);
+        *rm_at(d, 3, u) = make_double2((d.sig[S11][p] - q) / r2, (d.sig[S22][p] - q) / r2);
+        *rm_at(d, 4, u) = make_double2((d.sig[S33][p] - q) / r2, -d.sig[S12][p] / r2);
+        *rm_at(d, 5, u) = make_double2(-d.sig[S13][p] / r2, -d.sig[S23][p] / r2);
     }
 }
 
@@ -515,9 +634,10 @@ __device__ __forceinline__ double accel_start(const DevState& d, int i, int k) {
     return d.contacttype > 0 ? 0.0 + d.rep[k][i] : 0.0;
 }
 
-__device__ __forceinline__ void accel_plain(const DevState& d, int i, const MRec& me, double acc[3]) {
+template <bool FAST>
+__device__ __forceinline__ void accel_plain(const DevState& d, int i, int t, const MRec& me, double acc[3]) {
     const int n = d.ntot[i];
-    const int* lst = nbr_list(d, i);
+    const int* lst = nbr_row(d, t);
     acc[0] = accel_start(d, i, 0); acc[1] = accel_start(d, i, 1); acc[2] = accel_start(d, i, 2);
     NbrStream<MRec> st(lst, d.rm, n, i);
 #pragma unroll 2
@@ -526,8 +646,8 @@ __device__ __forceinline__ void accel_plain(const DevState& d, int i, const MRec
         MRec r;
         st.next(j, r);
         double havg = 0.5 * (me.h + r.h);
-        double dwdx[3], rad;
-        kernel_grad(me.x - r.x, me.y - r.y, me.z - r.z, havg, d.pi, dwdx, &rad);
+        double dwdx[3];
+        pair_grad<FAST>(me.x - r.x, me.y - r.y, me.z - r.z, havg, d.pi, dwdx);
         for (int nn = 0; nn < 3; nn++) {
             double ds = 0.0;
             for (int mm = 0; mm < 3; mm++) {
@@ -541,20 +661,20 @@ __device__ __forceinline__ void accel_plain(const DevState& d, int i, const MRec
 
 // kernel value and gradient centred at the neighbour: W(x_j, x_i),
 // grad_j W (mcm_corr_acceleration.f90:73-74)
+template <bool FAST>
 __device__ __forceinline__ void corr_pair_j(const MRec& r, const MRec& me, double pi, double& w, double dwdx[3]) {
     double havg = 0.5 * (me.h + r.h);
-    double rad;
-    kernel_grad(r.x - me.x, r.y - me.y, r.z - me.z, havg, pi, dwdx, &rad);
-    w = kernel_w(rad, havg, pi);
+    pair_wgrad<FAST>(r.x - me.x, r.y - me.y, r.z - me.z, havg, pi, w, dwdx);
 }
 
 // mcm_corr_acceleration.f90:37-163 in the reference's three passes: the self
 // slot (k == nnbr) contributes nothing to deltasig, the mass of the last
 // neighbour multiplies the sum and B^-1 enters transposed
-__device__ __forceinline__ void accel_corr(const DevState& d, int i, const MRec& me, double acc[3]) {
+template <bool FAST>
+__device__ __forceinline__ void accel_corr(const DevState& d, int i, int t, const MRec& me, double acc[3]) {
     const int n = d.ntot[i];
     const int self_k = d.nreal[i] - 1;
-    const int* lst = nbr_list(d, i);
+    const int* lst = nbr_row(d, t);
     double sw = 0.0, sg[3] = {0.0, 0.0, 0.0}, massj = 0.0;
     int j;
     NbrStream<MRec> s1(lst, d.rm, n, i);
@@ -563,13 +683,14 @@ __device__ __forceinline__ void accel_corr(const DevState& d, int i, const MRec&
         MRec r;
         s1.next(j, r);
         double w, dwdx[3];
-        corr_pair_j(r, me, d.pi, w, dwdx);
+        corr_pair_j<FAST>(r, me, d.pi, w, dwdx);
         sw = sw + r.vol * w;
         for (int l = 0; l < 3; l++) sg[l] = sg[l] + r.vol * dwdx[l];
         massj = r.m;
     }
     const double sw2 = sw * sw;
     const Divisor isw2(sw2);
+    const bool unchecked = FAST && __all(fast_sums(sw, sg));
     double B[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, invB[9];
     NbrStream<MRec> s2(lst, d.rm, n, i);
 #pragma unroll 2
@@ -577,8 +698,8 @@ __device__ __forceinline__ void accel_corr(const DevState& d, int i, const MRec&
         MRec r;
         s2.next(j, r);
         double w, dwdx[3], cg[3];
-        corr_pair_j(r, me, d.pi, w, dwdx);
-        isw2.div3(dwdx[0] * sw - w * sg[0], dwdx[1] * sw - w * sg[1], dwdx[2] * sw - w * sg[2], cg);
+        corr_pair_j<FAST>(r, me, d.pi, w, dwdx);
+        corr_grad(isw2, unchecked, sw, sg, w, dwdx, cg);
         double dx[3] = {r.x - me.x, r.y - me.y, r.z - me.z};
         for (int m = 0; m < 3; m++)
             for (int nn = 0; nn < 3; nn++) B[nn + 3 * m] = B[nn + 3 * m] + r.vol * dx[nn] * cg[m];
@@ -593,8 +714,8 @@ __device__ __forceinline__ void accel_corr(const DevState& d, int i, const MRec&
         s3.next(j, r);
         if (k == self_k) continue;   // dsig = 0 there
         double w, dwdx[3], cg[3];
-        corr_pair_j(r, me, d.pi, w, dwdx);
-        isw2.div3(dwdx[0] * sw - w * sg[0], dwdx[1] * sw - w * sg[1], dwdx[2] * sw - w * sg[2], cg);
+        corr_pair_j<FAST>(r, me, d.pi, w, dwdx);
+        corr_grad(isw2, unchecked, sw, sg, w, dwdx, cg);
         for (int c = 0; c < 6; c++) {
             double dsig = r.A[c] + me.A[c];
             D[c][0] = D[c][0] + dsig * cg[0];
@@ -613,10 +734,11 @@ __device__ __forceinline__ void accel_corr(const DevState& d, int i, const MRec&
 }
 
 // corrected acceleration in one pass from single-pass moments (fast variant)
-__device__ __forceinline__ void accel_corr_fast(const DevState& d, int i, const MRec& me, double acc[3]) {
+template <bool FAST>
+__device__ __forceinline__ void accel_corr_fast(const DevState& d, int i, int t, const MRec& me, double acc[3]) {
     const int n = d.ntot[i];
     const int self_k = d.nreal[i] - 1;
-    const int* lst = nbr_list(d, i);
+    const int* lst = nbr_row(d, t);
     double sw = 0.0, sg[3] = {0, 0, 0}, P[3][3] = {{0}}, Pw[3] = {0, 0, 0};
     double E1[6][3] = {{0}}, E2[6] = {0, 0, 0, 0, 0, 0};
     double mlast = 0.0;
@@ -626,7 +748,7 @@ __device__ __forceinline__ void accel_corr_fast(const DevState& d, int i, const 
         MRec r;
         st.next(j, r);
         double w, dwdx[3];
-        corr_pair_j(r, me, d.pi, w, dwdx);
+        corr_pair_j<FAST>(r, me, d.pi, w, dwdx);
         double dx[3] = {r.x - me.x, r.y - me.y, r.z - me.z};
         double vw = r.vol * w;
         sw += vw;
@@ -689,53 +811,6 @@ __device__ __forceinline__ void sym_pen(const DevState& d, double x[3], double v
     }
 }
 
-// energy and coordinate-bound partials: [cmin 3][cmax 3][ke nm][ie nm][mom 3nm]
-#define RED_C (6 + 5 * MCM_MAX_MAT)
-
-// One thread's share of the step's end-of-step reductions: its particle's
-// new position (coordinate bounds) and its terms of mcm_calctotale.f90:33-51
-// for its material mi (mi < 0: no particle).
-struct CPart {
-    double x[3];
-    double ke, ie, mom[3];
-    int mi;
-};
-
-__device__ __forceinline__ void init_c_part(CPart& c) {
-    for (int a = 0; a < 3; a++) { c.x[a] = 0.0; c.mom[a] = 0.0; }
-    c.ke = 0.0; c.ie = 0.0; c.mi = -1;
-}
-
-__device__ __forceinline__ void energy_terms(CPart& c, int mi, double mass, double vabs, double e, const double v[3]) {
-    c.mi = mi;
-    c.ke = 0.5 * mass * (vabs * vabs);
-    c.ie = e;
-    for (int a = 0; a < 3; a++) c.mom[a] = mass * fabs(v[a]);
-}
-
-// Block partials [cmin 3][cmax 3][ke nm][ie nm][mom 3nm] of every block
-// (folded by k_c_fold).  Every thread of the block calls it.
-__device__ void reduce_c_partials(const DevState& d, const CPart& c, bool with_bounds, int lb) {
-    const int nm = d.nummat;
-    double* out = d.red + (long)lb * RED_C;
-    const bool has = c.mi >= 0;
-    for (int a = 0; a < 3; a++) {
-        double lo = block_reduce<1>(has && with_bounds ? c.x[a] : 1.0e20);
-        double hi = block_reduce<2>(has && with_bounds ? c.x[a] : -1.0e20);
-        if (threadIdx.x == 0) { out[a] = lo; out[3 + a] = hi; }
-    }
-    for (int m = 0; m < nm; m++) {
-        const bool mine = c.mi == m;
-        double ke = block_reduce<0>(mine ? c.ke : 0.0);
-        double ie = block_reduce<0>(mine ? c.ie : 0.0);
-        if (threadIdx.x == 0) { out[6 + m] = ke; out[6 + nm + m] = ie; }
-        for (int a = 0; a < 3; a++) {
-            double mo = block_reduce<0>(mine ? c.mom[a] : 0.0);
-            if (threadIdx.x == 0) out[6 + 2 * nm + 3 * m + a] = mo;
-        }
-    }
-}
-
 // Coordinate bounds (with_bounds: the step's move), per-material energies
 // and their totals from the partials of nblk blocks.  One workgroup; every
 // thread folds all components of its blocks in one sweep (block order per
@@ -792,12 +867,8 @@ __device__ __forceinline__ void integrate_particle(const DevState& d, int i, dou
     if (d.baseaccel)   // mcm_boundcond.f90:23-29
         for (int a = 0; a < 3; a++) acc[a] = acc[a] + d.base_a[a];
     boundcond(d.dispbc[i], acc);
-    const int mi = d.pmat[i];
-    const double mass = d.mass[i];
+    cp.mi = d.pmat[i];   // (its energy terms: the constitutive pass)
     double v[3] = {d.v[0][i], d.v[1][i], d.v[2][i]};
-    double vabs = d.vabs[i];
-    // mcm_calctotale.f90:33-51
-    energy_terms(cp, mi, mass, vabs, d.e[i], v);
     // mcm_update_velocity.f90:29-56
     double dtn = 0.5 * (sc->dt + sc->dtold);
     double va = 0.0;
@@ -829,29 +900,42 @@ __device__ __forceinline__ void integrate_particle(const DevState& d, int i, dou
 
 // Thread-per-particle momentum kernel (CORR as k_strain_const).
 template <int MODE, int CORR, bool DIAG>
-__global__ __launch_bounds__(MCM_BLOCK) void k_momentum(DevState d_, int cur) {
+__global__ __launch_bounds__(MCM_BLOCK) __attribute__((amdgpu_waves_per_eu(CORR == 2 ? MCM_WPE_MOM1 : MCM_WPE_MOM3))) void k_momentum(
+    DevState d_, int cur) {
     Blk bk;
     const DevState& d = enter<true>(d_, bk);
     const int lb = bk.b;
-    const int i = lb * MCM_BLOCK + threadIdx.x;
     Scalars* sc = d.sc;
-    const bool live = i < d.np && !sc->abort;
+    // particles in the search's sorted order (the order of the lists)
+    const int t = lb * MCM_BLOCK + threadIdx.x;
+    const bool live = t < d.np && !sc->abort;
+    const int i = live ? d.sorted[t] : t;
     CPart cp;
     init_c_part(cp);
     if (live) {
-        const MRec me = load_mrec(d.rm, i);
+        const MRec me = load_mrec(d, i);
         const double xi[3] = {me.x, me.y, me.z};
         double acc[3];
-        if (CORR == 1) accel_corr(d, i, me, acc);
-        else if (CORR == 2) accel_corr_fast(d, i, me, acc);
-        else accel_plain(d, i, me, acc);
+        // records within the fast-path ranges (flag written by the search,
+        // the strain pass and the constitutive pass)
+        const bool fast = (d.range[0] & RANGE_RM) == 0;
+        if (CORR == 1) {
+            if (fast) accel_corr<true>(d, i, t, me, acc);
+            else accel_corr<false>(d, i, t, me, acc);
+        } else if (CORR == 2) {
+            if (fast) accel_corr_fast<true>(d, i, t, me, acc);
+            else accel_corr_fast<false>(d, i, t, me, acc);
+        } else {
+            if (fast) accel_plain<true>(d, i, t, me, acc);
+            else accel_plain<false>(d, i, t, me, acc);
+        }
         if (MODE == MODE_MOMENTUM) {
             for (int a = 0; a < 3; a++) d.a[a][i] = acc[a];
         } else {
             integrate_particle<DIAG, MODE != MODE_FVEL>(d, i, acc, xi, cp);
         }
     }
-    if (MODE == MODE_FUSED || MODE == MODE_FVEL) reduce_c_partials(d, cp, MODE == MODE_FUSED, lb);
+    if (MODE == MODE_FUSED) reduce_bound_partials(d, cp, true, lb);   // with XSPH: k_xsph_move
 }
 
 // ------------------------------------------------------ optional pair passes
@@ -921,7 +1005,7 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_xsph(DevState d_, int cur) {
     const double rhoi = d.rbuf[nxt][i];
     double s[3] = {0.0, 0.0, 0.0};
     const int n = d.ntot[i];
-    const int* lst = nbr_list(d, i);
+    const int* lst = nbr_row(d, d.spos[i]);
     for (int k = 0; k < n; k++) {
         const int u = lst[(long)k * 64];
         double rad = 0.0;

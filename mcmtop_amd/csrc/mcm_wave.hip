@@ -113,7 +113,7 @@ __global__ __launch_bounds__(256) void k_list_wave(DevState d_, int cur) {
     for (int p = 0; p < 64; p++) {
         const int i = wbase + p;
         if (i >= np) break;
-        int* out = nbr_list(d, i);
+        int* out = nbr_row(d, d.spos[i]);
         int cnt = 0;
         double mind = 1.0e20, mind_cp = 1.0e20;
         const int c = d.cell_of[i];
@@ -256,7 +256,7 @@ struct SPair {
 
 __device__ __forceinline__ void strain_pair(const DevState& d, int j, const double xi[3], double hi,
                                             SPair& q) {
-    SRec r = load_srec(d.rs, j);   // position: the strain pass's (half step back for real j)
+    SRec r = load_srec(d, j);   // position: the strain pass's (half step back for real j)
     q.xj[0] = r.x;
     q.xj[1] = r.y;
     q.xj[2] = r.z;
@@ -274,7 +274,7 @@ template <int CORR>
 __device__ __forceinline__ void strain_wave(const DevState& d, double* buf, int i, int n,
                                             const double xi[3], const double vi[3], double hi, double (&gout)[9]) {
     const int lane = threadIdx.x & 63;
-    const int* row = nbr_list(d, i);
+    const int* row = nbr_row(d, d.spos[i]);
     const bool two = n > 64;   // second list slot in use (uniform)
     SPair q[WAVE_SLOTS];
 #pragma unroll
@@ -414,7 +414,7 @@ template <int CORR>
 __device__ __forceinline__ void accel_wave(const DevState& d, double* buf, int i, int n, int self_k, const MRec& me,
                                            double (&acc)[3]) {
     const int lane = threadIdx.x & 63;
-    const int* row = nbr_list(d, i);
+    const int* row = nbr_row(d, d.spos[i]);
     const bool two = n > 64;   // second list slot in use (uniform)
     MPair q[WAVE_SLOTS];
 #pragma unroll
@@ -422,7 +422,7 @@ __device__ __forceinline__ void accel_wave(const DevState& d, double* buf, int i
         if (s > 0 && !two) break;
         const int k = lane + 64 * s;
         const int j = k < n ? row[(long)k * 64] : i;
-        MRec r = load_mrec(d.rm, j);
+        MRec r = load_mrec(d, j);
         const double havg = 0.5 * (me.h + r.h);
         double rad;
         if (CORR == 0) {
@@ -550,7 +550,7 @@ __global__ __launch_bounds__(256) void k_momentum_wave(DevState d_, int cur) {
             if (i >= d.np) break;
             const int n = d.ntot[i];
             const int self_k = d.nreal[i] - 1;
-            const MRec me = load_mrec(d.rm, i);
+            const MRec me = load_mrec(d, i);
             double acc[3];
             accel_wave<CORR>(d, buf, i, n, self_k, me, acc);
             keep_if_lane<3>(p, acc, mine);
@@ -560,10 +560,10 @@ __global__ __launch_bounds__(256) void k_momentum_wave(DevState d_, int cur) {
     init_c_part(cp);
     const int i = wbase + lane;
     if (!abort && i < d.np) {
-        const MRec me = load_mrec(d.rm, i);
+        const MRec me = load_mrec(d, i);
         const double xi[3] = {me.x, me.y, me.z};
         double acc[3] = {mine[0], mine[1], mine[2]};
         integrate_particle<false, true>(d, i, acc, xi, cp);
     }
-    reduce_c_partials(d, cp, true, lb);
+    reduce_bound_partials(d, cp, true, lb);   // energies: the constitutive pass
 }

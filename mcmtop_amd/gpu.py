@@ -31,7 +31,7 @@ SYMBOLS = (
     "mcm_gpu_dt_update", "mcm_gpu_momentum", "mcm_gpu_boundcond", "mcm_gpu_calctotale",
     "mcm_gpu_update_velocity", "mcm_gpu_move", "mcm_gpu_calccritts", "mcm_gpu_step",
     "mcm_gpu_get_scalars", "mcm_gpu_download", "mcm_gpu_download_neighbours", "mcm_gpu_dump",
-    "mcm_gpu_time_steps", "mcm_gpu_convert_deck", "mcm_gpu_create_from_memory",
+    "mcm_gpu_time_steps", "mcm_gpu_time_kernels", "mcm_gpu_convert_deck", "mcm_gpu_create_from_memory",
     "mcm_gpu_batch_create", "mcm_gpu_batch_add_deck", "mcm_gpu_batch_add_memory", "mcm_gpu_batch_size",
     "mcm_gpu_batch_run", "mcm_gpu_batch_last_error", "mcm_gpu_batch_initial", "mcm_gpu_batch_step",
     "mcm_gpu_batch_time_steps", "mcm_gpu_batch_destroy", "mcm_gpu_output_fields", "mcm_vtk_write",
@@ -97,6 +97,7 @@ def load_library(path: str = LIB_PATH):
     lib.mcm_gpu_download_neighbours.argtypes = [vp, i32p, i32p, i32p, ctypes.c_int64, i32p]
     lib.mcm_gpu_dump.argtypes = [vp, ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32]
     lib.mcm_gpu_time_steps.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
+    lib.mcm_gpu_time_kernels.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_double), ctypes.c_int32]
     lib.mcm_gpu_convert_deck.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int32]
     lib.mcm_gpu_create_from_memory.argtypes = [ctypes.c_char_p, ctypes.c_int64, ctypes.POINTER(Options),
                                                ctypes.POINTER(vp)]
@@ -342,10 +343,11 @@ class Solver:
 
     def time_steps(self, n: int) -> np.ndarray:
         """Average ms per step of: neighbour build, strain-pass kernel,
-        constitutive update, momentum kernel, whole step (HIP events on the
-        run's stream)."""
-        ms = (ctypes.c_double * 5)()
-        self._check(self._lib.mcm_gpu_time_steps(self._h, int(n), ms), "time_steps")
+        constitutive update, momentum kernel, whole step, and the list build
+        kernel alone (part of the neighbour build), with HIP events on the
+        run's stream."""
+        ms = (ctypes.c_double * 6)()
+        self._check(self._lib.mcm_gpu_time_kernels(self._h, int(n), ms, 6), "time_kernels")
         return np.array(ms[:])
 
     # ---- differentiation (plain SPH, material model 10)
