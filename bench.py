@@ -65,9 +65,9 @@ KERNELS = {
     ("momentum", 1): {"name": "k_momentum<MODE_FUSED, corrected one-pass>",
                       "symbol": "void k_momentum<0, 2, false>(DevState, int)",
                       "bytes": (208.0, 4.0), "flops": (130.0, 260.0)},
-    ("list", 1): {"name": "k_list<one material>", "symbol": "void k_list<false>(DevState, int)",
+    ("list", 1): {"name": "k_list<one material>", "symbol": "void k_list<false, false>(DevState, int)",
                   "bytes": (56.0, 4.0), "flops": None},
-    ("list", 3): {"name": "k_list<one material>", "symbol": "void k_list<false>(DevState, int)",
+    ("list", 3): {"name": "k_list<one material>", "symbol": "void k_list<false, false>(DevState, int)",
                   "bytes": (56.0, 4.0), "flops": None},
 }
 PROFILE = os.path.join(HERE, "profiles", "step_kernels_taylor_bar_corrected_2.1M.md")
@@ -132,12 +132,13 @@ def kernel_ms(ms):
             "step": ms[4]}
 
 
-def timed_run(deck_path: str, passes: int, steps: int, warmup: int):
+def timed_run(deck_path: str, passes: int, steps: int, warmup: int, skin: float = 0.0):
     """warmup untimed steps, then `steps` timed steps of the bar (corrected SPH
-    in `passes` passes) between two synchronisations of the run's stream,
-    then per-kernel times of 10 more steps with HIP events."""
+    in `passes` passes, neighbour skin of width skin h_max) between two
+    synchronisations of the run's stream, then per-kernel times of 10 more
+    steps with HIP events."""
     from mcmtop_amd import gpu
-    with gpu.Solver(deck_path, corrected_one_pass=passes == 1) as s:
+    with gpu.Solver(deck_path, corrected_one_pass=passes == 1, nbr_skin=skin) as s:
         s.initial()
         s.step(warmup)
         # timed region: the step call enqueues on the context stream and
@@ -146,13 +147,15 @@ def timed_run(deck_path: str, passes: int, steps: int, warmup: int):
         s.step(steps)
         t1 = time.perf_counter()
         sc = s.scalars()
+        builds_timed = sc.skin_builds
         ms = s.time_steps(10)
         counts, gcounts, _ = s.neighbour_lists()
         nbar = float((counts + gcounts).mean())
     secs = t1 - t0
-    print("bench: %d-pass corrected, %.3f ms/step" % (passes, 1e3 * secs / steps), file=sys.stderr, flush=True)
+    print("bench: %d-pass corrected, skin %g, %.3f ms/step" % (passes, skin, 1e3 * secs / steps), file=sys.stderr,
+          flush=True)
     return {"value": sc.np * steps / secs, "secs": secs, "ms_per_step": 1e3 * secs / steps, "ms": ms, "nbar": nbar,
-            "sc": sc}
+            "sc": sc, "skin_builds": builds_timed}
 
 
 def batched_leg(nruns: int, seed: int, steps: int, warmup: int, check_every: int, plain: bool = True):
@@ -460,7 +463,14 @@ def main():
                     help="corrected SPH of the headline run: 1, single-pass moments (FP64, within the "
                          "target's accuracy bar, tests/test_gpu_one_pass.py); 3, the reference's three "
                          "passes (bit-exact). The other mode is timed beside it.")
-    ap.add_argument("--other-steps", type=int, default=100, help="timed steps of the other mode (0: skip it)")
+    ap.add_argument("--skin", type=float, default=0.0,
+                    help="neighbour skin of the headline run as a fraction of h_max (0: the reference's search "
+                         "every step; > 0: skin lists with the same neighbour sets, tests/test_gpu_skin.py; "
+                         "slower than the search over the first 200 steps of this workload, 5.2 against "
+                         "4.4 ms per step at 0.1)")
+    ap.add_argument("--other-steps", type=int, default=100,
+                    help="timed steps of the other mode, the reference's three passes without a skin "
+                         "(bit-exact; 0: skip it)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device copies of MIB MiB (a known-size kernel for scaling the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -485,7 +495,7 @@ def main():
     t = time.perf_counter()
     d = build_deck(deck_path, args.dx, 25.4e-3)
     print("bench: deck %d particles in %.1f s" % (d.np_, time.perf_counter() - t), file=sys.stderr, flush=True)
-    head = timed_run(deck_path, args.passes, args.steps, args.warmup)
+    head = timed_run(deck_path, args.passes, args.steps, args.warmup, args.skin)
     n = d.np_
     value = head["value"]
     ms, nbar, sc1 = head["ms"], head["nbar"], head["sc"]
@@ -505,11 +515,10 @@ def main():
                 "frac": flops_launch / t / 1e12 / FP64_PEAK_TFLOPS}
     other = None
     if args.other_steps > 0:
-        o = timed_run(deck_path, 4 - args.passes, args.other_steps, args.warmup)
-        other = {"corrected_passes": 4 - args.passes, "value": o["value"], "ms_per_step": o["ms_per_step"],
+        o = timed_run(deck_path, 3, args.other_steps, args.warmup, 0.0)
+        other = {"corrected_passes": 3, "skin": 0.0, "value": o["value"], "ms_per_step": o["ms_per_step"],
                  "steps": args.other_steps, "kernel_ms": kernel_ms(o["ms"]),
-                 "accuracy": "bit-exact (the reference's three passes)" if args.passes == 1 else
-                             "single-pass moments, within the 1e-10 / 1e-7 / 1e-9 bar"}
+                 "accuracy": "bit-exact (the reference's three passes and list order)"}
     res = {
         "metric": "FP64 particle-steps/s, batched vs plain @\u22641e-7 L2; adjoint/forward time ratio",
         "value": value,
@@ -524,10 +533,12 @@ def main():
         "dtype": "f64",
         "data": "synthetic Taylor-bar deck (seeded lattice + jitter)",
         "config": {"workload": "taylor_bar_corrected_f3dm10_eqos4_hopt1", "particles": n, "dx_mm": args.dx * 1e3,
-                   "corrected_passes": args.passes,
-                   "accuracy": ("single-pass moments: per-step fields within 1e-10, 1000-step L2 within 1e-7, "
-                                "energies within 1e-9 of the reference (tests/test_gpu_one_pass.py)")
-                   if args.passes == 1 else "bit-exact with the reference",
+                   "corrected_passes": args.passes, "nbr_skin": args.skin,
+                   "skin_builds": head["skin_builds"], "steps_since_initial": args.warmup + args.steps,
+                   "accuracy": ("the reference's neighbour sets; per-step fields within 1e-10, 1000-step L2 within "
+                                "1e-7, energies within 1e-9 of the reference (tests/test_gpu_one_pass.py, "
+                                "tests/test_gpu_skin.py)")
+                   if args.passes == 1 or args.skin > 0 else "bit-exact with the reference",
                    "mean_neighbours": round(nbar, 2), "ghosts": sc1.ngp, "ptime_end": sc1.ptime,
                    "pair_interactions_per_s": n * nbar * args.steps / head["secs"],
                    "kernel_ms": kernel_ms(ms)},

@@ -43,7 +43,15 @@ typedef struct {
     int32_t pair_mapping;   /* list build and pair kernels of the fused step: 0 = one thread per
                                particle, 1 = one wavefront per particle (lists up to 128
                                entries, else 0); both give identical results */
-    int32_t reserved[3];
+    int32_t nbr_skin_milli; /* neighbour skin width in thousandths of the largest h (0 = none).
+                               With a skin the real partners come from lists of the partners
+                               within h_i + h_j + width, built again only when the particles
+                               have moved far enough for a partner to be missing, and filtered
+                               each step with the reference's test (mcm_ll_neighbours.f90:140-148):
+                               the same neighbour sets, in another order, so the sums round
+                               differently.  Not with penalty contact, pair_mapping 1 or the
+                               adjoint. */
+    int32_t reserved[2];
 } mcm_gpu_options;
 
 /* host-visible scalar state (mcm_database.f90:29-31,92-99 counterparts) */
@@ -55,9 +63,11 @@ typedef struct {
     int32_t error_flags;      /* bit 0: neighbour capacity, 1: ghost capacity, 2: cell capacity,
                                  3: particle in a ghost-layer cell, 4: EOS warning,
                                  5: penalty contact with ghost contact partners (undefined in
-                                 the reference; the step stops with MCM_GPU_ERR_ARG) */
+                                 the reference; the step stops with MCM_GPU_ERR_ARG),
+                                 6: skin list capacity */
     int32_t nummat;
-    int32_t reserved[2];
+    int32_t skin_builds;      /* skin lists built so far (nbr_skin_milli > 0) */
+    int32_t reserved[1];
 } mcm_gpu_scalars;
 
 /* Create a context from an input deck: the reference's version-2 text deck

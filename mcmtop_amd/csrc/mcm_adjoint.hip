@@ -1331,6 +1331,7 @@ int mcm_gpu_adj_forward(mcm_gpu_ctx* ctx, int32_t nsteps, int32_t ckpt_every, co
     else if (d.tcrit_opt != 0 && d.tcrit_opt != 1) why = "the adjoint covers tcrit_opt 0 and 1";
     else if (d.contacttype > 0 || d.veloc_opt == 1 || d.drelax || d.baseaccel)
         why = "the adjoint does not cover penalty contact, XSPH, dynamic relaxation or base acceleration";
+    else if (d.skin > 0.0) why = "the adjoint covers runs without a neighbour skin";
     for (int m = 0; m < d.nummat && why.empty(); m++)
         if (ctx->deck.mat[m].model != 10 && ctx->deck.mat[m].model != 1)
             why = "the adjoint covers material models 1 and 10";

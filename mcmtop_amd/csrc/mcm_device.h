@@ -63,7 +63,35 @@ struct Scalars {
     int error_flags;    // MCM error bits, see mcm_gpu.h
     int abort;          // set when a capacity was exceeded: later kernels of the step do nothing
     int timestep;
+    // neighbour skin (DevState::skin > 0): the displacement of all real
+    // particles since the skin lists were built, as ordered bit patterns
+    // (ordbits) of the per-axis minimum and maximum of x - x_build and the
+    // maximum of |h - h_build|; whether two adjacent blocks moved too far
+    // apart (skin_near); the skin width and the largest h of the build; the
+    // block grid of the build (origin, edge, blocks per axis); whether the
+    // lists are valid, are built in this step, how often they were built,
+    // the longest skin list and the particles of the build
+    unsigned long long skin_dmin[3], skin_dmax[3], skin_dh;
+    int skin_near;
+    double skin_len, skin_hcap;
+    double skin_borig[3], skin_bedge;
+    int skin_bdim[3];
+    int skin_valid, skin_rebuild, skin_builds, skin_max, skin_nbuilt;
 };
+
+// Doubles as unsigned integers with the same order (for atomicMin / atomicMax
+// on signed values), and back.
+__host__ __device__ __forceinline__ unsigned long long ordbits(double x) {
+    unsigned long long b;
+    __builtin_memcpy(&b, &x, 8);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__host__ __device__ __forceinline__ double ordval(unsigned long long b) {
+    b = (b >> 63) ? (b & 0x7fffffffffffffffull) : ~b;
+    double x;
+    __builtin_memcpy(&x, &b, 8);
+    return x;
+}
 
 // Boundary planes in the order xmin xmax ymin ymax zmin zmax
 struct PlaneDev {
@@ -120,6 +148,24 @@ struct DevState {
     int* gany;                 // [cellcap] 1 where a cell holds ghosts
     int* nbr;                  // [npad/64][ncap][64]: entry k of particle i at ((i/64)*ncap + k)*64 + i%64
     int *nreal, *ntot;
+    // neighbour skin (skin > 0, skin = width as a fraction of the largest h):
+    // real partners within h_i + h_j + width at the last build, blocked like
+    // nbr by the particle's sorted position at that build
+    double skin;
+    int skin_cap, skin_blkcap;
+    int* skin_list;            // [npad/64][skin_cap][64]
+    int* skin_n;               // [np] entries of each particle's skin list
+    int* skin_pos;             // [np] sorted position of each particle at the build
+    int* skin_id;              // [np] particle at each sorted position of the build
+    int* skin_blk;             // [np] block of the particle at each sorted position of the build
+    int* skin_mat;             // [np] material of the particle at each sorted position of the build
+    double* skin_base;         // [np][4] x, y, z, h at the build, by sorted position of the build
+    unsigned long long* skin_bstat[2]; // [skin_blkcap][8] per block: ordbits of min x - x_build (3), max (3),
+                               // max |h - h_build|; one buffer filled per step, the other cleared
+    double* skin_red;          // [skin_nred][8] per-workgroup extents of k_skin_blocks
+    int skin_nred;             // workgroups of k_skin_blocks
+    double* xh;                // [np][4] x, y, z, h this step by sorted position of the skin build
+                               // (h negated: no candidate this step)
     int* cont;                 // penalty contact lists, blocked like nbr (contacttype > 0)
     int* ncont;                // [np] real contact partners
     double* rep[3];            // [np] contact acceleration of mcm_rep_cont
@@ -174,6 +220,10 @@ __device__ __forceinline__ const DevState& enter(const DevState& d0, Blk& bk) {
 // order.  Entry k of the particle at position t is nbr_row(d, t)[k * 64].
 __device__ __forceinline__ int* nbr_row(const DevState& d, int t) {
     return d.nbr + (long)(t >> 6) * d.ncap * 64 + (t & 63);
+}
+// entry k of the skin list of the particle at sorted position t of the build
+__device__ __forceinline__ int* skin_row(const DevState& d, int t) {
+    return d.skin_list + (long)(t >> 6) * d.skin_cap * 64 + (t & 63);
 }
 // Pair gather records, one aligned record per entry of the unified index
 // space, in 16-byte slots:

@@ -598,6 +598,22 @@ int alloc_all(mcm_gpu_ctx* ctx) {
     rc |= dalloc(ctx, &d.cand_id, 2 * n);
     rc |= dalloc(ctx, &d.spos, n);
     rc |= dalloc(ctx, &d.nreal, n); rc |= dalloc(ctx, &d.ntot, n);
+    if (d.skin > 0.0) {
+        d.skin_cap = d.ncap + 64;
+        d.skin_blkcap = std::max(4096, d.cellcap / 8);
+        rc |= dalloc(ctx, &d.skin_list, (size_t)d.skin_cap * d.npad);
+        rc |= dalloc(ctx, &d.skin_n, n);
+        rc |= dalloc(ctx, &d.skin_pos, n);
+        rc |= dalloc(ctx, &d.skin_id, n);
+        rc |= dalloc(ctx, &d.skin_blk, n);
+        rc |= dalloc(ctx, &d.skin_mat, n);
+        // workgroups of k_skin_blocks, one partial each
+        d.skin_nred = std::max(1, std::min(128, (d.skin_blkcap + MCM_BLOCK - 1) / MCM_BLOCK));
+        rc |= dalloc(ctx, &d.skin_red, 8 * (size_t)d.skin_nred);
+        rc |= dalloc(ctx, &d.skin_base, 4 * n);
+        rc |= dalloc(ctx, &d.xh, 4 * n);
+        for (int k = 0; k < 2; k++) rc |= dalloc(ctx, &d.skin_bstat[k], 8 * (size_t)d.skin_blkcap);
+    }
     if (d.contacttype > 0) {
         rc |= dalloc(ctx, &d.ncont, n);
         for (int k = 0; k < 3; k++) rc |= dalloc(ctx, &d.rep[k], n);
@@ -627,6 +643,7 @@ int cell_blocks(const DevState& d) {
 int tile_blocks(const DevState& d) {
     return std::max(1, std::min(blocks_for(d.cellcap, SCAN_TILE), blocks_for(d.np, SCAN_TILE) + 1));
 }
+int skin_block_blocks(const DevState& d) { return d.skin_nred; }
 int ghost_blocks(const DevState& d) {
     return std::max(1, std::min({blocks_for(d.ngcap, MCM_BLOCK), 1024, blocks_for(d.np, 2048)}));
 }
@@ -641,8 +658,10 @@ bool use_wave(const mcm_gpu_ctx* ctx) {
     return ctx->pair_mapping == 1 && wave_capable(ctx) && ctx->d.ncap <= 64 * WAVE_SLOTS && corr_variant(ctx) != 2;
 }
 
-// list-build kernel of a run: 0 one material, 1 several, 2 wavefront per particle
+// list-build kernel of a run: 0 one material, 1 several, 2 wavefront per
+// particle; with a neighbour skin 3 one material, 4 several
 int list_variant(const mcm_gpu_ctx* c) {
+    if (c->d.skin > 0.0) return c->d.nummat > 1 ? 4 : 3;
     return c->pair_mapping == 1 && wave_capable(c) ? 2 : (c->d.nummat > 1 ? 1 : 0);
 }
 
@@ -657,7 +676,7 @@ int mom_variant(const mcm_gpu_ctx* c) { return c->d.veloc_opt == 1 ? 5 + corr_va
 // workgroups one run needs in a launch of that kind and kernel variant v;
 // 0 means the run takes no part in it.
 enum LaunchKind { L_ONE, L_PART, L_CELLS, L_TILES, L_GSCAN, L_GFILL, L_GSTRESS, L_LIST, L_PAIR, L_MOM, L_CFOLD,
-                  L_REP, L_XSPH, L_NKIND };
+                  L_REP, L_XSPH, L_SKIN, L_SKINP, L_SKINB, L_NKIND };
 constexpr int kNVariant = MCM_MAX_MAT + 1;
 
 int kind_blocks(const mcm_gpu_ctx* c, int kind, int v) {
@@ -676,6 +695,9 @@ int kind_blocks(const mcm_gpu_ctx* c, int kind, int v) {
         case L_CFOLD: return d.nummat == v ? 1 : 0;
         case L_REP: return v == 0 && d.contacttype > 0 ? c->nb : 0;
         case L_XSPH: return v == 0 && d.veloc_opt == 1 ? c->nb : 0;
+        case L_SKIN: return v == 0 && d.skin > 0.0 ? 1 : 0;
+        case L_SKINP: return v == 0 && d.skin > 0.0 ? c->nb : 0;
+        case L_SKINB: return v == 0 && d.skin > 0.0 ? skin_block_blocks(d) : 0;
     }
     return 0;
 }
@@ -752,9 +774,18 @@ void launch_search(const Target& T, int cur) {
 
 // the list build
 void launch_lists(const Target& T, int cur) {
-    launch(T, L_LIST, 0, MCM_BLOCK, k_list<false>, cur);
-    launch(T, L_LIST, 1, MCM_BLOCK, k_list<true>, cur);
+    launch(T, L_LIST, 0, MCM_BLOCK, k_list<false, false>, cur);
+    launch(T, L_LIST, 1, MCM_BLOCK, k_list<true, false>, cur);
     launch(T, L_LIST, 2, MCM_BLOCK, k_list_wave, cur);
+    // neighbour skin: displacement since the build, the check, the build
+    // where needed, then the filter
+    launch(T, L_SKINP, 0, MCM_BLOCK, k_skin_stats, cur);
+    launch(T, L_SKINB, 0, MCM_BLOCK, k_skin_blocks, cur);
+    launch(T, L_SKIN, 0, 64, k_skin_check);
+    launch(T, L_LIST, 3, MCM_BLOCK, k_list<false, true>, cur);
+    launch(T, L_LIST, 3, MCM_BLOCK, k_skin_filter<false>, cur);
+    launch(T, L_LIST, 4, MCM_BLOCK, k_list<true, true>, cur);
+    launch(T, L_LIST, 4, MCM_BLOCK, k_skin_filter<true>, cur);
 }
 
 void launch_neighbours(const Target& T, int cur) {
@@ -909,9 +940,16 @@ int regrow(mcm_gpu_ctx* ctx, const Scalars& sc_host) {
         int rc = alloc_neighbour_arrays(ctx);
         if (rc) return rc;
     }
+    if (flags & 64) {  // skin lists
+        dfree(ctx, d.skin_list);
+        d.skin_cap = std::max(d.skin_cap + 16, (int)(sc_host.skin_max * 1.25) + 16);
+        int rc = dalloc(ctx, &d.skin_list, (size_t)d.skin_cap * d.npad);
+        if (rc) return rc;
+    }
     Scalars s2 = sc_host;
     s2.abort = 0;
     s2.error_flags = 0;
+    s2.skin_valid = 0;   // the step is repeated from its start: build the skin lists again
     return put_scalars_raw(ctx, s2);
 }
 
@@ -1068,6 +1106,7 @@ int scatter_rows(mcm_gpu_ctx* ctx, const double* rows, const mcm_gpu_scalars& sc
     s.timestep = sc.timestep;
     s.abort = 0;
     s.error_flags = 0;
+    s.skin_valid = 0;   // positions replaced: the skin lists are built again
     if ((rc = put_scalars_raw(ctx, s))) return rc;
     ctx->phase = 0;
     ctx->done = 0;
@@ -1316,6 +1355,12 @@ int setup_run(mcm_gpu_ctx* ctx, const mcm_gpu_options* opt) {
     }
     ctx->nb = blocks_for(D.np, MCM_BLOCK);
     d.nblk = ctx->nb;
+    d.skin = opt ? opt->nbr_skin_milli / 1000.0 : 0.0;
+    if (opt && opt->nbr_skin_milli < 0) { ctx->err = "nbr_skin_milli must not be negative"; return MCM_GPU_ERR_ARG; }
+    if (d.skin > 0.0 && (d.contacttype > 0 || ctx->pair_mapping != 0)) {
+        ctx->err = "a neighbour skin needs kernel contact (contacttype 0) and pair_mapping 0";
+        return MCM_GPU_ERR_ARG;
+    }
     int rc = alloc_all(ctx);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -1768,6 +1813,7 @@ int mcm_gpu_get_scalars(mcm_gpu_ctx* ctx, mcm_gpu_scalars* out) {
     out->np = ctx->d.np; out->ngp = s.ngp; out->max_count = s.max_count; out->timestep = s.timestep;
     out->error_flags = s.error_flags;
     out->nummat = ctx->d.nummat;
+    out->skin_builds = s.skin_builds;
     return 0;
 }
 

@@ -115,6 +115,13 @@ __global__ void k_grid_setup(DevState d_) {
     sc->ngp = 0;
     sc->max_count = 0;
     sc->hcap_bits = 0;
+    for (int k = 0; k < 3; k++) {
+        sc->skin_dmin[k] = ~0ull;
+        sc->skin_dmax[k] = 0ull;
+    }
+    sc->skin_dh = 0ull;
+    sc->skin_near = 0;
+    sc->skin_rebuild = 0;
 }
 
 __global__ void k_clear_cells(DevState d_) {
@@ -140,6 +147,231 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_rs_shift(DevState d) {
     *rs_at(d, 0, i) = make_double2(s0, s1);
     *lo_of(rs_at(d, 1, i)) = s2;
     if (!(fast_coord(s0) & fast_coord(s1) & fast_coord(s2))) atomicOr(d.range, (int)RANGE_RS);
+}
+
+// ------------------------------------------------------- neighbour skin
+// With a skin the real partners of a particle are taken from its skin list:
+// every real particle j with |x_i - x_j| < h_i + h_j + w at the last build
+// (w the skin width), in the walk order of that build.  Each step keeps
+// those with the reference's test at the current x and h, so the lists hold
+// the reference's sets (mcm_ll_neighbours.f90:140-148), not its order.
+//
+// Why the skin holds every partner.  At the build the particles are put in
+// cubic blocks of edge b >= 2 (2 hc + w) (hc the largest h then), by their
+// position x_b then.  Let D_k = x_k - x_b,k be the displacement of particle
+// k since the build and E_k = |h_k - h_b,k|.  Take a pair (i, j) that is not
+// in the skin: |x_b,i - x_b,j| >= h_b,i + h_b,j + w.
+//  * i and j in the same or adjacent blocks A, B: with R_AB the diagonal of
+//    the box spanned by the D_k of both blocks and H_AB the largest E_k there,
+//      |x_i - x_j| >= |x_b,i - x_b,j| - |D_i - D_j|
+//                 >= h_b,i + h_b,j + w - R_AB >= h_i + h_j + w - R_AB - 2 H_AB,
+//    so the pair fails the reference's test while R_AB + 2 H_AB < w;
+//  * blocks further apart: more than b separates x_b,i and x_b,j along some
+//    axis, so with R and H the same extents over all particles
+//      |x_i - x_j| >= |x_b,i - x_b,j| - |D_i - D_j| > b - R,
+//    while h_i + h_j <= 2 (hc + H): the pair fails the test while
+//    R + 2 H <= b - 2 hc.
+// Every step records the per-block extents of D and E (k_skin_stats) and
+// checks both conditions (k_skin_blocks, k_skin_check); the lists are built
+// again in the first step that breaks one of them.  The build takes w wider
+// by a relative 1e-9, and the checks compare with a margin of the same size,
+// far above the rounding of the distances.
+//
+// The skin lists hold sorted positions of the build, and the candidates
+// are read from xh by that position, {x, y, z, h} of this step: particles
+// that were neighbours in space at the build lie near each other in that
+// order, so the gathers of a wavefront share cache lines as the list
+// build's candidate records do.  h is negated for a particle that is no
+// candidate this step (outside the grid, or in a cell holding ghosts, which
+// hides its real particles as in the reference's linked lists).
+
+// xh of real particle i with valid skin lists (binned: it lies in the grid)
+__device__ __forceinline__ void skin_write_xh(const DevState& d, int i, bool binned, int cur) {
+    double2* xr = reinterpret_cast<double2*>(d.xh) + 2 * (long)d.skin_pos[i];
+    const double h = d.hbuf[cur][i];
+    xr[0] = make_double2(d.x0[0][i], d.x0[1][i]);
+    xr[1] = make_double2(d.x0[2][i], binned ? h : -h);
+}
+
+// Per-block extents of the displacement since the build: one thread per
+// sorted position of the build; the lanes of a wavefront mostly share a
+// block, so each distinct block of the wavefront takes one atomic per
+// value.  Buffer skin_bstat[cur] (cleared in the step before).
+__global__ __launch_bounds__(MCM_BLOCK) void k_skin_stats(DevState d_, int cur) {
+    Blk bk;
+    const DevState& d = enter<false>(d_, bk);
+    Scalars* sc = d.sc;
+    if (sc->abort || !sc->skin_valid) return;
+    const int tb = bk.b * blockDim.x + threadIdx.x;
+    int blk = -1;
+    double dv[3] = {0.0, 0.0, 0.0}, e = 0.0;
+    (void)cur;
+    if (tb < sc->skin_nbuilt) {
+        const double2* br = reinterpret_cast<const double2*>(d.skin_base) + 2 * (long)tb;
+        const double2* xr = reinterpret_cast<const double2*>(d.xh) + 2 * (long)tb;
+        const double2 b0 = br[0], b1 = br[1], x0 = xr[0], x1 = xr[1];
+        dv[0] = x0.x - b0.x;
+        dv[1] = x0.y - b0.y;
+        dv[2] = x1.x - b1.x;
+        e = fabs(fabs(x1.y) - b1.y);
+        // a NaN position or h makes the skin useless: force a build
+        if (!(dv[0] == dv[0] && dv[1] == dv[1] && dv[2] == dv[2] && e == e)) e = 1.0e300;
+        blk = d.skin_blk[tb];
+    }
+    unsigned long long* st = d.skin_bstat[cur];
+    unsigned long long todo = __ballot(blk >= 0);
+    while (todo) {
+        const int leader = __builtin_ctzll(todo);
+        const int lb = __builtin_amdgcn_readlane(blk, leader);
+        const bool mine = blk == lb;
+        double lo[3], hi[3], em;
+        for (int k = 0; k < 3; k++) {
+            lo[k] = wave_min(mine ? dv[k] : 1.0e300);
+            hi[k] = wave_max(mine ? dv[k] : -1.0e300);
+        }
+        em = wave_max(mine ? e : 0.0);
+        if ((threadIdx.x & 63) == leader) {
+            unsigned long long* b = st + 8 * (long)lb;
+            for (int k = 0; k < 3; k++) {
+                atomicMin(&b[k], ordbits(lo[k]));
+                atomicMax(&b[3 + k], ordbits(hi[k]));
+            }
+            atomicMax(&b[6], ordbits(em));
+        }
+        todo &= ~__ballot(mine);
+    }
+}
+
+// One thread per block A of the build: the joint extent of A with each
+// adjacent block B (B after A in linear order, and A itself) against the
+// skin width; folds the extents of all blocks into the run's totals; clears
+// the other buffer for the next step.
+__global__ __launch_bounds__(MCM_BLOCK) void k_skin_blocks(DevState d_, int cur) {
+    Blk bk;
+    const DevState& d = enter<false>(d_, bk);
+    Scalars* sc = d.sc;
+    if (sc->abort) return;
+    unsigned long long* clr = d.skin_bstat[cur ^ 1];
+    for (long q = (long)bk.b * blockDim.x + threadIdx.x; q < 8L * d.skin_blkcap; q += (long)bk.n * blockDim.x)
+        clr[q] = (q & 7) < 3 ? ~0ull : 0ull;
+    if (!sc->skin_valid) return;
+    const unsigned long long* st = d.skin_bstat[cur];
+    const int n0 = sc->skin_bdim[0], n1 = sc->skin_bdim[1], n2 = sc->skin_bdim[2];
+    const int nblk = n0 * n1 * n2;
+    const double w = sc->skin_len;
+    double lo[3] = {1.0e300, 1.0e300, 1.0e300}, hi[3] = {-1.0e300, -1.0e300, -1.0e300}, em = 0.0;
+    int near = 0;
+    for (int a = bk.b * blockDim.x + threadIdx.x; a < nblk; a += bk.n * blockDim.x) {
+        const unsigned long long* A = st + 8 * (long)a;
+        if (A[0] > A[3]) continue;   // no particle in the block
+        double alo[3], ahi[3];
+        for (int k = 0; k < 3; k++) {
+            alo[k] = ordval(A[k]);
+            ahi[k] = ordval(A[3 + k]);
+            lo[k] = fmin(lo[k], alo[k]);
+            hi[k] = fmax(hi[k], ahi[k]);
+        }
+        const double ae = ordval(A[6]);
+        em = fmax(em, ae);
+        const int a0 = a % n0, a1 = (a / n0) % n1, a2 = a / (n0 * n1);
+        for (int o = 13; o < 27; o++) {   // offsets (dz, dy, dx) with B >= A
+            const int c0 = a0 + o % 3 - 1, c1 = a1 + (o / 3) % 3 - 1, c2 = a2 + o / 9 - 1;
+            if (c0 < 0 || c0 >= n0 || c1 < 0 || c1 >= n1 || c2 < 0 || c2 >= n2) continue;
+            const unsigned long long* B = st + 8 * (long)((c2 * n1 + c1) * n0 + c0);
+            if (B[0] > B[3]) continue;
+            double r2 = 0.0;
+            for (int k = 0; k < 3; k++) {
+                const double ext = fmax(ahi[k], ordval(B[3 + k])) - fmin(alo[k], ordval(B[k]));
+                r2 += ext * ext;
+            }
+            const double bound = (sqrt(r2) + 2.0 * fmax(ae, ordval(B[6]))) * (1.0 + 1.0e-9);
+            if (!(bound < w)) near = 1;
+        }
+    }
+    // the workgroup's extents and flag, folded by k_skin_check
+    __shared__ double red[8][MCM_BLOCK / 64];
+    for (int k = 0; k < 3; k++) { lo[k] = wave_min(lo[k]); hi[k] = wave_max(hi[k]); }
+    em = wave_max(em);
+    const double fl = __any(near) ? 1.0 : 0.0;
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        for (int k = 0; k < 3; k++) { red[k][wv] = lo[k]; red[3 + k][wv] = hi[k]; }
+        red[6][wv] = em;
+        red[7][wv] = fl;
+    }
+    __syncthreads();
+    if (threadIdx.x < 8) {
+        const int k = threadIdx.x;
+        double v = red[k][0];
+        for (int q = 1; q < MCM_BLOCK / 64; q++) v = k < 3 ? fmin(v, red[k][q]) : fmax(v, red[k][q]);
+        d.skin_red[8 * (long)bk.b + k] = v;
+    }
+}
+
+// One thread: whether the skin lists must be built in this step, and if so
+// their width and block grid.  A build is needed when they are not valid
+// (first search, grown capacities, replaced state), when one of the two
+// conditions above fails, or when a particle lay outside the grid at the
+// last build (it is in no skin list, and may be back in the grid now).
+__global__ __launch_bounds__(64) void k_skin_check(DevState d_) {
+    Blk bk;
+    const DevState& d = enter<false>(d_, bk);
+    Scalars* sc = d.sc;
+    if (sc->abort) return;
+    // fold the partial extents of k_skin_blocks (one wavefront)
+    double lo[3] = {1.0e300, 1.0e300, 1.0e300}, hi[3] = {-1.0e300, -1.0e300, -1.0e300}, em = 0.0, fl = 0.0;
+    if (sc->skin_valid) {
+        for (int q = threadIdx.x; q < d.skin_nred; q += 64) {
+            const double* r = d.skin_red + 8 * (long)q;
+            for (int k = 0; k < 3; k++) { lo[k] = fmin(lo[k], r[k]); hi[k] = fmax(hi[k], r[3 + k]); }
+            em = fmax(em, r[6]);
+            fl = fmax(fl, r[7]);
+        }
+        for (int k = 0; k < 3; k++) { lo[k] = wave_min(lo[k]); hi[k] = wave_max(hi[k]); }
+        em = wave_max(em);
+        fl = wave_max(fl);
+    }
+    if (threadIdx.x != 0) return;
+    for (int k = 0; k < 3; k++) { sc->skin_dmin[k] = ordbits(lo[k]); sc->skin_dmax[k] = ordbits(hi[k]); }
+    sc->skin_dh = ordbits(em);
+    sc->skin_near = fl > 0.0 ? 1 : 0;
+    bool need = !sc->skin_valid || fl > 0.0;
+    if (!need) {
+        double r2 = 0.0;
+        for (int k = 0; k < 3; k++) {
+            const double e = hi[k] - lo[k];
+            r2 += e * e;
+        }
+        const double bound = (sqrt(r2) + 2.0 * em) * (1.0 + 1.0e-9);
+        need = !(bound < sc->skin_bedge - 2.0 * sc->skin_hcap);
+    }
+    sc->skin_rebuild = need ? 1 : 0;
+    if (!need) return;
+    const double hc = __longlong_as_double((long long)sc->hcap_bits);
+    const double w = d.skin * hc;
+    sc->skin_len = w;
+    sc->skin_hcap = hc;
+    sc->skin_valid = sc->nout == 0 ? 1 : 0;
+    sc->skin_builds += 1;
+    // blocks over the particles' bounds (all lie inside the search grid,
+    // which starts at gridmin); the edge doubles until the blocks fit
+    double edge = 2.0 * (2.0 * hc + w) * (1.0 + 1.0e-9);
+    int n[3];
+    for (;;) {
+        long long tot = 1;
+        for (int k = 0; k < 3; k++) {
+            const double span = (sc->coord_max[k] - sc->grid.gridmin[k]) / edge;
+            n[k] = span > 0.0 && span < 1.0e9 ? (int)span + 1 : 1;
+            tot *= n[k];
+        }
+        if (tot <= d.skin_blkcap) break;
+        edge *= 2.0;
+    }
+    for (int k = 0; k < 3; k++) {
+        sc->skin_borig[k] = sc->grid.gridmin[k];
+        sc->skin_bdim[k] = n[k];
+    }
+    sc->skin_bedge = edge;
 }
 
 // histogram: cell of every real particle (mcm_setuplist.f90:320-345); also
@@ -197,6 +429,7 @@ __global__ __launch_bounds__(256) void k_bin(DevState d_, int cur) {
         }
         d.cell_of[i] = cell;
     }
+    if (d.skin > 0.0 && live && d.sc->skin_valid) skin_write_xh(d, i, cell >= 0, cur);
     // histogram with one atomic per distinct cell of the wavefront: lanes of
     // the same cell take consecutive slots (the cell sort orders them later)
     const int lane = threadIdx.x & 63;
@@ -432,6 +665,16 @@ __global__ __launch_bounds__(1024) void k_ghost_scan(DevState d_, int pi) {
                 d.gstart[PL.dim][tc] = base;
                 d.gcount[PL.dim][tc] = cnt;
                 d.gany[tc] = 1;
+                // a cell holding ghosts hides its real particles from the
+                // search: no candidates of the skin filter this step
+                if (d.skin > 0.0 && sc->skin_valid) {
+                    double2* xh = reinterpret_cast<double2*>(d.xh);
+                    const int s0 = d.cell_start[tc];
+                    for (int q = 0; q < d.cell_count[tc]; q++) {
+                        double* hq = &xh[2 * (long)d.skin_pos[d.sorted[s0 + q]] + 1].y;
+                        *hq = -fabs(*hq);
+                    }
+                }
             }
         }
         __syncthreads();
@@ -488,6 +731,88 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_ghost_fill(DevState d_, int pi, i
 }
 
 // ------------------------------------------------------------ list build
+// What a skin build records of the particle i at sorted position t (cell c,
+// x, h of this step): its position, the particle and material at t, x and h
+// at the build, its block of the build, and this step's candidate record
+// (h negated where a ghost-holding cell hides it; k_ghost_scan marks such
+// particles in the steps between builds).  A block is clamped to the block
+// grid: a clamped block is only nearer to the others, which keeps both skin
+// conditions safe.
+__device__ __forceinline__ void skin_record(const DevState& d, int t, int i, int c, double xi, double yi, double zi,
+                                            double hi, const GridDev& g) {
+    (void)g;
+    const Scalars* sc = d.sc;
+    d.skin_pos[i] = t;
+    d.skin_id[t] = i;
+    if (d.nummat > 1) d.skin_mat[t] = d.pmat[i];
+    double2* br = reinterpret_cast<double2*>(d.skin_base) + 2 * (long)t;
+    br[0] = make_double2(xi, yi);
+    br[1] = make_double2(zi, hi);
+    double2* xr = reinterpret_cast<double2*>(d.xh) + 2 * (long)t;
+    xr[0] = make_double2(xi, yi);
+    xr[1] = make_double2(zi, d.gany[c] ? -hi : hi);
+    const double xv[3] = {xi, yi, zi};
+    int kb[3];
+    for (int a = 0; a < 3; a++) {
+        const double q = (xv[a] - sc->skin_borig[a]) / sc->skin_bedge;
+        const int nb = sc->skin_bdim[a];
+        kb[a] = q >= 0.0 ? (q < (double)nb ? (int)q : nb - 1) : 0;
+    }
+    d.skin_blk[t] = (kb[2] * sc->skin_bdim[1] + kb[1]) * sc->skin_bdim[0] + kb[0];
+}
+
+// Ghost candidates of particle i (material mi, at x, h) in the cells
+// [lo, hi] around its own (mcm_ll_neighbours.f90:181-228): the ghost runs
+// of each ghost-holding cell, z then y then x planes, each from its last
+// ghost down; accepted ghosts are appended to out, counted in cnt (gccnt:
+// other-material ghosts of penalty contact, only counted), their distances
+// folded into mind / mind_cp.
+template <bool MULTI>
+__device__ __forceinline__ void ghost_walk(const DevState& d, int cur, int i, int mi, double xi, double yi, double zi,
+                                           double hi, int lo0, int hi0, int lo1, int hi1, int lo2, int hi2, int L0,
+                                           int L1, bool penalty, int* out, int& cnt, int& gccnt, double& mind,
+                                           double& mind_cp) {
+    (void)i;
+    const int ncap = d.ncap;
+    for (int pass = 0; pass < (MULTI ? 2 : 1); pass++) {
+        for (int n = lo2; n <= hi2; n++)
+            for (int m = lo1; m <= hi1; m++)
+                for (int l = lo0; l <= hi0; l++) {
+                    int cc = (n * L1 + m) * L0 + l;
+                    if (!d.gany[cc]) continue;
+                    for (int dd = 2; dd >= 0; dd--) {
+                        int gn = d.gcount[dd][cc];
+                        if (gn == 0) continue;
+                        int gs = d.gstart[dd][cc];
+                        for (int q = gn - 1; q >= 0; q--) {
+                            int u = d.np + gs + q;
+                            double dist_sq = 0.0;
+                            double dx = d.x0[0][u] - xi; dist_sq = dist_sq + dx * dx;
+                            double dy = d.x0[1][u] - yi; dist_sq = dist_sq + dy * dy;
+                            double dz = d.x0[2][u] - zi; dist_sq = dist_sq + dz * dz;
+                            double h_avg = 2.0 * 0.5 * (d.hbuf[cur][u] + hi);
+                            if (dist_sq < h_avg * h_avg) {
+                                const bool same = !MULTI || d.pmat[u] == mi;
+                                if (pass == 0 && same) {
+                                    mind = fmin(mind, dist_sq);
+                                    if (cnt < ncap) out[(long)cnt * 64] = u;
+                                    cnt++;
+                                } else if (MULTI && pass == 1 && !same) {
+                                    mind_cp = fmin(mind_cp, dist_sq);
+                                    if (penalty) {
+                                        gccnt++;
+                                    } else {
+                                        if (cnt < ncap) out[(long)cnt * 64] = u;
+                                        cnt++;
+                                    }
+                                }
+                            }
+                        }
+                    }
+                }
+    }
+}
+
 // mcm_ll_neighbours.f90:96-299 for one real particle, one thread per
 // particle (list layout: nbr_row in mcm_device.h).  MULTI: more than one
 // material, so same-material and other-material passes.
@@ -501,7 +826,15 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_ghost_fill(DevState d_, int pi, i
 // faces).  Ghosts are not binned by position and are never skipped.
 // Skipping rejected candidates leaves the list, its order and mindist
 // unchanged.
-template <bool MULTI>
+//
+// SKIN: the skin lists of a step that needs them (k_skin_check), with the
+// same walk: every binned real particle j != i with
+// |x_i - x_j| < (h_i + h_j + w)(1 + 1e-9), w the skin width, all materials
+// in walk order, stored as sorted positions of this build; the real
+// particles of ghost-holding cells are taken too (xh hides them in the
+// steps where their cell holds ghosts).  Records each particle's sorted
+// position, its x and h, its block and its candidate record.
+template <bool MULTI, bool SKIN>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 3 : 5))) void k_list(DevState d_, int cur) {
     // threads take the binned particles in cell-sorted order, so that the
     // lanes of a wavefront share cells and read the same candidates
@@ -510,18 +843,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 3 :
     const int t = bk.b * blockDim.x + threadIdx.x;
     Scalars* sc = d.sc;
     if (sc->abort) return;
+    if (SKIN && !sc->skin_rebuild) return;
     const GridDev g = sc->grid;
     const bool live = g.ncell > 0 && t < d.cell_start[g.ncell - 1] + d.cell_count[g.ncell - 1];
     int cnt = 0;
     if (live) {
     const int i = d.sorted[t];
     const int L0 = g.lim[0], L1 = g.lim[1];
-    const int ncap = d.ncap;
-    int* out = nbr_row(d, t);
+    const int ncap = SKIN ? d.skin_cap : d.ncap;
+    int* out = SKIN ? skin_row(d, t) : nbr_row(d, t);
     // penalty contact: other-material partners go to the contact list, ghosts
     // of other materials are only counted (mcm_ll_neighbours.f90:162-174,
     // 209-223; merged into the lists only for kernel contact, :239-267)
-    const bool penalty = MULTI && d.contacttype > 0;
+    const bool penalty = !SKIN && MULTI && d.contacttype > 0;
     int ccnt = 0, gccnt = 0;
     double mind = 1.0e20, mind_cp = 1.0e20;
     const int c = d.cell_of[i];
@@ -535,7 +869,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 3 :
         const int lo1 = max(b1 - 1, 0), hi1 = min(b1 + 1, L1 - 1);
         const int lo2 = max(b2 - 1, 0), hi2 = min(b2 + 1, g.lim[2] - 1);
         // distances from the particle to the lower and upper faces of its cell
-        const double reach = (hi + __longlong_as_double((long long)sc->hcap_bits)) * (1.0 + 1.0e-9);
+        const double w = SKIN ? sc->skin_len : 0.0;
+        const double grow = SKIN ? 1.0 + 1.0e-9 : 1.0;
+        const double reach = (hi + __longlong_as_double((long long)sc->hcap_bits) + w) * grow * (1.0 + 1.0e-9);
         const double reach2 = reach * reach;
         double flo[3], fhi[3];
         {
@@ -551,15 +887,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 3 :
         const double2* cd = reinterpret_cast<const double2*>(d.cand);
         const int2* ci = reinterpret_cast<const int2*>(d.cand_id);
         const int spos_i = t;
-        for (int pass = 0; pass < (MULTI ? 2 : 1); pass++) {
+        for (int pass = 0; pass < (MULTI && !SKIN ? 2 : 1); pass++) {
             // candidate at sorted slot s; with one material its id is read
             // only when it is accepted (the self test uses the sorted slot)
             auto consider = [&](int s, double2 pa, double2 pb, int2 idm) {
-                if (MULTI ? idm.x == i : s == spos_i) return;
+                if (MULTI && !SKIN ? idm.x == i : s == spos_i) return;
                 double dx = pa.x - xi;
                 double dist_sq = dx * dx;   // 0.0 + dx*dx is dx*dx exactly
                 double dy = pa.y - yi; dist_sq = dist_sq + dy * dy;
                 double dz = pb.x - zi; dist_sq = dist_sq + dz * dz;
+                if (SKIN) {
+                    const double r = (pb.y + hi + w) * grow;
+                    if (dist_sq < r * r) {
+                        if (cnt < ncap) out[(long)cnt * 64] = s;
+                        cnt++;
+                    }
+                    return;
+                }
                 double h_avg = 2.0 * 0.5 * (pb.y + hi);
                 if (dist_sq < h_avg * h_avg) {
                     const bool same = !MULTI || idm.y == mi;
@@ -589,7 +933,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 3 :
 #pragma unroll
                     for (int u = 0; u < 4; u++) {
                         const int q = min(s + u, en - 1);
-                        if (MULTI) idm[u] = ci[q];
+                        if (MULTI && !SKIN) idm[u] = ci[q];
                         pa[u] = cd[2 * q];
                         pb[u] = cd[2 * q + 1];
                     }
@@ -615,7 +959,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 3 :
                     // cell's real particles stay hidden
                     const int c0 = (n * L1 + m) * L0;
                     bool ghosty = false;
-                    for (int l = l0; l <= l1; l++) ghosty |= d.gany[c0 + l] != 0;
+                    if (!SKIN)
+                        for (int l = l0; l <= l1; l++) ghosty |= d.gany[c0 + l] != 0;
                     if (!ghosty) {
                         walk(d.cell_start[c0 + l0], d.cell_start[c0 + l1] + d.cell_count[c0 + l1]);
                     } else {
@@ -628,50 +973,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 3 :
                 }
             }
         }
-        if (d.disctype == 1) {
-            if (cnt < ncap) out[(long)cnt * 64] = i;
-            cnt++;
-        }
-        nreal = cnt;
-        if (d.boundary) {
-            for (int pass = 0; pass < (MULTI ? 2 : 1); pass++) {
-                for (int n = lo2; n <= hi2; n++)
-                    for (int m = lo1; m <= hi1; m++)
-                        for (int l = lo0; l <= hi0; l++) {
-                            int cc = (n * L1 + m) * L0 + l;
-                            if (!d.gany[cc]) continue;
-                            for (int dd = 2; dd >= 0; dd--) {
-                                int gn = d.gcount[dd][cc];
-                                if (gn == 0) continue;
-                                int gs = d.gstart[dd][cc];
-                                for (int q = gn - 1; q >= 0; q--) {
-                                    int u = d.np + gs + q;
-                                    double dist_sq = 0.0;
-                                    double dx = d.x0[0][u] - xi; dist_sq = dist_sq + dx * dx;
-                                    double dy = d.x0[1][u] - yi; dist_sq = dist_sq + dy * dy;
-                                    double dz = d.x0[2][u] - zi; dist_sq = dist_sq + dz * dz;
-                                    double h_avg = 2.0 * 0.5 * (d.hbuf[cur][u] + hi);
-                                    if (dist_sq < h_avg * h_avg) {
-                                        const bool same = !MULTI || d.pmat[u] == mi;
-                                        if (pass == 0 && same) {
-                                            mind = fmin(mind, dist_sq);
-                                            if (cnt < ncap) out[(long)cnt * 64] = u;
-                                            cnt++;
-                                        } else if (MULTI && pass == 1 && !same) {
-                                            mind_cp = fmin(mind_cp, dist_sq);
-                                            if (penalty) {
-                                                gccnt++;
-                                            } else {
-                                                if (cnt < ncap) out[(long)cnt * 64] = u;
-                                                cnt++;
-                                            }
-                                        }
-                                    }
-                                }
-                            }
-                        }
+        if (SKIN) {
+            skin_record(d, t, i, c, xi, yi, zi, hi, g);
+        } else {
+            if (d.disctype == 1) {
+                if (cnt < ncap) out[(long)cnt * 64] = i;
+                cnt++;
             }
         }
+        nreal = cnt;
+        if (!SKIN && d.boundary)
+            ghost_walk<MULTI>(d, cur, i, mi, xi, yi, zi, hi, lo0, hi0, lo1, hi1, lo2, hi2, L0, L1, penalty, out, cnt,
+                              gccnt, mind, mind_cp);
         mind = sqrt(penalty ? mind : fmin(mind, mind_cp));
         // a particle with real and ghost contact partners: the reference
         // reads the ghost partners from the real contact list past its end
@@ -681,19 +994,120 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 3 :
             atomicExch(&sc->abort, 1);
         }
     } else {
-        if (d.disctype == 1) {
+        if (!SKIN && d.disctype == 1) {
             if (cnt < ncap) out[0] = i;
             cnt++;
         }
         nreal = cnt;
     }
-    d.nreal[i] = nreal;
-    d.ntot[i] = cnt;
-    d.mindist[i] = mind;
-    if (penalty) d.ncont[i] = ccnt;
-    cnt = max(cnt, ccnt);   // both lists share the capacity ncap
+    if (SKIN) {
+        d.skin_n[i] = cnt;
+    } else {
+        d.nreal[i] = nreal;
+        d.ntot[i] = cnt;
+        d.mindist[i] = mind;
+        if (penalty) d.ncont[i] = ccnt;
+        cnt = max(cnt, ccnt);   // both lists share the capacity ncap
+    }
     }
     // longest list of the wavefront, one atomic per wavefront
+    for (int off = 32; off > 0; off >>= 1) cnt = max(cnt, __shfl_xor(cnt, off, 64));
+    if ((threadIdx.x & 63) == 0 && cnt > 0) {
+        atomicMax(SKIN ? &sc->skin_max : &sc->max_count, cnt);
+        if (cnt > (SKIN ? d.skin_cap : d.ncap)) {
+            atomicOr(&sc->error_flags, SKIN ? 64 : 1);
+            atomicExch(&sc->abort, 1);
+        }
+    }
+    if (SKIN && t == 0) sc->skin_nbuilt = live ? d.cell_start[g.ncell - 1] + d.cell_count[g.ncell - 1] : 0;
+}
+
+// ----------------------------------------------------- skin list filter
+// The neighbour lists of one step from the skin lists (see the skin notes
+// above): each binned particle, in this step's cell-sorted thread order,
+// keeps the entries of its skin list that pass the reference's test
+// (mcm_ll_neighbours.f90:140-148, the expression of k_list) with x and h of
+// this step, same material first, then other materials (kernel contact),
+// in skin-list order; then itself (corrected SPH) and its ghosts as k_list
+// finds them.  The sets equal k_list's; the order of the real partners is
+// the skin build's.  Runs with penalty contact take no skin.
+template <bool MULTI>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 3 : 5))) void k_skin_filter(DevState d_,
+                                                                                                        int cur) {
+    Blk bk;
+    const DevState& d = enter<true>(d_, bk);
+    const int t = bk.b * blockDim.x + threadIdx.x;
+    Scalars* sc = d.sc;
+    if (sc->abort) return;
+    const GridDev g = sc->grid;
+    const bool live = g.ncell > 0 && t < d.cell_start[g.ncell - 1] + d.cell_count[g.ncell - 1];
+    int cnt = 0;
+    if (live) {
+        const int L0 = g.lim[0], L1 = g.lim[1];
+        const int ncap = d.ncap;
+        int* out = nbr_row(d, t);
+        const double2* cd = reinterpret_cast<const double2*>(d.cand);
+        const int2 own = reinterpret_cast<const int2*>(d.cand_id)[t];
+        const int i = own.x, mi = own.y;
+        const int c = d.cell_of[i];
+        const double2 pa0 = cd[2 * t], pb0 = cd[2 * t + 1];
+        const double xi = pa0.x, yi = pa0.y, zi = pb0.x, hi = pb0.y;
+        const int* sl = skin_row(d, d.skin_pos[i]);
+        const int sn = d.skin_n[i];
+        const double2* xh = reinterpret_cast<const double2*>(d.xh);
+        double mind = 1.0e20, mind_cp = 1.0e20;
+        for (int pass = 0; pass < (MULTI ? 2 : 1); pass++) {
+            for (int k = 0; k < sn; k += 4) {
+                int j[4];
+                double2 pa[4], pb[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) j[u] = sl[(long)min(k + u, sn - 1) * 64];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    pa[u] = xh[2 * (long)j[u]];
+                    pb[u] = xh[2 * (long)j[u] + 1];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    if (k + u >= sn) continue;
+                    double dx = pa[u].x - xi;
+                    double dist_sq = dx * dx;   // 0.0 + dx*dx is dx*dx exactly
+                    double dy = pa[u].y - yi; dist_sq = dist_sq + dy * dy;
+                    double dz = pb[u].x - zi; dist_sq = dist_sq + dz * dz;
+                    double h_avg = 2.0 * 0.5 * (pb[u].y + hi);
+                    if (pb[u].y > 0.0 && dist_sq < h_avg * h_avg) {
+                        const bool same = !MULTI || d.skin_mat[j[u]] == mi;
+                        if (pass == 0 && same) {
+                            mind = fmin(mind, dist_sq);
+                            if (cnt < ncap) out[(long)cnt * 64] = d.skin_id[j[u]];
+                            cnt++;
+                        } else if (MULTI && pass == 1 && !same) {
+                            mind_cp = fmin(mind_cp, dist_sq);
+                            if (cnt < ncap) out[(long)cnt * 64] = d.skin_id[j[u]];
+                            cnt++;
+                        }
+                    }
+                }
+            }
+        }
+        if (d.disctype == 1) {
+            if (cnt < ncap) out[(long)cnt * 64] = i;
+            cnt++;
+        }
+        const int nreal = cnt;
+        if (d.boundary) {
+            const int b0 = c % L0, b1 = (c / L0) % L1, b2 = c / (L0 * L1);
+            const int lo0 = max(b0 - 1, 0), hi0 = min(b0 + 1, L0 - 1);
+            const int lo1 = max(b1 - 1, 0), hi1 = min(b1 + 1, L1 - 1);
+            const int lo2 = max(b2 - 1, 0), hi2 = min(b2 + 1, g.lim[2] - 1);
+            int gccnt = 0;
+            ghost_walk<MULTI>(d, cur, i, mi, xi, yi, zi, hi, lo0, hi0, lo1, hi1, lo2, hi2, L0, L1, false, out, cnt,
+                              gccnt, mind, mind_cp);
+        }
+        d.nreal[i] = nreal;
+        d.ntot[i] = cnt;
+        d.mindist[i] = sqrt(fmin(mind, mind_cp));
+    }
     for (int off = 32; off > 0; off >>= 1) cnt = max(cnt, __shfl_xor(cnt, off, 64));
     if ((threadIdx.x & 63) == 0 && cnt > 0) {
         atomicMax(&sc->max_count, cnt);

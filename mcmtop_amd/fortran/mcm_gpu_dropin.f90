@@ -104,7 +104,7 @@ contains
     s%timestep = mcm_timestep
     s%kinetice = 0; s%internale = 0; s%totale = 0
     s%np = mcm_np; s%ngp = 0; s%max_count = 0; s%error_flags = 0; s%nummat = mcm_nummat
-    s%reserved = 0
+    s%skin_builds = 0; s%reserved = 0
   end subroutine
 
   ! the device run, made on first use from the deck the host read, with the

@@ -13,7 +13,8 @@ module mcm_gpu_iface
     integer(c_int32_t) :: device = 0
     integer(c_int32_t) :: corrected_one_pass = 0
     integer(c_int32_t) :: pair_mapping = 0
-    integer(c_int32_t) :: reserved(3) = 0
+    integer(c_int32_t) :: nbr_skin_milli = 0
+    integer(c_int32_t) :: reserved(2) = 0
   end type
 
   type, bind(c) :: mcm_gpu_scalars
@@ -22,7 +23,8 @@ module mcm_gpu_iface
     real(c_double) :: coord_min(3), coord_max(3)
     integer(c_int32_t) :: np, ngp, max_count, timestep
     integer(c_int32_t) :: error_flags, nummat
-    integer(c_int32_t) :: reserved(2)
+    integer(c_int32_t) :: skin_builds
+    integer(c_int32_t) :: reserved(1)
   end type
 
   type(c_ptr), save :: mcm_gpu_context = c_null_ptr

@@ -56,7 +56,8 @@ class McmGpuError(RuntimeError):
 class Options(ctypes.Structure):
     _fields_ = [("nbr_capacity", ctypes.c_int32), ("ghost_capacity", ctypes.c_int32),
                 ("device", ctypes.c_int32), ("corrected_one_pass", ctypes.c_int32),
-                ("pair_mapping", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
+                ("pair_mapping", ctypes.c_int32), ("nbr_skin_milli", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 2)]
 
 
 class Scalars(ctypes.Structure):
@@ -66,7 +67,12 @@ class Scalars(ctypes.Structure):
                 ("coord_min", ctypes.c_double * 3), ("coord_max", ctypes.c_double * 3),
                 ("np", ctypes.c_int32), ("ngp", ctypes.c_int32), ("max_count", ctypes.c_int32),
                 ("timestep", ctypes.c_int32), ("error_flags", ctypes.c_int32), ("nummat", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 2)]
+                ("skin_builds", ctypes.c_int32), ("reserved", ctypes.c_int32 * 1)]
+
+
+def _options(nbr_capacity, ghost_capacity, device, corrected_one_pass, pair_mapping, nbr_skin) -> Options:
+    return Options(nbr_capacity, ghost_capacity, device, int(bool(corrected_one_pass)), int(pair_mapping),
+                   int(round(nbr_skin * 1000)))
 
 
 _LIB = None
@@ -184,11 +190,13 @@ class Solver:
     mcmtop_amd/deck.py, or binary deck bytes / a Deck object."""
 
     def __init__(self, deck, nbr_capacity: int = 0, ghost_capacity: int = 0, device: int = 0,
-                 corrected_one_pass: bool = False, pair_mapping: int = 0):
-        """pair_mapping: 0 one thread per particle, 1 one wavefront per particle (same results)."""
+                 corrected_one_pass: bool = False, pair_mapping: int = 0, nbr_skin: float = 0.0):
+        """pair_mapping: 0 one thread per particle, 1 one wavefront per particle (same results).
+        nbr_skin: neighbour skin width as a fraction of the largest h (0: lists searched every step
+        in the reference's order; > 0: the same neighbour sets from skin lists, in another order)."""
         self._lib = load_library()
         self._owned = True
-        opt = Options(nbr_capacity, ghost_capacity, device, int(bool(corrected_one_pass)), int(pair_mapping))
+        opt = _options(nbr_capacity, ghost_capacity, device, corrected_one_pass, pair_mapping, nbr_skin)
         h = ctypes.c_void_p()
         if isinstance(deck, str):
             rc = self._lib.mcm_gpu_create_from_deck(deck.encode(), ctypes.byref(opt), ctypes.byref(h))
@@ -467,9 +475,9 @@ class Batch:
     same result as when stepped alone."""
 
     def __init__(self, decks=(), nbr_capacity: int = 0, ghost_capacity: int = 0, device: int = 0,
-                 corrected_one_pass: bool = False, pair_mapping: int = 0):
+                 corrected_one_pass: bool = False, pair_mapping: int = 0, nbr_skin: float = 0.0):
         self._lib = load_library()
-        opt = Options(nbr_capacity, ghost_capacity, device, int(bool(corrected_one_pass)), int(pair_mapping))
+        opt = _options(nbr_capacity, ghost_capacity, device, corrected_one_pass, pair_mapping, nbr_skin)
         h = ctypes.c_void_p()
         rc = self._lib.mcm_gpu_batch_create(ctypes.byref(opt), ctypes.byref(h))
         self._h = h
