@@ -64,7 +64,10 @@ typedef struct {
                                  3: particle in a ghost-layer cell, 4: EOS warning,
                                  5: penalty contact with ghost contact partners (undefined in
                                  the reference; the step stops with MCM_GPU_ERR_ARG),
-                                 6: skin list capacity */
+                                 6: skin list capacity,
+                                 7: the cap model's return did not converge in 100
+                                    iterations (the reference stops; the step stops
+                                    with MCM_GPU_ERR_MODEL) */
     int32_t nummat;
     int32_t skin_builds;      /* skin lists built so far (nbr_skin_milli > 0) */
     int32_t reserved[1];
@@ -206,6 +209,14 @@ int mcm_gpu_download(mcm_gpu_ctx *ctx, int32_t field, double *host, int64_t n);
 #define MCM_STATE_WORDS 86
 int mcm_gpu_download_state(mcm_gpu_ctx *ctx, double *rows, int64_t np, mcm_gpu_scalars *sc);
 int mcm_gpu_upload_state(mcm_gpu_ctx *ctx, const double *rows, int64_t np, const mcm_gpu_scalars *sc);
+/* The cap model's (material model 4, mcm_f3dm25.f90) state beside the
+   rows: par%capa and par%ptype of every real particle (mcm_database.f90:195-
+   196), capa[np] and ptype[np] (either may be NULL).  upload = 0 copies the
+   run's into the arrays; upload = 1 replaces the run's with them, at a step
+   boundary (a host that uploads rows of a run with a cap-model material
+   uploads these too; a run that has not stepped holds zeros).  Particles of
+   other models hold zeros. */
+int mcm_gpu_cap_state(mcm_gpu_ctx *ctx, double *capa, int32_t *ptype, int64_t np, int32_t upload);
 /* Neighbour lists in reference order: counts[np] (real incl. self),
    gcounts[np] (ghost), ids[np*cap] row-major with 0-based ids (ghosts as
    np + ghost index); returns the capacity used per row in *cap.  ids may be

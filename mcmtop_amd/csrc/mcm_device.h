@@ -177,6 +177,15 @@ struct DevState {
     double* red;               // per-block partial reductions
     int* range;                // RANGE_* bits: records outside the pair loops' fast-path ranges
     double* defps_prev;        // f3dm10 increment carried for failed particles
+    // the cap model (material model 4) writes only the upper triangle of the
+    // stress (mcm_f3dm25.f90:801-806): runs with a cap-model material (asym)
+    // keep the lower triangle too, and the momentum passes read all nine
+    // components
+    int asym;
+    double* capa;              // [np] cap parameter
+    int* ptype;                // [np] mode of the last cap-model update
+    double* sigl[3];           // [np] sigma 21, 31, 32 (the upper triangle's mirror for other models)
+    double* rml;               // [U][4] (sigma 21, 31, 32) / rho^2 of the momentum record, and a pad
     // batched launch (null for a launch over one run): workgroup -> instance
     // block, and the states of all instances
     const BlockEntry* bmap;

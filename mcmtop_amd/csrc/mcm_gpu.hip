@@ -137,6 +137,7 @@ void setup_material(MatParams& m) {
     if ((m.eos == 13 || m.eos == 28) && m.eosinput[3] == 0.0) m.eosinput[3] = 1.0;
     // mcm_calcg.f90:31-59
     if (m.model == 1 || m.model == 3) m.g = m.strinput[0] / (2.0 * (1.0 + m.strinput[5]));
+    else if (m.model == 4) m.g = m.strinput[5];
     else if (m.model == 9) m.g = 0.0;
     else if (m.model == 10) m.g = m.strinput[0];
 }
@@ -164,17 +165,24 @@ int finish_deck(mcm_gpu_ctx* ctx) {
             if (M.eos != 1 && M.eos != 4 && M.eos != 13 && M.eos != 28 && M.eos != 41) {
                 ctx->err = "unsupported equation of state"; return MCM_GPU_ERR_MODEL;
             }
-        } else if (M.model == 1 || M.model == 3) {
-            if (M.eos != 0) { ctx->err = "models 1/3 take no EOS (mcm_matin.f90:72-79)"; return MCM_GPU_ERR_MODEL; }
+        } else if (M.model == 1 || M.model == 3 || M.model == 4) {
+            if (M.eos != 0) { ctx->err = "models 1/3/4 take no EOS (mcm_matin.f90:72-79)"; return MCM_GPU_ERR_MODEL; }
+            // the cap model's mode tests are complete for soil (1) and rock (2)
+            // only; for another ltype the reference keeps the last call's mode
+            const int lt = (int)M.strinput[1 + 5 * 5];
+            if (M.model == 4 && lt != 1 && lt != 2) {
+                ctx->err = "cap model (material model 4): ltype (card 2 field 6) must be 1 (soil) or 2 (rock)";
+                return MCM_GPU_ERR_MODEL;
+            }
         } else {
-            // model 4 (f3dm25): the reference's yield test reads the local capa before
-            // any assignment (mcm_f3dm25.f90:205-206), so its result is undefined
-            ctx->err = M.model == 4 ? "material model 4 (f3dm25) has no defined result: the reference reads an "
-                                      "unset local in its yield test (mcm_f3dm25.f90:205-206)"
-                                    : "material model not supported on the GPU path (1, 3, 9, 10 are)";
+            ctx->err = "material model not supported on the GPU path (1, 3, 4, 9, 10 are)";
             return MCM_GPU_ERR_MODEL;
         }
         D.mat_raw.push_back(M);
+        if (M.model == 4 && cap_sets4(M.strinput)) {   // mcm_matin.f90:152-161 -> mcm_sets4
+            ctx->err = "cap model (material model 4): the initial cap iteration of mcm_sets4 does not converge";
+            return MCM_GPU_ERR_MODEL;
+        }
         setup_material(M);
         if (M.visc_type != 1) { ctx->err = "only nodal viscosity (visc_type 1) exists"; return MCM_GPU_ERR_MODEL; }
     }
@@ -592,6 +600,12 @@ int alloc_all(mcm_gpu_ctx* ctx) {
     }
     for (int k = 0; k < 3; k++) { rc |= dalloc(ctx, &d.spin[k], n); rc |= dalloc(ctx, &d.a[k], n); }
     rc |= dalloc(ctx, &d.yield, n);
+    if (d.asym) {   // the cap model's state and the stress's lower triangle
+        rc |= dalloc(ctx, &d.capa, n);
+        rc |= dalloc(ctx, &d.ptype, n);
+        for (int k = 0; k < 3; k++) rc |= dalloc(ctx, &d.sigl[k], n);
+        rc |= dalloc(ctx, &d.rml, 4 * u);
+    }
     rc |= dalloc(ctx, &d.cell_of, n); rc |= dalloc(ctx, &d.slot_of, n); rc |= dalloc(ctx, &d.sorted, n);
     rc |= dalloc(ctx, &d.unsorted, n);
     rc |= dalloc(ctx, &d.cand, 4 * n);
@@ -655,7 +669,8 @@ int corr_variant(const mcm_gpu_ctx* ctx) { return ctx->d.disctype == 1 ? (ctx->c
 // with penalty contact or XSPH take the thread-per-particle kernels
 bool wave_capable(const mcm_gpu_ctx* ctx) { return ctx->d.contacttype == 0 && ctx->d.veloc_opt == 0; }
 bool use_wave(const mcm_gpu_ctx* ctx) {
-    return ctx->pair_mapping == 1 && wave_capable(ctx) && ctx->d.ncap <= 64 * WAVE_SLOTS && corr_variant(ctx) != 2;
+    return ctx->pair_mapping == 1 && wave_capable(ctx) && ctx->d.ncap <= 64 * WAVE_SLOTS && corr_variant(ctx) != 2 &&
+           !ctx->d.asym;
 }
 
 // list-build kernel of a run: 0 one material, 1 several, 2 wavefront per
@@ -670,14 +685,20 @@ int pair_variant(const mcm_gpu_ctx* c) { return corr_variant(c) + (use_wave(c) ?
 
 // momentum kernel of a run: as the strain pass, or 5 + corr_variant when
 // XSPH moves the particles after the velocity update
-int mom_variant(const mcm_gpu_ctx* c) { return c->d.veloc_opt == 1 ? 5 + corr_variant(c) : pair_variant(c); }
+// (runs with a cap-model material: 8 + corr_variant, or 11 + corr_variant
+// with XSPH, the kernels that read the stress's lower triangle)
+int mom_variant(const mcm_gpu_ctx* c) {
+    if (c->d.asym) return (c->d.veloc_opt == 1 ? 11 : 8) + corr_variant(c);
+    return c->d.veloc_opt == 1 ? 5 + corr_variant(c) : pair_variant(c);
+}
 
 // What a launch of the step spans, per run.  kind_blocks gives the
 // workgroups one run needs in a launch of that kind and kernel variant v;
 // 0 means the run takes no part in it.
 enum LaunchKind { L_ONE, L_PART, L_CELLS, L_TILES, L_GSCAN, L_GFILL, L_GSTRESS, L_LIST, L_PAIR, L_MOM, L_CFOLD,
                   L_REP, L_XSPH, L_SKIN, L_SKINP, L_SKINB, L_NKIND };
-constexpr int kNVariant = MCM_MAX_MAT + 1;
+constexpr int kNVariant = 16;
+static_assert(kNVariant > MCM_MAX_MAT && kNVariant > 13, "launch variants");
 
 int kind_blocks(const mcm_gpu_ctx* c, int kind, int v) {
     const DevState& d = c->d;
@@ -835,6 +856,12 @@ void launch_momentum_kernel(const Target& T, int cur) {
     launch(T, L_MOM, 5, MCM_BLOCK, k_momentum<MODE_FVEL, 0, false>, cur);
     launch(T, L_MOM, 6, MCM_BLOCK, k_momentum<MODE_FVEL, 1, false>, cur);
     launch(T, L_MOM, 7, MCM_BLOCK, k_momentum<MODE_FVEL, 2, false>, cur);
+    launch(T, L_MOM, 8, MCM_BLOCK, k_momentum<MODE_FUSED, 4, false>, cur);
+    launch(T, L_MOM, 9, MCM_BLOCK, k_momentum<MODE_FUSED, 5, false>, cur);
+    launch(T, L_MOM, 10, MCM_BLOCK, k_momentum<MODE_FUSED, 6, false>, cur);
+    launch(T, L_MOM, 11, MCM_BLOCK, k_momentum<MODE_FVEL, 4, false>, cur);
+    launch(T, L_MOM, 12, MCM_BLOCK, k_momentum<MODE_FVEL, 5, false>, cur);
+    launch(T, L_MOM, 13, MCM_BLOCK, k_momentum<MODE_FVEL, 6, false>, cur);
     launch(T, L_XSPH, 0, MCM_BLOCK, k_xsph, cur);
     launch(T, L_XSPH, 0, MCM_BLOCK, k_xsph_move, cur);
 }
@@ -914,6 +941,11 @@ int regrow(mcm_gpu_ctx* ctx, const Scalars& sc_host) {
                    "is undefined there (mcm_get_j_cont_info.f90:44-48 reads the real contact list past its end)";
         return MCM_GPU_ERR_ARG;
     }
+    if (flags & 128) {
+        ctx->err = "cap model (material model 4): the iterated return of mode 10 or 11 did not reach |f| <= 1e-8 in "
+                   "100 iterations (the reference stops, mcm_f3dm25.f90:977-983, 1031-1037)";
+        return MCM_GPU_ERR_MODEL;
+    }
     drop_graph(ctx);
     HIPCHK(hipStreamSynchronize(ctx->stream));
     if (flags & 2) {   // ghosts: unified arrays grow, real part kept
@@ -928,6 +960,7 @@ int regrow(mcm_gpu_ctx* ctx, const Scalars& sc_host) {
         rc |= dregrow(ctx, &d.pmat, u, keep);
         rc |= dregrow(ctx, &d.rs, 2 * RS_SLOTS * u, 2 * RS_SLOTS * keep);
         rc |= dregrow(ctx, &d.rm, 2 * RM_SLOTS * u, 2 * RM_SLOTS * keep);
+        if (d.asym) rc |= dregrow(ctx, &d.rml, 4 * u, 4 * keep);
         rc |= dregrow(ctx, &d.gpar, ng_new, 0);
         rc |= dregrow(ctx, &d.gsgn, ng_new, 0);
         if (rc) return rc;
@@ -966,10 +999,11 @@ void init_particle(PStateT<T>& s, const M& Mt, const HostDeck& D, long i, double
     s.rho = D.init_rhoe == 1 ? D.pr[i] : 0.0;
     s.e = D.init_rhoe == 1 ? D.pe[i] : 0.0;
     s.pcut = pcut;
-    if (Mt.model == 1 || Mt.model == 3) {
+    if (Mt.model == 1 || Mt.model == 3 || Mt.model == 4) {
         if (D.init_rhoe != 1) { s.rho = s.rho0; s.e = 0.0; }
         s.p = 0.0;
-        s.c = sqrt(Mt.strinput[1] / s.rho);
+        // mcm_calcrhopc.f90:44-67: lambda + 2G, model 4's from mcm_sets4
+        s.c = sqrt(Mt.strinput[Mt.model == 4 ? 9 : 1] / s.rho);
     } else {
         const auto* ei = Mt.eosinput;
         if (D.init_rhoe != 1) {
@@ -1021,6 +1055,9 @@ int gather_rows(mcm_gpu_ctx* ctx, const Scalars& s, std::vector<double>& rows, s
     auto tr = get(d.tracerod, np), pcut = get(d.pcut, np), p_cut = get(d.p_cut, np), efps = get(d.efps, np);
     auto fail = get(d.fail, np), mind = get(d.mindist, np), crit = get(d.critts_p, np), rho0 = get(d.rho0, np);
     auto h0 = get(d.h0, np), q = get(d.q, np);
+    std::vector<double> SL[3];   // the stress's lower triangle of a run with a cap-model material
+    if (d.asym)
+        for (int k = 0; k < 3; k++) SL[k] = get(d.sigl[k], np);
     if (err) { ctx->err = std::string("state download: ") + hipGetErrorString(err); return MCM_GPU_ERR_HIP; }
     static const int map[9] = {S11, S12, S13, S12, S22, S23, S13, S23, S33};
     rows.assign(np * MCM_STATE_WORDS, 0.0);
@@ -1037,6 +1074,7 @@ int gather_rows(mcm_gpu_ctx* ctx, const Scalars& s, std::vector<double>& rows, s
             row[55 + k] = (k == 0 || k == 4 || k == 8) ? q[i] : 0.0;
             row[64 + k] = ROD[m][i];
         }
+        if (d.asym) { row[28 + 1] = SL[0][i]; row[28 + 2] = SL[1][i]; row[28 + 5] = SL[2][i]; }
         // spin: column-major (r,c) -> W12 at (0,1)=k3, W13 at (0,2)=k6, W23 at (1,2)=k7
         double w12 = SP[0][i], w13 = SP[1][i], w23 = SP[2][i];
         double sp[9] = {0.0, -w12, -w13, w12, 0.0, -w23, w13, w23, 0.0};
@@ -1083,6 +1121,7 @@ int scatter_rows(mcm_gpu_ctx* ctx, const double* rows, const mcm_gpu_scalars& sc
         put(d.sig[k], 28 + upper[k]); put(d.sd[k], 37 + upper[k]); put(d.alfa[k], 46 + upper[k]);
         put(d.rod[k], 64 + upper[k]);
     }
+    if (d.asym) { put(d.sigl[0], 28 + 1); put(d.sigl[1], 28 + 2); put(d.sigl[2], 28 + 5); }   // 21 31 32
     put(d.q, 55);                                          // q is isotropic: q(1,1)
     put(d.spin[0], 73 + 3); put(d.spin[1], 73 + 6); put(d.spin[2], 73 + 7);   // W12 W13 W23
     put(d.h0, 82);
@@ -1309,6 +1348,8 @@ int setup_run(mcm_gpu_ctx* ctx, const mcm_gpu_options* opt) {
     d.pi = acos(-1.0);
     d.ngcap = (opt && opt->ghost_capacity > 0) ? opt->ghost_capacity : D.np / 2 + 4096;
     d.ncap = (opt && opt->nbr_capacity > 0) ? opt->nbr_capacity : 128;
+    d.asym = 0;
+    for (int m = 0; m < D.nummat; m++) d.asym |= D.mat[m].model == 4 ? 1 : 0;
     // boundary planes (mcm_inp_symmetry_v2.f90:83-137, mcm_init_boundary.f90:28-116)
     d.boundary = 0;
     d.nplanes = 0;
@@ -1676,9 +1717,12 @@ int mcm_gpu_momentum(mcm_gpu_ctx* ctx) {
     if (rc) return rc;
     DevState& d = ctx->d;
     if (d.boundary) k_ghost_stress<<<ghost_blocks(d), MCM_BLOCK, 0, ctx->stream>>>(d, ctx->cur);
-    switch (corr_variant(ctx)) {
+    switch (corr_variant(ctx) + (d.asym ? 4 : 0)) {
         case 1: k_momentum<MODE_MOMENTUM, 1, true><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, ctx->cur); break;
         case 2: k_momentum<MODE_MOMENTUM, 2, true><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, ctx->cur); break;
+        case 4: k_momentum<MODE_MOMENTUM, 4, true><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, ctx->cur); break;
+        case 5: k_momentum<MODE_MOMENTUM, 5, true><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, ctx->cur); break;
+        case 6: k_momentum<MODE_MOMENTUM, 6, true><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, ctx->cur); break;
         default: k_momentum<MODE_MOMENTUM, 0, true><<<ctx->nb, MCM_BLOCK, 0, ctx->stream>>>(d, ctx->cur); break;
     }
     HIPCHK(hipGetLastError());
@@ -1973,6 +2017,15 @@ int mcm_gpu_dump(mcm_gpu_ctx* ctx, const char* path, int32_t step, int32_t stage
         double z = 0.0;
         for (size_t g = 0; g < ng; g++) fwrite(&z, 8, 1, f);
     }
+    if (d.asym) {   // cap-model section: 'CAP4', capa(np), ptype(np)
+        std::vector<double> capa(np);
+        (void)hipMemcpy(capa.data(), d.capa, np * sizeof(double), hipMemcpyDeviceToHost);
+        auto ptype = geti(d.ptype, np);
+        int32_t tag = 0x43415034;
+        fwrite(&tag, 4, 1, f);
+        fwrite(capa.data(), 8, np, f);
+        fwrite(ptype.data(), 4, np, f);
+    }
     fclose(f);
     return 0;
 }
@@ -2010,6 +2063,34 @@ int mcm_gpu_upload_state(mcm_gpu_ctx* ctx, const double* rows, int64_t np, const
         return MCM_GPU_ERR_ARG;
     }
     return scatter_rows(ctx, rows, *sc);
+}
+
+int mcm_gpu_cap_state(mcm_gpu_ctx* ctx, double* capa, int32_t* ptype, int64_t np, int32_t upload) {
+    if (!ctx || (upload != 0 && upload != 1)) return MCM_GPU_ERR_ARG;
+    DevState& d = ctx->d;
+    if (np != d.np) {
+        ctx->err = "mcm_gpu_cap_state: np must be the run's particle count";
+        return MCM_GPU_ERR_ARG;
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (upload) {
+        int rc = stage_check(ctx, "mcm_gpu_cap_state", ctx->phase == 0 && ctx->done == 0,
+                             "the state is replaced at a step boundary");
+        if (rc) return rc;
+    }
+    if (!d.asym) {   // no cap-model material: zeros out, nothing to take in
+        if (!upload) {
+            if (capa) std::fill(capa, capa + np, 0.0);
+            if (ptype) std::fill(ptype, ptype + np, 0);
+        }
+        return MCM_GPU_OK;
+    }
+    const hipMemcpyKind kind = upload ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
+    if (capa) HIPCHK(upload ? hipMemcpy(d.capa, capa, np * sizeof(double), kind)
+                            : hipMemcpy(capa, d.capa, np * sizeof(double), kind));
+    if (ptype) HIPCHK(upload ? hipMemcpy(d.ptype, ptype, np * sizeof(int32_t), kind)
+                             : hipMemcpy(ptype, d.ptype, np * sizeof(int32_t), kind));
+    return MCM_GPU_OK;
 }
 
 int mcm_gpu_time_kernels(mcm_gpu_ctx* ctx, int32_t nsteps, double* ms, int32_t n) {
@@ -2063,7 +2144,13 @@ int mcm_gpu_write_vtk(mcm_gpu_ctx* ctx, const char* path, int32_t material) {
     if (rc) return rc;
     std::vector<int32_t> mat(np);
     for (size_t i = 0; i < np; i++) mat[i] = ctx->deck.pmat[i] + 1;
-    rc = mcm_vtk_write(path, (int64_t)np, f.data(), mat.data(), material);
+    std::vector<int32_t> ptype;
+    if (ctx->d.asym) {
+        ptype.resize(np);
+        HIPCHK(hipMemcpy(ptype.data(), ctx->d.ptype, np * sizeof(int32_t), hipMemcpyDeviceToHost));
+    }
+    rc = vtkout::write_vtk(path, (long)np, f.data(), mat.data(), material, ptype.empty() ? nullptr : ptype.data()) == 0
+             ? MCM_GPU_OK : MCM_GPU_ERR_IO;
     if (rc) ctx->err = std::string("cannot write ") + path;
     return rc;
 }

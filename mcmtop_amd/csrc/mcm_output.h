@@ -12,9 +12,10 @@
 //     I1..I5 (mcm_VTK.f90:56-85, 213-215) and its cell lines with I6, which
 //     overflow to asterisks from 100,000 (counts) and 1,000,000 (cell ids)
 //     particles and make the file unreadable;
-//   - the per-material file's PTYPE block is 0 for every particle: ptype is
-//     only set by the f3dm25 cap model, which this build does not run, and is
-//     otherwise never initialised (mcm_database.f90:196).
+//   - the per-material file's PTYPE block holds the cap model's mode of the
+//     last update (mcm_f3dm25.f90 sets par%ptype) and 0 for particles of other
+//     models, where the reference leaves ptype uninitialised
+//     (mcm_database.f90:196).
 // List-directed lines ("POINTS n double", "CELLS n 2n") are written with
 // single blanks; VTK readers split on white space.
 #pragma once
@@ -90,8 +91,10 @@ enum { OX = 0, OV = 3, OP = 6, OS11 = 7, OS22 = 8, OS33 = 9, OS12 = 10, OS13 = 1
 
 // Writes the VTK file of the particles whose 1-based material equals
 // material (material 0: every particle, the layout of WriteVTK; material > 0:
-// WriteVTKmat, with its PTYPE block).  mat may be null when material is 0.
-inline int write_vtk(const char* path, long np, const double* f, const int32_t* mat, int material) {
+// WriteVTKmat, with its PTYPE block).  mat may be null when material is 0;
+// ptype (the cap model's mode per particle) may be null: all zero.
+inline int write_vtk(const char* path, long np, const double* f, const int32_t* mat, int material,
+                     const int32_t* ptype = nullptr) {
     FILE* out = fopen(path, "w");
     if (!out) return -1;
     auto take = [&](long i) { return material == 0 || (mat && mat[i] == material); };
@@ -143,7 +146,8 @@ inline int write_vtk(const char* path, long np, const double* f, const int32_t* 
     if (material > 0) {
         put("SCALARS PTYPE double 1");
         put("LOOKUP_TABLE default");
-        for (long k = 0; k < n; k++) put(fmt_i(0, 5));
+        for (long i = 0; i < np; i++)
+            if (take(i)) put(fmt_i(ptype ? ptype[i] : 0, 5));
     }
     return fclose(out) == 0 ? 0 : -1;
 }

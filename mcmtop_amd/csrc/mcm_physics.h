@@ -942,3 +942,5 @@ MCM_HD void eos_p(PStateT<T>& s, const M& m) {
     }
 }
 #undef CM5
+
+#include "mcm_cap.h"

@@ -94,7 +94,11 @@ __device__ __forceinline__ void load_state(const DevState& d, int i, int cur, PS
 // passes rm[u] = {x, y, z, h, m, m/rho, A11, A22, A33, A12, A13, A23}.  A
 // pass loads only the planes it uses (the others are dead loads).
 struct SRec { double x, y, z, vx, vy, vz, vol, h; };
-struct MRec { double x, y, z, h, m, vol, A[6]; };
+// ASYM: runs with a cap-model material, whose stress is not symmetric; the
+// record then also holds A21, A31, A32 from rml[u]
+template <bool ASYM> struct MRecT { double x, y, z, h, m, vol, A[6]; };
+template <> struct MRecT<true> { double x, y, z, h, m, vol, A[6], AL[3]; };
+using MRec = MRecT<false>;
 
 __device__ __forceinline__ SRec load_srec(const double* rs, int j) {
     const double2* r = reinterpret_cast<const double2*>(rs) + RS_SLOTS * (long)j;
@@ -106,12 +110,32 @@ __device__ __forceinline__ MRec load_mrec(const double* rm, int j) {
     const double2 a = r[0], b = r[1], c = r[2], e = r[3], f = r[4], h = r[5];
     return MRec{a.x, a.y, b.x, b.y, c.x, c.y, {e.x, e.y, f.x, f.y, h.x, h.y}};
 }
+__device__ __forceinline__ MRecT<true> load_mreca(const double* rm, const double* rml, int j) {
+    const MRec a = load_mrec(rm, j);
+    const double2* l = reinterpret_cast<const double2*>(rml) + 2 * (long)j;
+    const double2 p = l[0], q = l[1];
+    return MRecT<true>{a.x, a.y, a.z, a.h, a.m, a.vol, {a.A[0], a.A[1], a.A[2], a.A[3], a.A[4], a.A[5]},
+                       {p.x, p.y, q.x}};
+}
 __device__ __forceinline__ SRec load_srec(const DevState& d, int j) { return load_srec(d.rs, j); }
 __device__ __forceinline__ MRec load_mrec(const DevState& d, int j) { return load_mrec(d.rm, j); }
+template <bool ASYM> __device__ __forceinline__ MRecT<ASYM> load_mrec_t(const DevState& d, int j);
+template <> __device__ __forceinline__ MRec load_mrec_t<false>(const DevState& d, int j) { return load_mrec(d.rm, j); }
+template <> __device__ __forceinline__ MRecT<true> load_mrec_t<true>(const DevState& d, int j) {
+    return load_mreca(d.rm, d.rml, j);
+}
 
-template <typename Rec> __device__ __forceinline__ Rec load_rec(const double* base, int j);
-template <> __device__ __forceinline__ SRec load_rec<SRec>(const double* base, int j) { return load_srec(base, j); }
-template <> __device__ __forceinline__ MRec load_rec<MRec>(const double* base, int j) { return load_mrec(base, j); }
+// records by type; base2 is the second array of a two-array record (rml)
+template <typename Rec> __device__ __forceinline__ Rec load_rec(const double* base, const double* base2, int j);
+template <> __device__ __forceinline__ SRec load_rec<SRec>(const double* base, const double*, int j) {
+    return load_srec(base, j);
+}
+template <> __device__ __forceinline__ MRec load_rec<MRec>(const double* base, const double*, int j) {
+    return load_mrec(base, j);
+}
+template <> __device__ __forceinline__ MRecT<true> load_rec<MRecT<true>>(const double* base, const double* base2, int j) {
+    return load_mreca(base, base2, j);
+}
 
 // Walks one particle's list in order with the next entry's record loaded
 // one entry ahead and its index two entries ahead, so that the gathers of a
@@ -121,22 +145,23 @@ template <typename Rec>
 struct NbrStream {
     const int* lst;
     const double* base;
+    const double* base2;
     int n, k, self, jn, jnn;
     Rec rn;
     // list slots are read unconditionally (slots 0..ncap-1 are allocated),
     // values past the end are replaced by self before use
-    __device__ __forceinline__ NbrStream(const int* l, const double* b, int n_, int self_)
-        : lst(l), base(b), n(n_), k(0), self(self_) {
+    __device__ __forceinline__ NbrStream(const int* l, const double* b, int n_, int self_, const double* b2 = nullptr)
+        : lst(l), base(b), base2(b2), n(n_), k(0), self(self_) {
         const int l0 = lst[0], l1 = lst[n > 1 ? 64 : 0];
         jn = n > 0 ? l0 : self;
-        rn = load_rec<Rec>(base, jn);
+        rn = load_rec<Rec>(base, base2, jn);
         jnn = n > 1 ? l1 : self;
     }
     __device__ __forceinline__ void next(int& j, Rec& r) {
         j = jn;
         r = rn;
         jn = jnn;
-        rn = load_rec<Rec>(base, jn);
+        rn = load_rec<Rec>(base, base2, jn);
         const int kk = k + 2 < n ? k + 2 : k;
         const int v = lst[(long)kk * 64];
         jnn = k + 2 < n ? v : self;
@@ -333,7 +358,31 @@ __device__ __forceinline__ void constitutive_particle(const DevState& d, int i, 
     if (m.model == 3) for (int k = 0; k < 6; k++) s.alfa[k] = d.alfa[k][i];
     if (m.model == 9) for (int k = 0; k < 6; k++) s.s[k] = d.sd[k][i];
     double defps_prev = d.defps_prev[i];
-    int active = constitutive_one(s, m, mi + 1, dt, defps_prev, &eos_err);
+    int active = 0;
+    // lower triangle of sigma: 21, 31, 32
+    double sl[3];
+    if (m.model == 4) {
+        // mcm_constitutive.f90:49-54: stress_rot1, f3dm25, aviscosity, lieupd.
+        // The rotation reads sigma 12, 23 and 31 (mcm_stress_rot1.f90:33-41,
+        // 163-171) and writes a symmetric tensor; the cap model then writes
+        // the upper triangle only
+        double capa = d.capa[i];
+        s.sig[S13] = d.sigl[1][i];
+        stress_rot1(s, dt);
+        sl[0] = s.sig[S12]; sl[1] = s.sig[S13]; sl[2] = s.sig[S23];
+        int mode = f3dm25(s, cap_const(m), dt, capa);
+        if (mode < 0) {
+            eos_err |= 2;
+            mode = 0;
+        }
+        nodalav(s, m);
+        lieupd(s, dt);
+        d.capa[i] = capa;
+        d.ptype[i] = mode;
+    } else {
+        active = constitutive_one(s, m, mi + 1, dt, defps_prev, &eos_err);
+        sl[0] = s.sig[S12]; sl[1] = s.sig[S13]; sl[2] = s.sig[S23];
+    }
     s.critts = critts(s, m, d.tcrit_opt, d.mindist[i]);
     crit = s.critts;
     // write back
@@ -364,6 +413,12 @@ __device__ __forceinline__ void constitutive_particle(const DevState& d, int i, 
     *rm_at(d, 3, i) = make_double2((s.sig[S11] - s.q) / r2, (s.sig[S22] - s.q) / r2);
     *rm_at(d, 4, i) = make_double2((s.sig[S33] - s.q) / r2, (s.sig[S12] - 0.0) / r2);
     *rm_at(d, 5, i) = make_double2((s.sig[S13] - 0.0) / r2, (s.sig[S23] - 0.0) / r2);
+    if (d.asym) {
+        for (int k = 0; k < 3; k++) d.sigl[k][i] = sl[k];
+        double2* l = reinterpret_cast<double2*>(d.rml) + 2 * (long)i;
+        l[0] = make_double2((sl[0] - 0.0) / r2, (sl[1] - 0.0) / r2);
+        l[1] = make_double2((sl[2] - 0.0) / r2, 0.0);
+    }
     if (DIAG) {
         for (int k = 0; k < 6; k++) d.rod[k][i] = s.rod[k];
         for (int k = 0; k < 3; k++) d.spin[k][i] = s.spin[k];
@@ -441,7 +496,11 @@ __device__ void reduce_c_partials(const DevState& d, const CPart& c, bool with_b
 // the block's partials (folded by k_strain_fold, before the momentum pass
 // writes its coordinate bounds there).  Every thread of the block calls it.
 __device__ __forceinline__ void strain_block_partials(const DevState& d, int lb, double crit, double hm, int eos_err) {
-    if (eos_err) atomicOr(&d.sc->error_flags, 16);
+    if (eos_err & 1) atomicOr(&d.sc->error_flags, 16);
+    if (eos_err & 2) {   // the cap model's iterated return did not converge (the reference stops)
+        atomicOr(&d.sc->error_flags, 128);
+        atomicExch(&d.sc->abort, 1);
+    }
     double cb = block_reduce<1>(crit);
     double hb = block_reduce<2>(hm);
     if (threadIdx.x == 0) {
@@ -619,12 +678,40 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_ghost_stress(DevState d_, int cur
         *rm_at(d, 3, u) = make_double2((d.sig[S11][p] - q) / r2, (d.sig[S22][p] - q) / r2);
         *rm_at(d, 4, u) = make_double2((d.sig[S33][p] - q) / r2, -d.sig[S12][p] / r2);
         *rm_at(d, 5, u) = make_double2(-d.sig[S13][p] / r2, -d.sig[S23][p] / r2);
+        if (d.asym) {   // the lower triangle, mirrored the same way
+            double2* l = reinterpret_cast<double2*>(d.rml) + 2 * (long)u;
+            l[0] = make_double2(-d.sigl[0][p] / r2, -d.sigl[1][p] / r2);
+            l[1] = make_double2(-d.sigl[2][p] / r2, 0.0);
+        }
     }
 }
 
 // ------------------------------------------------------------- momentum
 __device__ __forceinline__ int sym_index(int r, int c) {
     return r == c ? r : ((r + c == 1) ? S12 : (r + c == 2 ? S13 : S23));
+}
+
+// (sigma - q)/rho^2 of a record at (row, col); below the diagonal the
+// lower triangle of a record that has one (21, 31, 32 in AL)
+template <bool ASYM>
+__device__ __forceinline__ double acomp(const MRecT<ASYM>& r, int row, int col) {
+    if constexpr (ASYM) {
+        if (row > col) return r.AL[row + col - 1];
+    }
+    return r.A[sym_index(row, col)];
+}
+// components held by the corrected sums: the 6 of a symmetric tensor, or
+// all 9 (row * 3 + col)
+template <bool ASYM> constexpr int kNComp = ASYM ? 9 : 6;
+template <bool ASYM>
+__device__ __forceinline__ double ccomp(const MRecT<ASYM>& r, int c) {
+    if constexpr (ASYM) return acomp<true>(r, c / 3, c % 3);
+    else return r.A[c];
+}
+template <bool ASYM>
+__device__ __forceinline__ int cindex(int row, int col) {
+    if constexpr (ASYM) return row * 3 + col;
+    else return sym_index(row, col);
 }
 
 // mcm_oldacceleration.f90:60-89
@@ -634,26 +721,23 @@ __device__ __forceinline__ double accel_start(const DevState& d, int i, int k) {
     return d.contacttype > 0 ? 0.0 + d.rep[k][i] : 0.0;
 }
 
-template <bool FAST>
-__device__ __forceinline__ void accel_plain(const DevState& d, int i, int t, const MRec& me, double acc[3]) {
+template <bool FAST, bool ASYM>
+__device__ __forceinline__ void accel_plain(const DevState& d, int i, int t, const MRecT<ASYM>& me, double acc[3]) {
     const int n = d.ntot[i];
     const int* lst = nbr_row(d, t);
     acc[0] = accel_start(d, i, 0); acc[1] = accel_start(d, i, 1); acc[2] = accel_start(d, i, 2);
-    NbrStream<MRec> st(lst, d.rm, n, i);
+    NbrStream<MRecT<ASYM>> st(lst, d.rm, n, i, d.rml);
 #pragma unroll 2
     for (int k = 0; k < n; k++) {
         int j;
-        MRec r;
+        MRecT<ASYM> r;
         st.next(j, r);
         double havg = 0.5 * (me.h + r.h);
         double dwdx[3];
         pair_grad<FAST>(me.x - r.x, me.y - r.y, me.z - r.z, havg, d.pi, dwdx);
         for (int nn = 0; nn < 3; nn++) {
             double ds = 0.0;
-            for (int mm = 0; mm < 3; mm++) {
-                int c = sym_index(mm, nn);
-                ds = ds + (r.A[c] + me.A[c]) * dwdx[mm];
-            }
+            for (int mm = 0; mm < 3; mm++) ds = ds + (acomp(r, mm, nn) + acomp(me, mm, nn)) * dwdx[mm];
             acc[nn] = acc[nn] + r.m * ds;
         }
     }
@@ -661,8 +745,8 @@ __device__ __forceinline__ void accel_plain(const DevState& d, int i, int t, con
 
 // kernel value and gradient centred at the neighbour: W(x_j, x_i),
 // grad_j W (mcm_corr_acceleration.f90:73-74)
-template <bool FAST>
-__device__ __forceinline__ void corr_pair_j(const MRec& r, const MRec& me, double pi, double& w, double dwdx[3]) {
+template <bool FAST, class R>
+__device__ __forceinline__ void corr_pair_j(const R& r, const R& me, double pi, double& w, double dwdx[3]) {
     double havg = 0.5 * (me.h + r.h);
     pair_wgrad<FAST>(r.x - me.x, r.y - me.y, r.z - me.z, havg, pi, w, dwdx);
 }
@@ -670,17 +754,17 @@ __device__ __forceinline__ void corr_pair_j(const MRec& r, const MRec& me, doubl
 // mcm_corr_acceleration.f90:37-163 in the reference's three passes: the self
 // slot (k == nnbr) contributes nothing to deltasig, the mass of the last
 // neighbour multiplies the sum and B^-1 enters transposed
-template <bool FAST>
-__device__ __forceinline__ void accel_corr(const DevState& d, int i, int t, const MRec& me, double acc[3]) {
+template <bool FAST, bool ASYM>
+__device__ __forceinline__ void accel_corr(const DevState& d, int i, int t, const MRecT<ASYM>& me, double acc[3]) {
     const int n = d.ntot[i];
     const int self_k = d.nreal[i] - 1;
     const int* lst = nbr_row(d, t);
     double sw = 0.0, sg[3] = {0.0, 0.0, 0.0}, massj = 0.0;
     int j;
-    NbrStream<MRec> s1(lst, d.rm, n, i);
+    NbrStream<MRecT<ASYM>> s1(lst, d.rm, n, i, d.rml);
 #pragma unroll 2
     for (int k = 0; k < n; k++) {
-        MRec r;
+        MRecT<ASYM> r;
         s1.next(j, r);
         double w, dwdx[3];
         corr_pair_j<FAST>(r, me, d.pi, w, dwdx);
@@ -692,10 +776,10 @@ __device__ __forceinline__ void accel_corr(const DevState& d, int i, int t, cons
     const Divisor isw2(sw2);
     const bool unchecked = FAST && __all(fast_sums(sw, sg));
     double B[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, invB[9];
-    NbrStream<MRec> s2(lst, d.rm, n, i);
+    NbrStream<MRecT<ASYM>> s2(lst, d.rm, n, i, d.rml);
 #pragma unroll 2
     for (int k = 0; k < n; k++) {
-        MRec r;
+        MRecT<ASYM> r;
         s2.next(j, r);
         double w, dwdx[3], cg[3];
         corr_pair_j<FAST>(r, me, d.pi, w, dwdx);
@@ -705,19 +789,20 @@ __device__ __forceinline__ void accel_corr(const DevState& d, int i, int t, cons
             for (int nn = 0; nn < 3; nn++) B[nn + 3 * m] = B[nn + 3 * m] + r.vol * dx[nn] * cg[m];
     }
     inverse3(B, invB);
-    // deltasig(m,n,l) is symmetric in (m,n): 6 x 3 accumulators
-    double D[6][3] = {{0.0}};
-    NbrStream<MRec> s3(lst, d.rm, n, i);
+    // deltasig(m,n,l) is symmetric in (m,n) unless the stress is not: 6 or
+    // 9 components x 3 accumulators
+    double D[kNComp<ASYM>][3] = {{0.0}};
+    NbrStream<MRecT<ASYM>> s3(lst, d.rm, n, i, d.rml);
 #pragma unroll 2
     for (int k = 0; k < n; k++) {
-        MRec r;
+        MRecT<ASYM> r;
         s3.next(j, r);
         if (k == self_k) continue;   // dsig = 0 there
         double w, dwdx[3], cg[3];
         corr_pair_j<FAST>(r, me, d.pi, w, dwdx);
         corr_grad(isw2, unchecked, sw, sg, w, dwdx, cg);
-        for (int c = 0; c < 6; c++) {
-            double dsig = r.A[c] + me.A[c];
+        for (int c = 0; c < kNComp<ASYM>; c++) {
+            double dsig = ccomp(r, c) + ccomp(me, c);
             D[c][0] = D[c][0] + dsig * cg[0];
             D[c][1] = D[c][1] + dsig * cg[1];
             D[c][2] = D[c][2] + dsig * cg[2];
@@ -726,7 +811,7 @@ __device__ __forceinline__ void accel_corr(const DevState& d, int i, int t, cons
     for (int nn = 0; nn < 3; nn++) {
         double a = accel_start(d, i, nn);
         for (int mm = 0; mm < 3; mm++) {
-            int c = sym_index(nn, mm);
+            int c = cindex<ASYM>(nn, mm);
             for (int l = 0; l < 3; l++) a = a + massj * D[c][l] * invB[l + 3 * mm];
         }
         acc[nn] = a;
@@ -734,18 +819,18 @@ __device__ __forceinline__ void accel_corr(const DevState& d, int i, int t, cons
 }
 
 // corrected acceleration in one pass from single-pass moments (fast variant)
-template <bool FAST>
-__device__ __forceinline__ void accel_corr_fast(const DevState& d, int i, int t, const MRec& me, double acc[3]) {
+template <bool FAST, bool ASYM>
+__device__ __forceinline__ void accel_corr_fast(const DevState& d, int i, int t, const MRecT<ASYM>& me, double acc[3]) {
     const int n = d.ntot[i];
     const int self_k = d.nreal[i] - 1;
     const int* lst = nbr_row(d, t);
     double sw = 0.0, sg[3] = {0, 0, 0}, P[3][3] = {{0}}, Pw[3] = {0, 0, 0};
-    double E1[6][3] = {{0}}, E2[6] = {0, 0, 0, 0, 0, 0};
+    double E1[kNComp<ASYM>][3] = {{0}}, E2[kNComp<ASYM>] = {0};
     double mlast = 0.0;
     int j;
-    NbrStream<MRec> st(lst, d.rm, n, i);
+    NbrStream<MRecT<ASYM>> st(lst, d.rm, n, i, d.rml);
     for (int k = 0; k < n; k++) {
-        MRec r;
+        MRecT<ASYM> r;
         st.next(j, r);
         double w, dwdx[3];
         corr_pair_j<FAST>(r, me, d.pi, w, dwdx);
@@ -758,8 +843,8 @@ __device__ __forceinline__ void accel_corr_fast(const DevState& d, int i, int t,
             for (int b = 0; b < 3; b++) P[a][b] += dx[a] * (r.vol * dwdx[b]);
         }
         if (k != self_k) {
-            for (int c = 0; c < 6; c++) {
-                double ds = r.A[c] + me.A[c];
+            for (int c = 0; c < kNComp<ASYM>; c++) {
+                double ds = ccomp(r, c) + ccomp(me, c);
                 E2[c] += ds * w;
                 E1[c][0] += ds * dwdx[0];
                 E1[c][1] += ds * dwdx[1];
@@ -776,7 +861,7 @@ __device__ __forceinline__ void accel_corr_fast(const DevState& d, int i, int t,
     for (int nn = 0; nn < 3; nn++) {
         double sacc = 0.0;
         for (int mm = 0; mm < 3; mm++) {
-            int c = sym_index(nn, mm);
+            int c = cindex<ASYM>(nn, mm);
             for (int l = 0; l < 3; l++) sacc += (E1[c][l] * sw - E2[c] * sg[l]) / sw2 * invB[l + 3 * mm];
         }
         acc[nn] = accel_start(d, i, nn) + mlast * sacc;
@@ -898,9 +983,10 @@ __device__ __forceinline__ void integrate_particle(const DevState& d, int i, dou
         for (int a = 0; a < 3; a++) d.a[a][i] = acc[a];
 }
 
-// Thread-per-particle momentum kernel (CORR as k_strain_const).
+// Thread-per-particle momentum kernel (CORR & 3 as k_strain_const's CORR;
+// CORR & 4: runs with a cap-model material, whose stress is not symmetric).
 template <int MODE, int CORR, bool DIAG>
-__global__ __launch_bounds__(MCM_BLOCK) __attribute__((amdgpu_waves_per_eu(CORR == 2 ? MCM_WPE_MOM1 : MCM_WPE_MOM3))) void k_momentum(
+__global__ __launch_bounds__(MCM_BLOCK) __attribute__((amdgpu_waves_per_eu((CORR & 3) == 2 ? MCM_WPE_MOM1 : MCM_WPE_MOM3))) void k_momentum(
     DevState d_, int cur) {
     Blk bk;
     const DevState& d = enter<true>(d_, bk);
@@ -912,22 +998,24 @@ __global__ __launch_bounds__(MCM_BLOCK) __attribute__((amdgpu_waves_per_eu(CORR 
     const int i = live ? d.sorted[t] : t;
     CPart cp;
     init_c_part(cp);
+    constexpr bool ASYM = (CORR & 4) != 0;
+    constexpr int DISC = CORR & 3;
     if (live) {
-        const MRec me = load_mrec(d, i);
+        const MRecT<ASYM> me = load_mrec_t<ASYM>(d, i);
         const double xi[3] = {me.x, me.y, me.z};
         double acc[3];
         // records within the fast-path ranges (flag written by the search,
         // the strain pass and the constitutive pass)
         const bool fast = (d.range[0] & RANGE_RM) == 0;
-        if (CORR == 1) {
-            if (fast) accel_corr<true>(d, i, t, me, acc);
-            else accel_corr<false>(d, i, t, me, acc);
-        } else if (CORR == 2) {
-            if (fast) accel_corr_fast<true>(d, i, t, me, acc);
-            else accel_corr_fast<false>(d, i, t, me, acc);
+        if (DISC == 1) {
+            if (fast) accel_corr<true, ASYM>(d, i, t, me, acc);
+            else accel_corr<false, ASYM>(d, i, t, me, acc);
+        } else if (DISC == 2) {
+            if (fast) accel_corr_fast<true, ASYM>(d, i, t, me, acc);
+            else accel_corr_fast<false, ASYM>(d, i, t, me, acc);
         } else {
-            if (fast) accel_plain<true>(d, i, t, me, acc);
-            else accel_plain<false>(d, i, t, me, acc);
+            if (fast) accel_plain<true, ASYM>(d, i, t, me, acc);
+            else accel_plain<false, ASYM>(d, i, t, me, acc);
         }
         if (MODE == MODE_MOMENTUM) {
             for (int a = 0; a < 3; a++) d.a[a][i] = acc[a];

@@ -442,6 +442,22 @@ def f3dm10(G, sy, Eh, pcut=0.0, a1=0.0, a2=0.0, spall=0.0, epf=0.0):
     return {0: G, 1: sy, 2: Eh, 3: pcut, 4: a1, 5: a2, 6: spall, 7: epf}
 
 
+def f3dm25(K, G, alpha, theta, gamma, beta, R, D, W, X0, Xint, ltype=1, tcut=0.0, vec=0.0, falfac=0.0):
+    """model 4 (the cap model, mcm_f3dm25.f90) property cards as
+    cm(5, 6): bulk and shear modulus, the failure surface
+    Fe = alpha - gamma exp(-beta J1) + theta J1, cap aspect ratio R, the
+    hardening law W (1 - exp(D (X0 - X))), the initial cap position Xint
+    (mcm_sets4.f90 solves the initial cap parameter from it), soil (ltype 1)
+    or rock (2), the tension cut-off tcut; vec != 0 selects the iterated
+    (rather than the four-step) return of modes 10 and 11; falfac lowers
+    alpha.  Stresses compression positive, in the deck's stress unit."""
+    def cm(a, b):
+        return (a - 1) + 5 * (b - 1)
+    return {cm(1, 1): K, cm(1, 2): G, cm(1, 3): alpha, cm(1, 4): theta, cm(1, 5): gamma, cm(1, 6): beta,
+            cm(2, 1): R, cm(2, 2): D, cm(2, 3): W, cm(2, 4): X0, cm(2, 5): Xint, cm(2, 6): float(ltype),
+            cm(3, 1): tcut, cm(3, 5): vec, cm(4, 1): falfac}
+
+
 def block(dx, nx, ny, nz, origin=(0.0, 0.0, 0.0)):
     """simple-cubic lattice block, nodes at origin + (i+1/2) dx"""
     i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")

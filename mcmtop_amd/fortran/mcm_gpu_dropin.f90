@@ -120,6 +120,30 @@ contains
     call pack_par(rows)
     call host_scalars(s)
     call mcm_gpu_check(mcm_gpu_upload_state(mcm_gpu_context, rows, int(mcm_np, c_int64_t), s), 'upload_state')
+    call cap_state(1_c_int32_t)
+  end subroutine
+
+  ! par%capa and par%ptype (the cap model's, mcm_database.f90:195-196)
+  ! between host and device: upload 1 to the device, 0 from it
+  subroutine cap_state(upload)
+    integer(c_int32_t), intent(in) :: upload
+    real(c_double), allocatable :: capa(:)
+    integer(c_int32_t), allocatable :: ptype(:)
+    integer :: i
+    allocate(capa(mcm_np), ptype(mcm_np))
+    if (upload == 1) then
+      do i = 1, mcm_np
+        capa(i) = par(i)%capa
+        ptype(i) = par(i)%ptype
+      end do
+    end if
+    call mcm_gpu_check(mcm_gpu_cap_state(mcm_gpu_context, capa, ptype, int(mcm_np, c_int64_t), upload), 'cap_state')
+    if (upload == 0) then
+      do i = 1, mcm_np
+        par(i)%capa = capa(i)
+        par(i)%ptype = ptype(i)
+      end do
+    end if
   end subroutine
 
   ! the host's time step (mcm_solution.f90:58-62) to the device
@@ -167,6 +191,7 @@ contains
     call mcm_gpu_check(mcm_gpu_download_state(mcm_gpu_context, rows, int(mcm_np, c_int64_t), c_null_ptr), &
                        'download_state')
     call unpack_par(rows)
+    call cap_state(0_c_int32_t)
     call mcm_gpu_check(mcm_gpu_download_counts(mcm_gpu_context, nn, gn, c_null_ptr, 0_c_int64_t, cap), &
                        'download_neighbours')
     do i = 1, mcm_np

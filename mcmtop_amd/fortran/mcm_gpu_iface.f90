@@ -93,6 +93,15 @@ module mcm_gpu_iface
       integer(c_int64_t), value :: np
       type(mcm_gpu_scalars), intent(in) :: s
     end function
+    ! the cap model's par%capa and par%ptype (upload 0: from the device)
+    integer(c_int) function mcm_gpu_cap_state(ctx, capa, ptype, np, upload) bind(c, name='mcm_gpu_cap_state')
+      import :: c_int, c_ptr, c_double, c_int32_t, c_int64_t
+      type(c_ptr), value :: ctx
+      real(c_double), dimension(*), intent(inout) :: capa
+      integer(c_int32_t), dimension(*), intent(inout) :: ptype
+      integer(c_int64_t), value :: np
+      integer(c_int32_t), value :: upload
+    end function
     integer(c_int) function mcm_gpu_download_state(ctx, rows, np, s) bind(c, name='mcm_gpu_download_state')
       import :: c_int, c_ptr, c_double, c_int64_t
       type(c_ptr), value :: ctx

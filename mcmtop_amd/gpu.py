@@ -39,7 +39,7 @@ SYMBOLS = (
     "mcm_gpu_set_material_params", "mcm_gpu_get_material_params", "mcm_gpu_set_initial_velocity",
     "mcm_gpu_adj_forward", "mcm_gpu_adj_backward", "mcm_gpu_adj_last_error", "mcm_gpu_adj_destroy",
     "mcm_gpu_set_time", "mcm_gpu_download_state", "mcm_gpu_upload_state", "mcm_gpu_get_energies",
-    "mcm_gpu_calctotale_noalloc",
+    "mcm_gpu_calctotale_noalloc", "mcm_gpu_cap_state",
 )
 
 STATE_WORDS = 86   # row of the per-particle state (include/mcm_gpu.h, the dump row layout)
@@ -138,6 +138,7 @@ def load_library(path: str = LIB_PATH):
     lib.mcm_gpu_adj_destroy.argtypes = [vp]
     lib.mcm_gpu_adj_destroy.restype = None
     lib.mcm_gpu_set_time.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+    lib.mcm_gpu_cap_state.argtypes = [vp, dp, i32p, ctypes.c_int64, ctypes.c_int32]
     lib.mcm_gpu_download_state.argtypes = [vp, dp, ctypes.c_int64, ctypes.POINTER(Scalars)]
     lib.mcm_gpu_upload_state.argtypes = [vp, dp, ctypes.c_int64, ctypes.POINTER(Scalars)]
     lib.mcm_gpu_get_energies.argtypes = [vp, dp, ctypes.c_int32]
@@ -300,6 +301,22 @@ class Solver:
         r = np.ascontiguousarray(rows, dtype=np.float64)
         self._check(self._lib.mcm_gpu_upload_state(self._h, r.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                                                    r.shape[0], ctypes.byref(sc)), "upload_state")
+
+    def cap_state(self):
+        """(capa[np], ptype[np]): the cap model's state (zeros without a cap-model material)."""
+        n = self.scalars().np
+        capa, ptype = np.zeros(n), np.zeros(n, np.int32)
+        self._check(self._lib.mcm_gpu_cap_state(self._h, capa.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                ptype.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), n, 0),
+                    "cap_state")
+        return capa, ptype
+
+    def upload_cap_state(self, capa, ptype):
+        c = np.ascontiguousarray(capa, dtype=np.float64)
+        t = np.ascontiguousarray(ptype, dtype=np.int32)
+        self._check(self._lib.mcm_gpu_cap_state(self._h, c.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                t.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), len(c), 1),
+                    "cap_state")
 
     def step(self, n: int = 1):
         """n fused steps (device-side dt, captured hipGraph)."""
@@ -511,32 +528,6 @@ class Batch:
 
     def initial(self):
         self._check(self._lib.mcm_gpu_batch_initial(self._h), "batch_initial")
-
-    def energies(self) -> np.ndarray:
-        """(nummat, 5): ke, ie, mom(3) per material from the last calctotale or fused step."""
-        nm = self.scalars().nummat
-        out = np.zeros(5 * nm)
-        self._check(self._lib.mcm_gpu_get_energies(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 5 * nm),
-                    "get_energies")
-        return out.reshape(nm, 5)
-
-    def set_time(self, dt: float, dtold: float, ptime: float):
-        """The host's dt, dtold and ptime (mcm_solution.f90:58-62 on the host)."""
-        self._check(self._lib.mcm_gpu_set_time(self._h, float(dt), float(dtold), float(ptime)), "set_time")
-
-    def download_state(self):
-        """(rows[np, STATE_WORDS], Scalars): the state as the host's particle records hold it."""
-        n = self.scalars().np
-        rows = np.zeros((n, STATE_WORDS))
-        s = Scalars()
-        self._check(self._lib.mcm_gpu_download_state(self._h, rows.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
-                                                     n, ctypes.byref(s)), "download_state")
-        return rows, s
-
-    def upload_state(self, rows, sc: Scalars):
-        r = np.ascontiguousarray(rows, dtype=np.float64)
-        self._check(self._lib.mcm_gpu_upload_state(self._h, r.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
-                                                   r.shape[0], ctypes.byref(sc)), "upload_state")
 
     def step(self, n: int = 1):
         self._check(self._lib.mcm_gpu_batch_step(self._h, int(n)), "batch_step")

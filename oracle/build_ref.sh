@@ -52,11 +52,17 @@ mcm_rep_contact mcm_get_j_cont_info
 mcm_gttxsg mcm_matin mcm_nodein mcm_nodehist mcm_Inputvel mcm_inp_symmetry_v2 mcm_contactype mcm_baseaccel_in
 mcm_termin mcm_transducers_in"
 OBJS=""
+# mcm_f3dm25.f90 (the cap model) reads locals it never assigns (capa at
+# :205-206, blk and davg at :182); it is compiled as the reference's Debug
+# configuration compiles every file (MCM.vfproj: /Qsave), where locals are
+# static and zero until assigned: flang's -fno-automatic.
 for f in $FILES; do
   src="$REF/$f.f90"
   obj="$OUT/obj/$f.o"
-  if [ ! -f "$obj" ] || [ "$src" -nt "$obj" ]; then
-    "$FC" -O2 -fPIC -module-dir "$OUT/mod" -c "$src" -o "$obj"
+  extra=""
+  if [ "$f" = "mcm_f3dm25" ]; then extra="-fno-automatic"; fi
+  if [ ! -f "$obj" ] || [ "$src" -nt "$obj" ] || [ "$0" -nt "$obj" ]; then
+    "$FC" -O2 -fPIC $extra -module-dir "$OUT/mod" -c "$src" -o "$obj"
   fi
   OBJS="$OBJS $obj"
 done

@@ -5,7 +5,8 @@ Layout (little-endian): see the comment above ``probe_dump`` in ref_deck.f90;
 ``probe_dump_rows`` writes the same file up to the particle rows.
 Particle rows hold NF=86 doubles; the column names are ``FIELDS`` below.
 Neighbour ids are 1-based as in the reference (real ids into the particle
-array, ghost ids into the ghost array).
+array, ghost ids into the ghost array).  Decks with a cap-model material
+(model 4) end with 'CAP4', capa(np) and ptype(np).
 """
 from __future__ import annotations
 
@@ -13,6 +14,7 @@ import numpy as np
 
 DUMP_MAGIC = int.from_bytes(b"MCMD", "big")
 ROWS_MAGIC = int.from_bytes(b"MCMR", "big")   # state rows only (probe_dump_rows in ref_deck.f90)
+CAP_TAG = int.from_bytes(b"CAP4", "big")      # cap-model section: capa(np), ptype(np)
 NF = 86
 
 FIELDS = {
@@ -58,7 +60,8 @@ class Dump:
         self.nbr_off = np.concatenate([[0], np.cumsum(self.nnbr)])
         self.nbr = flat
         self.gnbr_off = np.concatenate([[0], np.cumsum(self.g_nnbr)])
-        if off < len(buf):
+        self.capa = self.ptype = None
+        if off < len(buf) and int(np.frombuffer(buf, "<i4", 1, off)[0]) != CAP_TAG:
             gt = int(self.g_nnbr.sum())
             self.gnbr = np.frombuffer(buf, "<i4", gt, off).copy(); off += 4 * gt
             k = self.ngp
@@ -71,6 +74,10 @@ class Dump:
         else:
             self.gnbr = np.zeros(0, np.int32)
             self.g_par = np.zeros(0, np.int32)
+        if off < len(buf) and int(np.frombuffer(buf, "<i4", 1, off)[0]) == CAP_TAG:
+            off += 4
+            self.capa = np.frombuffer(buf, "<f8", n, off).copy(); off += 8 * n
+            self.ptype = np.frombuffer(buf, "<i4", n, off).copy(); off += 4 * n
 
     def f(self, name: str) -> np.ndarray:
         return self.rows[:, FIELDS[name]]

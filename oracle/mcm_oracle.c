@@ -55,6 +55,8 @@ static void alloc_particles(orc_state *st) {
                      &st->p_cut, &st->efps, &st->fail, &st->mindist, &st->critts_p};
     for (size_t k = 0; k < sizeof(s1) / sizeof(s1[0]); k++) *s1[k] = xcalloc(n, 8);
     st->rep = xcalloc(3 * n, 8);
+    st->capa = xcalloc(n, 8);
+    st->ptype = xcalloc(n, sizeof(int));
     double **s9[] = {&st->sigma, &st->s, &st->alfa, &st->q, &st->qold, &st->rod, &st->spin};
     for (size_t k = 0; k < sizeof(s9) / sizeof(s9[0]); k++) *s9[k] = xcalloc(9 * n, 8);
     st->nnbr = xcalloc(n, sizeof(int)); st->g_nnbr = xcalloc(n, sizeof(int));
@@ -80,7 +82,7 @@ void orc_free(orc_state *st) {
                     st->g_nnbr, st->ncont, st->g_ncont, st->nbr, st->cont, st->gnbr, st->gcont, st->g_par,
                     st->g_mat, st->g_ll, st->g_x, st->g_xzero, st->g_v, st->g_mass, st->g_h, st->g_hold,
                     st->g_rho, st->g_sigma, st->g_q, st->mat, st->mat_ke, st->mat_ie, st->mat_mom,
-                    st->llgrid, st->rep};
+                    st->llgrid, st->rep, st->capa, st->ptype};
     for (size_t k = 0; k < sizeof(ptrs) / sizeof(ptrs[0]); k++) free(ptrs[k]);
     free(st);
 }
@@ -131,6 +133,63 @@ static void sets10(double *cm) {
         if (prop[i + 15 - 1] < prop[i + 16 - 1]) cm[16 - 1] = (double)i;
 }
 
+/* initel in mcm_sets4.f90:43-94 (implicit double precision): the initial cap
+   parameter kappa of the cap model from the initial cap position xint, and
+   the tension cut-off fcut, by Newton iteration with the single-precision
+   tolerance 1.0e-3.  The reference warns and keeps iterating past maxit; here
+   a run that needs more than 100000 iterations is rejected. */
+static int initel(double xint, double alpha, double gama, double beta, double theta, double r, double *capint_out,
+                  double *fcut_out) {
+    const double tol = (double)1.0e-3f, tiny = (double)1.e-15f;
+    double capint = 0.0, delcap = 1.e10, delf = 1.e10, fcut = 0.0;
+    for (long it = 0;; it++) {
+        if (it > 100000) return -1;
+        double fe = alpha - gama * exp(-(beta * capint)) + theta * capint;
+        double psik = xint - capint - r * fe;
+        double err1 = fabs(psik);
+        double err2 = fabs(delcap) / (fabs(capint) + tiny);
+        if (!(err1 > tol && err2 > tol)) break;
+        double fep = beta * gama * exp(-(beta * capint)) + theta;
+        double dpsik = -1.0 - r * fep;
+        delcap = psik / dpsik;
+        capint = capint - delcap;
+    }
+    for (long it = 0;; it++) {
+        if (it > 100000) return -1;
+        double fe = alpha - gama * exp(-(beta * fcut)) + theta * fcut;
+        double err1 = fabs(fe);
+        double err2 = fabs(delf) / (fabs(fcut) + tiny);
+        if (!(err1 > tol && err2 > tol)) break;
+        double fep = gama * beta * exp(-(beta * fcut)) + theta;
+        delf = fe / fep;
+        fcut = fcut - delf;
+    }
+    *capint_out = capint;
+    *fcut_out = fcut;
+    return 0;
+}
+
+/* mcm_sets4.f90:19-41 (model 4, the cap model): lambda + 2G from the bulk and
+   shear moduli in cm(5,2), the initial cap parameter in cm(3,3), the tension
+   cut-off in cm(3,4).  Only the second tempNi of the reference is used. */
+static int sets4(double *cm) {
+#define C(a, b) cm[((a) - 1) + 5 * ((b) - 1)]
+    double alpha = C(1, 3), theta = C(1, 4), gama = C(1, 5), beta = C(1, 6), r = C(2, 1);
+    double xint = C(2, 5);
+    double tempE = (9.0 * C(1, 1) * C(1, 2)) / (3.0 * C(1, 1) + C(1, 2));
+    double tempNi = (3.0 * C(1, 1) - tempE) / (6.0 * C(1, 1));
+    double q1 = tempE * tempNi / ((1.0 + tempNi) * (1.0 - 2.0 * tempNi));
+    double q2 = tempE * 0.5 / (1.0 + tempNi);
+    double q3 = q1 + 2.0 * q2;
+    C(5, 2) = q3;
+    double capint, fcut;
+    if (initel(xint, alpha, gama, beta, theta, r, &capint, &fcut)) return -1;
+    C(3, 3) = capint;
+    C(3, 4) = fcut;
+    return 0;
+#undef C
+}
+
 /* ------------------------------------------------------------------ deck */
 #define DECK_MAGIC 0x4D434D31
 #define CONTACT_TAG 0x434F4E54   /* 'CONT': penalty contact section after the particles */
@@ -177,6 +236,13 @@ orc_state *orc_load_deck(const char *path, int ncap, int gcap) {
         switch (mt->model) {   /* mcm_matin.f90:152-161 */
             case 1: sets1(mt->strinput); break;
             case 3: sets3(mt->strinput); break;
+            case 4: {
+                /* the cap model's modes are complete for soil (1) and rock (2)
+                   only; another ltype leaves the mode of the previous call */
+                int lt = (int)mt->strinput[((2) - 1) + 5 * ((6) - 1)];
+                if ((lt != 1 && lt != 2) || sets4(mt->strinput)) set_err(st, ORC_ERR_MODEL);
+                break;
+            }
             case 10: sets10(mt->strinput); break;
             case 9: break;
             default: set_err(st, ORC_ERR_MODEL); break;
@@ -749,7 +815,9 @@ static void stress_rot1(orc_state *st, int i) {
     Q[0][1] = Q[0][1] - bwzz; Q[1][0] = Q[1][0] + bwzz;
     Q[0][2] = Q[0][2] + bwyy; Q[2][0] = Q[2][0] - bwyy;
     Q[1][2] = Q[1][2] - bwxx; Q[2][1] = Q[2][1] + bwxx;
-    /* straux is symmetric; the reference reads (3,1) for (1,3) etc. */
+    /* the reference reads sigma (1,2), (2,3) and (3,1): for the cap model,
+       whose update writes the upper triangle only, (3,1) is the last
+       rotation's, not the model's (1,3) */
     double s11 = sa[0][0], s22 = sa[1][1], s33 = sa[2][2], s12 = sa[0][1], s23 = sa[1][2], s31 = sa[2][0];
     double qs[3][3];
     for (int r = 0; r < 3; r++) {
@@ -961,6 +1029,413 @@ static void f3dm10(orc_state *st, int i, const double *cm) {
         SD(2, 1) = SD(1, 2); SD(3, 1) = SD(1, 3); SD(3, 2) = SD(2, 3);
         st->p_cut[i] = st->pcut[i];
     }
+}
+
+/* ------------------------------------------------------------ cap model
+   mcm_f3dm25.f90 (material model 4, the cap model of granular material).
+   f3dm25a, feit and fcit have no implicit none: their undeclared locals are
+   REAL*4 (a-h, o-z) or INTEGER (i-n), and the declared ones REAL*8.  Every
+   operation below is carried out in the wider kind of its two operands and
+   every assignment rounds to its target's kind, as Fortran does; float
+   literals are the reference's single-precision literals.  Single-precision
+   exp is the C library's expf, the routine flang calls for it.
+   The reference's Debug configuration compiles with /Qsave, so these locals
+   are static and zero before their first assignment: the local capa that
+   the yield test reads (:205-206) is never assigned and so is 0; blk and
+   davg (:182) are never assigned either, and their product is unused. */
+static float fmaxF(float a, float b) { return a > b ? a : b; }
+static float fminF(float a, float b) { return a < b ? a : b; }
+
+/* feit, mcm_f3dm25.f90:926-983: failure-surface iterations (mode 10, vector
+   option 0); returns 0 when converged, -1 after maxit iterations (the
+   reference stops) */
+static int feit(double capan, float xj1tr, float xj2dtr, float alpha, float gama, float beta, float theta, float w,
+                float dDyna, float r, double bulk, float shear, float *xj1_out, double *capa, double *sig) {
+    const float tol = 1.0e-8f;
+    const int maxit = 100;
+    int iter = 1;
+    *capa = capan;
+    float xj1 = xj1tr, xj2d = xj2dtr;
+    float s1 = (float)(sig[0] - (double)(xj1tr / 3.0f));
+    float s2 = (float)(sig[1] - (double)(xj1tr / 3.0f));
+    float s3 = (float)(sig[2] - (double)(xj1tr / 3.0f));
+    float s4 = (float)sig[3], s5 = (float)sig[4], s6 = (float)sig[5];
+    for (;;) {
+        float xk = (float)(*capa + (double)r * ((double)alpha - (double)gama * exp(-((double)beta * *capa)) +
+                                                (double)theta * *capa));
+        float dhdk = (float)((double)(w * dDyna * expf(-(dDyna * xk))) *
+                             (1.0 + (double)r * ((double)theta + (double)(gama * beta) * exp(-((double)beta * *capa)))));
+        float omega = theta + gama * beta * expf(-(beta * xj1));
+        float dcd = (float)((double)shear + 9.0 * bulk * (double)(omega * omega));
+        float f = xj2d - (alpha - gama * expf(-(beta * xj1)) + theta * xj1);
+        float delcap = -(3.0f * omega * f / (dcd * dhdk));
+        float delam = -(dhdk * delcap / (3.0f * omega));
+        *capa = *capa + (double)delcap;
+        float fac1 = shear / xj2d;
+        float fac2 = (float)(3.0 * bulk * (double)omega);
+        sig[0] = sig[0] - (double)(delam * (fac1 * s1 - fac2));
+        sig[1] = sig[1] - (double)(delam * (fac1 * s2 - fac2));
+        sig[2] = sig[2] - (double)(delam * (fac1 * s3 - fac2));
+        sig[3] = sig[3] - (double)(delam * fac1 * s4);
+        sig[4] = sig[4] - (double)(delam * fac1 * s5);
+        sig[5] = sig[5] - (double)(delam * fac1 * s6);
+        xj1 = (float)(sig[0] + sig[1] + sig[2]);
+        s1 = (float)(sig[0] - (double)(xj1 / 3.0f));
+        s2 = (float)(sig[1] - (double)(xj1 / 3.0f));
+        s3 = (float)(sig[2] - (double)(xj1 / 3.0f));
+        s4 = (float)sig[3]; s5 = (float)sig[4]; s6 = (float)sig[5];
+        xj2d = sqrtf(0.5f * (s1 * s1 + s2 * s2 + s3 * s3) + s4 * s4 + s5 * s5 + s6 * s6);
+        f = xj2d - (alpha - gama * expf(-(beta * xj1)) + theta * xj1);
+        *xj1_out = xj1;
+        if (fabsf(f) <= tol) return 0;
+        if (iter < maxit) iter = iter + 1;
+        else return -1;
+    }
+}
+
+/* fcit, mcm_f3dm25.f90:985-1037: cap-surface iterations (mode 11, vector
+   option 0) */
+static int fcit(double capan, float xj1tr, float xj2dtr, float alpha, float gama, float beta, float theta, float w,
+                float dDyna, float r, double bulk, float shear, float *xj1_out, double *capa, double *sig) {
+    const float tol = 1.0e-8f;
+    const int maxit = 100;
+    const float unit = 1.0f;
+    int iter = 1;
+    *capa = capan;
+    float xj1 = xj1tr, xj2d = xj2dtr;
+    float s1 = (float)(sig[0] - (double)(xj1tr / 3.0f));
+    float s2 = (float)(sig[1] - (double)(xj1tr / 3.0f));
+    float s3 = (float)(sig[2] - (double)(xj1tr / 3.0f));
+    float s4 = (float)sig[3], s5 = (float)sig[4], s6 = (float)sig[5];
+    for (;;) {
+        float xk = (float)(*capa + (double)r * ((double)alpha - (double)gama * exp(-((double)beta * *capa)) +
+                                                (double)theta * *capa));
+        float dhdk = (float)((double)(w * dDyna * expf(-(dDyna * xk))) *
+                             (1.0 + (double)r * ((double)theta + (double)(gama * beta) * exp(-((double)beta * *capa)))));
+        float dfdk;
+        if (*capa >= 0.0)
+            dfdk = (float)(-2.0 * (fabs((double)xj1 - *capa) / (double)(r * r) +
+                                   ((double)alpha - (double)gama * exp(-((double)beta * *capa)) + (double)theta * *capa) *
+                                       ((double)theta + (double)(gama * beta) * exp(-((double)beta * *capa)))));
+        else
+            dfdk = (float)((double)(-2.0f * xk) *
+                           (1.0 + (double)r * ((double)theta + (double)(gama * beta) * exp(-((double)beta * *capa)))) /
+                           (double)(r * r));
+        float elcap = (float)fmax_(*capa, (double)(0.0f * unit));
+        float omega = fminF(-2.0f * fabsf(xj1 - elcap) / (r * r), -1.e-15f * unit);
+        float t2 = 2.0f * (xj1 - elcap) / (r * r);
+        float dcd = (float)((double)(4.0f * shear * (xj2d * xj2d)) + 9.0 * bulk * (double)(t2 * t2));
+        float f = xj2d * xj2d + (xj1 - elcap) * (xj1 - elcap) / (r * r) - (xk - elcap) * (xk - elcap) / (r * r);
+        float delcap = -f / (dfdk + dcd * dhdk / (3.0f * omega));
+        float delam = -(dhdk * delcap / (3.0f * omega));
+        *capa = *capa + (double)delcap;
+        /* xk again at the new cap: the convergence test below reads it */
+        xk = (float)(*capa + (double)r * ((double)alpha - (double)gama * exp(-((double)beta * *capa)) +
+                                          (double)theta * *capa));
+        float fac1 = 2.0f * shear;
+        float fac2 = (float)(6.0 * bulk * (double)fabsf(xj1 - elcap) / (double)(r * r));
+        sig[0] = sig[0] - (double)(delam * (fac1 * s1 + fac2));
+        sig[1] = sig[1] - (double)(delam * (fac1 * s2 + fac2));
+        sig[2] = sig[2] - (double)(delam * (fac1 * s3 + fac2));
+        sig[3] = sig[3] - (double)(delam * fac1 * s4);
+        sig[4] = sig[4] - (double)(delam * fac1 * s5);
+        sig[5] = sig[5] - (double)(delam * fac1 * s6);
+        xj1 = (float)(sig[0] + sig[1] + sig[2]);
+        s1 = (float)(sig[0] - (double)(xj1 / 3.0f));
+        s2 = (float)(sig[1] - (double)(xj1 / 3.0f));
+        s3 = (float)(sig[2] - (double)(xj1 / 3.0f));
+        s4 = (float)sig[3]; s5 = (float)sig[4]; s6 = (float)sig[5];
+        xj2d = sqrtf(0.5f * (s1 * s1 + s2 * s2 + s3 * s3) + s4 * s4 + s5 * s5 + s6 * s6);
+        f = xj2d * xj2d + (xj1 - elcap) * (xj1 - elcap) / (r * r) - (xk - elcap) * (xk - elcap) / (r * r);
+        *xj1_out = xj1;
+        if (fabsf(f) <= tol) {
+            if (xj1 < elcap) {
+                xj1 = elcap + fabsf(xj1 - elcap);
+                sig[0] = (double)(s1 + xj1 / 3.0f);
+                sig[1] = (double)(s2 + xj1 / 3.0f);
+                sig[2] = (double)(s3 + xj1 / 3.0f);
+                *xj1_out = xj1;
+            }
+            return 0;
+        }
+        if (iter < maxit) iter = iter + 1;
+        else return -1;
+    }
+}
+
+/* one unrolled failure-surface iteration of f3dm25a's mode 10 with vector
+   option 1 (mcm_f3dm25.f90:489-516, repeated at :517-541, :545-570,
+   :571-596); the same statements as feit's loop body */
+static void cap_fail_step(double *capa, double *sig, float *x1, float *x2d, float s[6], float alpha, float gama,
+                          float beta, float theta, float w, float dDyna, float r, double bulk, float shear) {
+    float xk = (float)(*capa + (double)r * ((double)alpha - (double)gama * exp(-((double)beta * *capa)) +
+                                            (double)theta * *capa));
+    float dhdk = (float)((double)(w * dDyna * expf(-(dDyna * xk))) *
+                         (1.0 + (double)r * ((double)theta + (double)(gama * beta) * exp(-((double)beta * *capa)))));
+    float omega = theta + gama * beta * expf(-(beta * *x1));
+    float dcd = (float)((double)shear + 9.0 * bulk * (double)(omega * omega));
+    float f = *x2d - (alpha - gama * expf(-(beta * *x1)) + theta * *x1);
+    float delcap = -(3.0f * omega * f / (dcd * dhdk));
+    float delam = -(dhdk * delcap / (3.0f * omega));
+    *capa = *capa + (double)delcap;
+    float fac1 = shear / *x2d;
+    float fac2 = (float)(3.0 * bulk * (double)omega);
+    for (int k = 0; k < 3; k++) sig[k] = sig[k] - (double)(delam * (fac1 * s[k] - fac2));
+    for (int k = 3; k < 6; k++) sig[k] = sig[k] - (double)(delam * fac1 * s[k]);
+    *x1 = (float)(sig[0] + sig[1] + sig[2]);
+    for (int k = 0; k < 3; k++) s[k] = (float)(sig[k] - (double)(*x1 / 3.0f));
+    for (int k = 3; k < 6; k++) s[k] = (float)sig[k];
+    *x2d = sqrtf(0.5f * (s[0] * s[0] + s[1] * s[1] + s[2] * s[2]) + s[3] * s[3] + s[4] * s[4] + s[5] * s[5]);
+}
+
+/* one unrolled cap-surface iteration of mode 11 with vector option 1
+   (mcm_f3dm25.f90:663-692, repeated at :697-726, :731-760, :765-794); the
+   same statements as fcit's loop body */
+static void cap_cap_step(double *capa, double *sig, float *x1, float *x2d, float s[6], float *elcap_out, float alpha,
+                         float gama, float beta, float theta, float w, float dDyna, float r, double bulk, float shear) {
+    const float unit = 1.0f;
+    float xk = (float)(*capa + (double)r * ((double)alpha - (double)gama * exp(-((double)beta * *capa)) +
+                                            (double)theta * *capa));
+    float dhdk = (float)((double)(w * dDyna * expf(-(dDyna * xk))) *
+                         (1.0 + (double)r * ((double)theta + (double)(gama * beta) * exp(-((double)beta * *capa)))));
+    float dfdk;
+    if (*capa >= 0.0)
+        dfdk = (float)(-2.0 * (fabs((double)*x1 - *capa) / (double)(r * r) +
+                               ((double)alpha - (double)gama * exp(-((double)beta * *capa)) + (double)theta * *capa) *
+                                   ((double)theta + (double)(gama * beta) * exp(-((double)beta * *capa)))));
+    else
+        dfdk = (float)((double)(-2.0f * xk) *
+                       (1.0 + (double)r * ((double)theta + (double)(gama * beta) * exp(-((double)beta * *capa)))) /
+                       (double)(r * r));
+    float elcap = (float)fmax_(*capa, (double)(0.0f * unit));
+    float omega = fminF(-2.0f * fabsf(*x1 - elcap) / (r * r), -1.e-15f * unit);
+    float t2 = 2.0f * (*x1 - elcap) / (r * r);
+    float dcd = (float)((double)(4.0f * shear * (*x2d * *x2d)) + 9.0 * bulk * (double)(t2 * t2));
+    float f = *x2d * *x2d + (*x1 - elcap) * (*x1 - elcap) / (r * r) - (xk - elcap) * (xk - elcap) / (r * r);
+    float delcap = -f / (dfdk + dcd * dhdk / (3.0f * omega));
+    float delam = -(dhdk * delcap / (3.0f * omega));
+    *capa = *capa + (double)delcap;
+    float fac1 = 2.0f * shear;
+    float fac2 = (float)(6.0 * bulk * (double)fabsf(*x1 - elcap) / (double)(r * r));
+    for (int k = 0; k < 3; k++) sig[k] = sig[k] - (double)(delam * (fac1 * s[k] + fac2));
+    for (int k = 3; k < 6; k++) sig[k] = sig[k] - (double)(delam * fac1 * s[k]);
+    *x1 = (float)(sig[0] + sig[1] + sig[2]);
+    for (int k = 0; k < 3; k++) s[k] = (float)(sig[k] - (double)(*x1 / 3.0f));
+    for (int k = 3; k < 6; k++) s[k] = (float)sig[k];
+    *x2d = sqrtf(0.5f * (s[0] * s[0] + s[1] * s[1] + s[2] * s[2]) + s[3] * s[3] + s[4] * s[4] + s[5] * s[5]);
+    *elcap_out = elcap;
+}
+
+/* f3dm25a, mcm_f3dm25.f90:34-838: trial stress, mode selection (:229-261),
+   the update of the mode (:263-797) and the stress, rate of deformation and
+   pressure written back (:801-831).  Stresses are negated on entry
+   (compression positive) and on exit; only the upper triangle of sigma is
+   written, so its lower triangle keeps the rotated stress of stress_rot1. */
+static void f3dm25(orc_state *st, int i, const double *cmv) {
+#define C(a, b) cmv[((a) - 1) + 5 * ((b) - 1)]
+    const float tol = 1.0e-7f;
+    const double othird = 1.0 / 3.0;
+    const float third = 1.0f / 3.0f;
+    const float unit = 1.0f, geop = 0.0f;
+    const float capa_l = 0.0f;   /* the never-assigned local capa */
+    double *capa = &st->capa[i];
+    double bulk = C(1, 1);
+    float shear = (float)C(1, 2), alpha = (float)C(1, 3), theta = (float)C(1, 4), gama = (float)C(1, 5);
+    float beta = (float)C(1, 6), r = (float)C(2, 1), dDyna = (float)C(2, 2), w = (float)C(2, 3);
+    float falfac = (float)C(4, 1);
+    alpha = alpha - falfac;
+    float z = (float)C(2, 4);
+    int ivec = 1;
+    if (C(3, 5) != 0.0) ivec = 0;
+    int ltype = (int)C(2, 6);
+    float tcut = (float)C(3, 1), fcut = (float)C(3, 4);
+    float t = fmaxF(fcut, tcut + 3.0f * geop);
+    float fet = alpha - gama * expf(-(beta * t)) + theta * t;
+    float fept = gama * beta * expf(-(beta * t)) + theta;
+    *capa = fmax_(*capa, C(3, 3));
+    double sig[6] = {-SG(1, 1), -SG(2, 2), -SG(3, 3), -SG(1, 2), -SG(1, 3), -SG(2, 3)};
+    double dd1 = -RD(1, 1), dd2 = -RD(2, 2), dd3 = -RD(3, 3);
+    double dd4 = -(RD(1, 2) / 2.0), dd5 = -(RD(1, 3) / 2.0), dd6 = -(RD(2, 3) / 2.0);
+    double dd = (dd1 + dd2 + dd3);
+    double davgm = (double)(1.0f / 3.0f) * dd;
+    float dt1 = (float)st->dt;
+    double p = 3.0 * bulk * (double)dt1 * davgm;
+    double gdt = st->dt * (double)shear;
+    double gd2 = 2.0 * gdt;
+    float gtr11 = (float)(sig[0] + p + gd2 * (dd1 - davgm));
+    float gtr22 = (float)(sig[1] + p + gd2 * (dd2 - davgm));
+    float gtr33 = (float)(sig[2] + p + gd2 * (dd3 - davgm));
+    float gtr12 = (float)(sig[3] + gd2 * dd4);
+    float gtr23 = (float)(sig[4] + gd2 * dd5);
+    float gtr31 = (float)(sig[5] + gd2 * dd6);
+    float x1tr = gtr11 + gtr22 + gtr33;
+    float s1 = gtr11 - (1.0f / 3.0f) * x1tr, s2 = gtr22 - (1.0f / 3.0f) * x1tr, s3 = gtr33 - (1.0f / 3.0f) * x1tr;
+    float sdtr11 = s1, sdtr22 = s2, sdtr33 = s3;
+    float x2dtr = 0.5f * (s1 * s1 + s2 * s2 + s3 * s3 + 2.0f * (gtr12 * gtr12 + gtr23 * gtr23 + gtr31 * gtr31));
+    x2dtr = sqrtf(x2dtr);
+    float fe = alpha - gama * expf(-(beta * capa_l)) + theta * capa_l;
+    float xkn = capa_l + r * fe;
+    float elcapn = (float)fmax_(*capa, (double)(0.0f * unit));
+    float tmises = fabsf(xkn - elcapn) / r;
+    fe = alpha - gama * expf(-(beta * x1tr)) + theta * x1tr;
+    float f1tr = x2dtr - fminF(fe, tmises);
+    float tcrit = (float)((double)t - 9.0 * bulk / (double)shear * (double)(x2dtr - fet) * (double)fept);
+    fe = alpha - gama * expf(-(beta * elcapn)) + theta * elcapn;
+    float fep = beta * gama * expf(-(beta * elcapn)) + theta;
+    float fc2tr = ((xkn - elcapn) * (xkn - elcapn) - (x1tr - elcapn) * (x1tr - elcapn)) / (r * r);
+    float f2tr = x2dtr * x2dtr - fc2tr;
+    float x1crt = (float)((double)elcapn - (double)((x2dtr - fe) * 9.0f) * bulk * (double)fep / (double)shear);
+    fe = (float)((double)alpha - (double)gama * exp(-((double)beta * *capa)) + (double)theta * *capa);
+    float capal = (float)fmin_((double)(0.0f * unit), *capa - (double)(tol * fe));
+    fe = alpha - gama * expf(-(beta * capal)) + theta * capal;
+    float xk = capal + r * fe;
+    float evpn = w * (1.0f - expf(dDyna * (z - xkn)));
+    float devpb = w * (1.0f - expf(dDyna * (z - xk))) - evpn;
+    double capan = *capa;
+    int imode = st->cap_imode;   /* a saved local: the last call's mode if no case below matches */
+    if (x1tr <= t && x2dtr <= fet && ltype == 1 && capan > 0.0 && devpb >= 0.0f) imode = 2;
+    else if (x1tr <= t && x2dtr <= fet && ltype == 1 && capan > 0.0 && devpb < 0.0f) imode = 7;
+    else if (x1tr <= t && x2dtr <= fet && (ltype == 2 || capan <= 0.0)) imode = 3;
+    else if (x1tr <= t && x2dtr > fet && x1tr <= tcrit && ltype == 1 && capan > 0.0 && devpb >= 0.0f) imode = 4;
+    else if (x1tr <= t && x2dtr > fet && x1tr <= tcrit && ltype == 1 && capan > 0.0 && devpb < 0.0f) imode = 6;
+    else if (x1tr <= t && x2dtr > fet && x1tr <= tcrit && (ltype == 2 || capan <= 0.0)) imode = 5;
+    else if (x1tr <= t && x2dtr > fet && x1tr > tcrit && capan >= 0.0) imode = 10;
+    else if (x1tr <= t && x2dtr > fet && x1tr > tcrit && capan < 0.0) imode = 8;
+    else if (x1tr > t && x1tr <= elcapn && f1tr <= tol) imode = 1;
+    else if (x1tr > t && x1tr <= elcapn && f1tr > tol && x1tr >= x1crt) imode = 9;
+    else if (x1tr > t && x1tr <= elcapn && f1tr > tol && x1tr < x1crt && capan >= 0.0) imode = 10;
+    else if (x1tr > t && x1tr <= elcapn && f1tr > tol && x1tr < x1crt && capan < 0.0) imode = 8;
+    else if (x1tr > t && x1tr > elcapn && f2tr < tol) imode = 1;
+    else if (x1tr > t && x1tr > elcapn && f2tr >= tol) imode = 11;
+    st->cap_imode = imode;
+    st->ptype[i] = imode;
+    float x1 = 0.0f, x2d = 0.0f, ratio;
+    double pp;
+    const float s_tr[3] = {s1, s2, s3};
+    const float g_sh[3] = {gtr12, gtr23, gtr31};
+    switch (imode) {
+        case 1: case 2: case 3:
+            /* :263-315 elastic, and the tension cut-off without a cap change */
+            x1 = imode == 1 ? x1tr : t;
+            if (imode == 1) *capa = capan;
+            else if (imode == 2) *capa = 0.0;
+            else *capa = capan;
+            pp = (double)(third * x1 + geop);
+            for (int k = 0; k < 3; k++) sig[k] = (double)s_tr[k] + pp;
+            for (int k = 0; k < 3; k++) sig[3 + k] = (double)g_sh[k];
+            break;
+        case 4: case 5:
+            /* :320-354 tension corner */
+            x1 = t;
+            x2d = fet;
+            *capa = imode == 4 ? 0.0 : capan;
+            pp = (double)(third * x1 + geop);
+            ratio = x2d / x2dtr;
+            for (int k = 0; k < 3; k++) sig[k] = (double)(s_tr[k] * ratio) + pp;
+            for (int k = 0; k < 3; k++) sig[3 + k] = (double)(g_sh[k] * ratio);
+            break;
+        case 6: case 7: {
+            /* :357-398 tension corner / cut-off with the cap moving */
+            x1 = t;
+            x2d = imode == 6 ? fet : x2dtr;
+            float x1old = (float)(sig[0] + sig[1] + sig[2]);
+            float devp = (float)(dd - (double)(x1 - x1old) / (3.0 * bulk));
+            float xhat = fminF(devpb, devp);
+            *capa = *capa + (double)devp * ((double)capal - capan) / (double)xhat;
+            *capa = fmax_(*capa, (double)(0.0f * unit));
+            pp = (double)(third * x1 + geop);
+            if (imode == 6) {
+                ratio = x2d / x2dtr;
+                for (int k = 0; k < 3; k++) sig[k] = (double)(s_tr[k] * ratio) + pp;
+                for (int k = 0; k < 3; k++) sig[3 + k] = (double)(g_sh[k] * ratio);
+            } else {
+                for (int k = 0; k < 3; k++) sig[k] = (double)s_tr[k] + pp;
+                for (int k = 0; k < 3; k++) sig[3 + k] = (double)g_sh[k];
+            }
+            break;
+        }
+        case 8: case 9:
+            /* :405-442 failure surface without cap motion / at the cap corner */
+            if (imode == 8) {
+                x1 = x1tr;
+                fe = alpha - gama * expf(-(beta * x1)) + theta * x1;
+                x2d = fminF(fe, tmises);
+            } else {
+                x1 = elcapn;
+                x2d = alpha - gama * expf(-(beta * x1)) + theta * x1;
+            }
+            *capa = capan;
+            ratio = x2d / x2dtr;
+            pp = (double)(third * x1 + geop);
+            for (int k = 0; k < 3; k++) sig[k] = (double)(s_tr[k] * ratio) + pp;
+            for (int k = 0; k < 3; k++) sig[3 + k] = (double)(g_sh[k] * ratio);
+            break;
+        case 10: {
+            /* :445-624 failure surface with the cap moving */
+            const float gt[6] = {gtr11, gtr22, gtr33, gtr12, gtr23, gtr31};
+            for (int k = 0; k < 6; k++) sig[k] = (double)gt[k];
+            if (ivec == 0) {
+                if (feit(capan, x1tr, x2dtr, alpha, gama, beta, theta, w, dDyna, r, bulk, shear, &x1, capa, sig)) {
+                    set_err(st, ORC_ERR_MODEL);
+                    return;
+                }
+            } else {
+                *capa = capan;
+                x1 = x1tr;
+                x2d = x2dtr;
+                float sv[6];
+                for (int k = 0; k < 3; k++) sv[k] = (float)(sig[k] - (double)(x1tr / 3.0f));
+                for (int k = 3; k < 6; k++) sv[k] = (float)sig[k];
+                for (int it = 0; it < 4; it++)
+                    cap_fail_step(capa, sig, &x1, &x2d, sv, alpha, gama, beta, theta, w, dDyna, r, bulk, shear);
+            }
+            if (x1 > elcapn) x1 = elcapn;
+            if (ltype == 2 || capan == 0.0) *capa = capan;
+            if ((double)x1 > *capa) *capa = (double)x1;
+            if (ltype == 1 && capan >= 0.0) *capa = fmax_(*capa, (double)(0.0f * unit));
+            fe = alpha - gama * expf(-(beta * x1)) + theta * x1;
+            x2d = fminF(fe, tmises);
+            ratio = x2d / x2dtr;
+            pp = (double)(third * x1 + geop);
+            /* ivec 0 scales the trial deviator s1..s3, ivec 1 its saved copy: the same values */
+            const float sd[3] = {sdtr11, sdtr22, sdtr33};
+            for (int k = 0; k < 3; k++) sig[k] = (double)((ivec == 0 ? s_tr[k] : sd[k]) * ratio) + pp;
+            for (int k = 0; k < 3; k++) sig[3 + k] = (double)(g_sh[k] * ratio);
+            break;
+        }
+        case 11: {
+            /* :626-797 cap surface */
+            const float gt[6] = {gtr11, gtr22, gtr33, gtr12, gtr23, gtr31};
+            for (int k = 0; k < 6; k++) sig[k] = (double)gt[k];
+            if (ivec == 0) {
+                if (fcit(capan, x1tr, x2dtr, alpha, gama, beta, theta, w, dDyna, r, bulk, shear, &x1, capa, sig)) {
+                    set_err(st, ORC_ERR_MODEL);
+                    return;
+                }
+            } else {
+                *capa = capan;
+                x1 = x1tr;
+                x2d = x2dtr;
+                float sv[6], elcap = 0.0f;
+                for (int k = 0; k < 3; k++) sv[k] = (float)(sig[k] - (double)(x1tr / 3.0f));
+                for (int k = 3; k < 6; k++) sv[k] = (float)sig[k];
+                for (int it = 0; it < 4; it++)
+                    cap_cap_step(capa, sig, &x1, &x2d, sv, &elcap, alpha, gama, beta, theta, w, dDyna, r, bulk, shear);
+                if (x1 < elcap) {
+                    x1 = elcap + fabsf(x1 - elcap);
+                    for (int k = 0; k < 3; k++) sig[k] = (double)(sv[k] + x1 / 3.0f);
+                }
+            }
+            break;
+        }
+        default: break;   /* mode 0: the stress stays the rotated one */
+    }
+    SG(1, 1) = -sig[0]; SG(2, 2) = -sig[1]; SG(3, 3) = -sig[2];
+    SG(1, 2) = -sig[3]; SG(1, 3) = -sig[4]; SG(2, 3) = -sig[5];
+    RD(1, 1) = -dd1; RD(2, 2) = -dd2; RD(3, 3) = -dd3;
+    RD(1, 2) = -(dd4 * 2.0); RD(1, 3) = -(dd5 * 2.0); RD(2, 3) = -(dd6 * 2.0);
+    st->p[i] = -(othird * (SG(1, 1) + SG(2, 2) + SG(3, 3)));
+#undef C
 }
 
 /* mcm_nodalav.f90:32-47 via mcm_aviscosity.f90:28-39 */
@@ -1233,6 +1708,7 @@ int orc_constitutive(orc_state *st) {
         switch (mt->model) {
             case 1: stress_rot1(st, i); f3dm1(st, i, mt->strinput); aviscosity(st, i); lieupd(st, i); break;
             case 3: stress_rot1(st, i); f3dm3(st, i, mt->strinput); aviscosity(st, i); lieupd(st, i); break;
+            case 4: stress_rot1(st, i); f3dm25(st, i, mt->strinput); aviscosity(st, i); lieupd(st, i); break;
             case 9:
                 stress_rot1(st, i); f3dm9(st, i, mt->strinput); sueos(st, i); aviscosity(st, i);
                 hieupd(st, i); eqos(st, i); stress_up(st, i);
@@ -1652,6 +2128,11 @@ int orc_initial(orc_state *st) {
                 st->p[i] = 0.0;
                 st->c[i] = sqrt(mt->strinput[1] / st->rho[i]);
                 break;
+            case 4:   /* :57-67, the sound speed from lambda + 2G of mcm_sets4 */
+                if (st->init_rhoe != 1) { st->rho[i] = st->rho0[i]; st->e[i] = 0.0; }
+                st->p[i] = 0.0;
+                st->c[i] = sqrt(mt->strinput[9] / st->rho[i]);
+                break;
             case 9:
                 eoscalc(st, i);
                 if (st->p[i] < mt->strinput[0]) st->p[i] = mt->strinput[0];
@@ -1783,6 +2264,14 @@ int orc_dump(orc_state *st, const char *path, int step, int stage) {
         fwrite(st->g_rho, 8, st->ngp, f);
         fwrite(st->g_hold, 8, st->ngp, f);
     }
+    int cap = 0;
+    for (int m = 0; m < st->nummat; m++) cap |= st->mat[m].model == 4;
+    if (cap) {   /* cap-model section: 'CAP4', capa, ptype */
+        int32_t tag = 0x43415034;
+        fwrite(&tag, 4, 1, f);
+        fwrite(st->capa, 8, st->np, f);
+        for (int i = 0; i < st->np; i++) { int32_t v = st->ptype[i]; fwrite(&v, 4, 1, f); }
+    }
     fclose(f);
     return ORC_OK;
 }
@@ -1815,6 +2304,10 @@ void orc_get_scalars(const orc_state *st, double *out) {
     const double v[9] = {st->dt, st->dtold, st->ptime, st->critts, st->hmax, st->kinetice, st->internale,
                          st->totale, (double)st->ngp};
     for (int k = 0; k < 9; k++) out[k] = v[k];
+}
+
+void orc_get_capa(const orc_state *st, double *capa, int32_t *ptype) {
+    for (int i = 0; i < st->np; i++) { capa[i] = st->capa[i]; ptype[i] = st->ptype[i]; }
 }
 
 void orc_get_x(const orc_state *st, double *out) {

@@ -13,7 +13,10 @@
  * against closed forms and brute force).  Each function cites the
  * reference routine it follows (MCM/<file>.f90:<lines>).
  *
- * Every branch restated here is exercised by a reference digest.  Penalty
+ * Every branch restated here is exercised by a reference digest, except
+ * three modes of the cap model f3dm25 (material model 4): parity unpinned
+ * for modes 6 and 7 (0 < capa < 1e-7 fe) and 8 (capa < 0), which no deck
+ * reaches (mcm_f3dm25.f90:229-261; capa stays >= 0 from its zero start).  Penalty
  * contact with ghost contacts is not restated: the reference reads its ghost contact partners
  * from the real contact list past its end (mcm_get_j_cont_info.f90:44-48),
  * so its result is undefined there; rep_cont stops with ORC_ERR_UNDEFINED.
@@ -68,6 +71,9 @@ typedef struct {
     double *tracerod, *pcut, *p_cut, *efps, *fail, *mindist, *critts_p;
     double *sigma, *s, *alfa, *q, *qold, *rod, *spin; /* 9 per particle */
     double *rep;                /* 3 per particle: contact acceleration of mcm_rep_cont */
+    double *capa;               /* cap position of the cap model (material model 4) */
+    int *ptype;                 /* mode taken by the cap model in its last update */
+    int cap_imode;              /* the cap model's saved mode variable (a static local there) */
     /* neighbour lists (0-based ids), capacity ncap / gcap per particle */
     int ncap, gcap;
     int *nnbr, *g_nnbr, *ncont, *g_ncont;
@@ -116,6 +122,8 @@ void orc_get_x(const orc_state *st, double *out);
 void orc_get_rows(const orc_state *st, double *rows);
 int orc_get_lists(const orc_state *st, int32_t *counts, int32_t *gcounts, int32_t *ids, int64_t cap);
 void orc_get_scalars(const orc_state *st, double *out);
+/* cap position and last mode of the cap model, per particle */
+void orc_get_capa(const orc_state *st, double *capa, int32_t *ptype);
 
 /* closed-form pieces exposed for unit tests */
 double orc_kernel_w(const double *xi, const double *xj, double havg, double pi);

@@ -53,6 +53,8 @@ def _lib():
     lib.orc_get_lists.restype = ctypes.c_int
     lib.orc_get_scalars.argtypes = [ctypes.c_void_p, dbl3]
     lib.orc_get_scalars.restype = None
+    lib.orc_get_capa.argtypes = [ctypes.c_void_p, dbl3, i32p]
+    lib.orc_get_capa.restype = None
     return lib
 
 
@@ -128,6 +130,15 @@ class Oracle:
         if rc != 0:
             raise ValueError("oracle lists longer than %d" % cap)
         return c, g, ids
+
+    def capa(self, np_: int):
+        """(capa, ptype) of the cap model (material model 4), per particle."""
+        import numpy
+        c = numpy.zeros(np_)
+        t = numpy.zeros(np_, numpy.int32)
+        self._l.orc_get_capa(self.h, c.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                             t.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+        return c, t
 
     def scalars(self) -> dict:
         import numpy

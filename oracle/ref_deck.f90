@@ -64,6 +64,11 @@ contains
 
   call mcm_allocate_memory
   call mcm_initialise_database
+  ! mcm_initialise_database leaves the cap model's capa and ptype unset, and
+  ! the model's first update reads capa (mcm_f3dm25.f90:162); a freshly
+  ! mapped allocation holds zeros, which is set here explicitly
+  par(:)%capa = 0.0_d
+  par(:)%ptype = 0
 
   do m = 1, mcm_nummat
     mcm_mat(m)%model = 0
@@ -229,6 +234,11 @@ contains
   ! mcm_getinput.f90:112-127, the reference's own routines
   call mcm_allocate_memory
   call mcm_initialise_database
+  ! mcm_initialise_database leaves the cap model's capa and ptype unset, and
+  ! the model's first update reads capa (mcm_f3dm25.f90:162); a freshly
+  ! mapped allocation holds zeros, which is set here explicitly
+  par(:)%capa = 0.0_d
+  par(:)%ptype = 0
   call mcm_matin
   call mcm_nodein
   if (mcm_thnode .gt. 0) call mcm_nodehist
@@ -251,6 +261,7 @@ contains
  !   int32 nnbr(np), g_nnbr(np), llpointer(np)
  !   int32 neighbour ids, particle by particle (nnbr entries), then ghost ids (g_nnbr entries)
  !   ghosts: int32 par(ngp), mat(ngp); f64 [ngp][8] (x3, v3, mass, h), f64 rho(ngp), hold(ngp)
+ !   with a cap-model material: int32 'CAP4', f64 capa(np), int32 ptype(np)
  subroutine probe_dump(od, istep, stage)
   character(len=*) :: od
   integer :: istep, stage, u, i, k, m
@@ -286,6 +297,12 @@ contains
     enddo
     write(u) (gpar(i)%rho, i=1,mcm_ngp)
     write(u) (gpar(i)%hold, i=1,mcm_ngp)
+  endif
+  ! decks with the cap model (material model 4): 'CAP4', capa(np), ptype(np)
+  if (any(mcm_mat(1:mcm_nummat)%model.eq.4)) then
+    write(u) 1128353844
+    write(u) (par(i)%capa, i=1,mcm_np)
+    write(u) (par(i)%ptype, i=1,mcm_np)
   endif
   close(u)
  end subroutine probe_dump

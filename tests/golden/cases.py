@@ -202,6 +202,41 @@ def build(name: str) -> deck.Deck:
         v[: len(a), 2] = 150.0
         v[len(a):, 2] = -150.0
         return deck.make_deck(pts, v, [m], disctype=1, h_opt=1, jitter_dx=0.01 * dx)
+    if name.startswith("cap_"):
+        # the cap model (material model 4) in mm, tonne, s: stresses in MPa,
+        # the scale of the model's absolute tolerances.  A soil (ltype 1)
+        # with K 100, G 60, failure surface 2 - 1.8 exp(-0.1 J1) + 0.05 J1,
+        # cap aspect 4, hardening 0.05 (1 - exp(0.02 (6 - X))), initial cap at
+        # 6 MPa and a tension cut-off at -0.5 MPa
+        dxm, rho = 2.0, 1.8e-9
+        hm = 1.2 * dxm
+        card = deck.f3dm25(100.0, 60.0, 2.0, 0.05, 1.8, 0.1, 4.0, 0.02, 0.05, 6.0, 6.0, ltype=1, tcut=-0.5)
+        if name == "cap_soil_impact":
+            # a soil column striking its symmetry plane at 20 m/s
+            pts = deck.block(dxm, 5, 5, 8)
+            m = deck.material(4, rho, rho * dxm ** 3 * len(pts), hm, card)
+            return deck.make_deck(pts, (0.0, 0.0, -2.0e4), [m], jitter_dx=0.01 * dxm,
+                                  sym_flag=(0, 0, 0, 0, 1, 0))
+        if name in ("cap_rock_iterated_impact", "cap_soil_iterated_impact"):
+            # the iterated return of modes 10 and 11 (feit, fcit) converges to
+            # the reference's single-precision tolerance of 1e-8 only where
+            # stresses are small numbers: the same materials in mm, ms, kg
+            # (stresses in GPa), as rock (ltype 2) or soil, at 20 m/s
+            rho_g = 1.8e-6
+            rock = name == "cap_rock_iterated_impact"
+            cg = deck.f3dm25(0.1, 0.06, 2.0e-3, 0.05, 1.8e-3, 100.0, 4.0, 20.0, 0.05, 6.0e-3, 6.0e-3,
+                             ltype=2 if rock else 1, tcut=-5.0e-4, vec=1.0)
+            pts = deck.block(dxm, 5, 5, 8)
+            m = deck.material(4, rho_g, rho_g * dxm ** 3 * len(pts), hm, cg)
+            return deck.make_deck(pts, (0.0, 0.0, -20.0), [m], jitter_dx=0.01 * dxm,
+                                  sym_flag=(0, 0, 0, 0, 1, 0))
+        if name == "cap_soil_stretch_corr":
+            # corrected SPH: a soil block stretched in x and sheared in y-z
+            pts = deck.block(dxm, 5, 5, 5)
+            m = deck.material(4, rho, rho * dxm ** 3 * len(pts), hm, card)
+            c = pts - pts.mean(axis=0)
+            v = np.stack([3.0e3 * c[:, 0], 2.0e3 * c[:, 2], -1.5e3 * c[:, 2]], axis=1)
+            return deck.make_deck(pts, v, [m], disctype=1, jitter_dx=0.01 * dxm)
     raise KeyError(name)
 
 
@@ -226,4 +261,8 @@ CASES = {
     "fluid_f3dm9_mu0_eos1": (30, (2,)),
     "eos4_negative_c2": (30, (2,)),
     "corr_collide_f3dm10_eos4_hopt1": (30, (2,)),
+    "cap_soil_impact": (30, (2,)),
+    "cap_soil_stretch_corr": (30, (2,)),
+    "cap_rock_iterated_impact": (30, (2,)),
+    "cap_soil_iterated_impact": (30, (2,)),
 }

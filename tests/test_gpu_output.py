@@ -70,3 +70,23 @@ def test_material_files_of_a_two_material_run(tmp_path):
     for m in (0, 1, 2):
         gpu.vtk_write(str(tmp_path / ("h%d.vtk" % m)), f, ids, m)
         assert open(tmp_path / ("m%d.vtk" % m), "rb").read() == open(tmp_path / ("h%d.vtk" % m), "rb").read()
+
+
+def test_material_file_of_a_cap_run_writes_the_modes(tmp_path):
+    """WriteVTKmat's PTYPE block (mcm_VTK.f90) holds par%ptype, the cap
+    model's mode of each particle's last update: the oracle's modes after
+    the same steps."""
+    path = os.path.join(DATA, "cap_soil_impact.bin")
+    o = Oracle(path)
+    o.initial()
+    o.run(20)
+    want = o.state(20, 10).ptype
+    with gpu.Solver(path) as s:
+        s.initial()
+        s.step(20)
+        s.write_vtk(str(tmp_path / "m1.vtk"), 1)
+    lines = open(tmp_path / "m1.vtk").read().splitlines()
+    k = lines.index("SCALARS PTYPE double 1")
+    got = np.array([int(v) for v in lines[k + 2:k + 2 + len(want)]])
+    assert len(got) == len(want) and len(set(want.tolist())) > 1
+    np.testing.assert_array_equal(got, want)

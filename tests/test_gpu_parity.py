@@ -52,6 +52,17 @@ def lists_equal_as_sets(g: Dump, o: Dump):
     return True
 
 
+def check_cap(g: Dump, o: Dump, tol, where):
+    """The cap model's state (decks with a model-4 material): the mode of
+    each particle's last update equal, capa within tol."""
+    if o.capa is None:
+        assert g.capa is None, where
+        return
+    assert g.capa is not None, where
+    assert np.array_equal(g.ptype, o.ptype), (where, "ptype", int((g.ptype != o.ptype).sum()))
+    assert rel(g.capa, o.capa) <= tol, (where, "capa", rel(g.capa, o.capa))
+
+
 def gdump(s: gpu.Solver, tmp_path, step, stage):
     p = str(tmp_path / ("g_%d_%d.bin" % (step, stage)))
     s.dump(p, step, stage)
@@ -84,6 +95,7 @@ def test_stage_by_stage(name, tmp_path):
                 errs = field_errors(gd, od)
                 worst = max(errs, key=errs.get)
                 assert errs[worst] <= PER_STEP_TOL, (step, st, worst, errs[worst])
+                check_cap(gd, od, PER_STEP_TOL, (step, st))
                 for sc in ("dt", "dtold", "ptime", "critts", "kinetice", "internale"):
                     a, b = getattr(gd, sc), getattr(od, sc)
                     assert abs(a - b) <= PER_STEP_TOL * max(abs(b), 1e-300), (step, st, sc, a, b)
@@ -105,6 +117,7 @@ def test_fused_steps(name, tmp_path):
     assert lists_equal_as_sets(gd, od)
     errs = field_errors(gd, od, STATE)
     assert max(errs.values()) <= 1e-9, errs
+    check_cap(gd, od, 1e-9, nsteps)
     assert gd.ptime == pytest.approx(od.ptime, rel=1e-12)
     assert gd.kinetice == pytest.approx(od.kinetice, rel=1e-9)
     assert gd.internale == pytest.approx(od.internale, rel=1e-9)
@@ -269,9 +282,11 @@ def test_long_lists_use_both_wave_slots(corrected, tmp_path):
 
 
 # decks whose step calls pow (eos28: (rho/rho0)^gamma; h_opt 2: h0 (rho0/rho)^(1/3); tcrit_opt 0:
-# (m/rho)^(1/3) and ^(2/3); penalty contact: f^n): the device pow and the host libm's may differ in
-# the last bit, so these are held to 1e-9 by test_fused_steps only
-POW_DECKS = {"f3dm10_eos41_hopt2_tcrit0", "fluid_f3dm9_eos28_dispbc", "penalty_contact_f3dm1"}
+# (m/rho)^(1/3) and ^(2/3); penalty contact: f^n) or the double exp (the cap model's yield surface
+# exp(-beta capa)): the device's and the host libm's may differ in the last bit, so these are held
+# to 1e-9 by test_fused_steps only
+POW_DECKS = {"f3dm10_eos41_hopt2_tcrit0", "fluid_f3dm9_eos28_dispbc", "penalty_contact_f3dm1"} | \
+    {n for n in cases.CASES if n.startswith("cap_")}
 
 
 @pytest.mark.parametrize("name", sorted(set(cases.CASES) - POW_DECKS))

@@ -30,3 +30,19 @@ def test_fast_path_arithmetic_equals_the_device_division_and_sqrt(which):
     assert lib.quot_check(which, 1 << 26, 0x5eed + which, out) == 0
     assert out[0] == 0, "%s: %d mismatches, first a=%r b=%r got %r want %r" % (
         CASES[which], int(out[0]), out[1], out[2], out[3], out[4])
+
+
+def test_cap_model_expf_on_the_device_equals_the_c_library():
+    """The cap model's single-precision exp as the device evaluates it
+    (cap_expf, mcmtop_amd/csrc/mcm_cap.h) against the host C library's expf,
+    which the reference's flang build calls: equal bits (or both NaN) for all
+    2^32 float bit patterns (tests/gpu_checks/expf_check.hip)."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "gpu_checks", "libexpf_check.so")
+    if not os.path.exists(path):
+        pytest.fail("tests/gpu_checks/libexpf_check.so not built (__graft_entry__.build())")
+    chk = ctypes.CDLL(path)
+    chk.expf_check_device.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_ulonglong)]
+    out = (ctypes.c_ulonglong * 2)()
+    assert chk.expf_check_device(0, 1 << 32, 16, out) == 0
+    assert out[0] == 0, "%d mismatches, first pattern 0x%08x" % (out[0], out[1])

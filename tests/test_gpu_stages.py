@@ -119,10 +119,13 @@ def test_host_side_dt(name, tmp_path):
     assert abs(got.ptime - want.ptime) <= tol * want.ptime
 
 
-@pytest.mark.parametrize("name", ["corr_f3dm10_eos4_hopt1", "two_materials_contact", "xsph_corr_f3dm10_eos4_sym"])
+@pytest.mark.parametrize("name", ["corr_f3dm10_eos4_hopt1", "two_materials_contact", "xsph_corr_f3dm10_eos4_sym",
+                                  "cap_soil_stretch_corr"])
 def test_state_round_trip(name, tmp_path):
     """The host's copy of the state (download), uploaded into a second run of
-    the same deck, steps on exactly as the first run does."""
+    the same deck, steps on exactly as the first run does; with a cap-model
+    material the rows carry the asymmetric stress and mcm_gpu_cap_state the
+    model's capa and mode."""
     path = os.path.join(DATA, name + ".bin")
     with gpu.Solver(path) as a, gpu.Solver(path) as b:
         a.initial()
@@ -130,6 +133,9 @@ def test_state_round_trip(name, tmp_path):
         rows, sc = a.download_state()
         assert rows.shape == (sc.np, gpu.STATE_WORDS)
         b.upload_state(rows, sc)
+        capa, ptype = a.cap_state()
+        assert (np.abs(capa).max() > 0) == name.startswith("cap_")
+        b.upload_cap_state(capa, ptype)
         with pytest.raises(gpu.McmGpuError):
             b.strain()                      # the records of an uploaded state need a search first
         a.step(10)
@@ -141,3 +147,6 @@ def test_state_round_trip(name, tmp_path):
     np.testing.assert_array_equal(da.nbr, db.nbr)
     assert da.ptime == db.ptime and da.kinetice == db.kinetice
     np.testing.assert_array_equal(rows_b, db.rows)
+    if name.startswith("cap_"):
+        np.testing.assert_array_equal(da.capa, db.capa)
+        np.testing.assert_array_equal(da.ptype, db.ptype)

@@ -233,3 +233,19 @@ def test_gruneisen_sound_speed_at_rest_and_cfl_timestep():
     np.testing.assert_allclose(d.f("c"), np.sqrt(4.0 * G / (3.0 * rho0) + C0 ** 2), rtol=1e-14)
     crit = d.f("h") / (d.f("c") + d.f("vabs"))
     assert d.dt == pytest.approx(dk.tssf * crit.min(), rel=1e-15)
+
+
+def test_cap_model_expf_equals_the_c_library_on_every_float():
+    """The cap model's single-precision exp on the host (cap_expf,
+    mcmtop_amd/csrc/mcm_cap.h, which the GPU path evaluates) against the C
+    library's expf, which the reference's flang build and the oracle call:
+    equal bits (or both NaN) for all 2^32 float bit patterns."""
+    path = os.path.join(os.path.dirname(__file__), "gpu_checks", "libexpf_check.so")
+    if not os.path.exists(path):
+        pytest.fail("tests/gpu_checks/libexpf_check.so not built (__graft_entry__.build())")
+    chk = ctypes.CDLL(path)
+    chk.expf_check_host.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
+                                    ctypes.POINTER(ctypes.c_ulonglong)]
+    out = (ctypes.c_ulonglong * 2)()
+    assert chk.expf_check_host(0, 1 << 32, min(8, os.cpu_count() or 1), out) == 0
+    assert out[0] == 0, "%d mismatches, first pattern 0x%08x" % (out[0], out[1])

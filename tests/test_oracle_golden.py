@@ -1,8 +1,9 @@
 """The CPU oracle against the reference's own results.
 
 tests/golden/data/digests.json holds SHA-256 digests of state dumps written
-by the flang build of the reference (tests/golden/make_golden.py) for eleven
-decks covering both discretisations, models 1/3/9/10, EOS 1/4/13/28/41,
+by the flang build of the reference (tests/golden/make_golden.py) for the
+decks of tests/golden/cases.py, covering both discretisations, models
+1/3/4/9/10 (4: the cap model, with its capa and mode), EOS 1/4/13/28/41,
 symmetry / periodic / corner ghosts, two materials in kernel contact and in
 penalty contact, XSPH, boundary codes and the three timestep options.  The oracle writes the same dump
 layout, so equal digests mean bit-identical state: positions, velocities,
@@ -93,6 +94,29 @@ def test_decks_reach_their_branches():
     # reports taking that branch
     assert DIGESTS["eos4_negative_c2"]["warnings"]["complex_sound_speed"] > 0
     assert all(DIGESTS[k]["warnings"]["complex_sound_speed"] == 0 for k in DIGESTS if k != "eos4_negative_c2")
+
+
+def test_cap_decks_reach_the_cap_models_modes():
+    """The cap model (mcm_f3dm25.f90:229-261 selects one of eleven modes per
+    particle and step): over the cap decks, every step, the modes taken by
+    the vector (four-step) and the iterated returns.  Modes 6 and 7 need
+    0 < capa < 1e-7 Fe and mode 8 needs capa < 0, which no deck reaches: the
+    update keeps capa >= 0 from its zero start."""
+    seen = {}
+    for name in [n for n in cases.CASES if n.startswith("cap_")]:
+        o = Oracle(os.path.join(DATA, name + ".bin"))
+        assert o.initial() == 0
+        np_ = DIGESTS[name]["np"]
+        modes = set()
+        for _ in range(DIGESTS[name]["nsteps"]):
+            assert o.step() == 0
+            modes |= set(o.capa(np_)[1].tolist())
+        seen[name] = modes
+    vector = seen["cap_soil_impact"] | seen["cap_soil_stretch_corr"]
+    iterated = seen["cap_rock_iterated_impact"] | seen["cap_soil_iterated_impact"]
+    assert {1, 2, 4, 9, 10, 11} <= vector, vector
+    assert {1, 2, 3, 5, 9, 10, 11} <= iterated, iterated
+    assert not (vector | iterated) & {0, 6, 7, 8}
 
 
 @pytest.mark.parametrize("name", sorted(json.load(open(os.path.join(DATA, "dropin.json")))))
