@@ -821,11 +821,20 @@ __device__ __forceinline__ void ghost_walk(const DevState& d, int cur, int i, in
 // accepted only when its distance is below h_i + h_j <= h_i + hcap (hcap:
 // largest h of the binned particles), so the real particles of a cell whose
 // box lies at least that far away can never be accepted: such rows and
-// x-run end cells are skipped when every lane of the wavefront can skip them
-// (with a relative margin of 1e-9 that covers the rounding of the box
-// faces).  Ghosts are not binned by position and are never skipped.
-// Skipping rejected candidates leaves the list, its order and mindist
-// unchanged.
+// x-run end cells are skipped (with a relative margin of 1e-9 that covers
+// the rounding of the box faces).  Ghosts are not binned by position and
+// are never skipped.  Skipping rejected candidates leaves the list, its
+// order and mindist unchanged.
+//
+// The rows are visited in the same relative order by every lane (z, then y
+// offsets -1, 0, 1), which is each particle's walk order.  For each row the
+// lanes of a wavefront read overlapping runs of the sorted order (their
+// particles sit in a few neighbouring cells), so the wavefront first copies
+// the span of its lanes' runs into its own LDS slice, 32 bytes per
+// candidate, and the lanes then walk their runs from there: every candidate
+// record is fetched from the caches once per wavefront and row instead of
+// once per lane.  A span longer than the slice (a wavefront that straddles a
+// plane of cells) is staged and walked in pieces, in order.
 //
 // SKIN: the skin lists of a step that needs them (k_skin_check), with the
 // same walk: every binned real particle j != i with
@@ -834,181 +843,224 @@ __device__ __forceinline__ void ghost_walk(const DevState& d, int cur, int i, in
 // particles of ghost-holding cells are taken too (xh hides them in the
 // steps where their cell holds ghosts).  Records each particle's sorted
 // position, its x and h, its block and its candidate record.
+#define LIST_STAGE 256   // candidates per wavefront slice (8 KB)
 template <bool MULTI, bool SKIN>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 3 : 5))) void k_list(DevState d_, int cur) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 3 : 4))) void k_list(DevState d_, int cur) {
     // threads take the binned particles in cell-sorted order, so that the
     // lanes of a wavefront share cells and read the same candidates
+    __shared__ double2 s_rec[4][2 * LIST_STAGE];
+    // the candidates' ids (and materials): SKIN lists hold sorted positions
+    __shared__ int2 s_idm[MULTI && !SKIN ? 4 : 1][MULTI && !SKIN ? LIST_STAGE : 1];
+    __shared__ int s_id[!MULTI && !SKIN ? 4 : 1][!MULTI && !SKIN ? LIST_STAGE : 1];
     Blk bk;
     const DevState& d = enter<true>(d_, bk);
     const int t = bk.b * blockDim.x + threadIdx.x;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     Scalars* sc = d.sc;
     if (sc->abort) return;
     if (SKIN && !sc->skin_rebuild) return;
     const GridDev g = sc->grid;
     const bool live = g.ncell > 0 && t < d.cell_start[g.ncell - 1] + d.cell_count[g.ncell - 1];
-    int cnt = 0;
-    if (live) {
-    const int i = d.sorted[t];
-    const int L0 = g.lim[0], L1 = g.lim[1];
+    const int L0 = g.lim[0], L1 = g.lim[1], L2 = g.lim[2];
     const int ncap = SKIN ? d.skin_cap : d.ncap;
+    const int i = live ? d.sorted[t] : 0;
+    const int c = live ? d.cell_of[i] : -1;
+    const bool binned = live && c >= 0;
     int* out = SKIN ? skin_row(d, t) : nbr_row(d, t);
     // penalty contact: other-material partners go to the contact list, ghosts
     // of other materials are only counted (mcm_ll_neighbours.f90:162-174,
     // 209-223; merged into the lists only for kernel contact, :239-267)
     const bool penalty = !SKIN && MULTI && d.contacttype > 0;
-    int ccnt = 0, gccnt = 0;
+    int cnt = 0, ccnt = 0, gccnt = 0;
     double mind = 1.0e20, mind_cp = 1.0e20;
-    const int c = d.cell_of[i];
-    int nreal;
-    if (c >= 0) {
-        const double xi = d.x0[0][i], yi = d.x0[1][i], zi = d.x0[2][i];
-        const double hi = d.hbuf[cur][i];
-        const int mi = d.pmat[i];
-        const int b0 = c % L0, b1 = (c / L0) % L1, b2 = c / (L0 * L1);
-        const int lo0 = max(b0 - 1, 0), hi0 = min(b0 + 1, L0 - 1);
-        const int lo1 = max(b1 - 1, 0), hi1 = min(b1 + 1, L1 - 1);
-        const int lo2 = max(b2 - 1, 0), hi2 = min(b2 + 1, g.lim[2] - 1);
-        // distances from the particle to the lower and upper faces of its cell
-        const double w = SKIN ? sc->skin_len : 0.0;
-        const double grow = SKIN ? 1.0 + 1.0e-9 : 1.0;
+    double xi = 0.0, yi = 0.0, zi = 0.0, hi = 0.0;
+    int mi = 0, b0 = 0, b1 = 0, b2 = 0;
+    const double w = SKIN ? sc->skin_len : 0.0;
+    const double grow = SKIN ? 1.0 + 1.0e-9 : 1.0;
+    double reach2 = 0.0;
+    // distances from the particle to the lower and upper faces of its cell
+    double flo[3] = {0.0, 0.0, 0.0}, fhi[3] = {0.0, 0.0, 0.0};
+    if (binned) {
+        xi = d.x0[0][i]; yi = d.x0[1][i]; zi = d.x0[2][i];
+        hi = d.hbuf[cur][i];
+        mi = d.pmat[i];
+        b0 = c % L0; b1 = (c / L0) % L1; b2 = c / (L0 * L1);
         const double reach = (hi + __longlong_as_double((long long)sc->hcap_bits) + w) * grow * (1.0 + 1.0e-9);
-        const double reach2 = reach * reach;
-        double flo[3], fhi[3];
-        {
-            const double xv[3] = {xi, yi, zi};
-            const int bv[3] = {b0, b1, b2};
-            for (int a = 0; a < 3; a++) {
-                const double cw = 1.0 / g.gridsize[a];
-                const double face = g.gridmin[a] + bv[a] * cw;
-                flo[a] = fmax(xv[a] - face, 0.0);
-                fhi[a] = fmax(face + cw - xv[a], 0.0);
-            }
+        reach2 = reach * reach;
+        const double xv[3] = {xi, yi, zi};
+        const int bv[3] = {b0, b1, b2};
+        for (int a = 0; a < 3; a++) {
+            const double cw = 1.0 / g.gridsize[a];
+            const double face = g.gridmin[a] + bv[a] * cw;
+            flo[a] = fmax(xv[a] - face, 0.0);
+            fhi[a] = fmax(face + cw - xv[a], 0.0);
         }
-        const double2* cd = reinterpret_cast<const double2*>(d.cand);
-        const int2* ci = reinterpret_cast<const int2*>(d.cand_id);
-        const int spos_i = t;
-        for (int pass = 0; pass < (MULTI && !SKIN ? 2 : 1); pass++) {
-            // candidate at sorted slot s; with one material its id is read
-            // only when it is accepted (the self test uses the sorted slot)
-            auto consider = [&](int s, double2 pa, double2 pb, int2 idm) {
-                if (MULTI && !SKIN ? idm.x == i : s == spos_i) return;
-                double dx = pa.x - xi;
-                double dist_sq = dx * dx;   // 0.0 + dx*dx is dx*dx exactly
-                double dy = pa.y - yi; dist_sq = dist_sq + dy * dy;
-                double dz = pb.x - zi; dist_sq = dist_sq + dz * dz;
-                if (SKIN) {
-                    const double r = (pb.y + hi + w) * grow;
-                    if (dist_sq < r * r) {
-                        if (cnt < ncap) out[(long)cnt * 64] = s;
+    }
+    const double2* cd = reinterpret_cast<const double2*>(d.cand);
+    const int2* ci = reinterpret_cast<const int2*>(d.cand_id);
+    double2* rec = s_rec[wv];
+    int2* idl = s_idm[MULTI && !SKIN ? wv : 0];
+    int* idsl = s_id[!MULTI && !SKIN ? wv : 0];
+    const int spos_i = t;
+    for (int pass = 0; pass < (MULTI && !SKIN ? 2 : 1); pass++) {
+        // candidate at sorted slot s with id and material idm (with one
+        // material the self test uses the sorted slot)
+        auto consider = [&](int s, double2 pa, double2 pb, int2 idm) {
+            if (MULTI && !SKIN ? idm.x == i : s == spos_i) return;
+            double dx = pa.x - xi;
+            double dist_sq = dx * dx;   // 0.0 + dx*dx is dx*dx exactly
+            double dy = pa.y - yi; dist_sq = dist_sq + dy * dy;
+            double dz = pb.x - zi; dist_sq = dist_sq + dz * dz;
+            if (SKIN) {
+                const double r = (pb.y + hi + w) * grow;
+                if (dist_sq < r * r) {
+                    if (cnt < ncap) out[(long)cnt * 64] = s;
+                    cnt++;
+                }
+                return;
+            }
+            double h_avg = 2.0 * 0.5 * (pb.y + hi);
+            if (dist_sq < h_avg * h_avg) {
+                const bool same = !MULTI || idm.y == mi;
+                if (pass == 0 && same) {
+                    mind = fmin(mind, dist_sq);
+                    if (cnt < ncap) out[(long)cnt * 64] = idm.x;
+                    cnt++;
+                } else if (MULTI && pass == 1 && !same) {
+                    mind_cp = fmin(mind_cp, dist_sq);
+                    if (penalty) {
+                        if (ccnt < ncap) cont_list(d, i)[(long)ccnt * 64] = idm.x;
+                        ccnt++;
+                    } else {
+                        if (cnt < ncap) out[(long)cnt * 64] = idm.x;
                         cnt++;
                     }
-                    return;
                 }
-                double h_avg = 2.0 * 0.5 * (pb.y + hi);
-                if (dist_sq < h_avg * h_avg) {
-                    const bool same = !MULTI || idm.y == mi;
-                    if (pass == 0 && same) {
-                        mind = fmin(mind, dist_sq);
-                        if (cnt < ncap) out[(long)cnt * 64] = MULTI ? idm.x : ci[s].x;
-                        cnt++;
-                    } else if (MULTI && pass == 1 && !same) {
-                        mind_cp = fmin(mind_cp, dist_sq);
-                        if (penalty) {
-                            if (ccnt < ncap) cont_list(d, i)[(long)ccnt * 64] = idm.x;
-                            ccnt++;
-                        } else {
-                            if (cnt < ncap) out[(long)cnt * 64] = idm.x;
-                            cnt++;
-                        }
-                    }
-                }
-            };
-            auto walk = [&](int st, int en) {
-                // candidates four at a time: the loads of a group are issued
-                // together (past the run's end they re-read its last
-                // candidate), the tests run in walk order
-                for (int s = st; s < en; s += 4) {
-                    int2 idm[4];
-                    double2 pa[4], pb[4];
+            }
+        };
+        // candidates [st, en) of the slice holding [base, base +
+        // LIST_STAGE), four at a time: the loads of a group are issued
+        // together (past the run's end they re-read its last candidate), the
+        // tests run in walk order
+        auto walk = [&](int st, int en, int base) {
+            for (int s = st; s < en; s += 4) {
+                int2 idm[4];
+                double2 pa[4], pb[4];
 #pragma unroll
-                    for (int u = 0; u < 4; u++) {
-                        const int q = min(s + u, en - 1);
-                        if (MULTI && !SKIN) idm[u] = ci[q];
-                        pa[u] = cd[2 * q];
-                        pb[u] = cd[2 * q + 1];
-                    }
-#pragma unroll
-                    for (int u = 0; u < 4; u++)
-                        if (s + u < en) consider(s + u, pa[u], pb[u], idm[u]);
+                for (int u = 0; u < 4; u++) {
+                    const int q = min(s + u, en - 1) - base;
+                    if (MULTI && !SKIN) idm[u] = idl[q];
+                    if (!MULTI && !SKIN) idm[u].x = idsl[q];
+                    pa[u] = rec[2 * q];
+                    pb[u] = rec[2 * q + 1];
                 }
-            };
-            for (int n = lo2; n <= hi2; n++) {
-                const double fz = n < b2 ? flo[2] : (n > b2 ? fhi[2] : 0.0);
-                for (int m = lo1; m <= hi1; m++) {
-                    const double fy = m < b1 ? flo[1] : (m > b1 ? fhi[1] : 0.0);
-                    const double dyz = fz * fz + fy * fy;
-                    // rows and x-run end cells out of reach are skipped when
-                    // they are for every lane of the wavefront, so that the
-                    // lanes of a cell keep walking the same candidates together
-                    if (__all(dyz >= reach2)) continue;
-                    int l0 = lo0, l1 = hi0;
-                    if (__all(l0 == b0 || dyz + flo[0] * flo[0] >= reach2)) l0 = b0;
-                    if (__all(l1 == b0 || dyz + fhi[0] * fhi[0] >= reach2)) l1 = b0;
-                    // the run is contiguous in the sorted order; a row holding a
-                    // ghost-headed cell is walked cell by cell so that that
-                    // cell's real particles stay hidden
-                    const int c0 = (n * L1 + m) * L0;
-                    bool ghosty = false;
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+                    if (s + u < en) consider(s + u, pa[u], pb[u], idm[u]);
+            }
+        };
+#pragma unroll 1
+        for (int dz = -1; dz <= 1; dz++) {
+#pragma unroll 1
+            for (int dy = -1; dy <= 1; dy++) {
+                const int n = b2 + dz, m = b1 + dy;
+                const double fz = dz < 0 ? flo[2] : (dz > 0 ? fhi[2] : 0.0);
+                const double fy = dy < 0 ? flo[1] : (dy > 0 ? fhi[1] : 0.0);
+                const double dyz = fz * fz + fy * fy;
+                // this lane's run: the row's cells b0 - 1 .. b0 + 1, less the
+                // end cells and the row out of reach
+                const bool on = binned && n >= 0 && n < L2 && m >= 0 && m < L1 && dyz < reach2;
+                int l0 = b0, l1 = b0, st = 0, en = 0;
+                const int c0 = (n * L1 + m) * L0;
+                bool ghosty = false;
+                if (on) {
+                    if (b0 > 0 && dyz + flo[0] * flo[0] < reach2) l0 = b0 - 1;
+                    if (b0 + 1 < L0 && dyz + fhi[0] * fhi[0] < reach2) l1 = b0 + 1;
+                    st = d.cell_start[c0 + l0];
+                    en = d.cell_start[c0 + l1] + d.cell_count[c0 + l1];
+                    // a row holding a ghost-headed cell is walked cell by cell
+                    // so that that cell's real particles stay hidden
                     if (!SKIN)
                         for (int l = l0; l <= l1; l++) ghosty |= d.gany[c0 + l] != 0;
+                }
+                // the wavefront's span of runs
+                int lo = st < en ? st : INT_MAX, hi_s = st < en ? en : INT_MIN;
+                for (int off = 32; off > 0; off >>= 1) {
+                    lo = min(lo, __shfl_xor(lo, off, 64));
+                    hi_s = max(hi_s, __shfl_xor(hi_s, off, 64));
+                }
+                // the span in slices, each walked by the lanes over their
+                // runs' share of it, in order
+                for (int cb = lo; cb < hi_s; cb += LIST_STAGE) {
+                    const int ce = min(cb + LIST_STAGE, hi_s);
+                    __builtin_amdgcn_wave_barrier();       // the slice's last readers are done
+                    for (int k = lane; k < ce - cb; k += 64) {
+                        rec[2 * k] = cd[2 * (cb + k)];
+                        rec[2 * k + 1] = cd[2 * (cb + k) + 1];
+                        if (MULTI && !SKIN) idl[k] = ci[cb + k];
+                        if (!MULTI && !SKIN) idsl[k] = ci[cb + k].x;
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    if (!on) continue;
                     if (!ghosty) {
-                        walk(d.cell_start[c0 + l0], d.cell_start[c0 + l1] + d.cell_count[c0 + l1]);
+                        walk(max(st, cb), min(en, ce), cb);
                     } else {
                         for (int l = l0; l <= l1; l++) {
                             const int cc = c0 + l;
                             if (d.gany[cc]) continue;
-                            walk(d.cell_start[cc], d.cell_start[cc] + d.cell_count[cc]);
+                            const int cs = d.cell_start[cc];
+                            walk(max(cs, cb), min(cs + d.cell_count[cc], ce), cb);
                         }
                     }
                 }
             }
         }
-        if (SKIN) {
-            skin_record(d, t, i, c, xi, yi, zi, hi, g);
+    }
+    if (live) {
+        int nreal;
+        if (binned) {
+            if (SKIN) {
+                skin_record(d, t, i, c, xi, yi, zi, hi, g);
+            } else {
+                if (d.disctype == 1) {
+                    if (cnt < ncap) out[(long)cnt * 64] = i;
+                    cnt++;
+                }
+            }
+            nreal = cnt;
+            if (!SKIN && d.boundary) {
+                const int lo0 = max(b0 - 1, 0), hi0 = min(b0 + 1, L0 - 1);
+                const int lo1 = max(b1 - 1, 0), hi1 = min(b1 + 1, L1 - 1);
+                const int lo2 = max(b2 - 1, 0), hi2 = min(b2 + 1, L2 - 1);
+                ghost_walk<MULTI>(d, cur, i, mi, xi, yi, zi, hi, lo0, hi0, lo1, hi1, lo2, hi2, L0, L1, penalty, out,
+                                  cnt, gccnt, mind, mind_cp);
+            }
+            mind = sqrt(penalty ? mind : fmin(mind, mind_cp));
+            // a particle with real and ghost contact partners: the reference
+            // reads the ghost partners from the real contact list past its end
+            // (mcm_get_j_cont_info.f90:44-48), so its repulsion is undefined
+            if (penalty && ccnt > 0 && gccnt > 0) {
+                atomicOr(&sc->error_flags, 32);
+                atomicExch(&sc->abort, 1);
+            }
         } else {
-            if (d.disctype == 1) {
-                if (cnt < ncap) out[(long)cnt * 64] = i;
+            if (!SKIN && d.disctype == 1) {
+                if (cnt < ncap) out[0] = i;
                 cnt++;
             }
+            nreal = cnt;
         }
-        nreal = cnt;
-        if (!SKIN && d.boundary)
-            ghost_walk<MULTI>(d, cur, i, mi, xi, yi, zi, hi, lo0, hi0, lo1, hi1, lo2, hi2, L0, L1, penalty, out, cnt,
-                              gccnt, mind, mind_cp);
-        mind = sqrt(penalty ? mind : fmin(mind, mind_cp));
-        // a particle with real and ghost contact partners: the reference
-        // reads the ghost partners from the real contact list past its end
-        // (mcm_get_j_cont_info.f90:44-48), so its repulsion is undefined
-        if (penalty && ccnt > 0 && gccnt > 0) {
-            atomicOr(&sc->error_flags, 32);
-            atomicExch(&sc->abort, 1);
+        if (SKIN) {
+            d.skin_n[i] = cnt;
+        } else {
+            d.nreal[i] = nreal;
+            d.ntot[i] = cnt;
+            d.mindist[i] = mind;
+            if (penalty) d.ncont[i] = ccnt;
+            cnt = max(cnt, ccnt);   // both lists share the capacity ncap
         }
-    } else {
-        if (!SKIN && d.disctype == 1) {
-            if (cnt < ncap) out[0] = i;
-            cnt++;
-        }
-        nreal = cnt;
-    }
-    if (SKIN) {
-        d.skin_n[i] = cnt;
-    } else {
-        d.nreal[i] = nreal;
-        d.ntot[i] = cnt;
-        d.mindist[i] = mind;
-        if (penalty) d.ncont[i] = ccnt;
-        cnt = max(cnt, ccnt);   // both lists share the capacity ncap
-    }
     }
     // longest list of the wavefront, one atomic per wavefront
     for (int off = 32; off > 0; off >>= 1) cnt = max(cnt, __shfl_xor(cnt, off, 64));
