@@ -848,10 +848,12 @@ template <bool MULTI, bool SKIN>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 3 : 4))) void k_list(DevState d_, int cur) {
     // threads take the binned particles in cell-sorted order, so that the
     // lanes of a wavefront share cells and read the same candidates
-    __shared__ double2 s_rec[4][2 * LIST_STAGE];
+    // each slice has room for the three records a group of four reads past
+    // a run's end (unused values)
+    __shared__ double2 s_rec[4][2 * (LIST_STAGE + 4)];
     // the candidates' ids (and materials): SKIN lists hold sorted positions
-    __shared__ int2 s_idm[MULTI && !SKIN ? 4 : 1][MULTI && !SKIN ? LIST_STAGE : 1];
-    __shared__ int s_id[!MULTI && !SKIN ? 4 : 1][!MULTI && !SKIN ? LIST_STAGE : 1];
+    __shared__ int2 s_idm[MULTI && !SKIN ? 4 : 1][MULTI && !SKIN ? LIST_STAGE + 4 : 1];
+    __shared__ int s_id[!MULTI && !SKIN ? 4 : 1][!MULTI && !SKIN ? LIST_STAGE + 4 : 1];
     Blk bk;
     const DevState& d = enter<true>(d_, bk);
     const int t = bk.b * blockDim.x + threadIdx.x;
@@ -902,9 +904,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 3 :
     int2* idl = s_idm[MULTI && !SKIN ? wv : 0];
     int* idsl = s_id[!MULTI && !SKIN ? wv : 0];
     const int spos_i = t;
+    int* op = out;   // entry cnt of the list
     for (int pass = 0; pass < (MULTI && !SKIN ? 2 : 1); pass++) {
         // candidate at sorted slot s with id and material idm (with one
         // material the self test uses the sorted slot)
+        // entry cnt of the list, at op
+        auto append = [&](int v) {
+            if (cnt < ncap) {
+                *op = v;
+                op += 64;
+            }
+            cnt++;
+        };
         auto consider = [&](int s, double2 pa, double2 pb, int2 idm) {
             if (MULTI && !SKIN ? idm.x == i : s == spos_i) return;
             double dx = pa.x - xi;
@@ -913,10 +924,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 3 :
             double dz = pb.x - zi; dist_sq = dist_sq + dz * dz;
             if (SKIN) {
                 const double r = (pb.y + hi + w) * grow;
-                if (dist_sq < r * r) {
-                    if (cnt < ncap) out[(long)cnt * 64] = s;
-                    cnt++;
-                }
+                if (dist_sq < r * r) append(s);
                 return;
             }
             double h_avg = 2.0 * 0.5 * (pb.y + hi);
@@ -924,35 +932,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 3 :
                 const bool same = !MULTI || idm.y == mi;
                 if (pass == 0 && same) {
                     mind = fmin(mind, dist_sq);
-                    if (cnt < ncap) out[(long)cnt * 64] = idm.x;
-                    cnt++;
+                    append(idm.x);
                 } else if (MULTI && pass == 1 && !same) {
                     mind_cp = fmin(mind_cp, dist_sq);
                     if (penalty) {
                         if (ccnt < ncap) cont_list(d, i)[(long)ccnt * 64] = idm.x;
                         ccnt++;
                     } else {
-                        if (cnt < ncap) out[(long)cnt * 64] = idm.x;
-                        cnt++;
+                        append(idm.x);
                     }
                 }
             }
         };
         // candidates [st, en) of the slice holding [base, base +
         // LIST_STAGE), four at a time: the loads of a group are issued
-        // together (past the run's end they re-read its last candidate), the
-        // tests run in walk order
+        // together (past the run's end they read slots whose values go
+        // unused), the tests run in walk order
         auto walk = [&](int st, int en, int base) {
-            for (int s = st; s < en; s += 4) {
+            const double2* r = rec + 2 * (st - base);
+            const int2* pm = idl + (st - base);
+            const int* pi = idsl + (st - base);
+            for (int s = st; s < en; s += 4, r += 8, pm += 4, pi += 4) {
                 int2 idm[4];
                 double2 pa[4], pb[4];
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
-                    const int q = min(s + u, en - 1) - base;
-                    if (MULTI && !SKIN) idm[u] = idl[q];
-                    if (!MULTI && !SKIN) idm[u].x = idsl[q];
-                    pa[u] = rec[2 * q];
-                    pb[u] = rec[2 * q + 1];
+                    if (MULTI && !SKIN) idm[u] = pm[u];
+                    if (!MULTI && !SKIN) idm[u].x = pi[u];
+                    pa[u] = r[2 * u];
+                    pb[u] = r[2 * u + 1];
                 }
 #pragma unroll
                 for (int u = 0; u < 4; u++)

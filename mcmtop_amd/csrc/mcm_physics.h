@@ -231,6 +231,41 @@ __device__ __forceinline__ void pair_grad_fast(double xij0, double xij1, double 
     double w;
     pair_wgrad_fast(xij0, xij1, xij2, havg, pi, w, dwdx);
 }
+
+// W and grad W of one pair for the one-pass corrected kernels, which are
+// held to a tolerance rather than to the reference's bits: one reciprocal
+// of havg and the reciprocal square root of rad^2, each refined by one
+// Newton step (relative error near 2^-50), instead of three correctly
+// rounded quotients and a square root, with contracted multiply-adds; ipi =
+// 1 / pi.  Same operand ranges as pair_wgrad_fast (the fast path's); the
+// self pair (rad = 0) gives W(0) and a zero gradient.
+__device__ __forceinline__ void pair_wgrad_1p(double xij0, double xij1, double xij2, double havg, double ipi,
+                                              double& w, double dwdx[3]) {
+#pragma clang fp contract(fast)
+    const double rad2 = xij0 * xij0 + xij1 * xij1 + xij2 * xij2;
+    double y = __builtin_amdgcn_rsq(rad2);
+    y = y * (1.5 - 0.5 * rad2 * (y * y));
+    const bool zero = rad2 == 0.0;
+    const double rad = zero ? 0.0 : rad2 * y;
+    double ih = __builtin_amdgcn_rcp(havg);
+    ih = fma(ih, fma(-havg, ih, 1.0), ih);
+    const double const1 = ipi * (ih * ih * ih);
+    const double const2 = const1 * 0.25;
+    const double z = rad * ih;
+    const double t = 2.0 - z;
+    const double d1 = const1 * ((2.25 * z - 3.0) * z);
+    const double d2 = -(3.0 * const2 * (t * t));
+    const double d12 = __builtin_unpredictable(z < 2.0) ? d2 : 0.0;
+    const double dw = __builtin_unpredictable(z < 1.0) ? d1 : d12;
+    const double w1 = const1 * (((0.75 * z - 1.5) * z) * z + 1.0);
+    const double w2 = const2 * (t * (t * t));
+    const double w12 = __builtin_unpredictable(z < 2.0) ? w2 : 0.0;
+    w = __builtin_unpredictable(z < 1.0) ? w1 : w12;
+    const double g = zero ? 0.0 : dw * (ih * y);   // dW/dr / (rad havg)
+    dwdx[0] = g * xij0;
+    dwdx[1] = g * xij1;
+    dwdx[2] = g * xij2;
+}
 #endif
 
 // W and grad W of one pair: the fast path, or the checked functions

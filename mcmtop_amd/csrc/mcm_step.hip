@@ -281,13 +281,30 @@ __device__ __forceinline__ void strain_corr(const DevState& d, int i, int t, con
     }
 }
 
+// W and grad W of one pair of the one-pass kernels: pair_wgrad_1p on the
+// fast path, else the checked functions
+template <bool FAST>
+__device__ __forceinline__ void pair_wgrad_one_pass(double xij0, double xij1, double xij2, double havg, double pi,
+                                                    double ipi, double& w, double dwdx[3]) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (FAST) {
+        pair_wgrad_1p(xij0, xij1, xij2, havg, ipi, w, dwdx);
+        return;
+    }
+#endif
+    (void)ipi;
+    pair_wgrad<false>(xij0, xij1, xij2, havg, pi, w, dwdx);
+}
+
 // corrected strain rate in one pass: the correction is linear in the pass-1
 // sums, so B and the contracted gradient follow from single-pass moments
 // (same result up to rounding; the fast variant)
 template <bool FAST>
 __device__ __forceinline__ void strain_corr_fast(const DevState& d, int i, int t, const double xi[3],
                                                  double hi, double f[3][3]) {
+#pragma clang fp contract(fast)
     const int n = d.ntot[i];
+    const double ipi = 1.0 / d.pi;
     const int* lst = nbr_row(d, t);
     double sw = 0.0, sg[3] = {0, 0, 0}, P[3][3] = {{0}}, Pw[3] = {0, 0, 0}, Q[3][3] = {{0}}, Qw[3] = {0, 0, 0};
     NbrStream<SRec> st(lst, d.rs, n, i);
@@ -295,8 +312,9 @@ __device__ __forceinline__ void strain_corr_fast(const DevState& d, int i, int t
         int j;
         SRec r;
         st.next(j, r);
-        double xj[3], w, dwdx[3];
-        corr_pair<FAST>(r, xi, hi, d.pi, xj, w, dwdx);
+        double xj[3] = {r.x, r.y, r.z}, w, dwdx[3];
+        pair_wgrad_one_pass<FAST>(xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2], 0.5 * (hi + r.h), d.pi, ipi, w,
+                                  dwdx);
         double vw = r.vol * w;
         double vj[3] = {r.vx, r.vy, r.vz};
         sw += vw;
@@ -821,7 +839,9 @@ __device__ __forceinline__ void accel_corr(const DevState& d, int i, int t, cons
 // corrected acceleration in one pass from single-pass moments (fast variant)
 template <bool FAST, bool ASYM>
 __device__ __forceinline__ void accel_corr_fast(const DevState& d, int i, int t, const MRecT<ASYM>& me, double acc[3]) {
+#pragma clang fp contract(fast)
     const int n = d.ntot[i];
+    const double ipi = 1.0 / d.pi;
     const int self_k = d.nreal[i] - 1;
     const int* lst = nbr_row(d, t);
     double sw = 0.0, sg[3] = {0, 0, 0}, P[3][3] = {{0}}, Pw[3] = {0, 0, 0};
@@ -833,8 +853,8 @@ __device__ __forceinline__ void accel_corr_fast(const DevState& d, int i, int t,
         MRecT<ASYM> r;
         st.next(j, r);
         double w, dwdx[3];
-        corr_pair_j<FAST>(r, me, d.pi, w, dwdx);
         double dx[3] = {r.x - me.x, r.y - me.y, r.z - me.z};
+        pair_wgrad_one_pass<FAST>(dx[0], dx[1], dx[2], 0.5 * (me.h + r.h), d.pi, ipi, w, dwdx);
         double vw = r.vol * w;
         sw += vw;
         for (int a = 0; a < 3; a++) {
