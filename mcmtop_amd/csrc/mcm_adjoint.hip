@@ -912,6 +912,82 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_inter(DevState d, InterLayout L, 
     }
 }
 
+// One recorded step's neighbour-search outputs, everything of the search
+// that later kernels read: the blocked lists' rows below the step's longest
+// list, nreal, ntot, mindist, the thread order (sorted, spos), the real
+// particles' strain records, the ghosts (position, half-step position,
+// velocity, mass, both h and rho buffers, material, parent, mirror signs,
+// strain records and momentum records) and the range flags; the scalars
+// after the search are
+// kept beside it.  Slots are sized for the run's capacities (npad, ncap,
+// ngcap) when the recording starts and used only while they are unchanged.
+struct SearchSlot {
+    int *rows, *nreal, *ntot, *sorted, *spos, *gpmat, *gpar, *gsgn, *range;
+    double *mindist, *rs, *rmg, *gd;   // rmg: the ghosts' momentum records; gd: [14][ngcap] x0 3, x1 3, v 3,
+                                       // mass, hbuf 2, rbuf 2
+};
+constexpr int SNAP_GD = 14;
+
+__device__ __forceinline__ double* snap_gfield(const DevState& d, int f) {
+    if (f < 3) return d.x0[f];
+    if (f < 6) return d.x1[f - 3];
+    if (f < 9) return d.v[f - 6];
+    if (f == 9) return d.mass;
+    if (f < 12) return d.hbuf[f - 10];
+    return d.rbuf[f - 12];
+}
+
+// save (1) or restore (0) one step's search outputs; on restore the
+// step's scalars (ngp, max_count) are in place first
+__global__ __launch_bounds__(MCM_BLOCK) void k_adj_search(DevState d, SearchSlot p, int save) {
+    const long np = d.np, ngp = d.sc->ngp, gcap = d.ngcap;
+    const long tid = blockIdx.x * (long)blockDim.x + threadIdx.x, nt = (long)gridDim.x * blockDim.x;
+    const long rows = min(d.sc->max_count, d.ncap), nblk64 = (d.npad + 63) / 64;
+    const long span = rows * 64, n = nblk64 * span;
+    for (long k = tid; k < n; k += nt) {
+        const long b = k / span, q = b * d.ncap * 64 + (k - b * span);
+        if (save) p.rows[q] = d.nbr[q];
+        else d.nbr[q] = p.rows[q];
+    }
+    for (long i = tid; i < np; i += nt) {
+        if (save) {
+            p.nreal[i] = d.nreal[i]; p.ntot[i] = d.ntot[i]; p.sorted[i] = d.sorted[i]; p.spos[i] = d.spos[i];
+            p.mindist[i] = d.mindist[i];
+        } else {
+            d.nreal[i] = p.nreal[i]; d.ntot[i] = p.ntot[i]; d.sorted[i] = p.sorted[i]; d.spos[i] = p.spos[i];
+            d.mindist[i] = p.mindist[i];
+        }
+    }
+    // strain records of the reals and the step's ghosts
+    const long nrs = RS_SLOTS * 2 * (np + ngp);
+    for (long k = tid; k < nrs; k += nt) {
+        if (save) p.rs[k] = d.rs[k];
+        else d.rs[k] = p.rs[k];
+    }
+    const long nrm = RM_SLOTS * 2 * ngp, rm0 = RM_SLOTS * 2 * np;
+    for (long k = tid; k < nrm; k += nt) {
+        if (save) p.rmg[k] = d.rm[rm0 + k];
+        else d.rm[rm0 + k] = p.rmg[k];
+    }
+    for (long g = tid; g < ngp; g += nt) {
+        const long u = np + g;
+        if (save) {
+            p.gpmat[g] = d.pmat[u]; p.gpar[g] = d.gpar[g]; p.gsgn[g] = d.gsgn[g];
+        } else {
+            d.pmat[u] = p.gpmat[g]; d.gpar[g] = p.gpar[g]; d.gsgn[g] = p.gsgn[g];
+        }
+        for (int f = 0; f < SNAP_GD; f++) {
+            double* a = snap_gfield(d, f);
+            if (save) p.gd[f * gcap + g] = a[u];
+            else a[u] = p.gd[f * gcap + g];
+        }
+    }
+    if (tid == 0) {
+        if (save) p.range[0] = d.range[0];
+        else d.range[0] = p.range[0];
+    }
+}
+
 // ================================================================== host
 struct mcm_gpu_adjoint {
     mcm_gpu_ctx* ctx = nullptr;
@@ -934,6 +1010,16 @@ struct mcm_gpu_adjoint {
     std::vector<void*> iallocs;       // intermediate slots (sized by the run's capacities)
     int* d_sample = nullptr;
     double* d_hist = nullptr;         // [nsteps][nsample][3]: positions, then their adjoints
+    // every recorded step's neighbour-search outputs (k > 1, when they fit),
+    // so that the reverse sweep's recomputation restores them instead of
+    // searching again: the same lists, order, ghosts and records
+    int* snap_i = nullptr;            // [nsteps][snap_ni]
+    double* snap_d = nullptr;         // [nsteps][snap_nd]
+    Scalars* snap_sc = nullptr;       // [nsteps]: the scalars after the search
+    size_t snap_ni = 0, snap_nd = 0;
+    int snap_npad = 0, snap_ncap = 0, snap_ngcap = 0;   // capacities the slots are sized for
+    bool snap_ok = false;             // every step of the last recording saved at those capacities
+    std::vector<void*> lallocs;
     AdjDev a{};
     std::vector<void*> allocs, gallocs;   // run-long and ghost-sized allocations
     std::string err;
@@ -992,18 +1078,97 @@ int adj_restore(mcm_gpu_adjoint* A, bool ck, int s) {
     return 0;
 }
 
+// search slots for every step when they fit in an eighth of the free device
+// memory and in 32 GB; otherwise the reverse sweep searches again
+void adj_snap_alloc(mcm_gpu_adjoint* A) {
+    const DevState& d = A->ctx->d;
+    const size_t ni = (size_t)d.npad * d.ncap + 4 * (size_t)d.np + 3 * (size_t)d.ngcap + 4;
+    const size_t nd = (size_t)d.np + (size_t)RS_SLOTS * 2 * ((size_t)d.np + d.ngcap) +
+                      (size_t)RM_SLOTS * 2 * d.ngcap + SNAP_GD * (size_t)d.ngcap;
+    const size_t bytes = (size_t)A->nsteps * (ni * sizeof(int) + nd * sizeof(double) + sizeof(Scalars));
+    size_t freeb = 0, totalb = 0;
+    if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) return;
+    if (bytes > freeb / 8 || bytes > (32ull << 30)) return;
+    if (adj_alloc(A, A->lallocs, &A->snap_i, ni * A->nsteps) || adj_alloc(A, A->lallocs, &A->snap_d, nd * A->nsteps) ||
+        adj_alloc(A, A->lallocs, &A->snap_sc, (size_t)A->nsteps)) {
+        for (void* p : A->lallocs) (void)hipFree(p);
+        A->lallocs.clear();
+        A->snap_i = nullptr;
+        A->snap_d = nullptr;
+        A->snap_sc = nullptr;
+        (void)hipGetLastError();
+        A->err.clear();
+        return;
+    }
+    A->snap_ni = ni;
+    A->snap_nd = nd;
+    A->snap_npad = d.npad;
+    A->snap_ncap = d.ncap;
+    A->snap_ngcap = d.ngcap;
+}
+
+bool adj_snap_fits(const mcm_gpu_adjoint* A) {
+    const DevState& d = A->ctx->d;
+    return A->snap_i && d.npad == A->snap_npad && d.ncap == A->snap_ncap && d.ngcap == A->snap_ngcap;
+}
+
+// save (1) or restore (0) step n's search outputs
+int adj_search(mcm_gpu_adjoint* A, int n, int save) {
+    mcm_gpu_ctx* ctx = A->ctx;
+    const DevState& d = ctx->d;
+    SearchSlot p;
+    int* pi = A->snap_i + (size_t)n * A->snap_ni;
+    double* pd = A->snap_d + (size_t)n * A->snap_nd;
+    const long np = d.np, gc = d.ngcap;
+    p.rows = pi; pi += (size_t)d.npad * d.ncap;
+    p.nreal = pi; pi += np;
+    p.ntot = pi; pi += np;
+    p.sorted = pi; pi += np;
+    p.spos = pi; pi += np;
+    p.gpmat = pi; pi += gc;
+    p.gpar = pi; pi += gc;
+    p.gsgn = pi; pi += gc;
+    p.range = pi;
+    p.mindist = pd; pd += np;
+    p.rs = pd; pd += (size_t)RS_SLOTS * 2 * (np + gc);
+    p.rmg = pd; pd += (size_t)RM_SLOTS * 2 * gc;
+    p.gd = pd;
+    if (save) {
+        HIPCHK_A(hipMemcpyAsync(A->snap_sc + n, ctx->sc, sizeof(Scalars), hipMemcpyDeviceToDevice, ctx->stream));
+    } else {
+        HIPCHK_A(hipMemcpyAsync(ctx->sc, A->snap_sc + n, sizeof(Scalars), hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    const long work = std::max((long)d.npad * d.ncap, (long)RS_SLOTS * 2 * (np + gc));
+    k_adj_search<<<(unsigned)std::min<long>(blocks_for(work / 8 + 1, MCM_BLOCK), 2048), MCM_BLOCK, 0, ctx->stream>>>(
+        d, p, save);
+    HIPCHK_A(hipGetLastError());
+    return 0;
+}
+
 // nsteps fused steps from the current state, the sampled positions after each
 // step written to hist (device) when given; a step that outgrows a capacity
 // makes the run grow and the call start again from checkpoint s, which holds
-// its first state.
-int adj_steps(mcm_gpu_adjoint* A, int nsteps, double* hist, int s) {
+// its first state.  first: the index of the first step in the recording
+// (whose list-build outputs are kept when the recording keeps them).
+int adj_steps(mcm_gpu_adjoint* A, int nsteps, double* hist, int s, int first) {
     mcm_gpu_ctx* ctx = A->ctx;
     int rc = 0;
     for (int attempt = 0; attempt < 16; attempt++) {
         Scalars s0;
         if ((rc = get_scalars_raw(ctx, &s0))) { A->err = ctx->err; return rc; }
+        const bool keep = A->snap_ok && adj_snap_fits(A);
+        if (!keep) A->snap_ok = false;
         for (int n = 0; n < nsteps; n++) {
-            launch_step(one(ctx), ctx->cur);
+            if (keep) {
+                const Target T = one(ctx);
+                launch_neighbours(T, ctx->cur);
+                if ((rc = adj_search(A, first + n, 1))) return rc;
+                launch_strain(T, ctx->cur);
+                launch_constitutive(T, ctx->cur);
+                launch_momentum(T, ctx->cur);
+            } else {
+                launch_step(one(ctx), ctx->cur);
+            }
             ctx->cur ^= 1;
             if (hist) k_adj_sample<<<blocks_for(A->nsample, 64), 64, 0, ctx->stream>>>(
                 ctx->d, A->d_sample, A->nsample, hist + (size_t)n * 3 * A->nsample);
@@ -1095,12 +1260,17 @@ int adj_granular_rest(mcm_gpu_adjoint* A) {
 // intermediate the reverse step reads, as the granular stages do, in four
 // pair and particle launches instead of eleven (plain SPH, fixed h, no
 // contact or XSPH: the adjoint's scope)
-int adj_fused_diag_step(mcm_gpu_adjoint* A) {
+int adj_fused_diag_step(mcm_gpu_adjoint* A, int n) {
     mcm_gpu_ctx* ctx = A->ctx;
     const DevState& d = ctx->d;
     hipStream_t st = ctx->stream;
     const int cur = ctx->cur;
-    launch_neighbours(one(ctx), cur);
+    if (A->snap_ok && adj_snap_fits(A)) {   // the recording's search of step n
+        int rc = adj_search(A, n, 0);
+        if (rc) return rc;
+    } else {
+        launch_neighbours(one(ctx), cur);
+    }
     k_adj_check<<<1, 1, 0, st>>>(ctx->d, A->a);
     k_strain_const<MODE_FSTRAIN, 0, false><<<ctx->nb, MCM_BLOCK, 0, st>>>(ctx->d, cur);
     k_strain_const<MODE_FCONST, 0, true><<<ctx->nb, MCM_BLOCK, 0, st>>>(ctx->d, cur);
@@ -1254,10 +1424,11 @@ int adj_record(mcm_gpu_adjoint* A, double* hist) {
     if (A->store) {
         if ((rc = adj_record_store(A))) return rc;
     } else {
+        A->snap_ok = adj_snap_fits(A);
         for (int sgi = 0; sgi < A->nseg; sgi++) {
             const int s0 = sgi * A->k, s1 = std::min(s0 + A->k, nsteps);
             if ((rc = adj_save(A, true, sgi))) return rc;
-            if ((rc = adj_steps(A, s1 - s0, nsample > 0 ? A->d_hist + (size_t)s0 * 3 * nsample : nullptr, sgi)))
+            if ((rc = adj_steps(A, s1 - s0, nsample > 0 ? A->d_hist + (size_t)s0 * 3 * nsample : nullptr, sgi, s0)))
                 return rc;
         }
     }
@@ -1393,6 +1564,7 @@ int mcm_gpu_adj_forward(mcm_gpu_ctx* ctx, int32_t nsteps, int32_t ckpt_every, co
     rc |= adj_alloc(A, A->allocs, &a.as, AS_N);
     rc |= adj_alloc(A, A->allocs, &a.err, 1);
     if (rc) return rc;
+    if (!A->store) adj_snap_alloc(A);
     a.gtcap = 0;
     a.ngcap = 0;
     if (nsample > 0)
@@ -1474,7 +1646,7 @@ static int adj_sweep(mcm_gpu_adjoint* A, const double* xbar, const double* vbar,
         if ((rc = adj_restore(A, true, sgi))) return rc;
         for (int n = s0; n < s1; n++) {
             if ((rc = adj_save(A, false, n - s0))) return rc;
-            if ((rc = adj_fused_diag_step(A))) return rc;
+            if ((rc = adj_fused_diag_step(A, n))) return rc;
             if ((rc = adj_inter(A, n - s0, true))) return rc;
         }
         for (int n = s1 - 1; n >= s0; n--) {
@@ -1542,6 +1714,7 @@ void mcm_gpu_adj_destroy(mcm_gpu_adjoint* A) {
     for (void* p : A->allocs) (void)hipFree(p);
     for (void* p : A->gallocs) (void)hipFree(p);
     for (void* p : A->iallocs) (void)hipFree(p);
+    for (void* p : A->lallocs) (void)hipFree(p);
     delete A;
 }
 
