@@ -158,9 +158,11 @@ def timed_run(deck_path: str, passes: int, steps: int, warmup: int, skin: float 
             "sc": sc, "skin_builds": builds_timed}
 
 
-def batched_leg(nruns: int, seed: int, steps: int, warmup: int, check_every: int, plain: bool = True):
+def batched_leg(nruns: int, seed: int, steps: int, warmup: int, check_every: int, plain: bool = True,
+                passes: int = 1):
     """Throughput of nruns uneven bars stepped as one batch and (plain) stepped
-    one run at a time; equality of sampled runs between the two."""
+    one run at a time, both with the headline's corrected passes; equality of
+    sampled runs between the two."""
     import numpy as np
     from mcmtop_amd import deck, gpu
     t = time.perf_counter()
@@ -170,7 +172,7 @@ def batched_leg(nruns: int, seed: int, steps: int, warmup: int, check_every: int
     print("bench: %d batch decks, %d particles in %.1f s" % (nruns, total, time.perf_counter() - t),
           file=sys.stderr, flush=True)
     t = time.perf_counter()
-    b = gpu.Batch(decks)
+    b = gpu.Batch(decks, corrected_one_pass=passes == 1)
     b.initial()
     print("bench: batch initialised in %.1f s" % (time.perf_counter() - t), file=sys.stderr, flush=True)
     b.step(warmup)
@@ -188,7 +190,7 @@ def batched_leg(nruns: int, seed: int, steps: int, warmup: int, check_every: int
     secs_p = 0.0
     worst = 0.0
     for k, dk in enumerate(decks if plain else []):
-        with gpu.Solver(dk, pair_mapping=mapping) as s:
+        with gpu.Solver(dk, pair_mapping=mapping, corrected_one_pass=passes == 1) as s:
             s.initial()
             s.step(warmup + tune_steps)
             t0 = time.perf_counter()
@@ -206,7 +208,7 @@ def batched_leg(nruns: int, seed: int, steps: int, warmup: int, check_every: int
     v_p = total * steps / secs_p if plain else None
     return {"runs": nruns, "particles": total, "seed": seed,
             "run_particles": {"min": int(sizes.min()), "median": float(np.median(sizes)), "max": int(sizes.max())},
-            "steps": steps, "warmup": warmup, "pair_mapping": mapping,
+            "steps": steps, "warmup": warmup, "pair_mapping": mapping, "corrected_passes": passes,
             "value": v_b, "unit": "particle-steps/s", "ms_per_step": 1e3 * secs_b / steps,
             "plain_value": v_p, "plain_ms_per_step": 1e3 * secs_p / steps if plain else None,
             "speedup": v_b / v_p if plain else None,
@@ -548,7 +550,8 @@ def main():
         "other_mode": other,
     }
     if args.batch > 0:
-        res["batched"] = batched_leg(args.batch, args.batch_seed, args.batch_steps, 4, 16, bool(args.batch_plain))
+        res["batched"] = batched_leg(args.batch, args.batch_seed, args.batch_steps, 4, 16, bool(args.batch_plain),
+                                     args.passes)
     res["adjoint_forward_ratio"] = None
     if args.ident_config5 > 0:
         res["identification_at_size"] = ident_config5(args.ident_config5, 600, 40, args.ident_config5_iters,
