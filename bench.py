@@ -55,12 +55,12 @@ FP64_PEAK_TFLOPS = 78.6
 # (kernel, corrected passes).
 KERNELS = {
     ("strain", 3): {"name": "k_strain_const<MODE_FSTRAIN, corrected>",
-                    "symbol": "void k_strain_const<4, 1, false>(DevState, int)",
+                    "symbol": "void k_strain_const<4, 1, false, 0>(DevState, int)",
                     "bytes": (188.0, 4.0), "flops": (242.0, 81.0)},
     ("momentum", 3): {"name": "k_momentum<MODE_FUSED, corrected>", "symbol": "void k_momentum<0, 1, false>(DevState, int)",
                       "bytes": (208.0, 4.0), "flops": (209.0, 163.0)},
     ("strain", 1): {"name": "k_strain_const<MODE_FSTRAIN, corrected one-pass>",
-                    "symbol": "void k_strain_const<4, 2, false>(DevState, int)",
+                    "symbol": "void k_strain_const<4, 2, false, 0>(DevState, int)",
                     "bytes": (188.0, 4.0), "flops": (100.0, 150.0)},
     ("momentum", 1): {"name": "k_momentum<MODE_FUSED, corrected one-pass>",
                       "symbol": "void k_momentum<0, 2, false>(DevState, int)",

@@ -687,6 +687,14 @@ int pair_variant(const mcm_gpu_ctx* c) { return corr_variant(c) + (use_wave(c) ?
 // XSPH moves the particles after the velocity update
 // (runs with a cap-model material: 8 + corr_variant, or 11 + corr_variant
 // with XSPH, the kernels that read the stress's lower triangle)
+// constitutive kernel of the fused step: 1 when every material is f3dm10 (the
+// kernel compiled for that model alone), else 0
+int const_variant(const mcm_gpu_ctx* c) {
+    for (int m = 0; m < c->d.nummat; m++)
+        if (c->deck.mat[m].model != 10) return 0;
+    return 1;
+}
+
 int mom_variant(const mcm_gpu_ctx* c) {
     if (c->d.asym) return (c->d.veloc_opt == 1 ? 11 : 8) + corr_variant(c);
     return c->d.veloc_opt == 1 ? 5 + corr_variant(c) : pair_variant(c);
@@ -696,7 +704,7 @@ int mom_variant(const mcm_gpu_ctx* c) {
 // workgroups one run needs in a launch of that kind and kernel variant v;
 // 0 means the run takes no part in it.
 enum LaunchKind { L_ONE, L_PART, L_CELLS, L_TILES, L_GSCAN, L_GFILL, L_GSTRESS, L_LIST, L_PAIR, L_MOM, L_CFOLD,
-                  L_REP, L_XSPH, L_SKIN, L_SKINP, L_SKINB, L_NKIND };
+                  L_REP, L_XSPH, L_SKIN, L_SKINP, L_SKINB, L_CONST, L_NKIND };
 constexpr int kNVariant = 16;
 static_assert(kNVariant > MCM_MAX_MAT && kNVariant > 13, "launch variants");
 
@@ -719,6 +727,7 @@ int kind_blocks(const mcm_gpu_ctx* c, int kind, int v) {
         case L_SKIN: return v == 0 && d.skin > 0.0 ? 1 : 0;
         case L_SKINP: return v == 0 && d.skin > 0.0 ? c->nb : 0;
         case L_SKINB: return v == 0 && d.skin > 0.0 ? skin_block_blocks(d) : 0;
+        case L_CONST: return const_variant(c) == v ? c->nb : 0;
     }
     return 0;
 }
@@ -838,7 +847,8 @@ void launch_strain(const Target& T, int cur) {
 // update_h + constitutive + critts of the fused step, then dt and ptime, then
 // the contact repulsion of runs with penalty contact (mcm_solution.f90:58-64)
 void launch_constitutive(const Target& T, int cur) {
-    launch(T, L_PART, 0, MCM_BLOCK, k_strain_const<MODE_FCONST, 0, false>, cur);
+    launch(T, L_CONST, 0, MCM_BLOCK, k_strain_const<MODE_FCONST, 0, false>, cur);
+    launch(T, L_CONST, 1, MCM_BLOCK, k_strain_const<MODE_FCONST, 0, false, 1 << 10>, cur);
     launch(T, L_ONE, 0, MCM_BLOCK, k_strain_fold<MODE_FCONST>);
     launch(T, L_REP, 0, MCM_BLOCK, k_rep_cont, cur);
 }

@@ -899,25 +899,34 @@ MCM_HD T critts(const PStateT<T>& s, const M& m, int tcrit_opt, double mindist) 
 
 // Material update of one particle in the order of mcm_constitutive.f90:29-83.
 // Returns the yield active-set flag of the plastic models.
+// MODELS: 0 for any material model, else the bit set (1 << model) of the
+// models the run's materials use, which leaves the others out of the code
+template <int MODELS = 0>
 MCM_HD int constitutive_one(PState& s, const MatParams& m, int matid1, double dt, double& defps_prev,
                             int* eos_err) {
+    constexpr bool m1 = MODELS == 0 || (MODELS & (1 << 1)), m3 = MODELS == 0 || (MODELS & (1 << 3));
+    constexpr bool m9 = MODELS == 0 || (MODELS & (1 << 9)), m10 = MODELS == 0 || (MODELS & (1 << 10));
     int active = 0;
     switch (m.model) {
         case 1:
-            stress_rot1(s, dt); f3dm1(s, m, dt); nodalav(s, m); lieupd(s, dt);
+            if (m1) { stress_rot1(s, dt); f3dm1(s, m, dt); nodalav(s, m); lieupd(s, dt); }
             break;
         case 3:
-            stress_rot1(s, dt); active = f3dm3(s, m, dt); nodalav(s, m); lieupd(s, dt);
+            if (m3) { stress_rot1(s, dt); active = f3dm3(s, m, dt); nodalav(s, m); lieupd(s, dt); }
             break;
         case 9:
-            stress_rot1(s, dt); f3dm9(s, m, matid1);
-            *eos_err |= sueos(s, m);
-            nodalav(s, m); hieupd(s, dt); eqos(s, m, dt); stress_up(s);
+            if (m9) {
+                stress_rot1(s, dt); f3dm9(s, m, matid1);
+                *eos_err |= sueos(s, m);
+                nodalav(s, m); hieupd(s, dt); eqos(s, m, dt); stress_up(s);
+            }
             break;
         case 10:
-            stress_rot1(s, dt); active = f3dm10(s, m, dt, defps_prev);
-            *eos_err |= sueos(s, m);
-            nodalav(s, m); hieupd(s, dt); eqos(s, m, dt); stress_up(s);
+            if (m10) {
+                stress_rot1(s, dt); active = f3dm10(s, m, dt, defps_prev);
+                *eos_err |= sueos(s, m);
+                nodalav(s, m); hieupd(s, dt); eqos(s, m, dt); stress_up(s);
+            }
             break;
         default: break;
     }

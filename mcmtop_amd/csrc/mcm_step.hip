@@ -281,6 +281,14 @@ __device__ __forceinline__ void strain_corr(const DevState& d, int i, int t, con
     }
 }
 
+// loop unrolling of the one-pass pair loops (strain, momentum)
+#ifndef MCM_UNROLL_S1
+#define MCM_UNROLL_S1 1
+#endif
+#ifndef MCM_UNROLL_M1
+#define MCM_UNROLL_M1 2
+#endif
+
 // W and grad W of one pair of the one-pass kernels: pair_wgrad_1p on the
 // fast path, else the checked functions
 template <bool FAST>
@@ -308,6 +316,7 @@ __device__ __forceinline__ void strain_corr_fast(const DevState& d, int i, int t
     const int* lst = nbr_row(d, t);
     double sw = 0.0, sg[3] = {0, 0, 0}, P[3][3] = {{0}}, Pw[3] = {0, 0, 0}, Q[3][3] = {{0}}, Qw[3] = {0, 0, 0};
     NbrStream<SRec> st(lst, d.rs, n, i);
+#pragma unroll MCM_UNROLL_S1
     for (int k = 0; k < n; k++) {
         int j;
         SRec r;
@@ -362,7 +371,7 @@ __device__ __forceinline__ double strain_finish(const DevState& d, int i, double
 // (rod, spin, tracerod); MODE_CONST reads the state staged by the strain and
 // update_h stages.  Returns the particle's critical timestep, the updated h
 // (0 when not updated) and sets eos_err.
-template <int MODE, bool DIAG>
+template <int MODE, bool DIAG, int MODELS = 0>
 __device__ __forceinline__ void constitutive_particle(const DevState& d, int i, int cur, double dt, PState& s,
                                                       double x1z, double& crit, double& hm, int& eos_err) {
     const int nxt = cur ^ 1;
@@ -373,13 +382,14 @@ __device__ __forceinline__ void constitutive_particle(const DevState& d, int i, 
         if (update_h(s, d.h_opt, dt)) hm = s.h;
     }
     rhoupd(s, m, dt);
-    if (m.model == 3) for (int k = 0; k < 6; k++) s.alfa[k] = d.alfa[k][i];
-    if (m.model == 9) for (int k = 0; k < 6; k++) s.s[k] = d.sd[k][i];
+    constexpr bool has3 = MODELS == 0 || (MODELS & (1 << 3)), has9 = MODELS == 0 || (MODELS & (1 << 9));
+    if (has3 && m.model == 3) for (int k = 0; k < 6; k++) s.alfa[k] = d.alfa[k][i];
+    if (has9 && m.model == 9) for (int k = 0; k < 6; k++) s.s[k] = d.sd[k][i];
     double defps_prev = d.defps_prev[i];
     int active = 0;
     // lower triangle of sigma: 21, 31, 32
     double sl[3];
-    if (m.model == 4) {
+    if ((MODELS == 0 || (MODELS & (1 << 4))) && m.model == 4) {
         // mcm_constitutive.f90:49-54: stress_rot1, f3dm25, aviscosity, lieupd.
         // The rotation reads sigma 12, 23 and 31 (mcm_stress_rot1.f90:33-41,
         // 163-171) and writes a symmetric tensor; the cap model then writes
@@ -398,7 +408,7 @@ __device__ __forceinline__ void constitutive_particle(const DevState& d, int i, 
         d.capa[i] = capa;
         d.ptype[i] = mode;
     } else {
-        active = constitutive_one(s, m, mi + 1, dt, defps_prev, &eos_err);
+        active = constitutive_one<MODELS>(s, m, mi + 1, dt, defps_prev, &eos_err);
         sl[0] = s.sig[S12]; sl[1] = s.sig[S13]; sl[2] = s.sig[S23];
     }
     s.critts = critts(s, m, d.tcrit_opt, d.mindist[i]);
@@ -408,9 +418,9 @@ __device__ __forceinline__ void constitutive_particle(const DevState& d, int i, 
     d.hbuf[nxt][i] = s.h;
     if (d.h_opt > 0) d.hold[i] = s.hold;
     for (int k = 0; k < 6; k++) d.sig[k][i] = s.sig[k];
-    if (m.model == 3) for (int k = 0; k < 6; k++) d.alfa[k][i] = s.alfa[k];
-    if (m.model == 9 || (DIAG && m.model == 10)) for (int k = 0; k < 6; k++) d.sd[k][i] = s.s[k];
-    if (DIAG && m.model == 3) for (int k = 0; k < 6; k++) d.sd[k][i] = s.s[k];
+    if (has3 && m.model == 3) for (int k = 0; k < 6; k++) d.alfa[k][i] = s.alfa[k];
+    if ((has9 && m.model == 9) || (DIAG && m.model == 10)) for (int k = 0; k < 6; k++) d.sd[k][i] = s.s[k];
+    if (DIAG && has3 && m.model == 3) for (int k = 0; k < 6; k++) d.sd[k][i] = s.s[k];
     d.p[i] = s.p;
     d.e[i] = s.e;
     d.q[i] = s.q;
@@ -551,7 +561,7 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_strain_fold(DevState d_) {
 
 // Thread-per-particle strain (+ constitutive) kernel.  CORR: 0 plain SPH,
 // 1 corrected (reference order), 2 corrected (one pass).
-template <int MODE, int CORR, bool DIAG>
+template <int MODE, int CORR, bool DIAG, int MODELS = 0>
 // Waves per SIMD of the pair kernels (A/B-timed on the 2.1 M-particle bar).
 // The gathers are bound by the vector L1 path, and fewer resident waves
 // evict fewer of each other's neighbour lines: the one-pass corrected
@@ -635,7 +645,7 @@ __global__ __launch_bounds__(MCM_BLOCK) __attribute__((amdgpu_waves_per_eu(
             x1z = d.x1[2][i];
         }
         if (MODE != MODE_STRAIN && MODE != MODE_FSTRAIN)
-            constitutive_particle<MODE, DIAG>(d, i, cur, dt, s, x1z, crit, hm, eos_err);
+            constitutive_particle<MODE, DIAG, MODELS>(d, i, cur, dt, s, x1z, crit, hm, eos_err);
         if (MODE == MODE_FCONST) {   // mcm_calctotale.f90:33-51 of the fused step
             const double v[3] = {d.v[0][i], d.v[1][i], d.v[2][i]};
             energy_terms(ce, d.pmat[i], s.mass, s.vabs, s.e, v);
@@ -849,6 +859,7 @@ __device__ __forceinline__ void accel_corr_fast(const DevState& d, int i, int t,
     double mlast = 0.0;
     int j;
     NbrStream<MRecT<ASYM>> st(lst, d.rm, n, i, d.rml);
+#pragma unroll MCM_UNROLL_M1
     for (int k = 0; k < n; k++) {
         MRecT<ASYM> r;
         st.next(j, r);
