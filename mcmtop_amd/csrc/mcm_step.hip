@@ -462,6 +462,10 @@ __device__ __forceinline__ void constitutive_particle(const DevState& d, int i, 
 // energy and coordinate-bound partials: [cmin 3][cmax 3][ke nm][ie nm][mom 3nm]
 #define RED_C (6 + 5 * MCM_MAX_MAT)
 
+// partial k of block lb: the partials are stored value by value, [k][nblk],
+// so that the one-workgroup folds read them coalesced
+__device__ __forceinline__ double* red_at(const DevState& d, int lb, int k) { return d.red + (long)k * d.nblk + lb; }
+
 // One thread's share of the step's end-of-step reductions: its particle's
 // new position (coordinate bounds) and its terms of mcm_calctotale.f90:33-51
 // for its material mi (mi < 0: no particle).
@@ -486,12 +490,11 @@ __device__ __forceinline__ void energy_terms(CPart& c, int mi, double mass, doub
 // Block partials of every block (folded by k_c_fold): the coordinate
 // bounds [cmin 3][cmax 3] (sentinels without bounds) ...
 __device__ void reduce_bound_partials(const DevState& d, const CPart& c, bool with_bounds, int lb) {
-    double* out = d.red + (long)lb * RED_C;
     const bool has = c.mi >= 0;
     for (int a = 0; a < 3; a++) {
         double lo = block_reduce<1>(has && with_bounds ? c.x[a] : 1.0e20);
         double hi = block_reduce<2>(has && with_bounds ? c.x[a] : -1.0e20);
-        if (threadIdx.x == 0) { out[a] = lo; out[3 + a] = hi; }
+        if (threadIdx.x == 0) { *red_at(d, lb, a) = lo; *red_at(d, lb, 3 + a) = hi; }
     }
 }
 
@@ -502,15 +505,14 @@ __device__ void reduce_bound_partials(const DevState& d, const CPart& c, bool wi
 // calls it.
 __device__ void reduce_energy_partials(const DevState& d, const CPart& c, int lb) {
     const int nm = d.nummat;
-    double* out = d.red + (long)lb * RED_C;
     for (int m = 0; m < nm; m++) {
         const bool mine = c.mi == m;
         double ke = block_reduce<0>(mine ? c.ke : 0.0);
         double ie = block_reduce<0>(mine ? c.ie : 0.0);
-        if (threadIdx.x == 0) { out[6 + m] = ke; out[6 + nm + m] = ie; }
+        if (threadIdx.x == 0) { *red_at(d, lb, 6 + m) = ke; *red_at(d, lb, 6 + nm + m) = ie; }
         for (int a = 0; a < 3; a++) {
             double mo = block_reduce<0>(mine ? c.mom[a] : 0.0);
-            if (threadIdx.x == 0) out[6 + 2 * nm + 3 * m + a] = mo;
+            if (threadIdx.x == 0) *red_at(d, lb, 6 + 2 * nm + 3 * m + a) = mo;
         }
     }
 }
@@ -532,8 +534,8 @@ __device__ __forceinline__ void strain_block_partials(const DevState& d, int lb,
     double cb = block_reduce<1>(crit);
     double hb = block_reduce<2>(hm);
     if (threadIdx.x == 0) {
-        d.red[(long)lb * RED_C] = cb;
-        d.red[(long)lb * RED_C + 1] = hb;
+        *red_at(d, lb, 0) = cb;
+        *red_at(d, lb, 1) = hb;
     }
 }
 
@@ -545,8 +547,8 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_strain_fold(DevState d_) {
     const DevState& d = enter<false>(d_, bk);
     const int nblk = d.nblk;
     Scalars* sc = d.sc;
-    double c1 = fold<1>(d.red, RED_C, 0, nblk, 1.0e20);
-    double h1 = fold<2>(d.red, RED_C, 1, nblk, 0.0);
+    double c1 = fold<1>(d.red, 1, 0, nblk, 1.0e20);
+    double h1 = fold<2>(d.red + nblk, 1, 0, nblk, 0.0);
     if (threadIdx.x == 0 && !sc->abort) {
         sc->critts = c1;
         if ((MODE == MODE_FUSED || MODE == MODE_FCONST) && d.h_opt > 0) sc->hmax = h1;
@@ -943,10 +945,9 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_c_fold(DevState d_, int with_boun
 #pragma unroll
     for (int k = 0; k < nv; k++) acc[k] = k < 3 ? 1.0e20 : (k < 6 ? -1.0e20 : 0.0);
     for (int b = threadIdx.x; b < nblk; b += MCM_BLOCK) {
-        const double* p = d.red + (long)b * RED_C;
 #pragma unroll
         for (int k = 0; k < nv; k++) {
-            const double x = p[k];
+            const double x = d.red[(long)k * nblk + b];
             acc[k] = k < 3 ? fmin(acc[k], x) : (k < 6 ? fmax(acc[k], x) : acc[k] + x);
         }
     }
@@ -1171,11 +1172,10 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_xsph_move(DevState d_, int cur) {
             d.x0[a][i] = x[a];
         }
     }
-    double* out = d.red + (long)lb * RED_C;
     for (int a = 0; a < 3; a++) {
         const double lo = block_reduce<1>(live ? xb[a] : 1.0e20);
         const double hi = block_reduce<2>(live ? xb[a] : -1.0e20);
-        if (threadIdx.x == 0) { out[a] = lo; out[3 + a] = hi; }
+        if (threadIdx.x == 0) { *red_at(d, lb, a) = lo; *red_at(d, lb, 3 + a) = hi; }
     }
 }
 
