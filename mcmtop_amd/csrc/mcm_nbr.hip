@@ -430,21 +430,24 @@ __global__ __launch_bounds__(256) void k_bin(DevState d_, int cur) {
         d.cell_of[i] = cell;
     }
     if (d.skin > 0.0 && live && d.sc->skin_valid) skin_write_xh(d, i, cell >= 0, cur);
-    // histogram with one atomic per distinct cell of the wavefront: lanes of
-    // the same cell take consecutive slots (the cell sort orders them later)
+    // histogram with one atomic per run of lanes in the same cell, all runs'
+    // atomics issued together: the lanes of a run take consecutive slots (the
+    // cell sort orders every cell's members by id later, so any distinct slots
+    // do).  Particles in id order sit in runs along the lattice rows.
     const int lane = threadIdx.x & 63;
-    const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    unsigned long long todo = __ballot(cell >= 0);
-    while (todo) {
-        const int leader = __builtin_ctzll(todo);
-        const int lc = __builtin_amdgcn_readlane(cell, leader);
-        const unsigned long long same = __ballot(cell == lc);
-        int base = 0;
-        if (lane == leader) base = atomicAdd(&d.cell_count[lc], __popcll(same));
-        base = __builtin_amdgcn_readlane(base, leader);
-        if (cell == lc) d.slot_of[i] = base + __popcll(same & below);
-        todo &= ~same;
-    }
+    const int prev = __shfl_up(cell, 1, 64);
+    const bool head = cell >= 0 && (lane == 0 || prev != cell);
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long bounds = __ballot(head || cell < 0);
+    const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);   // lanes 0 .. lane
+    const unsigned long long after = bounds & ~upto;
+    const int next = after ? __builtin_ctzll(after) : 64;   // first lane past this lane's run
+    int base = 0;
+    if (head) base = atomicAdd(&d.cell_count[cell], next - lane);
+    const unsigned long long mine = heads & upto;
+    const int hl = mine ? 63 - __builtin_clzll(mine) : lane;   // the run's head
+    base = __shfl(base, hl, 64);
+    if (cell >= 0) d.slot_of[i] = base + (lane - hl);
     // workgroup maximum, one atomic per workgroup (positive doubles order as
     // their bit patterns)
     __shared__ double wmax[4];
