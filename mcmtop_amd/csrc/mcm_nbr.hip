@@ -14,6 +14,14 @@
 #include "mcm_device.h"
 
 // ---------------------------------------------------------------- helpers
+// min of two doubles that are not NaN, in one instruction (fmin also
+// quiets signalling NaNs, two more instructions)
+__device__ __forceinline__ double min_f64(double a, double b) {
+    double r;
+    asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
 __device__ __forceinline__ int wave_incl_scan(int x) {
     const int lane = threadIdx.x & 63;
 #pragma unroll
@@ -907,18 +915,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 3 :
     int2* idl = s_idm[MULTI && !SKIN ? wv : 0];
     int* idsl = s_id[!MULTI && !SKIN ? wv : 0];
     const int spos_i = t;
-    int* op = out;   // entry cnt of the list
+    // the walk's entries go to element oi of the list array (entry cnt of
+    // this particle's row: rowbase + 64 cnt), clamped to the row's last slot:
+    // a list that outgrows the capacity sets the error flag below and the
+    // step is run again with rows long enough
+    int* const lbase = SKIN ? d.skin_list : d.nbr;
+    const unsigned rowbase = (unsigned)(t >> 6) * (unsigned)ncap * 64u + (unsigned)(t & 63);
+    const unsigned lastoff = rowbase + (unsigned)(ncap - 1) * 64u;
+    unsigned oi = rowbase;
     for (int pass = 0; pass < (MULTI && !SKIN ? 2 : 1); pass++) {
+        auto append = [&](int v) {
+            lbase[min(oi, lastoff)] = v;
+            oi += 64u;
+        };
         // candidate at sorted slot s with id and material idm (with one
         // material the self test uses the sorted slot)
-        // entry cnt of the list, at op
-        auto append = [&](int v) {
-            if (cnt < ncap) {
-                *op = v;
-                op += 64;
-            }
-            cnt++;
-        };
         auto consider = [&](int s, double2 pa, double2 pb, int2 idm) {
             if (MULTI && !SKIN ? idm.x == i : s == spos_i) return;
             double dx = pa.x - xi;
@@ -934,10 +945,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 3 :
             if (dist_sq < h_avg * h_avg) {
                 const bool same = !MULTI || idm.y == mi;
                 if (pass == 0 && same) {
-                    mind = fmin(mind, dist_sq);
+                    mind = min_f64(mind, dist_sq);
                     append(idm.x);
                 } else if (MULTI && pass == 1 && !same) {
-                    mind_cp = fmin(mind_cp, dist_sq);
+                    mind_cp = min_f64(mind_cp, dist_sq);
                     if (penalty) {
                         if (ccnt < ncap) cont_list(d, i)[(long)ccnt * 64] = idm.x;
                         ccnt++;
@@ -955,7 +966,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 3 :
             const double2* r = rec + 2 * (st - base);
             const int2* pm = idl + (st - base);
             const int* pi = idsl + (st - base);
-            for (int s = st; s < en; s += 4, r += 8, pm += 4, pi += 4) {
+            auto group = [&](int s, bool tail) {
                 int2 idm[4];
                 double2 pa[4], pb[4];
 #pragma unroll
@@ -967,8 +978,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 3 :
                 }
 #pragma unroll
                 for (int u = 0; u < 4; u++)
-                    if (s + u < en) consider(s + u, pa[u], pb[u], idm[u]);
-            }
+                    if (!tail || s + u < en) consider(s + u, pa[u], pb[u], idm[u]);
+            };
+            int s = st;
+            for (; s + 4 <= en; s += 4, r += 8, pm += 4, pi += 4) group(s, false);
+            if (s < en) group(s, true);
         };
 #pragma unroll 1
         for (int dz = -1; dz <= 1; dz++) {
@@ -1029,6 +1043,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 3 :
             }
         }
     }
+    cnt = (int)((oi - rowbase) >> 6);
     if (live) {
         int nreal;
         if (binned) {
