@@ -122,9 +122,18 @@ def cpu_baseline(tmpdir: str, dx: float):
         t0 = time.perf_counter()
         o.run(steps)
         secs = time.perf_counter() - t0
+    cpu = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return {"value": d.np_ * steps / secs, "unit": "particle-steps/s", "cores": 1, "kind": kind,
             "sample": "same bar section and dx (0.073 mm), 1.0 mm long (%d particles), %d steps, %s"
-                      % (d.np_, steps, "flang -O2 build of the reference" if kind == "reference" else "C oracle")}
+                      % (d.np_, steps, "flang -O2 build of the reference" if kind == "reference" else "C oracle"),
+            "cpu": cpu, "host_threads": os.cpu_count()}
 
 
 def kernel_ms(ms):

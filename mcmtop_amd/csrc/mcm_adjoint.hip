@@ -131,8 +131,16 @@ __device__ __forceinline__ void load_g9(const double* lbp, int i, double G[9]) {
 }
 
 // save (1) or restore (0) the real-particle state; grid.y = field
-__global__ __launch_bounds__(MCM_BLOCK) void k_snap(DevState d, double* base, int save) {
+// the state's fields (grid.y = field) and, in workgroup (0, 0), the
+// scalars (ssc); nothing else here reads the scalars
+__global__ __launch_bounds__(MCM_BLOCK) void k_snap(DevState d, double* base, Scalars* ssc, int save) {
     const int f = blockIdx.y;
+    if (f == 0 && blockIdx.x == 0) {
+        const int nw = (int)(sizeof(Scalars) / sizeof(int));
+        int* dst = save ? reinterpret_cast<int*>(ssc) : reinterpret_cast<int*>(d.sc);
+        const int* src = save ? reinterpret_cast<const int*>(d.sc) : reinterpret_cast<const int*>(ssc);
+        for (int w = threadIdx.x; w < nw; w += MCM_BLOCK) dst[w] = src[w];
+    }
     double* p = snap_field(d, f);
     double* b = base + (long)f * d.np;
     for (int i = blockIdx.x * MCM_BLOCK + threadIdx.x; i < d.np; i += gridDim.x * MCM_BLOCK) {
@@ -817,6 +825,14 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_adj_gmap_strain(DevState d, AdjDe
     if (threadIdx.x == 0) a.red[(long)blockIdx.x * RA_N + RA_SDT] = sdt;
 }
 
+// the transposed lists' counts to zero before a reverse step fills them
+__global__ __launch_bounds__(MCM_BLOCK) void k_adj_zero_counts(AdjDev a, long ng, long np) {
+    for (long k = blockIdx.x * (long)MCM_BLOCK + threadIdx.x; k < std::max(ng, np); k += (long)gridDim.x * MCM_BLOCK) {
+        if (k < ng) a.gt_cnt[k] = 0;
+        if (k < np) a.gof_cnt[k] = 0;
+    }
+}
+
 // a recomputed step that stopped on a capacity (cannot happen: the forward ran
 // the same states at the same or smaller capacities) is reported at the end
 __global__ void k_adj_check(DevState d, AdjDev a) {
@@ -880,24 +896,33 @@ __device__ __forceinline__ void inter_field(const DevState& d, const InterLayout
     }
 }
 
-// save (1) or restore (0) the intermediates; grid.y = field.  On restore the
-// step's scalars (ngp) are restored first.
+// save (1) or restore (0) the intermediates (grid.y = field) and the step's
+// scalars after the step (ssc, copied by workgroup (0, 0)); the counts
+// (ghosts, rows) come from the live scalars when saving and from the slot's
+// when restoring
 __global__ __launch_bounds__(MCM_BLOCK) void k_inter(DevState d, InterLayout L, double* dbase, int* ibase,
-                                                     int save) {
+                                                     Scalars* ssc, int save) {
+    const Scalars& S = save ? *d.sc : *ssc;
+    if (blockIdx.x == 0 && blockIdx.y == 0) {
+        const int nw = (int)(sizeof(Scalars) / sizeof(int));
+        int* dst = save ? reinterpret_cast<int*>(ssc) : reinterpret_cast<int*>(d.sc);
+        const int* src = save ? reinterpret_cast<const int*>(d.sc) : reinterpret_cast<const int*>(ssc);
+        for (int w = threadIdx.x; w < nw; w += MCM_BLOCK) dst[w] = src[w];
+    }
     double* dp;
     int* ip;
     long len, off;
-    inter_field(d, L, blockIdx.y, d.sc->ngp, dp, ip, len, off);
+    inter_field(d, L, blockIdx.y, S.ngp, dp, ip, len, off);
     if (blockIdx.y == INTER_ND) {
-        // the blocked lists [npad/64][ncap][64]: rows k < the step's largest count
-        const long rows = min(d.sc->max_count, d.ncap), nblk64 = L.np / 64 + 1;
-        const long span = rows * 64;
-        const long n = nblk64 * span;
-        for (long i = blockIdx.x * (long)MCM_BLOCK + threadIdx.x; i < n; i += (long)gridDim.x * MCM_BLOCK) {
-            const long b = i / span, q = b * d.ncap * 64 + (i - b * span);
-            if (q >= L.nbrlen) continue;
-            if (save) ibase[off + q] = ip[q];
-            else ip[q] = ibase[off + q];
+        // the blocked lists [npad/64][ncap][64]: rows k < the step's largest
+        // count, a contiguous run of each 64-particle block
+        const long rows = min(S.max_count, d.ncap), nblk64 = L.nbrlen / ((long)d.ncap * 64);
+        for (long b = blockIdx.x; b < nblk64; b += gridDim.x) {
+            const long q0 = b * d.ncap * 64;
+            for (long k = threadIdx.x; k < rows * 64; k += MCM_BLOCK) {
+                if (save) ibase[off + q0 + k] = ip[q0 + k];
+                else ip[q0 + k] = ibase[off + q0 + k];
+            }
         }
         return;
     }
@@ -937,18 +962,31 @@ __device__ __forceinline__ double* snap_gfield(const DevState& d, int f) {
     return d.rbuf[f - 12];
 }
 
-// save (1) or restore (0) one step's search outputs; on restore the
-// step's scalars (ngp, max_count) are in place first
-__global__ __launch_bounds__(MCM_BLOCK) void k_adj_search(DevState d, SearchSlot p, int save) {
-    const long np = d.np, ngp = d.sc->ngp, gcap = d.ngcap;
-    const long tid = blockIdx.x * (long)blockDim.x + threadIdx.x, nt = (long)gridDim.x * blockDim.x;
-    const long rows = min(d.sc->max_count, d.ncap), nblk64 = (d.npad + 63) / 64;
-    const long span = rows * 64, n = nblk64 * span;
-    for (long k = tid; k < n; k += nt) {
-        const long b = k / span, q = b * d.ncap * 64 + (k - b * span);
-        if (save) p.rows[q] = d.nbr[q];
-        else d.nbr[q] = p.rows[q];
+// save (1) or restore (0) one step's search outputs and the scalars after
+// the search (ssc).  Workgroups [0, nrowb) copy one 64-particle block of list
+// rows each (contiguous, 16 bytes per lane); the others take the per-particle
+// and per-ghost arrays; workgroup 0 also copies the scalars.  The counts
+// (ghosts, rows) come from the live scalars when saving and from the slot's
+// when restoring, which workgroup 0 writes back meanwhile.
+__global__ __launch_bounds__(MCM_BLOCK) void k_adj_search(DevState d, SearchSlot p, Scalars* ssc, int nrowb, int save) {
+    const Scalars& S = save ? *d.sc : *ssc;
+    const long np = d.np, ngp = S.ngp, gcap = d.ngcap;
+    const long rows = min(S.max_count, d.ncap);
+    if (blockIdx.x == 0) {
+        const int nw = (int)(sizeof(Scalars) / sizeof(int));
+        int* dst = save ? reinterpret_cast<int*>(ssc) : reinterpret_cast<int*>(d.sc);
+        const int* src = save ? reinterpret_cast<const int*>(d.sc) : reinterpret_cast<const int*>(ssc);
+        for (int w = threadIdx.x; w < nw; w += MCM_BLOCK) dst[w] = src[w];
     }
+    if ((int)blockIdx.x < nrowb) {
+        const long off = (long)blockIdx.x * d.ncap * 64;
+        int4* dst = reinterpret_cast<int4*>(save ? p.rows + off : d.nbr + off);
+        const int4* src = reinterpret_cast<const int4*>(save ? d.nbr + off : p.rows + off);
+        for (long k = threadIdx.x; k < rows * 16; k += MCM_BLOCK) dst[k] = src[k];
+        return;
+    }
+    const long tid = (blockIdx.x - nrowb) * (long)blockDim.x + threadIdx.x;
+    const long nt = (long)(gridDim.x - nrowb) * blockDim.x;
     for (long i = tid; i < np; i += nt) {
         if (save) {
             p.nreal[i] = d.nreal[i]; p.ntot[i] = d.ntot[i]; p.sorted[i] = d.sorted[i]; p.spos[i] = d.spos[i];
@@ -958,16 +996,16 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_adj_search(DevState d, SearchSlot
             d.mindist[i] = p.mindist[i];
         }
     }
-    // strain records of the reals and the step's ghosts
-    const long nrs = RS_SLOTS * 2 * (np + ngp);
+    // strain records of the reals and the step's ghosts, 16 bytes at a time
+    const long nrs = RS_SLOTS * (np + ngp);
     for (long k = tid; k < nrs; k += nt) {
-        if (save) p.rs[k] = d.rs[k];
-        else d.rs[k] = p.rs[k];
+        if (save) reinterpret_cast<double2*>(p.rs)[k] = reinterpret_cast<const double2*>(d.rs)[k];
+        else reinterpret_cast<double2*>(d.rs)[k] = reinterpret_cast<const double2*>(p.rs)[k];
     }
-    const long nrm = RM_SLOTS * 2 * ngp, rm0 = RM_SLOTS * 2 * np;
+    const long nrm = RM_SLOTS * ngp, rm0 = RM_SLOTS * np;
     for (long k = tid; k < nrm; k += nt) {
-        if (save) p.rmg[k] = d.rm[rm0 + k];
-        else d.rm[rm0 + k] = p.rmg[k];
+        if (save) reinterpret_cast<double2*>(p.rmg)[k] = reinterpret_cast<const double2*>(d.rm)[rm0 + k];
+        else reinterpret_cast<double2*>(d.rm)[rm0 + k] = reinterpret_cast<const double2*>(p.rmg)[k];
     }
     for (long g = tid; g < ngp; g += nt) {
         const long u = np + g;
@@ -1056,9 +1094,7 @@ double* slot_ptr(mcm_gpu_adjoint* A, bool ck, int s) {
 int adj_save(mcm_gpu_adjoint* A, bool ck, int s) {
     mcm_gpu_ctx* ctx = A->ctx;
     dim3 grid((unsigned)std::min<long>(blocks_for(A->np, MCM_BLOCK), 1024), SN_N);
-    k_snap<<<grid, MCM_BLOCK, 0, ctx->stream>>>(ctx->d, slot_ptr(A, ck, s), 1);
-    HIPCHK_A(hipMemcpyAsync((ck ? A->ckpt_sc : A->seg_sc) + s, ctx->sc, sizeof(Scalars), hipMemcpyDeviceToDevice,
-                            ctx->stream));
+    k_snap<<<grid, MCM_BLOCK, 0, ctx->stream>>>(ctx->d, slot_ptr(A, ck, s), (ck ? A->ckpt_sc : A->seg_sc) + s, 1);
     (ck ? A->ckpt_cur : A->seg_cur)[s] = ctx->cur;
     HIPCHK_A(hipGetLastError());
     return 0;
@@ -1067,9 +1103,7 @@ int adj_save(mcm_gpu_adjoint* A, bool ck, int s) {
 int adj_restore(mcm_gpu_adjoint* A, bool ck, int s) {
     mcm_gpu_ctx* ctx = A->ctx;
     dim3 grid((unsigned)std::min<long>(blocks_for(A->np, MCM_BLOCK), 1024), SN_N);
-    k_snap<<<grid, MCM_BLOCK, 0, ctx->stream>>>(ctx->d, slot_ptr(A, ck, s), 0);
-    HIPCHK_A(hipMemcpyAsync(ctx->sc, (ck ? A->ckpt_sc : A->seg_sc) + s, sizeof(Scalars), hipMemcpyDeviceToDevice,
-                            ctx->stream));
+    k_snap<<<grid, MCM_BLOCK, 0, ctx->stream>>>(ctx->d, slot_ptr(A, ck, s), (ck ? A->ckpt_sc : A->seg_sc) + s, 0);
     ctx->cur = (ck ? A->ckpt_cur : A->seg_cur)[s];
     ctx->phase = 0;
     ctx->done = 0;
@@ -1082,9 +1116,11 @@ int adj_restore(mcm_gpu_adjoint* A, bool ck, int s) {
 // memory and in 32 GB; otherwise the reverse sweep searches again
 void adj_snap_alloc(mcm_gpu_adjoint* A) {
     const DevState& d = A->ctx->d;
-    const size_t ni = (size_t)d.npad * d.ncap + 4 * (size_t)d.np + 3 * (size_t)d.ngcap + 4;
-    const size_t nd = (size_t)d.np + (size_t)RS_SLOTS * 2 * ((size_t)d.np + d.ngcap) +
-                      (size_t)RM_SLOTS * 2 * d.ngcap + SNAP_GD * (size_t)d.ngcap;
+    // slots rounded to 256 bytes: every part that is copied 16 bytes at a
+    // time starts 16-byte aligned
+    const size_t ni = ((size_t)d.npad * d.ncap + 4 * (size_t)d.np + 3 * (size_t)d.ngcap + 4 + 63) / 64 * 64;
+    const size_t nd = ((size_t)d.np + (size_t)RS_SLOTS * 2 * ((size_t)d.np + d.ngcap) +
+                       (size_t)RM_SLOTS * 2 * d.ngcap + SNAP_GD * (size_t)d.ngcap + 31) / 32 * 32;
     const size_t bytes = (size_t)A->nsteps * (ni * sizeof(int) + nd * sizeof(double) + sizeof(Scalars));
     size_t freeb = 0, totalb = 0;
     if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) return;
@@ -1129,18 +1165,13 @@ int adj_search(mcm_gpu_adjoint* A, int n, int save) {
     p.gpar = pi; pi += gc;
     p.gsgn = pi; pi += gc;
     p.range = pi;
-    p.mindist = pd; pd += np;
-    p.rs = pd; pd += (size_t)RS_SLOTS * 2 * (np + gc);
+    p.rs = pd; pd += (size_t)RS_SLOTS * 2 * (np + gc);   // first: 16-byte aligned, as is rmg after it
     p.rmg = pd; pd += (size_t)RM_SLOTS * 2 * gc;
+    p.mindist = pd; pd += np;
     p.gd = pd;
-    if (save) {
-        HIPCHK_A(hipMemcpyAsync(A->snap_sc + n, ctx->sc, sizeof(Scalars), hipMemcpyDeviceToDevice, ctx->stream));
-    } else {
-        HIPCHK_A(hipMemcpyAsync(ctx->sc, A->snap_sc + n, sizeof(Scalars), hipMemcpyDeviceToDevice, ctx->stream));
-    }
-    const long work = std::max((long)d.npad * d.ncap, (long)RS_SLOTS * 2 * (np + gc));
-    k_adj_search<<<(unsigned)std::min<long>(blocks_for(work / 8 + 1, MCM_BLOCK), 2048), MCM_BLOCK, 0, ctx->stream>>>(
-        d, p, save);
+    const int nrowb = (d.npad + 63) / 64;
+    const int nrest = std::max(1, std::min(blocks_for(RS_SLOTS * (np + gc), MCM_BLOCK), 512));
+    k_adj_search<<<nrowb + nrest, MCM_BLOCK, 0, ctx->stream>>>(d, p, A->snap_sc + n, nrowb, save);
     HIPCHK_A(hipGetLastError());
     return 0;
 }
@@ -1227,13 +1258,8 @@ int adj_inter(mcm_gpu_adjoint* A, int s, bool save) {
     mcm_gpu_ctx* ctx = A->ctx;
     const long big = std::max(A->lay.nbrlen, INTER_UCAP * A->lay.ucap);
     dim3 grid((unsigned)std::min<long>(blocks_for(big / 8 + 1, MCM_BLOCK), 1024), INTER_ND + INTER_NI);
-    if (save) {
-        HIPCHK_A(hipMemcpyAsync(A->inter_sc + s, ctx->sc, sizeof(Scalars), hipMemcpyDeviceToDevice, ctx->stream));
-    } else {
-        HIPCHK_A(hipMemcpyAsync(ctx->sc, A->inter_sc + s, sizeof(Scalars), hipMemcpyDeviceToDevice, ctx->stream));
-    }
     k_inter<<<grid, MCM_BLOCK, 0, ctx->stream>>>(ctx->d, A->lay, A->inter_d + (size_t)s * A->inter_dn,
-                                                 A->inter_i + (size_t)s * A->inter_in, save ? 1 : 0);
+                                                 A->inter_i + (size_t)s * A->inter_in, A->inter_sc + s, save ? 1 : 0);
     HIPCHK_A(hipGetLastError());
     return 0;
 }
@@ -1270,8 +1296,8 @@ int adj_fused_diag_step(mcm_gpu_adjoint* A, int n) {
         if (rc) return rc;
     } else {
         launch_neighbours(one(ctx), cur);
+        k_adj_check<<<1, 1, 0, st>>>(ctx->d, A->a);
     }
-    k_adj_check<<<1, 1, 0, st>>>(ctx->d, A->a);
     k_strain_const<MODE_FSTRAIN, 0, false><<<ctx->nb, MCM_BLOCK, 0, st>>>(ctx->d, cur);
     k_strain_const<MODE_FCONST, 0, true><<<ctx->nb, MCM_BLOCK, 0, st>>>(ctx->d, cur);
     k_strain_fold<MODE_FCONST><<<1, MCM_BLOCK, 0, st>>>(ctx->d);
@@ -1329,8 +1355,8 @@ int adj_reverse_step(mcm_gpu_adjoint* A, bool ck, int s) {
     const int wb = blocks_for(A->np, MCM_BLOCK / kLpp);                                  // kLpp lanes per particle
     const int wg = std::max(1, std::min(blocks_for(d.ngcap, MCM_BLOCK / 64), 4096));   // wave per ghost
     // transposed lists; an overflow sets err bit 1 (checked after the sweep)
-    HIPCHK_A(hipMemsetAsync(a.gt_cnt, 0, sizeof(int) * a.ngcap, st));
-    HIPCHK_A(hipMemsetAsync(a.gof_cnt, 0, sizeof(int) * A->np, st));
+    k_adj_zero_counts<<<std::max(1, std::min(blocks_for(std::max((long)a.ngcap, A->np), MCM_BLOCK), 1024)), MCM_BLOCK, 0,
+                        st>>>(a, a.ngcap, A->np);
     k_adj_gt_fill<<<nb, MCM_BLOCK, 0, st>>>(d, a);
     k_adj_gof_fill<<<gb, MCM_BLOCK, 0, st>>>(d, a);
     k_adj_gt_sort<<<wg, MCM_BLOCK, 0, st>>>(d, a);
