@@ -901,7 +901,7 @@ __device__ __forceinline__ void inter_field(const DevState& d, const InterLayout
 // (ghosts, rows) come from the live scalars when saving and from the slot's
 // when restoring
 __global__ __launch_bounds__(MCM_BLOCK) void k_inter(DevState d, InterLayout L, double* dbase, int* ibase,
-                                                     Scalars* ssc, int save) {
+                                                     Scalars* ssc, int save, int skip_rows) {
     const Scalars& S = save ? *d.sc : *ssc;
     if (blockIdx.x == 0 && blockIdx.y == 0) {
         const int nw = (int)(sizeof(Scalars) / sizeof(int));
@@ -914,6 +914,7 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_inter(DevState d, InterLayout L, 
     long len, off;
     inter_field(d, L, blockIdx.y, S.ngp, dp, ip, len, off);
     if (blockIdx.y == INTER_ND) {
+        if (skip_rows) return;   // the rows stay in the search slot of the step
         // the blocked lists [npad/64][ncap][64]: rows k < the step's largest
         // count, a contiguous run of each 64-particle block
         const long rows = min(S.max_count, d.ncap), nblk64 = L.nbrlen / ((long)d.ncap * 64);
@@ -946,6 +947,9 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_inter(DevState d, InterLayout L, 
 // after the search are
 // kept beside it.  Slots are sized for the run's capacities (npad, ncap,
 // ngcap) when the recording starts and used only while they are unchanged.
+// The list rows are not copied: while a recorded step runs, and while it is
+// recomputed and reversed, the run's list array is the slot's rows, which
+// the list build writes and the pair and reverse kernels read in place.
 struct SearchSlot {
     int *rows, *nreal, *ntot, *sorted, *spos, *gpmat, *gpar, *gsgn, *range;
     double *mindist, *rs, *rmg, *gd;   // rmg: the ghosts' momentum records; gd: [14][ngcap] x0 3, x1 3, v 3,
@@ -962,31 +966,21 @@ __device__ __forceinline__ double* snap_gfield(const DevState& d, int f) {
     return d.rbuf[f - 12];
 }
 
-// save (1) or restore (0) one step's search outputs and the scalars after
-// the search (ssc).  Workgroups [0, nrowb) copy one 64-particle block of list
-// rows each (contiguous, 16 bytes per lane); the others take the per-particle
-// and per-ghost arrays; workgroup 0 also copies the scalars.  The counts
-// (ghosts, rows) come from the live scalars when saving and from the slot's
-// when restoring, which workgroup 0 writes back meanwhile.
-__global__ __launch_bounds__(MCM_BLOCK) void k_adj_search(DevState d, SearchSlot p, Scalars* ssc, int nrowb, int save) {
+// save (1) or restore (0) one step's search outputs but the list rows (in
+// place, above) and the scalars after the search (ssc, copied by workgroup
+// 0).  The ghost count comes from the live scalars when saving and from the
+// slot's when restoring, which workgroup 0 writes back meanwhile.
+__global__ __launch_bounds__(MCM_BLOCK) void k_adj_search(DevState d, SearchSlot p, Scalars* ssc, int save) {
     const Scalars& S = save ? *d.sc : *ssc;
     const long np = d.np, ngp = S.ngp, gcap = d.ngcap;
-    const long rows = min(S.max_count, d.ncap);
     if (blockIdx.x == 0) {
         const int nw = (int)(sizeof(Scalars) / sizeof(int));
         int* dst = save ? reinterpret_cast<int*>(ssc) : reinterpret_cast<int*>(d.sc);
         const int* src = save ? reinterpret_cast<const int*>(d.sc) : reinterpret_cast<const int*>(ssc);
         for (int w = threadIdx.x; w < nw; w += MCM_BLOCK) dst[w] = src[w];
     }
-    if ((int)blockIdx.x < nrowb) {
-        const long off = (long)blockIdx.x * d.ncap * 64;
-        int4* dst = reinterpret_cast<int4*>(save ? p.rows + off : d.nbr + off);
-        const int4* src = reinterpret_cast<const int4*>(save ? d.nbr + off : p.rows + off);
-        for (long k = threadIdx.x; k < rows * 16; k += MCM_BLOCK) dst[k] = src[k];
-        return;
-    }
-    const long tid = (blockIdx.x - nrowb) * (long)blockDim.x + threadIdx.x;
-    const long nt = (long)(gridDim.x - nrowb) * blockDim.x;
+    const long tid = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const long nt = (long)gridDim.x * blockDim.x;
     for (long i = tid; i < np; i += nt) {
         if (save) {
             p.nreal[i] = d.nreal[i]; p.ntot[i] = d.ntot[i]; p.sorted[i] = d.sorted[i]; p.spos[i] = d.spos[i];
@@ -1057,6 +1051,7 @@ struct mcm_gpu_adjoint {
     size_t snap_ni = 0, snap_nd = 0;
     int snap_npad = 0, snap_ncap = 0, snap_ngcap = 0;   // capacities the slots are sized for
     bool snap_ok = false;             // every step of the last recording saved at those capacities
+    int* nbr_own = nullptr;           // the run's own list array while a slot's rows stand in for it
     std::vector<void*> lallocs;
     AdjDev a{};
     std::vector<void*> allocs, gallocs;   // run-long and ghost-sized allocations
@@ -1148,6 +1143,25 @@ bool adj_snap_fits(const mcm_gpu_adjoint* A) {
     return A->snap_i && d.npad == A->snap_npad && d.ncap == A->snap_ncap && d.ngcap == A->snap_ngcap;
 }
 
+// the run's list array becomes step n's slot rows (same layout and size)
+void adj_rows_in_slot(mcm_gpu_adjoint* A, int n) {
+    if (!A->nbr_own) A->nbr_own = A->ctx->d.nbr;
+    A->ctx->d.nbr = A->snap_i + (size_t)n * A->snap_ni;
+}
+
+// the run's own list array back (before anything that may free or grow it)
+void adj_rows_home(mcm_gpu_adjoint* A) {
+    if (A->nbr_own) {
+        A->ctx->d.nbr = A->nbr_own;
+        A->nbr_own = nullptr;
+    }
+}
+
+struct RowsHome {
+    mcm_gpu_adjoint* A;
+    ~RowsHome() { adj_rows_home(A); }
+};
+
 // save (1) or restore (0) step n's search outputs
 int adj_search(mcm_gpu_adjoint* A, int n, int save) {
     mcm_gpu_ctx* ctx = A->ctx;
@@ -1169,9 +1183,8 @@ int adj_search(mcm_gpu_adjoint* A, int n, int save) {
     p.rmg = pd; pd += (size_t)RM_SLOTS * 2 * gc;
     p.mindist = pd; pd += np;
     p.gd = pd;
-    const int nrowb = (d.npad + 63) / 64;
-    const int nrest = std::max(1, std::min(blocks_for(RS_SLOTS * (np + gc), MCM_BLOCK), 512));
-    k_adj_search<<<nrowb + nrest, MCM_BLOCK, 0, ctx->stream>>>(d, p, A->snap_sc + n, nrowb, save);
+    const int nblk = std::max(1, std::min(blocks_for(RS_SLOTS * (np + gc), MCM_BLOCK), 512));
+    k_adj_search<<<nblk, MCM_BLOCK, 0, ctx->stream>>>(d, p, A->snap_sc + n, save);
     HIPCHK_A(hipGetLastError());
     return 0;
 }
@@ -1183,6 +1196,7 @@ int adj_search(mcm_gpu_adjoint* A, int n, int save) {
 // (whose list-build outputs are kept when the recording keeps them).
 int adj_steps(mcm_gpu_adjoint* A, int nsteps, double* hist, int s, int first) {
     mcm_gpu_ctx* ctx = A->ctx;
+    RowsHome home{A};
     int rc = 0;
     for (int attempt = 0; attempt < 16; attempt++) {
         Scalars s0;
@@ -1191,6 +1205,7 @@ int adj_steps(mcm_gpu_adjoint* A, int nsteps, double* hist, int s, int first) {
         if (!keep) A->snap_ok = false;
         for (int n = 0; n < nsteps; n++) {
             if (keep) {
+                adj_rows_in_slot(A, first + n);
                 const Target T = one(ctx);
                 launch_neighbours(T, ctx->cur);
                 if ((rc = adj_search(A, first + n, 1))) return rc;
@@ -1207,6 +1222,7 @@ int adj_steps(mcm_gpu_adjoint* A, int nsteps, double* hist, int s, int first) {
         HIPCHK_A(hipGetLastError());
         Scalars s1;
         if ((rc = get_scalars_raw(ctx, &s1))) { A->err = ctx->err; return rc; }
+        adj_rows_home(A);
         if (!s1.abort && s1.timestep - s0.timestep == nsteps) return MCM_GPU_OK;
         if ((rc = regrow(ctx, s1))) { A->err = ctx->err; return rc; }
         if ((rc = adj_restore(A, true, s))) return rc;
@@ -1258,8 +1274,10 @@ int adj_inter(mcm_gpu_adjoint* A, int s, bool save) {
     mcm_gpu_ctx* ctx = A->ctx;
     const long big = std::max(A->lay.nbrlen, INTER_UCAP * A->lay.ucap);
     dim3 grid((unsigned)std::min<long>(blocks_for(big / 8 + 1, MCM_BLOCK), 1024), INTER_ND + INTER_NI);
+    const int skip_rows = A->snap_ok && adj_snap_fits(A) && A->nbr_own != nullptr;
     k_inter<<<grid, MCM_BLOCK, 0, ctx->stream>>>(ctx->d, A->lay, A->inter_d + (size_t)s * A->inter_dn,
-                                                 A->inter_i + (size_t)s * A->inter_in, A->inter_sc + s, save ? 1 : 0);
+                                                 A->inter_i + (size_t)s * A->inter_in, A->inter_sc + s, save ? 1 : 0,
+                                                 skip_rows);
     HIPCHK_A(hipGetLastError());
     return 0;
 }
@@ -1292,6 +1310,7 @@ int adj_fused_diag_step(mcm_gpu_adjoint* A, int n) {
     hipStream_t st = ctx->stream;
     const int cur = ctx->cur;
     if (A->snap_ok && adj_snap_fits(A)) {   // the recording's search of step n
+        adj_rows_in_slot(A, n);
         int rc = adj_search(A, n, 0);
         if (rc) return rc;
     } else {
@@ -1457,6 +1476,12 @@ int adj_record(mcm_gpu_adjoint* A, double* hist) {
             if ((rc = adj_steps(A, s1 - s0, nsample > 0 ? A->d_hist + (size_t)s0 * 3 * nsample : nullptr, sgi, s0)))
                 return rc;
         }
+        // the last step's lists (built in its slot) into the run's own array,
+        // which describes the run's state after the recording
+        if (A->snap_ok && adj_snap_fits(A))
+            HIPCHK_A(hipMemcpyAsync(ctx->d.nbr, A->snap_i + (size_t)(nsteps - 1) * A->snap_ni,
+                                    sizeof(int) * (size_t)ctx->d.npad * ctx->d.ncap, hipMemcpyDeviceToDevice,
+                                    ctx->stream));
     }
     if (nsample > 0)
         HIPCHK_A(hipMemcpyAsync(hist, A->d_hist, sizeof(double) * 3 * nsample * (size_t)nsteps,
@@ -1626,6 +1651,7 @@ static int adj_sweep(mcm_gpu_adjoint* A, const double* xbar, const double* vbar,
     hipStream_t st = ctx->stream;
     AdjDev& a = A->a;
     const long np = A->np;
+    RowsHome home{A};
     int rc = 0;
     // adjoint of the end state (the run's stream is non-blocking: plain
     // copies are ordered with it by synchronising it)
@@ -1676,6 +1702,7 @@ static int adj_sweep(mcm_gpu_adjoint* A, const double* xbar, const double* vbar,
             if ((rc = adj_inter(A, n - s0, true))) return rc;
         }
         for (int n = s1 - 1; n >= s0; n--) {
+            if (A->snap_ok && adj_snap_fits(A)) adj_rows_in_slot(A, n);   // step n's lists, in its slot
             if ((rc = adj_inter(A, n - s0, false))) return rc;
             if (seeds)
                 k_adj_seed<<<blocks_for(A->nsample, 64), 64, 0, st>>>(a, A->d_sample, A->nsample,
