@@ -203,23 +203,64 @@ __device__ __forceinline__ void hess_apply(const KGH& k, const double xij[3], co
 }
 
 // ------------------------------------------------------- ghost transposes
-// real partners of each ghost, from the real particles' lists
-__global__ __launch_bounds__(MCM_BLOCK) void k_adj_gt_fill(DevState d, AdjDev a) {
-    const int i = blockIdx.x * MCM_BLOCK + threadIdx.x;
-    if (i >= d.np) return;
-    const int* lst = nbr_row(d, d.spos[i]);
-    for (int k = d.nreal[i]; k < d.ntot[i]; k++) {
-        const int g = lst[(long)k * 64] - d.np;
-        const int slot = atomicAdd(&a.gt_cnt[g], 1);
-        if (slot < a.gtcap) a.gt[(long)g * a.gtcap + slot] = i;
-        else atomicOr(a.err, 1);
+// Real partners of each ghost, from the real particles' lists, and the ghosts
+// of each real particle, in one launch: blocks [0, nb) take the lists, the
+// rest the ghosts.  A wavefront of the list part takes 64 particles and
+// spreads their ghost entries (list slots nreal..ntot-1) over its lanes,
+// ADJ_GT_U entries a lane per round, so that the atomics of a round are in
+// flight together; the order of the entries within a ghost's list is then
+// fixed by k_adj_sort.
+constexpr int ADJ_GT_U = 8;
+
+__device__ __forceinline__ void gt_fill_wave(const DevState& d, const AdjDev& a, int i) {
+    const int lane = threadIdx.x & 63;
+    int sp = 0, k0 = 0, n = 0;   // list position, first ghost slot, ghost entries
+    if (i < d.np) {
+        sp = d.spos[i];
+        k0 = d.nreal[i];
+        n = d.ntot[i] - k0;
+    }
+    int inc = n;   // inclusive prefix of the entry counts over the wavefront
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += v;
+    }
+    const int exc = inc - n;
+    const int total = __shfl(inc, 63, 64);
+    const int wbase = i - lane;
+    for (int t0 = 0; t0 < total; t0 += ADJ_GT_U * 64) {
+        int g[ADJ_GT_U], who[ADJ_GT_U], slot[ADJ_GT_U];
+#pragma unroll
+        for (int u = 0; u < ADJ_GT_U; u++) {
+            const int t = t0 + u * 64 + lane;
+            // owner of entry t: the last lane whose exclusive prefix is <= t
+            // (lane 0's is 0; every probe stays below lane 64)
+            int lo = 0;
+            for (int step = 32; step > 0; step >>= 1)
+                if (__shfl(exc, lo + step, 64) <= t) lo += step;
+            const int k = __shfl(k0, lo, 64) + (t - __shfl(exc, lo, 64));
+            const int row = __shfl(sp, lo, 64);
+            who[u] = wbase + lo;
+            g[u] = t < total ? nbr_row(d, row)[(long)k * 64] - d.np : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < ADJ_GT_U; u++) slot[u] = g[u] >= 0 ? atomicAdd(&a.gt_cnt[g[u]], 1) : 0;
+#pragma unroll
+        for (int u = 0; u < ADJ_GT_U; u++) {
+            if (g[u] < 0) continue;
+            if (slot[u] < a.gtcap) a.gt[(long)g[u] * a.gtcap + slot[u]] = who[u];
+            else atomicOr(a.err, 1);
+        }
     }
 }
 
-// ghosts of each real particle
-__global__ __launch_bounds__(MCM_BLOCK) void k_adj_gof_fill(DevState d, AdjDev a) {
-    const int ngp = d.sc->ngp;
-    for (int g = blockIdx.x * MCM_BLOCK + threadIdx.x; g < ngp; g += gridDim.x * MCM_BLOCK) {
+__global__ __launch_bounds__(MCM_BLOCK) void k_adj_fill(DevState d, AdjDev a, int nb) {
+    if ((int)blockIdx.x < nb) {
+        gt_fill_wave(d, a, blockIdx.x * MCM_BLOCK + threadIdx.x);
+        return;
+    }
+    const int ngp = d.sc->ngp, gb = gridDim.x - nb;
+    for (int g = (blockIdx.x - nb) * MCM_BLOCK + threadIdx.x; g < ngp; g += gb * MCM_BLOCK) {
         const int p = d.gpar[g];
         const int slot = atomicAdd(&a.gof_cnt[p], 1);
         if (slot < ADJ_GOF) a.gof[(long)p * ADJ_GOF + slot] = g;
@@ -239,13 +280,20 @@ __device__ __forceinline__ void isort(int* v, int n, long stride) {
     }
 }
 
-// ascending order, so that every sum over a transposed list has a fixed
-// order: one wavefront per ghost, each entry goes to the slot given by the
-// number of smaller ids in the list (the ids of one list are distinct)
-__global__ __launch_bounds__(MCM_BLOCK) void k_adj_gt_sort(DevState d, AdjDev a) {
+// Both lists in ascending order, so that every sum over them has a fixed
+// order, in one launch: blocks [0, wg) take the ghosts, one wavefront per
+// ghost, each entry going to the slot given by the number of smaller ids in
+// the list (the ids of one list are distinct); the rest take the real
+// particles, one thread each.
+__global__ __launch_bounds__(MCM_BLOCK) void k_adj_sort(DevState d, AdjDev a, int wg) {
+    if ((int)blockIdx.x >= wg) {
+        const int i = (blockIdx.x - wg) * MCM_BLOCK + threadIdx.x;
+        if (i < d.np) isort(a.gof + (long)i * ADJ_GOF, min(a.gof_cnt[i], ADJ_GOF), 1);
+        return;
+    }
     const int ngp = d.sc->ngp;
     const int lane = threadIdx.x & 63, wpb = MCM_BLOCK / 64;
-    for (int g = blockIdx.x * wpb + (threadIdx.x >> 6); g < ngp; g += gridDim.x * wpb) {
+    for (int g = blockIdx.x * wpb + (threadIdx.x >> 6); g < ngp; g += wg * wpb) {
         const int* src = a.gt + (long)g * a.gtcap;
         int* dst = a.gts + (long)g * a.gtcap;
         const int cnt = min(a.gt_cnt[g], a.gtcap);
@@ -257,12 +305,6 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_adj_gt_sort(DevState d, AdjDev a)
             if (k < cnt) dst[rank] = v;
         }
     }
-}
-
-__global__ __launch_bounds__(MCM_BLOCK) void k_adj_gof_sort(DevState d, AdjDev a) {
-    const int i = blockIdx.x * MCM_BLOCK + threadIdx.x;
-    if (i >= d.np) return;
-    isort(a.gof + (long)i * ADJ_GOF, min(a.gof_cnt[i], ADJ_GOF), 1);
 }
 
 // ------------------------------------------------- move and velocity update
@@ -1376,10 +1418,8 @@ int adj_reverse_step(mcm_gpu_adjoint* A, bool ck, int s) {
     // transposed lists; an overflow sets err bit 1 (checked after the sweep)
     k_adj_zero_counts<<<std::max(1, std::min(blocks_for(std::max((long)a.ngcap, A->np), MCM_BLOCK), 1024)), MCM_BLOCK, 0,
                         st>>>(a, a.ngcap, A->np);
-    k_adj_gt_fill<<<nb, MCM_BLOCK, 0, st>>>(d, a);
-    k_adj_gof_fill<<<gb, MCM_BLOCK, 0, st>>>(d, a);
-    k_adj_gt_sort<<<wg, MCM_BLOCK, 0, st>>>(d, a);
-    k_adj_gof_sort<<<nb, MCM_BLOCK, 0, st>>>(d, a);
+    k_adj_fill<<<nb + gb, MCM_BLOCK, 0, st>>>(d, a, nb);
+    k_adj_sort<<<wg + nb, MCM_BLOCK, 0, st>>>(d, a, wg);
     k_adj_move<<<nb, MCM_BLOCK, 0, st>>>(d, a);
     k_adj_fold_move<<<1, MCM_BLOCK, 0, st>>>(d, a, nb);
     k_adj_mom<<<wb, MCM_BLOCK, 0, st>>>(d, a);
