@@ -339,7 +339,9 @@ int mcm_gpu_adj_backward(mcm_gpu_adjoint *a, const double *xbar, const double *v
                          double *v0bar, double *pbar);
 const char *mcm_gpu_adj_last_error(const mcm_gpu_adjoint *a);
 /* Release a recording.  mcm_gpu_destroy releases the recordings of its run
-   as well, after which their handles are no longer valid. */
+   as well, after which their handles are no longer valid.  A recording made
+   with MCM_ADJ_TIMING=1 in the environment prints the summed per-kernel times
+   of its reverse steps to stderr here. */
 void mcm_gpu_adj_destroy(mcm_gpu_adjoint *a);
 
 #ifdef __cplusplus

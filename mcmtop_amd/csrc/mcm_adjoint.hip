@@ -205,17 +205,20 @@ __device__ __forceinline__ void hess_apply(const KGH& k, const double xij[3], co
 // ------------------------------------------------------- ghost transposes
 // Real partners of each ghost, from the real particles' lists, and the ghosts
 // of each real particle, in one launch: blocks [0, nb) take the lists, the
-// rest the ghosts.  A wavefront of the list part takes 64 particles and
-// spreads their ghost entries (list slots nreal..ntot-1) over its lanes,
-// ADJ_GT_U entries a lane per round, so that the atomics of a round are in
-// flight together; the order of the entries within a ghost's list is then
-// fixed by k_adj_sort.
+// rest the ghosts.  A wavefront of the list part takes ADJ_GT_PPW particles
+// and spreads their ghost entries (list slots nreal..ntot-1) over its 64
+// lanes, ADJ_GT_U entries a lane per round, so that the loads and atomics of
+// a round are in flight together.  Particles near a symmetry plane have tens
+// of ghost entries each, and few particles per wavefront keep its rounds few.
+// The order of the entries within a ghost's list is fixed by k_adj_sort.
 constexpr int ADJ_GT_U = 8;
+constexpr int ADJ_GT_PPW = 8;
 
-__device__ __forceinline__ void gt_fill_wave(const DevState& d, const AdjDev& a, int i) {
+__device__ __forceinline__ void gt_fill_wave(const DevState& d, const AdjDev& a, int wave) {
     const int lane = threadIdx.x & 63;
+    const int wbase = wave * ADJ_GT_PPW, i = wbase + lane;
     int sp = 0, k0 = 0, n = 0;   // list position, first ghost slot, ghost entries
-    if (i < d.np) {
+    if (lane < ADJ_GT_PPW && i < d.np) {
         sp = d.spos[i];
         k0 = d.nreal[i];
         n = d.ntot[i] - k0;
@@ -227,16 +230,15 @@ __device__ __forceinline__ void gt_fill_wave(const DevState& d, const AdjDev& a,
     }
     const int exc = inc - n;
     const int total = __shfl(inc, 63, 64);
-    const int wbase = i - lane;
     for (int t0 = 0; t0 < total; t0 += ADJ_GT_U * 64) {
         int g[ADJ_GT_U], who[ADJ_GT_U], slot[ADJ_GT_U];
 #pragma unroll
         for (int u = 0; u < ADJ_GT_U; u++) {
             const int t = t0 + u * 64 + lane;
-            // owner of entry t: the last lane whose exclusive prefix is <= t
-            // (lane 0's is 0; every probe stays below lane 64)
+            // owner of entry t < total: the last lane whose exclusive prefix
+            // is <= t (lane 0's is 0; lanes from ADJ_GT_PPW on hold total)
             int lo = 0;
-            for (int step = 32; step > 0; step >>= 1)
+            for (int step = ADJ_GT_PPW / 2; step > 0; step >>= 1)
                 if (__shfl(exc, lo + step, 64) <= t) lo += step;
             const int k = __shfl(k0, lo, 64) + (t - __shfl(exc, lo, 64));
             const int row = __shfl(sp, lo, 64);
@@ -256,7 +258,7 @@ __device__ __forceinline__ void gt_fill_wave(const DevState& d, const AdjDev& a,
 
 __global__ __launch_bounds__(MCM_BLOCK) void k_adj_fill(DevState d, AdjDev a, int nb) {
     if ((int)blockIdx.x < nb) {
-        gt_fill_wave(d, a, blockIdx.x * MCM_BLOCK + threadIdx.x);
+        gt_fill_wave(d, a, blockIdx.x * (MCM_BLOCK / 64) + (threadIdx.x >> 6));
         return;
     }
     const int ngp = d.sc->ngp, gb = gridDim.x - nb;
@@ -1063,6 +1065,23 @@ __global__ __launch_bounds__(MCM_BLOCK) void k_adj_search(DevState d, SearchSlot
 }
 
 // ================================================================== host
+// Per-kernel timing of the reverse step, for profiling.  With MCM_ADJ_TIMING=1
+// in the environment when an adjoint is created, an event is recorded before
+// the first and after every kernel of each of its reverse steps; the
+// intervals (each one kernel and the launch gap before it) are summed per
+// kernel and printed to stderr when the adjoint is destroyed.  The events are
+// read at the end of every reverse step, which holds the host there.
+constexpr int ADJ_TK = 14;
+const char* const adj_tk_names[ADJ_TK] = {
+    "k_adj_zero_counts", "k_adj_fill", "k_adj_sort",  "k_adj_move",         "k_adj_fold_move",
+    "k_adj_mom",         "k_adj_mom_ghost", "k_adj_gmap_mom", "k_adj_const", "k_adj_const_fin",
+    "k_adj_strain",      "k_adj_strain_ghost", "k_adj_gmap_strain", "k_adj_fold_end"};
+struct AdjTiming {
+    hipEvent_t ev[ADJ_TK + 1] = {};
+    double ms[ADJ_TK] = {};
+    long steps = 0;
+};
+
 struct mcm_gpu_adjoint {
     mcm_gpu_ctx* ctx = nullptr;
     int nsteps = 0, k = 1, nseg = 0, nsample = 0;
@@ -1097,6 +1116,7 @@ struct mcm_gpu_adjoint {
     std::vector<void*> lallocs;
     AdjDev a{};
     std::vector<void*> allocs, gallocs;   // run-long and ghost-sized allocations
+    AdjTiming* timing = nullptr;          // MCM_ADJ_TIMING=1 only
     std::string err;
 };
 
@@ -1416,22 +1436,52 @@ int adj_reverse_step(mcm_gpu_adjoint* A, bool ck, int s) {
     const int wb = blocks_for(A->np, MCM_BLOCK / kLpp);                                  // kLpp lanes per particle
     const int wg = std::max(1, std::min(blocks_for(d.ngcap, MCM_BLOCK / 64), 4096));   // wave per ghost
     // transposed lists; an overflow sets err bit 1 (checked after the sweep)
+    AdjTiming* T = A->timing;
+    int tk = 0;
+    auto mark = [&]() {
+        if (T) (void)hipEventRecord(T->ev[tk++], st);
+    };
+    mark();
     k_adj_zero_counts<<<std::max(1, std::min(blocks_for(std::max((long)a.ngcap, A->np), MCM_BLOCK), 1024)), MCM_BLOCK, 0,
                         st>>>(a, a.ngcap, A->np);
-    k_adj_fill<<<nb + gb, MCM_BLOCK, 0, st>>>(d, a, nb);
+    mark();
+    const int nbf = blocks_for(A->np, (MCM_BLOCK / 64) * ADJ_GT_PPW);
+    k_adj_fill<<<nbf + gb, MCM_BLOCK, 0, st>>>(d, a, nbf);
+    mark();
     k_adj_sort<<<wg + nb, MCM_BLOCK, 0, st>>>(d, a, wg);
+    mark();
     k_adj_move<<<nb, MCM_BLOCK, 0, st>>>(d, a);
+    mark();
     k_adj_fold_move<<<1, MCM_BLOCK, 0, st>>>(d, a, nb);
+    mark();
     k_adj_mom<<<wb, MCM_BLOCK, 0, st>>>(d, a);
+    mark();
     k_adj_mom_ghost<<<wg, MCM_BLOCK, 0, st>>>(d, a);
+    mark();
     k_adj_gmap_mom<<<nb, MCM_BLOCK, 0, st>>>(d, a);
+    mark();
     k_adj_const<<<dim3(nb, ADJ_NPASS), MCM_BLOCK, 0, st>>>(d, a);
+    mark();
     k_adj_const_fin<<<nb, MCM_BLOCK, 0, st>>>(d, a);
+    mark();
     k_adj_strain<<<wb, MCM_BLOCK, 0, st>>>(d, a);
+    mark();
     k_adj_strain_ghost<<<wg, MCM_BLOCK, 0, st>>>(d, a);
+    mark();
     k_adj_gmap_strain<<<nb, MCM_BLOCK, 0, st>>>(d, a);
+    mark();
     k_adj_fold_end<<<1, MCM_BLOCK, 0, st>>>(d, a, nb);
+    mark();
     HIPCHK_A(hipGetLastError());
+    if (T) {
+        HIPCHK_A(hipEventSynchronize(T->ev[ADJ_TK]));
+        for (int k = 0; k < ADJ_TK; k++) {
+            float ms = 0.0f;
+            HIPCHK_A(hipEventElapsedTime(&ms, T->ev[k], T->ev[k + 1]));
+            T->ms[k] += ms;
+        }
+        T->steps++;
+    }
     return 0;
 }
 
@@ -1621,6 +1671,16 @@ int mcm_gpu_adj_forward(mcm_gpu_ctx* ctx, int32_t nsteps, int32_t ckpt_every, co
     ctx->recs.push_back(A);
     A->nsteps = nsteps;
     A->k = ckpt_every;
+    if (const char* e = getenv("MCM_ADJ_TIMING")) {
+        if (e[0] == '1') {
+            A->timing = new AdjTiming();
+            for (hipEvent_t& ev : A->timing->ev)
+                if (hipEventCreate(&ev) != hipSuccess) {
+                    A->err = "MCM_ADJ_TIMING: hipEventCreate failed";
+                    return MCM_GPU_ERR_HIP;
+                }
+        }
+    }
     A->nseg = (nsteps + ckpt_every - 1) / ckpt_every;
     A->store = ckpt_every == 1;
     A->ninter = A->store ? nsteps : ckpt_every;
@@ -1808,6 +1868,20 @@ void mcm_gpu_adj_destroy(mcm_gpu_adjoint* A) {
     for (void* p : A->gallocs) (void)hipFree(p);
     for (void* p : A->iallocs) (void)hipFree(p);
     for (void* p : A->lallocs) (void)hipFree(p);
+    if (AdjTiming* T = A->timing) {
+        if (T->steps > 0) {
+            double tot = 0.0;
+            for (int k = 0; k < ADJ_TK; k++) tot += T->ms[k];
+            fprintf(stderr, "adjoint timing: %d steps, checkpoint every %d, %ld reverse steps, %.1f us of reverse kernels per step\n",
+                    A->nsteps, A->k, T->steps, 1e3 * tot / T->steps);
+            for (int k = 0; k < ADJ_TK; k++)
+                fprintf(stderr, "adjoint timing: %-20s %9.2f us per step %6.1f %%\n", adj_tk_names[k],
+                        1e3 * T->ms[k] / T->steps, 100.0 * T->ms[k] / tot);
+        }
+        for (hipEvent_t ev : T->ev)
+            if (ev) (void)hipEventDestroy(ev);
+        delete T;
+    }
     delete A;
 }
 
