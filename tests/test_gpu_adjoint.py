@@ -167,6 +167,28 @@ def test_checkpoint_interval_does_not_change_the_gradient(problem):
         np.testing.assert_array_equal(got[2], ref[2])
 
 
+def test_reverse_kernel_timing_leaves_the_gradient_unchanged(problem, capfd, monkeypatch):
+    """MCM_ADJ_TIMING=1 at recording time: the reverse steps time their kernels
+    with events, the totals are printed when the recording is released (with
+    its run, at the end of _gpu_grad), and the gradient is the same bit for bit."""
+    dk, sample, p_eval, target = problem
+    ref = _gpu_grad(dk, sample, p_eval, target, 7)
+    capfd.readouterr()
+    monkeypatch.setenv("MCM_ADJ_TIMING", "1")
+    got = _gpu_grad(dk, sample, p_eval, target, 7)
+    monkeypatch.delenv("MCM_ADJ_TIMING")
+    err = capfd.readouterr().err
+    assert got[0] == ref[0]
+    np.testing.assert_array_equal(got[1], ref[1])
+    np.testing.assert_array_equal(got[2], ref[2])
+    lines = [ln for ln in err.splitlines() if ln.startswith("adjoint timing:")]
+    assert len(lines) == 15, err
+    assert "%d steps, checkpoint every 7, %d reverse steps" % (NSTEPS, NSTEPS) in lines[0]
+    for name in ("k_adj_fill", "k_adj_sort", "k_adj_const", "k_adj_mom", "k_adj_strain", "k_adj_fold_end"):
+        assert any(ln.split()[2] == name for ln in lines[1:]), name
+    assert all(float(ln.split()[3]) > 0.0 for ln in lines[1:])
+
+
 def test_unsupported_decks_are_rejected():
     dk = deck.taylor_bar(dx=DX, radius=RADIUS, length=12.8e-3, model="f3dm10", tcrit_opt=1, corrected=True)
     with gpu.Solver(dk) as s:
